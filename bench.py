@@ -1,0 +1,234 @@
+"""Benchmark: LuxGI GI hot path on one MI355X, plain FP32 GPU form at full size.
+
+Workload (one "step" = one frame of the hot path on resident inputs):
+  * DDGI update, SDF branch: 32x16x32 probes x 256 rays (4,194,304 rays) traced
+    through a 256^3 FP32 global SDF of 2,000 seeded primitives (+ 64^3 coarse
+    mip) with surface-atlas radiance at hits (2048^2 RGBA8 atlas, 2% emissive
+    texels), then the irradiance (8x8) / depth (16x16) blend with hysteresis and
+    the border copy fused into the same pass.
+  * SVGF reflection denoise at 1920x1080 on a G-buffer from SDF primary rays:
+    reprojection + moments + 17x17 clip + tile classification, 5 a-trous passes.
+value = rays / DDGI-update time (Grays/s). All timing uses HIP events on the
+stream the kernels are launched on (torch's current stream), W warm-up steps
+then exactly K timed steps between device synchronisations.
+
+roofline: the trace kernel (dominant), HBM-bound; achieved = algorithmic bytes
+per launch (from a counting run of the same kernel on the same frame:
+mip+fine lookups x 8 voxels x 4 B + normal evaluations x 48 x 4 B + tile
+gathers x 4 texels x (4 B depth + 4 B colour) + rays x 32 B of output) / the
+mean trace-launch time. traffic = HBM bytes per launch measured with rocprofv3
+FETCH_SIZE / WRITE_SIZE (profiles/trace_hbm_traffic.json), null when absent.
+cpu_baseline: the FP32 CPU transcription (oracle/liboracle.so, std::thread over
+probes) on a 32x16x2 slab of the same volume (1,024 probes x 256 rays: trace +
+blend), on the host cores, scaled to Grays/s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
+    ap.add_argument("--svgf-iterations", type=int, default=5)
+    ap.add_argument("--quiet", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg, quiet=False):
+    if not quiet:
+        print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (probe volumes would shard by "
+              "probe z-slices, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from luxgi_amd import abi, scene
+    from luxgi_amd.ddgi import DDGIVolume, DeviceScene, bake_sdf, build_mip, default_trace_params, to_device
+    from luxgi_amd.svgf import ReflectionDenoiser, make_gbuffer
+
+    q = args.quiet
+    torch.cuda.set_device(0)
+    t_setup = time.perf_counter()
+    # ---------------- scene (not timed) ----------------
+    res = 256
+    prims = scene.scene_random(n=2000, seed=2)
+    sdf = bake_sdf(prims, res)
+    mip = build_mip(sdf, res, max_distance=2.0)
+    g = abi.make_sdf_data(resolution=res)
+    log(f"[bench] sdf {res}^3 baked", q)
+    atlas = scene.build_atlas(prims, resolution=2048, tile=16, color_storage=abi.STORAGE_UNORM8, seed=3,
+                              threshold=1.05 * 2.0 / res)
+    log("[bench] atlas 2048^2 built", q)
+    ds = DeviceScene(g, sdf, mip, atlas)
+    counts = (32, 16, 32)
+    step = [2.0 / (c - 1) for c in counts]
+    u = abi.make_uniform(counts, (-1.0, -1.0, -1.0), step, 256, irr_oct=8, depth_oct=16, hysteresis=0.98)
+    vol = DDGIVolume(u)
+    rays_per_frame = u.num_probes * u.raysPerProbe
+    # SVGF inputs: 1080p G-buffer pair from SDF primary rays
+    W, H = 1920, 1080
+    VP, VPp, eye, delta = scene.camera_pair(W, H, seed=4, eye=(0.0, 0.3, 2.4))
+    VPI = np.linalg.inv(VP)
+    prims_dev = to_device(np.frombuffer(prims.tobytes(), dtype=np.uint8))
+    cur, noisy, _ = make_gbuffer(prims_dev, len(prims), g, sdf, res, VPI, VPp, W, H, seed=1, with_mask=False)
+    VPI_prev = np.linalg.inv(VPp)
+    prev, _, _ = make_gbuffer(prims_dev, len(prims), g, sdf, res, VPI_prev, VPp, W, H, seed=2, with_mask=False)
+    dn = ReflectionDenoiser(W, H, iterations=args.svgf_iterations)
+    dn.set_camera(VPI, VPp, eye, delta)
+    torch.cuda.synchronize()
+    log(f"[bench] setup {time.perf_counter() - t_setup:.1f}s", q)
+
+    rng = np.random.default_rng(12345)
+    rotations = [scene.random_rotation(int(s)) for s in rng.integers(0, 2 ** 31, size=args.warmup + args.steps)]
+
+    def frame(i, ev=None):
+        p = default_trace_params(rotations[i], frame=vol.frames)
+        if ev:
+            ev[0].record()
+        vol.trace_rays(ds, p)
+        if ev:
+            ev[1].record()
+        vol.probe_update(fuse_border=True)
+        if ev:
+            ev[2].record()
+        vol.end_frame()
+        dn.denoise(cur, prev, noisy)
+        if ev:
+            ev[3].record()
+
+    for i in range(args.warmup):
+        frame(i)
+    torch.cuda.synchronize()
+    log(f"[bench] warmup {args.warmup} steps done", q)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        frame(args.warmup + k, evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_trace = np.array([e[0].elapsed_time(e[1]) for e in evs])
+    t_upd = np.array([e[1].elapsed_time(e[2]) for e in evs])
+    t_svgf = np.array([e[2].elapsed_time(e[3]) for e in evs])
+    ms_step = wall * 1e3 / args.steps
+    ddgi_ms = float(np.mean(t_trace + t_upd))
+    log(f"[bench] timed {args.steps} steps: trace {t_trace.mean():.3f} ms, update {t_upd.mean():.3f} ms, "
+        f"svgf {t_svgf.mean():.3f} ms, wall/step {ms_step:.3f} ms", q)
+
+    # ---------------- counting run (algorithmic bytes of the trace) ----------------
+    counters = torch.zeros(8, dtype=torch.int64, device="cuda")
+    vol.trace_rays(ds, default_trace_params(rotations[-1], frame=vol.frames), counters=counters)
+    torch.cuda.synchronize()
+    c = counters.cpu().numpy().astype(np.float64)
+    n_lookup, n_normal, n_tile, n_rays = c[0] + c[1], c[2], c[4], c[5]
+    trace_bytes = n_lookup * 8 * 4 + n_normal * 48 * 4 + n_tile * 4 * (4 + 4) + n_rays * 32
+    trace_s = float(np.mean(t_trace)) * 1e-3
+    achieved = trace_bytes / trace_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "trace_hbm_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    # ---------------- CPU baseline (bounded sample, FP32 transcription) ----------------
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(u, g, sdf, mip, res, atlas, rotations[-1], args.cpu_threads, q)
+
+    out = {
+        "metric": "DDGI update Grays/s (32x16x32 probes x 256 rays) + 1080p SVGF ms",
+        "value": rays_per_frame / (ddgi_ms * 1e-3) / 1e9,
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded primitives, atlas and G-buffer)",
+        "config": {"workload": "plain FP32 GPU form: DDGI 32x16x32 probes x 256 rays, 256^3 SDF of 2000 "
+                               "primitives, 2048^2 RGBA8 atlas, oct 8/16 + fused border; SVGF 1920x1080 "
+                               f"x{args.svgf_iterations} a-trous",
+                   "probes": list(counts), "rays_per_probe": 256, "sdf_resolution": res,
+                   "svgf": [W, H, args.svgf_iterations], "parallelism": "single GPU"},
+        "ddgi_update_ms": ddgi_ms,
+        "trace_ms": float(np.mean(t_trace)),
+        "probe_update_ms": float(np.mean(t_upd)),
+        "svgf_ms": float(np.mean(t_svgf)),
+        "trace_grays_per_s": rays_per_frame / trace_s / 1e9,
+        "trace_counts": {"lookups": n_lookup, "normals": n_normal, "objects": c[3], "tile_gathers": n_tile,
+                         "rays": n_rays, "steps": c[6], "hits": c[7]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "trace_kernel",
+                     "algorithmic_bytes_per_launch": trace_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(u_full, g, sdf_t, mip_t, res, atlas, rotation, threads, quiet):
+    import numpy as np
+
+    import oracle
+    from luxgi_amd import abi
+    from luxgi_amd.ddgi import default_trace_params
+
+    hw = oracle.threads()
+    n = threads if threads > 0 else min(16, hw)
+    sdf = sdf_t.cpu().numpy()
+    mip = mip_t.cpu().numpy()
+    p = default_trace_params(rotation)
+
+    def run(nz):
+        # a 32x16xnz slab of the same volume (same spacing, centred in z)
+        counts = (u_full.probeCounts[0], u_full.probeCounts[1], nz)
+        z0 = u_full.startPosition[2] + u_full.step[2] * ((u_full.probeCounts[2] - nz) // 2)
+        u = abi.make_uniform(counts, (u_full.startPosition[0], u_full.startPosition[1], z0),
+                             (u_full.step[0], u_full.step[1], u_full.step[2]), u_full.raysPerProbe, irr_oct=8,
+                             depth_oct=16, hysteresis=0.98)
+        t0 = time.perf_counter()
+        rad, dd, fl, _ = oracle.trace(u, g, sdf, mip, res, (res + 3) // 4, atlas.data, atlas.chunks, atlas.culled,
+                                      atlas.objects, atlas.tiles, atlas.depth, atlas.color_f32, p, precision="f32",
+                                      nthreads=n)
+        t1 = time.perf_counter()
+        z_i = np.zeros((u.irradianceTextureHeight, u.irradianceTextureWidth, 4), np.float32)
+        z_d = np.zeros((u.depthTextureHeight, u.depthTextureWidth, 2), np.float32)
+        oracle.probe_update(u, rad, dd, z_i, z_d, 0, 7, precision="f32", nthreads=n)
+        t2 = time.perf_counter()
+        return u, t1 - t0, t2 - t1
+
+    # size the sample to ~10 s of CPU work: probe 2 z-layers, then scale (at most the whole volume)
+    u, tt, tu = run(2)
+    nz = int(min(u_full.probeCounts[2], max(2, 2 * 10.0 / max(tt + tu, 1e-3))))
+    if nz > 2:
+        u, tt, tu = run(nz)
+    rays = u.num_probes * u.raysPerProbe
+    log(f"[bench] cpu baseline: {u.num_probes} probes, trace {tt:.2f}s update {tu:.2f}s on {n} threads", quiet)
+    return {"value": rays / (tt + tu) / 1e9, "unit": "Grays/s", "cores": n, "kind": "port",
+            "sample": f"FP32 C++ transcription, {u.num_probes} probes (32x16x{nz} slab of the same volume) x "
+                      f"{u.raysPerProbe} rays: trace {tt:.2f} s + blend {tu:.2f} s on {n} threads "
+                      f"(host reports {hw} hardware threads)",
+            "trace_s": tt, "update_s": tu}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,424 @@
+/*
+ * luxgi.h -- C ABI of the MI355X (gfx950) global-illumination hot path.
+ *
+ * The entry points replace the GPU dispatches that LuxGI's ECS systems record
+ * through Renderer::dispatch / Pipeline::dispatchIndirect for:
+ *   - the DDGI probe update, SDF ("soft trace") branch
+ *       trace_rays::system   Code/Maple/src/Engine/DDGI/DDGIRenderer.cpp:217-330
+ *       probe_update::system Code/Maple/src/Engine/DDGI/DDGIRenderer.cpp:377-413
+ *       border_update::system Code/Maple/src/Engine/DDGI/DDGIRenderer.cpp:441-464
+ *   - the global-SDF coarse mip + flood passes
+ *       merge_sdf::system / fillFlood Code/Maple/src/Engine/DDGI/GlobalDistanceField.cpp:537-573,823-846
+ *   - the SVGF denoisers
+ *       raytraced_reflection::denoise::system Code/Maple/src/Engine/Raytrace/RaytracedReflection.cpp:346-561
+ *       raytraced_shadow::denoise::system     Code/Maple/src/Engine/Raytrace/RaytracedShadow.cpp:406-610
+ *
+ * Conventions
+ *   - All buffer pointers are DEVICE pointers owned by the caller.
+ *   - Images are linear, row-major [height][width] arrays of texels with the
+ *     reference's texel addressing and channel order (Vulkan images are opaque
+ *     tiled; a caller sharing memory with Vulkan must use linear images or copy).
+ *   - Matrices are column-major float[16] (GLSL / glm layout).
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Every call is stream-ordered; none synchronises the host or allocates.
+ *   - Return value: LUXGI_OK (0) or a negative LuxgiStatus.
+ */
+#ifndef LUXGI_H
+#define LUXGI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Status codes and storage formats                                           */
+/* ------------------------------------------------------------------------- */
+typedef enum LuxgiStatus {
+    LUXGI_OK = 0,
+    LUXGI_ERR_INVALID_ARGUMENT = -1,
+    LUXGI_ERR_HIP = -2,
+    LUXGI_ERR_UNSUPPORTED = -3,
+    LUXGI_ERR_WORKSPACE_TOO_SMALL = -4
+} LuxgiStatus;
+
+/* Texel storage of a buffer. Channel count is fixed by the buffer's role. */
+typedef enum LuxgiStorage {
+    LUXGI_STORAGE_FP32 = 0,   /* RGBA32F / RG32F / R32F: the plain FP32 form      */
+    LUXGI_STORAGE_FP16 = 1,   /* RGBA16F / RG16F / R16F: the reference's formats  */
+    LUXGI_STORAGE_UNORM8 = 2  /* RGBA8 unorm (surface-atlas colour only)          */
+} LuxgiStorage;
+
+/* ------------------------------------------------------------------------- */
+/* POD mirrors of the reference's GPU structs (byte-identical layouts)         */
+/* ------------------------------------------------------------------------- */
+
+/* DDGIUniform: Shaders/DDGI/DDGICommon.glsl:11-31, host Engine/DDGI/DDGIRenderer.h:22-42.
+ * 96 bytes, scalar layout. */
+typedef struct LuxgiDDGIUniform {
+    float   startPosition[4];
+    float   step[4];
+    int32_t probeCounts[4];
+    float   maxDistance;
+    float   sharpness;
+    float   hysteresis;
+    float   normalBias;
+    float   ddgiGamma;
+    int32_t irradianceProbeSideLength;
+    int32_t irradianceTextureWidth;
+    int32_t irradianceTextureHeight;
+    int32_t depthProbeSideLength;
+    int32_t depthTextureWidth;
+    int32_t depthTextureHeight;
+    int32_t raysPerProbe;
+} LuxgiDDGIUniform;
+
+/* GlobalSDFData: Shaders/SDF/GlobalSDFData.glsl:4-12. 96 bytes. */
+typedef struct LuxgiGlobalSDFData {
+    float    cascadePosDistance[4][4]; /* xyz = cascade centre, w = half extent */
+    float    cascadeVoxelSize[4];
+    uint32_t cascadesCount;
+    float    resolution;
+    float    nearPlane;
+    float    farPlane;
+} LuxgiGlobalSDFData;
+
+/* GlobalSurfaceAtlasData: Shaders/SDF/AtlasCommon.glsl:8-16. 32 bytes. */
+typedef struct LuxgiSurfaceAtlasData {
+    float    cameraPos[3];
+    float    chunkSize;
+    uint32_t culledObjectsCapacity;
+    uint32_t resolution;
+    uint32_t objectsCount;
+    uint32_t padding;
+} LuxgiSurfaceAtlasData;
+
+/* ObjectBuffer: Shaders/SDF/AtlasCommon.glsl:18-25 (std430). 128 bytes. */
+typedef struct LuxgiAtlasObject {
+    float    objectBounds[4]; /* bounding sphere: centre xyz, radius w */
+    uint32_t tileOffset[6];   /* 0 = no tile */
+    int32_t  padding[2];
+    float    transform[16];   /* local -> world, column-major */
+    float    extends[4];      /* local half extents xyz */
+} LuxgiAtlasObject;
+
+/* TileBuffer: Shaders/SDF/AtlasCommon.glsl:27-32 (std430). 96 bytes. */
+typedef struct LuxgiAtlasTile {
+    float extends[4];      /* atlas rect (x, y, w, h) in UV */
+    float transform[16];   /* object-local -> tile view, column-major */
+    float objectBounds[4]; /* tile view box size xyz */
+} LuxgiAtlasTile;
+
+/* GISDFRays.comp push constants (Shaders/DDGI/GISDFRays.comp:53-60, 80 bytes)
+ * followed by options that extend the reference. */
+typedef struct LuxgiTraceParams {
+    float    randomOrientation[16]; /* column-major; only the upper 3x3 is used */
+    uint32_t numFrames;
+    uint32_t infiniteBounces;
+    int32_t  numLights;
+    float    intensity;
+    /* --- extensions (defaults reproduce the reference) --- */
+    int32_t  maxSteps;      /* per-cascade step cap; reference = 250 (SDFCommon.glsl:143) */
+    float    hitEpsilon;    /* 0 = reference thickness test (SDFCommon.glsl:166-167); >0 = fixed eps */
+    int32_t  pad0;
+    int32_t  pad1;
+    float    skyColor[4];   /* sky radiance when no cubemap is bound (reference binds a unit cube) */
+} LuxgiTraceParams;
+
+/* ------------------------------------------------------------------------- */
+/* Resource descriptors                                                       */
+/* ------------------------------------------------------------------------- */
+
+/* Global SDF clipmap + coarse mip. Layout [res][res][cascades*res] (z, y, x),
+ * x fastest, cascades packed along x (GlobalDistanceField.cpp:611-622).
+ * Values are normalised distances (world distance / (2*halfExtent)). */
+typedef struct LuxgiSDFVolume {
+    const void* sdf;      /* R32F or R16F */
+    const void* mip;      /* R32F or R16F, [mipRes][mipRes][cascades*mipRes] */
+    int32_t     resolution;
+    int32_t     mipResolution;
+    int32_t     storage;  /* LuxgiStorage: FP32 or FP16 */
+    int32_t     pad;
+} LuxgiSDFVolume;
+
+/* Surface atlas + object/tile/chunk buffers read by the hit lookup
+ * (Shaders/SDF/AtlasCommon.glsl:115-157). */
+typedef struct LuxgiSurfaceAtlas {
+    LuxgiSurfaceAtlasData   data;
+    const uint32_t*         chunks;         /* [40^3] start index into culledObjects, 0 = empty */
+    const uint32_t*         culledObjects;  /* [count, id0, id1, ...] runs */
+    const LuxgiAtlasObject* objects;
+    const LuxgiAtlasTile*   tiles;
+    const void*             color;          /* light cache [res][res] RGBA (storage below) */
+    const float*            depth;          /* tile depth [res][res] R32F, >=1 = empty */
+    int32_t                 objectCapacity; /* number of entries in `objects` */
+    int32_t                 colorStorage;   /* FP32, FP16 or UNORM8 */
+} LuxgiSurfaceAtlas;
+
+/* Optional sky cubemap: 6 faces (+X,-X,+Y,-Y,+Z,-Z) of size^2 RGBA32F texels. */
+typedef struct LuxgiSkybox {
+    const float* faces;
+    int32_t      size;
+    int32_t      pad;
+} LuxgiSkybox;
+
+/* DDGI ray buffers: radiance (rgb, 0) and direction/distance (xyz, t),
+ * texel (rayId, probeId) -> element probeId*width + rayId
+ * (DDGIRenderer.cpp:171-177). */
+typedef struct LuxgiRayBuffers {
+    void*   radiance;
+    void*   directionDistance;
+    int32_t width;    /* texels per row (>= uniform.raysPerProbe) */
+    int32_t height;   /* rows (>= probe count) */
+    int32_t storage;  /* FP32 (RGBA32F) or FP16 (RGBA16F) */
+    int32_t pad;
+} LuxgiRayBuffers;
+
+/* One ping-pong side of the probe atlases (DDGIRenderer.cpp:179-209):
+ * irradiance RGBA, depth RG, sizes taken from the uniform. */
+typedef struct LuxgiProbeAtlas {
+    void*   irradiance;
+    void*   depth;
+    int32_t storage;  /* FP32 or FP16 */
+    int32_t pad;
+} LuxgiProbeAtlas;
+
+/* Probe-update flags */
+#define LUXGI_UPDATE_IRRADIANCE   0x1u  /* run IrradianceProbeUpdate */
+#define LUXGI_UPDATE_DEPTH        0x2u  /* run DepthProbeUpdate */
+#define LUXGI_UPDATE_FUSE_BORDER  0x4u  /* also write the border texels (BorderUpdate) */
+#define LUXGI_UPDATE_PER_PROBE    0x8u  /* force the per-probe weight path (no shared-direction GEMM) */
+
+/* ------------------------------------------------------------------------- */
+/* DDGI                                                                        */
+/* ------------------------------------------------------------------------- */
+
+/* Atlas sizes for a probe grid and octahedral side lengths
+ * (DDGIRenderer.cpp:182-185): width = (oct+2)*px*py+2, height = (oct+2)*pz+2.
+ * Fills the *TextureWidth/Height fields of `uniform`. Host-only. */
+int luxgi_ddgi_fill_atlas_sizes(LuxgiDDGIUniform* uniform);
+
+/* Workspace bytes needed by luxgi_ddgi_trace_sdf for `objectCapacity` atlas objects. */
+size_t luxgi_ddgi_trace_workspace_size(int32_t objectCapacity);
+
+/* GISDFRays.comp (Shaders/DDGI/GISDFRays.comp:63-128): probe ray generation,
+ * global-SDF sphere trace (SDFCommon.glsl:98-194), hit normal, surface-atlas
+ * lookup (AtlasCommon.glsl:115-157) and ray-result encode.
+ * Replaces the dispatch at DDGIRenderer.cpp:318-321.
+ *   probeOffsets  optional float4[P] relocation offsets added to probe positions (NULL = none)
+ *   rayFlags      optional uint8[P*width] per-ray class: 0 miss, 1 front hit, 2 inside/backface (NULL = none)
+ *   workspace     device scratch of luxgi_ddgi_trace_workspace_size() bytes (per-object inverses) */
+int luxgi_ddgi_trace_sdf(const LuxgiDDGIUniform* uniform,
+                         const LuxgiGlobalSDFData* sdfData,
+                         const LuxgiSDFVolume* volume,
+                         const LuxgiSurfaceAtlas* atlas,
+                         const LuxgiSkybox* skybox,
+                         const LuxgiTraceParams* params,
+                         const float* probeOffsets,
+                         const LuxgiRayBuffers* rays,
+                         uint8_t* rayFlags,
+                         void* workspace, size_t workspaceBytes,
+                         void* stream);
+
+/* Counting run of the same trace: accumulates into counters[8] (uint64, device):
+ *   [0] mip lookups, [1] fine lookups, [2] hits needing a normal (6 lookups each),
+ *   [3] atlas objects tested, [4] tile samples (4-texel gathers), [5] rays,
+ *   [6] total march steps, [7] rays that hit.
+ * The caller zeroes counters first. Outputs are written exactly as by luxgi_ddgi_trace_sdf. */
+int luxgi_ddgi_trace_sdf_count(const LuxgiDDGIUniform* uniform,
+                               const LuxgiGlobalSDFData* sdfData,
+                               const LuxgiSDFVolume* volume,
+                               const LuxgiSurfaceAtlas* atlas,
+                               const LuxgiSkybox* skybox,
+                               const LuxgiTraceParams* params,
+                               const float* probeOffsets,
+                               const LuxgiRayBuffers* rays,
+                               uint8_t* rayFlags,
+                               void* workspace, size_t workspaceBytes,
+                               unsigned long long* counters,
+                               void* stream);
+
+/* Workspace bytes needed by luxgi_ddgi_probe_update. */
+size_t luxgi_ddgi_probe_update_workspace_size(const LuxgiDDGIUniform* uniform);
+
+/* IrradianceProbeUpdate.comp + DepthProbeUpdate.comp (Shaders/DDGI/ProbeUpdate.glsl:53-152),
+ * dispatched at DDGIRenderer.cpp:347-375. Reads `prev` (the pingPong side),
+ * writes `out` (the 1-pingPong side). firstFrame != 0 disables hysteresis.
+ * With LUXGI_UPDATE_FUSE_BORDER the border texels are written as by
+ * luxgi_ddgi_border_update in the same pass. */
+int luxgi_ddgi_probe_update(const LuxgiDDGIUniform* uniform,
+                            const LuxgiRayBuffers* rays,
+                            const LuxgiProbeAtlas* prev,
+                            const LuxgiProbeAtlas* out,
+                            uint32_t firstFrame,
+                            uint32_t flags,
+                            void* workspace, size_t workspaceBytes,
+                            void* stream);
+
+/* IrradianceBorderUpdate.comp / DepthBorderUpdate.comp (Shaders/DDGI/BorderUpdate.glsl:136-156),
+ * dispatched at DDGIRenderer.cpp:420-438. In-place on `atlas`.
+ * which: LUXGI_UPDATE_IRRADIANCE and/or LUXGI_UPDATE_DEPTH. */
+int luxgi_ddgi_border_update(const LuxgiDDGIUniform* uniform,
+                             const LuxgiProbeAtlas* atlas,
+                             uint32_t which,
+                             void* stream);
+
+/* Probe relocation + classification (no counterpart in the reference; rules
+ * after Majercik et al., JCGT 2021). Reads the ray distances and rayFlags of a
+ * trace, writes offsets (float4[P], xyz used, w = 0) and states (uint8[P]:
+ * 0 active, 1 inactive). `offsets` is read and updated in place. */
+typedef struct LuxgiRelocateParams {
+    float backfaceThreshold;      /* fraction of inside/backface rays that marks a probe as inside (0.25) */
+    float minFrontfaceDistance;   /* push away from frontfaces closer than this (world units) */
+    float maxOffsetFraction;      /* offsets are clamped to +-this * step per axis (0.45) */
+    int32_t relocate;             /* 0 = classification only */
+} LuxgiRelocateParams;
+
+int luxgi_ddgi_relocate_classify(const LuxgiDDGIUniform* uniform,
+                                 const LuxgiRayBuffers* rays,
+                                 const uint8_t* rayFlags,
+                                 const LuxgiRelocateParams* params,
+                                 float* offsets,
+                                 uint8_t* states,
+                                 void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* Global SDF coarse mip (GlobalSDFMipmap.comp:32-68 + fillFlood GlobalDistanceField.cpp:537-573) */
+/* ------------------------------------------------------------------------- */
+
+/* Builds `mip` ([mipRes][mipRes][cascades*mipRes], FP32) for one cascade from
+ * `sdf` ([res][res][cascades*res], FP32): the 7-tap min pass followed by
+ * floodPasses (reference: 4) ping-pong passes through `temp`
+ * ([mipRes]^3 FP32 scratch). maxDistance = cascade full extent (2*halfExtent). */
+int luxgi_sdf_build_mip(const float* sdf, int32_t resolution, int32_t cascades, int32_t cascade,
+                        float maxDistance, int32_t floodPasses,
+                        float* mip, float* temp, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* SVGF (reflection = RGB, shadow = scalar)                                   */
+/* ------------------------------------------------------------------------- */
+
+/* G-buffer channel contract (DeferredColor.frag:98-101, GBuffer.cpp:15-24):
+ * color   = (albedo.rgb, meshId)            RGBA32F
+ * normals = (octNormal.xy, prevUV - curUV)  RGBA32F
+ * pbr     = (metallic, roughness, curvature, linearZ) RGBA32F
+ * depth   = NDC depth, 1 = cleared / sky    R32F   */
+typedef struct LuxgiGBuffer {
+    const float* color;
+    const float* normals;
+    const float* pbr;
+    const float* depth;
+} LuxgiGBuffer;
+
+/* Reflection/DenoiseReprojection.comp UBO (:23-28) + push constants (:67-73). */
+typedef struct LuxgiReflectionReprojectParams {
+    float   viewProjInv[16];
+    float   prevViewProj[16];
+    float   cameraPos[4];
+    float   cameraDelta[3];
+    float   alpha;
+    float   momentsAlpha;
+    int32_t approximateWithDDGI;
+} LuxgiReflectionReprojectParams;
+
+/* Reflection/DenoiseAtrous.comp push constants (:31-39). */
+typedef struct LuxgiReflectionAtrousParams {
+    int32_t radius;
+    int32_t stepSize;
+    float   phiColor;
+    float   phiNormal;
+    float   sigmaDepth;
+    int32_t approximateWithDDGI;
+} LuxgiReflectionAtrousParams;
+
+/* Shadow/DenoiseReprojection.comp UBO (:24-27) + push constants (:61-65). */
+typedef struct LuxgiShadowReprojectParams {
+    float viewProjInv[16];
+    float alpha;
+    float momentsAlpha;
+    float pad[2];
+} LuxgiShadowReprojectParams;
+
+/* Shadow/DenoiseAtrous.comp push constants (:239-249). */
+typedef struct LuxgiShadowAtrousParams {
+    int32_t radius;
+    int32_t stepSize;
+    float   phiVisibility;
+    float   phiNormal;
+    float   sigmaDepth;
+    float   power;
+    float   near_;
+    float   far_;
+} LuxgiShadowAtrousParams;
+
+/* Number of 8x8 tiles for a width x height image (the reference's tile lists,
+ * RaytracedReflection.cpp:658-668); the tile map is one byte per tile. */
+int32_t luxgi_svgf_tile_count(int32_t width, int32_t height);
+
+/* Reflection/DenoiseReprojection.comp:117-218 (temporal accumulation, moments,
+ * 17x17 history clip, tile classification). outColor = (rgb, variance),
+ * outMoments = (m1, m2, historyLength, 0). tileMap[t] = 1 denoise / 0 copy
+ * (deterministic OR over the tile; the reference's flag is last-writer-wins).
+ * Images: `storage` FP32 (RGBA32F) or FP16 (RGBA16F), width x height. */
+int luxgi_svgf_reflection_reproject(const LuxgiReflectionReprojectParams* params,
+                                    int32_t width, int32_t height, int32_t storage,
+                                    const LuxgiGBuffer* cur, const LuxgiGBuffer* prev,
+                                    const void* noisy, const void* historyColor, const void* historyMoments,
+                                    void* outColor, void* outMoments, uint8_t* tileMap,
+                                    void* stream);
+
+/* One a-trous iteration (Reflection/DenoiseAtrous.comp:63-134) fused with the
+ * copy-tiles pass (Reflection/DenoiseCopyTiles.comp:21-25): denoise tiles are
+ * filtered, copy tiles copied. historyOut (nullable) receives the same output
+ * (the history copy after iteration 1, RaytracedReflection.cpp:488-491). */
+int luxgi_svgf_reflection_atrous(const LuxgiReflectionAtrousParams* params,
+                                 int32_t width, int32_t height, int32_t storage,
+                                 const LuxgiGBuffer* cur, const uint8_t* tileMap,
+                                 const void* input, void* output, void* historyOut,
+                                 void* stream);
+
+/* Whole reflection denoise of one frame (denoise::system, RaytracedReflection.cpp:496-520):
+ * reprojection then `iterations` a-trous passes (step 1<<i), history copy after
+ * iteration 1. outputs[0] = accumulation target, outputs[1] = history colour;
+ * moments[pingPong] is written, moments[1-pingPong] read; pingPong[0/1] buffers
+ * of the a-trous chain. *result receives which atrous buffer holds the output. */
+int luxgi_svgf_reflection_denoise(const LuxgiReflectionReprojectParams* reprojectParams,
+                                  const LuxgiReflectionAtrousParams* atrousParams,
+                                  int32_t iterations,
+                                  int32_t width, int32_t height, int32_t storage,
+                                  const LuxgiGBuffer* cur, const LuxgiGBuffer* prev,
+                                  const void* noisy,
+                                  void* output0, void* output1,
+                                  void* moments0, void* moments1, int32_t pingPong,
+                                  void* atrous0, void* atrous1,
+                                  uint8_t* tileMap, int32_t* result,
+                                  void* stream);
+
+/* Shadow/DenoiseReprojection.comp:71-210. mask = R32UI [ceil(H/4)][ceil(W/8)],
+ * bit (y%4)*8+(x%8) of word (y/4, x/8) = visibility of pixel (x, y).
+ * outColor = (visibility, variance) RG, outMoments RGBA (m1, m2, historyLength, 0). */
+int luxgi_svgf_shadow_reproject(const LuxgiShadowReprojectParams* params,
+                                int32_t width, int32_t height, int32_t storage,
+                                const LuxgiGBuffer* cur, const LuxgiGBuffer* prev,
+                                const uint32_t* mask, const void* historyColor, const void* historyMoments,
+                                void* outColor, void* outMoments, uint8_t* tileMap,
+                                void* stream);
+
+/* Shadow/DenoiseAtrous.comp:275-339 fused with Shadow/DenoiseCopyTiles.comp:356-360
+ * (copy tiles write 0). */
+int luxgi_svgf_shadow_atrous(const LuxgiShadowAtrousParams* params,
+                             int32_t width, int32_t height, int32_t storage,
+                             const LuxgiGBuffer* cur, const uint8_t* tileMap,
+                             const void* input, void* output, void* historyOut,
+                             void* stream);
+
+/* Library version string. */
+const char* luxgi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LUXGI_H */

@@ -1,0 +1,203 @@
+"""ctypes mirrors of the POD structs declared in include/luxgi.h.
+
+The layouts are byte-identical to the reference's GPU structs (sizes are
+asserted at import time):
+  DDGIUniform            Shaders/DDGI/DDGICommon.glsl:11-31        96 B
+  GlobalSDFData          Shaders/SDF/GlobalSDFData.glsl:4-12       96 B
+  GlobalSurfaceAtlasData Shaders/SDF/AtlasCommon.glsl:8-16         32 B
+  ObjectBuffer           Shaders/SDF/AtlasCommon.glsl:18-25       128 B
+  TileBuffer             Shaders/SDF/AtlasCommon.glsl:27-32        96 B
+  GISDFRays push consts  Shaders/DDGI/GISDFRays.comp:53-60         80 B (+ extensions)
+"""
+import ctypes as C
+
+import numpy as np
+
+LUXGI_OK = 0
+STORAGE_FP32 = 0
+STORAGE_FP16 = 1
+STORAGE_UNORM8 = 2
+
+UPDATE_IRRADIANCE = 0x1
+UPDATE_DEPTH = 0x2
+UPDATE_FUSE_BORDER = 0x4
+UPDATE_PER_PROBE = 0x8
+
+GLOBAL_SDF_WORLD_SIZE = 60000.0  # Shaders/SDF/SDFCommon.glsl:6
+ATLAS_CHUNKS_RESOLUTION = 40     # Shaders/SDF/AtlasCommon.glsl:5
+
+f4 = C.c_float * 4
+f16 = C.c_float * 16
+i4 = C.c_int32 * 4
+
+
+class DDGIUniform(C.Structure):
+    _fields_ = [
+        ("startPosition", f4), ("step", f4), ("probeCounts", i4),
+        ("maxDistance", C.c_float), ("sharpness", C.c_float), ("hysteresis", C.c_float), ("normalBias", C.c_float),
+        ("ddgiGamma", C.c_float), ("irradianceProbeSideLength", C.c_int32),
+        ("irradianceTextureWidth", C.c_int32), ("irradianceTextureHeight", C.c_int32),
+        ("depthProbeSideLength", C.c_int32), ("depthTextureWidth", C.c_int32),
+        ("depthTextureHeight", C.c_int32), ("raysPerProbe", C.c_int32),
+    ]
+
+    @property
+    def num_probes(self):
+        return self.probeCounts[0] * self.probeCounts[1] * self.probeCounts[2]
+
+
+class GlobalSDFData(C.Structure):
+    _fields_ = [
+        ("cascadePosDistance", f4 * 4), ("cascadeVoxelSize", f4), ("cascadesCount", C.c_uint32),
+        ("resolution", C.c_float), ("nearPlane", C.c_float), ("farPlane", C.c_float),
+    ]
+
+
+class SurfaceAtlasData(C.Structure):
+    _fields_ = [
+        ("cameraPos", C.c_float * 3), ("chunkSize", C.c_float), ("culledObjectsCapacity", C.c_uint32),
+        ("resolution", C.c_uint32), ("objectsCount", C.c_uint32), ("padding", C.c_uint32),
+    ]
+
+
+class AtlasObject(C.Structure):
+    _fields_ = [
+        ("objectBounds", f4), ("tileOffset", C.c_uint32 * 6), ("padding", C.c_int32 * 2),
+        ("transform", f16), ("extends", f4),
+    ]
+
+
+class AtlasTile(C.Structure):
+    _fields_ = [("extends", f4), ("transform", f16), ("objectBounds", f4)]
+
+
+class TraceParams(C.Structure):
+    _fields_ = [
+        ("randomOrientation", f16), ("numFrames", C.c_uint32), ("infiniteBounces", C.c_uint32),
+        ("numLights", C.c_int32), ("intensity", C.c_float),
+        ("maxSteps", C.c_int32), ("hitEpsilon", C.c_float), ("pad0", C.c_int32), ("pad1", C.c_int32),
+        ("skyColor", f4),
+    ]
+
+
+class SDFVolume(C.Structure):
+    _fields_ = [("sdf", C.c_void_p), ("mip", C.c_void_p), ("resolution", C.c_int32),
+                ("mipResolution", C.c_int32), ("storage", C.c_int32), ("pad", C.c_int32)]
+
+
+class SurfaceAtlas(C.Structure):
+    _fields_ = [
+        ("data", SurfaceAtlasData), ("chunks", C.c_void_p), ("culledObjects", C.c_void_p),
+        ("objects", C.c_void_p), ("tiles", C.c_void_p), ("color", C.c_void_p), ("depth", C.c_void_p),
+        ("objectCapacity", C.c_int32), ("colorStorage", C.c_int32),
+    ]
+
+
+class Skybox(C.Structure):
+    _fields_ = [("faces", C.c_void_p), ("size", C.c_int32), ("pad", C.c_int32)]
+
+
+class RayBuffers(C.Structure):
+    _fields_ = [("radiance", C.c_void_p), ("directionDistance", C.c_void_p), ("width", C.c_int32),
+                ("height", C.c_int32), ("storage", C.c_int32), ("pad", C.c_int32)]
+
+
+class ProbeAtlas(C.Structure):
+    _fields_ = [("irradiance", C.c_void_p), ("depth", C.c_void_p), ("storage", C.c_int32), ("pad", C.c_int32)]
+
+
+class RelocateParams(C.Structure):
+    _fields_ = [("backfaceThreshold", C.c_float), ("minFrontfaceDistance", C.c_float),
+                ("maxOffsetFraction", C.c_float), ("relocate", C.c_int32)]
+
+
+class GBuffer(C.Structure):
+    _fields_ = [("color", C.c_void_p), ("normals", C.c_void_p), ("pbr", C.c_void_p), ("depth", C.c_void_p)]
+
+
+class ReflectionReprojectParams(C.Structure):
+    _fields_ = [("viewProjInv", f16), ("prevViewProj", f16), ("cameraPos", f4), ("cameraDelta", C.c_float * 3),
+                ("alpha", C.c_float), ("momentsAlpha", C.c_float), ("approximateWithDDGI", C.c_int32)]
+
+
+class ReflectionAtrousParams(C.Structure):
+    _fields_ = [("radius", C.c_int32), ("stepSize", C.c_int32), ("phiColor", C.c_float), ("phiNormal", C.c_float),
+                ("sigmaDepth", C.c_float), ("approximateWithDDGI", C.c_int32)]
+
+
+class ShadowReprojectParams(C.Structure):
+    _fields_ = [("viewProjInv", f16), ("alpha", C.c_float), ("momentsAlpha", C.c_float), ("pad", C.c_float * 2)]
+
+
+class ShadowAtrousParams(C.Structure):
+    _fields_ = [("radius", C.c_int32), ("stepSize", C.c_int32), ("phiVisibility", C.c_float),
+                ("phiNormal", C.c_float), ("sigmaDepth", C.c_float), ("power", C.c_float),
+                ("near_", C.c_float), ("far_", C.c_float)]
+
+
+_SIZES = {DDGIUniform: 96, GlobalSDFData: 96, SurfaceAtlasData: 32, AtlasObject: 128, AtlasTile: 96,
+          TraceParams: 112, ReflectionReprojectParams: 168, ReflectionAtrousParams: 24,
+          ShadowReprojectParams: 80, ShadowAtrousParams: 32}
+for _t, _n in _SIZES.items():
+    assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
+
+
+def mat4_colmajor(m):
+    """numpy 4x4 (row-major math convention, m[row, col]) -> column-major float[16]."""
+    m = np.asarray(m, dtype=np.float32)
+    return (C.c_float * 16)(*m.T.reshape(-1).tolist())
+
+
+def make_uniform(probe_counts, start, step, rays_per_probe, irr_oct=8, depth_oct=16, max_distance=None,
+                 sharpness=50.0, hysteresis=0.98, normal_bias=0.1, gamma=5.0):
+    """DDGIUniform with the reference defaults (DDGIRenderer.h:28-41, on_game_start DDGIRenderer.cpp:666-682).
+
+    max_distance defaults to 1.5 * max(step) (DDGIRenderer.cpp:674: 1.5 * probeDistance)."""
+    u = DDGIUniform()
+    for i in range(3):
+        u.startPosition[i] = float(start[i])
+        u.step[i] = float(step[i])
+        u.probeCounts[i] = int(probe_counts[i])
+    u.startPosition[3] = 0.0
+    u.step[3] = 0.0
+    u.probeCounts[3] = 0
+    u.maxDistance = float(1.5 * max(step) if max_distance is None else max_distance)
+    u.sharpness = float(sharpness)
+    u.hysteresis = float(hysteresis)
+    u.normalBias = float(normal_bias)
+    u.ddgiGamma = float(gamma)
+    u.irradianceProbeSideLength = int(irr_oct)
+    u.depthProbeSideLength = int(depth_oct)
+    px, py, pz = (int(c) for c in probe_counts)
+    u.irradianceTextureWidth = (irr_oct + 2) * px * py + 2
+    u.irradianceTextureHeight = (irr_oct + 2) * pz + 2
+    u.depthTextureWidth = (depth_oct + 2) * px * py + 2
+    u.depthTextureHeight = (depth_oct + 2) * pz + 2
+    u.raysPerProbe = int(rays_per_probe)
+    return u
+
+
+def make_sdf_data(center=(0.0, 0.0, 0.0), half_extent=1.0, resolution=64):
+    """Single-cascade GlobalSDFData over [center - half, center + half]^3."""
+    g = GlobalSDFData()
+    g.cascadePosDistance[0][0] = center[0]
+    g.cascadePosDistance[0][1] = center[1]
+    g.cascadePosDistance[0][2] = center[2]
+    g.cascadePosDistance[0][3] = half_extent
+    g.cascadeVoxelSize[0] = 2.0 * half_extent / resolution
+    g.cascadesCount = 1
+    g.resolution = float(resolution)
+    g.nearPlane = 0.1
+    g.farPlane = 1000.0
+    return g
+
+
+def addr(x):
+    """Device or host address of a torch tensor / numpy array / None."""
+    if x is None:
+        return None
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data
+    return int(x)

@@ -1,0 +1,177 @@
+// common.hpp -- device helpers shared by the gfx950 kernels of libluxgi.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/luxgi.h"
+
+namespace luxgi {
+
+#define LUXGI_HIP_CHECK(expr)                                                                  \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess) {                                                                \
+            fprintf(stderr, "luxgi: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(_e),   \
+                    __FILE__, __LINE__);                                                       \
+            return LUXGI_ERR_HIP;                                                              \
+        }                                                                                      \
+    } while (0)
+
+#define LUXGI_LAUNCH_CHECK() LUXGI_HIP_CHECK(hipGetLastError())
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------------------
+// small float3 helpers (explicit, so the operation order matches the oracle)
+// ---------------------------------------------------------------------------
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ f3 normalize3(f3 a) {
+    float l = len3(a);
+    return {a.x / l, a.y / l, a.z / l};
+}
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ float fractf(float v) { return v - floorf(v); }
+
+// column-major mat4 * (v, w)
+__device__ __forceinline__ f3 mat4_mul(const float* m, f3 v, float w) {
+    return {m[0] * v.x + m[4] * v.y + m[8] * v.z + m[12] * w, m[1] * v.x + m[5] * v.y + m[9] * v.z + m[13] * w,
+            m[2] * v.x + m[6] * v.y + m[10] * v.z + m[14] * w};
+}
+__device__ __forceinline__ float4 mat4_mul4(const float* m, float4 v) {
+    return make_float4(m[0] * v.x + m[4] * v.y + m[8] * v.z + m[12] * v.w,
+                       m[1] * v.x + m[5] * v.y + m[9] * v.z + m[13] * v.w,
+                       m[2] * v.x + m[6] * v.y + m[10] * v.z + m[14] * v.w,
+                       m[3] * v.x + m[7] * v.y + m[11] * v.z + m[15] * v.w);
+}
+
+// ---------------------------------------------------------------------------
+// texel storage: float (RGBA32F/RG32F/R32F) or __half (RGBA16F/RG16F/R16F)
+// ---------------------------------------------------------------------------
+template <class S> struct Texel;
+
+template <> struct Texel<float> {
+    static __device__ __forceinline__ float4 ld4(const float* p, size_t i) {
+        return reinterpret_cast<const float4*>(p)[i];
+    }
+    static __device__ __forceinline__ void st4(float* p, size_t i, float4 v) { reinterpret_cast<float4*>(p)[i] = v; }
+    static __device__ __forceinline__ float2 ld2(const float* p, size_t i) {
+        return reinterpret_cast<const float2*>(p)[i];
+    }
+    static __device__ __forceinline__ void st2(float* p, size_t i, float2 v) { reinterpret_cast<float2*>(p)[i] = v; }
+    static __device__ __forceinline__ float ld1(const float* p, size_t i) { return p[i]; }
+    static __device__ __forceinline__ void st1(float* p, size_t i, float v) { p[i] = v; }
+};
+
+template <> struct Texel<__half> {
+    static __device__ __forceinline__ float4 ld4(const __half* p, size_t i) {
+        uint2 u = reinterpret_cast<const uint2*>(p)[i];
+        __half2 a = *reinterpret_cast<__half2*>(&u.x), b = *reinterpret_cast<__half2*>(&u.y);
+        float2 fa = __half22float2(a), fb = __half22float2(b);
+        return make_float4(fa.x, fa.y, fb.x, fb.y);
+    }
+    static __device__ __forceinline__ void st4(__half* p, size_t i, float4 v) {
+        __half2 a = __floats2half2_rn(v.x, v.y), b = __floats2half2_rn(v.z, v.w);
+        uint2 u;
+        u.x = *reinterpret_cast<unsigned*>(&a);
+        u.y = *reinterpret_cast<unsigned*>(&b);
+        reinterpret_cast<uint2*>(p)[i] = u;
+    }
+    static __device__ __forceinline__ float2 ld2(const __half* p, size_t i) {
+        return __half22float2(reinterpret_cast<const __half2*>(p)[i]);
+    }
+    static __device__ __forceinline__ void st2(__half* p, size_t i, float2 v) {
+        reinterpret_cast<__half2*>(p)[i] = __floats2half2_rn(v.x, v.y);
+    }
+    static __device__ __forceinline__ float ld1(const __half* p, size_t i) { return __half2float(p[i]); }
+    static __device__ __forceinline__ void st1(__half* p, size_t i, float v) { p[i] = __float2half_rn(v); }
+};
+
+// RGBA8 unorm colour (surface atlas only)
+struct Unorm8 {};
+template <> struct Texel<Unorm8> {
+    static __device__ __forceinline__ float4 ld4(const Unorm8* p, size_t i) {
+        uint32_t u = reinterpret_cast<const uint32_t*>(p)[i];
+        const float s = 1.0f / 255.0f;
+        return make_float4(float(u & 255u) * s, float((u >> 8) & 255u) * s, float((u >> 16) & 255u) * s,
+                           float(u >> 24) * s);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// DDGI direction helpers (Shaders/DDGI/DDGICommon.glsl:42-92)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float sign_not_zero(float k) { return k >= 0.0f ? 1.0f : -1.0f; }
+
+__device__ __forceinline__ f3 oct_decode(float ox, float oy) {
+    f3 v = {ox, oy, 1.0f - fabsf(ox) - fabsf(oy)};
+    if (v.z < 0.0f) {
+        float nx = (1.0f - fabsf(v.y)) * sign_not_zero(v.x);
+        float ny = (1.0f - fabsf(v.x)) * sign_not_zero(v.y);
+        v.x = nx;
+        v.y = ny;
+    }
+    return normalize3(v);
+}
+
+// texel direction of interior texel (lx, ly) of an oct map of side n
+// (normalizedOctCoord of DDGICommon.glsl:85-92 evaluated at the texel centre)
+__device__ __forceinline__ f3 texel_direction(int lx, int ly, int n) {
+    float s = 2.0f / float(n);
+    return oct_decode((float(lx) + 0.5f) * s - 1.0f, (float(ly) + 0.5f) * s - 1.0f);
+}
+
+__device__ __forceinline__ f3 spherical_fibonacci(float i, float n) {
+    const float PHI = sqrtf(5.0f) * 0.5f + 0.5f;
+    const float PI_F = 3.14159265358979323846f;
+    float a = i * (PHI - 1.0f);
+    float phi = 2.0f * PI_F * (a - floorf(a));
+    float cosTheta = 1.0f - (2.0f * i + 1.0f) * (1.0f / n);
+    float sinTheta = sqrtf(clampf(1.0f - cosTheta * cosTheta, 0.0f, 1.0f));
+    return {cosf(phi) * sinTheta, sinf(phi) * sinTheta, cosTheta};
+}
+
+// ---------------------------------------------------------------------------
+// sampling
+// ---------------------------------------------------------------------------
+// texture(sampler3D) linear + clamp-to-edge on a [D][H][W] volume
+template <class S>
+__device__ __forceinline__ float sample3d(const S* __restrict__ vol, int W, int H, int D, float u, float v, float w) {
+    float x = u * float(W) - 0.5f, y = v * float(H) - 0.5f, z = w * float(D) - 0.5f;
+    float fx0 = floorf(x), fy0 = floorf(y), fz0 = floorf(z);
+    float ax = x - fx0, ay = y - fy0, az = z - fz0;
+    int x0 = int(fx0), y0 = int(fy0), z0 = int(fz0);
+    int x1 = clampi(x0 + 1, 0, W - 1), y1 = clampi(y0 + 1, 0, H - 1), z1 = clampi(z0 + 1, 0, D - 1);
+    x0 = clampi(x0, 0, W - 1);
+    y0 = clampi(y0, 0, H - 1);
+    z0 = clampi(z0, 0, D - 1);
+    size_t r00 = ((size_t)z0 * H + y0) * W, r10 = ((size_t)z0 * H + y1) * W;
+    size_t r01 = ((size_t)z1 * H + y0) * W, r11 = ((size_t)z1 * H + y1) * W;
+    float v000 = Texel<S>::ld1(vol, r00 + x0), v100 = Texel<S>::ld1(vol, r00 + x1);
+    float v010 = Texel<S>::ld1(vol, r10 + x0), v110 = Texel<S>::ld1(vol, r10 + x1);
+    float v001 = Texel<S>::ld1(vol, r01 + x0), v101 = Texel<S>::ld1(vol, r01 + x1);
+    float v011 = Texel<S>::ld1(vol, r11 + x0), v111 = Texel<S>::ld1(vol, r11 + x1);
+    float c00 = v000 * (1.0f - ax) + v100 * ax;
+    float c10 = v010 * (1.0f - ax) + v110 * ax;
+    float c01 = v001 * (1.0f - ax) + v101 * ax;
+    float c11 = v011 * (1.0f - ax) + v111 * ax;
+    float c0 = c00 * (1.0f - ay) + c10 * ay;
+    float c1 = c01 * (1.0f - ay) + c11 * ay;
+    return c0 * (1.0f - az) + c1 * az;
+}
+
+}  // namespace luxgi

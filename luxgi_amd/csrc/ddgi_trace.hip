@@ -1,0 +1,445 @@
+// ddgi_trace.hip -- DDGI probe rays through the global SDF with surface-atlas
+// radiance at the hits, for gfx950.
+//
+// Reference: Shaders/DDGI/GISDFRays.comp:63-128 (ray generation and encode),
+// Shaders/SDF/SDFCommon.glsl:98-194 (tracyGlobalSDF), :64-71 (cascade UV),
+// :84-95 (lineHitAABB), :196-199 (surface threshold),
+// Shaders/SDF/AtlasCommon.glsl:50-157 (surface-atlas lookup);
+// host dispatch Engine/DDGI/DDGIRenderer.cpp:304-321.
+//
+// One lane per ray; the 64 lanes of a wave trace 64 rays of the same probe, so
+// early march steps of a wave read neighbouring voxels. The reference inverts
+// each candidate object's transform per hit (AtlasCommon.glsl:134); here the
+// inverses are computed once per call into the workspace (object_inverse_kernel).
+#include "common.hpp"
+
+namespace luxgi {
+
+constexpr float kWorldSize = 60000.0f;  // GLOBAL_SDF_WORLD_SIZE, SDFCommon.glsl:6
+constexpr int kChunksRes = 40;          // GLOBAL_SURFACE_ATLAS_CHUNKS_RESOLUTION, AtlasCommon.glsl:5
+
+struct TraceArgs {
+    LuxgiDDGIUniform u;
+    LuxgiGlobalSDFData g;
+    LuxgiSurfaceAtlasData ad;
+    float rot[9];  // column-major mat3 of randomOrientation
+    float sky[3];
+    int res, mipRes, maxSteps, skySize, P, R, rayWidth;
+    float hitEps;
+};
+
+// GLSL inverse(mat4) by cofactors (column-major), evaluated once per object.
+__global__ void object_inverse_kernel(const LuxgiAtlasObject* __restrict__ objects, int n, float* __restrict__ inv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* m = objects[i].transform;
+    float o[16];
+    o[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] + m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+    o[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] - m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+    o[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] + m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+    o[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] - m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+    o[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] - m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+    o[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] + m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+    o[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] - m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+    o[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] + m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+    o[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] + m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+    o[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] - m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+    o[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] + m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+    o[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] - m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+    o[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] - m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+    o[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] + m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+    o[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] - m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+    o[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] + m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+    const float det = m[0] * o[0] + m[1] * o[4] + m[2] * o[8] + m[3] * o[12];
+    const float r = 1.0f / det;
+    for (int k = 0; k < 16; ++k) inv[(size_t)i * 16 + k] = o[k] * r;
+}
+
+struct Counts {
+    unsigned long long mip, fine, normals, objects, tiles, steps, hits;
+};
+
+// textureGather of a repeat-addressed [res][res] image (Vulkan footprint order)
+template <class SC>
+__device__ __forceinline__ void gather_footprint(int res, float u, float v, size_t idx[4]) {
+    float x = u * float(res) - 0.5f, y = v * float(res) - 0.5f;
+    int i0 = int(floorf(x)), j0 = int(floorf(y));
+    int i1 = i0 + 1, j1 = j0 + 1;
+    auto wrap = [res](int i) {
+        int m = i % res;
+        return m < 0 ? m + res : m;
+    };
+    i0 = wrap(i0); i1 = wrap(i1); j0 = wrap(j0); j1 = wrap(j1);
+    idx[0] = (size_t)j1 * res + i0;
+    idx[1] = (size_t)j1 * res + i1;
+    idx[2] = (size_t)j0 * res + i1;
+    idx[3] = (size_t)j0 * res + i0;
+}
+
+// sampleGlobalSurfaceAtlasTile (AtlasCommon.glsl:58-112)
+template <class SC, bool COUNT>
+__device__ float4 sample_tile(const LuxgiSurfaceAtlasData& ad, const LuxgiAtlasTile* __restrict__ tp,
+                              const SC* __restrict__ color, const float* __restrict__ depth, f3 lp, f3 ln,
+                              float threshold, Counts& c) {
+    float tm[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) tm[k] = tp->transform[k];
+    const float4 ob = *reinterpret_cast<const float4*>(tp->objectBounds);
+    const float4 ex = *reinterpret_cast<const float4*>(tp->extends);
+    f3 n = normalize3(mat4_mul(tm, ln, 1.0f));
+    float nw = clampf(n.z, 0.0f, 1.0f);
+    nw = (nw - 0.05f) / (1.0f - 0.05f);
+    if (nw <= 0.0f) return make_float4(0.f, 0.f, 0.f, 0.f);
+    f3 t = mat4_mul(tm, lp, 1.0f);
+    const float tileDepth = t.z / ob.z;
+    const float tu = clampf(t.x / ob.x + 0.5f, 0.0f, 1.0f), tv = clampf(t.y / ob.y + 0.5f, 0.0f, 1.0f);
+    const float au = tu * ex.z + ex.x, av = tv * ex.w + ex.y;
+    const float bx = fractf(au * float(ad.resolution) + 0.5f), by = fractf(av * float(ad.resolution) + 0.5f);
+    float bw[4] = {(1.0f - bx) * by, bx * by, bx * (1.0f - by), (1.0f - bx) * (1.0f - by)};
+    size_t idx[4];
+    gather_footprint<SC>((int)ad.resolution, au, av, idx);
+    if (COUNT) c.tiles++;
+    const float thr = 2.0f * threshold / ob.z;
+    float vis[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float tz = depth[idx[i]];
+        vis[i] = 1.0f - clampf((fabsf(tileDepth - tz) - thr) / (0.5f * thr), 0.0f, 1.0f);
+        if (tz >= 1.0f) vis[i] = 0.0f;
+    }
+    float sw = vis[0] * bw[0] + vis[1] * bw[1] + vis[2] * bw[2] + vis[3] * bw[3];
+    sw *= nw;
+    if (sw <= 0.0f) return make_float4(0.f, 0.f, 0.f, 0.f);
+    float r = 0.f, g = 0.f, b = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float w = bw[i] * vis[i];
+        const float4 cc = Texel<SC>::ld4(color, idx[i]);
+        r += cc.x * w;
+        g += cc.y * w;
+        b += cc.z * w;
+    }
+    return make_float4(r * sw, g * sw, b * sw, sw);
+}
+
+// sampleGlobalSurfaceAtlas (AtlasCommon.glsl:115-157), debug = false
+template <class SC, bool COUNT>
+__device__ float3 sample_atlas(const LuxgiSurfaceAtlasData& ad, const uint32_t* __restrict__ chunks,
+                               const uint32_t* __restrict__ culled, const LuxgiAtlasObject* __restrict__ objects,
+                               const float* __restrict__ objInv, const LuxgiAtlasTile* __restrict__ tiles,
+                               const SC* __restrict__ color, const float* __restrict__ depth, f3 wp, f3 wn,
+                               float threshold, Counts& c) {
+    float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
+    int cc[3];
+    const float pc[3] = {wp.x, wp.y, wp.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) cc[i] = clampi(int(floorf(pc[i] / ad.chunkSize + float(kChunksRes) * 0.5f)), 0, kChunksRes - 1);
+    uint32_t start = chunks[cc[2] * kChunksRes * kChunksRes + cc[1] * kChunksRes + cc[0]];
+    if (start == 0) return make_float3(0.f, 0.f, 0.f);
+    const uint32_t count = culled[start];
+    if (count > ad.objectsCount) return make_float3(0.f, 0.f, 0.f);
+    start++;
+    for (uint32_t oi = 0; oi < count; ++oi) {
+        const uint32_t id = culled[start++];
+        const LuxgiAtlasObject* o = objects + id;
+        if (COUNT) c.objects++;
+        const float4 bs = *reinterpret_cast<const float4*>(o->objectBounds);
+        if (len3(mk(bs.x, bs.y, bs.z) - wp) > bs.w) continue;
+        const float* inv = objInv + (size_t)id * 16;
+        float m[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) m[k] = inv[k];
+        const float4 l4 = mat4_mul4(m, make_float4(wp.x, wp.y, wp.z, 1.0f));
+        const f3 lp = mk(l4.x, l4.y, l4.z);
+        const float4 e = *reinterpret_cast<const float4*>(o->extends);
+        if (fabsf(lp.x) > e.x + threshold || fabsf(lp.y) > e.y + threshold || fabsf(lp.z) > e.z + threshold) continue;
+        f3 ln = mk(m[0] * wn.x + m[4] * wn.y + m[8] * wn.z, m[1] * wn.x + m[5] * wn.y + m[9] * wn.z,
+                   m[2] * wn.x + m[6] * wn.y + m[10] * wn.z);
+        ln = normalize3(ln);
+        for (int t = 0; t < 6; ++t) {
+            const uint32_t to = o->tileOffset[t];
+            if (to != 0) {
+                float4 s = sample_tile<SC, COUNT>(ad, tiles + to, color, depth, lp, ln, threshold, c);
+                res.x += s.x;
+                res.y += s.y;
+                res.z += s.z;
+                res.w += s.w;
+            }
+        }
+    }
+    const float a = fmaxf(res.w, 0.0001f);
+    return make_float3(res.x / a, res.y / a, res.z / a);
+}
+
+// cubemap lookup, bilinear inside the face, clamp-to-edge
+__device__ float3 sample_cube(const float* __restrict__ faces, int size, f3 d) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int face;
+    float sc, tc, ma;
+    if (ax >= ay && ax >= az) { ma = ax; face = d.x >= 0 ? 0 : 1; sc = d.x >= 0 ? -d.z : d.z; tc = -d.y; }
+    else if (ay >= az) { ma = ay; face = d.y >= 0 ? 2 : 3; sc = d.x; tc = d.y >= 0 ? d.z : -d.z; }
+    else { ma = az; face = d.z >= 0 ? 4 : 5; sc = d.z >= 0 ? d.x : -d.x; tc = -d.y; }
+    const float s = 0.5f * (sc / ma + 1.0f), t = 0.5f * (tc / ma + 1.0f);
+    const float x = s * float(size) - 0.5f, y = t * float(size) - 0.5f;
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float a = x - fx0, b = y - fy0;
+    int x0 = int(fx0), y0 = int(fy0);
+    const int x1 = clampi(x0 + 1, 0, size - 1), y1 = clampi(y0 + 1, 0, size - 1);
+    x0 = clampi(x0, 0, size - 1);
+    y0 = clampi(y0, 0, size - 1);
+    const float4* f = reinterpret_cast<const float4*>(faces) + (size_t)face * size * size;
+    const float4 t00 = f[(size_t)y0 * size + x0], t10 = f[(size_t)y0 * size + x1];
+    const float4 t01 = f[(size_t)y1 * size + x0], t11 = f[(size_t)y1 * size + x1];
+    return make_float3((t00.x * (1.f - a) + t10.x * a) * (1.f - b) + (t01.x * (1.f - a) + t11.x * a) * b,
+                       (t00.y * (1.f - a) + t10.y * a) * (1.f - b) + (t01.y * (1.f - a) + t11.y * a) * b,
+                       (t00.z * (1.f - a) + t10.z * a) * (1.f - b) + (t01.z * (1.f - a) + t11.z * a) * b);
+}
+
+template <class SS, class SC, class SR, bool COUNT>
+__global__ __launch_bounds__(256) void trace_kernel(TraceArgs A, const SS* __restrict__ sdf, const SS* __restrict__ mip,
+                                                    const uint32_t* __restrict__ chunks,
+                                                    const uint32_t* __restrict__ culled,
+                                                    const LuxgiAtlasObject* __restrict__ objects,
+                                                    const float* __restrict__ objInv,
+                                                    const LuxgiAtlasTile* __restrict__ tiles,
+                                                    const SC* __restrict__ color, const float* __restrict__ atlasDepth,
+                                                    const float* __restrict__ skyFaces,
+                                                    const float* __restrict__ probeOffsets, SR* __restrict__ outRad,
+                                                    SR* __restrict__ outDD, uint8_t* __restrict__ flags,
+                                                    unsigned long long* __restrict__ counters) {
+    const int Rp = (A.R + 63) & ~63;
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int probe = int(gid / Rp), ray = int(gid % Rp);
+    const bool active = probe < A.P && ray < A.R;
+    Counts c = {0, 0, 0, 0, 0, 0, 0};
+    if (active) {
+        // probeLocation (DDGICommon.glsl:106-114)
+        const int px = A.u.probeCounts[0], pxy = A.u.probeCounts[0] * A.u.probeCounts[1];
+        const int gx = probe % px, gy = (probe % pxy) / px, gz = probe / pxy;
+        f3 origin = mk(A.u.step[0] * float(gx) + A.u.startPosition[0], A.u.step[1] * float(gy) + A.u.startPosition[1],
+                       A.u.step[2] * float(gz) + A.u.startPosition[2]);
+        if (probeOffsets) {
+            const float4 o = reinterpret_cast<const float4*>(probeOffsets)[probe];
+            origin = origin + mk(o.x, o.y, o.z);
+        }
+        // GISDFRays.comp:73
+        const f3 sf = spherical_fibonacci(float(ray), float(A.R));
+        const f3 dir = normalize3(mk(A.rot[0] * sf.x + A.rot[3] * sf.y + A.rot[6] * sf.z,
+                                     A.rot[1] * sf.x + A.rot[4] * sf.y + A.rot[7] * sf.z,
+                                     A.rot[2] * sf.x + A.rot[5] * sf.y + A.rot[8] * sf.z));
+
+        // tracyGlobalSDF (SDFCommon.glsl:98-194)
+        const int C = (int)A.g.cascadesCount;
+        const float traceMax = fminf(kWorldSize, A.g.cascadePosDistance[C - 1][3] * 2.0f);
+        const float chunkSizeDistance = 32.0f / A.g.resolution;
+        const float chunkMarginDistance = 4.0f / A.g.resolution;
+        float nextStart = 0.0f;
+        const f3 traceEnd = origin + dir * traceMax;
+        float hitTime = -1.0f, hitSdf = 0.0f;
+        int hitCascade = 0;
+        f3 hitNormal = mk(0.f, 0.f, 0.f);
+        for (int cas = 0; cas < C && hitTime < 0.0f; ++cas) {
+            const f3 cpos = mk(A.g.cascadePosDistance[cas][0], A.g.cascadePosDistance[cas][1], A.g.cascadePosDistance[cas][2]);
+            const float half = A.g.cascadePosDistance[cas][3];
+            const float voxel = A.g.cascadeVoxelSize[cas];
+            const float voxelHalf = voxel * 0.5f;
+            // lineHitAABB (SDFCommon.glsl:84-95)
+            const f3 dl = traceEnd - origin;
+            const f3 inv = mk(1.0f / dl.x, 1.0f / dl.y, 1.0f / dl.z);
+            const f3 en = (cpos - mk(half, half, half) - origin) * inv;
+            const f3 ex = (cpos + mk(half, half, half) - origin) * inv;
+            float ix = fmaxf(fminf(en.x, ex.x), fmaxf(fminf(en.y, ex.y), fminf(en.z, ex.z)));
+            float iy = fminf(fmaxf(en.x, ex.x), fminf(fmaxf(en.y, ex.y), fmaxf(en.z, ex.z)));
+            ix = clampf(ix, 0.0f, 1.0f) * traceMax;
+            iy = clampf(iy, 0.0f, 1.0f) * traceMax;
+            ix = fmaxf(ix, nextStart);
+            float stepTime = ix;
+            if (ix >= iy) stepTime = iy;
+            else nextStart = iy;
+            const float cmax = half * 2.0f;
+            const int texW = A.res * C, mipW = A.mipRes * C;
+            int step = 0;
+            for (; step < A.maxSteps && stepTime < iy; ++step) {
+                const f3 p = origin + dir * stepTime;
+                const float ux = clampf((p.x - cpos.x) / cmax + 0.5f, 0.0f, 1.0f);
+                const float uy = clampf((p.y - cpos.y) / cmax + 0.5f, 0.0f, 1.0f);
+                const float uz = clampf((p.z - cpos.z) / cmax + 0.5f, 0.0f, 1.0f);
+                const float tx = (float(cas) + ux) / float(C);
+                float sd = sample3d<SS>(mip, mipW, A.mipRes, A.mipRes, tx, uy, uz);
+                if (COUNT) c.mip++;
+                if (sd < chunkSizeDistance) {
+                    const float st = sample3d<SS>(sdf, texW, A.res, A.res, tx, uy, uz);
+                    if (COUNT) c.fine++;
+                    if (st < chunkMarginDistance * 2.0f) sd = st;
+                } else {
+                    sd = chunkSizeDistance;
+                }
+                sd *= cmax;
+                const float thick = A.hitEps > 0.0f ? A.hitEps : voxelHalf * clampf(stepTime / voxel, 0.0f, 1.0f);
+                if (sd < thick) {
+                    hitTime = fmaxf(stepTime + sd - thick, 0.0f);
+                    hitCascade = cas;
+                    hitSdf = sd;
+                    const float o = 1.0f / A.g.resolution;
+                    const float xp = sample3d<SS>(sdf, texW, A.res, A.res, tx + o, uy, uz);
+                    const float xn = sample3d<SS>(sdf, texW, A.res, A.res, tx - o, uy, uz);
+                    const float yp = sample3d<SS>(sdf, texW, A.res, A.res, tx, uy + o, uz);
+                    const float yn = sample3d<SS>(sdf, texW, A.res, A.res, tx, uy - o, uz);
+                    const float zp = sample3d<SS>(sdf, texW, A.res, A.res, tx, uy, uz + o);
+                    const float zn = sample3d<SS>(sdf, texW, A.res, A.res, tx, uy, uz - o);
+                    hitNormal = normalize3(mk(xp - xn, yp - yn, zp - zn));
+                    if (COUNT) c.normals++;
+                    break;
+                }
+                stepTime += fmaxf(sd, voxel);
+            }
+            if (COUNT) c.steps += step;
+        }
+        // ray result encode (GISDFRays.comp:89-127)
+        float4 rad = make_float4(0.f, 0.f, 0.f, 0.f);
+        float dist;
+        uint8_t f = 0;
+        if (hitTime >= 0.0f) {
+            if (COUNT) c.hits++;
+            if (hitSdf <= 0.0f && hitTime <= A.g.cascadeVoxelSize[0]) {
+                dist = kWorldSize;
+                f = 2;
+            } else {
+                const f3 hp = origin + dir * hitTime;
+                const float thr = A.g.cascadeVoxelSize[hitCascade] * 1.05f;
+                const float3 s = sample_atlas<SC, COUNT>(A.ad, chunks, culled, objects, objInv, tiles, color,
+                                                         atlasDepth, hp, hitNormal, thr, c);
+                rad = make_float4(s.x, s.y, s.z, 0.f);
+                dist = fmaxf(hitTime + A.g.cascadeVoxelSize[hitCascade] * 0.5f, 0.0f);
+                f = 1;
+            }
+        } else {
+            const float3 s = skyFaces ? sample_cube(skyFaces, A.skySize, dir) : make_float3(A.sky[0], A.sky[1], A.sky[2]);
+            rad = make_float4(s.x, s.y, s.z, 0.f);
+            dist = kWorldSize;
+        }
+        const size_t e = (size_t)probe * A.rayWidth + ray;
+        Texel<SR>::st4(outRad, e, rad);
+        Texel<SR>::st4(outDD, e, make_float4(dir.x, dir.y, dir.z, dist));
+        if (flags) flags[e] = f;
+    }
+    if (COUNT) {
+        __shared__ unsigned long long sc[8];
+        if (threadIdx.x < 8) sc[threadIdx.x] = 0;
+        __syncthreads();
+        if (active) {
+            atomicAdd(&sc[0], c.mip);
+            atomicAdd(&sc[1], c.fine);
+            atomicAdd(&sc[2], c.normals);
+            atomicAdd(&sc[3], c.objects);
+            atomicAdd(&sc[4], c.tiles);
+            atomicAdd(&sc[5], 1ull);
+            atomicAdd(&sc[6], c.steps);
+            atomicAdd(&sc[7], c.hits);
+        }
+        __syncthreads();
+        if (threadIdx.x < 8) atomicAdd(&counters[threadIdx.x], sc[threadIdx.x]);
+    }
+}
+
+}  // namespace luxgi
+
+using namespace luxgi;
+
+extern "C" size_t luxgi_ddgi_trace_workspace_size(int32_t objectCapacity) {
+    return (size_t)(objectCapacity > 0 ? objectCapacity : 0) * 16 * sizeof(float) + 256;
+}
+
+template <class SS, class SC, class SR, bool COUNT>
+static int launch_trace(const TraceArgs& A, const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at,
+                        const LuxgiSkybox* sky, const float* probeOffsets, const LuxgiRayBuffers* rays,
+                        uint8_t* rayFlags, float* objInv, unsigned long long* counters, hipStream_t st) {
+    const long long total = (long long)A.P * ((A.R + 63) & ~63);
+    const int blocks = (int)((total + 255) / 256);
+    hipLaunchKernelGGL((trace_kernel<SS, SC, SR, COUNT>), dim3(blocks), dim3(256), 0, st, A,
+                       static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), at->chunks,
+                       at->culledObjects, at->objects, objInv, at->tiles, static_cast<const SC*>(at->color), at->depth,
+                       sky && sky->faces ? sky->faces : nullptr, probeOffsets, static_cast<SR*>(rays->radiance),
+                       static_cast<SR*>(rays->directionDistance), rayFlags, counters);
+    LUXGI_LAUNCH_CHECK();
+    return LUXGI_OK;
+}
+
+template <class SS, class SR, bool COUNT>
+static int dispatch_color(const TraceArgs& A, const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at,
+                          const LuxgiSkybox* sky, const float* off, const LuxgiRayBuffers* rays, uint8_t* fl,
+                          float* inv, unsigned long long* cnt, hipStream_t st) {
+    switch (at->colorStorage) {
+        case LUXGI_STORAGE_FP32: return launch_trace<SS, float, SR, COUNT>(A, vol, at, sky, off, rays, fl, inv, cnt, st);
+        case LUXGI_STORAGE_FP16: return launch_trace<SS, __half, SR, COUNT>(A, vol, at, sky, off, rays, fl, inv, cnt, st);
+        case LUXGI_STORAGE_UNORM8: return launch_trace<SS, Unorm8, SR, COUNT>(A, vol, at, sky, off, rays, fl, inv, cnt, st);
+        default: return LUXGI_ERR_UNSUPPORTED;
+    }
+}
+
+template <bool COUNT>
+static int trace_entry(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, const LuxgiSDFVolume* vol,
+                       const LuxgiSurfaceAtlas* at, const LuxgiSkybox* sky, const LuxgiTraceParams* p,
+                       const float* probeOffsets, const LuxgiRayBuffers* rays, uint8_t* rayFlags, void* workspace,
+                       size_t workspaceBytes, unsigned long long* counters, void* stream) {
+    if (!u || !g || !vol || !at || !p || !rays) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (!vol->sdf || !vol->mip || !rays->radiance || !rays->directionDistance) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (!at->chunks || !at->culledObjects || !at->objects || !at->tiles || !at->color || !at->depth)
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    if (g->cascadesCount < 1 || g->cascadesCount > 4) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (vol->resolution <= 0 || vol->mipResolution <= 0) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (at->data.resolution == 0 || at->objectCapacity < 0) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (sky && sky->faces && sky->size <= 0) return LUXGI_ERR_INVALID_ARGUMENT;
+    TraceArgs A;
+    A.u = *u;
+    A.g = *g;
+    A.ad = at->data;
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) A.rot[c * 3 + r] = p->randomOrientation[c * 4 + r];
+    for (int i = 0; i < 3; ++i) A.sky[i] = p->skyColor[i];
+    A.res = vol->resolution;
+    A.mipRes = vol->mipResolution;
+    A.maxSteps = p->maxSteps > 0 ? p->maxSteps : 250;
+    A.skySize = sky ? sky->size : 0;
+    A.P = u->probeCounts[0] * u->probeCounts[1] * u->probeCounts[2];
+    A.R = u->raysPerProbe;
+    A.rayWidth = rays->width;
+    A.hitEps = p->hitEpsilon;
+    if (A.P <= 0 || A.R <= 0 || A.R > rays->width || rays->height < A.P) return LUXGI_ERR_INVALID_ARGUMENT;
+    const size_t need = luxgi_ddgi_trace_workspace_size(at->objectCapacity);
+    if (at->objectCapacity > 0 && (!workspace || workspaceBytes < need)) return LUXGI_ERR_WORKSPACE_TOO_SMALL;
+    hipStream_t st = as_stream(stream);
+    float* inv = static_cast<float*>(workspace);
+    if (at->objectCapacity > 0) {
+        hipLaunchKernelGGL(object_inverse_kernel, dim3((at->objectCapacity + 255) / 256), dim3(256), 0, st,
+                           at->objects, at->objectCapacity, inv);
+        LUXGI_LAUNCH_CHECK();
+    }
+    if (vol->storage == LUXGI_STORAGE_FP32 && rays->storage == LUXGI_STORAGE_FP32)
+        return dispatch_color<float, float, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, inv, counters, st);
+    if (vol->storage == LUXGI_STORAGE_FP32 && rays->storage == LUXGI_STORAGE_FP16)
+        return dispatch_color<float, __half, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, inv, counters, st);
+    if (vol->storage == LUXGI_STORAGE_FP16 && rays->storage == LUXGI_STORAGE_FP32)
+        return dispatch_color<__half, float, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, inv, counters, st);
+    if (vol->storage == LUXGI_STORAGE_FP16 && rays->storage == LUXGI_STORAGE_FP16)
+        return dispatch_color<__half, __half, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, inv, counters, st);
+    return LUXGI_ERR_UNSUPPORTED;
+}
+
+extern "C" int luxgi_ddgi_trace_sdf(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, const LuxgiSDFVolume* vol,
+                                    const LuxgiSurfaceAtlas* at, const LuxgiSkybox* sky, const LuxgiTraceParams* p,
+                                    const float* probeOffsets, const LuxgiRayBuffers* rays, uint8_t* rayFlags,
+                                    void* workspace, size_t workspaceBytes, void* stream) {
+    return trace_entry<false>(u, g, vol, at, sky, p, probeOffsets, rays, rayFlags, workspace, workspaceBytes, nullptr,
+                              stream);
+}
+
+extern "C" int luxgi_ddgi_trace_sdf_count(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g,
+                                          const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at,
+                                          const LuxgiSkybox* sky, const LuxgiTraceParams* p, const float* probeOffsets,
+                                          const LuxgiRayBuffers* rays, uint8_t* rayFlags, void* workspace,
+                                          size_t workspaceBytes, unsigned long long* counters, void* stream) {
+    if (!counters) return LUXGI_ERR_INVALID_ARGUMENT;
+    return trace_entry<true>(u, g, vol, at, sky, p, probeOffsets, rays, rayFlags, workspace, workspaceBytes, counters,
+                             stream);
+}

@@ -1,0 +1,467 @@
+// ddgi_update.hip -- DDGI probe blend (irradiance + depth) and border copy for gfx950.
+//
+// Reference passes:
+//   IrradianceProbeUpdate.comp / DepthProbeUpdate.comp  Shaders/DDGI/ProbeUpdate.glsl:53-152
+//   IrradianceBorderUpdate.comp / DepthBorderUpdate.comp Shaders/DDGI/BorderUpdate.glsl:136-156
+//   host dispatches Engine/DDGI/DDGIRenderer.cpp:347-375 (update), :416-438 (border)
+//
+// Design. The reference gives every probe one workgroup and loops over its rays
+// in 64-ray LDS chunks, evaluating a weight per (texel, ray). Every probe of a
+// frame traces the same ray directions (GISDFRays.comp:73 depends only on the
+// ray id and the frame's rotation), so the weight matrix W[texel][ray] is the
+// same for all probes and the blend is a contraction W . B with B = per-ray
+// (radiance | distance, distance^2). The library builds W once per call from
+// ray row 0 (build_weights), and the update kernel checks every staged ray's
+// direction against row 0 bit for bit: chunks that match use W, chunks that do
+// not fall back to evaluating the weights from their own directions, which is
+// the reference's arithmetic. The border copy (BorderUpdate.glsl) is fused into
+// the epilogue: the finished interior texels of a probe are staged in LDS and
+// the 4n+4 border texels are written from there.
+#include "common.hpp"
+
+namespace luxgi {
+
+constexpr int kProbesPerBlock = 4;
+constexpr int kRayChunk = 256;
+constexpr int kMaxSide = 16;  // oct side <= 16 (reference: 8 and 16)
+constexpr float kFltEps = 0.00000001f;  // ProbeUpdate.glsl:51
+
+struct UpdateArgs {
+    int px, py, pz, P, R, rayWidth;
+    int ni, nd;
+    int Wi, Wd;
+    float maxDistance, sharpness, hysteresis, invGamma;
+    int firstFrame, doIrr, doDepth, fuseBorder, forcePerProbe;
+};
+
+// Shared weights W[r][t] (thresholded at FLT_EPS as ProbeUpdate.glsl:93) and
+// their row sums S[t], from the directions of ray row 0.
+template <class SR>
+__global__ void build_weights_kernel(const SR* __restrict__ dirDist, int R, int n, int depth, float sharpness,
+                                     float* __restrict__ W, float* __restrict__ S) {
+    const int T = n * n;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const f3 td = texel_direction(t % n, t / n, n);
+    float total = 0.0f;
+    for (int r = 0; r < R; ++r) {
+        float4 d = Texel<SR>::ld4(dirDist, (size_t)r);
+        float w = fmaxf(0.0f, dot3(td, mk(d.x, d.y, d.z)));
+        if (depth) w = powf(w, sharpness);
+        if (w >= kFltEps) total += w;
+        else w = 0.0f;
+        W[(size_t)r * T + t] = w;
+    }
+    S[t] = total;
+}
+
+// Rays of one chunk staged in LDS: a = (r, g, b, depth distance), b = (dx, dy, dz, distance^2)
+struct RayLds {
+    float4 a[kProbesPerBlock][kRayChunk];
+    float4 b[kProbesPerBlock][kRayChunk];
+};
+struct OutLds {
+    float4 irr[kProbesPerBlock][kMaxSide * kMaxSide];
+    float2 dep[kProbesPerBlock][kMaxSide * kMaxSide];
+};
+union UpdateLds {
+    RayLds rays;
+    OutLds out;
+};
+
+// accumulate one texel of one map for one probe over a chunk
+template <int NC>
+__device__ __forceinline__ void accumulate(const UpdateLds& lds, int g, int t, int T, int r0, int nr, bool shared,
+                                           const float* __restrict__ W, f3 texelDir, float sharpness, float* acc,
+                                           float& total) {
+    if (shared) {
+        const float* w = W + (size_t)r0 * T + t;
+        if (NC == 3) {
+#pragma unroll 4
+            for (int r = 0; r < nr; ++r) {
+                float wr = w[(size_t)r * T];
+                float4 a = lds.rays.a[g][r];
+                acc[0] += a.x * wr;
+                acc[1] += a.y * wr;
+                acc[2] += a.z * wr;
+                total += wr;
+            }
+        } else {
+#pragma unroll 4
+            for (int r = 0; r < nr; ++r) {
+                float wr = w[(size_t)r * T];
+                float dr = lds.rays.a[g][r].w, d2 = lds.rays.b[g][r].w;
+                acc[0] += dr * wr;
+                acc[1] += d2 * wr;
+                total += wr;
+            }
+        }
+    } else {
+        // per-probe weights (ProbeUpdate.glsl:66-103)
+        for (int r = 0; r < nr; ++r) {
+            float4 a = lds.rays.a[g][r], b = lds.rays.b[g][r];
+            float w = fmaxf(0.0f, dot3(texelDir, mk(b.x, b.y, b.z)));
+            if (NC == 2) w = powf(w, sharpness);
+            if (w >= kFltEps) {
+                if (NC == 3) {
+                    acc[0] += a.x * w;
+                    acc[1] += a.y * w;
+                    acc[2] += a.z * w;
+                } else {
+                    acc[0] += a.w * w;
+                    acc[1] += b.w * w;
+                }
+                total += w;
+            }
+        }
+    }
+}
+
+template <class SR, class SA>
+__global__ __launch_bounds__(256) void probe_update_kernel(UpdateArgs A, const SR* __restrict__ radiance,
+                                                           const SR* __restrict__ dirDist,
+                                                           const SA* __restrict__ prevIrr,
+                                                           const SA* __restrict__ prevDepth, SA* __restrict__ outIrr,
+                                                           SA* __restrict__ outDepth, const float* __restrict__ Wi,
+                                                           const float* __restrict__ Si, const float* __restrict__ Wd,
+                                                           const float* __restrict__ Sd) {
+    __shared__ UpdateLds lds;
+    __shared__ int sShared[kProbesPerBlock];
+    const int tid = threadIdx.x;
+    const int p0 = blockIdx.x * kProbesPerBlock;
+    const int Ti = A.ni * A.ni, Td = A.nd * A.nd;
+    const int pxy = A.px * A.py;
+
+    // Work items. Depth: thread t < Td owns texel t of all probes of the block.
+    // Irradiance: with Ti <= 64 each wave owns one probe (t = tid % 64); else like depth.
+    const bool irrPerWave = Ti <= 64;
+    const int ti = irrPerWave ? (tid & 63) : tid;
+    const int giBegin = irrPerWave ? (tid >> 6) : 0;
+    const int giEnd = irrPerWave ? giBegin + 1 : kProbesPerBlock;
+    const bool irrActive = A.doIrr && ti < Ti;
+    const bool depActive = A.doDepth && tid < Td;
+    const f3 dirI = texel_direction(ti % A.ni, ti / A.ni, A.ni);
+    const f3 dirD = texel_direction(tid % A.nd, tid / A.nd, A.nd);
+
+    float accI[kProbesPerBlock][3], totI[kProbesPerBlock];
+    float accD[kProbesPerBlock][2], totD[kProbesPerBlock];
+#pragma unroll
+    for (int g = 0; g < kProbesPerBlock; ++g) {
+        accI[g][0] = accI[g][1] = accI[g][2] = totI[g] = 0.0f;
+        accD[g][0] = accD[g][1] = totD[g] = 0.0f;
+    }
+
+    for (int r0 = 0; r0 < A.R; r0 += kRayChunk) {
+        const int nr = min(kRayChunk, A.R - r0);
+        if (tid < kProbesPerBlock) sShared[tid] = A.forcePerProbe ? 0 : 1;
+        __syncthreads();
+        // stage rays of the chunk (ProbeUpdate.glsl:53-64, 75-78) and compare directions with row 0
+        for (int i = tid; i < kProbesPerBlock * nr; i += blockDim.x) {
+            const int g = i / nr, r = i % nr;
+            const int p = p0 + g;
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(1.f, 0.f, 0.f, 0.f);
+            if (p < A.P) {
+                const size_t e = (size_t)p * A.rayWidth + r0 + r;
+                float4 dd = Texel<SR>::ld4(dirDist, e);
+                float4 rr = A.doIrr ? Texel<SR>::ld4(radiance, e) : make_float4(0.f, 0.f, 0.f, 0.f);
+                float dist = fminf(A.maxDistance, dd.w - 0.01f);
+                if (dist == -1.0f) dist = A.maxDistance;
+                a = make_float4(rr.x, rr.y, rr.z, dist);
+                b = make_float4(dd.x, dd.y, dd.z, dist * dist);
+                float4 d0 = Texel<SR>::ld4(dirDist, (size_t)(r0 + r));
+                if (d0.x != dd.x || d0.y != dd.y || d0.z != dd.z) sShared[g] = 0;
+            }
+            lds.rays.a[g][r] = a;
+            lds.rays.b[g][r] = b;
+        }
+        __syncthreads();
+        if (depActive) {
+#pragma unroll
+            for (int g = 0; g < kProbesPerBlock; ++g) {
+                const bool sh = sShared[g] != 0;
+                accumulate<2>(lds, g, tid, Td, r0, nr, sh, Wd, dirD, A.sharpness, accD[g], totD[g]);
+            }
+        }
+        if (irrActive) {
+            for (int g = giBegin; g < giEnd; ++g) {
+                const bool sh = sShared[g] != 0;
+                accumulate<3>(lds, g, ti, Ti, r0, nr, sh, Wi, dirI, A.sharpness, accI[g], totI[g]);
+            }
+        }
+        __syncthreads();
+    }
+
+    // Epilogue: normalise, gamma, hysteresis, store interior, stage for the border.
+    if (depActive) {
+        const int lx = tid % A.nd, ly = tid / A.nd;
+        for (int g = 0; g < kProbesPerBlock; ++g) {
+            const int p = p0 + g;
+            if (p >= A.P) break;
+            const float total = totD[g];
+            float x = accD[g][0], y = accD[g][1];
+            if (total > kFltEps) {
+                float s = 1.0f / (2.0f * total);
+                x *= s;
+                y *= s;
+            }
+            const int cx = (p % pxy) * (A.nd + 2) + lx + 2, cy = (p / pxy) * (A.nd + 2) + ly + 2;
+            const size_t e = (size_t)cy * A.Wd + cx;
+            if (!A.firstFrame) {
+                float2 pv = Texel<SA>::ld2(prevDepth, e);
+                x = mixf(x, pv.x, A.hysteresis);
+                y = mixf(y, pv.y, A.hysteresis);
+            }
+            Texel<SA>::st2(outDepth, e, make_float2(x, y));
+            lds.out.dep[g][tid] = make_float2(x, y);
+        }
+    }
+    if (irrActive) {
+        const int lx = ti % A.ni, ly = ti / A.ni;
+        for (int g = giBegin; g < giEnd; ++g) {
+            const int p = p0 + g;
+            if (p >= A.P) break;
+            const float total = totI[g];
+            float r = accI[g][0], gg = accI[g][1], b = accI[g][2];
+            if (total > kFltEps) {
+                float s = 1.0f / (2.0f * total);
+                r *= s;
+                gg *= s;
+                b *= s;
+            }
+            r = powf(r, A.invGamma);
+            gg = powf(gg, A.invGamma);
+            b = powf(b, A.invGamma);
+            const int cx = (p % pxy) * (A.ni + 2) + lx + 2, cy = (p / pxy) * (A.ni + 2) + ly + 2;
+            const size_t e = (size_t)cy * A.Wi + cx;
+            if (!A.firstFrame) {
+                float4 pv = Texel<SA>::ld4(prevIrr, e);
+                r = mixf(r, pv.x, A.hysteresis);
+                gg = mixf(gg, pv.y, A.hysteresis);
+                b = mixf(b, pv.z, A.hysteresis);
+            }
+            float4 v = make_float4(r, gg, b, 1.0f);
+            Texel<SA>::st4(outIrr, e, v);
+            lds.out.irr[g][ti] = v;
+        }
+    }
+    if (!A.fuseBorder) return;
+    __syncthreads();
+    // Border texels (BorderUpdate.glsl:25-133 rule): 4n+4 copies per probe from interior texels.
+    for (int map = 0; map < 2; ++map) {
+        if ((map == 0 && !A.doIrr) || (map == 1 && !A.doDepth)) continue;
+        const int n = map == 0 ? A.ni : A.nd;
+        const int nb = 4 * n + 4;
+        for (int i = tid; i < kProbesPerBlock * nb; i += blockDim.x) {
+            const int g = i / nb, k = i % nb;
+            const int p = p0 + g;
+            if (p >= A.P) continue;
+            int sx, sy, dx, dy;
+            const int side = k / n, j = k % n + 1;
+            if (k < 4 * n) {
+                if (side == 0) { sx = n + 1 - j; sy = 1; dx = j; dy = 0; }
+                else if (side == 1) { sx = n + 1 - j; sy = n; dx = j; dy = n + 1; }
+                else if (side == 2) { sx = 1; sy = n + 1 - j; dx = 0; dy = j; }
+                else { sx = n; sy = n + 1 - j; dx = n + 1; dy = j; }
+            } else {
+                const int c = k - 4 * n;
+                const int cs[4][4] = {{n, n, 0, 0}, {1, n, n + 1, 0}, {n, 1, 0, n + 1}, {1, 1, n + 1, n + 1}};
+                sx = cs[c][0]; sy = cs[c][1]; dx = cs[c][2]; dy = cs[c][3];
+            }
+            const int ox = (p % pxy) * (n + 2) + 1, oy = (p / pxy) * (n + 2) + 1;
+            const int src = (sy - 1) * n + (sx - 1);
+            if (map == 0) {
+                Texel<SA>::st4(outIrr, (size_t)(oy + dy) * A.Wi + ox + dx, lds.out.irr[g][src]);
+            } else {
+                Texel<SA>::st2(outDepth, (size_t)(oy + dy) * A.Wd + ox + dx, lds.out.dep[g][src]);
+            }
+        }
+    }
+}
+
+// Stand-alone border copy, in place (BorderUpdate.glsl:136-156).
+template <class SA, int CH>
+__global__ void border_kernel(SA* __restrict__ atlas, int W, int n, int pxy, int P) {
+    const int p = blockIdx.x;
+    if (p >= P) return;
+    const int nb = 4 * n + 4;
+    const int ox = (p % pxy) * (n + 2) + 1, oy = (p / pxy) * (n + 2) + 1;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        int sx, sy, dx, dy;
+        const int side = k / n, j = k % n + 1;
+        if (k < 4 * n) {
+            if (side == 0) { sx = n + 1 - j; sy = 1; dx = j; dy = 0; }
+            else if (side == 1) { sx = n + 1 - j; sy = n; dx = j; dy = n + 1; }
+            else if (side == 2) { sx = 1; sy = n + 1 - j; dx = 0; dy = j; }
+            else { sx = n; sy = n + 1 - j; dx = n + 1; dy = j; }
+        } else {
+            const int c = k - 4 * n;
+            const int cs[4][4] = {{n, n, 0, 0}, {1, n, n + 1, 0}, {n, 1, 0, n + 1}, {1, 1, n + 1, n + 1}};
+            sx = cs[c][0]; sy = cs[c][1]; dx = cs[c][2]; dy = cs[c][3];
+        }
+        const size_t s = (size_t)(oy + sy) * W + ox + sx, d = (size_t)(oy + dy) * W + ox + dx;
+        if (CH == 4) Texel<SA>::st4(atlas, d, Texel<SA>::ld4(atlas, s));
+        else Texel<SA>::st2(atlas, d, Texel<SA>::ld2(atlas, s));
+    }
+}
+
+static int check_uniform(const LuxgiDDGIUniform* u) {
+    if (!u) return LUXGI_ERR_INVALID_ARGUMENT;
+    const int px = u->probeCounts[0], py = u->probeCounts[1], pz = u->probeCounts[2];
+    if (px <= 0 || py <= 0 || pz <= 0) return LUXGI_ERR_INVALID_ARGUMENT;
+    const int ni = u->irradianceProbeSideLength, nd = u->depthProbeSideLength;
+    if (ni < 1 || nd < 1) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (u->irradianceTextureWidth != (ni + 2) * px * py + 2 || u->irradianceTextureHeight != (ni + 2) * pz + 2)
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    if (u->depthTextureWidth != (nd + 2) * px * py + 2 || u->depthTextureHeight != (nd + 2) * pz + 2)
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    return LUXGI_OK;
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+}  // namespace luxgi
+
+using namespace luxgi;
+
+extern "C" int luxgi_ddgi_fill_atlas_sizes(LuxgiDDGIUniform* u) {
+    if (!u) return LUXGI_ERR_INVALID_ARGUMENT;
+    const int px = u->probeCounts[0], py = u->probeCounts[1], pz = u->probeCounts[2];
+    u->irradianceTextureWidth = (u->irradianceProbeSideLength + 2) * px * py + 2;
+    u->irradianceTextureHeight = (u->irradianceProbeSideLength + 2) * pz + 2;
+    u->depthTextureWidth = (u->depthProbeSideLength + 2) * px * py + 2;
+    u->depthTextureHeight = (u->depthProbeSideLength + 2) * pz + 2;
+    return LUXGI_OK;
+}
+
+extern "C" size_t luxgi_ddgi_probe_update_workspace_size(const LuxgiDDGIUniform* u) {
+    if (!u) return 0;
+    const size_t R = (size_t)(u->raysPerProbe > 0 ? u->raysPerProbe : 0);
+    const size_t Ti = (size_t)u->irradianceProbeSideLength * u->irradianceProbeSideLength;
+    const size_t Td = (size_t)u->depthProbeSideLength * u->depthProbeSideLength;
+    return align256(R * Ti * 4) + align256(Ti * 4) + align256(R * Td * 4) + align256(Td * 4);
+}
+
+template <class SR, class SA>
+static int launch_update(const UpdateArgs& A, const LuxgiRayBuffers* rays, const LuxgiProbeAtlas* prev,
+                         const LuxgiProbeAtlas* out, float* Wi, float* Si, float* Wd, float* Sd, hipStream_t st) {
+    const SR* rad = static_cast<const SR*>(rays->radiance);
+    const SR* dd = static_cast<const SR*>(rays->directionDistance);
+    if (!A.forcePerProbe) {
+        if (A.doIrr) {
+            const int T = A.ni * A.ni;
+            hipLaunchKernelGGL(build_weights_kernel<SR>, dim3((T + 255) / 256), dim3(256), 0, st, dd, A.R, A.ni, 0,
+                               A.sharpness, Wi, Si);
+            LUXGI_LAUNCH_CHECK();
+        }
+        if (A.doDepth) {
+            const int T = A.nd * A.nd;
+            hipLaunchKernelGGL(build_weights_kernel<SR>, dim3((T + 255) / 256), dim3(256), 0, st, dd, A.R, A.nd, 1,
+                               A.sharpness, Wd, Sd);
+            LUXGI_LAUNCH_CHECK();
+        }
+    }
+    const int blocks = (A.P + kProbesPerBlock - 1) / kProbesPerBlock;
+    hipLaunchKernelGGL((probe_update_kernel<SR, SA>), dim3(blocks), dim3(256), 0, st, A, rad, dd,
+                       static_cast<const SA*>(prev ? prev->irradiance : nullptr),
+                       static_cast<const SA*>(prev ? prev->depth : nullptr), static_cast<SA*>(out->irradiance),
+                       static_cast<SA*>(out->depth), Wi, Si, Wd, Sd);
+    LUXGI_LAUNCH_CHECK();
+    return LUXGI_OK;
+}
+
+extern "C" int luxgi_ddgi_probe_update(const LuxgiDDGIUniform* u, const LuxgiRayBuffers* rays,
+                                       const LuxgiProbeAtlas* prev, const LuxgiProbeAtlas* out, uint32_t firstFrame,
+                                       uint32_t flags, void* workspace, size_t workspaceBytes, void* stream) {
+    int rc = check_uniform(u);
+    if (rc) return rc;
+    if (!rays || !out || !rays->directionDistance) return LUXGI_ERR_INVALID_ARGUMENT;
+    UpdateArgs A{};
+    A.px = u->probeCounts[0];
+    A.py = u->probeCounts[1];
+    A.pz = u->probeCounts[2];
+    A.P = A.px * A.py * A.pz;
+    A.R = u->raysPerProbe;
+    A.rayWidth = rays->width;
+    A.ni = u->irradianceProbeSideLength;
+    A.nd = u->depthProbeSideLength;
+    A.Wi = u->irradianceTextureWidth;
+    A.Wd = u->depthTextureWidth;
+    A.maxDistance = u->maxDistance;
+    A.sharpness = u->sharpness;
+    A.hysteresis = u->hysteresis;
+    A.invGamma = 1.0f / u->ddgiGamma;
+    A.firstFrame = firstFrame != 0;
+    A.doIrr = (flags & LUXGI_UPDATE_IRRADIANCE) != 0;
+    A.doDepth = (flags & LUXGI_UPDATE_DEPTH) != 0;
+    A.fuseBorder = (flags & LUXGI_UPDATE_FUSE_BORDER) != 0;
+    A.forcePerProbe = (flags & LUXGI_UPDATE_PER_PROBE) != 0;
+    if (A.R <= 0 || A.R > rays->width || rays->height < A.P) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (A.ni > kMaxSide || A.nd > kMaxSide) return LUXGI_ERR_UNSUPPORTED;
+    if (!A.doIrr && !A.doDepth) return LUXGI_OK;
+    if (A.doIrr && (!out->irradiance || !rays->radiance)) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (A.doDepth && !out->depth) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (!A.firstFrame && (!prev || (A.doIrr && !prev->irradiance) || (A.doDepth && !prev->depth)))
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    if (prev && !A.firstFrame && prev->storage != out->storage) return LUXGI_ERR_INVALID_ARGUMENT;
+    float *Wi = nullptr, *Si = nullptr, *Wd = nullptr, *Sd = nullptr;
+    if (!A.forcePerProbe) {
+        const size_t need = luxgi_ddgi_probe_update_workspace_size(u);
+        if (!workspace || workspaceBytes < need) return LUXGI_ERR_WORKSPACE_TOO_SMALL;
+        const size_t Ti = (size_t)A.ni * A.ni, Td = (size_t)A.nd * A.nd;
+        char* w = static_cast<char*>(workspace);
+        Wi = reinterpret_cast<float*>(w);
+        w += align256((size_t)A.R * Ti * 4);
+        Si = reinterpret_cast<float*>(w);
+        w += align256(Ti * 4);
+        Wd = reinterpret_cast<float*>(w);
+        w += align256((size_t)A.R * Td * 4);
+        Sd = reinterpret_cast<float*>(w);
+    }
+    hipStream_t st = as_stream(stream);
+    const int rs = rays->storage, as = out->storage;
+    if (rs == LUXGI_STORAGE_FP32 && as == LUXGI_STORAGE_FP32)
+        return launch_update<float, float>(A, rays, prev, out, Wi, Si, Wd, Sd, st);
+    if (rs == LUXGI_STORAGE_FP16 && as == LUXGI_STORAGE_FP16)
+        return launch_update<__half, __half>(A, rays, prev, out, Wi, Si, Wd, Sd, st);
+    if (rs == LUXGI_STORAGE_FP32 && as == LUXGI_STORAGE_FP16)
+        return launch_update<float, __half>(A, rays, prev, out, Wi, Si, Wd, Sd, st);
+    if (rs == LUXGI_STORAGE_FP16 && as == LUXGI_STORAGE_FP32)
+        return launch_update<__half, float>(A, rays, prev, out, Wi, Si, Wd, Sd, st);
+    return LUXGI_ERR_UNSUPPORTED;
+}
+
+extern "C" int luxgi_ddgi_border_update(const LuxgiDDGIUniform* u, const LuxgiProbeAtlas* atlas, uint32_t which,
+                                        void* stream) {
+    int rc = check_uniform(u);
+    if (rc) return rc;
+    if (!atlas) return LUXGI_ERR_INVALID_ARGUMENT;
+    const int pxy = u->probeCounts[0] * u->probeCounts[1];
+    const int P = pxy * u->probeCounts[2];
+    hipStream_t st = as_stream(stream);
+    if (which & LUXGI_UPDATE_IRRADIANCE) {
+        if (!atlas->irradiance) return LUXGI_ERR_INVALID_ARGUMENT;
+        if (atlas->storage == LUXGI_STORAGE_FP32)
+            hipLaunchKernelGGL((border_kernel<float, 4>), dim3(P), dim3(64), 0, st,
+                               static_cast<float*>(atlas->irradiance), u->irradianceTextureWidth,
+                               u->irradianceProbeSideLength, pxy, P);
+        else if (atlas->storage == LUXGI_STORAGE_FP16)
+            hipLaunchKernelGGL((border_kernel<__half, 4>), dim3(P), dim3(64), 0, st,
+                               static_cast<__half*>(atlas->irradiance), u->irradianceTextureWidth,
+                               u->irradianceProbeSideLength, pxy, P);
+        else return LUXGI_ERR_UNSUPPORTED;
+        LUXGI_LAUNCH_CHECK();
+    }
+    if (which & LUXGI_UPDATE_DEPTH) {
+        if (!atlas->depth) return LUXGI_ERR_INVALID_ARGUMENT;
+        if (atlas->storage == LUXGI_STORAGE_FP32)
+            hipLaunchKernelGGL((border_kernel<float, 2>), dim3(P), dim3(64), 0, st, static_cast<float*>(atlas->depth),
+                               u->depthTextureWidth, u->depthProbeSideLength, pxy, P);
+        else if (atlas->storage == LUXGI_STORAGE_FP16)
+            hipLaunchKernelGGL((border_kernel<__half, 2>), dim3(P), dim3(64), 0, st,
+                               static_cast<__half*>(atlas->depth), u->depthTextureWidth, u->depthProbeSideLength, pxy,
+                               P);
+        else return LUXGI_ERR_UNSUPPORTED;
+        LUXGI_LAUNCH_CHECK();
+    }
+    return LUXGI_OK;
+}

@@ -1,0 +1,712 @@
+// svgf.hip -- SVGF temporal accumulation and a-trous filtering for gfx950.
+//
+// Reference (paths under Code/Maple/src/):
+//   Shaders/Reflection/DenoiseReprojection.comp:78-218, DenoiseAtrous.comp:41-134,
+//   DenoiseCopyTiles.comp:21-25, DenoiseReset.comp:23-32 (RGB reflection)
+//   Shaders/Shadow/DenoiseReprojection.comp:71-210, DenoiseAtrous.comp:251-339,
+//   DenoiseCopyTiles.comp:356-360 (scalar shadow visibility)
+//   Shaders/Raytraced/Reprojection.glsl:12-405, EdgeStop.glsl:7-40
+//   host drivers Engine/Raytrace/RaytracedReflection.cpp:346-520, RaytracedShadow.cpp:406-560
+//
+// Differences in structure (not in results):
+//   - The 17x17 neighbourhood statistics are separable box sums over an LDS
+//     halo tile (the reference fetches 289 texels per pixel).
+//   - Tile classification writes a deterministic per-8x8-tile byte map (OR of
+//     the pixels' votes) instead of appending tile origins with atomics; the
+//     a-trous kernel runs over all tiles and copies or filters per the map,
+//     which fuses DenoiseCopyTiles into DenoiseAtrous. The indirect-args reset
+//     pass (DenoiseReset) has nothing left to reset.
+//   - The history copy after iteration 1 (RaytracedReflection.cpp:488-491) is a
+//     second store of the same a-trous output.
+// texelFetch outside the image returns 0 (robust image access), as in the oracle.
+#include "common.hpp"
+
+namespace luxgi {
+
+constexpr float kNormalDistance = 0.1f;  // Reprojection.glsl:7
+constexpr float kPlaneDistance = 5.0f;   // Reprojection.glsl:8
+
+struct GBuf {
+    const float4* color;
+    const float4* normals;
+    const float4* pbr;
+    const float* depth;
+};
+
+__device__ __forceinline__ bool inside(int x, int y, int W, int H) { return x >= 0 && y >= 0 && x < W && y < H; }
+__device__ __forceinline__ float4 fetch4(const float4* p, int x, int y, int W, int H) {
+    return inside(x, y, W, H) ? p[(size_t)y * W + x] : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ float fetch1(const float* p, int x, int y, int W, int H) {
+    return inside(x, y, W, H) ? p[(size_t)y * W + x] : 0.0f;
+}
+template <class S>
+__device__ __forceinline__ float4 fetchS4(const S* p, int x, int y, int W, int H) {
+    return inside(x, y, W, H) ? Texel<S>::ld4(p, (size_t)y * W + x) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+template <class S>
+__device__ __forceinline__ float2 fetchS2(const S* p, int x, int y, int W, int H) {
+    return inside(x, y, W, H) ? Texel<S>::ld2(p, (size_t)y * W + x) : make_float2(0.f, 0.f);
+}
+
+// Math.glsl:21-27
+__device__ __forceinline__ f3 world_from_depth(float u, float v, float depth, const float* m) {
+    const float4 w = mat4_mul4(m, make_float4(u * 2.0f - 1.0f, v * 2.0f - 1.0f, depth, 1.0f));
+    return mk(w.x / w.w, w.y / w.w, w.z / w.w);
+}
+
+// isValid (Reprojection.glsl:42-53)
+__device__ __forceinline__ bool is_valid(int hx, int hy, f3 pos, f3 prevPos, f3 n, f3 pn, float meshId,
+                                         float prevMeshId, int W, int H) {
+    if (hx < 0 || hy < 0 || hx > W - 1 || hy > H - 1) return false;
+    if (meshId != prevMeshId) return false;
+    if (fabsf(dot3(pos - prevPos, n)) > kPlaneDistance) return false;
+    const float d = fabsf(dot3(n, pn));
+    if (d * d <= kNormalDistance) return false;
+    return true;
+}
+
+__device__ __forceinline__ float luminance(float r, float g, float b) {  // ReflectionCommon.glsl:19-22
+    return fmaxf(r * 0.299f + g * 0.587f + b * 0.114f, 0.0001f);
+}
+
+// ---------------------------------------------------------------------------
+// reflection reprojection
+// ---------------------------------------------------------------------------
+struct ReflArgs {
+    LuxgiReflectionReprojectParams pc;
+    int W, H, tilesX;
+};
+
+// reproject, RGB overload (Reprojection.glsl:242-405)
+template <class S>
+__device__ bool reproject_rgb(const ReflArgs& A, int cx, int cy, float depth, float rayLength, const GBuf& cur,
+                              const GBuf& prev, const S* histOut, const S* histMom, f3& hc, float2& hm, float& hl) {
+    const int W = A.W, H = A.H;
+    const float sx = float(W), sy = float(H);
+    const float tcx = (float(cx) + 0.5f) / sx, tcy = (float(cy) + 0.5f) / sy;
+    const float4 normal = fetch4(cur.normals, cx, cy, W, H);
+    const float mx = normal.z, my = normal.w;
+    const f3 wpos = world_from_depth(tcx, tcy, depth, A.pc.viewProjInv);
+    const float meshId = fetch4(cur.color, cx, cy, W, H).w;
+    const float curvature = fetch4(cur.pbr, cx, cy, W, H).z;
+    const float htx = tcx + mx, hty = tcy + my;
+    float rx = float(cx) + mx * sx, ry = float(cy) + my * sy;
+    if (rayLength > 0.0f && curvature == 0.0f) {  // virtualPointReprojection (Reprojection.glsl:56-75)
+        const f3 ro = world_from_depth(float(cx) / sx, float(cy) / sy, depth, A.pc.viewProjInv);
+        const f3 cp = mk(A.pc.cameraPos[0], A.pc.cameraPos[1], A.pc.cameraPos[2]);
+        f3 cr = ro - cp;
+        const float cl = len3(cr);
+        cr = normalize3(cr);
+        const f3 ph = cp + cr * (cl + rayLength);
+        float4 rp = mat4_mul4(A.pc.prevViewProj, make_float4(ph.x, ph.y, ph.z, 1.0f));
+        rp.x /= rp.w;
+        rp.y /= rp.w;
+        rx = (rp.x * 0.5f + 0.5f) * sx;
+        ry = (rp.y * 0.5f + 0.5f) * sy;
+    }
+    const int hx = int(rx), hy = int(ry);  // GLSL ivec2(): truncation
+    const f3 n = oct_decode(normal.x, normal.y);
+    bool v[4];
+    bool valid = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int lx = hx + (i & 1), ly = hy + (i >> 1);
+        const float4 pn = fetch4(prev.normals, lx, ly, W, H);
+        const float pd = fetch1(prev.depth, lx, ly, W, H);
+        const f3 pp = world_from_depth(htx, hty, pd, A.pc.viewProjInv);
+        const float pm = fetch4(prev.color, lx, ly, W, H).w;
+        v[i] = is_valid(hx, hy, wpos, pp, n, oct_decode(pn.x, pn.y), meshId, pm, W, H);
+        valid = valid || v[i];
+    }
+    f3 col = mk(0.f, 0.f, 0.f);
+    float2 mom = make_float2(0.f, 0.f);
+    if (valid) {
+        float sumw = 0.0f;
+        const float x = fractf(rx), y = fractf(ry);
+        const float w[4] = {(1.0f - x) * (1.0f - y), x * (1.0f - y), (1.0f - x) * y, x * y};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int lx = hx + (i & 1), ly = hy + (i >> 1);
+            if (v[i]) {
+                const float4 c = fetchS4<S>(histOut, lx, ly, W, H);
+                const float4 m = fetchS4<S>(histMom, lx, ly, W, H);
+                col = col + mk(c.x, c.y, c.z) * w[i];
+                mom.x += w[i] * m.x;
+                mom.y += w[i] * m.y;
+                sumw += w[i];
+            }
+        }
+        valid = sumw >= 0.01f;
+        if (valid) {
+            col = col * (1.0f / sumw);
+            mom.x /= sumw;
+            mom.y /= sumw;
+        } else {
+            col = mk(0.f, 0.f, 0.f);
+            mom = make_float2(0.f, 0.f);
+        }
+    }
+    if (!valid) {
+        float count = 0.0f;
+        for (int yy = -1; yy <= 1; ++yy)
+            for (int xx = -1; xx <= 1; ++xx) {
+                const int px = hx + xx, py = hy + yy;
+                const float4 pn = fetch4(prev.normals, px, py, W, H);
+                const float pd = fetch1(prev.depth, px, py, W, H);
+                const f3 pp = world_from_depth(htx, hty, pd, A.pc.viewProjInv);
+                const float pm = fetch4(prev.color, px, py, W, H).w;
+                if (is_valid(hx, hy, wpos, pp, n, oct_decode(pn.x, pn.y), meshId, pm, W, H)) {
+                    const float4 c = fetchS4<S>(histOut, px, py, W, H);
+                    const float4 m = fetchS4<S>(histMom, px, py, W, H);
+                    col = col + mk(c.x, c.y, c.z);
+                    mom.x += m.x;
+                    mom.y += m.y;
+                    count += 1.0f;
+                }
+            }
+        if (count > 0.0f) {
+            valid = true;
+            col = col * (1.0f / count);
+            mom.x /= count;
+            mom.y /= count;
+        }
+    }
+    if (valid) {
+        hl = fetchS4<S>(histMom, hx, hy, W, H).z;
+    } else {
+        col = mk(0.f, 0.f, 0.f);
+        mom = make_float2(0.f, 0.f);
+        hl = 0.0f;
+    }
+    hc = col;
+    hm = mom;
+    return valid;
+}
+
+// 16x16 pixels per block; LDS halo of 8 for the 17x17 statistics.
+template <class S>
+__global__ __launch_bounds__(256) void reflection_reproject_kernel(ReflArgs A, GBuf cur, GBuf prev,
+                                                                   const S* __restrict__ noisy,
+                                                                   const S* __restrict__ histOut,
+                                                                   const S* __restrict__ histMom, S* __restrict__ outColor,
+                                                                   S* __restrict__ outMom, uint8_t* __restrict__ tileMap) {
+    __shared__ float3 tile[32][33];
+    __shared__ float3 h1[32][16], h2[32][16];
+    __shared__ int vote[4];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int bx = blockIdx.x * 16, by = blockIdx.y * 16;
+    const int W = A.W, H = A.H;
+    if (threadIdx.x < 4) vote[threadIdx.x] = 0;
+    for (int i = threadIdx.x; i < 32 * 32; i += 256) {
+        const int lx = i & 31, ly = i >> 5;
+        const float4 c = fetchS4<S>(noisy, bx - 8 + lx, by - 8 + ly, W, H);
+        tile[ly][lx] = make_float3(c.x, c.y, c.z);
+    }
+    __syncthreads();
+    for (int row = ty; row < 32; row += 16) {
+        float3 s1 = make_float3(0.f, 0.f, 0.f), s2 = make_float3(0.f, 0.f, 0.f);
+        for (int k = 0; k < 17; ++k) {
+            const float3 c = tile[row][tx + k];
+            s1.x += c.x; s1.y += c.y; s1.z += c.z;
+            s2.x += c.x * c.x; s2.y += c.y * c.y; s2.z += c.z * c.z;
+        }
+        h1[row][tx] = s1;
+        h2[row][tx] = s2;
+    }
+    __syncthreads();
+    float3 m1 = make_float3(0.f, 0.f, 0.f), m2 = make_float3(0.f, 0.f, 0.f);
+    for (int k = 0; k < 17; ++k) {
+        const float3 a = h1[ty + k][tx], b = h2[ty + k][tx];
+        m1.x += a.x; m1.y += a.y; m1.z += a.z;
+        m2.x += b.x; m2.y += b.y; m2.z += b.z;
+    }
+    const int x = bx + tx, y = by + ty;
+    if (x < W && y < H) {
+        const float depth = cur.depth[(size_t)y * W + x];
+        const float rough = cur.pbr[(size_t)y * W + x].y;
+        float4 oc = make_float4(0.f, 0.f, 0.f, 0.f), om = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (depth != 1.0f) {
+            const float4 cr = Texel<S>::ld4(noisy, (size_t)y * W + x);
+            f3 hc;
+            float2 hm;
+            float hl;
+            const bool ok = reproject_rgb<S>(A, x, y, depth, cr.w, cur, prev, histOut, histMom, hc, hm, hl);
+            hl = fminf(32.0f, ok ? hl + 1.0f : 1.0f);
+            if (ok) {  // neighborhoodStandardDeviation + clipAABB (Reflection/DenoiseReprojection.comp:78-114)
+                const float inv = 1.0f / 289.0f;
+                const f3 mean = mk(m1.x * inv, m1.y * inv, m1.z * inv);
+                const f3 var = mk(m2.x * inv, m2.y * inv, m2.z * inv) - mean * mean;
+                const f3 sd = mk(sqrtf(fmaxf(var.x, 0.f)), sqrtf(fmaxf(var.y, 0.f)), sqrtf(fmaxf(var.z, 0.f)));
+                const f3 mn = mean - sd, mx = mean + sd;
+                const f3 ctr = (mx + mn) * 0.5f;
+                const f3 ext = (mx - mn) * 0.5f + mk(0.001f, 0.001f, 0.001f);
+                const f3 cv = hc - ctr;
+                const float mu = fmaxf(fmaxf(fabsf(cv.x / ext.x), fabsf(cv.y / ext.y)), fabsf(cv.z / ext.z));
+                if (mu > 1.0f) hc = ctr + cv * (1.0f / mu);
+            }
+            const float cd = sqrtf(A.pc.cameraDelta[0] * A.pc.cameraDelta[0] + A.pc.cameraDelta[1] * A.pc.cameraDelta[1] +
+                                   A.pc.cameraDelta[2] * A.pc.cameraDelta[2]);
+            const float maxAcc = cd > 0.0f ? 8.0f : hl;
+            const float alpha = ok ? fmaxf(A.pc.alpha, 1.0f / maxAcc) : 1.0f;
+            const float alphaM = ok ? fmaxf(A.pc.momentsAlpha, 1.0f / maxAcc) : 1.0f;
+            const float l = luminance(cr.x, cr.y, cr.z);
+            const float mm0 = mixf(hm.x, l, alphaM), mm1 = mixf(hm.y, l * l, alphaM);
+            const float var = fmaxf(0.0f, mm1 - mm0 * mm0);
+            oc = make_float4(mixf(hc.x, cr.x, alpha), mixf(hc.y, cr.y, alpha), mixf(hc.z, cr.z, alpha), var);
+            om = make_float4(mm0, mm1, hl, 0.0f);
+            if (!(rough <= 0.05f || rough >= 0.45f)) vote[(ty >> 3) * 2 + (tx >> 3)] = 1;
+        }
+        Texel<S>::st4(outMom, (size_t)y * W + x, om);
+        Texel<S>::st4(outColor, (size_t)y * W + x, oc);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int tX = blockIdx.x * 2 + (threadIdx.x & 1), tY = blockIdx.y * 2 + (threadIdx.x >> 1);
+        if (tX * 8 < W && tY * 8 < H) tileMap[tY * A.tilesX + tX] = (uint8_t)vote[threadIdx.x];
+    }
+}
+
+// computeEdgeStoppingWeight (EdgeStop.glsl:24-40)
+__device__ __forceinline__ float edge_stop(float cd, float sd, float phiZ, f3 cn, f3 sn, float phiN, float cl, float sl,
+                                           float phiL) {
+    const float wZ = expf(-fabsf(cd - sd) / phiZ);
+    const float wN = powf(clampf(dot3(cn, sn), 0.0f, 1.0f), phiN);
+    const float wL = fabsf(cl - sl) / phiL;
+    return expf(0.0f - fmaxf(wL, 0.0f) - fmaxf(wZ, 0.0f)) * wN;
+}
+
+struct AtrousArgs {
+    int W, H, tilesX, radius, stepSize;
+    float phi, phiNormal, sigmaDepth, power;
+};
+
+// one 8x8 tile per block: Reflection/DenoiseAtrous.comp:63-134 or DenoiseCopyTiles.comp:21-25
+template <class S>
+__global__ __launch_bounds__(64) void reflection_atrous_kernel(AtrousArgs A, const float4* __restrict__ normals,
+                                                               const float4* __restrict__ pbr,
+                                                               const uint8_t* __restrict__ tileMap,
+                                                               const S* __restrict__ in, S* __restrict__ out,
+                                                               S* __restrict__ hist) {
+    const int x = blockIdx.x * 8 + (threadIdx.x & 7), y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    const int W = A.W, H = A.H;
+    if (x >= W || y >= H) return;
+    const size_t e = (size_t)y * W + x;
+    const float4 c = Texel<S>::ld4(in, e);
+    float4 o;
+    if (!tileMap[blockIdx.y * A.tilesX + blockIdx.x]) {
+        o = c;
+    } else {
+        const float cl = luminance(c.x, c.y, c.z);
+        const float kv[2][2] = {{1.0f / 4.0f, 1.0f / 8.0f}, {1.0f / 8.0f, 1.0f / 16.0f}};
+        float var = 0.0f;  // computeVarianceCenter (:41-61)
+        for (int yy = -1; yy <= 1; ++yy)
+            for (int xx = -1; xx <= 1; ++xx) var += fetchS4<S>(in, x + xx, y + yy, W, H).w * kv[abs(xx)][abs(yy)];
+        const float4 nrm = normals[e];
+        const float4 pb = pbr[e];
+        const float cd = pb.w, rough = pb.y;
+        if (cd == 1.0f) {
+            o = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else if (rough <= 0.05f || rough >= 0.45f) {
+            o = c;
+        } else {
+            const float kw[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
+            const float phiC = A.phi * sqrtf(fmaxf(0.0f, var)) + 1e-10f;
+            float sw = 1.0f;
+            float4 s = c;
+            const f3 cn = oct_decode(nrm.x, nrm.y);
+            for (int yy = -A.radius; yy <= A.radius; ++yy)
+                for (int xx = -A.radius; xx <= A.radius; ++xx) {
+                    const int px = x + xx * A.stepSize, py = y + yy * A.stepSize;
+                    if (inside(px, py, W, H) && (xx != 0 || yy != 0)) {
+                        const float k = kw[abs(xx)] * kw[abs(yy)];
+                        const size_t q = (size_t)py * W + px;
+                        const float4 sc = Texel<S>::ld4(in, q);
+                        const float sl = luminance(sc.x, sc.y, sc.z);
+                        const float4 sn = normals[q];
+                        const float d = pbr[q].w;
+                        const float w = edge_stop(cd, d, A.sigmaDepth, cn, oct_decode(sn.x, sn.y), A.phiNormal, cl, sl, phiC);
+                        const float wv = w * k;
+                        sw += wv;
+                        s.x += wv * sc.x;
+                        s.y += wv * sc.y;
+                        s.z += wv * sc.z;
+                        s.w += wv * wv * sc.w;
+                    }
+                }
+            o = make_float4(s.x / sw, s.y / sw, s.z / sw, s.w / (sw * sw));
+        }
+    }
+    Texel<S>::st4(out, e, o);
+    if (hist) Texel<S>::st4(hist, e, o);
+}
+
+// ---------------------------------------------------------------------------
+// shadow
+// ---------------------------------------------------------------------------
+struct ShadowArgs {
+    LuxgiShadowReprojectParams pc;
+    int W, H, tilesX, maskW, maskH;
+};
+
+__device__ __forceinline__ float shadow_bit(const uint32_t* mask, int mw, int mh, int x, int y) {
+    if (x < 0 || y < 0) return 0.0f;
+    const int wx = x >> 3, wy = y >> 2;
+    if (wx >= mw || wy >= mh) return 0.0f;
+    return float((mask[(size_t)wy * mw + wx] >> ((y & 3) * 8 + (x & 7))) & 1u);
+}
+
+// reproject, scalar overload (Reprojection.glsl:90-238)
+template <class S>
+__device__ bool reproject_scalar(const ShadowArgs& A, int cx, int cy, float depth, const GBuf& cur, const GBuf& prev,
+                                 const S* histOut, const S* histMom, float& hc, float2& hm, float& hl) {
+    const int W = A.W, H = A.H;
+    const float sx = float(W), sy = float(H);
+    const float pcx = float(cx) + 0.5f, pcy = float(cy) + 0.5f;
+    const float tcx = pcx / sx, tcy = pcy / sy;
+    const float4 normal = fetch4(cur.normals, cx, cy, W, H);
+    const float mx = normal.z, my = normal.w;
+    const f3 wpos = world_from_depth(tcx, tcy, depth, A.pc.viewProjInv);
+    const float meshId = fetch4(cur.color, cx, cy, W, H).w;
+    const int hx = int(pcx + mx * sx), hy = int(pcy + my * sy);
+    const float fx = floorf(float(cx)) + mx * sx, fy = floorf(float(cy)) + my * sy;
+    const int fxi = int(fx), fyi = int(fy);
+    const float htx = tcx + mx, hty = tcy + my;
+    const f3 n = oct_decode(normal.x, normal.y);
+    bool v[4];
+    bool valid = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int lx = fxi + (i & 1), ly = fyi + (i >> 1);
+        const float4 pn = fetch4(prev.normals, lx, ly, W, H);
+        const float pd = fetch1(prev.depth, lx, ly, W, H);
+        const f3 pp = world_from_depth(htx, hty, pd, A.pc.viewProjInv);
+        const float pm = fetch4(prev.color, lx, ly, W, H).w;
+        v[i] = is_valid(hx, hy, wpos, pp, n, oct_decode(pn.x, pn.y), meshId, pm, W, H);
+        valid = valid || v[i];
+    }
+    float col = 0.0f;
+    float2 mom = make_float2(0.f, 0.f);
+    if (valid) {
+        float sumw = 0.0f;
+        const float x = fractf(fx), y = fractf(fy);
+        const float w[4] = {(1.0f - x) * (1.0f - y), x * (1.0f - y), (1.0f - x) * y, x * y};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int lx = fxi + (i & 1), ly = fyi + (i >> 1);
+            if (v[i]) {
+                col += w[i] * fetchS2<S>(histOut, lx, ly, W, H).x;
+                const float4 m = fetchS4<S>(histMom, lx, ly, W, H);
+                mom.x += w[i] * m.x;
+                mom.y += w[i] * m.y;
+                sumw += w[i];
+            }
+        }
+        valid = sumw >= 0.01f;
+        if (valid) {
+            col /= sumw;
+            mom.x /= sumw;
+            mom.y /= sumw;
+        } else {
+            col = 0.0f;
+            mom = make_float2(0.f, 0.f);
+        }
+    }
+    if (!valid) {
+        float count = 0.0f;
+        for (int yy = -1; yy <= 1; ++yy)
+            for (int xx = -1; xx <= 1; ++xx) {
+                const int px = hx + xx, py = hy + yy;
+                const float4 pn = fetch4(prev.normals, px, py, W, H);
+                const float pd = fetch1(prev.depth, px, py, W, H);
+                const f3 pp = world_from_depth(htx, hty, pd, A.pc.viewProjInv);
+                const float pm = fetch4(prev.color, px, py, W, H).w;
+                if (is_valid(hx, hy, wpos, pp, n, oct_decode(pn.x, pn.y), meshId, pm, W, H)) {
+                    col += fetchS2<S>(histOut, px, py, W, H).x;
+                    const float4 m = fetchS4<S>(histMom, px, py, W, H);
+                    mom.x += m.x;
+                    mom.y += m.y;
+                    count += 1.0f;
+                }
+            }
+        if (count > 0.0f) {
+            valid = true;
+            col /= count;
+            mom.x /= count;
+            mom.y /= count;
+        }
+    }
+    if (valid) hl = fetchS4<S>(histMom, hx, hy, W, H).z;
+    else {
+        col = 0.0f;
+        mom = make_float2(0.f, 0.f);
+        hl = 0.0f;
+    }
+    hc = col;
+    hm = mom;
+    return valid;
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void shadow_reproject_kernel(ShadowArgs A, GBuf cur, GBuf prev,
+                                                               const uint32_t* __restrict__ mask,
+                                                               const S* __restrict__ histOut,
+                                                               const S* __restrict__ histMom, S* __restrict__ outColor,
+                                                               S* __restrict__ outMom, uint8_t* __restrict__ tileMap) {
+    __shared__ float bits[32][33];
+    __shared__ float hs[32][16];
+    __shared__ int vote[4];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int bx = blockIdx.x * 16, by = blockIdx.y * 16;
+    const int W = A.W, H = A.H;
+    if (threadIdx.x < 4) vote[threadIdx.x] = 0;
+    for (int i = threadIdx.x; i < 32 * 32; i += 256) {
+        const int lx = i & 31, ly = i >> 5;
+        bits[ly][lx] = shadow_bit(mask, A.maskW, A.maskH, bx - 8 + lx, by - 8 + ly);
+    }
+    __syncthreads();
+    for (int row = ty; row < 32; row += 16) {
+        float s = 0.0f;
+        for (int k = 0; k < 17; ++k) s += bits[row][tx + k];
+        hs[row][tx] = s;
+    }
+    __syncthreads();
+    float sum = 0.0f;
+    for (int k = 0; k < 17; ++k) sum += hs[ty + k][tx];
+    const float mean = sum / 289.0f;  // computeMean (Shadow/DenoiseReprojection.comp:103-124)
+    const int x = bx + tx, y = by + ty;
+    if (x < W && y < H) {
+        const float depth = cur.depth[(size_t)y * W + x];
+        float2 ov = make_float2(0.f, 0.f), om = make_float2(0.f, 0.f);
+        float hl = 0.0f;
+        if (depth != 1.0f) {
+            const float vis = bits[ty + 8][tx + 8];
+            float hv;
+            float2 hm;
+            const bool ok = reproject_scalar<S>(A, x, y, depth, cur, prev, histOut, histMom, hv, hm, hl);
+            hl = fminf(32.0f, ok ? hl + 1.0f : 1.0f);
+            if (ok) {
+                const float sv = fmaxf(mean - mean * mean, 0.0f);
+                const float sd = sqrtf(sv);
+                hv = clampf(hv, mean - 0.5f * sd, mean + 0.5f * sd);
+            }
+            const float alpha = ok ? fmaxf(A.pc.alpha, 1.0f / hl) : 1.0f;
+            const float alphaM = ok ? fmaxf(A.pc.momentsAlpha, 1.0f / hl) : 1.0f;
+            om.x = mixf(hm.x, vis, alphaM);
+            om.y = mixf(hm.y, vis * vis, alphaM);
+            ov.x = mixf(hv, vis, alpha);
+            ov.y = fmaxf(0.0f, om.y - om.x * om.x);
+            if (ov.x > 0.0f) vote[(ty >> 3) * 2 + (tx >> 3)] = 1;
+        }
+        Texel<S>::st4(outMom, (size_t)y * W + x, make_float4(om.x, om.y, hl, 0.0f));
+        Texel<S>::st2(outColor, (size_t)y * W + x, ov);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int tX = blockIdx.x * 2 + (threadIdx.x & 1), tY = blockIdx.y * 2 + (threadIdx.x >> 1);
+        if (tX * 8 < W && tY * 8 < H) tileMap[tY * A.tilesX + tX] = (uint8_t)vote[threadIdx.x];
+    }
+}
+
+// one 8x8 tile per block: Shadow/DenoiseAtrous.comp:275-339 or Shadow/DenoiseCopyTiles.comp (writes 0)
+template <class S>
+__global__ __launch_bounds__(64) void shadow_atrous_kernel(AtrousArgs A, const float4* __restrict__ normals,
+                                                           const float4* __restrict__ pbr,
+                                                           const uint8_t* __restrict__ tileMap, const S* __restrict__ in,
+                                                           S* __restrict__ out, S* __restrict__ hist) {
+    const int x = blockIdx.x * 8 + (threadIdx.x & 7), y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    const int W = A.W, H = A.H;
+    if (x >= W || y >= H) return;
+    const size_t e = (size_t)y * W + x;
+    float2 o;
+    if (!tileMap[blockIdx.y * A.tilesX + blockIdx.x]) {
+        o = make_float2(0.f, 0.f);
+    } else {
+        const float2 c = Texel<S>::ld2(in, e);
+        const float kv[2][2] = {{1.0f / 4.0f, 1.0f / 8.0f}, {1.0f / 8.0f, 1.0f / 16.0f}};
+        float var = 0.0f;
+        for (int yy = -1; yy <= 1; ++yy)
+            for (int xx = -1; xx <= 1; ++xx) var += fetchS2<S>(in, x + xx, y + yy, W, H).y * kv[abs(xx)][abs(yy)];
+        const float4 nrm = normals[e];
+        const float cd = pbr[e].w;
+        if (cd < 0.0f) {
+            o = c;
+        } else {
+            const float kw[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
+            const float phiV = A.phi * sqrtf(fmaxf(0.0f, 1e-10f + var));
+            float sw = 1.0f, s0 = c.x, s1 = c.y;
+            const f3 cn = oct_decode(nrm.x, nrm.y);
+            for (int yy = -A.radius; yy <= A.radius; ++yy)
+                for (int xx = -A.radius; xx <= A.radius; ++xx) {
+                    const int px = x + xx * A.stepSize, py = y + yy * A.stepSize;
+                    if (inside(px, py, W, H) && (xx != 0 || yy != 0)) {
+                        const float k = kw[abs(xx)] * kw[abs(yy)];
+                        const size_t q = (size_t)py * W + px;
+                        const float2 sv = Texel<S>::ld2(in, q);
+                        const float4 sn = normals[q];
+                        const float d = pbr[q].w;
+                        const float w = edge_stop(cd, d, A.sigmaDepth, cn, oct_decode(sn.x, sn.y), A.phiNormal, c.x, sv.x, phiV);
+                        const float wv = w * k;
+                        sw += wv;
+                        s0 += wv * sv.x;
+                        s1 += wv * wv * sv.y;
+                    }
+                }
+            o = make_float2(s0 / sw, s1 / (sw * sw));
+            if (A.power != 0.0f) o.x = powf(o.x, A.power);
+        }
+    }
+    Texel<S>::st2(out, e, o);
+    if (hist) Texel<S>::st2(hist, e, o);
+}
+
+static GBuf to_gbuf(const LuxgiGBuffer* g) {
+    return GBuf{static_cast<const float4*>(static_cast<const void*>(g->color)),
+                static_cast<const float4*>(static_cast<const void*>(g->normals)),
+                static_cast<const float4*>(static_cast<const void*>(g->pbr)), g->depth};
+}
+static bool gbuf_ok(const LuxgiGBuffer* g) { return g && g->color && g->normals && g->pbr && g->depth; }
+
+}  // namespace luxgi
+
+using namespace luxgi;
+
+extern "C" int32_t luxgi_svgf_tile_count(int32_t width, int32_t height) {
+    if (width <= 0 || height <= 0) return 0;
+    return ((width + 7) / 8) * ((height + 7) / 8);
+}
+
+template <class S>
+static int refl_reproject(const LuxgiReflectionReprojectParams* p, int W, int H, const LuxgiGBuffer* cur,
+                          const LuxgiGBuffer* prev, const void* noisy, const void* hc, const void* hm, void* oc,
+                          void* om, uint8_t* tiles, hipStream_t st) {
+    ReflArgs A{*p, W, H, (W + 7) / 8};
+    hipLaunchKernelGGL(reflection_reproject_kernel<S>, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, st, A,
+                       to_gbuf(cur), to_gbuf(prev), static_cast<const S*>(noisy), static_cast<const S*>(hc),
+                       static_cast<const S*>(hm), static_cast<S*>(oc), static_cast<S*>(om), tiles);
+    LUXGI_LAUNCH_CHECK();
+    return LUXGI_OK;
+}
+
+extern "C" int luxgi_svgf_reflection_reproject(const LuxgiReflectionReprojectParams* p, int32_t W, int32_t H,
+                                               int32_t storage, const LuxgiGBuffer* cur, const LuxgiGBuffer* prev,
+                                               const void* noisy, const void* historyColor, const void* historyMoments,
+                                               void* outColor, void* outMoments, uint8_t* tileMap, void* stream) {
+    if (!p || W <= 0 || H <= 0 || !gbuf_ok(cur) || !gbuf_ok(prev) || !noisy || !historyColor || !historyMoments ||
+        !outColor || !outMoments || !tileMap)
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    hipStream_t st = as_stream(stream);
+    if (storage == LUXGI_STORAGE_FP32)
+        return refl_reproject<float>(p, W, H, cur, prev, noisy, historyColor, historyMoments, outColor, outMoments,
+                                     tileMap, st);
+    if (storage == LUXGI_STORAGE_FP16)
+        return refl_reproject<__half>(p, W, H, cur, prev, noisy, historyColor, historyMoments, outColor, outMoments,
+                                      tileMap, st);
+    return LUXGI_ERR_UNSUPPORTED;
+}
+
+extern "C" int luxgi_svgf_reflection_atrous(const LuxgiReflectionAtrousParams* p, int32_t W, int32_t H,
+                                            int32_t storage, const LuxgiGBuffer* cur, const uint8_t* tileMap,
+                                            const void* input, void* output, void* historyOut, void* stream) {
+    if (!p || W <= 0 || H <= 0 || !cur || !cur->normals || !cur->pbr || !tileMap || !input || !output)
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    if (p->radius < 0 || p->radius > 2) return LUXGI_ERR_UNSUPPORTED;  // kernelWeights has 3 entries
+    AtrousArgs A{W, H, (W + 7) / 8, p->radius, p->stepSize, p->phiColor, p->phiNormal, p->sigmaDepth, 0.0f};
+    hipStream_t st = as_stream(stream);
+    const dim3 grid((W + 7) / 8, (H + 7) / 8);
+    const float4* nrm = static_cast<const float4*>(static_cast<const void*>(cur->normals));
+    const float4* pbr = static_cast<const float4*>(static_cast<const void*>(cur->pbr));
+    if (storage == LUXGI_STORAGE_FP32)
+        hipLaunchKernelGGL(reflection_atrous_kernel<float>, grid, dim3(64), 0, st, A, nrm, pbr, tileMap,
+                           static_cast<const float*>(input), static_cast<float*>(output), static_cast<float*>(historyOut));
+    else if (storage == LUXGI_STORAGE_FP16)
+        hipLaunchKernelGGL(reflection_atrous_kernel<__half>, grid, dim3(64), 0, st, A, nrm, pbr, tileMap,
+                           static_cast<const __half*>(input), static_cast<__half*>(output),
+                           static_cast<__half*>(historyOut));
+    else return LUXGI_ERR_UNSUPPORTED;
+    LUXGI_LAUNCH_CHECK();
+    return LUXGI_OK;
+}
+
+extern "C" int luxgi_svgf_reflection_denoise(const LuxgiReflectionReprojectParams* rp,
+                                             const LuxgiReflectionAtrousParams* ap, int32_t iterations, int32_t W,
+                                             int32_t H, int32_t storage, const LuxgiGBuffer* cur,
+                                             const LuxgiGBuffer* prev, const void* noisy, void* output0, void* output1,
+                                             void* moments0, void* moments1, int32_t pingPong, void* atrous0,
+                                             void* atrous1, uint8_t* tileMap, int32_t* result, void* stream) {
+    if (iterations < 0 || !atrous0 || !atrous1 || !moments0 || !moments1 || !output0 || !output1)
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    void* mom[2] = {moments0, moments1};
+    void* atr[2] = {atrous0, atrous1};
+    // accumulation (RaytracedReflection.cpp:362-417)
+    int rc = luxgi_svgf_reflection_reproject(rp, W, H, storage, cur, prev, noisy, output1, mom[1 - (pingPong & 1)],
+                                             output0, mom[pingPong & 1], tileMap, stream);
+    if (rc) return rc;
+    // atrousFilter (RaytracedReflection.cpp:419-494)
+    bool pp = false;
+    int writeIdx = 1;
+    for (int i = 0; i < iterations; ++i) {
+        const int readIdx = pp ? 1 : 0;
+        writeIdx = pp ? 0 : 1;
+        LuxgiReflectionAtrousParams a = *ap;
+        a.stepSize = 1 << i;
+        rc = luxgi_svgf_reflection_atrous(&a, W, H, storage, cur, tileMap, i == 0 ? output0 : atr[readIdx],
+                                          atr[writeIdx], i == 1 ? output1 : nullptr, stream);
+        if (rc) return rc;
+        pp = !pp;
+    }
+    if (result) *result = iterations > 0 ? writeIdx : -1;
+    return LUXGI_OK;
+}
+
+template <class S>
+static int shadow_reproject(const LuxgiShadowReprojectParams* p, int W, int H, const LuxgiGBuffer* cur,
+                            const LuxgiGBuffer* prev, const uint32_t* mask, const void* hc, const void* hm, void* oc,
+                            void* om, uint8_t* tiles, hipStream_t st) {
+    ShadowArgs A{*p, W, H, (W + 7) / 8, (W + 7) / 8, (H + 3) / 4};
+    hipLaunchKernelGGL(shadow_reproject_kernel<S>, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, st, A,
+                       to_gbuf(cur), to_gbuf(prev), mask, static_cast<const S*>(hc), static_cast<const S*>(hm),
+                       static_cast<S*>(oc), static_cast<S*>(om), tiles);
+    LUXGI_LAUNCH_CHECK();
+    return LUXGI_OK;
+}
+
+extern "C" int luxgi_svgf_shadow_reproject(const LuxgiShadowReprojectParams* p, int32_t W, int32_t H, int32_t storage,
+                                           const LuxgiGBuffer* cur, const LuxgiGBuffer* prev, const uint32_t* mask,
+                                           const void* historyColor, const void* historyMoments, void* outColor,
+                                           void* outMoments, uint8_t* tileMap, void* stream) {
+    if (!p || W <= 0 || H <= 0 || !gbuf_ok(cur) || !gbuf_ok(prev) || !mask || !historyColor || !historyMoments ||
+        !outColor || !outMoments || !tileMap)
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    hipStream_t st = as_stream(stream);
+    if (storage == LUXGI_STORAGE_FP32)
+        return shadow_reproject<float>(p, W, H, cur, prev, mask, historyColor, historyMoments, outColor, outMoments,
+                                       tileMap, st);
+    if (storage == LUXGI_STORAGE_FP16)
+        return shadow_reproject<__half>(p, W, H, cur, prev, mask, historyColor, historyMoments, outColor, outMoments,
+                                        tileMap, st);
+    return LUXGI_ERR_UNSUPPORTED;
+}
+
+extern "C" int luxgi_svgf_shadow_atrous(const LuxgiShadowAtrousParams* p, int32_t W, int32_t H, int32_t storage,
+                                        const LuxgiGBuffer* cur, const uint8_t* tileMap, const void* input,
+                                        void* output, void* historyOut, void* stream) {
+    if (!p || W <= 0 || H <= 0 || !cur || !cur->normals || !cur->pbr || !tileMap || !input || !output)
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    if (p->radius < 0 || p->radius > 2) return LUXGI_ERR_UNSUPPORTED;
+    AtrousArgs A{W, H, (W + 7) / 8, p->radius, p->stepSize, p->phiVisibility, p->phiNormal, p->sigmaDepth, p->power};
+    hipStream_t st = as_stream(stream);
+    const dim3 grid((W + 7) / 8, (H + 7) / 8);
+    const float4* nrm = static_cast<const float4*>(static_cast<const void*>(cur->normals));
+    const float4* pbr = static_cast<const float4*>(static_cast<const void*>(cur->pbr));
+    if (storage == LUXGI_STORAGE_FP32)
+        hipLaunchKernelGGL(shadow_atrous_kernel<float>, grid, dim3(64), 0, st, A, nrm, pbr, tileMap,
+                           static_cast<const float*>(input), static_cast<float*>(output), static_cast<float*>(historyOut));
+    else if (storage == LUXGI_STORAGE_FP16)
+        hipLaunchKernelGGL(shadow_atrous_kernel<__half>, grid, dim3(64), 0, st, A, nrm, pbr, tileMap,
+                           static_cast<const __half*>(input), static_cast<__half*>(output),
+                           static_cast<__half*>(historyOut));
+    else return LUXGI_ERR_UNSUPPORTED;
+    LUXGI_LAUNCH_CHECK();
+    return LUXGI_OK;
+}

@@ -1,0 +1,292 @@
+"""CPU restatement of LuxGI's GI hot path -- TEST INFRASTRUCTURE ONLY.
+
+This package is the checker: only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py import it. It wraps oracle/liboracle.so (built
+from oracle/luxgi_oracle.hpp + oracle/oracle_capi.cpp by oracle/Makefile):
+the *_f64 entry points are the FP64 oracle, the *_f32 entry points the FP32
+CPU transcription that bench.py times as the CPU baseline.
+
+Parity status of each oracle function (how it is pinned) is listed in the
+tests that exercise it; the GLSL sampler precision, fp16 store rounding and
+driver transcendental ULPs are not pinned by anything the reference holds.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from luxgi_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build(force=False):
+    """Compile liboracle.so with the committed Makefile."""
+    if force and os.path.exists(_LIB_PATH):
+        os.remove(_LIB_PATH)
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+P = C.c_void_p
+I = C.c_int
+
+
+def _declare(L):
+    L.oracle_border_table.argtypes = [I, P, I]
+    L.oracle_border_table.restype = I
+    L.oracle_atlas_sizes.argtypes = [I, I, I, I, P, P]
+    L.oracle_probe_id.argtypes = [I, I, I, I]
+    L.oracle_probe_id.restype = I
+    L.oracle_probe_index_to_grid.argtypes = [P, I, P]
+    L.oracle_grid_to_probe_index.argtypes = [P, I, I, I]
+    L.oracle_grid_to_probe_index.restype = I
+    L.oracle_probe_location.argtypes = [P, I, P]
+    L.oracle_oct_decode.argtypes = [C.c_double, C.c_double, P]
+    L.oracle_oct_encode.argtypes = [P, P]
+    L.oracle_normalized_oct_coord.argtypes = [I, I, I, P]
+    L.oracle_ray_directions.argtypes = [P, I, P]
+    L.oracle_spherical_fibonacci.argtypes = [I, I, P]
+    for s in ("f64", "f32"):
+        getattr(L, "oracle_probe_update_" + s).argtypes = [P, P, P, I, P, P, P, P, I, C.c_uint, I]
+        getattr(L, "oracle_border_update_" + s).argtypes = [P, I, I, I, I, I, I, I]
+        getattr(L, "oracle_trace_" + s).argtypes = [P, P, P, P, I, I, P, P, P, P, P, P, P, P, I, P, P, I, I, I,
+                                                    P, P, P, P, I]
+        getattr(L, "oracle_reflection_reproject_" + s).argtypes = [P, I, I, P, P, P, P, P, P, P, P, I]
+        getattr(L, "oracle_reflection_atrous_" + s).argtypes = [P, I, I, P, P, P, P, P, I]
+    L.oracle_sdf_mip.argtypes = [P, I, I, I, C.c_float, I, P, P]
+    L.oracle_shadow_reproject_f64.argtypes = [P, I, I, P, P, P, P, P, P, P, P, I]
+    L.oracle_shadow_atrous_f64.argtypes = [P, I, I, P, P, P, P, P, I]
+    L.oracle_hardware_threads.restype = I
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def threads():
+    return max(1, lib().oracle_hardware_threads())
+
+
+# ---------------------------------------------------------------------------
+# DDGI integer layer
+# ---------------------------------------------------------------------------
+def border_table(n):
+    out = np.zeros((4 * n + 4) * 4, dtype=np.int32)
+    cnt = lib().oracle_border_table(n, _p(out), out.size)
+    return out[: cnt * 4].reshape(cnt, 4)
+
+
+def atlas_sizes(oct_side, px, py, pz):
+    w, h = C.c_int(), C.c_int()
+    lib().oracle_atlas_sizes(oct_side, px, py, pz, C.byref(w), C.byref(h))
+    return w.value, h.value
+
+
+def probe_id(tx, ty, width, side):
+    return lib().oracle_probe_id(tx, ty, width, side)
+
+
+def probe_index_to_grid(u, index):
+    g = np.zeros(3, dtype=np.int32)
+    lib().oracle_probe_index_to_grid(C.byref(u), index, _p(g))
+    return tuple(int(v) for v in g)
+
+
+def grid_to_probe_index(u, x, y, z):
+    return lib().oracle_grid_to_probe_index(C.byref(u), x, y, z)
+
+
+def probe_location(u, index):
+    out = np.zeros(3)
+    lib().oracle_probe_location(C.byref(u), index, _p(out))
+    return out
+
+
+def oct_decode(x, y):
+    out = np.zeros(3)
+    lib().oracle_oct_decode(float(x), float(y), _p(out))
+    return out
+
+
+def oct_encode(v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.zeros(2)
+    lib().oracle_oct_encode(_p(v), _p(out))
+    return out
+
+
+def normalized_oct_coord(fx, fy, side):
+    out = np.zeros(2)
+    lib().oracle_normalized_oct_coord(fx, fy, side, _p(out))
+    return out
+
+
+def spherical_fibonacci(i, n):
+    out = np.zeros(3)
+    lib().oracle_spherical_fibonacci(i, n, _p(out))
+    return out
+
+
+def ray_directions(rot_colmajor, rays):
+    rot = np.ascontiguousarray(np.asarray(list(rot_colmajor), dtype=np.float32))
+    out = np.zeros((rays, 3))
+    lib().oracle_ray_directions(_p(rot), rays, _p(out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# probe update / border
+# ---------------------------------------------------------------------------
+def probe_update(u, radiance, dir_dist, prev_irr, prev_depth, first_frame, flags, precision="f64", nthreads=None):
+    """Returns (irr, depth) atlases of dtype float64 (f64) or float32 (f32).
+
+    radiance, dir_dist: float [rows][width][4]; prev_irr [H][W][4]; prev_depth [H][W][2].
+    Texels that the update does not write keep the values of prev_* (the caller's
+    output buffers start as copies of the previous contents in these tests)."""
+    dt = np.float64 if precision == "f64" else np.float32
+    radiance, dir_dist = _f32(radiance), _f32(dir_dist)
+    prev_irr, prev_depth = _f32(prev_irr), _f32(prev_depth)
+    out_irr = prev_irr.astype(dt).copy()
+    out_depth = prev_depth.astype(dt).copy()
+    fn = getattr(lib(), "oracle_probe_update_" + precision)
+    fn(C.byref(u), _p(radiance), _p(dir_dist), radiance.shape[1], _p(prev_irr), _p(prev_depth), _p(out_irr),
+       _p(out_depth), int(first_frame), int(flags), nthreads or threads())
+    return out_irr, out_depth
+
+
+def border_update(atlas, side, px, py, pz, precision="f64"):
+    dt = np.float64 if precision == "f64" else np.float32
+    a = np.ascontiguousarray(atlas, dtype=dt).copy()
+    H, W, ch = a.shape
+    getattr(lib(), "oracle_border_update_" + precision)(_p(a), W, H, side, ch, px, py, pz)
+    return a
+
+
+# ---------------------------------------------------------------------------
+# SDF mip and trace
+# ---------------------------------------------------------------------------
+def sdf_mip(sdf, res, cascades=1, cascade=0, max_distance=2.0, flood_passes=4, mip=None):
+    sdf = _f32(sdf)
+    mr = (res + 3) // 4
+    if mip is None:
+        mip = np.ones((mr, mr, mr * cascades), dtype=np.float32)
+    else:
+        mip = _f32(mip).copy()
+    temp = np.ones((mr, mr, mr), dtype=np.float32)
+    lib().oracle_sdf_mip(_p(sdf), res, cascades, cascade, float(max_distance), flood_passes, _p(mip), _p(temp))
+    return mip
+
+
+def trace(u, sdf_data, sdf, mip, res, mip_res, atlas_data, chunks, culled, objects, tiles, atlas_depth, atlas_color,
+          params, sky_faces=None, sky_size=0, probe_offsets=None, probe_begin=0, probe_end=None, precision="f64",
+          nthreads=None, want_counts=False):
+    """Trace probes [probe_begin, probe_end). Returns (radiance[P][R][4], dirdist[P][R][4], flags[P][R], counts)."""
+    P = u.num_probes
+    if probe_end is None:
+        probe_end = P
+    R = u.raysPerProbe
+    dt = np.float64 if precision == "f64" else np.float32
+    n = probe_end - probe_begin
+    rad = np.zeros((n, R, 4), dtype=dt)
+    dd = np.zeros((n, R, 4), dtype=dt)
+    flags = np.zeros((n, R), dtype=np.uint8)
+    counts = np.zeros(8, dtype=np.uint64) if want_counts else None
+    sdf, mip = _f32(sdf), _f32(mip)
+    atlas_depth, atlas_color = _f32(atlas_depth), _f32(atlas_color)
+    chunks = np.ascontiguousarray(chunks, dtype=np.uint32)
+    culled = np.ascontiguousarray(culled, dtype=np.uint32)
+    sky = None if sky_faces is None else _f32(sky_faces)
+    offs = None if probe_offsets is None else _f32(probe_offsets)
+    fn = getattr(lib(), "oracle_trace_" + precision)
+    fn(C.byref(u), C.byref(sdf_data), _p(sdf), _p(mip), res, mip_res, C.byref(atlas_data), _p(chunks), _p(culled),
+       C.cast(objects, C.c_void_p), C.cast(tiles, C.c_void_p), _p(atlas_depth), _p(atlas_color), _p(sky),
+       sky_size, C.byref(params), _p(offs), probe_begin, probe_end, R, _p(rad), _p(dd), _p(flags), _p(counts),
+       nthreads or threads())
+    return rad, dd, flags, counts
+
+
+# ---------------------------------------------------------------------------
+# SVGF
+# ---------------------------------------------------------------------------
+def _gb(g):
+    arrs = [_f32(g["color"]), _f32(g["normals"]), _f32(g["pbr"]), _f32(g["depth"])]
+    ptrs = (C.c_void_p * 4)(*[a.ctypes.data for a in arrs])
+    return arrs, ptrs
+
+
+def reflection_reproject(params, W, H, cur, prev, noisy, hist_out, hist_mom, precision="f64", nthreads=None):
+    dt = np.float64 if precision == "f64" else np.float32
+    ca, cp = _gb(cur)
+    pa, pp = _gb(prev)
+    noisy, hist_out, hist_mom = _f32(noisy), _f32(hist_out), _f32(hist_mom)
+    out_c = np.zeros((H, W, 4), dtype=dt)
+    out_m = np.zeros((H, W, 4), dtype=dt)
+    tiles = np.zeros(((H + 7) // 8) * ((W + 7) // 8), dtype=np.uint8)
+    getattr(lib(), "oracle_reflection_reproject_" + precision)(
+        C.byref(params), W, H, cp, pp, _p(noisy), _p(hist_out), _p(hist_mom), _p(out_c), _p(out_m), _p(tiles),
+        nthreads or threads())
+    return out_c, out_m, tiles
+
+
+def reflection_atrous(params, W, H, normals, pbr, tile_map, inp, precision="f64", nthreads=None):
+    dt = np.float64 if precision == "f64" else np.float32
+    normals, pbr, inp = _f32(normals), _f32(pbr), _f32(inp)
+    tile_map = np.ascontiguousarray(tile_map, dtype=np.uint8)
+    out = np.zeros((H, W, 4), dtype=dt)
+    getattr(lib(), "oracle_reflection_atrous_" + precision)(
+        C.byref(params), W, H, _p(normals), _p(pbr), _p(tile_map), _p(inp), _p(out), nthreads or threads())
+    return out
+
+
+def shadow_reproject(params, W, H, cur, prev, mask, hist_out, hist_mom, nthreads=None):
+    ca, cp = _gb(cur)
+    pa, pp = _gb(prev)
+    mask = np.ascontiguousarray(mask, dtype=np.uint32)
+    hist_out, hist_mom = _f32(hist_out), _f32(hist_mom)
+    out_c = np.zeros((H, W, 2))
+    out_m = np.zeros((H, W, 4))
+    tiles = np.zeros(((H + 7) // 8) * ((W + 7) // 8), dtype=np.uint8)
+    lib().oracle_shadow_reproject_f64(C.byref(params), W, H, cp, pp, _p(mask), _p(hist_out), _p(hist_mom),
+                                      _p(out_c), _p(out_m), _p(tiles), nthreads or threads())
+    return out_c, out_m, tiles
+
+
+def shadow_atrous(params, W, H, normals, pbr, tile_map, inp, nthreads=None):
+    normals, pbr, inp = _f32(normals), _f32(pbr), _f32(inp)
+    tile_map = np.ascontiguousarray(tile_map, dtype=np.uint8)
+    out = np.zeros((H, W, 2))
+    lib().oracle_shadow_atrous_f64(C.byref(params), W, H, _p(normals), _p(pbr), _p(tile_map), _p(inp), _p(out),
+                                   nthreads or threads())
+    return out
+
+
+# ---------------------------------------------------------------------------
+# relocation / classification
+# ---------------------------------------------------------------------------
+def relocate(u, dir_dist, flags, params, offsets):
+    """Returns (offsets float32 [P][4], states uint8 [P])."""
+    L = lib()
+    L.oracle_relocate.argtypes = [P, P, P, I, P, P, P]
+    dir_dist = _f32(dir_dist)
+    flags = np.ascontiguousarray(flags, dtype=np.uint8)
+    off = _f32(offsets).copy()
+    states = np.zeros(u.num_probes, dtype=np.uint8)
+    L.oracle_relocate(C.byref(u), _p(dir_dist), _p(flags), dir_dist.shape[1], C.byref(params), _p(off), _p(states))
+    return off, states

@@ -1,0 +1,303 @@
+// oracle_capi.cpp -- extern "C" entry points of the CPU restatement (TEST INFRASTRUCTURE).
+//
+// Loaded only by tests/, the smoke test and bench.py's CPU-baseline leg
+// (through oracle/__init__.py). Each *_f64 function is the FP64 oracle, each
+// *_f32 function the FP32 CPU transcription used as the CPU baseline.
+// All buffers are host memory; images are float arrays, row-major.
+#include "luxgi_oracle.hpp"
+
+using namespace oracle;
+
+namespace {
+
+template <class T>
+void probe_update_impl(const LuxgiDDGIUniform* u, const float* radiance, const float* dirDist, int rayWidth,
+                       const float* prevIrr, const float* prevDepth, T* outIrr, T* outDepth, int firstFrame,
+                       unsigned flags, int threads) {
+    const int P = u->probeCounts[0] * u->probeCounts[1] * u->probeCounts[2];
+    const bool doIrr = flags & LUXGI_UPDATE_IRRADIANCE, doDepth = flags & LUXGI_UPDATE_DEPTH;
+    parallel_for(P, threads, [&](int p) {
+        probe_update_probe<T>(*u, radiance, dirDist, rayWidth, prevIrr, prevDepth, outIrr, outDepth, firstFrame != 0,
+                              doIrr, doDepth, p);
+    });
+    if (flags & LUXGI_UPDATE_FUSE_BORDER) {
+        if (doIrr)
+            border_update<T>(outIrr, u->irradianceTextureWidth, u->irradianceTextureHeight,
+                             u->irradianceProbeSideLength, 4, u->probeCounts[0], u->probeCounts[1], u->probeCounts[2]);
+        if (doDepth)
+            border_update<T>(outDepth, u->depthTextureWidth, u->depthTextureHeight, u->depthProbeSideLength, 2,
+                             u->probeCounts[0], u->probeCounts[1], u->probeCounts[2]);
+    }
+}
+
+struct TraceArgs {
+    const LuxgiDDGIUniform* u;
+    const LuxgiGlobalSDFData* g;
+    const float* sdf;
+    const float* mip;
+    int res, mipRes;
+    const LuxgiSurfaceAtlasData* ad;
+    const uint32_t* chunks;
+    const uint32_t* culled;
+    const LuxgiAtlasObject* objects;
+    const LuxgiAtlasTile* tiles;
+    const float* atlasDepth;
+    const float* atlasColor;  // float4
+    const float* skyFaces;
+    int skySize;
+    const LuxgiTraceParams* params;
+    const float* probeOffsets;
+};
+
+template <class T>
+void trace_impl(const TraceArgs& a, int probeBegin, int probeEnd, int rayWidth, T* outRad, T* outDD, uint8_t* flags,
+                unsigned long long* counts, int threads) {
+    const int R = a.u->raysPerProbe;
+    const int nP = probeEnd - probeBegin;
+    std::vector<TraceCounts> tc(counts ? nP : 0);
+    parallel_for(nP, threads, [&](int i) {
+        int p = probeBegin + i;
+        TraceCounts* c = counts ? &tc[i] : nullptr;
+        for (int r = 0; r < R; ++r) {
+            size_t e = (size_t)i * rayWidth + r;
+            trace_ray<T>(*a.u, *a.g, a.sdf, a.mip, a.res, a.mipRes, *a.ad, a.chunks, a.culled, a.objects, a.tiles,
+                         a.atlasDepth, a.atlasColor, a.skyFaces, a.skySize, *a.params, a.probeOffsets, p, r,
+                         outRad + e * 4, outDD + e * 4, flags ? flags + e : nullptr, c);
+        }
+    });
+    if (counts) {
+        for (auto& c : tc) {
+            counts[0] += c.mip; counts[1] += c.fine; counts[2] += c.normals; counts[3] += c.objects;
+            counts[4] += c.tiles; counts[5] += c.rays; counts[6] += c.steps; counts[7] += c.hits;
+        }
+    }
+}
+
+GB make_gb(const float* const* g) { return GB{{g[0], 4}, {g[1], 4}, {g[2], 4}, {g[3], 1}}; }
+
+template <class T>
+void refl_reproject_impl(const LuxgiReflectionReprojectParams* pc, int W, int H, const float* const* cur,
+                         const float* const* prev, const float* noisy, const float* histOut, const float* histMom,
+                         T* outColor, T* outMom, uint8_t* tileMap, int threads) {
+    GB c = make_gb(cur), p = make_gb(prev);
+    const int tw = (W + 7) / 8, th = (H + 7) / 8;
+    std::vector<uint8_t> vote((size_t)W * H);
+    parallel_for(H, threads, [&](int y) {
+        for (int x = 0; x < W; ++x) {
+            size_t e = (size_t)y * W + x;
+            vote[e] = reflection_reproject_pixel<T>(*pc, W, H, x, y, c, p, {noisy, 4}, {histOut, 4}, {histMom, 4},
+                                                     outColor + e * 4, outMom + e * 4);
+        }
+    });
+    for (int t = 0; t < tw * th; ++t) tileMap[t] = 0;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+            if (vote[(size_t)y * W + x]) tileMap[(y / 8) * tw + x / 8] = 1;
+}
+
+template <class T>
+void refl_atrous_impl(const LuxgiReflectionAtrousParams* pc, int W, int H, const float* normals, const float* pbr,
+                      const uint8_t* tileMap, const float* in, T* out, int threads) {
+    const int tw = (W + 7) / 8;
+    parallel_for(H, threads, [&](int y) {
+        for (int x = 0; x < W; ++x) {
+            size_t e = (size_t)y * W + x;
+            if (tileMap[(y / 8) * tw + x / 8]) {
+                reflection_atrous_pixel<T>(*pc, W, H, x, y, {normals, 4}, {pbr, 4}, {in, 4}, out + e * 4);
+            } else {
+                for (int c = 0; c < 4; ++c) out[e * 4 + c] = T(in[e * 4 + c]);
+            }
+        }
+    });
+}
+
+template <class T>
+void shadow_reproject_impl(const LuxgiShadowReprojectParams* pc, int W, int H, const float* const* cur,
+                           const float* const* prev, const uint32_t* mask, const float* histOut,
+                           const float* histMom, T* outColor, T* outMom, uint8_t* tileMap, int threads) {
+    GB c = make_gb(cur), p = make_gb(prev);
+    const int tw = (W + 7) / 8, th = (H + 7) / 8;
+    std::vector<uint8_t> vote((size_t)W * H);
+    parallel_for(H, threads, [&](int y) {
+        for (int x = 0; x < W; ++x) {
+            size_t e = (size_t)y * W + x;
+            vote[e] = shadow_reproject_pixel<T>(*pc, W, H, x, y, c, p, mask, {histOut, 2}, {histMom, 4},
+                                                 outColor + e * 2, outMom + e * 4);
+        }
+    });
+    for (int t = 0; t < tw * th; ++t) tileMap[t] = 0;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+            if (vote[(size_t)y * W + x]) tileMap[(y / 8) * tw + x / 8] = 1;
+}
+
+template <class T>
+void shadow_atrous_impl(const LuxgiShadowAtrousParams* pc, int W, int H, const float* normals, const float* pbr,
+                        const uint8_t* tileMap, const float* in, T* out, int threads) {
+    const int tw = (W + 7) / 8;
+    parallel_for(H, threads, [&](int y) {
+        for (int x = 0; x < W; ++x) {
+            size_t e = (size_t)y * W + x;
+            if (tileMap[(y / 8) * tw + x / 8]) {
+                shadow_atrous_pixel<T>(*pc, W, H, x, y, {normals, 4}, {pbr, 4}, {in, 2}, out + e * 2);
+            } else {
+                out[e * 2 + 0] = T(0);
+                out[e * 2 + 1] = T(0);
+            }
+        }
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_border_table(int n, int* out, int cap) {
+    std::vector<int> t = border_table(n);
+    int cnt = (int)t.size() / 4;
+    if (cap < cnt) return -cnt;
+    std::memcpy(out, t.data(), t.size() * sizeof(int));
+    return cnt;
+}
+
+void oracle_atlas_sizes(int oct, int px, int py, int pz, int* w, int* h) { atlas_sizes(oct, px, py, pz, w, h); }
+
+int oracle_probe_id(int tx, int ty, int width, int side) { return get_probe_id(tx, ty, width, side); }
+
+void oracle_probe_index_to_grid(const LuxgiDDGIUniform* u, int index, int* g) {
+    probe_index_to_grid(*u, index, g, g + 1, g + 2);
+}
+
+int oracle_grid_to_probe_index(const LuxgiDDGIUniform* u, int x, int y, int z) { return grid_to_probe_index(*u, x, y, z); }
+
+void oracle_probe_location(const LuxgiDDGIUniform* u, int index, double* out) {
+    v3<double> p = probe_location<double>(*u, index);
+    out[0] = p.x; out[1] = p.y; out[2] = p.z;
+}
+
+void oracle_oct_decode(double x, double y, double* out) {
+    v3<double> v = oct_decode<double>({x, y});
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+}
+
+void oracle_oct_encode(const double* v, double* out) {
+    v2<double> o = oct_encode<double>({v[0], v[1], v[2]});
+    out[0] = o.x; out[1] = o.y;
+}
+
+void oracle_normalized_oct_coord(int fx, int fy, int side, double* out) {
+    v2<double> o = normalized_oct_coord<double>(fx, fy, side);
+    out[0] = o.x; out[1] = o.y;
+}
+
+void oracle_ray_directions(const float* rot, int rays, double* out) {
+    for (int r = 0; r < rays; ++r) {
+        v3<double> d = ray_direction<double>(rot, r, rays);
+        out[r * 3 + 0] = d.x; out[r * 3 + 1] = d.y; out[r * 3 + 2] = d.z;
+    }
+}
+
+void oracle_spherical_fibonacci(int i, int n, double* out) {
+    v3<double> d = spherical_fibonacci<double>(double(i), double(n));
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
+void oracle_probe_update_f64(const LuxgiDDGIUniform* u, const float* radiance, const float* dirDist, int rayWidth,
+                             const float* prevIrr, const float* prevDepth, double* outIrr, double* outDepth,
+                             int firstFrame, unsigned flags, int threads) {
+    probe_update_impl<double>(u, radiance, dirDist, rayWidth, prevIrr, prevDepth, outIrr, outDepth, firstFrame, flags,
+                              threads);
+}
+
+void oracle_probe_update_f32(const LuxgiDDGIUniform* u, const float* radiance, const float* dirDist, int rayWidth,
+                             const float* prevIrr, const float* prevDepth, float* outIrr, float* outDepth,
+                             int firstFrame, unsigned flags, int threads) {
+    probe_update_impl<float>(u, radiance, dirDist, rayWidth, prevIrr, prevDepth, outIrr, outDepth, firstFrame, flags,
+                             threads);
+}
+
+void oracle_border_update_f64(double* atlas, int W, int H, int side, int ch, int px, int py, int pz) {
+    border_update<double>(atlas, W, H, side, ch, px, py, pz);
+}
+void oracle_border_update_f32(float* atlas, int W, int H, int side, int ch, int px, int py, int pz) {
+    border_update<float>(atlas, W, H, side, ch, px, py, pz);
+}
+
+#define TRACE_ARGS                                                                                                    \
+    const LuxgiDDGIUniform *u, const LuxgiGlobalSDFData *g, const float *sdf, const float *mip, int res, int mipRes, \
+        const LuxgiSurfaceAtlasData *ad, const uint32_t *chunks, const uint32_t *culled,                            \
+        const LuxgiAtlasObject *objects, const LuxgiAtlasTile *tiles, const float *atlasDepth,                      \
+        const float *atlasColor, const float *skyFaces, int skySize, const LuxgiTraceParams *params,                \
+        const float *probeOffsets, int probeBegin, int probeEnd, int rayWidth
+#define TRACE_PACK                                                                                                    \
+    TraceArgs a{u, g, sdf, mip, res, mipRes, ad, chunks, culled, objects, tiles, atlasDepth, atlasColor, skyFaces,  \
+                skySize, params, probeOffsets}
+
+void oracle_trace_f64(TRACE_ARGS, double* outRad, double* outDD, uint8_t* flags, unsigned long long* counts,
+                      int threads) {
+    TRACE_PACK;
+    trace_impl<double>(a, probeBegin, probeEnd, rayWidth, outRad, outDD, flags, counts, threads);
+}
+
+void oracle_trace_f32(TRACE_ARGS, float* outRad, float* outDD, uint8_t* flags, unsigned long long* counts,
+                      int threads) {
+    TRACE_PACK;
+    trace_impl<float>(a, probeBegin, probeEnd, rayWidth, outRad, outDD, flags, counts, threads);
+}
+
+// Coarse mip of one cascade: initial 7-tap pass then `floodPasses` ping-pong passes
+// (GlobalDistanceField.cpp:823-846 and :537-573). Arithmetic in double, storage float.
+void oracle_sdf_mip(const float* sdf, int res, int cascades, int cascade, float maxDistance, int floodPasses,
+                    float* mip, float* temp) {
+    const int mr = (res + 3) / 4;
+    mip_pass<double>(sdf, res * cascades, res, res, res, 4, cascade * res, mip, mr * cascades, mr, mr, cascade * mr,
+                     maxDistance);
+    for (int i = 1; i <= floodPasses; ++i) {
+        if (i & 1)
+            mip_pass<double>(mip, mr * cascades, mr, mr, mr, 1, cascade * mr, temp, mr, mr, mr, 0, maxDistance);
+        else
+            mip_pass<double>(temp, mr, mr, mr, mr, 1, 0, mip, mr * cascades, mr, mr, cascade * mr, maxDistance);
+    }
+}
+
+// cur/prev: pointers {color, normals, pbr, depth}
+void oracle_reflection_reproject_f64(const LuxgiReflectionReprojectParams* pc, int W, int H, const float* const* cur,
+                                     const float* const* prev, const float* noisy, const float* histOut,
+                                     const float* histMom, double* outColor, double* outMom, uint8_t* tileMap,
+                                     int threads) {
+    refl_reproject_impl<double>(pc, W, H, cur, prev, noisy, histOut, histMom, outColor, outMom, tileMap, threads);
+}
+void oracle_reflection_reproject_f32(const LuxgiReflectionReprojectParams* pc, int W, int H, const float* const* cur,
+                                     const float* const* prev, const float* noisy, const float* histOut,
+                                     const float* histMom, float* outColor, float* outMom, uint8_t* tileMap,
+                                     int threads) {
+    refl_reproject_impl<float>(pc, W, H, cur, prev, noisy, histOut, histMom, outColor, outMom, tileMap, threads);
+}
+void oracle_reflection_atrous_f64(const LuxgiReflectionAtrousParams* pc, int W, int H, const float* normals,
+                                  const float* pbr, const uint8_t* tileMap, const float* in, double* out, int threads) {
+    refl_atrous_impl<double>(pc, W, H, normals, pbr, tileMap, in, out, threads);
+}
+void oracle_reflection_atrous_f32(const LuxgiReflectionAtrousParams* pc, int W, int H, const float* normals,
+                                  const float* pbr, const uint8_t* tileMap, const float* in, float* out, int threads) {
+    refl_atrous_impl<float>(pc, W, H, normals, pbr, tileMap, in, out, threads);
+}
+void oracle_shadow_reproject_f64(const LuxgiShadowReprojectParams* pc, int W, int H, const float* const* cur,
+                                 const float* const* prev, const uint32_t* mask, const float* histOut,
+                                 const float* histMom, double* outColor, double* outMom, uint8_t* tileMap,
+                                 int threads) {
+    shadow_reproject_impl<double>(pc, W, H, cur, prev, mask, histOut, histMom, outColor, outMom, tileMap, threads);
+}
+void oracle_shadow_atrous_f64(const LuxgiShadowAtrousParams* pc, int W, int H, const float* normals,
+                              const float* pbr, const uint8_t* tileMap, const float* in, double* out, int threads) {
+    shadow_atrous_impl<double>(pc, W, H, normals, pbr, tileMap, in, out, threads);
+}
+
+int oracle_hardware_threads(void) { return (int)std::thread::hardware_concurrency(); }
+
+}  // extern "C"
+
+extern "C" void oracle_relocate(const LuxgiDDGIUniform* u, const float* dirDist, const uint8_t* flags, int width,
+                                const LuxgiRelocateParams* prm, float* offsets, uint8_t* states) {
+    const int P = u->probeCounts[0] * u->probeCounts[1] * u->probeCounts[2];
+    for (int p = 0; p < P; ++p) relocate_probe(*u, dirDist, flags, width, *prm, offsets, states, p);
+}

@@ -1,0 +1,60 @@
+"""CPU checks of the C ABI: libluxgi.so loads and exports every function that
+include/luxgi.h and include/luxgi_scene.h declare; struct sizes match the
+reference layouts. No compute calls (no GPU here)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from luxgi_amd import _lib, abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in ("luxgi.h", "luxgi_scene.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"\b(luxgi_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declarations_have_python_signatures():
+    names = declared_functions()
+    assert len(names) >= 18
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="libluxgi.so not built")
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(_lib.LIB_PATH)
+    for n in declared_functions():
+        assert hasattr(L, n), n
+    assert b"gfx950" in _lib.lib().luxgi_version()
+
+
+def test_struct_sizes():
+    assert C.sizeof(abi.DDGIUniform) == 96
+    assert C.sizeof(abi.GlobalSDFData) == 96
+    assert C.sizeof(abi.AtlasObject) == 128
+    assert C.sizeof(abi.AtlasTile) == 96
+    assert C.sizeof(abi.SurfaceAtlasData) == 32
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="libluxgi.so not built")
+def test_host_only_entry_points():
+    u = abi.make_uniform((32, 16, 32), (-1, -1, -1), (0.1, 0.1, 0.1), 256)
+    u.irradianceTextureWidth = 0
+    assert _lib.lib().luxgi_ddgi_fill_atlas_sizes(C.byref(u)) == 0
+    assert (u.irradianceTextureWidth, u.irradianceTextureHeight, u.depthTextureWidth, u.depthTextureHeight) == \
+        (5122, 322, 9218, 578)
+    assert _lib.lib().luxgi_svgf_tile_count(1920, 1080) == 240 * 135
+    assert _lib.lib().luxgi_ddgi_probe_update_workspace_size(C.byref(u)) >= 256 * (64 + 256) * 4
+    # invalid arguments are rejected before any device work
+    assert _lib.lib().luxgi_ddgi_probe_update(None, None, None, None, 0, 3, None, 0, None) < 0
+    bad = abi.make_uniform((2, 2, 2), (0, 0, 0), (1, 1, 1), 64)
+    bad.depthTextureWidth += 1
+    rays = abi.RayBuffers(None, None, 64, 8, 0, 0)
+    assert _lib.lib().luxgi_ddgi_probe_update(C.byref(bad), C.byref(rays), None, None, 1, 3, None, 0, None) == -1

@@ -1,0 +1,134 @@
+"""GPU parity: probe blend + border (luxgi_ddgi_probe_update / luxgi_ddgi_border_update)
+against the FP64 oracle on the same seeded ray buffers.
+
+Tolerances: FP32 storage 2e-5 relative (+1e-6 absolute) on irradiance and depth;
+FP16 storage: the value rounded to fp16 (2^-11 relative) plus 2e-5. Border texels
+and untouched padding are compared bit for bit with their sources."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from luxgi_amd import abi, scene
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _rays(u, rot_seed=3, seed=0, perturb=False):
+    P, R = u.num_probes, u.raysPerProbe
+    dirs = oracle.ray_directions(abi.mat4_colmajor(scene.random_rotation(rot_seed)), R).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    rad = np.zeros((P, R, 4), np.float32)
+    dd = np.zeros((P, R, 4), np.float32)
+    dd[:, :, :3] = dirs[None]
+    if perturb:  # some probes get their own directions: the per-probe weight path must run for them
+        for p in range(0, P, 3):
+            d = dirs + rng.normal(scale=0.05, size=dirs.shape).astype(np.float32)
+            dd[p, :, :3] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rad[:, :, :3] = rng.uniform(0, 2, (P, R, 3))
+    dd[:, :, 3] = rng.uniform(0.05, 3.0, (P, R))
+    dd[:, ::17, 3] = 60000.0  # misses / inside hits
+    return rad, dd
+
+
+def _run_gpu(u, rad, dd, prev_i, prev_d, first, flags, storage=abi.STORAGE_FP32):
+    from luxgi_amd import _lib
+    from luxgi_amd.ddgi import ptr, stream_handle
+    dt = torch.float16 if storage == abi.STORAGE_FP16 else torch.float32
+    t_rad = torch.from_numpy(rad).to(dt).cuda()
+    t_dd = torch.from_numpy(dd).to(dt).cuda()
+    rays = abi.RayBuffers(ptr(t_rad), ptr(t_dd), rad.shape[1], rad.shape[0], storage, 0)
+    p_i = torch.from_numpy(prev_i).to(dt).cuda()
+    p_d = torch.from_numpy(prev_d).to(dt).cuda()
+    o_i = p_i.clone()
+    o_d = p_d.clone()
+    prev = abi.ProbeAtlas(ptr(p_i), ptr(p_d), storage, 0)
+    out = abi.ProbeAtlas(ptr(o_i), ptr(o_d), storage, 0)
+    ws = torch.empty(_lib.lib().luxgi_ddgi_probe_update_workspace_size(C.byref(u)), dtype=torch.uint8, device="cuda")
+    rc = _lib.lib().luxgi_ddgi_probe_update(C.byref(u), C.byref(rays), C.byref(prev), C.byref(out), int(first), flags,
+                                            ptr(ws), ws.numel(), stream_handle())
+    assert rc == 0
+    torch.cuda.synchronize()
+    return o_i.float().cpu().numpy(), o_d.float().cpu().numpy()
+
+
+CASES = [
+    # (probe counts, rays, irr oct, depth oct)
+    ((8, 8, 8), 64, 6, 14),     # config-1 shapes
+    ((4, 3, 5), 256, 8, 16),    # reference oct sizes, 256 rays
+    ((3, 2, 2), 300, 8, 16),    # ragged: rays not a multiple of the 256-ray chunk or of 64
+    ((1, 1, 1), 1, 4, 4),       # single probe, single ray
+]
+
+
+@pytest.mark.parametrize("counts,rays,ni,nd", CASES)
+@pytest.mark.parametrize("per_probe", [False, True])
+def test_probe_update_matches_oracle_fp32(counts, rays, ni, nd, per_probe):
+    u = abi.make_uniform(counts, (-1, -1, -1), (0.3, 0.4, 0.5), rays, irr_oct=ni, depth_oct=nd, hysteresis=0.97,
+                         max_distance=1.5)
+    rad, dd = _rays(u, seed=rays)
+    rng = np.random.default_rng(1)
+    prev_i = rng.uniform(0, 1, (u.irradianceTextureHeight, u.irradianceTextureWidth, 4)).astype(np.float32)
+    prev_d = rng.uniform(0, 1, (u.depthTextureHeight, u.depthTextureWidth, 2)).astype(np.float32)
+    flags = abi.UPDATE_IRRADIANCE | abi.UPDATE_DEPTH | abi.UPDATE_FUSE_BORDER
+    if per_probe:
+        flags |= abi.UPDATE_PER_PROBE
+    for first in (1, 0):
+        gi, gd = _run_gpu(u, rad, dd, prev_i, prev_d, first, flags)
+        oi, od = oracle.probe_update(u, rad, dd, prev_i, prev_d, first, flags)
+        np.testing.assert_allclose(gi, oi, rtol=2e-5, atol=1e-6)
+        np.testing.assert_allclose(gd, od, rtol=2e-5, atol=1e-6)
+        # global padding untouched, bit for bit
+        assert np.array_equal(gi[0], prev_i[0]) and np.array_equal(gi[:, 0], prev_i[:, 0])
+        assert np.array_equal(gd[-1], prev_d[-1]) and np.array_equal(gd[:, -1], prev_d[:, -1])
+        # every border texel equals its interior source bit for bit
+        for atlas, side in ((gi, ni), (gd, nd)):
+            ref = oracle.border_update(atlas.astype(np.float64), side, *counts).astype(np.float32)
+            assert np.array_equal(ref, atlas)
+
+
+def test_probe_update_unshared_directions():
+    u = abi.make_uniform((4, 2, 3), (-1, -1, -1), (0.3, 0.4, 0.5), 128, irr_oct=8, depth_oct=16, max_distance=1.5)
+    rad, dd = _rays(u, seed=4, perturb=True)
+    z_i = np.zeros((u.irradianceTextureHeight, u.irradianceTextureWidth, 4), np.float32)
+    z_d = np.zeros((u.depthTextureHeight, u.depthTextureWidth, 2), np.float32)
+    flags = abi.UPDATE_IRRADIANCE | abi.UPDATE_DEPTH | abi.UPDATE_FUSE_BORDER
+    gi, gd = _run_gpu(u, rad, dd, z_i, z_d, 1, flags)
+    oi, od = oracle.probe_update(u, rad, dd, z_i, z_d, 1, flags)
+    np.testing.assert_allclose(gi, oi, rtol=2e-5, atol=1e-6)
+    np.testing.assert_allclose(gd, od, rtol=2e-5, atol=1e-6)
+
+
+def test_probe_update_fp16_storage():
+    u = abi.make_uniform((4, 4, 4), (-1, -1, -1), (0.3, 0.4, 0.5), 256, irr_oct=8, depth_oct=16, max_distance=1.5)
+    rad, dd = _rays(u, seed=8)
+    rad16 = rad.astype(np.float16).astype(np.float32)
+    dd16 = dd.astype(np.float16).astype(np.float32)
+    z_i = np.zeros((u.irradianceTextureHeight, u.irradianceTextureWidth, 4), np.float32)
+    z_d = np.zeros((u.depthTextureHeight, u.depthTextureWidth, 2), np.float32)
+    flags = abi.UPDATE_IRRADIANCE | abi.UPDATE_DEPTH | abi.UPDATE_FUSE_BORDER
+    gi, gd = _run_gpu(u, rad16, dd16, z_i, z_d, 1, flags, storage=abi.STORAGE_FP16)
+    oi, od = oracle.probe_update(u, rad16, dd16, z_i, z_d, 1, flags)
+    np.testing.assert_allclose(gi, oi, rtol=2 ** -11 + 2e-5, atol=1e-6)
+    np.testing.assert_allclose(gd, od, rtol=2 ** -11 + 2e-5, atol=1e-6)
+
+
+def test_border_update_standalone_equals_fused():
+    from luxgi_amd import _lib
+    from luxgi_amd.ddgi import ptr, stream_handle
+    u = abi.make_uniform((5, 2, 3), (-1, -1, -1), (0.3, 0.4, 0.5), 64, irr_oct=8, depth_oct=16)
+    rad, dd = _rays(u, seed=2)
+    z_i = np.zeros((u.irradianceTextureHeight, u.irradianceTextureWidth, 4), np.float32)
+    z_d = np.zeros((u.depthTextureHeight, u.depthTextureWidth, 2), np.float32)
+    fused_i, fused_d = _run_gpu(u, rad, dd, z_i, z_d, 1, 7)
+    plain_i, plain_d = _run_gpu(u, rad, dd, z_i, z_d, 1, 3)
+    t_i = torch.from_numpy(plain_i).cuda()
+    t_d = torch.from_numpy(plain_d).cuda()
+    atlas = abi.ProbeAtlas(ptr(t_i), ptr(t_d), abi.STORAGE_FP32, 0)
+    assert _lib.lib().luxgi_ddgi_border_update(C.byref(u), C.byref(atlas), 3, stream_handle()) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(t_i.cpu().numpy(), fused_i)
+    assert np.array_equal(t_d.cpu().numpy(), fused_d)

@@ -140,8 +140,15 @@ typedef struct LuxgiSDFVolume {
     int32_t     resolution;
     int32_t     mipResolution;
     int32_t     storage;  /* LuxgiStorage: FP32 or FP16 */
-    int32_t     pad;
+    int32_t     layout;   /* LuxgiLayout of `sdf` (the mip is always linear) */
 } LuxgiSDFVolume;
+
+/* Memory layout of the fine SDF volume. */
+typedef enum LuxgiLayout {
+    LUXGI_LAYOUT_LINEAR = 0, /* [z][y][x], the reference texture's texel order */
+    LUXGI_LAYOUT_BRICK4 = 1  /* 4x4x4 bricks (x-fastest brick order), voxels z-major inside a brick;
+                                FP32 storage, every dimension a multiple of 4 (luxgi_sdf_to_bricks) */
+} LuxgiLayout;
 
 /* Surface atlas + object/tile/chunk buffers read by the hit lookup
  * (Shaders/SDF/AtlasCommon.glsl:115-157). */
@@ -200,8 +207,10 @@ typedef struct LuxgiProbeAtlas {
  * Fills the *TextureWidth/Height fields of `uniform`. Host-only. */
 int luxgi_ddgi_fill_atlas_sizes(LuxgiDDGIUniform* uniform);
 
-/* Workspace bytes needed by luxgi_ddgi_trace_sdf for `objectCapacity` atlas objects. */
-size_t luxgi_ddgi_trace_workspace_size(int32_t objectCapacity);
+/* Workspace bytes needed by luxgi_ddgi_trace_sdf for the probe grid of `uniform`
+ * and `objectCapacity` atlas objects (queue counter, per-object inverses, per-ray
+ * hit records). */
+size_t luxgi_ddgi_trace_workspace_size(const LuxgiDDGIUniform* uniform, int32_t objectCapacity);
 
 /* GISDFRays.comp (Shaders/DDGI/GISDFRays.comp:63-128): probe ray generation,
  * global-SDF sphere trace (SDFCommon.glsl:98-194), hit normal, surface-atlas
@@ -209,7 +218,8 @@ size_t luxgi_ddgi_trace_workspace_size(int32_t objectCapacity);
  * Replaces the dispatch at DDGIRenderer.cpp:318-321.
  *   probeOffsets  optional float4[P] relocation offsets added to probe positions (NULL = none)
  *   rayFlags      optional uint8[P*width] per-ray class: 0 miss, 1 front hit, 2 inside/backface (NULL = none)
- *   workspace     device scratch of luxgi_ddgi_trace_workspace_size() bytes (per-object inverses) */
+ *   workspace     device scratch of luxgi_ddgi_trace_workspace_size() bytes; calls that overlap on
+ *                 one workspace race */
 int luxgi_ddgi_trace_sdf(const LuxgiDDGIUniform* uniform,
                          const LuxgiGlobalSDFData* sdfData,
                          const LuxgiSDFVolume* volume,
@@ -223,7 +233,7 @@ int luxgi_ddgi_trace_sdf(const LuxgiDDGIUniform* uniform,
                          void* stream);
 
 /* Counting run of the same trace: accumulates into counters[8] (uint64, device):
- *   [0] mip lookups, [1] fine lookups, [2] hits needing a normal (6 lookups each),
+ *   [0] mip lookups, [1] fine lookups, [2] front hits that evaluate a normal (6 lookups each),
  *   [3] atlas objects tested, [4] tile samples (4-texel gathers), [5] rays,
  *   [6] total march steps, [7] rays that hit.
  * The caller zeroes counters first. Outputs are written exactly as by luxgi_ddgi_trace_sdf. */
@@ -295,6 +305,11 @@ int luxgi_ddgi_relocate_classify(const LuxgiDDGIUniform* uniform,
 int luxgi_sdf_build_mip(const float* sdf, int32_t resolution, int32_t cascades, int32_t cascade,
                         float maxDistance, int32_t floodPasses,
                         float* mip, float* temp, void* stream);
+
+/* Re-lays a linear FP32 volume [depth][height][width] as LUXGI_LAYOUT_BRICK4
+ * (no counterpart in the reference, whose Vulkan 3D image is tiled by the driver).
+ * width, height, depth multiples of 4; dst holds width*height*depth floats. */
+int luxgi_sdf_to_bricks(const float* src, int32_t width, int32_t height, int32_t depth, float* dst, void* stream);
 
 /* ------------------------------------------------------------------------- */
 /* SVGF (reflection = RGB, shadow = scalar)                                   */

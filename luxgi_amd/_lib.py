@@ -18,7 +18,7 @@ U = C.c_uint32
 SIGNATURES = {
     "luxgi_version": (C.c_char_p, []),
     "luxgi_ddgi_fill_atlas_sizes": (C.c_int, [P]),
-    "luxgi_ddgi_trace_workspace_size": (C.c_size_t, [I]),
+    "luxgi_ddgi_trace_workspace_size": (C.c_size_t, [P, I]),
     "luxgi_ddgi_trace_sdf": (C.c_int, [P, P, P, P, P, P, P, P, P, P, C.c_size_t, P]),
     "luxgi_ddgi_trace_sdf_count": (C.c_int, [P, P, P, P, P, P, P, P, P, P, C.c_size_t, P, P]),
     "luxgi_ddgi_probe_update_workspace_size": (C.c_size_t, [P]),
@@ -26,6 +26,7 @@ SIGNATURES = {
     "luxgi_ddgi_border_update": (C.c_int, [P, P, U, P]),
     "luxgi_ddgi_relocate_classify": (C.c_int, [P, P, P, P, P, P, P]),
     "luxgi_sdf_build_mip": (C.c_int, [P, I, I, I, C.c_float, I, P, P, P]),
+    "luxgi_sdf_to_bricks": (C.c_int, [P, I, I, I, P, P]),
     "luxgi_svgf_tile_count": (I, [I, I]),
     "luxgi_svgf_reflection_reproject": (C.c_int, [P, I, I, I, P, P, P, P, P, P, P, P, P]),
     "luxgi_svgf_reflection_atrous": (C.c_int, [P, I, I, I, P, P, P, P, P, P]),

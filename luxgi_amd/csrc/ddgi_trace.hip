@@ -10,8 +10,12 @@
 // One lane per ray; the 64 lanes of a wave trace 64 rays of the same probe, so
 // early march steps of a wave read neighbouring voxels. The reference inverts
 // each candidate object's transform per hit (AtlasCommon.glsl:134); here the
-// inverses are computed once per call into the workspace (object_inverse_kernel).
+// inverses are computed once per call into the workspace (trace_prepare_kernel).
+// The march (march_kernel) and the per-hit work (shade_kernel) are separate
+// launches, so the march loop carries only the registers it needs.
 #include "common.hpp"
+
+#include <algorithm>
 
 namespace luxgi {
 
@@ -26,10 +30,12 @@ struct TraceArgs {
     float sky[3];
     int res, mipRes, maxSteps, skySize, P, R, rayWidth;
     float hitEps;
+    int fineLayout;  // LuxgiLayout of the fine volume
 };
 
 // GLSL inverse(mat4) by cofactors (column-major), evaluated once per object.
-__global__ void object_inverse_kernel(const LuxgiAtlasObject* __restrict__ objects, int n, float* __restrict__ inv) {
+__device__ __forceinline__ void object_inverse_kernel_body(const LuxgiAtlasObject* __restrict__ objects, int n,
+                                                           float* __restrict__ inv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* m = objects[i].transform;
@@ -195,59 +201,77 @@ __device__ float3 sample_cube(const float* __restrict__ faces, int size, f3 d) {
                        (t00.z * (1.f - a) + t10.z * a) * (1.f - b) + (t01.z * (1.f - a) + t11.z * a) * b);
 }
 
-template <class SS, class SC, class SR, bool COUNT>
-__global__ __launch_bounds__(256) void trace_kernel(TraceArgs A, const SS* __restrict__ sdf, const SS* __restrict__ mip,
-                                                    const uint32_t* __restrict__ chunks,
-                                                    const uint32_t* __restrict__ culled,
-                                                    const LuxgiAtlasObject* __restrict__ objects,
-                                                    const float* __restrict__ objInv,
-                                                    const LuxgiAtlasTile* __restrict__ tiles,
-                                                    const SC* __restrict__ color, const float* __restrict__ atlasDepth,
-                                                    const float* __restrict__ skyFaces,
-                                                    const float* __restrict__ probeOffsets, SR* __restrict__ outRad,
-                                                    SR* __restrict__ outDD, uint8_t* __restrict__ flags,
-                                                    unsigned long long* __restrict__ counters) {
-    const int Rp = (A.R + 63) & ~63;
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int probe = int(gid / Rp), ray = int(gid % Rp);
-    const bool active = probe < A.P && ray < A.R;
-    Counts c = {0, 0, 0, 0, 0, 0, 0};
-    if (active) {
-        // probeLocation (DDGICommon.glsl:106-114)
-        const int px = A.u.probeCounts[0], pxy = A.u.probeCounts[0] * A.u.probeCounts[1];
-        const int gx = probe % px, gy = (probe % pxy) / px, gz = probe / pxy;
-        f3 origin = mk(A.u.step[0] * float(gx) + A.u.startPosition[0], A.u.step[1] * float(gy) + A.u.startPosition[1],
-                       A.u.step[2] * float(gz) + A.u.startPosition[2]);
-        if (probeOffsets) {
-            const float4 o = reinterpret_cast<const float4*>(probeOffsets)[probe];
-            origin = origin + mk(o.x, o.y, o.z);
-        }
-        // GISDFRays.comp:73
-        const f3 sf = spherical_fibonacci(float(ray), float(A.R));
-        const f3 dir = normalize3(mk(A.rot[0] * sf.x + A.rot[3] * sf.y + A.rot[6] * sf.z,
-                                     A.rot[1] * sf.x + A.rot[4] * sf.y + A.rot[7] * sf.z,
-                                     A.rot[2] * sf.x + A.rot[5] * sf.y + A.rot[8] * sf.z));
+// ---------------------------------------------------------------------------
+// March (tracyGlobalSDF, SDFCommon.glsl:98-194): one lane per ray, the 64 lanes
+// of a wave on 64 rays of one probe. Blocks are mapped to probes XCD by XCD:
+// the dispatcher deals blocks to the 8 XCDs round-robin, so block b runs on XCD
+// b % 8; the mapping gives each XCD a contiguous range of rays (a slab of
+// probes) that it walks in order, which keeps the SDF lines its rays touch in
+// its own L2. The hit record (hitTime, texture UV of the hit step, hit SDF,
+// cascade) goes to the workspace and shade_kernel turns it into the outputs.
+// ---------------------------------------------------------------------------
+constexpr int kXcds = 8;
 
-        // tracyGlobalSDF (SDFCommon.glsl:98-194)
+__device__ __forceinline__ long long xcd_block(long long b, long long nb) {
+    const long long full = (nb / kXcds) * kXcds;
+    if (b >= full) return b;
+    return (b % kXcds) * (nb / kXcds) + b / kXcds;
+}
+
+__device__ __forceinline__ void ray_setup(const TraceArgs& A, const float* __restrict__ probeOffsets, int probe,
+                                          int ray, f3& origin, f3& dir) {
+    // probeLocation (DDGICommon.glsl:106-114) + GISDFRays.comp:72-73
+    const int px = A.u.probeCounts[0], pxy = A.u.probeCounts[0] * A.u.probeCounts[1];
+    const int gx = probe % px, gy = (probe % pxy) / px, gz = probe / pxy;
+    origin = mk(A.u.step[0] * float(gx) + A.u.startPosition[0], A.u.step[1] * float(gy) + A.u.startPosition[1],
+                A.u.step[2] * float(gz) + A.u.startPosition[2]);
+    if (probeOffsets) {
+        const float4 o = reinterpret_cast<const float4*>(probeOffsets)[probe];
+        origin = origin + mk(o.x, o.y, o.z);
+    }
+    const f3 sf = spherical_fibonacci(float(ray), float(A.R));
+    dir = normalize3(mk(A.rot[0] * sf.x + A.rot[3] * sf.y + A.rot[6] * sf.z,
+                        A.rot[1] * sf.x + A.rot[4] * sf.y + A.rot[7] * sf.z,
+                        A.rot[2] * sf.x + A.rot[5] * sf.y + A.rot[8] * sf.z));
+}
+
+// fine-volume lookup in the volume's layout
+template <class SS, bool BRICK>
+__device__ __forceinline__ float fine_lookup(const SS* __restrict__ sdf, int W, int H, int D, float u, float v, float w) {
+    if constexpr (BRICK) return sample3d_brick4<SS>(sdf, W, H, D, u, v, w);
+    else return sample_volume<SS>(sdf, W, H, D, u, v, w);
+}
+
+template <class SS, bool COUNT, bool BRICK>
+__global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __restrict__ sdf, const SS* __restrict__ mip,
+                                                    const float* __restrict__ probeOffsets, float4* __restrict__ recA,
+                                                    float2* __restrict__ recB, unsigned long long* __restrict__ counters) {
+    const int Rp = (A.R + 63) & ~63;
+    const long long gb = xcd_block(blockIdx.x, gridDim.x);
+    const long long gid = gb * blockDim.x + threadIdx.x;
+    const int probe = int(gid / Rp), ray = int(gid % Rp);
+    unsigned long long cMip = 0, cFine = 0, cSteps = 0;
+    if (probe < A.P && ray < A.R) {
+        f3 o, d;
+        ray_setup(A, probeOffsets, probe, ray, o, d);
         const int C = (int)A.g.cascadesCount;
         const float traceMax = fminf(kWorldSize, A.g.cascadePosDistance[C - 1][3] * 2.0f);
         const float chunkSizeDistance = 32.0f / A.g.resolution;
         const float chunkMarginDistance = 4.0f / A.g.resolution;
+        const int texW = A.res * C, mipW = A.mipRes * C;
+        const f3 traceEnd = o + d * traceMax;
         float nextStart = 0.0f;
-        const f3 traceEnd = origin + dir * traceMax;
-        float hitTime = -1.0f, hitSdf = 0.0f;
-        int hitCascade = 0;
-        f3 hitNormal = mk(0.f, 0.f, 0.f);
-        for (int cas = 0; cas < C && hitTime < 0.0f; ++cas) {
+        float4 ra = make_float4(-1.0f, 0.f, 0.f, 0.f);
+        float2 rb = make_float2(0.f, 0.f);
+        for (int cas = 0; cas < C && ra.x < 0.0f; ++cas) {
             const f3 cpos = mk(A.g.cascadePosDistance[cas][0], A.g.cascadePosDistance[cas][1], A.g.cascadePosDistance[cas][2]);
             const float half = A.g.cascadePosDistance[cas][3];
             const float voxel = A.g.cascadeVoxelSize[cas];
-            const float voxelHalf = voxel * 0.5f;
-            // lineHitAABB (SDFCommon.glsl:84-95)
-            const f3 dl = traceEnd - origin;
+            // lineHitAABB (SDFCommon.glsl:84-95) and the cascade entry (:121-141)
+            const f3 dl = traceEnd - o;
             const f3 inv = mk(1.0f / dl.x, 1.0f / dl.y, 1.0f / dl.z);
-            const f3 en = (cpos - mk(half, half, half) - origin) * inv;
-            const f3 ex = (cpos + mk(half, half, half) - origin) * inv;
+            const f3 en = (cpos - mk(half, half, half) - o) * inv;
+            const f3 ex = (cpos + mk(half, half, half) - o) * inv;
             float ix = fmaxf(fminf(en.x, ex.x), fmaxf(fminf(en.y, ex.y), fminf(en.z, ex.z)));
             float iy = fminf(fmaxf(en.x, ex.x), fminf(fmaxf(en.y, ex.y), fmaxf(en.z, ex.z)));
             ix = clampf(ix, 0.0f, 1.0f) * traceMax;
@@ -257,58 +281,109 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A, const SS* __res
             if (ix >= iy) stepTime = iy;
             else nextStart = iy;
             const float cmax = half * 2.0f;
-            const int texW = A.res * C, mipW = A.mipRes * C;
             int step = 0;
             for (; step < A.maxSteps && stepTime < iy; ++step) {
-                const f3 p = origin + dir * stepTime;
+                const f3 p = o + d * stepTime;
                 const float ux = clampf((p.x - cpos.x) / cmax + 0.5f, 0.0f, 1.0f);
                 const float uy = clampf((p.y - cpos.y) / cmax + 0.5f, 0.0f, 1.0f);
                 const float uz = clampf((p.z - cpos.z) / cmax + 0.5f, 0.0f, 1.0f);
                 const float tx = (float(cas) + ux) / float(C);
-                float sd = sample3d<SS>(mip, mipW, A.mipRes, A.mipRes, tx, uy, uz);
-                if (COUNT) c.mip++;
+                // both lookups are issued together; the fine value is used only where
+                // the reference samples it (mip distance below the chunk size)
+                float sd = sample_volume<SS>(mip, mipW, A.mipRes, A.mipRes, tx, uy, uz);
+                const float st = fine_lookup<SS, BRICK>(sdf, texW, A.res, A.res, tx, uy, uz);
+                if (COUNT) cMip++;
                 if (sd < chunkSizeDistance) {
-                    const float st = sample3d<SS>(sdf, texW, A.res, A.res, tx, uy, uz);
-                    if (COUNT) c.fine++;
+                    if (COUNT) cFine++;
                     if (st < chunkMarginDistance * 2.0f) sd = st;
                 } else {
                     sd = chunkSizeDistance;
                 }
                 sd *= cmax;
-                const float thick = A.hitEps > 0.0f ? A.hitEps : voxelHalf * clampf(stepTime / voxel, 0.0f, 1.0f);
+                const float thick = A.hitEps > 0.0f ? A.hitEps : voxel * 0.5f * clampf(stepTime / voxel, 0.0f, 1.0f);
                 if (sd < thick) {
-                    hitTime = fmaxf(stepTime + sd - thick, 0.0f);
-                    hitCascade = cas;
-                    hitSdf = sd;
-                    const float o = 1.0f / A.g.resolution;
-                    const float xp = sample3d<SS>(sdf, texW, A.res, A.res, tx + o, uy, uz);
-                    const float xn = sample3d<SS>(sdf, texW, A.res, A.res, tx - o, uy, uz);
-                    const float yp = sample3d<SS>(sdf, texW, A.res, A.res, tx, uy + o, uz);
-                    const float yn = sample3d<SS>(sdf, texW, A.res, A.res, tx, uy - o, uz);
-                    const float zp = sample3d<SS>(sdf, texW, A.res, A.res, tx, uy, uz + o);
-                    const float zn = sample3d<SS>(sdf, texW, A.res, A.res, tx, uy, uz - o);
-                    hitNormal = normalize3(mk(xp - xn, yp - yn, zp - zn));
-                    if (COUNT) c.normals++;
+                    ra = make_float4(fmaxf(stepTime + sd - thick, 0.0f), tx, uy, uz);
+                    rb = make_float2(sd, float(cas));
                     break;
                 }
                 stepTime += fmaxf(sd, voxel);
             }
-            if (COUNT) c.steps += step;
+            if (COUNT) cSteps += step;
         }
-        // ray result encode (GISDFRays.comp:89-127)
+        const size_t rid = (size_t)probe * A.R + ray;
+        recA[rid] = ra;
+        recB[rid] = rb;
+    }
+    if (COUNT) {
+        __shared__ unsigned long long sc[3];
+        if (threadIdx.x < 3) sc[threadIdx.x] = 0;
+        __syncthreads();
+        atomicAdd(&sc[0], cMip);
+        atomicAdd(&sc[1], cFine);
+        atomicAdd(&sc[2], cSteps);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            atomicAdd(&counters[0], sc[0]);
+            atomicAdd(&counters[1], sc[1]);
+            atomicAdd(&counters[6], sc[2]);
+        }
+    }
+}
+
+// Normal, surface-atlas lookup and ray-result encode (SDFCommon.glsl:172-184,
+// GISDFRays.comp:89-127) for one ray per lane, 64 rays of a probe per wave.
+template <class SS, class SC, class SR, bool COUNT>
+__global__ __launch_bounds__(256) void shade_kernel(TraceArgs A, const SS* __restrict__ sdf,
+                                                    const uint32_t* __restrict__ chunks,
+                                                    const uint32_t* __restrict__ culled,
+                                                    const LuxgiAtlasObject* __restrict__ objects,
+                                                    const float* __restrict__ objInv,
+                                                    const LuxgiAtlasTile* __restrict__ tiles,
+                                                    const SC* __restrict__ color, const float* __restrict__ atlasDepth,
+                                                    const float* __restrict__ skyFaces,
+                                                    const float* __restrict__ probeOffsets,
+                                                    const float4* __restrict__ recA, const float2* __restrict__ recB,
+                                                    SR* __restrict__ outRad, SR* __restrict__ outDD,
+                                                    uint8_t* __restrict__ flags, unsigned long long* __restrict__ counters) {
+    const int Rp = (A.R + 63) & ~63;
+    const long long gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int probe = int(gid / Rp), ray = int(gid % Rp);
+    const bool active = probe < A.P && ray < A.R;
+    Counts c = {0, 0, 0, 0, 0, 0, 0};
+    if (active) {
+        f3 origin, dir;
+        ray_setup(A, probeOffsets, probe, ray, origin, dir);
+        const size_t rid = (size_t)probe * A.R + ray;
+        const float4 ra = recA[rid];
+        const float hitTime = ra.x;
         float4 rad = make_float4(0.f, 0.f, 0.f, 0.f);
         float dist;
         uint8_t f = 0;
         if (hitTime >= 0.0f) {
+            const float2 rb = recB[rid];
+            const float hitSdf = rb.x;
+            const int hitCascade = int(rb.y);
             if (COUNT) c.hits++;
             if (hitSdf <= 0.0f && hitTime <= A.g.cascadeVoxelSize[0]) {
                 dist = kWorldSize;
                 f = 2;
             } else {
+                const int C = (int)A.g.cascadesCount;
+                const int texW = A.res * C;
+                const float o = 1.0f / A.g.resolution;
+                const float tx = ra.y, uy = ra.z, uz = ra.w;
+                const float xp = (A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, tx + o, uy, uz) : sample_volume<SS>(sdf, texW, A.res, A.res, tx + o, uy, uz));
+                const float xn = (A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, tx - o, uy, uz) : sample_volume<SS>(sdf, texW, A.res, A.res, tx - o, uy, uz));
+                const float yp = (A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, tx, uy + o, uz) : sample_volume<SS>(sdf, texW, A.res, A.res, tx, uy + o, uz));
+                const float yn = (A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, tx, uy - o, uz) : sample_volume<SS>(sdf, texW, A.res, A.res, tx, uy - o, uz));
+                const float zp = (A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, tx, uy, uz + o) : sample_volume<SS>(sdf, texW, A.res, A.res, tx, uy, uz + o));
+                const float zn = (A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, tx, uy, uz - o) : sample_volume<SS>(sdf, texW, A.res, A.res, tx, uy, uz - o));
+                const f3 n = normalize3(mk(xp - xn, yp - yn, zp - zn));
+                if (COUNT) c.normals++;
                 const f3 hp = origin + dir * hitTime;
                 const float thr = A.g.cascadeVoxelSize[hitCascade] * 1.05f;
                 const float3 s = sample_atlas<SC, COUNT>(A.ad, chunks, culled, objects, objInv, tiles, color,
-                                                         atlasDepth, hp, hitNormal, thr, c);
+                                                         atlasDepth, hp, n, thr, c);
                 rad = make_float4(s.x, s.y, s.z, 0.f);
                 dist = fmaxf(hitTime + A.g.cascadeVoxelSize[hitCascade] * 0.5f, 0.0f);
                 f = 1;
@@ -328,38 +403,71 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A, const SS* __res
         if (threadIdx.x < 8) sc[threadIdx.x] = 0;
         __syncthreads();
         if (active) {
-            atomicAdd(&sc[0], c.mip);
-            atomicAdd(&sc[1], c.fine);
             atomicAdd(&sc[2], c.normals);
             atomicAdd(&sc[3], c.objects);
             atomicAdd(&sc[4], c.tiles);
             atomicAdd(&sc[5], 1ull);
-            atomicAdd(&sc[6], c.steps);
             atomicAdd(&sc[7], c.hits);
         }
         __syncthreads();
-        if (threadIdx.x < 8) atomicAdd(&counters[threadIdx.x], sc[threadIdx.x]);
+        if (threadIdx.x < 8 && threadIdx.x != 0 && threadIdx.x != 1 && threadIdx.x != 6)
+            atomicAdd(&counters[threadIdx.x], sc[threadIdx.x]);
     }
+}
+
+// per-object inverse transforms, once per call
+__global__ void trace_prepare_kernel(const LuxgiAtlasObject* __restrict__ objects, int n, float* __restrict__ inv) {
+    object_inverse_kernel_body(objects, n, inv);
 }
 
 }  // namespace luxgi
 
 using namespace luxgi;
 
-extern "C" size_t luxgi_ddgi_trace_workspace_size(int32_t objectCapacity) {
-    return (size_t)(objectCapacity > 0 ? objectCapacity : 0) * 16 * sizeof(float) + 256;
+static size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+struct TraceWorkspace {
+    float* inv;
+    float4* recA;
+    float2* recB;
+};
+
+static size_t trace_workspace_layout(size_t rays, int32_t objects, char* base, TraceWorkspace* w) {
+    size_t off = 0;
+    if (w) w->inv = reinterpret_cast<float*>(base + off);
+    off += align256((size_t)(objects > 0 ? objects : 0) * 16 * sizeof(float));
+    if (w) w->recA = reinterpret_cast<float4*>(base + off);
+    off += align256(rays * sizeof(float4));
+    if (w) w->recB = reinterpret_cast<float2*>(base + off);
+    off += align256(rays * sizeof(float2));
+    return off;
+}
+
+extern "C" size_t luxgi_ddgi_trace_workspace_size(const LuxgiDDGIUniform* u, int32_t objectCapacity) {
+    if (!u) return 0;
+    const size_t P = (size_t)u->probeCounts[0] * u->probeCounts[1] * u->probeCounts[2];
+    const size_t R = (size_t)(u->raysPerProbe > 0 ? u->raysPerProbe : 0);
+    return trace_workspace_layout(P * R, objectCapacity, nullptr, nullptr);
 }
 
 template <class SS, class SC, class SR, bool COUNT>
 static int launch_trace(const TraceArgs& A, const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at,
                         const LuxgiSkybox* sky, const float* probeOffsets, const LuxgiRayBuffers* rays,
-                        uint8_t* rayFlags, float* objInv, unsigned long long* counters, hipStream_t st) {
-    const long long total = (long long)A.P * ((A.R + 63) & ~63);
-    const int blocks = (int)((total + 255) / 256);
-    hipLaunchKernelGGL((trace_kernel<SS, SC, SR, COUNT>), dim3(blocks), dim3(256), 0, st, A,
-                       static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), at->chunks,
-                       at->culledObjects, at->objects, objInv, at->tiles, static_cast<const SC*>(at->color), at->depth,
-                       sky && sky->faces ? sky->faces : nullptr, probeOffsets, static_cast<SR*>(rays->radiance),
+                        uint8_t* rayFlags, const TraceWorkspace& w, unsigned long long* counters, hipStream_t st) {
+    const long long lanes = (long long)A.P * ((A.R + 63) & ~63);
+    if (A.fineLayout == LUXGI_LAYOUT_BRICK4)
+        hipLaunchKernelGGL((march_kernel<SS, COUNT, true>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st, A,
+                           static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), probeOffsets, w.recA,
+                           w.recB, counters);
+    else
+        hipLaunchKernelGGL((march_kernel<SS, COUNT, false>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st, A,
+                           static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), probeOffsets, w.recA,
+                           w.recB, counters);
+    LUXGI_LAUNCH_CHECK();
+    hipLaunchKernelGGL((shade_kernel<SS, SC, SR, COUNT>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st, A,
+                       static_cast<const SS*>(vol->sdf), at->chunks, at->culledObjects, at->objects, w.inv, at->tiles,
+                       static_cast<const SC*>(at->color), at->depth, sky && sky->faces ? sky->faces : nullptr,
+                       probeOffsets, w.recA, w.recB, static_cast<SR*>(rays->radiance),
                        static_cast<SR*>(rays->directionDistance), rayFlags, counters);
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
@@ -368,11 +476,11 @@ static int launch_trace(const TraceArgs& A, const LuxgiSDFVolume* vol, const Lux
 template <class SS, class SR, bool COUNT>
 static int dispatch_color(const TraceArgs& A, const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at,
                           const LuxgiSkybox* sky, const float* off, const LuxgiRayBuffers* rays, uint8_t* fl,
-                          float* inv, unsigned long long* cnt, hipStream_t st) {
+                          const TraceWorkspace& w, unsigned long long* cnt, hipStream_t st) {
     switch (at->colorStorage) {
-        case LUXGI_STORAGE_FP32: return launch_trace<SS, float, SR, COUNT>(A, vol, at, sky, off, rays, fl, inv, cnt, st);
-        case LUXGI_STORAGE_FP16: return launch_trace<SS, __half, SR, COUNT>(A, vol, at, sky, off, rays, fl, inv, cnt, st);
-        case LUXGI_STORAGE_UNORM8: return launch_trace<SS, Unorm8, SR, COUNT>(A, vol, at, sky, off, rays, fl, inv, cnt, st);
+        case LUXGI_STORAGE_FP32: return launch_trace<SS, float, SR, COUNT>(A, vol, at, sky, off, rays, fl, w, cnt, st);
+        case LUXGI_STORAGE_FP16: return launch_trace<SS, __half, SR, COUNT>(A, vol, at, sky, off, rays, fl, w, cnt, st);
+        case LUXGI_STORAGE_UNORM8: return launch_trace<SS, Unorm8, SR, COUNT>(A, vol, at, sky, off, rays, fl, w, cnt, st);
         default: return LUXGI_ERR_UNSUPPORTED;
     }
 }
@@ -405,24 +513,29 @@ static int trace_entry(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, c
     A.R = u->raysPerProbe;
     A.rayWidth = rays->width;
     A.hitEps = p->hitEpsilon;
+    A.fineLayout = vol->layout;
+    if (vol->layout != LUXGI_LAYOUT_LINEAR && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (vol->layout == LUXGI_LAYOUT_BRICK4 &&
+        (vol->storage != LUXGI_STORAGE_FP32 || (vol->resolution & 3) || ((vol->resolution * (int)g->cascadesCount) & 3)))
+        return LUXGI_ERR_INVALID_ARGUMENT;
     if (A.P <= 0 || A.R <= 0 || A.R > rays->width || rays->height < A.P) return LUXGI_ERR_INVALID_ARGUMENT;
-    const size_t need = luxgi_ddgi_trace_workspace_size(at->objectCapacity);
-    if (at->objectCapacity > 0 && (!workspace || workspaceBytes < need)) return LUXGI_ERR_WORKSPACE_TOO_SMALL;
+    if ((long long)A.P * A.R >= (1ll << 31)) return LUXGI_ERR_UNSUPPORTED;
+    const size_t need = luxgi_ddgi_trace_workspace_size(u, at->objectCapacity);
+    if (!workspace || workspaceBytes < need) return LUXGI_ERR_WORKSPACE_TOO_SMALL;
+    TraceWorkspace w;
+    trace_workspace_layout((size_t)A.P * A.R, at->objectCapacity, static_cast<char*>(workspace), &w);
     hipStream_t st = as_stream(stream);
-    float* inv = static_cast<float*>(workspace);
-    if (at->objectCapacity > 0) {
-        hipLaunchKernelGGL(object_inverse_kernel, dim3((at->objectCapacity + 255) / 256), dim3(256), 0, st,
-                           at->objects, at->objectCapacity, inv);
-        LUXGI_LAUNCH_CHECK();
-    }
+    hipLaunchKernelGGL(trace_prepare_kernel, dim3(std::max(1, (at->objectCapacity + 255) / 256)), dim3(256), 0, st,
+                       at->objects, at->objectCapacity, w.inv);
+    LUXGI_LAUNCH_CHECK();
     if (vol->storage == LUXGI_STORAGE_FP32 && rays->storage == LUXGI_STORAGE_FP32)
-        return dispatch_color<float, float, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, inv, counters, st);
+        return dispatch_color<float, float, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, w, counters, st);
     if (vol->storage == LUXGI_STORAGE_FP32 && rays->storage == LUXGI_STORAGE_FP16)
-        return dispatch_color<float, __half, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, inv, counters, st);
+        return dispatch_color<float, __half, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, w, counters, st);
     if (vol->storage == LUXGI_STORAGE_FP16 && rays->storage == LUXGI_STORAGE_FP32)
-        return dispatch_color<__half, float, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, inv, counters, st);
+        return dispatch_color<__half, float, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, w, counters, st);
     if (vol->storage == LUXGI_STORAGE_FP16 && rays->storage == LUXGI_STORAGE_FP16)
-        return dispatch_color<__half, __half, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, inv, counters, st);
+        return dispatch_color<__half, __half, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, w, counters, st);
     return LUXGI_ERR_UNSUPPORTED;
 }
 

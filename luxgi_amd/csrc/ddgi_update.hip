@@ -124,11 +124,19 @@ __global__ __launch_bounds__(256) void probe_update_kernel(UpdateArgs A, const S
                                                            const SA* __restrict__ prevDepth, SA* __restrict__ outIrr,
                                                            SA* __restrict__ outDepth, const float* __restrict__ Wi,
                                                            const float* __restrict__ Si, const float* __restrict__ Wd,
-                                                           const float* __restrict__ Sd) {
+                                                           const float* __restrict__ Sd,
+                                                           const uint8_t* __restrict__ onlyFlagged) {
     __shared__ UpdateLds lds;
     __shared__ int sShared[kProbesPerBlock];
     const int tid = threadIdx.x;
     const int p0 = blockIdx.x * kProbesPerBlock;
+    // fix-up mode: only the probes flagged by the MFMA pass are recomputed
+    auto wanted = [&](int p) { return p < A.P && (!onlyFlagged || onlyFlagged[p] != 0); };
+    if (onlyFlagged) {
+        bool any = false;
+        for (int g = 0; g < kProbesPerBlock; ++g) any |= wanted(p0 + g);
+        if (!any) return;
+    }
     const int Ti = A.ni * A.ni, Td = A.nd * A.nd;
     const int pxy = A.px * A.py;
 
@@ -197,6 +205,7 @@ __global__ __launch_bounds__(256) void probe_update_kernel(UpdateArgs A, const S
         for (int g = 0; g < kProbesPerBlock; ++g) {
             const int p = p0 + g;
             if (p >= A.P) break;
+            if (!wanted(p)) continue;
             const float total = totD[g];
             float x = accD[g][0], y = accD[g][1];
             if (total > kFltEps) {
@@ -220,6 +229,7 @@ __global__ __launch_bounds__(256) void probe_update_kernel(UpdateArgs A, const S
         for (int g = giBegin; g < giEnd; ++g) {
             const int p = p0 + g;
             if (p >= A.P) break;
+            if (!wanted(p)) continue;
             const float total = totI[g];
             float r = accI[g][0], gg = accI[g][1], b = accI[g][2];
             if (total > kFltEps) {
@@ -254,7 +264,7 @@ __global__ __launch_bounds__(256) void probe_update_kernel(UpdateArgs A, const S
         for (int i = tid; i < kProbesPerBlock * nb; i += blockDim.x) {
             const int g = i / nb, k = i % nb;
             const int p = p0 + g;
-            if (p >= A.P) continue;
+            if (!wanted(p)) continue;
             int sx, sy, dx, dy;
             const int side = k / n, j = k % n + 1;
             if (k < 4 * n) {
@@ -304,6 +314,225 @@ __global__ void border_kernel(SA* __restrict__ atlas, int W, int n, int pxy, int
     }
 }
 
+// ---------------------------------------------------------------------------
+// MFMA form. With shared ray directions the blend of a block of 16 probes is
+//   depth: C[Td x 32] = W_d[Td x R] . B_d[R x 32]   (B_d columns: probe g, (d, d^2))
+//   irr:   C[Ti x 48] = W_i[Ti x R] . B_i[R x 48]   (B_i columns: probe g, (r, g, b))
+// computed with v_mfma_f32_16x16x4_f32 (FP32 in, FP32 accumulate). Lane layout
+// of that instruction: A[l%16][l/16], B[l/16][l%16], D[4*(l/16)+v][l%16].
+// W comes from global memory (L2-resident, padded to 16-row tiles, zero rows
+// beyond T); B is staged in LDS per 32-ray chunk. The epilogue normalises by
+// 1/(2 S[t]), applies the gamma, stages the interior in LDS and writes each
+// probe's full bordered tile with the hysteresis blend. A probe whose staged
+// directions differ from ray row 0 is flagged; probe_update_kernel then
+// recomputes the flagged probes with their own directions.
+// ---------------------------------------------------------------------------
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+constexpr int kMG = 16;    // probes per block
+constexpr int kKC = 32;    // rays per staged chunk
+constexpr int kBStride = 48;  // LDS row stride of the B tiles (== 16 mod 32: conflict-free reads)
+
+__global__ void weights_kernel_tiled(const float* __restrict__ dirs0, int R, int n, int tpad, int depth,
+                                     float sharpness, float* __restrict__ W) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= R * tpad) return;
+    const int r = idx / tpad, t = idx % tpad;
+    float w = 0.0f;
+    if (t < n * n) {
+        const f3 td = texel_direction(t % n, t / n, n);
+        w = fmaxf(0.0f, dot3(td, mk(dirs0[r * 4 + 0], dirs0[r * 4 + 1], dirs0[r * 4 + 2])));
+        if (depth) w = powf(w, sharpness);
+        if (!(w >= kFltEps)) w = 0.0f;
+    }
+    W[idx] = w;
+}
+
+// S[t] = sum_r W[r][t] in ray order (ProbeUpdate.glsl:100 accumulates totalWeight in that order)
+__global__ void weight_sums_kernel(const float* __restrict__ W, int R, int tpad, float* __restrict__ S) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tpad) return;
+    float s = 0.0f;
+    for (int r = 0; r < R; ++r) s += W[(size_t)r * tpad + t];
+    S[t] = s;
+}
+
+// directions of ray row 0 as FP32 (the values every probe's rays are compared with)
+template <class SR>
+__global__ void row0_kernel(const SR* __restrict__ dirDist, int R, float* __restrict__ dirs0) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R) reinterpret_cast<float4*>(dirs0)[r] = Texel<SR>::ld4(dirDist, (size_t)r);
+}
+
+// source interior texel of bordered-tile texel (bx, by) of an n x n probe
+// (BorderUpdate.glsl:25-133: edges mirror across the edge's centre, corners take the opposite corner)
+__device__ __forceinline__ void border_source(int bx, int by, int n, int& sx, int& sy) {
+    const bool inX = bx >= 1 && bx <= n, inY = by >= 1 && by <= n;
+    if (inX && inY) { sx = bx; sy = by; }
+    else if (inX) { sx = n + 1 - bx; sy = by == 0 ? 1 : n; }
+    else if (inY) { sy = n + 1 - by; sx = bx == 0 ? 1 : n; }
+    else { sx = bx == 0 ? n : 1; sy = by == 0 ? n : 1; }
+}
+
+struct MfmaStage {
+    float bd[kKC][kBStride];
+    float bi[kKC][kBStride];
+};
+struct MfmaOut {
+    float2 dep[kMG][256];
+    float4 irr[kMG][64];
+};
+union MfmaLds {
+    MfmaStage st;
+    MfmaOut out;
+};
+
+template <class SR, class SA>
+__global__ __launch_bounds__(256) void probe_update_mfma_kernel(
+    UpdateArgs A, const SR* __restrict__ radiance, const SR* __restrict__ dirDist, const SA* __restrict__ prevIrr,
+    const SA* __restrict__ prevDepth, SA* __restrict__ outIrr, SA* __restrict__ outDepth, const float* __restrict__ dirs0,
+    const float* __restrict__ Wi, const float* __restrict__ Si, const float* __restrict__ Wd,
+    const float* __restrict__ Sd, uint8_t* __restrict__ mismatch, int tpadI, int tpadD) {
+    __shared__ MfmaLds lds;
+    __shared__ int sMis[kMG];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int p0 = blockIdx.x * kMG;
+    const int MtD = A.doDepth ? tpadD / 16 : 0, MtI = A.doIrr ? tpadI / 16 : 0;
+    if (tid < kMG) sMis[tid] = 0;
+    floatx4 accD[4][2], accI[3];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) accD[j][0] = accD[j][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) accI[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    for (int r0 = 0; r0 < A.R; r0 += kKC) {
+        __syncthreads();
+        for (int i = tid; i < kMG * kKC; i += 256) {
+            const int g = i / kKC, rr = i % kKC, r = r0 + rr, p = p0 + g;
+            float d0 = 0.f, d1 = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+            if (p < A.P && r < A.R) {
+                const size_t e = (size_t)p * A.rayWidth + r;
+                const float4 dd = Texel<SR>::ld4(dirDist, e);
+                float dist = fminf(A.maxDistance, dd.w - 0.01f);  // ProbeUpdate.glsl:75-78
+                if (dist == -1.0f) dist = A.maxDistance;
+                d0 = dist;
+                d1 = dist * dist;
+                if (A.doIrr) {
+                    const float4 rc = Texel<SR>::ld4(radiance, e);
+                    c0 = rc.x;
+                    c1 = rc.y;
+                    c2 = rc.z;
+                }
+                if (dd.x != dirs0[r * 4 + 0] || dd.y != dirs0[r * 4 + 1] || dd.z != dirs0[r * 4 + 2]) sMis[g] = 1;
+            }
+            lds.st.bd[rr][g * 2 + 0] = d0;
+            lds.st.bd[rr][g * 2 + 1] = d1;
+            lds.st.bi[rr][g * 3 + 0] = c0;
+            lds.st.bi[rr][g * 3 + 1] = c1;
+            lds.st.bi[rr][g * 3 + 2] = c2;
+        }
+        __syncthreads();
+        const int col = lane & 15;
+#pragma unroll
+        for (int ks = 0; ks < kKC / 4; ++ks) {
+            const int k = ks * 4 + (lane >> 4);
+            const int r = r0 + k;
+            const bool rv = r < A.R;
+            if (MtD) {
+                const float b0 = lds.st.bd[k][col], b1 = lds.st.bd[k][16 + col];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int m = wave + 4 * j;
+                    if (m < MtD) {
+                        const float a = rv ? Wd[(size_t)r * tpadD + m * 16 + col] : 0.0f;
+                        accD[j][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, accD[j][0], 0, 0, 0);
+                        accD[j][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, accD[j][1], 0, 0, 0);
+                    }
+                }
+            }
+            if (wave < MtI) {
+                const float a = rv ? Wi[(size_t)r * tpadI + wave * 16 + col] : 0.0f;
+                accI[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, lds.st.bi[k][col], accI[0], 0, 0, 0);
+                accI[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, lds.st.bi[k][16 + col], accI[1], 0, 0, 0);
+                accI[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, lds.st.bi[k][32 + col], accI[2], 0, 0, 0);
+            }
+        }
+    }
+    __syncthreads();  // the LDS now holds the finished interiors
+    const int Td = A.nd * A.nd, Ti = A.ni * A.ni;
+    const int col = lane & 15, rowq = (lane >> 4) * 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int m = wave + 4 * j;
+        if (m >= MtD) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int t = m * 16 + rowq + v;
+            if (t >= Td) continue;
+            const float s = Sd[t];
+            const float sc = s > kFltEps ? 1.0f / (2.0f * s) : 1.0f;
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                const int c = nt * 16 + col;
+                float* dst = reinterpret_cast<float*>(&lds.out.dep[c >> 1][t]);
+                dst[c & 1] = accD[j][nt][v] * sc;
+            }
+        }
+    }
+    if (wave < MtI) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int t = wave * 16 + rowq + v;
+            if (t >= Ti) continue;
+            const float s = Si[t];
+            const float sc = s > kFltEps ? 1.0f / (2.0f * s) : 1.0f;
+#pragma unroll
+            for (int nt = 0; nt < 3; ++nt) {
+                const int c = nt * 16 + col;
+                float* dst = reinterpret_cast<float*>(&lds.out.irr[c / 3][t]);
+                dst[c % 3] = powf(accI[nt][v] * sc, A.invGamma);
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < kMG && p0 + tid < A.P) mismatch[p0 + tid] = (uint8_t)sMis[tid];
+    const int pxy = A.px * A.py;
+    for (int map = 0; map < 2; ++map) {
+        if ((map == 0 && !A.doIrr) || (map == 1 && !A.doDepth)) continue;
+        const int n = map == 0 ? A.ni : A.nd, pw = n + 2;
+        const int total = kMG * pw * pw;
+        for (int i = tid; i < total; i += 256) {
+            const int bx = i % pw, g = (i / pw) % kMG, by = i / (pw * kMG);
+            const int p = p0 + g;
+            if (p >= A.P) continue;
+            const bool interior = bx >= 1 && bx <= n && by >= 1 && by <= n;
+            if (!interior && !A.fuseBorder) continue;
+            int sx, sy;
+            border_source(bx, by, n, sx, sy);
+            const int src = (sy - 1) * n + (sx - 1);
+            const int ox = (p % pxy) * pw + 1, oy = (p / pxy) * pw + 1;
+            if (map == 0) {
+                float4 v = lds.out.irr[g][src];
+                v.w = 1.0f;
+                if (!A.firstFrame) {
+                    const float4 pv = Texel<SA>::ld4(prevIrr, (size_t)(oy + sy) * A.Wi + ox + sx);
+                    v.x = mixf(v.x, pv.x, A.hysteresis);
+                    v.y = mixf(v.y, pv.y, A.hysteresis);
+                    v.z = mixf(v.z, pv.z, A.hysteresis);
+                }
+                Texel<SA>::st4(outIrr, (size_t)(oy + by) * A.Wi + ox + bx, v);
+            } else {
+                float2 v = lds.out.dep[g][src];
+                if (!A.firstFrame) {
+                    const float2 pv = Texel<SA>::ld2(prevDepth, (size_t)(oy + sy) * A.Wd + ox + sx);
+                    v.x = mixf(v.x, pv.x, A.hysteresis);
+                    v.y = mixf(v.y, pv.y, A.hysteresis);
+                }
+                Texel<SA>::st2(outDepth, (size_t)(oy + by) * A.Wd + ox + bx, v);
+            }
+        }
+    }
+}
+
 static int check_uniform(const LuxgiDDGIUniform* u) {
     if (!u) return LUXGI_ERR_INVALID_ARGUMENT;
     const int px = u->probeCounts[0], py = u->probeCounts[1], pz = u->probeCounts[2];
@@ -333,38 +562,100 @@ extern "C" int luxgi_ddgi_fill_atlas_sizes(LuxgiDDGIUniform* u) {
     return LUXGI_OK;
 }
 
+// workspace: row-0 directions, W and S for both maps (16-row padded), mismatch flags
+struct UpdateWorkspace {
+    float *dirs0, *Wi, *Si, *Wd, *Sd;
+    uint8_t* mismatch;
+};
+
+static int pad16(int t) { return (t + 15) & ~15; }
+
+static size_t update_workspace_layout(const LuxgiDDGIUniform* u, char* base, UpdateWorkspace* w) {
+    const size_t R = (size_t)(u->raysPerProbe > 0 ? u->raysPerProbe : 0);
+    const size_t P = (size_t)u->probeCounts[0] * u->probeCounts[1] * u->probeCounts[2];
+    const size_t Ti = pad16(u->irradianceProbeSideLength * u->irradianceProbeSideLength);
+    const size_t Td = pad16(u->depthProbeSideLength * u->depthProbeSideLength);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base ? base + off : nullptr;
+        off += align256(bytes);
+        return p;
+    };
+    char* d0 = take(R * 16);
+    char* wi = take(R * Ti * 4);
+    char* si = take(Ti * 4);
+    char* wd = take(R * Td * 4);
+    char* sd = take(Td * 4);
+    char* mm = take(P);
+    if (w) {
+        w->dirs0 = reinterpret_cast<float*>(d0);
+        w->Wi = reinterpret_cast<float*>(wi);
+        w->Si = reinterpret_cast<float*>(si);
+        w->Wd = reinterpret_cast<float*>(wd);
+        w->Sd = reinterpret_cast<float*>(sd);
+        w->mismatch = reinterpret_cast<uint8_t*>(mm);
+    }
+    return off;
+}
+
 extern "C" size_t luxgi_ddgi_probe_update_workspace_size(const LuxgiDDGIUniform* u) {
     if (!u) return 0;
-    const size_t R = (size_t)(u->raysPerProbe > 0 ? u->raysPerProbe : 0);
-    const size_t Ti = (size_t)u->irradianceProbeSideLength * u->irradianceProbeSideLength;
-    const size_t Td = (size_t)u->depthProbeSideLength * u->depthProbeSideLength;
-    return align256(R * Ti * 4) + align256(Ti * 4) + align256(R * Td * 4) + align256(Td * 4);
+    return update_workspace_layout(u, nullptr, nullptr);
 }
 
 template <class SR, class SA>
 static int launch_update(const UpdateArgs& A, const LuxgiRayBuffers* rays, const LuxgiProbeAtlas* prev,
-                         const LuxgiProbeAtlas* out, float* Wi, float* Si, float* Wd, float* Sd, hipStream_t st) {
+                         const LuxgiProbeAtlas* out, const UpdateWorkspace& w, hipStream_t st) {
     const SR* rad = static_cast<const SR*>(rays->radiance);
     const SR* dd = static_cast<const SR*>(rays->directionDistance);
+    const SA* pI = static_cast<const SA*>(prev ? prev->irradiance : nullptr);
+    const SA* pD = static_cast<const SA*>(prev ? prev->depth : nullptr);
+    SA* oI = static_cast<SA*>(out->irradiance);
+    SA* oD = static_cast<SA*>(out->depth);
+    const bool mfma = !A.forcePerProbe && A.ni * A.ni <= 64 && A.nd * A.nd <= 256;
+    if (mfma) {
+        const int tpI = pad16(A.ni * A.ni), tpD = pad16(A.nd * A.nd);
+        hipLaunchKernelGGL(row0_kernel<SR>, dim3((A.R + 255) / 256), dim3(256), 0, st, dd, A.R, w.dirs0);
+        LUXGI_LAUNCH_CHECK();
+        if (A.doIrr) {
+            hipLaunchKernelGGL(weights_kernel_tiled, dim3((A.R * tpI + 255) / 256), dim3(256), 0, st, w.dirs0, A.R, A.ni,
+                               tpI, 0, A.sharpness, w.Wi);
+            hipLaunchKernelGGL(weight_sums_kernel, dim3((tpI + 63) / 64), dim3(64), 0, st, w.Wi, A.R, tpI, w.Si);
+            LUXGI_LAUNCH_CHECK();
+        }
+        if (A.doDepth) {
+            hipLaunchKernelGGL(weights_kernel_tiled, dim3((A.R * tpD + 255) / 256), dim3(256), 0, st, w.dirs0, A.R, A.nd,
+                               tpD, 1, A.sharpness, w.Wd);
+            hipLaunchKernelGGL(weight_sums_kernel, dim3((tpD + 63) / 64), dim3(64), 0, st, w.Wd, A.R, tpD, w.Sd);
+            LUXGI_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL((probe_update_mfma_kernel<SR, SA>), dim3((A.P + kMG - 1) / kMG), dim3(256), 0, st, A, rad, dd,
+                           pI, pD, oI, oD, w.dirs0, w.Wi, w.Si, w.Wd, w.Sd, w.mismatch, tpI, tpD);
+        LUXGI_LAUNCH_CHECK();
+        // probes whose directions differ from row 0: reference arithmetic with their own directions
+        UpdateArgs F = A;
+        F.forcePerProbe = 1;
+        hipLaunchKernelGGL((probe_update_kernel<SR, SA>), dim3((A.P + kProbesPerBlock - 1) / kProbesPerBlock), dim3(256),
+                           0, st, F, rad, dd, pI, pD, oI, oD, nullptr, nullptr, nullptr, nullptr, w.mismatch);
+        LUXGI_LAUNCH_CHECK();
+        return LUXGI_OK;
+    }
     if (!A.forcePerProbe) {
         if (A.doIrr) {
             const int T = A.ni * A.ni;
             hipLaunchKernelGGL(build_weights_kernel<SR>, dim3((T + 255) / 256), dim3(256), 0, st, dd, A.R, A.ni, 0,
-                               A.sharpness, Wi, Si);
+                               A.sharpness, w.Wi, w.Si);
             LUXGI_LAUNCH_CHECK();
         }
         if (A.doDepth) {
             const int T = A.nd * A.nd;
             hipLaunchKernelGGL(build_weights_kernel<SR>, dim3((T + 255) / 256), dim3(256), 0, st, dd, A.R, A.nd, 1,
-                               A.sharpness, Wd, Sd);
+                               A.sharpness, w.Wd, w.Sd);
             LUXGI_LAUNCH_CHECK();
         }
     }
-    const int blocks = (A.P + kProbesPerBlock - 1) / kProbesPerBlock;
-    hipLaunchKernelGGL((probe_update_kernel<SR, SA>), dim3(blocks), dim3(256), 0, st, A, rad, dd,
-                       static_cast<const SA*>(prev ? prev->irradiance : nullptr),
-                       static_cast<const SA*>(prev ? prev->depth : nullptr), static_cast<SA*>(out->irradiance),
-                       static_cast<SA*>(out->depth), Wi, Si, Wd, Sd);
+    hipLaunchKernelGGL((probe_update_kernel<SR, SA>), dim3((A.P + kProbesPerBlock - 1) / kProbesPerBlock), dim3(256), 0,
+                       st, A, rad, dd, pI, pD, oI, oD, w.Wi, w.Si, w.Wd, w.Sd, nullptr);
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }
@@ -403,30 +694,19 @@ extern "C" int luxgi_ddgi_probe_update(const LuxgiDDGIUniform* u, const LuxgiRay
     if (!A.firstFrame && (!prev || (A.doIrr && !prev->irradiance) || (A.doDepth && !prev->depth)))
         return LUXGI_ERR_INVALID_ARGUMENT;
     if (prev && !A.firstFrame && prev->storage != out->storage) return LUXGI_ERR_INVALID_ARGUMENT;
-    float *Wi = nullptr, *Si = nullptr, *Wd = nullptr, *Sd = nullptr;
-    if (!A.forcePerProbe) {
-        const size_t need = luxgi_ddgi_probe_update_workspace_size(u);
-        if (!workspace || workspaceBytes < need) return LUXGI_ERR_WORKSPACE_TOO_SMALL;
-        const size_t Ti = (size_t)A.ni * A.ni, Td = (size_t)A.nd * A.nd;
-        char* w = static_cast<char*>(workspace);
-        Wi = reinterpret_cast<float*>(w);
-        w += align256((size_t)A.R * Ti * 4);
-        Si = reinterpret_cast<float*>(w);
-        w += align256(Ti * 4);
-        Wd = reinterpret_cast<float*>(w);
-        w += align256((size_t)A.R * Td * 4);
-        Sd = reinterpret_cast<float*>(w);
-    }
+    const size_t need = update_workspace_layout(u, nullptr, nullptr);
+    if (!workspace || workspaceBytes < need) return LUXGI_ERR_WORKSPACE_TOO_SMALL;
+    UpdateWorkspace w;
+    update_workspace_layout(u, static_cast<char*>(workspace), &w);
     hipStream_t st = as_stream(stream);
     const int rs = rays->storage, as = out->storage;
-    if (rs == LUXGI_STORAGE_FP32 && as == LUXGI_STORAGE_FP32)
-        return launch_update<float, float>(A, rays, prev, out, Wi, Si, Wd, Sd, st);
+    if (rs == LUXGI_STORAGE_FP32 && as == LUXGI_STORAGE_FP32) return launch_update<float, float>(A, rays, prev, out, w, st);
     if (rs == LUXGI_STORAGE_FP16 && as == LUXGI_STORAGE_FP16)
-        return launch_update<__half, __half>(A, rays, prev, out, Wi, Si, Wd, Sd, st);
+        return launch_update<__half, __half>(A, rays, prev, out, w, st);
     if (rs == LUXGI_STORAGE_FP32 && as == LUXGI_STORAGE_FP16)
-        return launch_update<float, __half>(A, rays, prev, out, Wi, Si, Wd, Sd, st);
+        return launch_update<float, __half>(A, rays, prev, out, w, st);
     if (rs == LUXGI_STORAGE_FP16 && as == LUXGI_STORAGE_FP32)
-        return launch_update<__half, float>(A, rays, prev, out, Wi, Si, Wd, Sd, st);
+        return launch_update<__half, float>(A, rays, prev, out, w, st);
     return LUXGI_ERR_UNSUPPORTED;
 }
 

@@ -38,13 +38,29 @@ __global__ void mip_pass_kernel(const float* __restrict__ src, int srcW, int src
     dst[((size_t)z * dstH + y) * dstW + x + mipOffsetX] = best;
 }
 
+__global__ void to_bricks_kernel(const float* __restrict__ src, int W, int H, int D, float* __restrict__ dst) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= W) return;
+    dst[brick4_index(x, y, z, W >> 2, H >> 2)] = src[((size_t)z * H + y) * W + x];
+}
+
 }  // namespace luxgi
 
 using namespace luxgi;
 
+extern "C" int luxgi_sdf_to_bricks(const float* src, int32_t W, int32_t H, int32_t D, float* dst, void* stream) {
+    if (!src || !dst || W <= 0 || H <= 0 || D <= 0 || (W & 3) || (H & 3) || (D & 3)) return LUXGI_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(to_bricks_kernel, dim3((W + 63) / 64, H, D), dim3(64), 0, as_stream(stream), src, W, H, D, dst);
+    LUXGI_LAUNCH_CHECK();
+    return LUXGI_OK;
+}
+
 extern "C" int luxgi_sdf_build_mip(const float* sdf, int32_t res, int32_t cascades, int32_t cascade, float maxDistance,
                                    int32_t floodPasses, float* mip, float* temp, void* stream) {
-    if (!sdf || !mip || res <= 0 || cascades <= 0 || cascade < 0 || cascade >= cascades || floodPasses < 0)
+    // floodPasses alternate Mip -> Tmp -> Mip; an odd count would leave the result in `temp`
+    if (!sdf || !mip || res <= 0 || cascades <= 0 || cascade < 0 || cascade >= cascades || floodPasses < 0 ||
+        (floodPasses & 1))
         return LUXGI_ERR_INVALID_ARGUMENT;
     if (floodPasses > 0 && !temp) return LUXGI_ERR_INVALID_ARGUMENT;
     const int mr = (res + 3) / 4;

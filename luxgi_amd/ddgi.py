@@ -41,14 +41,22 @@ def ptr(t):
 class DeviceScene:
     """SDF volume + surface atlas uploaded to the GPU (the inputs of the trace)."""
 
-    def __init__(self, sdf_data, sdf, mip, atlas, sdf_storage=abi.STORAGE_FP32, sky_faces=None, sky_size=0):
+    def __init__(self, sdf_data, sdf, mip, atlas, sdf_storage=abi.STORAGE_FP32, sky_faces=None, sky_size=0,
+                 layout=abi.LAYOUT_LINEAR):
         self.sdf_data = sdf_data
         self.res = int(sdf_data.resolution)
         self.mip_res = (self.res + 3) // 4
         sdt = torch.float16 if sdf_storage == abi.STORAGE_FP16 else torch.float32
         self.sdf = sdf if torch.is_tensor(sdf) else to_device(sdf, sdt)
         self.mip = mip if torch.is_tensor(mip) else to_device(mip, sdt)
-        self.volume = abi.SDFVolume(ptr(self.sdf), ptr(self.mip), self.res, self.mip_res, sdf_storage, 0)
+        if layout == abi.LAYOUT_BRICK4:
+            # the engine would keep its SDF in this layout; here it is converted once
+            C_ = int(sdf_data.cascadesCount)
+            bricked = torch.empty_like(self.sdf)
+            check(lib().luxgi_sdf_to_bricks(ptr(self.sdf), self.res * C_, self.res, self.res, ptr(bricked),
+                                            stream_handle()), "luxgi_sdf_to_bricks")
+            self.sdf = bricked
+        self.volume = abi.SDFVolume(ptr(self.sdf), ptr(self.mip), self.res, self.mip_res, sdf_storage, layout)
         self.atlas_host = atlas
         self.objects = to_device(atlas.objects)
         self.tiles = to_device(atlas.tiles)
@@ -66,8 +74,7 @@ class DeviceScene:
                                       atlas.color_storage)
         self.sky_faces = None if sky_faces is None else to_device(sky_faces)
         self.skybox = abi.Skybox(ptr(self.sky_faces), sky_size, 0)
-        self.workspace = torch.empty(lib().luxgi_ddgi_trace_workspace_size(atlas.n_objects), dtype=torch.uint8,
-                                     device="cuda")
+        self.n_objects = atlas.n_objects
 
 
 def default_trace_params(rotation, sky=(0.6, 0.7, 0.9), max_steps=250, hit_eps=0.0, frame=0):
@@ -109,15 +116,19 @@ class DDGIVolume:
         self.states = torch.zeros(P, dtype=torch.uint8, device="cuda")
         ws = lib().luxgi_ddgi_probe_update_workspace_size(C.byref(uniform))
         self.workspace = torch.empty(max(ws, 256), dtype=torch.uint8, device="cuda")
+        self.trace_workspace = None
         self.frames = 0
         self.ping_pong = 0
 
     # trace_rays::system (DDGIRenderer.cpp:217-330)
     def trace_rays(self, scene, params, use_offsets=False, counters=None):
         off = ptr(self.offsets) if use_offsets else None
+        need = lib().luxgi_ddgi_trace_workspace_size(C.byref(self.u), scene.n_objects)
+        if self.trace_workspace is None or self.trace_workspace.numel() < need:
+            self.trace_workspace = torch.empty(need, dtype=torch.uint8, device="cuda")
         args = [C.byref(self.u), C.byref(scene.sdf_data), C.byref(scene.volume), C.byref(scene.atlas),
                 C.byref(scene.skybox), C.byref(params), off, C.byref(self.rays), ptr(self.ray_flags),
-                ptr(scene.workspace), scene.workspace.numel()]
+                ptr(self.trace_workspace), self.trace_workspace.numel()]
         if counters is None:
             check(lib().luxgi_ddgi_trace_sdf(*args, stream_handle()), "luxgi_ddgi_trace_sdf")
         else:

@@ -84,6 +84,16 @@ def threads():
     return max(1, lib().oracle_hardware_threads())
 
 
+def sample3d(vol, u, v, w):
+    """texture(sampler3D, (u, v, w)) with linear filtering and clamp-to-edge on a [D][H][W] volume."""
+    L = lib()
+    L.oracle_sample3d.argtypes = [P, I, I, I, C.c_double, C.c_double, C.c_double]
+    L.oracle_sample3d.restype = C.c_double
+    vol = _f32(vol)
+    D, H, W = vol.shape
+    return L.oracle_sample3d(_p(vol), W, H, D, float(u), float(v), float(w))
+
+
 # ---------------------------------------------------------------------------
 # DDGI integer layer
 # ---------------------------------------------------------------------------
@@ -183,6 +193,8 @@ def border_update(atlas, side, px, py, pz, precision="f64"):
 # SDF mip and trace
 # ---------------------------------------------------------------------------
 def sdf_mip(sdf, res, cascades=1, cascade=0, max_distance=2.0, flood_passes=4, mip=None):
+    if flood_passes % 2:
+        raise ValueError("flood passes alternate Mip->Tmp->Mip; an odd count leaves the result in Tmp")
     sdf = _f32(sdf)
     mr = (res + 3) // 4
     if mip is None:

@@ -294,6 +294,10 @@ void oracle_shadow_atrous_f64(const LuxgiShadowAtrousParams* pc, int W, int H, c
 
 int oracle_hardware_threads(void) { return (int)std::thread::hardware_concurrency(); }
 
+double oracle_sample3d(const float* vol, int W, int H, int D, double u, double v, double w) {
+    return sample3d<double>(vol, W, H, D, u, v, w);
+}
+
 }  // extern "C"
 
 extern "C" void oracle_relocate(const LuxgiDDGIUniform* u, const float* dirDist, const uint8_t* flags, int width,

@@ -77,7 +77,7 @@ def main():
     atlas = scene.build_atlas(prims, resolution=2048, tile=16, color_storage=abi.STORAGE_UNORM8, seed=3,
                               threshold=1.05 * 2.0 / res)
     log("[bench] atlas 2048^2 built", q)
-    ds = DeviceScene(g, sdf, mip, atlas)
+    ds = DeviceScene(g, sdf, mip, atlas, layout=abi.LAYOUT_BRICK4)
     counts = (32, 16, 32)
     step = [2.0 / (c - 1) for c in counts]
     u = abi.make_uniform(counts, (-1.0, -1.0, -1.0), step, 256, irr_oct=8, depth_oct=16, hysteresis=0.98)

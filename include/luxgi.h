@@ -141,6 +141,9 @@ typedef struct LuxgiSDFVolume {
     int32_t     mipResolution;
     int32_t     storage;  /* LuxgiStorage: FP32 or FP16 */
     int32_t     layout;   /* LuxgiLayout of `sdf` (the mip is always linear) */
+    const float* guard;   /* optional [D/4][H/4][W/4] lower bounds of `sdf` per 4x4x4 brick and its
+                             +1 apron (luxgi_sdf_build_guard); lets the march skip fine lookups whose
+                             value the reference would not use. NULL = no skipping. */
 } LuxgiSDFVolume;
 
 /* Memory layout of the fine SDF volume. */
@@ -310,6 +313,13 @@ int luxgi_sdf_build_mip(const float* sdf, int32_t resolution, int32_t cascades, 
  * (no counterpart in the reference, whose Vulkan 3D image is tiled by the driver).
  * width, height, depth multiples of 4; dst holds width*height*depth floats. */
 int luxgi_sdf_to_bricks(const float* src, int32_t width, int32_t height, int32_t depth, float* dst, void* stream);
+
+/* Per-brick lower bounds for LuxgiSDFVolume.guard: guard[bz][by][bx] = min of the linear FP32
+ * volume `sdf` ([depth][height][width], dims multiples of 4) over voxels [4b, 4b+4] on each axis
+ * (clamped), i.e. every trilinear footprint whose lower corner lies in brick b. No counterpart in
+ * the reference; the trace results are identical with and without it. */
+int luxgi_sdf_build_guard(const float* sdf, int32_t width, int32_t height, int32_t depth, float* guard,
+                          void* stream);
 
 /* ------------------------------------------------------------------------- */
 /* SVGF (reflection = RGB, shadow = scalar)                                   */

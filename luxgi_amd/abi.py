@@ -86,7 +86,7 @@ LAYOUT_BRICK4 = 1
 
 class SDFVolume(C.Structure):
     _fields_ = [("sdf", C.c_void_p), ("mip", C.c_void_p), ("resolution", C.c_int32),
-                ("mipResolution", C.c_int32), ("storage", C.c_int32), ("layout", C.c_int32)]
+                ("mipResolution", C.c_int32), ("storage", C.c_int32), ("layout", C.c_int32), ("guard", C.c_void_p)]
 
 
 class SurfaceAtlas(C.Structure):

@@ -244,6 +244,7 @@ __device__ __forceinline__ float fine_lookup(const SS* __restrict__ sdf, int W, 
 
 template <class SS, bool COUNT, bool BRICK>
 __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __restrict__ sdf, const SS* __restrict__ mip,
+                                                    const float* __restrict__ guard,
                                                     const float* __restrict__ probeOffsets, float4* __restrict__ recA,
                                                     float2* __restrict__ recB, unsigned long long* __restrict__ counters) {
     const int Rp = (A.R + 63) & ~63;
@@ -288,14 +289,26 @@ __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __res
                 const float uy = clampf((p.y - cpos.y) / cmax + 0.5f, 0.0f, 1.0f);
                 const float uz = clampf((p.z - cpos.z) / cmax + 0.5f, 0.0f, 1.0f);
                 const float tx = (float(cas) + ux) / float(C);
-                // both lookups are issued together; the fine value is used only where
-                // the reference samples it (mip distance below the chunk size)
+                // The fine sample is used only where the mip distance is below the chunk size
+                // and the fine distance below twice the chunk margin (SDFCommon.glsl:150-162).
+                // A brick guard >= that margin proves the second test fails for every voxel of
+                // the footprint (trilinear filtering is a convex combination), so the fine
+                // lookup is skipped; the factor 1 + 1e-5 covers fp32 rounding of the filter.
+                bool needFine = true;
+                if (guard) {
+                    const int gx = clampi(int(floorf(tx * float(texW) - 0.5f)), 0, texW - 1);
+                    const int gy = clampi(int(floorf(uy * float(A.res) - 0.5f)), 0, A.res - 1);
+                    const int gz = clampi(int(floorf(uz * float(A.res) - 0.5f)), 0, A.res - 1);
+                    const float gv = guard[((size_t)(gz >> 2) * (A.res >> 2) + (gy >> 2)) * (texW >> 2) + (gx >> 2)];
+                    needFine = !(gv >= chunkMarginDistance * 2.0f * (1.0f + 1e-5f));
+                }
                 float sd = sample_volume<SS>(mip, mipW, A.mipRes, A.mipRes, tx, uy, uz);
-                const float st = fine_lookup<SS, BRICK>(sdf, texW, A.res, A.res, tx, uy, uz);
+                float st = 1.0f;
+                if (needFine) st = fine_lookup<SS, BRICK>(sdf, texW, A.res, A.res, tx, uy, uz);
                 if (COUNT) cMip++;
                 if (sd < chunkSizeDistance) {
                     if (COUNT) cFine++;
-                    if (st < chunkMarginDistance * 2.0f) sd = st;
+                    if (needFine && st < chunkMarginDistance * 2.0f) sd = st;
                 } else {
                     sd = chunkSizeDistance;
                 }
@@ -457,12 +470,12 @@ static int launch_trace(const TraceArgs& A, const LuxgiSDFVolume* vol, const Lux
     const long long lanes = (long long)A.P * ((A.R + 63) & ~63);
     if (A.fineLayout == LUXGI_LAYOUT_BRICK4)
         hipLaunchKernelGGL((march_kernel<SS, COUNT, true>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st, A,
-                           static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), probeOffsets, w.recA,
-                           w.recB, counters);
+                           static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), vol->guard, probeOffsets,
+                           w.recA, w.recB, counters);
     else
         hipLaunchKernelGGL((march_kernel<SS, COUNT, false>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st, A,
-                           static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), probeOffsets, w.recA,
-                           w.recB, counters);
+                           static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), vol->guard, probeOffsets,
+                           w.recA, w.recB, counters);
     LUXGI_LAUNCH_CHECK();
     hipLaunchKernelGGL((shade_kernel<SS, SC, SR, COUNT>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st, A,
                        static_cast<const SS*>(vol->sdf), at->chunks, at->culledObjects, at->objects, w.inv, at->tiles,
@@ -515,6 +528,8 @@ static int trace_entry(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, c
     A.hitEps = p->hitEpsilon;
     A.fineLayout = vol->layout;
     if (vol->layout != LUXGI_LAYOUT_LINEAR && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (vol->guard && ((vol->resolution & 3) || ((vol->resolution * (int)g->cascadesCount) & 3)))
+        return LUXGI_ERR_INVALID_ARGUMENT;
     if (vol->layout == LUXGI_LAYOUT_BRICK4 &&
         (vol->storage != LUXGI_STORAGE_FP32 || (vol->resolution & 3) || ((vol->resolution * (int)g->cascadesCount) & 3)))
         return LUXGI_ERR_INVALID_ARGUMENT;

@@ -19,6 +19,8 @@
 // the 4n+4 border texels are written from there.
 #include "common.hpp"
 
+#include <algorithm>
+
 namespace luxgi {
 
 constexpr int kProbesPerBlock = 4;
@@ -117,26 +119,16 @@ __device__ __forceinline__ void accumulate(const UpdateLds& lds, int g, int t, i
     }
 }
 
+// one group of kProbesPerBlock probes starting at p0 (the body of probe_update_kernel)
 template <class SR, class SA>
-__global__ __launch_bounds__(256) void probe_update_kernel(UpdateArgs A, const SR* __restrict__ radiance,
-                                                           const SR* __restrict__ dirDist,
-                                                           const SA* __restrict__ prevIrr,
-                                                           const SA* __restrict__ prevDepth, SA* __restrict__ outIrr,
-                                                           SA* __restrict__ outDepth, const float* __restrict__ Wi,
-                                                           const float* __restrict__ Si, const float* __restrict__ Wd,
-                                                           const float* __restrict__ Sd,
-                                                           const uint8_t* __restrict__ onlyFlagged) {
-    __shared__ UpdateLds lds;
-    __shared__ int sShared[kProbesPerBlock];
+__device__ __forceinline__ void update_group(const UpdateArgs& A, int p0, const SR* __restrict__ radiance,
+                                             const SR* __restrict__ dirDist, const SA* __restrict__ prevIrr,
+                                             const SA* __restrict__ prevDepth, SA* __restrict__ outIrr,
+                                             SA* __restrict__ outDepth, const float* __restrict__ Wi,
+                                             const float* __restrict__ Wd, const uint8_t* __restrict__ onlyFlagged,
+                                             UpdateLds& lds, int* sShared) {
     const int tid = threadIdx.x;
-    const int p0 = blockIdx.x * kProbesPerBlock;
-    // fix-up mode: only the probes flagged by the MFMA pass are recomputed
     auto wanted = [&](int p) { return p < A.P && (!onlyFlagged || onlyFlagged[p] != 0); };
-    if (onlyFlagged) {
-        bool any = false;
-        for (int g = 0; g < kProbesPerBlock; ++g) any |= wanted(p0 + g);
-        if (!any) return;
-    }
     const int Ti = A.ni * A.ni, Td = A.nd * A.nd;
     const int pxy = A.px * A.py;
 
@@ -288,6 +280,32 @@ __global__ __launch_bounds__(256) void probe_update_kernel(UpdateArgs A, const S
     }
 }
 
+// Grid-stride over groups of kProbesPerBlock probes. Fix-up mode (onlyFlagged != null):
+// only groups holding a flagged probe do any work, and only flagged probes are written.
+template <class SR, class SA>
+__global__ __launch_bounds__(256) void probe_update_kernel(UpdateArgs A, const SR* __restrict__ radiance,
+                                                           const SR* __restrict__ dirDist,
+                                                           const SA* __restrict__ prevIrr,
+                                                           const SA* __restrict__ prevDepth, SA* __restrict__ outIrr,
+                                                           SA* __restrict__ outDepth, const float* __restrict__ Wi,
+                                                           const float* __restrict__ Wd,
+                                                           const uint8_t* __restrict__ onlyFlagged) {
+    __shared__ UpdateLds lds;
+    __shared__ int sShared[kProbesPerBlock];
+    const int groups = (A.P + kProbesPerBlock - 1) / kProbesPerBlock;
+    for (int grp = blockIdx.x; grp < groups; grp += gridDim.x) {
+        const int p0 = grp * kProbesPerBlock;
+        if (onlyFlagged) {
+            bool any = false;
+            for (int g = 0; g < kProbesPerBlock; ++g) any |= p0 + g < A.P && onlyFlagged[p0 + g] != 0;
+            if (!any) continue;  // uniform across the block
+        }
+        update_group<SR, SA>(A, p0, radiance, dirDist, prevIrr, prevDepth, outIrr, outDepth, Wi, Wd, onlyFlagged,
+                             lds, sShared);
+        __syncthreads();  // the next group reuses the LDS
+    }
+}
+
 // Stand-alone border copy, in place (BorderUpdate.glsl:136-156).
 template <class SA, int CH>
 __global__ void border_kernel(SA* __restrict__ atlas, int W, int n, int pxy, int P) {
@@ -329,31 +347,38 @@ __global__ void border_kernel(SA* __restrict__ atlas, int W, int n, int pxy, int
 // ---------------------------------------------------------------------------
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr int kMG = 16;    // probes per block
-constexpr int kKC = 32;    // rays per staged chunk
+constexpr int kKC = 16;    // rays per staged chunk (4 MFMA k-steps)
 constexpr int kBStride = 48;  // LDS row stride of the B tiles (== 16 mod 32: conflict-free reads)
 
-__global__ void weights_kernel_tiled(const float* __restrict__ dirs0, int R, int n, int tpad, int depth,
-                                     float sharpness, float* __restrict__ W) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= R * tpad) return;
-    const int r = idx / tpad, t = idx % tpad;
-    float w = 0.0f;
-    if (t < n * n) {
-        const f3 td = texel_direction(t % n, t / n, n);
-        w = fmaxf(0.0f, dot3(td, mk(dirs0[r * 4 + 0], dirs0[r * 4 + 1], dirs0[r * 4 + 2])));
-        if (depth) w = powf(w, sharpness);
-        if (!(w >= kFltEps)) w = 0.0f;
-    }
-    W[idx] = w;
-}
-
-// S[t] = sum_r W[r][t] in ray order (ProbeUpdate.glsl:100 accumulates totalWeight in that order)
-__global__ void weight_sums_kernel(const float* __restrict__ W, int R, int tpad, float* __restrict__ S) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= tpad) return;
+// W[r][t] for one map (16-row padded, zero rows beyond T, thresholded as ProbeUpdate.glsl:93)
+// and S[t] = sum_r W[r][t]: one block per 16 texels, 16 ray lanes per texel, partial sums
+// combined in a fixed order.
+__global__ __launch_bounds__(256) void weights_kernel_tiled(const float* __restrict__ dirs0, int R, int n, int tpad,
+                                                            int depth, float sharpness, float* __restrict__ W,
+                                                            float* __restrict__ S) {
+    __shared__ float part[16][17];
+    const int tl = threadIdx.x & 15, j = threadIdx.x >> 4;
+    const int t = blockIdx.x * 16 + tl;
+    const bool valid = t < n * n;
+    const f3 td = valid ? texel_direction(t % n, t / n, n) : mk(0.f, 0.f, 1.f);
     float s = 0.0f;
-    for (int r = 0; r < R; ++r) s += W[(size_t)r * tpad + t];
-    S[t] = s;
+    for (int r = j; r < R; r += 16) {
+        float w = 0.0f;
+        if (valid) {
+            w = fmaxf(0.0f, dot3(td, mk(dirs0[r * 4 + 0], dirs0[r * 4 + 1], dirs0[r * 4 + 2])));
+            if (depth) w = powf(w, sharpness);
+            if (!(w >= kFltEps)) w = 0.0f;
+        }
+        W[(size_t)r * tpad + t] = w;
+        s += w;
+    }
+    part[j][tl] = s;
+    __syncthreads();
+    if (j == 0) {
+        float tot = 0.0f;
+        for (int k = 0; k < 16; ++k) tot += part[k][tl];
+        S[t] = tot;
+    }
 }
 
 // directions of ray row 0 as FP32 (the values every probe's rays are compared with)
@@ -373,7 +398,12 @@ __device__ __forceinline__ void border_source(int bx, int by, int n, int& sx, in
     else { sx = bx == 0 ? n : 1; sy = by == 0 ? n : 1; }
 }
 
+// One pipeline stage: W and B tiles of a 16-ray chunk. Row strides are == 16 (mod 32)
+// so the 4 row groups of an MFMA operand read hit disjoint LDS banks.
+constexpr int kWdStride = 256 + 16, kWiStride = 64 + 16;
 struct MfmaStage {
+    float wd[kKC][kWdStride];
+    float wi[kKC][kWiStride];
     float bd[kKC][kBStride];
     float bi[kKC][kBStride];
 };
@@ -382,9 +412,76 @@ struct MfmaOut {
     float4 irr[kMG][64];
 };
 union MfmaLds {
-    MfmaStage st;
+    MfmaStage st[2];
     MfmaOut out;
 };
+
+// registers holding the next chunk while the current one is multiplied
+struct MfmaPrefetch {
+    float4 wd[4];
+    float4 wi;
+    float4 dd, rc;
+};
+
+template <class SR>
+__device__ __forceinline__ void mfma_load_chunk(const UpdateArgs& A, int r0, int p0, int tpadI, int tpadD,
+                                                const SR* __restrict__ radiance, const SR* __restrict__ dirDist,
+                                                const float* __restrict__ Wi, const float* __restrict__ Wd,
+                                                MfmaPrefetch& f) {
+    const int tid = threadIdx.x;
+    const int qd = tpadD / 4, qi = tpadI / 4;  // float4 per W row
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = tid + k * 256;
+        const int rr = i / max(qd, 1), c4 = i % max(qd, 1);
+        const int r = r0 + rr;
+        f.wd[k] = (qd && rr < kKC && r < A.R) ? *reinterpret_cast<const float4*>(Wd + (size_t)r * tpadD + c4 * 4)
+                                             : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    {
+        const int rr = tid / max(qi, 1), c4 = tid % max(qi, 1);
+        const int r = r0 + rr;
+        f.wi = (qi && rr < kKC && r < A.R) ? *reinterpret_cast<const float4*>(Wi + (size_t)r * tpadI + c4 * 4)
+                                          : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    {
+        const int g = tid / kKC, rr = tid % kKC, r = r0 + rr, p = p0 + g;
+        f.dd = make_float4(0.f, 0.f, 0.f, 0.f);
+        f.rc = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (p < A.P && r < A.R) {
+            const size_t e = (size_t)p * A.rayWidth + r;
+            f.dd = Texel<SR>::ld4(dirDist, e);
+            if (A.doIrr) f.rc = Texel<SR>::ld4(radiance, e);
+        }
+    }
+}
+
+__device__ __forceinline__ void mfma_store_chunk(const UpdateArgs& A, int r0, int p0, int tpadI, int tpadD,
+                                                 const float* __restrict__ dirs0, const MfmaPrefetch& f,
+                                                 MfmaStage& st, int* sMis) {
+    const int tid = threadIdx.x;
+    const int qd = tpadD / 4, qi = tpadI / 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = tid + k * 256;
+        if (qd && i < kKC * qd) *reinterpret_cast<float4*>(&st.wd[i / qd][(i % qd) * 4]) = f.wd[k];
+    }
+    if (qi && tid < kKC * qi) *reinterpret_cast<float4*>(&st.wi[tid / qi][(tid % qi) * 4]) = f.wi;
+    const int g = tid / kKC, rr = tid % kKC, r = r0 + rr, p = p0 + g;
+    float d0 = 0.f, d1 = 0.f;
+    if (p < A.P && r < A.R) {
+        float dist = fminf(A.maxDistance, f.dd.w - 0.01f);  // ProbeUpdate.glsl:75-78
+        if (dist == -1.0f) dist = A.maxDistance;
+        d0 = dist;
+        d1 = dist * dist;
+        if (f.dd.x != dirs0[r * 4 + 0] || f.dd.y != dirs0[r * 4 + 1] || f.dd.z != dirs0[r * 4 + 2]) sMis[g] = 1;
+    }
+    st.bd[rr][g * 2 + 0] = d0;
+    st.bd[rr][g * 2 + 1] = d1;
+    st.bi[rr][g * 3 + 0] = f.rc.x;
+    st.bi[rr][g * 3 + 1] = f.rc.y;
+    st.bi[rr][g * 3 + 2] = f.rc.z;
+}
 
 template <class SR, class SA>
 __global__ __launch_bounds__(256) void probe_update_mfma_kernel(
@@ -397,6 +494,7 @@ __global__ __launch_bounds__(256) void probe_update_mfma_kernel(
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int p0 = blockIdx.x * kMG;
     const int MtD = A.doDepth ? tpadD / 16 : 0, MtI = A.doIrr ? tpadI / 16 : 0;
+    const int tpI = A.doIrr ? tpadI : 0, tpD = A.doDepth ? tpadD : 0;
     if (tid < kMG) sMis[tid] = 0;
     floatx4 accD[4][2], accI[3];
 #pragma unroll
@@ -404,62 +502,45 @@ __global__ __launch_bounds__(256) void probe_update_mfma_kernel(
 #pragma unroll
     for (int j = 0; j < 3; ++j) accI[j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    for (int r0 = 0; r0 < A.R; r0 += kKC) {
-        __syncthreads();
-        for (int i = tid; i < kMG * kKC; i += 256) {
-            const int g = i / kKC, rr = i % kKC, r = r0 + rr, p = p0 + g;
-            float d0 = 0.f, d1 = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
-            if (p < A.P && r < A.R) {
-                const size_t e = (size_t)p * A.rayWidth + r;
-                const float4 dd = Texel<SR>::ld4(dirDist, e);
-                float dist = fminf(A.maxDistance, dd.w - 0.01f);  // ProbeUpdate.glsl:75-78
-                if (dist == -1.0f) dist = A.maxDistance;
-                d0 = dist;
-                d1 = dist * dist;
-                if (A.doIrr) {
-                    const float4 rc = Texel<SR>::ld4(radiance, e);
-                    c0 = rc.x;
-                    c1 = rc.y;
-                    c2 = rc.z;
-                }
-                if (dd.x != dirs0[r * 4 + 0] || dd.y != dirs0[r * 4 + 1] || dd.z != dirs0[r * 4 + 2]) sMis[g] = 1;
-            }
-            lds.st.bd[rr][g * 2 + 0] = d0;
-            lds.st.bd[rr][g * 2 + 1] = d1;
-            lds.st.bi[rr][g * 3 + 0] = c0;
-            lds.st.bi[rr][g * 3 + 1] = c1;
-            lds.st.bi[rr][g * 3 + 2] = c2;
-        }
-        __syncthreads();
-        const int col = lane & 15;
+    const int nChunks = (A.R + kKC - 1) / kKC;
+    MfmaPrefetch f;
+    mfma_load_chunk<SR>(A, 0, p0, tpI, tpD, radiance, dirDist, Wi, Wd, f);
+    __syncthreads();  // sMis initialised
+    mfma_store_chunk(A, 0, p0, tpI, tpD, dirs0, f, lds.st[0], sMis);
+    __syncthreads();
+    const int col = lane & 15;
+    for (int c = 0; c < nChunks; ++c) {
+        const bool more = c + 1 < nChunks;
+        if (more) mfma_load_chunk<SR>(A, (c + 1) * kKC, p0, tpI, tpD, radiance, dirDist, Wi, Wd, f);
+        const MfmaStage& st = lds.st[c & 1];
 #pragma unroll
         for (int ks = 0; ks < kKC / 4; ++ks) {
             const int k = ks * 4 + (lane >> 4);
-            const int r = r0 + k;
-            const bool rv = r < A.R;
             if (MtD) {
-                const float b0 = lds.st.bd[k][col], b1 = lds.st.bd[k][16 + col];
+                const float b0 = st.bd[k][col], b1 = st.bd[k][16 + col];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const int m = wave + 4 * j;
                     if (m < MtD) {
-                        const float a = rv ? Wd[(size_t)r * tpadD + m * 16 + col] : 0.0f;
+                        const float a = st.wd[k][m * 16 + col];
                         accD[j][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, accD[j][0], 0, 0, 0);
                         accD[j][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, accD[j][1], 0, 0, 0);
                     }
                 }
             }
             if (wave < MtI) {
-                const float a = rv ? Wi[(size_t)r * tpadI + wave * 16 + col] : 0.0f;
-                accI[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, lds.st.bi[k][col], accI[0], 0, 0, 0);
-                accI[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, lds.st.bi[k][16 + col], accI[1], 0, 0, 0);
-                accI[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, lds.st.bi[k][32 + col], accI[2], 0, 0, 0);
+                const float a = st.wi[k][wave * 16 + col];
+                accI[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, st.bi[k][col], accI[0], 0, 0, 0);
+                accI[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, st.bi[k][16 + col], accI[1], 0, 0, 0);
+                accI[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, st.bi[k][32 + col], accI[2], 0, 0, 0);
             }
         }
+        if (more) mfma_store_chunk(A, (c + 1) * kKC, p0, tpI, tpD, dirs0, f, lds.st[(c + 1) & 1], sMis);
+        __syncthreads();
     }
-    __syncthreads();  // the LDS now holds the finished interiors
+    // the LDS now holds the finished interiors
     const int Td = A.nd * A.nd, Ti = A.ni * A.ni;
-    const int col = lane & 15, rowq = (lane >> 4) * 4;
+    const int rowq = (lane >> 4) * 4;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int m = wave + 4 * j;
@@ -618,15 +699,13 @@ static int launch_update(const UpdateArgs& A, const LuxgiRayBuffers* rays, const
         hipLaunchKernelGGL(row0_kernel<SR>, dim3((A.R + 255) / 256), dim3(256), 0, st, dd, A.R, w.dirs0);
         LUXGI_LAUNCH_CHECK();
         if (A.doIrr) {
-            hipLaunchKernelGGL(weights_kernel_tiled, dim3((A.R * tpI + 255) / 256), dim3(256), 0, st, w.dirs0, A.R, A.ni,
-                               tpI, 0, A.sharpness, w.Wi);
-            hipLaunchKernelGGL(weight_sums_kernel, dim3((tpI + 63) / 64), dim3(64), 0, st, w.Wi, A.R, tpI, w.Si);
+            hipLaunchKernelGGL(weights_kernel_tiled, dim3(tpI / 16), dim3(256), 0, st, w.dirs0, A.R, A.ni, tpI, 0,
+                               A.sharpness, w.Wi, w.Si);
             LUXGI_LAUNCH_CHECK();
         }
         if (A.doDepth) {
-            hipLaunchKernelGGL(weights_kernel_tiled, dim3((A.R * tpD + 255) / 256), dim3(256), 0, st, w.dirs0, A.R, A.nd,
-                               tpD, 1, A.sharpness, w.Wd);
-            hipLaunchKernelGGL(weight_sums_kernel, dim3((tpD + 63) / 64), dim3(64), 0, st, w.Wd, A.R, tpD, w.Sd);
+            hipLaunchKernelGGL(weights_kernel_tiled, dim3(tpD / 16), dim3(256), 0, st, w.dirs0, A.R, A.nd, tpD, 1,
+                               A.sharpness, w.Wd, w.Sd);
             LUXGI_LAUNCH_CHECK();
         }
         hipLaunchKernelGGL((probe_update_mfma_kernel<SR, SA>), dim3((A.P + kMG - 1) / kMG), dim3(256), 0, st, A, rad, dd,
@@ -635,8 +714,9 @@ static int launch_update(const UpdateArgs& A, const LuxgiRayBuffers* rays, const
         // probes whose directions differ from row 0: reference arithmetic with their own directions
         UpdateArgs F = A;
         F.forcePerProbe = 1;
-        hipLaunchKernelGGL((probe_update_kernel<SR, SA>), dim3((A.P + kProbesPerBlock - 1) / kProbesPerBlock), dim3(256),
-                           0, st, F, rad, dd, pI, pD, oI, oD, nullptr, nullptr, nullptr, nullptr, w.mismatch);
+        const int groups = (A.P + kProbesPerBlock - 1) / kProbesPerBlock;
+        hipLaunchKernelGGL((probe_update_kernel<SR, SA>), dim3(std::min(groups, 512)), dim3(256), 0, st, F, rad, dd, pI,
+                           pD, oI, oD, nullptr, nullptr, w.mismatch);
         LUXGI_LAUNCH_CHECK();
         return LUXGI_OK;
     }
@@ -655,7 +735,7 @@ static int launch_update(const UpdateArgs& A, const LuxgiRayBuffers* rays, const
         }
     }
     hipLaunchKernelGGL((probe_update_kernel<SR, SA>), dim3((A.P + kProbesPerBlock - 1) / kProbesPerBlock), dim3(256), 0,
-                       st, A, rad, dd, pI, pD, oI, oD, w.Wi, w.Si, w.Wd, w.Sd, nullptr);
+                       st, A, rad, dd, pI, pD, oI, oD, w.Wi, w.Wd, nullptr);
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }

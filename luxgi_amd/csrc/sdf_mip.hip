@@ -45,9 +45,29 @@ __global__ void to_bricks_kernel(const float* __restrict__ src, int W, int H, in
     dst[brick4_index(x, y, z, W >> 2, H >> 2)] = src[((size_t)z * H + y) * W + x];
 }
 
+__global__ void guard_kernel(const float* __restrict__ src, int W, int H, int D, float* __restrict__ guard) {
+    const int nbx = W >> 2, nby = H >> 2, nbz = D >> 2;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbx * nby * nbz) return;
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    float m = INFINITY;
+    for (int z = 4 * bz; z <= min(4 * bz + 4, D - 1); ++z)
+        for (int y = 4 * by; y <= min(4 * by + 4, H - 1); ++y)
+            for (int x = 4 * bx; x <= min(4 * bx + 4, W - 1); ++x) m = fminf(m, src[((size_t)z * H + y) * W + x]);
+    guard[b] = m;
+}
+
 }  // namespace luxgi
 
 using namespace luxgi;
+
+extern "C" int luxgi_sdf_build_guard(const float* sdf, int32_t W, int32_t H, int32_t D, float* guard, void* stream) {
+    if (!sdf || !guard || W <= 0 || H <= 0 || D <= 0 || (W & 3) || (H & 3) || (D & 3)) return LUXGI_ERR_INVALID_ARGUMENT;
+    const int n = (W >> 2) * (H >> 2) * (D >> 2);
+    hipLaunchKernelGGL(guard_kernel, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream), sdf, W, H, D, guard);
+    LUXGI_LAUNCH_CHECK();
+    return LUXGI_OK;
+}
 
 extern "C" int luxgi_sdf_to_bricks(const float* src, int32_t W, int32_t H, int32_t D, float* dst, void* stream) {
     if (!src || !dst || W <= 0 || H <= 0 || D <= 0 || (W & 3) || (H & 3) || (D & 3)) return LUXGI_ERR_INVALID_ARGUMENT;

@@ -42,13 +42,19 @@ class DeviceScene:
     """SDF volume + surface atlas uploaded to the GPU (the inputs of the trace)."""
 
     def __init__(self, sdf_data, sdf, mip, atlas, sdf_storage=abi.STORAGE_FP32, sky_faces=None, sky_size=0,
-                 layout=abi.LAYOUT_LINEAR):
+                 layout=abi.LAYOUT_LINEAR, guard=False):
         self.sdf_data = sdf_data
         self.res = int(sdf_data.resolution)
         self.mip_res = (self.res + 3) // 4
         sdt = torch.float16 if sdf_storage == abi.STORAGE_FP16 else torch.float32
         self.sdf = sdf if torch.is_tensor(sdf) else to_device(sdf, sdt)
         self.mip = mip if torch.is_tensor(mip) else to_device(mip, sdt)
+        self.guard = None
+        if guard:
+            C_ = int(sdf_data.cascadesCount)
+            self.guard = torch.empty(((self.res // 4) ** 3) * C_, dtype=torch.float32, device="cuda")
+            check(lib().luxgi_sdf_build_guard(ptr(self.sdf), self.res * C_, self.res, self.res, ptr(self.guard),
+                                              stream_handle()), "luxgi_sdf_build_guard")
         if layout == abi.LAYOUT_BRICK4:
             # the engine would keep its SDF in this layout; here it is converted once
             C_ = int(sdf_data.cascadesCount)
@@ -56,7 +62,8 @@ class DeviceScene:
             check(lib().luxgi_sdf_to_bricks(ptr(self.sdf), self.res * C_, self.res, self.res, ptr(bricked),
                                             stream_handle()), "luxgi_sdf_to_bricks")
             self.sdf = bricked
-        self.volume = abi.SDFVolume(ptr(self.sdf), ptr(self.mip), self.res, self.mip_res, sdf_storage, layout)
+        self.volume = abi.SDFVolume(ptr(self.sdf), ptr(self.mip), self.res, self.mip_res, sdf_storage, layout,
+                                    ptr(self.guard))
         self.atlas_host = atlas
         self.objects = to_device(atlas.objects)
         self.tiles = to_device(atlas.tiles)

@@ -116,20 +116,23 @@ def test_trace_counts_match_oracle():
 
 
 def test_trace_brick_layout_is_bitwise_identical():
-    """LUXGI_LAYOUT_BRICK4 only changes where voxels live: outputs equal the linear layout's bit for bit."""
+    """LUXGI_LAYOUT_BRICK4 only changes where voxels live and the brick guard only skips fine
+    lookups whose value is not used: outputs equal the plain linear trace's bit for bit."""
     from luxgi_amd.ddgi import DDGIVolume, DeviceScene
     prims, sdf, mip, g, atlas = _scenes.small_scene()
     u = _scenes.small_uniform(rays=128, counts=(6, 5, 4))
     p = _scenes.trace_params(rot_seed=21)
     outs = []
-    for layout in (abi.LAYOUT_LINEAR, abi.LAYOUT_BRICK4):
-        ds = DeviceScene(g, sdf, mip, atlas, layout=layout)
+    for layout, guard in ((abi.LAYOUT_LINEAR, False), (abi.LAYOUT_BRICK4, False), (abi.LAYOUT_BRICK4, True),
+                          (abi.LAYOUT_LINEAR, True)):
+        ds = DeviceScene(g, sdf, mip, atlas, layout=layout, guard=guard)
         vol = DDGIVolume(u, flags_buffer=True)
         vol.trace_rays(ds, p)
         torch.cuda.synchronize()
         outs.append((vol.radiance.cpu().numpy(), vol.dir_dist.cpu().numpy(), vol.ray_flags.cpu().numpy()))
-    for a, b in zip(*outs):
-        assert np.array_equal(a, b)
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)
 
 
 def test_trace_fp16_rays():

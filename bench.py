@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--svgf-iterations", type=int, default=5)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the run, launch a plain copy kernel (256 MiB read + 256 MiB written, 3 times) "
+                         "whose rocprofv3 FETCH_SIZE / WRITE_SIZE scale the trace counters to bytes")
     return ap.parse_args()
 
 
@@ -146,6 +149,14 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "trace_hbm_traffic.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     # ---------------- CPU baseline (bounded sample, FP32 transcription) ----------------
     cpu = None

@@ -211,8 +211,8 @@ typedef struct LuxgiProbeAtlas {
 int luxgi_ddgi_fill_atlas_sizes(LuxgiDDGIUniform* uniform);
 
 /* Workspace bytes needed by luxgi_ddgi_trace_sdf for the probe grid of `uniform`
- * and `objectCapacity` atlas objects (queue counter, per-object inverses, per-ray
- * hit records). */
+ * and `objectCapacity` atlas objects (per-object inverse transforms and the
+ * 24-byte per-ray march records that the shading kernel consumes). */
 size_t luxgi_ddgi_trace_workspace_size(const LuxgiDDGIUniform* uniform, int32_t objectCapacity);
 
 /* GISDFRays.comp (Shaders/DDGI/GISDFRays.comp:63-128): probe ray generation,

@@ -8,6 +8,9 @@ Workload (one "step" = one frame of the hot path on resident inputs):
     the border copy fused into the same pass.
   * SVGF reflection denoise at 1920x1080 on a G-buffer from SDF primary rays:
     reprojection + moments + 17x17 clip + tile classification, 5 a-trous passes.
+Reported beside the step (not in value): per-pixel probe sampling of the updated
+volume at 1920x1080 (probe_sample_ms; bytes = G-buffer depth + normal + RGBA32F
+output per pixel + both atlases once).
 value = rays / DDGI-update time (Grays/s). All timing uses HIP events on the
 stream the kernels are launched on (torch's current stream), W warm-up steps
 then exactly K timed steps between device synchronisations.
@@ -136,6 +139,21 @@ def main():
     log(f"[bench] timed {args.steps} steps: trace {t_trace.mean():.3f} ms, update {t_upd.mean():.3f} ms, "
         f"svgf {t_svgf.mean():.3f} ms, wall/step {ms_step:.3f} ms", q)
 
+    # ---------------- probe sampling at 1080p (sample_probe pass; reported beside the step) ----------------
+    pev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    irr_out = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    for _ in range(2):
+        vol.sample_probes(cur["depth"], cur["normals"], eye, VPI, out=irr_out)
+    pev[0].record()
+    for _ in range(args.steps):
+        vol.sample_probes(cur["depth"], cur["normals"], eye, VPI, out=irr_out)
+    pev[1].record()
+    torch.cuda.synchronize()
+    sample_ms = pev[0].elapsed_time(pev[1]) / args.steps
+    # G-buffer depth (4 B) + octahedral normal (8 B) + RGBA32F output (16 B) per pixel, atlases read once
+    atlas_bytes = sum(t.numel() * t.element_size() for t in (vol.current_irradiance, vol.current_depth))
+    sample_bytes = W * H * (4 + 8 + 16) + atlas_bytes
+
     # ---------------- counting run (algorithmic bytes of the trace) ----------------
     counters = torch.zeros(8, dtype=torch.int64, device="cuda")
     vol.trace_rays(ds, default_trace_params(rotations[-1], frame=vol.frames), counters=counters)
@@ -185,6 +203,8 @@ def main():
         "trace_ms": float(np.mean(t_trace)),
         "probe_update_ms": float(np.mean(t_upd)),
         "svgf_ms": float(np.mean(t_svgf)),
+        "probe_sample_ms": sample_ms,
+        "probe_sample_gbs": sample_bytes / (sample_ms * 1e-3) / 1e9,
         "trace_grays_per_s": rays_per_frame / trace_s / 1e9,
         "trace_counts": {"lookups": n_lookup, "normals": n_normal, "objects": c[3], "tile_gathers": n_tile,
                          "rays": n_rays, "steps": c[6], "hits": c[7]},

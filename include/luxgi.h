@@ -297,6 +297,28 @@ int luxgi_ddgi_relocate_classify(const LuxgiDDGIUniform* uniform,
                                  uint8_t* states,
                                  void* stream);
 
+/* SampleProbe.comp UniformBufferObject (Shaders/DDGI/SampleProbe.comp:29-33). */
+typedef struct LuxgiSampleProbeParams {
+    float cameraPosition[4];
+    float viewProjInv[16];   /* column-major inverse(projView) */
+} LuxgiSampleProbeParams;
+
+/* SampleProbe.comp:36-60 with sampleIrradiance (Shaders/DDGI/DDGICommon.glsl:163-233):
+ * per-pixel indirect irradiance from the 8 surrounding probes (Chebyshev depth
+ * visibility, smooth backface and trilinear weights), read from `atlas` with
+ * bilinear filtering and repeat addressing (the default sampler, Definitions.h:152-159).
+ * Dispatched at DDGIRenderer.cpp:467-503.
+ *   depth    R32F [height][width] NDC depth (1 = sky: output 0)
+ *   normals  RGBA32F [height][width], octahedral normal in .xy
+ *   out      RGBA [height][width] (irradiance, 1); outStorage FP32 or FP16 */
+int luxgi_ddgi_sample_probes(const LuxgiDDGIUniform* uniform,
+                             const LuxgiProbeAtlas* atlas,
+                             const LuxgiSampleProbeParams* params,
+                             int32_t width, int32_t height,
+                             const float* depth, const float* normals,
+                             int32_t outStorage, void* out,
+                             void* stream);
+
 /* ------------------------------------------------------------------------- */
 /* Global SDF coarse mip (GlobalSDFMipmap.comp:32-68 + fillFlood GlobalDistanceField.cpp:537-573) */
 /* ------------------------------------------------------------------------- */

@@ -25,6 +25,7 @@ SIGNATURES = {
     "luxgi_ddgi_probe_update": (C.c_int, [P, P, P, P, U, U, P, C.c_size_t, P]),
     "luxgi_ddgi_border_update": (C.c_int, [P, P, U, P]),
     "luxgi_ddgi_relocate_classify": (C.c_int, [P, P, P, P, P, P, P]),
+    "luxgi_ddgi_sample_probes": (C.c_int, [P, P, P, I, I, P, P, I, P, P]),
     "luxgi_sdf_build_mip": (C.c_int, [P, I, I, I, C.c_float, I, P, P, P]),
     "luxgi_sdf_to_bricks": (C.c_int, [P, I, I, I, P, P]),
     "luxgi_sdf_build_guard": (C.c_int, [P, I, I, I, P, P]),

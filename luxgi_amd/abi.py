@@ -115,6 +115,10 @@ class RelocateParams(C.Structure):
                 ("maxOffsetFraction", C.c_float), ("relocate", C.c_int32)]
 
 
+class SampleProbeParams(C.Structure):
+    _fields_ = [("cameraPosition", f4), ("viewProjInv", f16)]
+
+
 class GBuffer(C.Structure):
     _fields_ = [("color", C.c_void_p), ("normals", C.c_void_p), ("pbr", C.c_void_p), ("depth", C.c_void_p)]
 
@@ -141,7 +145,7 @@ class ShadowAtrousParams(C.Structure):
 
 _SIZES = {DDGIUniform: 96, GlobalSDFData: 96, SurfaceAtlasData: 32, AtlasObject: 128, AtlasTile: 96,
           TraceParams: 112, ReflectionReprojectParams: 168, ReflectionAtrousParams: 24,
-          ShadowReprojectParams: 80, ShadowAtrousParams: 32}
+          ShadowReprojectParams: 80, ShadowAtrousParams: 32, SampleProbeParams: 80}
 for _t, _n in _SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
 

@@ -59,6 +59,12 @@ __device__ __forceinline__ float4 mat4_mul4(const float* m, float4 v) {
                        m[3] * v.x + m[7] * v.y + m[11] * v.z + m[15] * v.w);
 }
 
+// worldPositionFromDepth, Common/Math.glsl:35-42
+__device__ __forceinline__ f3 world_from_depth(float u, float v, float depth, const float* m) {
+    const float4 w = mat4_mul4(m, make_float4(u * 2.0f - 1.0f, v * 2.0f - 1.0f, depth, 1.0f));
+    return mk(w.x / w.w, w.y / w.w, w.z / w.w);
+}
+
 // ---------------------------------------------------------------------------
 // texel storage: float (RGBA32F/RG32F/R32F) or __half (RGBA16F/RG16F/R16F)
 // ---------------------------------------------------------------------------
@@ -74,6 +80,7 @@ template <> struct Texel<float> {
     }
     static __device__ __forceinline__ void st2(float* p, size_t i, float2 v) { reinterpret_cast<float2*>(p)[i] = v; }
     static __device__ __forceinline__ float ld1(const float* p, size_t i) { return p[i]; }
+    static __device__ __forceinline__ float cvt(float v) { return v; }
     static __device__ __forceinline__ void st1(float* p, size_t i, float v) { p[i] = v; }
 };
 
@@ -98,6 +105,7 @@ template <> struct Texel<__half> {
         reinterpret_cast<__half2*>(p)[i] = __floats2half2_rn(v.x, v.y);
     }
     static __device__ __forceinline__ float ld1(const __half* p, size_t i) { return __half2float(p[i]); }
+    static __device__ __forceinline__ float cvt(__half v) { return __half2float(v); }
     static __device__ __forceinline__ void st1(__half* p, size_t i, float v) { p[i] = __float2half_rn(v); }
 };
 
@@ -148,6 +156,14 @@ __device__ __forceinline__ f3 spherical_fibonacci(float i, float n) {
 // ---------------------------------------------------------------------------
 // sampling
 // ---------------------------------------------------------------------------
+// Loads at a 32-bit byte offset from a wave-uniform base: the address stays one VGPR
+// (global_load ... v_offset, s_base) instead of a 64-bit VGPR pair. Volumes passed to the
+// samplers below are smaller than 4 GiB (checked by the callers' entry points).
+template <class T>
+__device__ __forceinline__ T load_at(const void* __restrict__ base, uint32_t byteOffset) {
+    return *reinterpret_cast<const T*>(static_cast<const char*>(base) + byteOffset);
+}
+
 // texture(sampler3D) linear + clamp-to-edge on a [D][H][W] volume
 template <class S>
 __device__ __forceinline__ float sample3d(const S* __restrict__ vol, int W, int H, int D, float u, float v, float w) {
@@ -159,12 +175,15 @@ __device__ __forceinline__ float sample3d(const S* __restrict__ vol, int W, int 
     x0 = clampi(x0, 0, W - 1);
     y0 = clampi(y0, 0, H - 1);
     z0 = clampi(z0, 0, D - 1);
-    size_t r00 = ((size_t)z0 * H + y0) * W, r10 = ((size_t)z0 * H + y1) * W;
-    size_t r01 = ((size_t)z1 * H + y0) * W, r11 = ((size_t)z1 * H + y1) * W;
-    float v000 = Texel<S>::ld1(vol, r00 + x0), v100 = Texel<S>::ld1(vol, r00 + x1);
-    float v010 = Texel<S>::ld1(vol, r10 + x0), v110 = Texel<S>::ld1(vol, r10 + x1);
-    float v001 = Texel<S>::ld1(vol, r01 + x0), v101 = Texel<S>::ld1(vol, r01 + x1);
-    float v011 = Texel<S>::ld1(vol, r11 + x0), v111 = Texel<S>::ld1(vol, r11 + x1);
+    // 32-bit element offsets: volumes are smaller than 4 GiB (checked by the entry points)
+    const uint32_t uW = (uint32_t)W, uH = (uint32_t)H;
+    const uint32_t r00 = ((uint32_t)z0 * uH + (uint32_t)y0) * uW, r10 = ((uint32_t)z0 * uH + (uint32_t)y1) * uW;
+    const uint32_t r01 = ((uint32_t)z1 * uH + (uint32_t)y0) * uW, r11 = ((uint32_t)z1 * uH + (uint32_t)y1) * uW;
+    constexpr uint32_t es = sizeof(S);
+    float v000 = Texel<S>::cvt(load_at<S>(vol, (r00 + x0) * es)), v100 = Texel<S>::cvt(load_at<S>(vol, (r00 + x1) * es));
+    float v010 = Texel<S>::cvt(load_at<S>(vol, (r10 + x0) * es)), v110 = Texel<S>::cvt(load_at<S>(vol, (r10 + x1) * es));
+    float v001 = Texel<S>::cvt(load_at<S>(vol, (r01 + x0) * es)), v101 = Texel<S>::cvt(load_at<S>(vol, (r01 + x1) * es));
+    float v011 = Texel<S>::cvt(load_at<S>(vol, (r11 + x0) * es)), v111 = Texel<S>::cvt(load_at<S>(vol, (r11 + x1) * es));
     float c00 = v000 * (1.0f - ax) + v100 * ax;
     float c10 = v010 * (1.0f - ax) + v110 * ax;
     float c01 = v001 * (1.0f - ax) + v101 * ax;
@@ -191,10 +210,11 @@ __device__ __forceinline__ float sample3d_pairs(const float* __restrict__ vol, i
     z0 = clampi(z0, 0, D - 1);
     const int b = clampi(x0, 0, W - 2);
     const bool left = x0 < 0, right = x0 >= W - 1;
-    const float2_a4 p00 = *reinterpret_cast<const float2_a4*>(vol + ((size_t)z0 * H + y0) * W + b);
-    const float2_a4 p10 = *reinterpret_cast<const float2_a4*>(vol + ((size_t)z0 * H + y1) * W + b);
-    const float2_a4 p01 = *reinterpret_cast<const float2_a4*>(vol + ((size_t)z1 * H + y0) * W + b);
-    const float2_a4 p11 = *reinterpret_cast<const float2_a4*>(vol + ((size_t)z1 * H + y1) * W + b);
+    const uint32_t uW = (uint32_t)W, uH = (uint32_t)H;
+    const float2_a4 p00 = load_at<float2_a4>(vol, (((uint32_t)z0 * uH + (uint32_t)y0) * uW + (uint32_t)b) * 4u);
+    const float2_a4 p10 = load_at<float2_a4>(vol, (((uint32_t)z0 * uH + (uint32_t)y1) * uW + (uint32_t)b) * 4u);
+    const float2_a4 p01 = load_at<float2_a4>(vol, (((uint32_t)z1 * uH + (uint32_t)y0) * uW + (uint32_t)b) * 4u);
+    const float2_a4 p11 = load_at<float2_a4>(vol, (((uint32_t)z1 * uH + (uint32_t)y1) * uW + (uint32_t)b) * 4u);
     const float v000 = right ? p00.y : p00.x, v100 = left ? p00.x : p00.y;
     const float v010 = right ? p10.y : p10.x, v110 = left ? p10.x : p10.y;
     const float v001 = right ? p01.y : p01.x, v101 = left ? p01.x : p01.y;
@@ -217,6 +237,11 @@ __device__ __forceinline__ size_t brick4_index(int x, int y, int z, int nbx, int
     return (b << 6) + ((z & 3) << 4) + ((y & 3) << 2) + (x & 3);
 }
 
+__device__ __forceinline__ uint32_t brick4_offset(int x, int y, int z, uint32_t nbx, uint32_t nby) {
+    const uint32_t b = ((uint32_t)(z >> 2) * nby + (uint32_t)(y >> 2)) * nbx + (uint32_t)(x >> 2);
+    return (b << 6) + (uint32_t)(((z & 3) << 4) + ((y & 3) << 2) + (x & 3));
+}
+
 template <class S>
 __device__ __forceinline__ float sample3d_brick4(const S* __restrict__ vol, int W, int H, int D, float u, float v,
                                                  float w) {
@@ -228,15 +253,16 @@ __device__ __forceinline__ float sample3d_brick4(const S* __restrict__ vol, int 
     x0 = clampi(x0, 0, W - 1);
     y0 = clampi(y0, 0, H - 1);
     z0 = clampi(z0, 0, D - 1);
-    const int nbx = W >> 2, nby = H >> 2;
-    float v000 = Texel<S>::ld1(vol, brick4_index(x0, y0, z0, nbx, nby));
-    float v100 = Texel<S>::ld1(vol, brick4_index(x1, y0, z0, nbx, nby));
-    float v010 = Texel<S>::ld1(vol, brick4_index(x0, y1, z0, nbx, nby));
-    float v110 = Texel<S>::ld1(vol, brick4_index(x1, y1, z0, nbx, nby));
-    float v001 = Texel<S>::ld1(vol, brick4_index(x0, y0, z1, nbx, nby));
-    float v101 = Texel<S>::ld1(vol, brick4_index(x1, y0, z1, nbx, nby));
-    float v011 = Texel<S>::ld1(vol, brick4_index(x0, y1, z1, nbx, nby));
-    float v111 = Texel<S>::ld1(vol, brick4_index(x1, y1, z1, nbx, nby));
+    const uint32_t nbx = (uint32_t)W >> 2, nby = (uint32_t)H >> 2;
+    constexpr uint32_t es = sizeof(S);
+    float v000 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x0, y0, z0, nbx, nby) * es));
+    float v100 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x1, y0, z0, nbx, nby) * es));
+    float v010 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x0, y1, z0, nbx, nby) * es));
+    float v110 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x1, y1, z0, nbx, nby) * es));
+    float v001 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x0, y0, z1, nbx, nby) * es));
+    float v101 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x1, y0, z1, nbx, nby) * es));
+    float v011 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x0, y1, z1, nbx, nby) * es));
+    float v111 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x1, y1, z1, nbx, nby) * es));
     float c00 = v000 * (1.0f - ax) + v100 * ax;
     float c10 = v010 * (1.0f - ax) + v110 * ax;
     float c01 = v001 * (1.0f - ax) + v101 * ax;

@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __res
 // Normal, surface-atlas lookup and ray-result encode (SDFCommon.glsl:172-184,
 // GISDFRays.comp:89-127) for one ray per lane, 64 rays of a probe per wave.
 template <class SS, class SC, class SR, bool COUNT>
-__global__ __launch_bounds__(256) void shade_kernel(TraceArgs A, const SS* __restrict__ sdf,
+__global__ __launch_bounds__(256, (!COUNT && sizeof(SS) == 4) ? 8 : 1) void shade_kernel(TraceArgs A, const SS* __restrict__ sdf,
                                                     const uint32_t* __restrict__ chunks,
                                                     const uint32_t* __restrict__ culled,
                                                     const LuxgiAtlasObject* __restrict__ objects,
@@ -535,6 +535,11 @@ static int trace_entry(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, c
         return LUXGI_ERR_INVALID_ARGUMENT;
     if (A.P <= 0 || A.R <= 0 || A.R > rays->width || rays->height < A.P) return LUXGI_ERR_INVALID_ARGUMENT;
     if ((long long)A.P * A.R >= (1ll << 31)) return LUXGI_ERR_UNSUPPORTED;
+    {  // the samplers address the volumes with 32-bit byte offsets
+        const size_t es = vol->storage == LUXGI_STORAGE_FP16 ? 2 : 4;
+        const size_t C = g->cascadesCount, r = (size_t)vol->resolution, m = (size_t)vol->mipResolution;
+        if (C * r * r * r * es >= (1ull << 32) || C * m * m * m * es >= (1ull << 32)) return LUXGI_ERR_UNSUPPORTED;
+    }
     const size_t need = luxgi_ddgi_trace_workspace_size(u, at->objectCapacity);
     if (!workspace || workspaceBytes < need) return LUXGI_ERR_WORKSPACE_TOO_SMALL;
     TraceWorkspace w;

@@ -53,7 +53,7 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
 }
 __device__ __forceinline__ float hash01(uint32_t x) { return float(hash32(x) >> 8) * (1.0f / 16777216.0f); }
 
-// directionToOctohedral (Common/Math.glsl:13-17)
+// directionToOctohedral (Common/Math.glsl:20-25)
 __device__ __forceinline__ float2 dir_to_oct(f3 n) {
     const float l1 = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
     float2 p = make_float2(n.x / l1, n.y / l1);
@@ -200,6 +200,7 @@ extern "C" int luxgi_scene_gbuffer(const LuxgiPrimitive* prims, int32_t n, const
     if (!prims || n <= 0 || !g || !sdf || res <= 0 || !viewProjInv || !prevViewProj || W <= 0 || H <= 0 || !color ||
         !normals || !pbr || !depth || !noisy)
         return LUXGI_ERR_INVALID_ARGUMENT;
+    if ((size_t)res * res * res * sizeof(float) >= (1ull << 32)) return LUXGI_ERR_UNSUPPORTED;  // 32-bit sampler offsets
     GArgs A;
     A.g = *g;
     for (int i = 0; i < 16; ++i) {

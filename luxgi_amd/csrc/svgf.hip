@@ -49,12 +49,6 @@ __device__ __forceinline__ float2 fetchS2(const S* p, int x, int y, int W, int H
     return inside(x, y, W, H) ? Texel<S>::ld2(p, (size_t)y * W + x) : make_float2(0.f, 0.f);
 }
 
-// Math.glsl:21-27
-__device__ __forceinline__ f3 world_from_depth(float u, float v, float depth, const float* m) {
-    const float4 w = mat4_mul4(m, make_float4(u * 2.0f - 1.0f, v * 2.0f - 1.0f, depth, 1.0f));
-    return mk(w.x / w.w, w.y / w.w, w.z / w.w);
-}
-
 // isValid (Reprojection.glsl:42-53)
 __device__ __forceinline__ bool is_valid(int hx, int hy, f3 pos, f3 prevPos, f3 n, f3 pn, float meshId,
                                          float prevMeshId, int W, int H) {
@@ -281,7 +275,52 @@ struct AtrousArgs {
     float phi, phiNormal, sigmaDepth, power;
 };
 
-// one 8x8 tile per block: Reflection/DenoiseAtrous.comp:63-134 or DenoiseCopyTiles.comp:21-25
+// Reflection/DenoiseAtrous.comp:63-134 for one filtered pixel. tap(px, py, sc, sn, d) gives the
+// input, decoded normal and linear depth of an in-image pixel. The sums run in one fixed order,
+// so the LDS-tiled and the direct kernel produce identical results.
+template <class Tap>
+__device__ __forceinline__ float4 reflection_filter(const AtrousArgs& A, int x, int y, float4 c, float var, f3 cn,
+                                                    float cd, Tap tap) {
+    const float kw[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
+    const float cl = luminance(c.x, c.y, c.z);
+    const float phiC = A.phi * sqrtf(fmaxf(0.0f, var)) + 1e-10f;
+    float sw = 1.0f;
+    float4 s = c;
+    for (int yy = -A.radius; yy <= A.radius; ++yy)
+        for (int xx = -A.radius; xx <= A.radius; ++xx) {
+            const int px = x + xx * A.stepSize, py = y + yy * A.stepSize;
+            if (inside(px, py, A.W, A.H) && (xx != 0 || yy != 0)) {
+                float4 sc;
+                f3 sn;
+                float d;
+                tap(px, py, sc, sn, d);
+                const float k = kw[abs(xx)] * kw[abs(yy)];
+                const float sl = luminance(sc.x, sc.y, sc.z);
+                const float w = edge_stop(cd, d, A.sigmaDepth, cn, sn, A.phiNormal, cl, sl, phiC);
+                const float wv = w * k;
+                sw += wv;
+                s.x += wv * sc.x;
+                s.y += wv * sc.y;
+                s.z += wv * sc.z;
+                s.w += wv * wv * sc.w;
+            }
+        }
+    return make_float4(s.x / sw, s.y / sw, s.z / sw, s.w / (sw * sw));
+}
+
+__device__ __forceinline__ f3 load_normal(const float4* normals, size_t q) {
+    const float2 o = reinterpret_cast<const float2*>(normals)[2 * q];  // octahedral xy only
+    return oct_decode(o.x, o.y);
+}
+__device__ __forceinline__ float load_linear_depth(const float4* pbr, size_t q) {
+    return reinterpret_cast<const float*>(pbr)[4 * q + 3];
+}
+__device__ __forceinline__ float load_roughness(const float4* pbr, size_t q) {
+    return reinterpret_cast<const float*>(pbr)[4 * q + 1];
+}
+
+// Direct form, one 8x8 tile per 64-lane block: every tap read from global memory. Used when the
+// footprint radius * stepSize exceeds the LDS halo.
 template <class S>
 __global__ __launch_bounds__(64) void reflection_atrous_kernel(AtrousArgs A, const float4* __restrict__ normals,
                                                                const float4* __restrict__ pbr,
@@ -293,48 +332,123 @@ __global__ __launch_bounds__(64) void reflection_atrous_kernel(AtrousArgs A, con
     if (x >= W || y >= H) return;
     const size_t e = (size_t)y * W + x;
     const float4 c = Texel<S>::ld4(in, e);
-    float4 o;
-    if (!tileMap[blockIdx.y * A.tilesX + blockIdx.x]) {
-        o = c;
-    } else {
-        const float cl = luminance(c.x, c.y, c.z);
+    float4 o = c;
+    if (tileMap[blockIdx.y * A.tilesX + blockIdx.x]) {
         const float kv[2][2] = {{1.0f / 4.0f, 1.0f / 8.0f}, {1.0f / 8.0f, 1.0f / 16.0f}};
         float var = 0.0f;  // computeVarianceCenter (:41-61)
         for (int yy = -1; yy <= 1; ++yy)
             for (int xx = -1; xx <= 1; ++xx) var += fetchS4<S>(in, x + xx, y + yy, W, H).w * kv[abs(xx)][abs(yy)];
-        const float4 nrm = normals[e];
-        const float4 pb = pbr[e];
-        const float cd = pb.w, rough = pb.y;
+        const float cd = load_linear_depth(pbr, e), rough = load_roughness(pbr, e);
         if (cd == 1.0f) {
             o = make_float4(0.f, 0.f, 0.f, 0.f);
-        } else if (rough <= 0.05f || rough >= 0.45f) {
-            o = c;
-        } else {
-            const float kw[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
-            const float phiC = A.phi * sqrtf(fmaxf(0.0f, var)) + 1e-10f;
-            float sw = 1.0f;
-            float4 s = c;
-            const f3 cn = oct_decode(nrm.x, nrm.y);
-            for (int yy = -A.radius; yy <= A.radius; ++yy)
-                for (int xx = -A.radius; xx <= A.radius; ++xx) {
-                    const int px = x + xx * A.stepSize, py = y + yy * A.stepSize;
-                    if (inside(px, py, W, H) && (xx != 0 || yy != 0)) {
-                        const float k = kw[abs(xx)] * kw[abs(yy)];
-                        const size_t q = (size_t)py * W + px;
-                        const float4 sc = Texel<S>::ld4(in, q);
-                        const float sl = luminance(sc.x, sc.y, sc.z);
-                        const float4 sn = normals[q];
-                        const float d = pbr[q].w;
-                        const float w = edge_stop(cd, d, A.sigmaDepth, cn, oct_decode(sn.x, sn.y), A.phiNormal, cl, sl, phiC);
-                        const float wv = w * k;
-                        sw += wv;
-                        s.x += wv * sc.x;
-                        s.y += wv * sc.y;
-                        s.z += wv * sc.z;
-                        s.w += wv * wv * sc.w;
-                    }
-                }
-            o = make_float4(s.x / sw, s.y / sw, s.z / sw, s.w / (sw * sw));
+        } else if (!(rough <= 0.05f || rough >= 0.45f)) {
+            auto tap = [&](int px, int py, float4& sc, f3& sn, float& d) {
+                const size_t q = (size_t)py * W + px;
+                sc = Texel<S>::ld4(in, q);
+                sn = load_normal(normals, q);
+                d = load_linear_depth(pbr, q);
+            };
+            o = reflection_filter(A, x, y, c, var, load_normal(normals, e), cd, tap);
+        }
+    }
+    Texel<S>::st4(out, e, o);
+    if (hist) Texel<S>::st4(hist, e, o);
+}
+
+// LDS-tiled form: a 16x16-pixel block (2x2 tiles) stages its input and guides (decoded normal,
+// linear depth) with a halo of max(radius * stepSize, 1) pixels once, so each tap is an LDS read
+// and each normal is decoded once per block instead of once per tap.
+constexpr int kAtrousBlock = 16;
+constexpr int kAtrousMaxHalo = 8;
+constexpr int kAtrousMaxSide = kAtrousBlock + 2 * kAtrousMaxHalo;
+
+struct AtrousRegion {
+    int h, side, ox, oy;
+};
+
+// Block-uniform: does any of the block's (up to) 2x2 tiles need filtering?
+__device__ __forceinline__ bool block_has_filter_tile(const AtrousArgs& A, const uint8_t* tileMap) {
+    const int tilesY = (A.H + 7) / 8;
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int tX = blockIdx.x * 2 + (i & 1), tY = blockIdx.y * 2 + (i >> 1);
+        if (tX < A.tilesX && tY < tilesY && tileMap[tY * A.tilesX + tX]) any = true;
+    }
+    return any;
+}
+
+template <class S, class V, class Load>
+__device__ __forceinline__ AtrousRegion stage_region(const AtrousArgs& A, const float4* normals, const float4* pbr,
+                                                     V* sIn, float4* sGuide, Load load) {
+    AtrousRegion r;
+    r.h = max(A.radius * A.stepSize, 1);
+    r.side = kAtrousBlock + 2 * r.h;
+    r.ox = blockIdx.x * kAtrousBlock - r.h;
+    r.oy = blockIdx.y * kAtrousBlock - r.h;
+    for (int i = threadIdx.x; i < r.side * r.side; i += 256) {
+        const int ly = i / r.side, lx = i - ly * r.side;
+        const int gx = r.ox + lx, gy = r.oy + ly;
+        V v{};
+        float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (inside(gx, gy, A.W, A.H)) {
+            const size_t q = (size_t)gy * A.W + gx;
+            v = load(q);
+            const f3 n = load_normal(normals, q);
+            g = make_float4(n.x, n.y, n.z, load_linear_depth(pbr, q));
+        }
+        sIn[i] = v;
+        sGuide[i] = g;
+    }
+    __syncthreads();
+    return r;
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void reflection_atrous_lds_kernel(AtrousArgs A, const float4* __restrict__ normals,
+                                                                    const float4* __restrict__ pbr,
+                                                                    const uint8_t* __restrict__ tileMap,
+                                                                    const S* __restrict__ in, S* __restrict__ out,
+                                                                    S* __restrict__ hist) {
+    __shared__ float4 sIn[kAtrousMaxSide * kAtrousMaxSide];
+    __shared__ float4 sGuide[kAtrousMaxSide * kAtrousMaxSide];
+    const int W = A.W, H = A.H;
+    const int tx = threadIdx.x & (kAtrousBlock - 1), ty = threadIdx.x / kAtrousBlock;
+    const int x = blockIdx.x * kAtrousBlock + tx, y = blockIdx.y * kAtrousBlock + ty;
+    const bool inImage = x < W && y < H;
+    const size_t e = (size_t)y * W + x;
+    if (!block_has_filter_tile(A, tileMap)) {  // copy tiles only (DenoiseCopyTiles.comp:21-25)
+        if (inImage) {
+            const float4 c = Texel<S>::ld4(in, e);
+            Texel<S>::st4(out, e, c);
+            if (hist) Texel<S>::st4(hist, e, c);
+        }
+        return;
+    }
+    const AtrousRegion r = stage_region<S>(A, normals, pbr, sIn, sGuide,
+                                           [&](size_t q) { return Texel<S>::ld4(in, q); });
+    if (!inImage) return;
+    const int lc = (ty + r.h) * r.side + (tx + r.h);
+    const float4 c = sIn[lc];
+    float4 o = c;
+    if (tileMap[(y >> 3) * A.tilesX + (x >> 3)]) {
+        const float kv[2][2] = {{1.0f / 4.0f, 1.0f / 8.0f}, {1.0f / 8.0f, 1.0f / 16.0f}};
+        float var = 0.0f;  // computeVarianceCenter (:41-61); out-of-image texels are staged as 0
+        for (int yy = -1; yy <= 1; ++yy)
+            for (int xx = -1; xx <= 1; ++xx) var += sIn[lc + yy * r.side + xx].w * kv[abs(xx)][abs(yy)];
+        const float4 gc = sGuide[lc];
+        const float cd = gc.w, rough = load_roughness(pbr, e);
+        if (cd == 1.0f) {
+            o = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else if (!(rough <= 0.05f || rough >= 0.45f)) {
+            auto tap = [&](int px, int py, float4& sc, f3& sn, float& d) {
+                const int l = (py - r.oy) * r.side + (px - r.ox);
+                sc = sIn[l];
+                const float4 g = sGuide[l];
+                sn = mk(g.x, g.y, g.z);
+                d = g.w;
+            };
+            o = reflection_filter(A, x, y, c, var, mk(gc.x, gc.y, gc.z), cd, tap);
         }
     }
     Texel<S>::st4(out, e, o);
@@ -508,7 +622,36 @@ __global__ __launch_bounds__(256) void shadow_reproject_kernel(ShadowArgs A, GBu
     }
 }
 
-// one 8x8 tile per block: Shadow/DenoiseAtrous.comp:275-339 or Shadow/DenoiseCopyTiles.comp (writes 0)
+// Shadow/DenoiseAtrous.comp:275-339 for one filtered pixel (same tap contract and fixed sum order
+// as reflection_filter).
+template <class Tap>
+__device__ __forceinline__ float2 shadow_filter(const AtrousArgs& A, int x, int y, float2 c, float var, f3 cn, float cd,
+                                                Tap tap) {
+    const float kw[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
+    const float phiV = A.phi * sqrtf(fmaxf(0.0f, 1e-10f + var));
+    float sw = 1.0f, s0 = c.x, s1 = c.y;
+    for (int yy = -A.radius; yy <= A.radius; ++yy)
+        for (int xx = -A.radius; xx <= A.radius; ++xx) {
+            const int px = x + xx * A.stepSize, py = y + yy * A.stepSize;
+            if (inside(px, py, A.W, A.H) && (xx != 0 || yy != 0)) {
+                float2 sv;
+                f3 sn;
+                float d;
+                tap(px, py, sv, sn, d);
+                const float k = kw[abs(xx)] * kw[abs(yy)];
+                const float w = edge_stop(cd, d, A.sigmaDepth, cn, sn, A.phiNormal, c.x, sv.x, phiV);
+                const float wv = w * k;
+                sw += wv;
+                s0 += wv * sv.x;
+                s1 += wv * wv * sv.y;
+            }
+        }
+    float2 o = make_float2(s0 / sw, s1 / (sw * sw));
+    if (A.power != 0.0f) o.x = powf(o.x, A.power);
+    return o;
+}
+
+// Direct form, one 8x8 tile per 64-lane block; copy tiles write 0 (Shadow/DenoiseCopyTiles.comp).
 template <class S>
 __global__ __launch_bounds__(64) void shadow_atrous_kernel(AtrousArgs A, const float4* __restrict__ normals,
                                                            const float4* __restrict__ pbr,
@@ -518,42 +661,75 @@ __global__ __launch_bounds__(64) void shadow_atrous_kernel(AtrousArgs A, const f
     const int W = A.W, H = A.H;
     if (x >= W || y >= H) return;
     const size_t e = (size_t)y * W + x;
-    float2 o;
-    if (!tileMap[blockIdx.y * A.tilesX + blockIdx.x]) {
-        o = make_float2(0.f, 0.f);
-    } else {
+    float2 o = make_float2(0.f, 0.f);
+    if (tileMap[blockIdx.y * A.tilesX + blockIdx.x]) {
         const float2 c = Texel<S>::ld2(in, e);
         const float kv[2][2] = {{1.0f / 4.0f, 1.0f / 8.0f}, {1.0f / 8.0f, 1.0f / 16.0f}};
         float var = 0.0f;
         for (int yy = -1; yy <= 1; ++yy)
             for (int xx = -1; xx <= 1; ++xx) var += fetchS2<S>(in, x + xx, y + yy, W, H).y * kv[abs(xx)][abs(yy)];
-        const float4 nrm = normals[e];
-        const float cd = pbr[e].w;
+        const float cd = load_linear_depth(pbr, e);
         if (cd < 0.0f) {
             o = c;
         } else {
-            const float kw[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
-            const float phiV = A.phi * sqrtf(fmaxf(0.0f, 1e-10f + var));
-            float sw = 1.0f, s0 = c.x, s1 = c.y;
-            const f3 cn = oct_decode(nrm.x, nrm.y);
-            for (int yy = -A.radius; yy <= A.radius; ++yy)
-                for (int xx = -A.radius; xx <= A.radius; ++xx) {
-                    const int px = x + xx * A.stepSize, py = y + yy * A.stepSize;
-                    if (inside(px, py, W, H) && (xx != 0 || yy != 0)) {
-                        const float k = kw[abs(xx)] * kw[abs(yy)];
-                        const size_t q = (size_t)py * W + px;
-                        const float2 sv = Texel<S>::ld2(in, q);
-                        const float4 sn = normals[q];
-                        const float d = pbr[q].w;
-                        const float w = edge_stop(cd, d, A.sigmaDepth, cn, oct_decode(sn.x, sn.y), A.phiNormal, c.x, sv.x, phiV);
-                        const float wv = w * k;
-                        sw += wv;
-                        s0 += wv * sv.x;
-                        s1 += wv * wv * sv.y;
-                    }
-                }
-            o = make_float2(s0 / sw, s1 / (sw * sw));
-            if (A.power != 0.0f) o.x = powf(o.x, A.power);
+            auto tap = [&](int px, int py, float2& sv, f3& sn, float& d) {
+                const size_t q = (size_t)py * W + px;
+                sv = Texel<S>::ld2(in, q);
+                sn = load_normal(normals, q);
+                d = load_linear_depth(pbr, q);
+            };
+            o = shadow_filter(A, x, y, c, var, load_normal(normals, e), cd, tap);
+        }
+    }
+    Texel<S>::st2(out, e, o);
+    if (hist) Texel<S>::st2(hist, e, o);
+}
+
+// LDS-tiled form (see reflection_atrous_lds_kernel).
+template <class S>
+__global__ __launch_bounds__(256) void shadow_atrous_lds_kernel(AtrousArgs A, const float4* __restrict__ normals,
+                                                                const float4* __restrict__ pbr,
+                                                                const uint8_t* __restrict__ tileMap,
+                                                                const S* __restrict__ in, S* __restrict__ out,
+                                                                S* __restrict__ hist) {
+    __shared__ float2 sIn[kAtrousMaxSide * kAtrousMaxSide];
+    __shared__ float4 sGuide[kAtrousMaxSide * kAtrousMaxSide];
+    const int W = A.W, H = A.H;
+    const int tx = threadIdx.x & (kAtrousBlock - 1), ty = threadIdx.x / kAtrousBlock;
+    const int x = blockIdx.x * kAtrousBlock + tx, y = blockIdx.y * kAtrousBlock + ty;
+    const bool inImage = x < W && y < H;
+    const size_t e = (size_t)y * W + x;
+    if (!block_has_filter_tile(A, tileMap)) {
+        if (inImage) {
+            Texel<S>::st2(out, e, make_float2(0.f, 0.f));
+            if (hist) Texel<S>::st2(hist, e, make_float2(0.f, 0.f));
+        }
+        return;
+    }
+    const AtrousRegion r = stage_region<S>(A, normals, pbr, sIn, sGuide,
+                                           [&](size_t q) { return Texel<S>::ld2(in, q); });
+    if (!inImage) return;
+    float2 o = make_float2(0.f, 0.f);
+    if (tileMap[(y >> 3) * A.tilesX + (x >> 3)]) {
+        const int lc = (ty + r.h) * r.side + (tx + r.h);
+        const float2 c = sIn[lc];
+        const float kv[2][2] = {{1.0f / 4.0f, 1.0f / 8.0f}, {1.0f / 8.0f, 1.0f / 16.0f}};
+        float var = 0.0f;
+        for (int yy = -1; yy <= 1; ++yy)
+            for (int xx = -1; xx <= 1; ++xx) var += sIn[lc + yy * r.side + xx].y * kv[abs(xx)][abs(yy)];
+        const float4 gc = sGuide[lc];
+        const float cd = gc.w;
+        if (cd < 0.0f) {
+            o = c;
+        } else {
+            auto tap = [&](int px, int py, float2& sv, f3& sn, float& d) {
+                const int l = (py - r.oy) * r.side + (px - r.ox);
+                sv = sIn[l];
+                const float4 g = sGuide[l];
+                sn = mk(g.x, g.y, g.z);
+                d = g.w;
+            };
+            o = shadow_filter(A, x, y, c, var, mk(gc.x, gc.y, gc.z), cd, tap);
         }
     }
     Texel<S>::st2(out, e, o);
@@ -570,6 +746,22 @@ static bool gbuf_ok(const LuxgiGBuffer* g) { return g && g->color && g->normals 
 }  // namespace luxgi
 
 using namespace luxgi;
+
+// Footprints within the LDS halo go through the tiled kernel (16x16 pixels, 256 lanes); larger
+// steps through the direct kernel (8x8 pixels, 64 lanes).
+template <class S, class K>
+static void launch_atrous(const AtrousArgs& A, K ldsKernel, K directKernel, hipStream_t st, const float4* nrm,
+                          const float4* pbr, const uint8_t* tileMap, const void* input, void* output, void* hist) {
+    const S* in = static_cast<const S*>(input);
+    S* out = static_cast<S*>(output);
+    S* h = static_cast<S*>(hist);
+    if (A.stepSize <= kAtrousMaxHalo && A.radius * A.stepSize <= kAtrousMaxHalo)
+        hipLaunchKernelGGL(ldsKernel, dim3((A.W + kAtrousBlock - 1) / kAtrousBlock, (A.H + kAtrousBlock - 1) / kAtrousBlock),
+                           dim3(256), 0, st, A, nrm, pbr, tileMap, in, out, h);
+    else
+        hipLaunchKernelGGL(directKernel, dim3((A.W + 7) / 8, (A.H + 7) / 8), dim3(64), 0, st, A, nrm, pbr, tileMap, in,
+                           out, h);
+}
 
 extern "C" int32_t luxgi_svgf_tile_count(int32_t width, int32_t height) {
     if (width <= 0 || height <= 0) return 0;
@@ -611,19 +803,18 @@ extern "C" int luxgi_svgf_reflection_atrous(const LuxgiReflectionAtrousParams* p
     if (!p || W <= 0 || H <= 0 || !cur || !cur->normals || !cur->pbr || !tileMap || !input || !output)
         return LUXGI_ERR_INVALID_ARGUMENT;
     if (p->radius < 0 || p->radius > 2) return LUXGI_ERR_UNSUPPORTED;  // kernelWeights has 3 entries
+    if (p->stepSize < 1 || p->stepSize > (1 << 16)) return LUXGI_ERR_INVALID_ARGUMENT;
     AtrousArgs A{W, H, (W + 7) / 8, p->radius, p->stepSize, p->phiColor, p->phiNormal, p->sigmaDepth, 0.0f};
+    if (storage != LUXGI_STORAGE_FP32 && storage != LUXGI_STORAGE_FP16) return LUXGI_ERR_UNSUPPORTED;
     hipStream_t st = as_stream(stream);
-    const dim3 grid((W + 7) / 8, (H + 7) / 8);
     const float4* nrm = static_cast<const float4*>(static_cast<const void*>(cur->normals));
     const float4* pbr = static_cast<const float4*>(static_cast<const void*>(cur->pbr));
     if (storage == LUXGI_STORAGE_FP32)
-        hipLaunchKernelGGL(reflection_atrous_kernel<float>, grid, dim3(64), 0, st, A, nrm, pbr, tileMap,
-                           static_cast<const float*>(input), static_cast<float*>(output), static_cast<float*>(historyOut));
-    else if (storage == LUXGI_STORAGE_FP16)
-        hipLaunchKernelGGL(reflection_atrous_kernel<__half>, grid, dim3(64), 0, st, A, nrm, pbr, tileMap,
-                           static_cast<const __half*>(input), static_cast<__half*>(output),
-                           static_cast<__half*>(historyOut));
-    else return LUXGI_ERR_UNSUPPORTED;
+        launch_atrous<float>(A, reflection_atrous_lds_kernel<float>, reflection_atrous_kernel<float>, st, nrm, pbr,
+                             tileMap, input, output, historyOut);
+    else
+        launch_atrous<__half>(A, reflection_atrous_lds_kernel<__half>, reflection_atrous_kernel<__half>, st, nrm, pbr,
+                              tileMap, input, output, historyOut);
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }
@@ -694,19 +885,18 @@ extern "C" int luxgi_svgf_shadow_atrous(const LuxgiShadowAtrousParams* p, int32_
     if (!p || W <= 0 || H <= 0 || !cur || !cur->normals || !cur->pbr || !tileMap || !input || !output)
         return LUXGI_ERR_INVALID_ARGUMENT;
     if (p->radius < 0 || p->radius > 2) return LUXGI_ERR_UNSUPPORTED;
+    if (p->stepSize < 1 || p->stepSize > (1 << 16)) return LUXGI_ERR_INVALID_ARGUMENT;
     AtrousArgs A{W, H, (W + 7) / 8, p->radius, p->stepSize, p->phiVisibility, p->phiNormal, p->sigmaDepth, p->power};
+    if (storage != LUXGI_STORAGE_FP32 && storage != LUXGI_STORAGE_FP16) return LUXGI_ERR_UNSUPPORTED;
     hipStream_t st = as_stream(stream);
-    const dim3 grid((W + 7) / 8, (H + 7) / 8);
     const float4* nrm = static_cast<const float4*>(static_cast<const void*>(cur->normals));
     const float4* pbr = static_cast<const float4*>(static_cast<const void*>(cur->pbr));
     if (storage == LUXGI_STORAGE_FP32)
-        hipLaunchKernelGGL(shadow_atrous_kernel<float>, grid, dim3(64), 0, st, A, nrm, pbr, tileMap,
-                           static_cast<const float*>(input), static_cast<float*>(output), static_cast<float*>(historyOut));
-    else if (storage == LUXGI_STORAGE_FP16)
-        hipLaunchKernelGGL(shadow_atrous_kernel<__half>, grid, dim3(64), 0, st, A, nrm, pbr, tileMap,
-                           static_cast<const __half*>(input), static_cast<__half*>(output),
-                           static_cast<__half*>(historyOut));
-    else return LUXGI_ERR_UNSUPPORTED;
+        launch_atrous<float>(A, shadow_atrous_lds_kernel<float>, shadow_atrous_kernel<float>, st, nrm, pbr, tileMap,
+                             input, output, historyOut);
+    else
+        launch_atrous<__half>(A, shadow_atrous_lds_kernel<__half>, shadow_atrous_kernel<__half>, st, nrm, pbr, tileMap,
+                              input, output, historyOut);
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }

@@ -7,6 +7,7 @@ Mirrors Engine/DDGI/DDGIRenderer.cpp:
   probe_update::system (:377-413)    -> DDGIVolume.probe_update
   border_update::system (:441-464)   -> DDGIVolume.border_update (or fused into probe_update)
   end_frame::system (:335-342)       -> DDGIVolume.end_frame
+  sample_probe::system (:467-503)    -> DDGIVolume.sample_probes
 Device memory comes from torch (ROCm); every compute call goes to libluxgi.so on
 the caller's current HIP stream.
 """
@@ -167,6 +168,21 @@ class DDGIVolume:
         check(lib().luxgi_ddgi_relocate_classify(C.byref(self.u), C.byref(self.rays), ptr(self.ray_flags),
                                                  C.byref(params), ptr(self.offsets), ptr(self.states),
                                                  stream_handle()), "luxgi_ddgi_relocate_classify")
+
+    # sample_probe::system (DDGIRenderer.cpp:467-503): per-pixel irradiance from the last update's atlases
+    def sample_probes(self, depth, normals, camera_position, view_proj_inv, out=None, out_storage=abi.STORAGE_FP32):
+        """depth [H][W] and normals [H][W][4] cuda float32 tensors; view_proj_inv a numpy 4x4 (row-major math)."""
+        H, W = depth.shape
+        if out is None:
+            out = torch.empty((H, W, 4), dtype=torch.float16 if out_storage == abi.STORAGE_FP16 else torch.float32,
+                              device="cuda")
+        prm = abi.SampleProbeParams()
+        prm.cameraPosition[:] = [float(camera_position[0]), float(camera_position[1]), float(camera_position[2]), 1.0]
+        prm.viewProjInv = abi.mat4_colmajor(view_proj_inv)
+        check(lib().luxgi_ddgi_sample_probes(C.byref(self.u), C.byref(self.atlas[self.ping_pong]), C.byref(prm), W, H,
+                                             ptr(depth), ptr(normals), out_storage, ptr(out), stream_handle()),
+              "luxgi_ddgi_sample_probes")
+        return out
 
     # end_frame::system (DDGIRenderer.cpp:335-342)
     def end_frame(self):

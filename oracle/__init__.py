@@ -66,6 +66,7 @@ def _declare(L):
                                                     P, P, P, P, I]
         getattr(L, "oracle_reflection_reproject_" + s).argtypes = [P, I, I, P, P, P, P, P, P, P, P, I]
         getattr(L, "oracle_reflection_atrous_" + s).argtypes = [P, I, I, P, P, P, P, P, I]
+        getattr(L, "oracle_sample_probes_" + s).argtypes = [P, P, I, I, P, P, P, P, P, I]
     L.oracle_sdf_mip.argtypes = [P, I, I, I, C.c_float, I, P, P]
     L.oracle_shadow_reproject_f64.argtypes = [P, I, I, P, P, P, P, P, P, P, P, I]
     L.oracle_shadow_atrous_f64.argtypes = [P, I, I, P, P, P, P, P, I]
@@ -302,3 +303,27 @@ def relocate(u, dir_dist, flags, params, offsets):
     states = np.zeros(u.num_probes, dtype=np.uint8)
     L.oracle_relocate(C.byref(u), _p(dir_dist), _p(flags), dir_dist.shape[1], C.byref(params), _p(off), _p(states))
     return off, states
+
+
+# ---------------------------------------------------------------------------
+# probe sampling (SampleProbe.comp + sampleIrradiance)
+# ---------------------------------------------------------------------------
+def sample_probes(u, params, W, H, depth, normals, irradiance, depth_atlas, precision="f64", nthreads=None):
+    """Returns [H][W][4] (irradiance, 1) per pixel, 0 on sky."""
+    dt = np.float64 if precision == "f64" else np.float32
+    depth, normals = _f32(depth), _f32(normals)
+    irradiance, depth_atlas = _f32(irradiance), _f32(depth_atlas)
+    out = np.zeros((H, W, 4), dtype=dt)
+    getattr(lib(), "oracle_sample_probes_" + precision)(C.byref(u), C.byref(params), W, H, _p(depth), _p(normals),
+                                                        _p(irradiance), _p(depth_atlas), _p(out),
+                                                        nthreads or threads())
+    return out
+
+
+def texture_coord_from_direction(d, probe_index, width, height, side):
+    L = lib()
+    L.oracle_texture_coord_from_direction.argtypes = [P, I, I, I, I, P]
+    dv = np.ascontiguousarray(d, dtype=np.float64)
+    out = np.zeros(2)
+    L.oracle_texture_coord_from_direction(_p(dv), probe_index, width, height, side, _p(out))
+    return out

@@ -151,7 +151,7 @@ template <class T> inline v3<T> oct_decode(v2<T> o) {
     }
     return normalize(v);
 }
-// Common/Math.glsl:31-37 (G-buffer decode; step(0,x)*2-1 == signNotZero)
+// Common/Math.glsl:27-33 (G-buffer decode; step(0,x)*2-1 == signNotZero)
 template <class T> inline v3<T> octahedral_to_direction(T ex, T ey) { return oct_decode<T>({ex, ey}); }
 
 // DDGICommon.glsl:42-52
@@ -608,7 +608,7 @@ template <class T> inline v4<T> fetch(Img im, int W, int H, int x, int y) {  // 
     return r;
 }
 
-// Math.glsl:21-27
+// worldPositionFromDepth, Common/Math.glsl:35-42
 template <class T> inline v3<T> world_from_depth(T u, T v, T depth, const float* vpi) {
     v4<T> n{u * T(2) - T(1), v * T(2) - T(1), depth, T(1)};
     T m[16];
@@ -1033,6 +1033,112 @@ inline void relocate_probe(const LuxgiDDGIUniform& u, const float* dirDist, cons
         }
         o[3] = 0.f;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Probe sampling: SampleProbe.comp:36-60 with sampleIrradiance (DDGICommon.glsl:163-233).
+// The atlases are read through the engine's default sampler (RHI/Definitions.h:152-159):
+// linear filtering, repeat addressing, normalized coordinates.
+// ---------------------------------------------------------------------------
+// texture(sampler2D, uv): bilinear, repeat, on a [H][W][ch] float image.
+template <class T> inline void sample2d_repeat(const float* img, int W, int H, int ch, T u, T v, T* out) {
+    const T x = u * T(W) - T(0.5), y = v * T(H) - T(0.5);
+    const T fx = std::floor(x), fy = std::floor(y);
+    const T ax = x - fx, ay = y - fy;
+    auto wrap = [](long long i, int n) { long long m = i % n; return int(m < 0 ? m + n : m); };
+    const int x0 = wrap((long long)fx, W), x1 = wrap((long long)fx + 1, W);
+    const int y0 = wrap((long long)fy, H), y1 = wrap((long long)fy + 1, H);
+    for (int c = 0; c < ch; ++c) {
+        const T t00 = img[((size_t)y0 * W + x0) * ch + c], t10 = img[((size_t)y0 * W + x1) * ch + c];
+        const T t01 = img[((size_t)y1 * W + x0) * ch + c], t11 = img[((size_t)y1 * W + x1) * ch + c];
+        out[c] = (t00 * (T(1) - ax) + t10 * ax) * (T(1) - ay) + (t01 * (T(1) - ax) + t11 * ax) * ay;
+    }
+}
+
+// textureCoordFromDirection (DDGICommon.glsl:141-158)
+template <class T> inline v2<T> texture_coord_from_direction(v3<T> dir, int probeIndex, int width, int height,
+                                                             int side) {
+    const v2<T> oc = oct_encode<T>(normalize(dir));
+    const T ox = (oc.x + T(1)) * T(0.5), oy = (oc.y + T(1)) * T(0.5);
+    const T pws = T(side) + T(2);
+    const int perRow = (width - 2) / int(pws);
+    const T tlx = T(probeIndex % perRow) * pws + T(2), tly = T(probeIndex / perRow) * pws + T(2);
+    return {tlx / T(width) + (ox * T(side)) / T(width), tly / T(height) + (oy * T(side)) / T(height)};
+}
+
+// sampleIrradiance (DDGICommon.glsl:163-233); irr = float4 atlas, dep = float2 atlas.
+template <class T>
+inline v3<T> sample_irradiance(const LuxgiDDGIUniform& u, v3<T> P, v3<T> N, v3<T> Wo, const float* irr,
+                               const float* dep) {
+    const v3<T> start{T(u.startPosition[0]), T(u.startPosition[1]), T(u.startPosition[2])};
+    const v3<T> stp{T(u.step[0]), T(u.step[1]), T(u.step[2])};
+    int base[3];
+    const T rel[3] = {(P.x - start.x) / stp.x, (P.y - start.y) / stp.y, (P.z - start.z) / stp.z};
+    for (int i = 0; i < 3; ++i) base[i] = std::min(std::max((int)rel[i], 0), u.probeCounts[i] - 1);  // baseGridCoord :125-128
+    const v3<T> basePos{stp.x * T(base[0]) + start.x, stp.y * T(base[1]) + start.y, stp.z * T(base[2]) + start.z};
+    const v3<T> alpha{clampT((P.x - basePos.x) / stp.x, T(0), T(1)), clampT((P.y - basePos.y) / stp.y, T(0), T(1)),
+                      clampT((P.z - basePos.z) / stp.z, T(0), T(1))};
+    v3<T> sum{0, 0, 0};
+    T sumW = 0;
+    for (int i = 0; i < 8; ++i) {
+        const int off[3] = {i & 1, (i >> 1) & 1, (i >> 2) & 1};
+        int g[3];
+        for (int k = 0; k < 3; ++k) g[k] = std::min(std::max(base[k] + off[k], 0), u.probeCounts[k] - 1);
+        const v3<T> probePos{stp.x * T(g[0]) + start.x, stp.y * T(g[1]) + start.y, stp.z * T(g[2]) + start.z};
+        const v3<T> tri{off[0] ? alpha.x : T(1) - alpha.x, off[1] ? alpha.y : T(1) - alpha.y,
+                        off[2] ? alpha.z : T(1) - alpha.z};
+        T w = 1;
+        const v3<T> dirToProbe = normalize(probePos - P);
+        const T sb = gmax(T(0.0001), (dot(dirToProbe, N) + T(1)) * T(0.5));
+        w *= sb * sb + T(0.2);
+        const int probeIdx = grid_to_probe_index(u, g[0], g[1], g[2]);
+        const v3<T> vBias = (N + Wo * T(3)) * T(u.normalBias);
+        const v3<T> probeToPoint = P - probePos + vBias;
+        const v3<T> dir = normalize(probeToPoint * T(-1));
+        v2<T> tc = texture_coord_from_direction<T>(dir * T(-1), probeIdx, u.depthTextureWidth, u.depthTextureHeight,
+                                                   u.depthProbeSideLength);
+        const T dist = length(probeToPoint);
+        T dm[2];
+        sample2d_repeat<T>(dep, u.depthTextureWidth, u.depthTextureHeight, 2, tc.x, tc.y, dm);
+        const T mean = dm[0];
+        const T variance = std::fabs(dm[0] * dm[0] - dm[1]);
+        const T md = gmax(dist - mean, T(0));
+        T cheb = variance / (variance + md * md);
+        cheb = gmax(cheb * cheb * cheb, T(0));
+        w *= (dist <= mean) ? T(1) : cheb;
+        w = gmax(T(0.000001), w);
+        tc = texture_coord_from_direction<T>(normalize(N), probeIdx, u.irradianceTextureWidth,
+                                             u.irradianceTextureHeight, u.irradianceProbeSideLength);
+        T ir[4];
+        sample2d_repeat<T>(irr, u.irradianceTextureWidth, u.irradianceTextureHeight, 4, tc.x, tc.y, ir);
+        const T ge = T(u.ddgiGamma * 0.5f);
+        const v3<T> pi{std::pow(ir[0], ge), std::pow(ir[1], ge), std::pow(ir[2], ge)};
+        const T crush = T(0.2f);
+        if (w < crush) w *= w * w * (T(1) / (crush * crush));
+        w *= tri.x * tri.y * tri.z;
+        sum = sum + pi * w;
+        sumW += w;
+    }
+    v3<T> net = sum * (T(1) / sumW);
+    net = net * net;
+    return net * T(2 * kPi);
+}
+
+// SampleProbe.comp:36-60 for pixel (x, y): out = (irradiance, 1), or 0 on sky (depth == 1).
+template <class T>
+inline void sample_probe_pixel(const LuxgiDDGIUniform& u, const LuxgiSampleProbeParams& pc, int W, int H, int x,
+                               int y, const float* depth, const float* normals, const float* irr, const float* dep,
+                               T* out) {
+    const T d = T(depth[(size_t)y * W + x]);
+    if (d == T(1)) { out[0] = out[1] = out[2] = out[3] = 0; return; }
+    const T tu = (T(x) + T(0.5)) / T(W), tv = (T(y) + T(0.5)) / T(H);
+    const v3<T> P = world_from_depth<T>(tu, tv, d, pc.viewProjInv);
+    const float* nn = normals + ((size_t)y * W + x) * 4;
+    const v3<T> N = octahedral_to_direction<T>(T(nn[0]), T(nn[1]));
+    const v3<T> cam{T(pc.cameraPosition[0]), T(pc.cameraPosition[1]), T(pc.cameraPosition[2])};
+    const v3<T> Wo = normalize(cam - P);
+    const v3<T> e = sample_irradiance<T>(u, P, N, Wo, irr, dep);
+    out[0] = e.x; out[1] = e.y; out[2] = e.z; out[3] = T(1);
 }
 
 // ---------------------------------------------------------------------------

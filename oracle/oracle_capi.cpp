@@ -305,3 +305,29 @@ extern "C" void oracle_relocate(const LuxgiDDGIUniform* u, const float* dirDist,
     const int P = u->probeCounts[0] * u->probeCounts[1] * u->probeCounts[2];
     for (int p = 0; p < P; ++p) relocate_probe(*u, dirDist, flags, width, *prm, offsets, states, p);
 }
+
+template <class T>
+static void sample_probes_impl(const LuxgiDDGIUniform* u, const LuxgiSampleProbeParams* pc, int W, int H,
+                               const float* depth, const float* normals, const float* irr, const float* dep, T* out,
+                               int threads) {
+    parallel_for(H, threads, [&](int y) {
+        for (int x = 0; x < W; ++x)
+            sample_probe_pixel<T>(*u, *pc, W, H, x, y, depth, normals, irr, dep, out + ((size_t)y * W + x) * 4);
+    });
+}
+extern "C" void oracle_sample_probes_f64(const LuxgiDDGIUniform* u, const LuxgiSampleProbeParams* pc, int W, int H,
+                                         const float* depth, const float* normals, const float* irr, const float* dep,
+                                         double* out, int threads) {
+    sample_probes_impl<double>(u, pc, W, H, depth, normals, irr, dep, out, threads);
+}
+extern "C" void oracle_sample_probes_f32(const LuxgiDDGIUniform* u, const LuxgiSampleProbeParams* pc, int W, int H,
+                                         const float* depth, const float* normals, const float* irr, const float* dep,
+                                         float* out, int threads) {
+    sample_probes_impl<float>(u, pc, W, H, depth, normals, irr, dep, out, threads);
+}
+extern "C" void oracle_texture_coord_from_direction(const double* dir, int probeIndex, int width, int height, int side,
+                                                    double* out) {
+    v2<double> r = texture_coord_from_direction<double>({dir[0], dir[1], dir[2]}, probeIndex, width, height, side);
+    out[0] = r.x;
+    out[1] = r.y;
+}

@@ -2,7 +2,7 @@
 and shadow scalar) against the FP64 oracle on the same seeded G-buffers.
 
 Tolerances. Tile maps are compared exactly. Reprojection outputs: world
-positions are rebuilt from NDC depth near 1 in fp32 (Math.glsl:21-27 as the
+positions are rebuilt from NDC depth near 1 in fp32 (Math.glsl:35-42 as the
 reference computes them), which moves reprojected coordinates by ~1e-4 px
 against FP64; with the seeded (uncorrelated) histories that changes the
 bilinear history by up to ~1e-4, and threshold decisions (disocclusion,
@@ -174,3 +174,49 @@ def test_reflection_fp16_storage_close_to_fp32():
         outs.append(dn.denoise(_dev(g), _dev(prev), torch.from_numpy(noisy).to(dt).cuda()).float().cpu().numpy())
     f = _frac_close(outs[1], outs[0], 4e-3, 4e-3)
     assert f >= 0.99, f
+
+
+@pytest.mark.parametrize("radius,step", [(0, 1), (1, 8), (2, 1), (2, 4), (2, 8), (1, 16)])
+def test_atrous_footprints_lds_and_direct(radius, step):
+    """Footprints within the 8-pixel LDS halo (radius * step <= 8) and beyond it (direct form):
+    reflection and shadow a-trous against the oracle, on a tile map with copy and filter tiles."""
+    from luxgi_amd import _lib
+    from luxgi_amd.ddgi import ptr, stream_handle
+    from luxgi_amd.svgf import gbuffer_struct
+    W, H = 77, 45
+    g, prev, noisy, hist, hmom, mask, cam = _svgf_inputs.make_pair(W, H, 5)
+    dg = _dev(g)
+    rng = np.random.default_rng(step * 10 + radius)
+    tiles = (rng.random(_lib.lib().luxgi_svgf_tile_count(W, H)) < 0.7).astype(np.uint8)
+    ttiles = torch.from_numpy(tiles).cuda()
+    x4 = rng.uniform(0, 1.5, (H, W, 4)).astype(np.float32)
+    ap = abi.ReflectionAtrousParams(radius, step, 10.0, 32.0, 1.0, 1)
+    tin, tout = torch.from_numpy(x4).cuda(), torch.zeros((H, W, 4), device="cuda")
+    assert _lib.lib().luxgi_svgf_reflection_atrous(C.byref(ap), W, H, 0, C.byref(gbuffer_struct(dg)), ptr(ttiles),
+                                                   ptr(tin), ptr(tout), None, stream_handle()) == 0
+    torch.cuda.synchronize()
+    ref = oracle.reflection_atrous(ap, W, H, g["normals"], g["pbr"], tiles, x4)
+    np.testing.assert_allclose(tout.cpu().numpy(), ref, rtol=1e-4, atol=1e-5)
+    x2 = np.ascontiguousarray(x4[..., :2] / 1.5)
+    sp = abi.ShadowAtrousParams(radius, step, 10.0, 32.0, 1.0, 1.2, 0.1, 1000.0)
+    sin, sout = torch.from_numpy(x2).cuda(), torch.ones((H, W, 2), device="cuda")
+    assert _lib.lib().luxgi_svgf_shadow_atrous(C.byref(sp), W, H, 0, C.byref(gbuffer_struct(dg)), ptr(ttiles),
+                                               ptr(sin), ptr(sout), None, stream_handle()) == 0
+    torch.cuda.synchronize()
+    sref = oracle.shadow_atrous(sp, W, H, g["normals"], g["pbr"], tiles, x2)
+    np.testing.assert_allclose(sout.cpu().numpy(), sref, rtol=1e-4, atol=1e-5)
+
+
+def test_atrous_rejects_bad_steps():
+    from luxgi_amd import _lib
+    from luxgi_amd.ddgi import ptr, stream_handle
+    from luxgi_amd.svgf import gbuffer_struct
+    W, H = 16, 16
+    g, *_ = _svgf_inputs.make_pair(W, H, 0)
+    dg = _dev(g)
+    t = torch.zeros((H, W, 4), device="cuda")
+    tiles = torch.ones(4, dtype=torch.uint8, device="cuda")
+    for step in (0, -2, 1 << 20):
+        ap = abi.ReflectionAtrousParams(1, step, 10.0, 32.0, 1.0, 1)
+        assert _lib.lib().luxgi_svgf_reflection_atrous(C.byref(ap), W, H, 0, C.byref(gbuffer_struct(dg)), ptr(tiles),
+                                                       ptr(t), ptr(t), None, stream_handle()) != 0

@@ -8,7 +8,9 @@ Workload (one "step" = one frame of the hot path on resident inputs):
     the border copy fused into the same pass.
   * SVGF reflection denoise at 1920x1080 on a G-buffer from SDF primary rays:
     reprojection + moments + 17x17 clip + tile classification, 5 a-trous passes.
-Reported beside the step (not in value): per-pixel probe sampling of the updated
+Reported beside the step (not in value): the reduced-precision form of the same
+frame (FP16 SDF bricks + FP32 hit re-evaluation, FP16 rays, FP16-MFMA blend, RGBA16F
+SVGF; reduced_precision), and per-pixel probe sampling of the updated
 volume at 1920x1080 (probe_sample_ms; bytes = G-buffer depth + normal + RGBA32F
 output per pixel + both atlases once).
 value = rays / DDGI-update time (Grays/s). All timing uses HIP events on the
@@ -46,6 +48,7 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--svgf-iterations", type=int, default=5)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, launch a plain copy kernel (256 MiB read + 256 MiB written, 3 times) "
                          "whose rocprofv3 FETCH_SIZE / WRITE_SIZE scale the trace counters to bytes")
@@ -154,6 +157,50 @@ def main():
     atlas_bytes = sum(t.numel() * t.element_size() for t in (vol.current_irradiance, vol.current_depth))
     sample_bytes = W * H * (4 + 8 + 16) + atlas_bytes
 
+    # ---------------- reduced-precision form (reported beside the step, not in value) ----------------
+    # FP16 SDF bricks with per-brick scale and FP32 re-evaluation at hits, FP16 ray buffers (the
+    # reference's RGBA16F), FP16-MFMA blend with FP32 accumulation, SVGF with RGBA16F intermediates
+    mixed = None
+    if not args.no_mixed:
+        ds16 = DeviceScene(g, sdf, mip, atlas, sdf_storage=abi.STORAGE_FP16, layout=abi.LAYOUT_BRICK4, brick_scale=True,
+                           refine=True)
+        vol16 = DDGIVolume(u, ray_storage=abi.STORAGE_FP16)
+        dn16 = ReflectionDenoiser(W, H, storage=abi.STORAGE_FP16, iterations=args.svgf_iterations)
+        dn16.set_camera(VPI, VPp, eye, delta)
+        noisy16 = noisy.half()
+
+        def frame16(i, ev=None):
+            p = default_trace_params(rotations[i], frame=vol16.frames)
+            if ev:
+                ev[0].record()
+            vol16.trace_rays(ds16, p)
+            if ev:
+                ev[1].record()
+            vol16.probe_update(fuse_border=True, half=True)
+            if ev:
+                ev[2].record()
+            vol16.end_frame()
+            dn16.denoise(cur, prev, noisy16)
+            if ev:
+                ev[3].record()
+
+        for i in range(args.warmup):
+            frame16(i)
+        ev16 = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        for k in range(args.steps):
+            frame16(args.warmup + k, ev16[k])
+        torch.cuda.synchronize()
+        m_trace = float(np.mean([e[0].elapsed_time(e[1]) for e in ev16]))
+        m_upd = float(np.mean([e[1].elapsed_time(e[2]) for e in ev16]))
+        m_svgf = float(np.mean([e[2].elapsed_time(e[3]) for e in ev16]))
+        mixed = {"trace_ms": m_trace, "probe_update_ms": m_upd, "svgf_ms": m_svgf, "ddgi_update_ms": m_trace + m_upd,
+                 "grays_per_s": rays_per_frame / ((m_trace + m_upd) * 1e-3) / 1e9,
+                 "speedup_ddgi_vs_fp32": ddgi_ms / (m_trace + m_upd),
+                 "speedup_step_vs_fp32": (ddgi_ms + float(np.mean(t_svgf))) / (m_trace + m_upd + m_svgf)}
+        log(f"[bench] reduced precision: trace {m_trace:.3f} ms, update {m_upd:.3f} ms, svgf {m_svgf:.3f} ms", q)
+        del ds16, vol16, dn16
+
     # ---------------- counting run (algorithmic bytes of the trace) ----------------
     counters = torch.zeros(8, dtype=torch.int64, device="cuda")
     vol.trace_rays(ds, default_trace_params(rotations[-1], frame=vol.frames), counters=counters)
@@ -204,6 +251,7 @@ def main():
         "probe_update_ms": float(np.mean(t_upd)),
         "svgf_ms": float(np.mean(t_svgf)),
         "probe_sample_ms": sample_ms,
+        "reduced_precision": mixed,
         "probe_sample_gbs": sample_bytes / (sample_ms * 1e-3) / 1e9,
         "trace_grays_per_s": rays_per_frame / trace_s / 1e9,
         "trace_counts": {"lookups": n_lookup, "normals": n_normal, "objects": c[3], "tile_gathers": n_tile,

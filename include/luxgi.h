@@ -144,13 +144,26 @@ typedef struct LuxgiSDFVolume {
     const float* guard;   /* optional [D/4][H/4][W/4] lower bounds of `sdf` per 4x4x4 brick and its
                              +1 apron (luxgi_sdf_build_guard); lets the march skip fine lookups whose
                              value the reference would not use. NULL = no skipping. */
+    const float* brickScale; /* optional, LUXGI_LAYOUT_BRICK4 only: distance = stored value * brickScale[brick]
+                                (luxgi_sdf_to_bricks with a scale output). NULL = scale 1. */
+    const float* refine;  /* optional FP32 fine volume (layout `refineLayout`) of the same distances: the
+                             hit step's distance, the hit time and the hit normal are re-evaluated from it,
+                             so a reduced-precision march returns FP32 hits. NULL = none. */
+    int32_t     refineLayout;
+    int32_t     pad;
 } LuxgiSDFVolume;
 
-/* Memory layout of the fine SDF volume. */
+/* Memory layout of the fine SDF volume and of the surface atlas. */
 typedef enum LuxgiLayout {
-    LUXGI_LAYOUT_LINEAR = 0, /* [z][y][x], the reference texture's texel order */
-    LUXGI_LAYOUT_BRICK4 = 1  /* 4x4x4 bricks (x-fastest brick order), voxels z-major inside a brick;
-                                FP32 storage, every dimension a multiple of 4 (luxgi_sdf_to_bricks) */
+    LUXGI_LAYOUT_LINEAR = 0, /* SDF: [z][y][x], the reference texture's texel order.
+                                Atlas: separate colour [res][res] and depth [res][res] images. */
+    LUXGI_LAYOUT_BRICK4 = 1  /* SDF: 4x4x4 bricks (x-fastest brick order), each stored with a +1 apron
+                                as 5x5x5 voxels (element k*25 + j*5 + i = voxel (4bx+i, 4by+j, 4bz+k),
+                                clamped to the volume) padded to 128 elements; FP32 or FP16, every
+                                dimension a multiple of 4 (luxgi_sdf_to_bricks, luxgi_sdf_bricks_bytes).
+                                Atlas: one 8-byte texel (RGBA8 colour, FP32 depth) per texel in 4x4-texel
+                                blocks of 128 bytes, blocks row-major; UNORM8 colour, res a multiple of 4
+                                (luxgi_atlas_pack); `color` points at it and `depth` is unused. */
 } LuxgiLayout;
 
 /* Surface atlas + object/tile/chunk buffers read by the hit lookup
@@ -165,6 +178,8 @@ typedef struct LuxgiSurfaceAtlas {
     const float*            depth;          /* tile depth [res][res] R32F, >=1 = empty */
     int32_t                 objectCapacity; /* number of entries in `objects` */
     int32_t                 colorStorage;   /* FP32, FP16 or UNORM8 */
+    int32_t                 layout;         /* LuxgiLayout of color/depth (LINEAR or BRICK4) */
+    int32_t                 pad;
 } LuxgiSurfaceAtlas;
 
 /* Optional sky cubemap: 6 faces (+X,-X,+Y,-Y,+Z,-Z) of size^2 RGBA32F texels. */
@@ -200,6 +215,8 @@ typedef struct LuxgiProbeAtlas {
 #define LUXGI_UPDATE_DEPTH        0x2u  /* run DepthProbeUpdate */
 #define LUXGI_UPDATE_FUSE_BORDER  0x4u  /* also write the border texels (BorderUpdate) */
 #define LUXGI_UPDATE_PER_PROBE    0x8u  /* force the per-probe weight path (no shared-direction GEMM) */
+#define LUXGI_UPDATE_HALF        0x10u  /* reduced precision: weights and ray data rounded to FP16 for the
+                                           FP16 MFMA contraction, FP32 accumulation and epilogue */
 
 /* ------------------------------------------------------------------------- */
 /* DDGI                                                                        */
@@ -331,10 +348,24 @@ int luxgi_sdf_build_mip(const float* sdf, int32_t resolution, int32_t cascades, 
                         float maxDistance, int32_t floodPasses,
                         float* mip, float* temp, void* stream);
 
-/* Re-lays a linear FP32 volume [depth][height][width] as LUXGI_LAYOUT_BRICK4
- * (no counterpart in the reference, whose Vulkan 3D image is tiled by the driver).
- * width, height, depth multiples of 4; dst holds width*height*depth floats. */
-int luxgi_sdf_to_bricks(const float* src, int32_t width, int32_t height, int32_t depth, float* dst, void* stream);
+/* Bytes of a LUXGI_LAYOUT_BRICK4 volume of width x height x depth voxels (multiples of 4) in
+ * `storage` (FP32 or FP16): (W/4)(H/4)(D/4) bricks of 128 elements. 0 for invalid arguments. */
+size_t luxgi_sdf_bricks_bytes(int32_t width, int32_t height, int32_t depth, int32_t storage);
+
+/* Re-lays a linear FP32 volume [depth][height][width] as LUXGI_LAYOUT_BRICK4 in `storage` (FP32,
+ * or FP16 rounded to nearest); `dst` holds luxgi_sdf_bricks_bytes() bytes. With `brickScale`
+ * (nullable, one float per brick) each brick is stored divided by the smallest power of two not
+ * below its largest magnitude (1 for an all-zero brick), which keeps FP16 bricks of any distance
+ * range in [-1, 1] and out of the subnormals; the division and the sampler's multiplication are
+ * exact. No counterpart in the reference, whose Vulkan 3D image is tiled by the driver. */
+int luxgi_sdf_to_bricks(const float* src, int32_t width, int32_t height, int32_t depth, int32_t storage, void* dst,
+                        float* brickScale, void* stream);
+
+/* Packs a UNORM8 colour atlas (RGBA8 [res][res]) and its FP32 depth ([res][res]) into the
+ * LUXGI_LAYOUT_BRICK4 atlas layout (res a multiple of 4; `packed` holds res*res*8 bytes). No
+ * counterpart in the reference (its atlas is two driver-tiled images); the bilinear footprints of
+ * the hit lookup then touch one 128-byte line in 9/16 of the cases instead of four lines. */
+int luxgi_atlas_pack(const uint32_t* color, const float* depth, int32_t resolution, void* packed, void* stream);
 
 /* Per-brick lower bounds for LuxgiSDFVolume.guard: guard[bz][by][bx] = min of the linear FP32
  * volume `sdf` ([depth][height][width], dims multiples of 4) over voxels [4b, 4b+4] on each axis

@@ -27,8 +27,10 @@ SIGNATURES = {
     "luxgi_ddgi_relocate_classify": (C.c_int, [P, P, P, P, P, P, P]),
     "luxgi_ddgi_sample_probes": (C.c_int, [P, P, P, I, I, P, P, I, P, P]),
     "luxgi_sdf_build_mip": (C.c_int, [P, I, I, I, C.c_float, I, P, P, P]),
-    "luxgi_sdf_to_bricks": (C.c_int, [P, I, I, I, P, P]),
+    "luxgi_sdf_bricks_bytes": (C.c_size_t, [I, I, I, I]),
+    "luxgi_sdf_to_bricks": (C.c_int, [P, I, I, I, I, P, P, P]),
     "luxgi_sdf_build_guard": (C.c_int, [P, I, I, I, P, P]),
+    "luxgi_atlas_pack": (C.c_int, [P, P, I, P, P]),
     "luxgi_svgf_tile_count": (I, [I, I]),
     "luxgi_svgf_reflection_reproject": (C.c_int, [P, I, I, I, P, P, P, P, P, P, P, P, P]),
     "luxgi_svgf_reflection_atrous": (C.c_int, [P, I, I, I, P, P, P, P, P, P]),

@@ -22,6 +22,7 @@ UPDATE_IRRADIANCE = 0x1
 UPDATE_DEPTH = 0x2
 UPDATE_FUSE_BORDER = 0x4
 UPDATE_PER_PROBE = 0x8
+UPDATE_HALF = 0x10
 
 GLOBAL_SDF_WORLD_SIZE = 60000.0  # Shaders/SDF/SDFCommon.glsl:6
 ATLAS_CHUNKS_RESOLUTION = 40     # Shaders/SDF/AtlasCommon.glsl:5
@@ -86,14 +87,15 @@ LAYOUT_BRICK4 = 1
 
 class SDFVolume(C.Structure):
     _fields_ = [("sdf", C.c_void_p), ("mip", C.c_void_p), ("resolution", C.c_int32),
-                ("mipResolution", C.c_int32), ("storage", C.c_int32), ("layout", C.c_int32), ("guard", C.c_void_p)]
+                ("mipResolution", C.c_int32), ("storage", C.c_int32), ("layout", C.c_int32), ("guard", C.c_void_p),
+                ("brickScale", C.c_void_p), ("refine", C.c_void_p), ("refineLayout", C.c_int32), ("pad", C.c_int32)]
 
 
 class SurfaceAtlas(C.Structure):
     _fields_ = [
         ("data", SurfaceAtlasData), ("chunks", C.c_void_p), ("culledObjects", C.c_void_p),
         ("objects", C.c_void_p), ("tiles", C.c_void_p), ("color", C.c_void_p), ("depth", C.c_void_p),
-        ("objectCapacity", C.c_int32), ("colorStorage", C.c_int32),
+        ("objectCapacity", C.c_int32), ("colorStorage", C.c_int32), ("layout", C.c_int32), ("pad", C.c_int32),
     ]
 
 

@@ -228,48 +228,61 @@ __device__ __forceinline__ float sample3d_pairs(const float* __restrict__ vol, i
     return c0 * (1.0f - az) + c1 * az;
 }
 
-// Bricked volume: 4x4x4 bricks of 64 voxels stored contiguously, bricks in
-// x-fastest order, voxels inside a brick z-major ((z&3)*16 + (y&3)*4 + (x&3)),
-// so one 128-byte line of an FP32 brick holds a 4x4x2 block. W, H, D are
-// multiples of 4. Same filtering arithmetic as sample3d.
-__device__ __forceinline__ size_t brick4_index(int x, int y, int z, int nbx, int nby) {
-    const size_t b = ((size_t)(z >> 2) * nby + (y >> 2)) * nbx + (x >> 2);
-    return (b << 6) + ((z & 3) << 4) + ((y & 3) << 2) + (x & 3);
-}
+// Bricked volume (LUXGI_LAYOUT_BRICK4): W/4 x H/4 x D/4 bricks in x-fastest order, each stored
+// with a +1 apron as 5x5x5 voxels (k*25 + j*5 + i for voxel (4bx+i, 4by+j, 4bz+k), clamped to the
+// volume) padded to 128 elements. Every trilinear footprint whose lower corner lies in brick b
+// is then inside b's 5^3 block: 4 x-adjacent voxel pairs, each one 2-element load, instead of 8
+// scattered loads. W, H, D are multiples of 4. Same filtering arithmetic as sample3d.
+constexpr uint32_t kBrickElems = 128;
 
-__device__ __forceinline__ uint32_t brick4_offset(int x, int y, int z, uint32_t nbx, uint32_t nby) {
-    const uint32_t b = ((uint32_t)(z >> 2) * nby + (uint32_t)(y >> 2)) * nbx + (uint32_t)(x >> 2);
-    return (b << 6) + (uint32_t)(((z & 3) << 4) + ((y & 3) << 2) + (x & 3));
+typedef _Float16 half2_a2 __attribute__((ext_vector_type(2), aligned(2)));
+template <class S> struct VoxelPair;
+template <> struct VoxelPair<float> {
+    typedef float2_a4 T;
+    static __device__ __forceinline__ float2 get(T p) { return make_float2(p.x, p.y); }
+};
+template <> struct VoxelPair<__half> {
+    typedef half2_a2 T;
+    static __device__ __forceinline__ float2 get(T p) { return make_float2(float(p.x), float(p.y)); }
+};
+
+__device__ __forceinline__ uint32_t brick_index(int x, int y, int z, uint32_t nbx, uint32_t nby) {
+    return ((uint32_t)(z >> 2) * nby + (uint32_t)(y >> 2)) * nbx + (uint32_t)(x >> 2);
 }
 
 template <class S>
 __device__ __forceinline__ float sample3d_brick4(const S* __restrict__ vol, int W, int H, int D, float u, float v,
-                                                 float w) {
+                                                 float w, const float* __restrict__ scale = nullptr) {
+    typedef typename VoxelPair<S>::T PT;
+    constexpr uint32_t es = sizeof(S);
     float x = u * float(W) - 0.5f, y = v * float(H) - 0.5f, z = w * float(D) - 0.5f;
     float fx0 = floorf(x), fy0 = floorf(y), fz0 = floorf(z);
     float ax = x - fx0, ay = y - fy0, az = z - fz0;
-    int x0 = int(fx0), y0 = int(fy0), z0 = int(fz0);
-    int x1 = clampi(x0 + 1, 0, W - 1), y1 = clampi(y0 + 1, 0, H - 1), z1 = clampi(z0 + 1, 0, D - 1);
-    x0 = clampi(x0, 0, W - 1);
-    y0 = clampi(y0, 0, H - 1);
-    z0 = clampi(z0, 0, D - 1);
-    const uint32_t nbx = (uint32_t)W >> 2, nby = (uint32_t)H >> 2;
-    constexpr uint32_t es = sizeof(S);
-    float v000 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x0, y0, z0, nbx, nby) * es));
-    float v100 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x1, y0, z0, nbx, nby) * es));
-    float v010 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x0, y1, z0, nbx, nby) * es));
-    float v110 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x1, y1, z0, nbx, nby) * es));
-    float v001 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x0, y0, z1, nbx, nby) * es));
-    float v101 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x1, y0, z1, nbx, nby) * es));
-    float v011 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x0, y1, z1, nbx, nby) * es));
-    float v111 = Texel<S>::cvt(load_at<S>(vol, brick4_offset(x1, y1, z1, nbx, nby) * es));
+    const int x0 = int(fx0), y0 = int(fy0), z0 = int(fz0);
+    // clamp-to-edge: a lower corner below 0 reads voxel 0 twice; one at the upper edge reads the
+    // apron, which holds the clamped edge voxel
+    const bool left = x0 < 0;
+    const int xc = clampi(x0, 0, W - 1), yc = clampi(y0, 0, H - 1), zc = clampi(z0, 0, D - 1);
+    const uint32_t b = brick_index(xc, yc, zc, (uint32_t)W >> 2, (uint32_t)H >> 2);
+    const int jl = yc & 3, kl = zc & 3;
+    const int jh = y0 < 0 ? jl : jl + 1, kh = z0 < 0 ? kl : kl + 1;
+    const uint32_t base = b * kBrickElems + (uint32_t)(xc & 3);
+    const float2 p00 = VoxelPair<S>::get(load_at<PT>(vol, (base + kl * 25 + jl * 5) * es));
+    const float2 p10 = VoxelPair<S>::get(load_at<PT>(vol, (base + kl * 25 + jh * 5) * es));
+    const float2 p01 = VoxelPair<S>::get(load_at<PT>(vol, (base + kh * 25 + jl * 5) * es));
+    const float2 p11 = VoxelPair<S>::get(load_at<PT>(vol, (base + kh * 25 + jh * 5) * es));
+    const float v000 = p00.x, v100 = left ? p00.x : p00.y;
+    const float v010 = p10.x, v110 = left ? p10.x : p10.y;
+    const float v001 = p01.x, v101 = left ? p01.x : p01.y;
+    const float v011 = p11.x, v111 = left ? p11.x : p11.y;
     float c00 = v000 * (1.0f - ax) + v100 * ax;
     float c10 = v010 * (1.0f - ax) + v110 * ax;
     float c01 = v001 * (1.0f - ax) + v101 * ax;
     float c11 = v011 * (1.0f - ax) + v111 * ax;
     float c0 = c00 * (1.0f - ay) + c10 * ay;
     float c1 = c01 * (1.0f - ay) + c11 * ay;
-    return c0 * (1.0f - az) + c1 * az;
+    const float r = c0 * (1.0f - az) + c1 * az;
+    return scale ? r * scale[b] : r;  // a power of two: exact
 }
 
 // volume lookup used by the march: pair loads for FP32 volumes of width >= 2

@@ -16,6 +16,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace luxgi {
 
@@ -31,6 +32,9 @@ struct TraceArgs {
     int res, mipRes, maxSteps, skySize, P, R, rayWidth;
     float hitEps;
     int fineLayout;  // LuxgiLayout of the fine volume
+    const float* brickScale;  // optional per-brick scale of a BRICK4 fine volume
+    const float* refine;      // optional FP32 fine volume re-evaluated at hits
+    int refineLayout;
 };
 
 // GLSL inverse(mat4) by cofactors (column-major), evaluated once per object.
@@ -65,7 +69,16 @@ struct Counts {
     unsigned long long mip, fine, normals, objects, tiles, steps, hits;
 };
 
-// textureGather of a repeat-addressed [res][res] image (Vulkan footprint order)
+// Packed surface atlas (LUXGI_LAYOUT_BRICK4): per texel uint2 (RGBA8 colour, FP32 depth bits),
+// 4x4-texel blocks of 128 bytes, blocks row-major.
+struct PackedAtlas {};
+
+__device__ __forceinline__ uint32_t atlas_block_index(int i, int j, int res) {
+    return (((uint32_t)(j >> 2) * (uint32_t)(res >> 2) + (uint32_t)(i >> 2)) << 4) + (uint32_t)((j & 3) * 4 + (i & 3));
+}
+
+// textureGather of a repeat-addressed [res][res] image (Vulkan footprint order), as element
+// indices of the atlas layout
 template <class SC>
 __device__ __forceinline__ void gather_footprint(int res, float u, float v, size_t idx[4]) {
     float x = u * float(res) - 0.5f, y = v * float(res) - 0.5f;
@@ -76,10 +89,23 @@ __device__ __forceinline__ void gather_footprint(int res, float u, float v, size
         return m < 0 ? m + res : m;
     };
     i0 = wrap(i0); i1 = wrap(i1); j0 = wrap(j0); j1 = wrap(j1);
-    idx[0] = (size_t)j1 * res + i0;
-    idx[1] = (size_t)j1 * res + i1;
-    idx[2] = (size_t)j0 * res + i1;
-    idx[3] = (size_t)j0 * res + i0;
+    if constexpr (std::is_same<SC, PackedAtlas>::value) {
+        idx[0] = atlas_block_index(i0, j1, res);
+        idx[1] = atlas_block_index(i1, j1, res);
+        idx[2] = atlas_block_index(i1, j0, res);
+        idx[3] = atlas_block_index(i0, j0, res);
+    } else {
+        idx[0] = (size_t)j1 * res + i0;
+        idx[1] = (size_t)j1 * res + i1;
+        idx[2] = (size_t)j0 * res + i1;
+        idx[3] = (size_t)j0 * res + i0;
+    }
+}
+
+__device__ __forceinline__ float4 unorm8x4(uint32_t u) {
+    const float s = 1.0f / 255.0f;
+    return make_float4(float(u & 255u) * s, float((u >> 8) & 255u) * s, float((u >> 16) & 255u) * s,
+                       float(u >> 24) * s);
 }
 
 // sampleGlobalSurfaceAtlasTile (AtlasCommon.glsl:58-112)
@@ -87,6 +113,7 @@ template <class SC, bool COUNT>
 __device__ float4 sample_tile(const LuxgiSurfaceAtlasData& ad, const LuxgiAtlasTile* __restrict__ tp,
                               const SC* __restrict__ color, const float* __restrict__ depth, f3 lp, f3 ln,
                               float threshold, Counts& c) {
+    constexpr bool kPacked = std::is_same<SC, PackedAtlas>::value;
     float tm[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) tm[k] = tp->transform[k];
@@ -107,9 +134,16 @@ __device__ float4 sample_tile(const LuxgiSurfaceAtlasData& ad, const LuxgiAtlasT
     if (COUNT) c.tiles++;
     const float thr = 2.0f * threshold / ob.z;
     float vis[4];
+    uint2 packed[4];
+    if constexpr (kPacked) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) packed[i] = reinterpret_cast<const uint2*>(color)[idx[i]];
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const float tz = depth[idx[i]];
+        float tz;
+        if constexpr (kPacked) tz = __uint_as_float(packed[i].y);
+        else tz = depth[idx[i]];
         vis[i] = 1.0f - clampf((fabsf(tileDepth - tz) - thr) / (0.5f * thr), 0.0f, 1.0f);
         if (tz >= 1.0f) vis[i] = 0.0f;
     }
@@ -120,7 +154,9 @@ __device__ float4 sample_tile(const LuxgiSurfaceAtlasData& ad, const LuxgiAtlasT
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const float w = bw[i] * vis[i];
-        const float4 cc = Texel<SC>::ld4(color, idx[i]);
+        float4 cc;
+        if constexpr (kPacked) cc = unorm8x4(packed[i].x);
+        else cc = Texel<SC>::ld4(color, idx[i]);
         r += cc.x * w;
         g += cc.y * w;
         b += cc.z * w;
@@ -237,9 +273,16 @@ __device__ __forceinline__ void ray_setup(const TraceArgs& A, const float* __res
 
 // fine-volume lookup in the volume's layout
 template <class SS, bool BRICK>
-__device__ __forceinline__ float fine_lookup(const SS* __restrict__ sdf, int W, int H, int D, float u, float v, float w) {
-    if constexpr (BRICK) return sample3d_brick4<SS>(sdf, W, H, D, u, v, w);
+__device__ __forceinline__ float fine_lookup(const SS* __restrict__ sdf, const float* __restrict__ scale, int W, int H,
+                                             int D, float u, float v, float w) {
+    if constexpr (BRICK) return sample3d_brick4<SS>(sdf, W, H, D, u, v, w, scale);
     else return sample_volume<SS>(sdf, W, H, D, u, v, w);
+}
+
+// the FP32 re-evaluation volume, in its own layout
+__device__ __forceinline__ float refine_lookup(const TraceArgs& A, int W, int H, int D, float u, float v, float w) {
+    return A.refineLayout == LUXGI_LAYOUT_BRICK4 ? sample3d_brick4<float>(A.refine, W, H, D, u, v, w)
+                                                 : sample_volume<float>(A.refine, W, H, D, u, v, w);
 }
 
 template <class SS, bool COUNT, bool BRICK>
@@ -302,9 +345,10 @@ __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __res
                     const float gv = guard[((size_t)(gz >> 2) * (A.res >> 2) + (gy >> 2)) * (texW >> 2) + (gx >> 2)];
                     needFine = !(gv >= chunkMarginDistance * 2.0f * (1.0f + 1e-5f));
                 }
-                float sd = sample_volume<SS>(mip, mipW, A.mipRes, A.mipRes, tx, uy, uz);
+                const float sm = sample_volume<SS>(mip, mipW, A.mipRes, A.mipRes, tx, uy, uz);
+                float sd = sm;
                 float st = 1.0f;
-                if (needFine) st = fine_lookup<SS, BRICK>(sdf, texW, A.res, A.res, tx, uy, uz);
+                if (needFine) st = fine_lookup<SS, BRICK>(sdf, A.brickScale, texW, A.res, A.res, tx, uy, uz);
                 if (COUNT) cMip++;
                 if (sd < chunkSizeDistance) {
                     if (COUNT) cFine++;
@@ -315,6 +359,12 @@ __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __res
                 sd *= cmax;
                 const float thick = A.hitEps > 0.0f ? A.hitEps : voxel * 0.5f * clampf(stepTime / voxel, 0.0f, 1.0f);
                 if (sd < thick) {
+                    if (A.refine) {  // FP32 re-evaluation of the hit step, same selection rule
+                        const float f32 = refine_lookup(A, texW, A.res, A.res, tx, uy, uz);
+                        const float s32 = sm < chunkSizeDistance ? (f32 < chunkMarginDistance * 2.0f ? f32 : sm)
+                                                                 : chunkSizeDistance;
+                        sd = s32 * cmax;
+                    }
                     ra = make_float4(fmaxf(stepTime + sd - thick, 0.0f), tx, uy, uz);
                     rb = make_float2(sd, float(cas));
                     break;
@@ -346,7 +396,7 @@ __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __res
 // Normal, surface-atlas lookup and ray-result encode (SDFCommon.glsl:172-184,
 // GISDFRays.comp:89-127) for one ray per lane, 64 rays of a probe per wave.
 template <class SS, class SC, class SR, bool COUNT>
-__global__ __launch_bounds__(256, (!COUNT && sizeof(SS) == 4) ? 8 : 1) void shade_kernel(TraceArgs A, const SS* __restrict__ sdf,
+__global__ __launch_bounds__(256) void shade_kernel(TraceArgs A, const SS* __restrict__ sdf,
                                                     const uint32_t* __restrict__ chunks,
                                                     const uint32_t* __restrict__ culled,
                                                     const LuxgiAtlasObject* __restrict__ objects,
@@ -385,12 +435,14 @@ __global__ __launch_bounds__(256, (!COUNT && sizeof(SS) == 4) ? 8 : 1) void shad
                 const int texW = A.res * C;
                 const float o = 1.0f / A.g.resolution;
                 const float tx = ra.y, uy = ra.z, uz = ra.w;
-                const float xp = (A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, tx + o, uy, uz) : sample_volume<SS>(sdf, texW, A.res, A.res, tx + o, uy, uz));
-                const float xn = (A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, tx - o, uy, uz) : sample_volume<SS>(sdf, texW, A.res, A.res, tx - o, uy, uz));
-                const float yp = (A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, tx, uy + o, uz) : sample_volume<SS>(sdf, texW, A.res, A.res, tx, uy + o, uz));
-                const float yn = (A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, tx, uy - o, uz) : sample_volume<SS>(sdf, texW, A.res, A.res, tx, uy - o, uz));
-                const float zp = (A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, tx, uy, uz + o) : sample_volume<SS>(sdf, texW, A.res, A.res, tx, uy, uz + o));
-                const float zn = (A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, tx, uy, uz - o) : sample_volume<SS>(sdf, texW, A.res, A.res, tx, uy, uz - o));
+                auto nl = [&](float u_, float v_, float w_) {
+                    if (A.refine) return refine_lookup(A, texW, A.res, A.res, u_, v_, w_);
+                    return A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, u_, v_, w_, A.brickScale)
+                                        : sample_volume<SS>(sdf, texW, A.res, A.res, u_, v_, w_);
+                };
+                const float xp = nl(tx + o, uy, uz), xn = nl(tx - o, uy, uz);
+                const float yp = nl(tx, uy + o, uz), yn = nl(tx, uy - o, uz);
+                const float zp = nl(tx, uy, uz + o), zn = nl(tx, uy, uz - o);
                 const f3 n = normalize3(mk(xp - xn, yp - yn, zp - zn));
                 if (COUNT) c.normals++;
                 const f3 hp = origin + dir * hitTime;
@@ -490,6 +542,8 @@ template <class SS, class SR, bool COUNT>
 static int dispatch_color(const TraceArgs& A, const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at,
                           const LuxgiSkybox* sky, const float* off, const LuxgiRayBuffers* rays, uint8_t* fl,
                           const TraceWorkspace& w, unsigned long long* cnt, hipStream_t st) {
+    if (at->layout == LUXGI_LAYOUT_BRICK4)
+        return launch_trace<SS, PackedAtlas, SR, COUNT>(A, vol, at, sky, off, rays, fl, w, cnt, st);
     switch (at->colorStorage) {
         case LUXGI_STORAGE_FP32: return launch_trace<SS, float, SR, COUNT>(A, vol, at, sky, off, rays, fl, w, cnt, st);
         case LUXGI_STORAGE_FP16: return launch_trace<SS, __half, SR, COUNT>(A, vol, at, sky, off, rays, fl, w, cnt, st);
@@ -505,8 +559,14 @@ static int trace_entry(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, c
                        size_t workspaceBytes, unsigned long long* counters, void* stream) {
     if (!u || !g || !vol || !at || !p || !rays) return LUXGI_ERR_INVALID_ARGUMENT;
     if (!vol->sdf || !vol->mip || !rays->radiance || !rays->directionDistance) return LUXGI_ERR_INVALID_ARGUMENT;
-    if (!at->chunks || !at->culledObjects || !at->objects || !at->tiles || !at->color || !at->depth)
+    if (!at->chunks || !at->culledObjects || !at->objects || !at->tiles || !at->color) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (at->layout == LUXGI_LAYOUT_LINEAR) {
+        if (!at->depth) return LUXGI_ERR_INVALID_ARGUMENT;
+    } else if (at->layout == LUXGI_LAYOUT_BRICK4) {
+        if (at->colorStorage != LUXGI_STORAGE_UNORM8 || (at->data.resolution & 3u)) return LUXGI_ERR_INVALID_ARGUMENT;
+    } else {
         return LUXGI_ERR_INVALID_ARGUMENT;
+    }
     if (g->cascadesCount < 1 || g->cascadesCount > 4) return LUXGI_ERR_INVALID_ARGUMENT;
     if (vol->resolution <= 0 || vol->mipResolution <= 0) return LUXGI_ERR_INVALID_ARGUMENT;
     if (at->data.resolution == 0 || at->objectCapacity < 0) return LUXGI_ERR_INVALID_ARGUMENT;
@@ -527,18 +587,28 @@ static int trace_entry(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, c
     A.rayWidth = rays->width;
     A.hitEps = p->hitEpsilon;
     A.fineLayout = vol->layout;
+    A.brickScale = vol->brickScale;
+    A.refine = vol->refine;
+    A.refineLayout = vol->refineLayout;
+    if (vol->brickScale && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (vol->refine && ((vol->refineLayout != LUXGI_LAYOUT_LINEAR && vol->refineLayout != LUXGI_LAYOUT_BRICK4) ||
+                        (vol->refineLayout == LUXGI_LAYOUT_BRICK4 && (vol->resolution & 3))))
+        return LUXGI_ERR_INVALID_ARGUMENT;
     if (vol->layout != LUXGI_LAYOUT_LINEAR && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
-    if (vol->guard && ((vol->resolution & 3) || ((vol->resolution * (int)g->cascadesCount) & 3)))
-        return LUXGI_ERR_INVALID_ARGUMENT;
-    if (vol->layout == LUXGI_LAYOUT_BRICK4 &&
-        (vol->storage != LUXGI_STORAGE_FP32 || (vol->resolution & 3) || ((vol->resolution * (int)g->cascadesCount) & 3)))
-        return LUXGI_ERR_INVALID_ARGUMENT;
+    // the guard bounds FP32 voxel values; FP16 storage would round below them
+    if (vol->guard && ((vol->resolution & 3) || vol->storage != LUXGI_STORAGE_FP32)) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (vol->layout == LUXGI_LAYOUT_BRICK4 && (vol->resolution & 3)) return LUXGI_ERR_INVALID_ARGUMENT;
     if (A.P <= 0 || A.R <= 0 || A.R > rays->width || rays->height < A.P) return LUXGI_ERR_INVALID_ARGUMENT;
     if ((long long)A.P * A.R >= (1ll << 31)) return LUXGI_ERR_UNSUPPORTED;
     {  // the samplers address the volumes with 32-bit byte offsets
         const size_t es = vol->storage == LUXGI_STORAGE_FP16 ? 2 : 4;
         const size_t C = g->cascadesCount, r = (size_t)vol->resolution, m = (size_t)vol->mipResolution;
-        if (C * r * r * r * es >= (1ull << 32) || C * m * m * m * es >= (1ull << 32)) return LUXGI_ERR_UNSUPPORTED;
+        const size_t fine = vol->layout == LUXGI_LAYOUT_BRICK4 ? C * (r / 4) * (r / 4) * (r / 4) * kBrickElems * es
+                                                               : C * r * r * r * es;
+        const size_t ref = vol->refineLayout == LUXGI_LAYOUT_BRICK4 ? C * (r / 4) * (r / 4) * (r / 4) * kBrickElems * 4
+                                                                    : C * r * r * r * 4;
+        if (fine >= (1ull << 32) || C * m * m * m * es >= (1ull << 32) || (vol->refine && ref >= (1ull << 32)))
+            return LUXGI_ERR_UNSUPPORTED;
     }
     const size_t need = luxgi_ddgi_trace_workspace_size(u, at->objectCapacity);
     if (!workspace || workspaceBytes < need) return LUXGI_ERR_WORKSPACE_TOO_SMALL;
@@ -575,4 +645,23 @@ extern "C" int luxgi_ddgi_trace_sdf_count(const LuxgiDDGIUniform* u, const Luxgi
     if (!counters) return LUXGI_ERR_INVALID_ARGUMENT;
     return trace_entry<true>(u, g, vol, at, sky, p, probeOffsets, rays, rayFlags, workspace, workspaceBytes, counters,
                              stream);
+}
+
+namespace luxgi {
+__global__ void atlas_pack_kernel(const uint32_t* __restrict__ color, const float* __restrict__ depth, int res,
+                                  uint2* __restrict__ packed) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= res) return;
+    const size_t e = (size_t)j * res + i;
+    packed[atlas_block_index(i, j, res)] = make_uint2(color[e], __float_as_uint(depth[e]));
+}
+}  // namespace luxgi
+
+extern "C" int luxgi_atlas_pack(const uint32_t* color, const float* depth, int32_t res, void* packed, void* stream) {
+    if (!color || !depth || !packed || res <= 0 || (res & 3) || (size_t)res * res * 8 >= (1ull << 32))
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(atlas_pack_kernel, dim3((res + 255) / 256, res), dim3(256), 0, as_stream(stream), color, depth,
+                       res, static_cast<uint2*>(packed));
+    LUXGI_LAUNCH_CHECK();
+    return LUXGI_OK;
 }

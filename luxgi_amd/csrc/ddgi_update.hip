@@ -33,7 +33,7 @@ struct UpdateArgs {
     int ni, nd;
     int Wi, Wd;
     float maxDistance, sharpness, hysteresis, invGamma;
-    int firstFrame, doIrr, doDepth, fuseBorder, forcePerProbe;
+    int firstFrame, doIrr, doDepth, fuseBorder, forcePerProbe, half;
 };
 
 // Shared weights W[r][t] (thresholded at FLT_EPS as ProbeUpdate.glsl:93) and
@@ -483,6 +483,51 @@ __device__ __forceinline__ void mfma_store_chunk(const UpdateArgs& A, int r0, in
     st.bi[rr][g * 3 + 2] = f.rc.z;
 }
 
+// Writes the kMG finished interiors staged in LDS as full bordered tiles (the border copy of
+// BorderUpdate.glsl when fuseBorder), blended with the previous atlas (ProbeUpdate.glsl:144-145).
+template <class SA>
+__device__ __forceinline__ void write_tiles(const UpdateArgs& A, const MfmaOut& o, int p0, const SA* __restrict__ prevIrr,
+                                            const SA* __restrict__ prevDepth, SA* __restrict__ outIrr,
+                                            SA* __restrict__ outDepth) {
+    const int tid = threadIdx.x;
+    const int pxy = A.px * A.py;
+    for (int map = 0; map < 2; ++map) {
+        if ((map == 0 && !A.doIrr) || (map == 1 && !A.doDepth)) continue;
+        const int n = map == 0 ? A.ni : A.nd, pw = n + 2;
+        const int total = kMG * pw * pw;
+        for (int i = tid; i < total; i += 256) {
+            const int bx = i % pw, g = (i / pw) % kMG, by = i / (pw * kMG);
+            const int p = p0 + g;
+            if (p >= A.P) continue;
+            const bool interior = bx >= 1 && bx <= n && by >= 1 && by <= n;
+            if (!interior && !A.fuseBorder) continue;
+            int sx, sy;
+            border_source(bx, by, n, sx, sy);
+            const int src = (sy - 1) * n + (sx - 1);
+            const int ox = (p % pxy) * pw + 1, oy = (p / pxy) * pw + 1;
+            if (map == 0) {
+                float4 v = o.irr[g][src];
+                v.w = 1.0f;
+                if (!A.firstFrame) {
+                    const float4 pv = Texel<SA>::ld4(prevIrr, (size_t)(oy + sy) * A.Wi + ox + sx);
+                    v.x = mixf(v.x, pv.x, A.hysteresis);
+                    v.y = mixf(v.y, pv.y, A.hysteresis);
+                    v.z = mixf(v.z, pv.z, A.hysteresis);
+                }
+                Texel<SA>::st4(outIrr, (size_t)(oy + by) * A.Wi + ox + bx, v);
+            } else {
+                float2 v = o.dep[g][src];
+                if (!A.firstFrame) {
+                    const float2 pv = Texel<SA>::ld2(prevDepth, (size_t)(oy + sy) * A.Wd + ox + sx);
+                    v.x = mixf(v.x, pv.x, A.hysteresis);
+                    v.y = mixf(v.y, pv.y, A.hysteresis);
+                }
+                Texel<SA>::st2(outDepth, (size_t)(oy + by) * A.Wd + ox + bx, v);
+            }
+        }
+    }
+}
+
 template <class SR, class SA>
 __global__ __launch_bounds__(256) void probe_update_mfma_kernel(
     UpdateArgs A, const SR* __restrict__ radiance, const SR* __restrict__ dirDist, const SA* __restrict__ prevIrr,
@@ -576,42 +621,154 @@ __global__ __launch_bounds__(256) void probe_update_mfma_kernel(
     }
     __syncthreads();
     if (tid < kMG && p0 + tid < A.P) mismatch[p0 + tid] = (uint8_t)sMis[tid];
-    const int pxy = A.px * A.py;
-    for (int map = 0; map < 2; ++map) {
-        if ((map == 0 && !A.doIrr) || (map == 1 && !A.doDepth)) continue;
-        const int n = map == 0 ? A.ni : A.nd, pw = n + 2;
-        const int total = kMG * pw * pw;
-        for (int i = tid; i < total; i += 256) {
-            const int bx = i % pw, g = (i / pw) % kMG, by = i / (pw * kMG);
-            const int p = p0 + g;
-            if (p >= A.P) continue;
-            const bool interior = bx >= 1 && bx <= n && by >= 1 && by <= n;
-            if (!interior && !A.fuseBorder) continue;
-            int sx, sy;
-            border_source(bx, by, n, sx, sy);
-            const int src = (sy - 1) * n + (sx - 1);
-            const int ox = (p % pxy) * pw + 1, oy = (p / pxy) * pw + 1;
-            if (map == 0) {
-                float4 v = lds.out.irr[g][src];
-                v.w = 1.0f;
-                if (!A.firstFrame) {
-                    const float4 pv = Texel<SA>::ld4(prevIrr, (size_t)(oy + sy) * A.Wi + ox + sx);
-                    v.x = mixf(v.x, pv.x, A.hysteresis);
-                    v.y = mixf(v.y, pv.y, A.hysteresis);
-                    v.z = mixf(v.z, pv.z, A.hysteresis);
+    write_tiles<SA>(A, lds.out, p0, prevIrr, prevDepth, outIrr, outDepth);
+}
+
+// ---------------------------------------------------------------------------
+// Reduced-precision blend (LUXGI_UPDATE_HALF): the same contraction on the FP16 MFMA
+// v_mfma_f32_16x16x32_f16 with FP32 accumulation. Operand lanes (one k-step = 32 rays):
+// A[m][k] = W at texel m, ray k: lane l holds k = 8(l/16) .. 8(l/16)+7 of row l%16, one 16-byte
+// load from the texel-major FP16 weights; B[k][n] = ray data of probe n: lane l holds rays
+// 8(l/16) .. +7 of probe l%16; D[4(l/16)+v][l%16] as for the FP32 form. W is rounded to FP16 and
+// normalised by the sum of the rounded weights (a constant input stays constant); depth moments
+// enter as r/maxDistance and (r/maxDistance)^2, which keeps them in FP16's normal range.
+// ---------------------------------------------------------------------------
+typedef _Float16 half8v __attribute__((ext_vector_type(8)));
+constexpr int kHK = 32;  // rays per k-step
+
+__global__ __launch_bounds__(256) void weights_half_kernel(const float* __restrict__ dirs0, int R, int Rpad, int n,
+                                                           int tpad, int depth, float sharpness,
+                                                           _Float16* __restrict__ Wh, float* __restrict__ Sh) {
+    __shared__ float part[16][17];
+    const int tl = threadIdx.x & 15, j = threadIdx.x >> 4;
+    const int t = blockIdx.x * 16 + tl;
+    const bool valid = t < n * n;
+    const f3 td = valid ? texel_direction(t % n, t / n, n) : mk(0.f, 0.f, 1.f);
+    float s = 0.0f;
+    for (int r = j; r < Rpad; r += 16) {
+        float w = 0.0f;
+        if (valid && r < R) {
+            w = fmaxf(0.0f, dot3(td, mk(dirs0[r * 4 + 0], dirs0[r * 4 + 1], dirs0[r * 4 + 2])));
+            if (depth) w = powf(w, sharpness);
+            if (!(w >= kFltEps)) w = 0.0f;
+        }
+        const _Float16 h = (_Float16)w;
+        Wh[(size_t)t * Rpad + r] = h;
+        s += (float)h;
+    }
+    part[j][tl] = s;
+    __syncthreads();
+    if (j == 0) {
+        float tot = 0.0f;
+        for (int k = 0; k < 16; ++k) tot += part[k][tl];
+        Sh[t] = tot;
+    }
+}
+
+template <class SR, class SA>
+__global__ __launch_bounds__(256) void probe_update_mfma16_kernel(
+    UpdateArgs A, const SR* __restrict__ radiance, const SR* __restrict__ dirDist, const SA* __restrict__ prevIrr,
+    const SA* __restrict__ prevDepth, SA* __restrict__ outIrr, SA* __restrict__ outDepth, const float* __restrict__ dirs0,
+    const _Float16* __restrict__ Wih, const float* __restrict__ Sih, const _Float16* __restrict__ Wdh,
+    const float* __restrict__ Sdh, uint8_t* __restrict__ mismatch, int tpadI, int tpadD, int Rpad) {
+    __shared__ MfmaOut out;
+    __shared__ int sMis[kMG];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int col = lane & 15, kq = lane >> 4;
+    const int p0 = blockIdx.x * kMG, p = p0 + col;
+    const int MtD = A.doDepth ? tpadD / 16 : 0, MtI = A.doIrr ? tpadI / 16 : 0;
+    const float invMax = 1.0f / A.maxDistance;
+    if (tid < kMG) sMis[tid] = 0;
+    __syncthreads();
+    floatx4 accD[4][2], accI[3];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) accD[j][0] = accD[j][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) accI[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    int mis = 0;
+    for (int k0 = 0; k0 < Rpad; k0 += kHK) {
+        const int r0 = k0 + 8 * kq;
+        half8v bR, bR2, bCr, bCg, bCb;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int r = r0 + j;
+            float m1 = 0.f, m2 = 0.f, cr = 0.f, cg = 0.f, cb = 0.f;
+            if (p < A.P && r < A.R) {
+                const size_t e = (size_t)p * A.rayWidth + r;
+                if (A.doDepth) {
+                    const float4 dd = Texel<SR>::ld4(dirDist, e);
+                    float dist = fminf(A.maxDistance, dd.w - 0.01f);  // ProbeUpdate.glsl:75-78
+                    if (dist == -1.0f) dist = A.maxDistance;
+                    m1 = dist * invMax;
+                    m2 = m1 * m1;
+                    if (wave == 0 && (dd.x != dirs0[r * 4 + 0] || dd.y != dirs0[r * 4 + 1] || dd.z != dirs0[r * 4 + 2]))
+                        mis = 1;
                 }
-                Texel<SA>::st4(outIrr, (size_t)(oy + by) * A.Wi + ox + bx, v);
-            } else {
-                float2 v = lds.out.dep[g][src];
-                if (!A.firstFrame) {
-                    const float2 pv = Texel<SA>::ld2(prevDepth, (size_t)(oy + sy) * A.Wd + ox + sx);
-                    v.x = mixf(v.x, pv.x, A.hysteresis);
-                    v.y = mixf(v.y, pv.y, A.hysteresis);
+                if (A.doIrr) {
+                    const float4 c = Texel<SR>::ld4(radiance, e);
+                    cr = c.x;
+                    cg = c.y;
+                    cb = c.z;
+                    if (!A.doDepth && wave == 0) {
+                        const float4 dd = Texel<SR>::ld4(dirDist, e);
+                        if (dd.x != dirs0[r * 4 + 0] || dd.y != dirs0[r * 4 + 1] || dd.z != dirs0[r * 4 + 2]) mis = 1;
+                    }
                 }
-                Texel<SA>::st2(outDepth, (size_t)(oy + by) * A.Wd + ox + bx, v);
+            }
+            bR[j] = (_Float16)m1;
+            bR2[j] = (_Float16)m2;
+            bCr[j] = (_Float16)cr;
+            bCg[j] = (_Float16)cg;
+            bCb[j] = (_Float16)cb;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int m = wave + 4 * j;
+            if (m < MtD) {
+                const half8v a = *reinterpret_cast<const half8v*>(Wdh + (size_t)(m * 16 + col) * Rpad + r0);
+                accD[j][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bR, accD[j][0], 0, 0, 0);
+                accD[j][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bR2, accD[j][1], 0, 0, 0);
             }
         }
+        if (wave < MtI) {
+            const half8v a = *reinterpret_cast<const half8v*>(Wih + (size_t)(wave * 16 + col) * Rpad + r0);
+            accI[0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bCr, accI[0], 0, 0, 0);
+            accI[1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bCg, accI[1], 0, 0, 0);
+            accI[2] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bCb, accI[2], 0, 0, 0);
+        }
     }
+    if (mis) atomicOr(&sMis[col], 1);
+    // interiors: normalise by 1/(2 S), undo the depth scaling, gamma-encode (ProbeUpdate.glsl:133-141)
+    const int Td = A.nd * A.nd, Ti = A.ni * A.ni;
+    const int rowq = kq * 4;
+    const float mScale[2] = {A.maxDistance, A.maxDistance * A.maxDistance};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int m = wave + 4 * j;
+        if (m >= MtD) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int t = m * 16 + rowq + v;
+            if (t >= Td) continue;
+            const float sm = Sdh[t];
+            const float sc = sm > kFltEps ? 1.0f / (2.0f * sm) : 1.0f;
+            out.dep[col][t] = make_float2(accD[j][0][v] * sc * mScale[0], accD[j][1][v] * sc * mScale[1]);
+        }
+    }
+    if (wave < MtI) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int t = wave * 16 + rowq + v;
+            if (t >= Ti) continue;
+            const float sm = Sih[t];
+            const float sc = sm > kFltEps ? 1.0f / (2.0f * sm) : 1.0f;
+            out.irr[col][t] = make_float4(powf(accI[0][v] * sc, A.invGamma), powf(accI[1][v] * sc, A.invGamma),
+                                          powf(accI[2][v] * sc, A.invGamma), 1.0f);
+        }
+    }
+    __syncthreads();
+    if (tid < kMG && p0 + tid < A.P) mismatch[p0 + tid] = (uint8_t)sMis[tid];
+    write_tiles<SA>(A, out, p0, prevIrr, prevDepth, outIrr, outDepth);
 }
 
 static int check_uniform(const LuxgiDDGIUniform* u) {
@@ -643,11 +800,16 @@ extern "C" int luxgi_ddgi_fill_atlas_sizes(LuxgiDDGIUniform* u) {
     return LUXGI_OK;
 }
 
-// workspace: row-0 directions, W and S for both maps (16-row padded), mismatch flags
+// workspace: row-0 directions, W and S for both maps (16-row padded), mismatch flags, and the
+// texel-major FP16 weights with the sums of their rounded values (reduced-precision blend)
 struct UpdateWorkspace {
     float *dirs0, *Wi, *Si, *Wd, *Sd;
     uint8_t* mismatch;
+    _Float16 *Wih, *Wdh;
+    float *Sih, *Sdh;
 };
+
+static int pad32(int r) { return (r + 31) & ~31; }
 
 static int pad16(int t) { return (t + 15) & ~15; }
 
@@ -668,7 +830,16 @@ static size_t update_workspace_layout(const LuxgiDDGIUniform* u, char* base, Upd
     char* wd = take(R * Td * 4);
     char* sd = take(Td * 4);
     char* mm = take(P);
+    const size_t Rp = (size_t)pad32((int)R);
+    char* wih = take(Rp * Ti * 2);
+    char* sih = take(Ti * 4);
+    char* wdh = take(Rp * Td * 2);
+    char* sdh = take(Td * 4);
     if (w) {
+        w->Wih = reinterpret_cast<_Float16*>(wih);
+        w->Sih = reinterpret_cast<float*>(sih);
+        w->Wdh = reinterpret_cast<_Float16*>(wdh);
+        w->Sdh = reinterpret_cast<float*>(sdh);
         w->dirs0 = reinterpret_cast<float*>(d0);
         w->Wi = reinterpret_cast<float*>(wi);
         w->Si = reinterpret_cast<float*>(si);
@@ -698,18 +869,34 @@ static int launch_update(const UpdateArgs& A, const LuxgiRayBuffers* rays, const
         const int tpI = pad16(A.ni * A.ni), tpD = pad16(A.nd * A.nd);
         hipLaunchKernelGGL(row0_kernel<SR>, dim3((A.R + 255) / 256), dim3(256), 0, st, dd, A.R, w.dirs0);
         LUXGI_LAUNCH_CHECK();
-        if (A.doIrr) {
-            hipLaunchKernelGGL(weights_kernel_tiled, dim3(tpI / 16), dim3(256), 0, st, w.dirs0, A.R, A.ni, tpI, 0,
-                               A.sharpness, w.Wi, w.Si);
-            LUXGI_LAUNCH_CHECK();
+        if (A.half) {
+            const int Rp = pad32(A.R);
+            if (A.doIrr) {
+                hipLaunchKernelGGL(weights_half_kernel, dim3(tpI / 16), dim3(256), 0, st, w.dirs0, A.R, Rp, A.ni, tpI, 0,
+                                   A.sharpness, w.Wih, w.Sih);
+                LUXGI_LAUNCH_CHECK();
+            }
+            if (A.doDepth) {
+                hipLaunchKernelGGL(weights_half_kernel, dim3(tpD / 16), dim3(256), 0, st, w.dirs0, A.R, Rp, A.nd, tpD, 1,
+                                   A.sharpness, w.Wdh, w.Sdh);
+                LUXGI_LAUNCH_CHECK();
+            }
+            hipLaunchKernelGGL((probe_update_mfma16_kernel<SR, SA>), dim3((A.P + kMG - 1) / kMG), dim3(256), 0, st, A,
+                               rad, dd, pI, pD, oI, oD, w.dirs0, w.Wih, w.Sih, w.Wdh, w.Sdh, w.mismatch, tpI, tpD, Rp);
+        } else {
+            if (A.doIrr) {
+                hipLaunchKernelGGL(weights_kernel_tiled, dim3(tpI / 16), dim3(256), 0, st, w.dirs0, A.R, A.ni, tpI, 0,
+                                   A.sharpness, w.Wi, w.Si);
+                LUXGI_LAUNCH_CHECK();
+            }
+            if (A.doDepth) {
+                hipLaunchKernelGGL(weights_kernel_tiled, dim3(tpD / 16), dim3(256), 0, st, w.dirs0, A.R, A.nd, tpD, 1,
+                                   A.sharpness, w.Wd, w.Sd);
+                LUXGI_LAUNCH_CHECK();
+            }
+            hipLaunchKernelGGL((probe_update_mfma_kernel<SR, SA>), dim3((A.P + kMG - 1) / kMG), dim3(256), 0, st, A,
+                               rad, dd, pI, pD, oI, oD, w.dirs0, w.Wi, w.Si, w.Wd, w.Sd, w.mismatch, tpI, tpD);
         }
-        if (A.doDepth) {
-            hipLaunchKernelGGL(weights_kernel_tiled, dim3(tpD / 16), dim3(256), 0, st, w.dirs0, A.R, A.nd, tpD, 1,
-                               A.sharpness, w.Wd, w.Sd);
-            LUXGI_LAUNCH_CHECK();
-        }
-        hipLaunchKernelGGL((probe_update_mfma_kernel<SR, SA>), dim3((A.P + kMG - 1) / kMG), dim3(256), 0, st, A, rad, dd,
-                           pI, pD, oI, oD, w.dirs0, w.Wi, w.Si, w.Wd, w.Sd, w.mismatch, tpI, tpD);
         LUXGI_LAUNCH_CHECK();
         // probes whose directions differ from row 0: reference arithmetic with their own directions
         UpdateArgs F = A;
@@ -766,6 +953,8 @@ extern "C" int luxgi_ddgi_probe_update(const LuxgiDDGIUniform* u, const LuxgiRay
     A.doDepth = (flags & LUXGI_UPDATE_DEPTH) != 0;
     A.fuseBorder = (flags & LUXGI_UPDATE_FUSE_BORDER) != 0;
     A.forcePerProbe = (flags & LUXGI_UPDATE_PER_PROBE) != 0;
+    A.half = (flags & LUXGI_UPDATE_HALF) != 0;
+    if (A.half && !(u->maxDistance > 0.0f)) return LUXGI_ERR_INVALID_ARGUMENT;
     if (A.R <= 0 || A.R > rays->width || rays->height < A.P) return LUXGI_ERR_INVALID_ARGUMENT;
     if (A.ni > kMaxSide || A.nd > kMaxSide) return LUXGI_ERR_UNSUPPORTED;
     if (!A.doIrr && !A.doDepth) return LUXGI_OK;

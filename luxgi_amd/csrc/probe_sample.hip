@@ -16,6 +16,17 @@ namespace luxgi {
 
 constexpr float kPiF = 3.1415926535897932384626433832795f;  // Common/Math.glsl:7
 
+// GLSL normalize / pow as the hardware evaluates them: reciprocal square root (v_rsq_f32) and
+// 2^(y log2 x) (v_log_f32, v_exp_f32), each within a few ulp of the correctly rounded value --
+// inside the precision GLSL grants these built-ins.
+__device__ __forceinline__ f3 fast_normalize(f3 a) {
+    const float r = __builtin_amdgcn_rsqf(dot3(a, a));
+    return mk(a.x * r, a.y * r, a.z * r);
+}
+__device__ __forceinline__ float fast_pow(float x, float y) {  // x >= 0
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+
 __device__ __forceinline__ float2 oct_encode(f3 v) {
     const float l1 = fabsf(v.x) + fabsf(v.y) + fabsf(v.z);
     float2 r = make_float2(v.x * (1.0f / l1), v.y * (1.0f / l1));
@@ -23,15 +34,21 @@ __device__ __forceinline__ float2 oct_encode(f3 v) {
     return r;
 }
 
-// textureCoordFromDirection: normalized atlas coordinate of direction `dir` in probe `probe`'s tile
-__device__ __forceinline__ float2 atlas_coord(f3 dir, int probe, int width, int height, int side) {
-    const float2 oc = oct_encode(normalize3(dir));
+// textureCoordFromDirection: normalized atlas coordinate of direction `dir` in probe `probe`'s tile.
+// octEncode divides by the L1 norm, so `dir` need not be normalized first.
+struct AtlasGeom {
+    int width, height, side, perRow;
+    float invW, invH;
+};
+__device__ __forceinline__ float2 atlas_coord(f3 dir, int probe, const AtlasGeom& g) {
+    const float2 oc = oct_encode(dir);
     const float ox = (oc.x + 1.0f) * 0.5f, oy = (oc.y + 1.0f) * 0.5f;
-    const float pws = float(side) + 2.0f;
-    const int perRow = (width - 2) / int(pws);
-    const float tlx = float(probe % perRow) * pws + 2.0f, tly = float(probe / perRow) * pws + 2.0f;
-    return make_float2(tlx / float(width) + (ox * float(side)) / float(width),
-                       tly / float(height) + (oy * float(side)) / float(height));
+    const float pws = float(g.side) + 2.0f;
+    const float tlx = float(probe % g.perRow) * pws + 2.0f, tly = float(probe / g.perRow) * pws + 2.0f;
+    return make_float2((tlx + ox * float(g.side)) * g.invW, (tly + oy * float(g.side)) * g.invH);
+}
+__host__ __device__ inline AtlasGeom atlas_geom(int width, int height, int side) {
+    return AtlasGeom{width, height, side, (width - 2) / (side + 2), 1.0f / float(width), 1.0f / float(height)};
 }
 
 __device__ __forceinline__ int wrap_index(int i, int n) {
@@ -70,13 +87,15 @@ __device__ __forceinline__ float2 bilinear2(const S* __restrict__ img, int W, in
 struct SampleArgs {
     LuxgiDDGIUniform u;
     LuxgiSampleProbeParams pc;
+    AtlasGeom irr, dep;
     int W, H;
 };
 
 // sampleIrradiance (DDGICommon.glsl:163-233)
 template <class SA>
-__device__ f3 sample_irradiance(const LuxgiDDGIUniform& u, f3 P, f3 N, f3 Wo, const SA* __restrict__ irr,
+__device__ f3 sample_irradiance(const SampleArgs& A, f3 P, f3 N, f3 Wo, const SA* __restrict__ irr,
                                 const SA* __restrict__ dep) {
+    const LuxgiDDGIUniform& u = A.u;
     const f3 start = mk(u.startPosition[0], u.startPosition[1], u.startPosition[2]);
     const f3 stp = mk(u.step[0], u.step[1], u.step[2]);
     const int bx = clampi(int((P.x - start.x) / stp.x), 0, u.probeCounts[0] - 1);
@@ -87,7 +106,6 @@ __device__ f3 sample_irradiance(const LuxgiDDGIUniform& u, f3 P, f3 N, f3 Wo, co
                         clampf((P.z - basePos.z) / stp.z, 0.0f, 1.0f));
     const float ge = u.ddgiGamma * 0.5f;
     const f3 vBias = (N + Wo * 3.0f) * u.normalBias;
-    const f3 nN = normalize3(N);
     f3 sum = mk(0.f, 0.f, 0.f);
     float sumW = 0.0f;
 #pragma unroll 2
@@ -99,16 +117,15 @@ __device__ f3 sample_irradiance(const LuxgiDDGIUniform& u, f3 P, f3 N, f3 Wo, co
         const float tri = (ox ? alpha.x : 1.0f - alpha.x) * (oy ? alpha.y : 1.0f - alpha.y) *
                           (oz ? alpha.z : 1.0f - alpha.z);
         float w = 1.0f;
-        const f3 dirToProbe = normalize3(probePos - P);
+        const f3 dirToProbe = fast_normalize(probePos - P);
         const float sb = fmaxf(0.0001f, (dot3(dirToProbe, N) + 1.0f) * 0.5f);
         w *= sb * sb + 0.2f;
         const int probeIdx = gx + gy * u.probeCounts[0] + gz * u.probeCounts[0] * u.probeCounts[1];
         const f3 probeToPoint = P - probePos + vBias;
-        const f3 dir = normalize3(probeToPoint * -1.0f);
-        float2 tc = atlas_coord(dir * -1.0f, probeIdx, u.depthTextureWidth, u.depthTextureHeight,
-                                u.depthProbeSideLength);
+        // textureCoordFromDirection(-normalize(-probeToPoint)) == the direction of probeToPoint
+        float2 tc = atlas_coord(probeToPoint, probeIdx, A.dep);
         const float dist = len3(probeToPoint);
-        const float2 dm = bilinear2<SA>(dep, u.depthTextureWidth, u.depthTextureHeight, tc.x, tc.y);
+        const float2 dm = bilinear2<SA>(dep, A.dep.width, A.dep.height, tc.x, tc.y);
         const float mean = dm.x;
         const float variance = fabsf(dm.x * dm.x - dm.y);
         const float md = fmaxf(dist - mean, 0.0f);
@@ -116,10 +133,9 @@ __device__ f3 sample_irradiance(const LuxgiDDGIUniform& u, f3 P, f3 N, f3 Wo, co
         cheb = fmaxf(cheb * cheb * cheb, 0.0f);
         w *= (dist <= mean) ? 1.0f : cheb;
         w = fmaxf(0.000001f, w);
-        tc = atlas_coord(nN, probeIdx, u.irradianceTextureWidth, u.irradianceTextureHeight,
-                         u.irradianceProbeSideLength);
-        const float4 ir = bilinear4<SA>(irr, u.irradianceTextureWidth, u.irradianceTextureHeight, tc.x, tc.y);
-        const f3 pi = mk(powf(ir.x, ge), powf(ir.y, ge), powf(ir.z, ge));
+        tc = atlas_coord(N, probeIdx, A.irr);
+        const float4 ir = bilinear4<SA>(irr, A.irr.width, A.irr.height, tc.x, tc.y);
+        const f3 pi = mk(fast_pow(ir.x, ge), fast_pow(ir.y, ge), fast_pow(ir.z, ge));
         if (w < 0.2f) w *= w * w * (1.0f / (0.2f * 0.2f));
         w *= tri;
         sum = sum + pi * w;
@@ -144,8 +160,8 @@ __global__ __launch_bounds__(256) void sample_probes_kernel(SampleArgs A, const 
         const f3 P = world_from_depth(tu, tv, d, A.pc.viewProjInv);
         const float2 on = reinterpret_cast<const float2*>(normals)[2 * e];
         const f3 N = oct_decode(on.x, on.y);
-        const f3 Wo = normalize3(mk(A.pc.cameraPosition[0], A.pc.cameraPosition[1], A.pc.cameraPosition[2]) - P);
-        const f3 ir = sample_irradiance<SA>(A.u, P, N, Wo, irr, dep);
+        const f3 Wo = fast_normalize(mk(A.pc.cameraPosition[0], A.pc.cameraPosition[1], A.pc.cameraPosition[2]) - P);
+        const f3 ir = sample_irradiance<SA>(A, P, N, Wo, irr, dep);
         o = make_float4(ir.x, ir.y, ir.z, 1.0f);
     }
     Texel<SO>::st4(out, e, o);
@@ -185,6 +201,8 @@ extern "C" int luxgi_ddgi_sample_probes(const LuxgiDDGIUniform* u, const LuxgiPr
     SampleArgs A;
     A.u = *u;
     A.pc = *params;
+    A.irr = atlas_geom(u->irradianceTextureWidth, u->irradianceTextureHeight, u->irradianceProbeSideLength);
+    A.dep = atlas_geom(u->depthTextureWidth, u->depthTextureHeight, u->depthProbeSideLength);
     A.W = W;
     A.H = H;
     hipStream_t st = as_stream(stream);

@@ -38,11 +38,36 @@ __global__ void mip_pass_kernel(const float* __restrict__ src, int srcW, int src
     dst[((size_t)z * dstH + y) * dstW + x + mipOffsetX] = best;
 }
 
-__global__ void to_bricks_kernel(const float* __restrict__ src, int W, int H, int D, float* __restrict__ dst) {
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y = blockIdx.y, z = blockIdx.z;
-    if (x >= W) return;
-    dst[brick4_index(x, y, z, W >> 2, H >> 2)] = src[((size_t)z * H + y) * W + x];
+// one block per brick: the 5^3 apron block (clamped to the volume) + 3 zero pad elements, optionally
+// divided by the brick's power-of-two scale (smallest 2^e >= max |v| over the block)
+template <class D_>
+__global__ void to_bricks_kernel(const float* __restrict__ src, int W, int H, int D, D_* __restrict__ dst,
+                                 float* __restrict__ scaleOut) {
+    __shared__ float red[kBrickElems / 64];
+    const uint32_t nbx = (uint32_t)W >> 2, nby = (uint32_t)H >> 2;
+    const uint32_t b = blockIdx.x;
+    const int bx = int(b % nbx), by = int((b / nbx) % nby), bz = int(b / (nbx * nby));
+    const uint32_t t = threadIdx.x;  // blockDim = kBrickElems
+    float v = 0.0f;
+    if (t < 125) {
+        const int i = int(t % 5), j = int((t / 5) % 5), k = int(t / 25);
+        const int x = min(4 * bx + i, W - 1), y = min(4 * by + j, H - 1), z = min(4 * bz + k, D - 1);
+        v = src[((size_t)z * H + y) * W + x];
+    }
+    if (scaleOut) {
+        float m = fabsf(v);
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if ((t & 63) == 0) red[t >> 6] = m;
+        __syncthreads();
+        m = fmaxf(red[0], red[1]);
+        // smallest power of two >= m (frexp: m = f * 2^e with f in [0.5, 1))
+        int e = 0;
+        const float f = frexpf(m, &e);
+        const float sc = m > 0.0f ? ldexpf(1.0f, f == 0.5f ? e - 1 : e) : 1.0f;
+        v = v / sc;  // exact: sc is a power of two
+        if (t == 0) scaleOut[b] = sc;
+    }
+    Texel<D_>::st1(dst, (size_t)b * kBrickElems + t, v);
 }
 
 __global__ void guard_kernel(const float* __restrict__ src, int W, int H, int D, float* __restrict__ guard) {
@@ -69,9 +94,25 @@ extern "C" int luxgi_sdf_build_guard(const float* sdf, int32_t W, int32_t H, int
     return LUXGI_OK;
 }
 
-extern "C" int luxgi_sdf_to_bricks(const float* src, int32_t W, int32_t H, int32_t D, float* dst, void* stream) {
-    if (!src || !dst || W <= 0 || H <= 0 || D <= 0 || (W & 3) || (H & 3) || (D & 3)) return LUXGI_ERR_INVALID_ARGUMENT;
-    hipLaunchKernelGGL(to_bricks_kernel, dim3((W + 63) / 64, H, D), dim3(64), 0, as_stream(stream), src, W, H, D, dst);
+extern "C" size_t luxgi_sdf_bricks_bytes(int32_t W, int32_t H, int32_t D, int32_t storage) {
+    if (W <= 0 || H <= 0 || D <= 0 || (W & 3) || (H & 3) || (D & 3)) return 0;
+    if (storage != LUXGI_STORAGE_FP32 && storage != LUXGI_STORAGE_FP16) return 0;
+    const size_t es = storage == LUXGI_STORAGE_FP16 ? 2 : 4;
+    return (size_t)(W >> 2) * (H >> 2) * (D >> 2) * kBrickElems * es;
+}
+
+extern "C" int luxgi_sdf_to_bricks(const float* src, int32_t W, int32_t H, int32_t D, int32_t storage, void* dst,
+                                   float* brickScale, void* stream) {
+    const size_t bytes = luxgi_sdf_bricks_bytes(W, H, D, storage);
+    if (!src || !dst || bytes == 0) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (bytes >= (1ull << 32)) return LUXGI_ERR_UNSUPPORTED;  // the samplers use 32-bit offsets
+    const dim3 grid((unsigned)((size_t)(W >> 2) * (H >> 2) * (D >> 2)));
+    if (storage == LUXGI_STORAGE_FP32)
+        hipLaunchKernelGGL(to_bricks_kernel<float>, grid, dim3(kBrickElems), 0, as_stream(stream), src, W, H, D,
+                           static_cast<float*>(dst), brickScale);
+    else
+        hipLaunchKernelGGL(to_bricks_kernel<__half>, grid, dim3(kBrickElems), 0, as_stream(stream), src, W, H, D,
+                           static_cast<__half*>(dst), brickScale);
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }

@@ -261,19 +261,36 @@ __global__ __launch_bounds__(256) void reflection_reproject_kernel(ReflArgs A, G
     }
 }
 
-// computeEdgeStoppingWeight (EdgeStop.glsl:24-40)
-__device__ __forceinline__ float edge_stop(float cd, float sd, float phiZ, f3 cn, f3 sn, float phiN, float cl, float sl,
-                                           float phiL) {
-    const float wZ = expf(-fabsf(cd - sd) / phiZ);
-    const float wN = powf(clampf(dot3(cn, sn), 0.0f, 1.0f), phiN);
-    const float wL = fabsf(cl - sl) / phiL;
-    return expf(0.0f - fmaxf(wL, 0.0f) - fmaxf(wZ, 0.0f)) * wN;
+// computeEdgeStoppingWeight (EdgeStop.glsl:24-40) with the per-pixel constants folded:
+// invPhiZ = 1/sigmaDepth, invPhiL = 1/phiLuma, and the normal power max(0, n.n')^phiN
+// evaluated by `sqN` squarings when phiN = 2^sqN (the reference's 32), else by powf.
+// exp(x) is evaluated as exp2(x * log2(e)) (v_exp_f32); the relative error stays below 1e-6
+// for the arguments here, well inside the driver-ULP freedom of GLSL exp/pow.
+__device__ __forceinline__ float pow_normal(float c, float phiN, int sqN) {
+    if (sqN < 0) return powf(c, phiN);
+    for (int i = 0; i < sqN; ++i) c *= c;
+    return c;
+}
+__device__ __forceinline__ float edge_stop(float cd, float sd, float invPhiZ, f3 cn, f3 sn, float phiN, int sqN,
+                                           float cl, float sl, float invPhiL) {
+    const float wZ = __expf(-fabsf(cd - sd) * invPhiZ);
+    const float wN = pow_normal(clampf(dot3(cn, sn), 0.0f, 1.0f), phiN, sqN);
+    const float wL = fabsf(cl - sl) * invPhiL;
+    return __expf(0.0f - fmaxf(wL, 0.0f) - fmaxf(wZ, 0.0f)) * wN;
 }
 
 struct AtrousArgs {
     int W, H, tilesX, radius, stepSize;
     float phi, phiNormal, sigmaDepth, power;
+    int sqNormal;  // log2(phiNormal) when it is a power of two in [1, 64], else -1
 };
+
+// squarings that evaluate x^phiNormal exactly as repeated products, or -1
+static inline int normal_squarings(float phiN) {
+    for (int k = 0; k <= 6; ++k)
+        if (phiN == float(1 << k)) return k;
+    return -1;
+}
 
 // Reflection/DenoiseAtrous.comp:63-134 for one filtered pixel. tap(px, py, sc, sn, d) gives the
 // input, decoded normal and linear depth of an in-image pixel. The sums run in one fixed order,
@@ -284,6 +301,7 @@ __device__ __forceinline__ float4 reflection_filter(const AtrousArgs& A, int x, 
     const float kw[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
     const float cl = luminance(c.x, c.y, c.z);
     const float phiC = A.phi * sqrtf(fmaxf(0.0f, var)) + 1e-10f;
+    const float invPhiC = 1.0f / phiC, invPhiZ = 1.0f / A.sigmaDepth;
     float sw = 1.0f;
     float4 s = c;
     for (int yy = -A.radius; yy <= A.radius; ++yy)
@@ -296,7 +314,7 @@ __device__ __forceinline__ float4 reflection_filter(const AtrousArgs& A, int x, 
                 tap(px, py, sc, sn, d);
                 const float k = kw[abs(xx)] * kw[abs(yy)];
                 const float sl = luminance(sc.x, sc.y, sc.z);
-                const float w = edge_stop(cd, d, A.sigmaDepth, cn, sn, A.phiNormal, cl, sl, phiC);
+                const float w = edge_stop(cd, d, invPhiZ, cn, sn, A.phiNormal, A.sqNormal, cl, sl, invPhiC);
                 const float wv = w * k;
                 sw += wv;
                 s.x += wv * sc.x;
@@ -629,6 +647,7 @@ __device__ __forceinline__ float2 shadow_filter(const AtrousArgs& A, int x, int 
                                                 Tap tap) {
     const float kw[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
     const float phiV = A.phi * sqrtf(fmaxf(0.0f, 1e-10f + var));
+    const float invPhiV = 1.0f / phiV, invPhiZ = 1.0f / A.sigmaDepth;
     float sw = 1.0f, s0 = c.x, s1 = c.y;
     for (int yy = -A.radius; yy <= A.radius; ++yy)
         for (int xx = -A.radius; xx <= A.radius; ++xx) {
@@ -639,7 +658,7 @@ __device__ __forceinline__ float2 shadow_filter(const AtrousArgs& A, int x, int 
                 float d;
                 tap(px, py, sv, sn, d);
                 const float k = kw[abs(xx)] * kw[abs(yy)];
-                const float w = edge_stop(cd, d, A.sigmaDepth, cn, sn, A.phiNormal, c.x, sv.x, phiV);
+                const float w = edge_stop(cd, d, invPhiZ, cn, sn, A.phiNormal, A.sqNormal, c.x, sv.x, invPhiV);
                 const float wv = w * k;
                 sw += wv;
                 s0 += wv * sv.x;
@@ -804,7 +823,8 @@ extern "C" int luxgi_svgf_reflection_atrous(const LuxgiReflectionAtrousParams* p
         return LUXGI_ERR_INVALID_ARGUMENT;
     if (p->radius < 0 || p->radius > 2) return LUXGI_ERR_UNSUPPORTED;  // kernelWeights has 3 entries
     if (p->stepSize < 1 || p->stepSize > (1 << 16)) return LUXGI_ERR_INVALID_ARGUMENT;
-    AtrousArgs A{W, H, (W + 7) / 8, p->radius, p->stepSize, p->phiColor, p->phiNormal, p->sigmaDepth, 0.0f};
+    AtrousArgs A{W, H, (W + 7) / 8, p->radius, p->stepSize, p->phiColor, p->phiNormal, p->sigmaDepth, 0.0f,
+                 normal_squarings(p->phiNormal)};
     if (storage != LUXGI_STORAGE_FP32 && storage != LUXGI_STORAGE_FP16) return LUXGI_ERR_UNSUPPORTED;
     hipStream_t st = as_stream(stream);
     const float4* nrm = static_cast<const float4*>(static_cast<const void*>(cur->normals));
@@ -886,7 +906,8 @@ extern "C" int luxgi_svgf_shadow_atrous(const LuxgiShadowAtrousParams* p, int32_
         return LUXGI_ERR_INVALID_ARGUMENT;
     if (p->radius < 0 || p->radius > 2) return LUXGI_ERR_UNSUPPORTED;
     if (p->stepSize < 1 || p->stepSize > (1 << 16)) return LUXGI_ERR_INVALID_ARGUMENT;
-    AtrousArgs A{W, H, (W + 7) / 8, p->radius, p->stepSize, p->phiVisibility, p->phiNormal, p->sigmaDepth, p->power};
+    AtrousArgs A{W, H, (W + 7) / 8, p->radius, p->stepSize, p->phiVisibility, p->phiNormal, p->sigmaDepth, p->power,
+                 normal_squarings(p->phiNormal)};
     if (storage != LUXGI_STORAGE_FP32 && storage != LUXGI_STORAGE_FP16) return LUXGI_ERR_UNSUPPORTED;
     hipStream_t st = as_stream(stream);
     const float4* nrm = static_cast<const float4*>(static_cast<const void*>(cur->normals));

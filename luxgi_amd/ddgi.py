@@ -43,28 +43,51 @@ class DeviceScene:
     """SDF volume + surface atlas uploaded to the GPU (the inputs of the trace)."""
 
     def __init__(self, sdf_data, sdf, mip, atlas, sdf_storage=abi.STORAGE_FP32, sky_faces=None, sky_size=0,
-                 layout=abi.LAYOUT_LINEAR, guard=False):
+                 layout=abi.LAYOUT_LINEAR, guard=False, atlas_layout=abi.LAYOUT_LINEAR, brick_scale=False,
+                 refine=False):
+        """sdf_storage FP16 with refine=True is the reduced-precision march with FP32 hits: the FP32
+        volume (same layout) is kept beside the FP16 one and re-evaluated at every hit."""
         self.sdf_data = sdf_data
         self.res = int(sdf_data.resolution)
         self.mip_res = (self.res + 3) // 4
         sdt = torch.float16 if sdf_storage == abi.STORAGE_FP16 else torch.float32
-        self.sdf = sdf if torch.is_tensor(sdf) else to_device(sdf, sdt)
-        self.mip = mip if torch.is_tensor(mip) else to_device(mip, sdt)
+        # the FP32 volume the stored forms derive from
+        src = sdf.cuda().float() if torch.is_tensor(sdf) else to_device(sdf, torch.float32)
+        self.sdf = src if sdt == torch.float32 else src.half()
+        self.mip = (mip.cuda() if torch.is_tensor(mip) else to_device(mip, torch.float32)).to(sdt)
         self.guard = None
         if guard:
+            if sdf_storage != abi.STORAGE_FP32:
+                raise ValueError("the brick guard bounds FP32 voxel values")
             C_ = int(sdf_data.cascadesCount)
             self.guard = torch.empty(((self.res // 4) ** 3) * C_, dtype=torch.float32, device="cuda")
-            check(lib().luxgi_sdf_build_guard(ptr(self.sdf), self.res * C_, self.res, self.res, ptr(self.guard),
+            check(lib().luxgi_sdf_build_guard(ptr(src), self.res * C_, self.res, self.res, ptr(self.guard),
                                               stream_handle()), "luxgi_sdf_build_guard")
+        C_ = int(sdf_data.cascadesCount)
+        self.brick_scale = None
+        self.refine = None
+
+        def bricks(storage, dt, scale=None):
+            nbytes = lib().luxgi_sdf_bricks_bytes(self.res * C_, self.res, self.res, storage)
+            if nbytes == 0:
+                raise ValueError("LAYOUT_BRICK4 needs dimensions that are multiples of 4")
+            out = torch.empty(nbytes // (2 if storage == abi.STORAGE_FP16 else 4), dtype=dt, device="cuda")
+            check(lib().luxgi_sdf_to_bricks(ptr(src), self.res * C_, self.res, self.res, storage, ptr(out), ptr(scale),
+                                            stream_handle()), "luxgi_sdf_to_bricks")
+            return out
+
+        if brick_scale and layout != abi.LAYOUT_BRICK4:
+            raise ValueError("a per-brick scale needs LAYOUT_BRICK4")
+        if refine:  # FP32 distances for the hits, in the same layout
+            self.refine = bricks(abi.STORAGE_FP32, torch.float32) if layout == abi.LAYOUT_BRICK4 else src
         if layout == abi.LAYOUT_BRICK4:
             # the engine would keep its SDF in this layout; here it is converted once
-            C_ = int(sdf_data.cascadesCount)
-            bricked = torch.empty_like(self.sdf)
-            check(lib().luxgi_sdf_to_bricks(ptr(self.sdf), self.res * C_, self.res, self.res, ptr(bricked),
-                                            stream_handle()), "luxgi_sdf_to_bricks")
-            self.sdf = bricked
+            if brick_scale:
+                self.brick_scale = torch.empty((self.res // 4) ** 3 * C_, dtype=torch.float32, device="cuda")
+            self.sdf = bricks(sdf_storage, sdt, self.brick_scale)
         self.volume = abi.SDFVolume(ptr(self.sdf), ptr(self.mip), self.res, self.mip_res, sdf_storage, layout,
-                                    ptr(self.guard))
+                                    ptr(self.guard), ptr(self.brick_scale), ptr(self.refine),
+                                    layout if refine else 0, 0)
         self.atlas_host = atlas
         self.objects = to_device(atlas.objects)
         self.tiles = to_device(atlas.tiles)
@@ -77,9 +100,16 @@ class DeviceScene:
         else:
             self.color = to_device(atlas.color)
         self.depth = to_device(atlas.depth)
+        if atlas_layout == abi.LAYOUT_BRICK4:
+            # colour + depth interleaved in 4x4-texel blocks (the engine would write its atlas this way)
+            res = int(atlas.data.resolution)
+            packed = torch.empty(res * res * 2, dtype=torch.int32, device="cuda")
+            check(lib().luxgi_atlas_pack(ptr(self.color), ptr(self.depth), res, ptr(packed), stream_handle()),
+                  "luxgi_atlas_pack")
+            self.color, self.depth = packed, None
         self.atlas = abi.SurfaceAtlas(atlas.data, ptr(self.chunks), ptr(self.culled), ptr(self.objects),
                                       ptr(self.tiles), ptr(self.color), ptr(self.depth), atlas.n_objects,
-                                      atlas.color_storage)
+                                      atlas.color_storage, atlas_layout, 0)
         self.sky_faces = None if sky_faces is None else to_device(sky_faces)
         self.skybox = abi.Skybox(ptr(self.sky_faces), sky_size, 0)
         self.n_objects = atlas.n_objects
@@ -143,12 +173,14 @@ class DDGIVolume:
             check(lib().luxgi_ddgi_trace_sdf_count(*args, ptr(counters), stream_handle()), "luxgi_ddgi_trace_sdf_count")
 
     # probe_update::system (DDGIRenderer.cpp:377-413), border fused by default
-    def probe_update(self, fuse_border=True, per_probe=False):
+    def probe_update(self, fuse_border=True, per_probe=False, half=False):
         flags = abi.UPDATE_IRRADIANCE | abi.UPDATE_DEPTH
         if fuse_border:
             flags |= abi.UPDATE_FUSE_BORDER
         if per_probe:
             flags |= abi.UPDATE_PER_PROBE
+        if half:
+            flags |= abi.UPDATE_HALF
         write = 1 - self.ping_pong
         check(lib().luxgi_ddgi_probe_update(C.byref(self.u), C.byref(self.rays), C.byref(self.atlas[self.ping_pong]),
                                             C.byref(self.atlas[write]), 1 if self.frames == 0 else 0, flags,

@@ -58,3 +58,17 @@ def test_host_only_entry_points():
     bad.depthTextureWidth += 1
     rays = abi.RayBuffers(None, None, 64, 8, 0, 0)
     assert _lib.lib().luxgi_ddgi_probe_update(C.byref(bad), C.byref(rays), None, None, 1, 3, None, 0, None) == -1
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="libluxgi.so not built")
+def test_brick_layout_sizes_and_rejections():
+    L = _lib.lib()
+    assert L.luxgi_sdf_bricks_bytes(256, 256, 256, abi.STORAGE_FP32) == 64 ** 3 * 128 * 4
+    assert L.luxgi_sdf_bricks_bytes(512, 256, 256, abi.STORAGE_FP16) == 128 * 64 * 64 * 128 * 2
+    assert L.luxgi_sdf_bricks_bytes(30, 32, 32, abi.STORAGE_FP32) == 0
+    assert L.luxgi_sdf_bricks_bytes(32, 32, 32, abi.STORAGE_UNORM8) == 0
+    # conversions and packs reject bad shapes before any device work
+    assert L.luxgi_sdf_to_bricks(None, 32, 32, 32, abi.STORAGE_FP32, None, None, None) < 0
+    assert L.luxgi_atlas_pack(None, None, 2048, None, None) < 0
+    dummy = C.c_void_p(16)
+    assert L.luxgi_atlas_pack(dummy, dummy, 2046, dummy, None) < 0

@@ -132,3 +132,55 @@ def test_border_update_standalone_equals_fused():
     torch.cuda.synchronize()
     assert np.array_equal(t_i.cpu().numpy(), fused_i)
     assert np.array_equal(t_d.cpu().numpy(), fused_d)
+
+
+@pytest.mark.parametrize("counts,rays,ni,nd", CASES)
+def test_probe_update_half_mfma_error_bound(counts, rays, ni, nd):
+    """Reduced-precision blend (LUXGI_UPDATE_HALF: FP16 weights and ray data on the FP16 MFMA, FP32
+    accumulation) against the FP64 oracle on the FP32 rays. Each product carries at most two FP16
+    roundings (2 x 2^-11 relative) and the normalisation uses the sum of the rounded weights, so
+    every texel stays within 1.5e-3 relative (+1e-5) and the mean relative error within 2e-4;
+    border and padding rules are the same as the FP32 form's (bit for bit)."""
+    u = abi.make_uniform(counts, (-1, -1, -1), (0.3, 0.4, 0.5), rays, irr_oct=ni, depth_oct=nd, hysteresis=0.97,
+                         max_distance=1.5)
+    rad, dd = _rays(u, seed=rays + 7)
+    rng = np.random.default_rng(2)
+    prev_i = rng.uniform(0, 1, (u.irradianceTextureHeight, u.irradianceTextureWidth, 4)).astype(np.float32)
+    prev_d = rng.uniform(0, 1, (u.depthTextureHeight, u.depthTextureWidth, 2)).astype(np.float32)
+    flags = abi.UPDATE_IRRADIANCE | abi.UPDATE_DEPTH | abi.UPDATE_FUSE_BORDER
+    for first in (1, 0):
+        gi, gd = _run_gpu(u, rad, dd, prev_i, prev_d, first, flags | abi.UPDATE_HALF)
+        oi, od = oracle.probe_update(u, rad, dd, prev_i, prev_d, first, flags)
+        for g, o in ((gi, oi), (gd, od)):
+            np.testing.assert_allclose(g, o, rtol=1.5e-3, atol=1e-5)
+            rel = np.abs(g - o) / (np.abs(o) + 1e-6)
+            assert float(rel.mean()) <= 2e-4, float(rel.mean())
+        for atlas, side in ((gi, ni), (gd, nd)):
+            ref = oracle.border_update(atlas.astype(np.float64), side, *counts).astype(np.float32)
+            assert np.array_equal(ref, atlas)
+
+
+def test_probe_update_half_constant_inputs_exact():
+    """Constant radiance and distance: the FP16 weights cancel in the normalisation, so the texels
+    equal the closed forms (L/2)^(1/gamma) and (r/2, r^2/2) to FP32 rounding."""
+    u = abi.make_uniform((4, 2, 2), (-1, -1, -1), (0.3, 0.4, 0.5), 256, irr_oct=8, depth_oct=16, max_distance=1.5)
+    P, R = u.num_probes, 256
+    dirs = oracle.ray_directions(abi.mat4_colmajor(scene.random_rotation(5)), R).astype(np.float32)
+    rad = np.zeros((P, R, 4), np.float32)
+    rad[..., :3] = 0.5
+    dd = np.zeros((P, R, 4), np.float32)
+    dd[..., :3] = dirs[None]
+    dd[..., 3] = 0.76  # r = 0.75: r / maxDistance = 0.5 and its square are exact in FP16
+    z_i = np.zeros((u.irradianceTextureHeight, u.irradianceTextureWidth, 4), np.float32)
+    z_d = np.zeros((u.depthTextureHeight, u.depthTextureWidth, 2), np.float32)
+    flags = abi.UPDATE_IRRADIANCE | abi.UPDATE_DEPTH | abi.UPDATE_HALF
+    gi, gd = _run_gpu(u, rad, dd, z_i, z_d, 1, flags)
+    n, nd = 8, 16
+    irr_int = np.stack([gi[1 + (p // 8) * 10 + 1: (p // 8) * 10 + n + 2, 1 + (p % 8) * 10 + 1: (p % 8) * 10 + n + 2]
+                        for p in range(P)])
+    r = np.float32(np.float32(0.76) - np.float32(0.01))
+    np.testing.assert_allclose(irr_int[..., :3], (0.25) ** (1 / 5.0), rtol=2e-6)
+    dep_int = np.stack([gd[1 + (p // 8) * 18 + 1: (p // 8) * 18 + nd + 2, 1 + (p % 8) * 18 + 1: (p % 8) * 18 + nd + 2]
+                        for p in range(P)])
+    np.testing.assert_allclose(dep_int[..., 0], r / 2, rtol=2e-6)
+    np.testing.assert_allclose(dep_int[..., 1], r * r / 2, rtol=2e-6)

@@ -144,3 +144,68 @@ def test_trace_fp16_rays():
     # fp16 storage is the fp32 result rounded to half
     np.testing.assert_array_equal(g16[1], g32[1].astype(np.float16).astype(np.float32))
     np.testing.assert_array_equal(g16[0], g32[0].astype(np.float16).astype(np.float32))
+
+
+def test_trace_packed_atlas_is_bitwise_identical():
+    """LUXGI_LAYOUT_BRICK4 surface atlas (colour + depth interleaved in 4x4-texel blocks) gives the
+    same hit radiance as the two linear images, bit for bit."""
+    from luxgi_amd.ddgi import DDGIVolume, DeviceScene
+    prims, sdf, mip, g, atlas = _scenes.small_scene(color_storage=abi.STORAGE_UNORM8)
+    u = _scenes.small_uniform(rays=96, counts=(6, 5, 4))
+    p = _scenes.trace_params(rot_seed=23)
+    outs = []
+    for al in (abi.LAYOUT_LINEAR, abi.LAYOUT_BRICK4):
+        ds = DeviceScene(g, sdf, mip, atlas, layout=abi.LAYOUT_BRICK4, atlas_layout=al)
+        vol = DDGIVolume(u, flags_buffer=True)
+        vol.trace_rays(ds, p)
+        torch.cuda.synchronize()
+        outs.append((vol.radiance.cpu().numpy(), vol.dir_dist.cpu().numpy(), vol.ray_flags.cpu().numpy()))
+    assert (outs[0][0][..., :3] > 0).any()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_trace_fp16_sdf_bricks_match_linear_and_oracle():
+    """FP16 SDF storage: the brick layout is bit-identical to the linear FP16 layout, and both match
+    the oracle run on the same FP16-rounded volume (same population statistics as FP32)."""
+    from luxgi_amd.ddgi import DDGIVolume, DeviceScene
+    prims, sdf, mip, g, atlas = _scenes.small_scene()
+    u = _scenes.small_uniform(rays=64)
+    p = _scenes.trace_params(rot_seed=17)
+    outs = []
+    for layout in (abi.LAYOUT_LINEAR, abi.LAYOUT_BRICK4):
+        ds = DeviceScene(g, sdf, mip, atlas, sdf_storage=abi.STORAGE_FP16, layout=layout)
+        vol = DDGIVolume(u, flags_buffer=True)
+        vol.trace_rays(ds, p)
+        torch.cuda.synchronize()
+        outs.append((vol.radiance.cpu().numpy(), vol.dir_dist.cpu().numpy(), vol.ray_flags.cpu().numpy()))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    s16 = sdf.astype(np.float16).astype(np.float32)
+    m16 = mip.astype(np.float16).astype(np.float32)
+    ref = _scenes.oracle_trace(u, g, s16, m16, atlas, p)
+    _compare(u, g, outs[1] + (None,), ref, 2.0 / 64)
+
+
+def test_trace_fp16_scaled_bricks_and_fp32_refine_vs_oracle():
+    """Reduced-precision march: FP16 bricks with the per-brick power-of-two scale, with and without
+    the FP32 re-evaluation at hits, against the FP64 oracle on the FP32 volume (same population
+    bounds as the FP32 trace). The re-evaluation makes the hit distances FP32-accurate, so their
+    error against the oracle does not grow."""
+    from luxgi_amd.ddgi import DDGIVolume, DeviceScene
+    prims, sdf, mip, g, atlas = _scenes.small_scene()
+    u = _scenes.small_uniform(rays=64)
+    p = _scenes.trace_params(rot_seed=19)
+    ref = _scenes.oracle_trace(u, g, sdf, mip, atlas, p)
+    errs = []
+    for refine in (False, True):
+        ds = DeviceScene(g, sdf, mip, atlas, sdf_storage=abi.STORAGE_FP16, layout=abi.LAYOUT_BRICK4,
+                         brick_scale=True, refine=refine)
+        vol = DDGIVolume(u, flags_buffer=True)
+        vol.trace_rays(ds, p)
+        torch.cuda.synchronize()
+        got = (vol.radiance.cpu().numpy(), vol.dir_dist.cpu().numpy(), vol.ray_flags.cpu().numpy(), None)
+        _compare(u, g, got, ref, 2.0 / 64)
+        hit = (got[2] == 1) & (ref[2] == 1)
+        errs.append(float(np.abs(got[1][..., 3] - ref[1][..., 3])[hit].mean()))
+    assert errs[1] <= errs[0] * 1.01 + 1e-7, errs

@@ -9,8 +9,8 @@ Workload (one "step" = one frame of the hot path on resident inputs):
   * SVGF reflection denoise at 1920x1080 on a G-buffer from SDF primary rays:
     reprojection + moments + 17x17 clip + tile classification, 5 a-trous passes.
 Reported beside the step (not in value): the reduced-precision form of the same
-frame (FP16 SDF bricks + FP32 hit re-evaluation, FP16 rays, FP16-MFMA blend, RGBA16F
-SVGF; reduced_precision), and per-pixel probe sampling of the updated
+frame (FP16 SDF bricks + FP32 hit re-evaluation, FP16-MFMA blend with one rounding per
+operand, RGBA16F SVGF; reduced_precision), and per-pixel probe sampling of the updated
 volume at 1920x1080 (probe_sample_ms; bytes = G-buffer depth + normal + RGBA32F
 output per pixel + both atlases once).
 value = rays / DDGI-update time (Grays/s). All timing uses HIP events on the
@@ -158,13 +158,14 @@ def main():
     sample_bytes = W * H * (4 + 8 + 16) + atlas_bytes
 
     # ---------------- reduced-precision form (reported beside the step, not in value) ----------------
-    # FP16 SDF bricks with per-brick scale and FP32 re-evaluation at hits, FP16 ray buffers (the
-    # reference's RGBA16F), FP16-MFMA blend with FP32 accumulation, SVGF with RGBA16F intermediates
+    # FP16 SDF bricks with per-brick scale and FP32 re-evaluation at hits, FP16-MFMA blend (one
+    # rounding per operand, FP32 accumulation), SVGF with RGBA16F intermediates. Ray buffers stay
+    # FP32: RGBA16F distances lose the depth moments of near hits to the cancellation in t - 0.01.
     mixed = None
     if not args.no_mixed:
         ds16 = DeviceScene(g, sdf, mip, atlas, sdf_storage=abi.STORAGE_FP16, layout=abi.LAYOUT_BRICK4, brick_scale=True,
                            refine=True)
-        vol16 = DDGIVolume(u, ray_storage=abi.STORAGE_FP16)
+        vol16 = DDGIVolume(u)
         dn16 = ReflectionDenoiser(W, H, storage=abi.STORAGE_FP16, iterations=args.svgf_iterations)
         dn16.set_camera(VPI, VPp, eye, delta)
         noisy16 = noisy.half()

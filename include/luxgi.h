@@ -122,7 +122,8 @@ typedef struct LuxgiTraceParams {
     /* --- extensions (defaults reproduce the reference) --- */
     int32_t  maxSteps;      /* per-cascade step cap; reference = 250 (SDFCommon.glsl:143) */
     float    hitEpsilon;    /* 0 = reference thickness test (SDFCommon.glsl:166-167); >0 = fixed eps */
-    int32_t  pad0;
+    int32_t  shadeOrder;    /* 0 = shade hit rays grouped by surface-atlas chunk (counting sort in the
+                               workspace), 1 = in ray order; the outputs are identical */
     int32_t  pad1;
     float    skyColor[4];   /* sky radiance when no cubemap is bound (reference binds a unit cube) */
 } LuxgiTraceParams;

@@ -76,7 +76,7 @@ class TraceParams(C.Structure):
     _fields_ = [
         ("randomOrientation", f16), ("numFrames", C.c_uint32), ("infiniteBounces", C.c_uint32),
         ("numLights", C.c_int32), ("intensity", C.c_float),
-        ("maxSteps", C.c_int32), ("hitEpsilon", C.c_float), ("pad0", C.c_int32), ("pad1", C.c_int32),
+        ("maxSteps", C.c_int32), ("hitEpsilon", C.c_float), ("shadeOrder", C.c_int32), ("pad1", C.c_int32),
         ("skyColor", f4),
     ]
 

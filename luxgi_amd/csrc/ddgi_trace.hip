@@ -35,7 +35,31 @@ struct TraceArgs {
     const float* brickScale;  // optional per-brick scale of a BRICK4 fine volume
     const float* refine;      // optional FP32 fine volume re-evaluated at hits
     int refineLayout;
+    int sortHits;             // shade the hit rays grouped by surface-atlas chunk
 };
+
+constexpr int kChunkBins = kChunksRes * kChunksRes * kChunksRes;
+constexpr uint32_t kNoChunk = 0xffffffffu;  // misses and inside hits: no atlas lookup
+
+// Adds the wave's active lanes to counters[key] with one atomic per distinct key and returns each
+// lane's slot (the counter's old value + the lane's rank among the lanes with its key). Called by
+// all 64 lanes of the wave together.
+__device__ __forceinline__ uint32_t wave_aggregated_add(uint32_t* __restrict__ counters, uint32_t key, bool active) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long todo = __ballot(active);
+    uint32_t slot = 0;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const uint32_t lk = (uint32_t)__shfl((int)key, leader);
+        const unsigned long long peers = todo & __ballot(key == lk);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&counters[lk], (uint32_t)__popcll(peers));
+        base = (uint32_t)__shfl((int)base, leader);
+        if ((peers >> lane) & 1ull) slot = base + (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+        todo &= ~peers;
+    }
+    return slot;
+}
 
 // GLSL inverse(mat4) by cofactors (column-major), evaluated once per object.
 __device__ __forceinline__ void object_inverse_kernel_body(const LuxgiAtlasObject* __restrict__ objects, int n,
@@ -285,17 +309,53 @@ __device__ __forceinline__ float refine_lookup(const TraceArgs& A, int W, int H,
                                                  : sample_volume<float>(A.refine, W, H, D, u, v, w);
 }
 
-template <class SS, bool COUNT, bool BRICK>
+// Result of a ray without an atlas lookup (GISDFRays.comp:93-96,121-124): an inside hit stores
+// zero radiance, a miss the sky in its direction; both store the world distance.
+template <class SR>
+__device__ __forceinline__ void store_nonhit(const TraceArgs& A, const float* __restrict__ skyFaces,
+                                             SR* __restrict__ outRad, SR* __restrict__ outDD,
+                                             uint8_t* __restrict__ flags, size_t e, f3 dir, bool inside) {
+    float4 rad = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!inside) {
+        const float3 s = skyFaces ? sample_cube(skyFaces, A.skySize, dir) : make_float3(A.sky[0], A.sky[1], A.sky[2]);
+        rad = make_float4(s.x, s.y, s.z, 0.f);
+    }
+    Texel<SR>::st4(outRad, e, rad);
+    Texel<SR>::st4(outDD, e, make_float4(dir.x, dir.y, dir.z, kWorldSize));
+    if (flags) flags[e] = inside ? 2 : 0;
+}
+
+// Slot of a hit in its chunk's run of the sorted order: one atomic per lane, or one per distinct
+// chunk of the wave.
+__device__ __forceinline__ uint32_t hist_add(uint32_t* __restrict__ bins, uint32_t key, bool active) {
+#if LUXGI_HIST_PER_LANE
+    return active ? atomicAdd(&bins[key], 1u) : 0u;
+#else
+    return wave_aggregated_add(bins, key, active);
+#endif
+}
+
+// Sphere trace of one ray per lane (SDFCommon.glsl:98-171). Writes the hit record; with the hit
+// sort it also writes the final result of the rays without an atlas lookup and the (chunk, slot)
+// of the others.
+template <class SS, class SR, bool COUNT, bool BRICK>
 __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __restrict__ sdf, const SS* __restrict__ mip,
                                                     const float* __restrict__ guard,
                                                     const float* __restrict__ probeOffsets, float4* __restrict__ recA,
-                                                    float2* __restrict__ recB, unsigned long long* __restrict__ counters) {
+                                                    float2* __restrict__ recB, uint2* __restrict__ hitKeys,
+                                                    uint32_t* __restrict__ bins, const float* __restrict__ skyFaces,
+                                                    SR* __restrict__ outRad, SR* __restrict__ outDD,
+                                                    uint8_t* __restrict__ flags,
+                                                    unsigned long long* __restrict__ counters) {
     const int Rp = (A.R + 63) & ~63;
     const long long gb = xcd_block(blockIdx.x, gridDim.x);
     const long long gid = gb * blockDim.x + threadIdx.x;
     const int probe = int(gid / Rp), ray = int(gid % Rp);
     unsigned long long cMip = 0, cFine = 0, cSteps = 0;
-    if (probe < A.P && ray < A.R) {
+    uint32_t key = kNoChunk;
+    const bool valid = probe < A.P && ray < A.R;
+    const size_t rid = (size_t)probe * A.R + ray;
+    if (valid) {
         f3 o, d;
         ray_setup(A, probeOffsets, probe, ray, o, d);
         const int C = (int)A.g.cascadesCount;
@@ -373,9 +433,26 @@ __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __res
             }
             if (COUNT) cSteps += step;
         }
-        const size_t rid = (size_t)probe * A.R + ray;
         recA[rid] = ra;
         recB[rid] = rb;
+        if (A.sortHits) {
+            const bool inside = ra.x >= 0.0f && rb.x <= 0.0f && ra.x <= A.g.cascadeVoxelSize[0];
+            if (ra.x >= 0.0f && !inside) {  // atlas chunk of a front hit (the lookup of sample_atlas)
+                const f3 hp = o + d * ra.x;
+                const float pc[3] = {hp.x, hp.y, hp.z};
+                int cc[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    cc[i] = clampi(int(floorf(pc[i] / A.ad.chunkSize + float(kChunksRes) * 0.5f)), 0, kChunksRes - 1);
+                key = (uint32_t)((cc[2] * kChunksRes + cc[1]) * kChunksRes + cc[0]);
+            } else {
+                store_nonhit<SR>(A, skyFaces, outRad, outDD, flags, (size_t)probe * A.rayWidth + ray, d, inside);
+            }
+        }
+    }
+    if (A.sortHits) {  // chunk histogram; each hit keeps its slot within its chunk
+        const uint32_t slot = hist_add(bins, key, key != kNoChunk);
+        if (valid) hitKeys[rid] = make_uint2(key, slot);
     }
     if (COUNT) {
         __shared__ unsigned long long sc[3];
@@ -407,10 +484,21 @@ __global__ __launch_bounds__(256) void shade_kernel(TraceArgs A, const SS* __res
                                                     const float* __restrict__ probeOffsets,
                                                     const float4* __restrict__ recA, const float2* __restrict__ recB,
                                                     SR* __restrict__ outRad, SR* __restrict__ outDD,
-                                                    uint8_t* __restrict__ flags, unsigned long long* __restrict__ counters) {
-    const int Rp = (A.R + 63) & ~63;
+                                                    uint8_t* __restrict__ flags, const uint32_t* __restrict__ order,
+                                                    const uint32_t* __restrict__ hitCount,
+                                                    unsigned long long* __restrict__ counters) {
     const long long gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const int probe = int(gid / Rp), ray = int(gid % Rp);
+    int probe, ray;
+    if (order) {  // the front hits grouped by atlas chunk: the object and tile loops of a wave run in step
+        const long long n = *hitCount;
+        const uint32_t rid = gid < n ? order[gid] : 0u;
+        probe = gid < n ? int(rid / (uint32_t)A.R) : A.P;
+        ray = int(rid % (uint32_t)A.R);
+    } else {
+        const int Rp = (A.R + 63) & ~63;
+        probe = int(gid / Rp);
+        ray = int(gid % Rp);
+    }
     const bool active = probe < A.P && ray < A.R;
     Counts c = {0, 0, 0, 0, 0, 0, 0};
     if (active) {
@@ -419,17 +507,14 @@ __global__ __launch_bounds__(256) void shade_kernel(TraceArgs A, const SS* __res
         const size_t rid = (size_t)probe * A.R + ray;
         const float4 ra = recA[rid];
         const float hitTime = ra.x;
-        float4 rad = make_float4(0.f, 0.f, 0.f, 0.f);
-        float dist;
-        uint8_t f = 0;
+        const size_t e = (size_t)probe * A.rayWidth + ray;
         if (hitTime >= 0.0f) {
             const float2 rb = recB[rid];
             const float hitSdf = rb.x;
             const int hitCascade = int(rb.y);
             if (COUNT) c.hits++;
             if (hitSdf <= 0.0f && hitTime <= A.g.cascadeVoxelSize[0]) {
-                dist = kWorldSize;
-                f = 2;
+                store_nonhit<SR>(A, skyFaces, outRad, outDD, flags, e, dir, true);
             } else {
                 const int C = (int)A.g.cascadesCount;
                 const int texW = A.res * C;
@@ -449,19 +534,14 @@ __global__ __launch_bounds__(256) void shade_kernel(TraceArgs A, const SS* __res
                 const float thr = A.g.cascadeVoxelSize[hitCascade] * 1.05f;
                 const float3 s = sample_atlas<SC, COUNT>(A.ad, chunks, culled, objects, objInv, tiles, color,
                                                          atlasDepth, hp, n, thr, c);
-                rad = make_float4(s.x, s.y, s.z, 0.f);
-                dist = fmaxf(hitTime + A.g.cascadeVoxelSize[hitCascade] * 0.5f, 0.0f);
-                f = 1;
+                const float dist = fmaxf(hitTime + A.g.cascadeVoxelSize[hitCascade] * 0.5f, 0.0f);
+                Texel<SR>::st4(outRad, e, make_float4(s.x, s.y, s.z, 0.f));
+                Texel<SR>::st4(outDD, e, make_float4(dir.x, dir.y, dir.z, dist));
+                if (flags) flags[e] = 1;
             }
         } else {
-            const float3 s = skyFaces ? sample_cube(skyFaces, A.skySize, dir) : make_float3(A.sky[0], A.sky[1], A.sky[2]);
-            rad = make_float4(s.x, s.y, s.z, 0.f);
-            dist = kWorldSize;
+            store_nonhit<SR>(A, skyFaces, outRad, outDD, flags, e, dir, false);
         }
-        const size_t e = (size_t)probe * A.rayWidth + ray;
-        Texel<SR>::st4(outRad, e, rad);
-        Texel<SR>::st4(outDD, e, make_float4(dir.x, dir.y, dir.z, dist));
-        if (flags) flags[e] = f;
     }
     if (COUNT) {
         __shared__ unsigned long long sc[8];
@@ -480,6 +560,64 @@ __global__ __launch_bounds__(256) void shade_kernel(TraceArgs A, const SS* __res
     }
 }
 
+// counting sort of the rays by hit chunk: zero the bins, exclusive scan, scatter
+__global__ void zero_bins_kernel(uint32_t* __restrict__ bins, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) bins[i] = 0u;
+}
+
+constexpr int kScanBlock = 1024;
+constexpr int kScanBlocks = (kChunkBins + kScanBlock - 1) / kScanBlock;
+static_assert(kScanBlocks <= 64, "the block totals are scanned by one wave");
+
+// inclusive prefix sum across the 64 lanes of a wave
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, (unsigned)o);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// exclusive scan of the bins within blocks of kScanBlock; the block totals go to `totals`
+__global__ __launch_bounds__(kScanBlock) void scan_bins_kernel(uint32_t* __restrict__ bins,
+                                                               uint32_t* __restrict__ totals, int n) {
+    __shared__ uint32_t waveSum[kScanBlock / 64];
+    const int i = blockIdx.x * kScanBlock + threadIdx.x;
+    const uint32_t v = i < n ? bins[i] : 0u;
+    const uint32_t x = wave_inclusive_scan(v);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) waveSum[wave] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int k = 0; k < wave; ++k) before += waveSum[k];
+    if (i < n) bins[i] = before + x - v;
+    if (threadIdx.x == kScanBlock - 1) totals[blockIdx.x] = before + x;
+}
+
+// order[start of the chunk + slot] = ray for every front hit; block 0 also stores the hit count
+__global__ __launch_bounds__(256) void scatter_hits_kernel(const uint2* __restrict__ keys,
+                                                           const uint32_t* __restrict__ starts,
+                                                           const uint32_t* __restrict__ totals,
+                                                           uint32_t* __restrict__ hitCount,
+                                                           uint32_t* __restrict__ order, long long n) {
+    __shared__ uint32_t blockStart[64];
+    if (threadIdx.x < 64) {
+        const uint32_t t = threadIdx.x < kScanBlocks ? totals[threadIdx.x] : 0u;
+        const uint32_t x = wave_inclusive_scan(t);
+        blockStart[threadIdx.x] = x - t;
+        if (blockIdx.x == 0 && threadIdx.x == 63) *hitCount = x;
+    }
+    __syncthreads();
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const uint2 k = keys[i];
+        if (k.x != kNoChunk) order[blockStart[k.x / kScanBlock] + starts[k.x] + k.y] = (uint32_t)i;
+    }
+}
+
 // per-object inverse transforms, once per call
 __global__ void trace_prepare_kernel(const LuxgiAtlasObject* __restrict__ objects, int n, float* __restrict__ inv) {
     object_inverse_kernel_body(objects, n, inv);
@@ -495,6 +633,10 @@ struct TraceWorkspace {
     float* inv;
     float4* recA;
     float2* recB;
+    uint2* keys;      // hit sort: (chunk, slot in the chunk) per ray
+    uint32_t* bins;   // per-chunk hit counts, then their in-block exclusive scan
+    uint32_t* totals; // scan block totals, then the hit count at [kScanBlocks]
+    uint32_t* order;  // front hits in chunk order
 };
 
 static size_t trace_workspace_layout(size_t rays, int32_t objects, char* base, TraceWorkspace* w) {
@@ -505,6 +647,14 @@ static size_t trace_workspace_layout(size_t rays, int32_t objects, char* base, T
     off += align256(rays * sizeof(float4));
     if (w) w->recB = reinterpret_cast<float2*>(base + off);
     off += align256(rays * sizeof(float2));
+    if (w) w->keys = reinterpret_cast<uint2*>(base + off);
+    off += align256(rays * sizeof(uint2));
+    if (w) w->bins = reinterpret_cast<uint32_t*>(base + off);
+    off += align256((size_t)kChunkBins * sizeof(uint32_t));
+    if (w) w->totals = reinterpret_cast<uint32_t*>(base + off);
+    off += align256((size_t)(kScanBlocks + 1) * sizeof(uint32_t));
+    if (w) w->order = reinterpret_cast<uint32_t*>(base + off);
+    off += align256(rays * sizeof(uint32_t));
     return off;
 }
 
@@ -520,20 +670,36 @@ static int launch_trace(const TraceArgs& A, const LuxgiSDFVolume* vol, const Lux
                         const LuxgiSkybox* sky, const float* probeOffsets, const LuxgiRayBuffers* rays,
                         uint8_t* rayFlags, const TraceWorkspace& w, unsigned long long* counters, hipStream_t st) {
     const long long lanes = (long long)A.P * ((A.R + 63) & ~63);
+    const long long nRays = (long long)A.P * A.R;
+    SR* outRad = static_cast<SR*>(rays->radiance);
+    SR* outDD = static_cast<SR*>(rays->directionDistance);
+    const float* skyFaces = sky && sky->faces ? sky->faces : nullptr;
+    if (A.sortHits) {
+        hipLaunchKernelGGL(zero_bins_kernel, dim3((kChunkBins + 255) / 256), dim3(256), 0, st, w.bins, kChunkBins);
+        LUXGI_LAUNCH_CHECK();
+    }
     if (A.fineLayout == LUXGI_LAYOUT_BRICK4)
-        hipLaunchKernelGGL((march_kernel<SS, COUNT, true>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st, A,
+        hipLaunchKernelGGL((march_kernel<SS, SR, COUNT, true>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st, A,
                            static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), vol->guard, probeOffsets,
-                           w.recA, w.recB, counters);
+                           w.recA, w.recB, w.keys, w.bins, skyFaces, outRad, outDD, rayFlags, counters);
     else
-        hipLaunchKernelGGL((march_kernel<SS, COUNT, false>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st, A,
+        hipLaunchKernelGGL((march_kernel<SS, SR, COUNT, false>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st, A,
                            static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), vol->guard, probeOffsets,
-                           w.recA, w.recB, counters);
+                           w.recA, w.recB, w.keys, w.bins, skyFaces, outRad, outDD, rayFlags, counters);
     LUXGI_LAUNCH_CHECK();
-    hipLaunchKernelGGL((shade_kernel<SS, SC, SR, COUNT>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st, A,
+    if (A.sortHits) {
+        hipLaunchKernelGGL(scan_bins_kernel, dim3(kScanBlocks), dim3(kScanBlock), 0, st, w.bins, w.totals, kChunkBins);
+        LUXGI_LAUNCH_CHECK();
+        hipLaunchKernelGGL(scatter_hits_kernel, dim3((int)((nRays + 255) / 256)), dim3(256), 0, st, w.keys, w.bins,
+                           w.totals, w.totals + kScanBlocks, w.order, nRays);
+        LUXGI_LAUNCH_CHECK();
+    }
+    // sorted: one lane per ray covers every possible hit count; the lanes past the count exit
+    const long long shadeLanes = A.sortHits ? nRays : lanes;
+    hipLaunchKernelGGL((shade_kernel<SS, SC, SR, COUNT>), dim3((int)((shadeLanes + 255) / 256)), dim3(256), 0, st, A,
                        static_cast<const SS*>(vol->sdf), at->chunks, at->culledObjects, at->objects, w.inv, at->tiles,
-                       static_cast<const SC*>(at->color), at->depth, sky && sky->faces ? sky->faces : nullptr,
-                       probeOffsets, w.recA, w.recB, static_cast<SR*>(rays->radiance),
-                       static_cast<SR*>(rays->directionDistance), rayFlags, counters);
+                       static_cast<const SC*>(at->color), at->depth, skyFaces, probeOffsets, w.recA, w.recB, outRad,
+                       outDD, rayFlags, A.sortHits ? w.order : nullptr, w.totals + kScanBlocks, counters);
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }
@@ -590,6 +756,8 @@ static int trace_entry(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, c
     A.brickScale = vol->brickScale;
     A.refine = vol->refine;
     A.refineLayout = vol->refineLayout;
+    if (p->shadeOrder != 0 && p->shadeOrder != 1) return LUXGI_ERR_INVALID_ARGUMENT;
+    A.sortHits = !COUNT && p->shadeOrder == 0;  // the counting run shades in ray order
     if (vol->brickScale && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
     if (vol->refine && ((vol->refineLayout != LUXGI_LAYOUT_LINEAR && vol->refineLayout != LUXGI_LAYOUT_BRICK4) ||
                         (vol->refineLayout == LUXGI_LAYOUT_BRICK4 && (vol->resolution & 3))))

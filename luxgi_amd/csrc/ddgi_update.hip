@@ -334,52 +334,16 @@ __global__ void border_kernel(SA* __restrict__ atlas, int W, int n, int pxy, int
 
 // ---------------------------------------------------------------------------
 // MFMA form. With shared ray directions the blend of a block of 16 probes is
-//   depth: C[Td x 32] = W_d[Td x R] . B_d[R x 32]   (B_d columns: probe g, (d, d^2))
-//   irr:   C[Ti x 48] = W_i[Ti x R] . B_i[R x 48]   (B_i columns: probe g, (r, g, b))
-// computed with v_mfma_f32_16x16x4_f32 (FP32 in, FP32 accumulate). Lane layout
-// of that instruction: A[l%16][l/16], B[l/16][l%16], D[4*(l/16)+v][l%16].
-// W comes from global memory (L2-resident, padded to 16-row tiles, zero rows
-// beyond T); B is staged in LDS per 32-ray chunk. The epilogue normalises by
-// 1/(2 S[t]), applies the gamma, stages the interior in LDS and writes each
-// probe's full bordered tile with the hysteresis blend. A probe whose staged
-// directions differ from ray row 0 is flagged; probe_update_kernel then
-// recomputes the flagged probes with their own directions.
+//   depth: C[Td x 16] = W_d[Td x R] . B_r[R x 16]  and  W_d . B_r2   (probe g, r and r^2)
+//   irr:   C[Ti x 16] = W_i[Ti x R] . B_c[R x 16]  for c = R, G, B
+// contracted on the FP16 MFMA (probe_update_mfma16_kernel below). The epilogue normalises by
+// 1/(2 S[t]), applies the gamma, stages the interiors in LDS and writes each probe's full
+// bordered tile with the hysteresis blend. A probe whose staged directions differ from ray
+// row 0 is flagged; probe_update_kernel then recomputes the flagged probes with their own
+// directions.
 // ---------------------------------------------------------------------------
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr int kMG = 16;    // probes per block
-constexpr int kKC = 16;    // rays per staged chunk (4 MFMA k-steps)
-constexpr int kBStride = 48;  // LDS row stride of the B tiles (== 16 mod 32: conflict-free reads)
-
-// W[r][t] for one map (16-row padded, zero rows beyond T, thresholded as ProbeUpdate.glsl:93)
-// and S[t] = sum_r W[r][t]: one block per 16 texels, 16 ray lanes per texel, partial sums
-// combined in a fixed order.
-__global__ __launch_bounds__(256) void weights_kernel_tiled(const float* __restrict__ dirs0, int R, int n, int tpad,
-                                                            int depth, float sharpness, float* __restrict__ W,
-                                                            float* __restrict__ S) {
-    __shared__ float part[16][17];
-    const int tl = threadIdx.x & 15, j = threadIdx.x >> 4;
-    const int t = blockIdx.x * 16 + tl;
-    const bool valid = t < n * n;
-    const f3 td = valid ? texel_direction(t % n, t / n, n) : mk(0.f, 0.f, 1.f);
-    float s = 0.0f;
-    for (int r = j; r < R; r += 16) {
-        float w = 0.0f;
-        if (valid) {
-            w = fmaxf(0.0f, dot3(td, mk(dirs0[r * 4 + 0], dirs0[r * 4 + 1], dirs0[r * 4 + 2])));
-            if (depth) w = powf(w, sharpness);
-            if (!(w >= kFltEps)) w = 0.0f;
-        }
-        W[(size_t)r * tpad + t] = w;
-        s += w;
-    }
-    part[j][tl] = s;
-    __syncthreads();
-    if (j == 0) {
-        float tot = 0.0f;
-        for (int k = 0; k < 16; ++k) tot += part[k][tl];
-        S[t] = tot;
-    }
-}
 
 // directions of ray row 0 as FP32 (the values every probe's rays are compared with)
 template <class SR>
@@ -398,247 +362,106 @@ __device__ __forceinline__ void border_source(int bx, int by, int n, int& sx, in
     else { sx = bx == 0 ? n : 1; sy = by == 0 ? n : 1; }
 }
 
-// One pipeline stage: W and B tiles of a 16-ray chunk. Row strides are == 16 (mod 32)
-// so the 4 row groups of an MFMA operand read hit disjoint LDS banks.
-constexpr int kWdStride = 256 + 16, kWiStride = 64 + 16;
-struct MfmaStage {
-    float wd[kKC][kWdStride];
-    float wi[kKC][kWiStride];
-    float bd[kKC][kBStride];
-    float bi[kKC][kBStride];
-};
 struct MfmaOut {
     float2 dep[kMG][256];
     float4 irr[kMG][64];
 };
-union MfmaLds {
-    MfmaStage st[2];
-    MfmaOut out;
-};
 
-// registers holding the next chunk while the current one is multiplied
-struct MfmaPrefetch {
-    float4 wd[4];
-    float4 wi;
-    float4 dd, rc;
-};
+// Writes the kMG finished interiors staged in LDS as full bordered tiles (the border copy of
+// BorderUpdate.glsl when fuseBorder), blended with the previous atlas (ProbeUpdate.glsl:144-145).
+// Consecutive lanes cover consecutive texels of an atlas row (the 16 probes' tiles are adjacent),
+// and each lane keeps kWB previous-atlas loads in flight before it blends and stores.
+constexpr int kWB = 4;
 
-template <class SR>
-__device__ __forceinline__ void mfma_load_chunk(const UpdateArgs& A, int r0, int p0, int tpadI, int tpadD,
-                                                const SR* __restrict__ radiance, const SR* __restrict__ dirDist,
-                                                const float* __restrict__ Wi, const float* __restrict__ Wd,
-                                                MfmaPrefetch& f) {
-    const int tid = threadIdx.x;
-    const int qd = tpadD / 4, qi = tpadI / 4;  // float4 per W row
+template <class SA, int CH>
+__device__ __forceinline__ void write_map(const UpdateArgs& A, const MfmaOut& o, int p0, int n, int W,
+                                          const SA* __restrict__ prev, SA* __restrict__ out) {
+    const int pw = n + 2, pxy = A.px * A.py;
+    const int total = kMG * pw * pw;
+    for (int i0 = threadIdx.x; i0 < total; i0 += 256 * kWB) {
+        float4 pv[kWB];
+        size_t dst[kWB];
+        int src[kWB];
+        bool ok[kWB];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int i = tid + k * 256;
-        const int rr = i / max(qd, 1), c4 = i % max(qd, 1);
-        const int r = r0 + rr;
-        f.wd[k] = (qd && rr < kKC && r < A.R) ? *reinterpret_cast<const float4*>(Wd + (size_t)r * tpadD + c4 * 4)
-                                             : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    {
-        const int rr = tid / max(qi, 1), c4 = tid % max(qi, 1);
-        const int r = r0 + rr;
-        f.wi = (qi && rr < kKC && r < A.R) ? *reinterpret_cast<const float4*>(Wi + (size_t)r * tpadI + c4 * 4)
-                                          : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    {
-        const int g = tid / kKC, rr = tid % kKC, r = r0 + rr, p = p0 + g;
-        f.dd = make_float4(0.f, 0.f, 0.f, 0.f);
-        f.rc = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (p < A.P && r < A.R) {
-            const size_t e = (size_t)p * A.rayWidth + r;
-            f.dd = Texel<SR>::ld4(dirDist, e);
-            if (A.doIrr) f.rc = Texel<SR>::ld4(radiance, e);
+        for (int u = 0; u < kWB; ++u) {
+            const int i = i0 + u * 256;
+            const int bx = i % pw, g = (i / pw) % kMG, by = i / (pw * kMG);
+            const int p = p0 + g;
+            const bool interior = bx >= 1 && bx <= n && by >= 1 && by <= n;
+            ok[u] = i < total && p < A.P && (interior || A.fuseBorder);
+            int sx = 0, sy = 0;
+            border_source(bx, by, n, sx, sy);
+            src[u] = g * 256 + (sy - 1) * n + (sx - 1);
+            const int ox = (p % pxy) * pw + 1, oy = (p / pxy) * pw + 1;
+            dst[u] = (size_t)(oy + by) * W + ox + bx;
+            pv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (ok[u] && !A.firstFrame) {
+                const size_t ps = (size_t)(oy + sy) * W + ox + sx;
+                if (CH == 4) pv[u] = Texel<SA>::ld4(prev, ps);
+                else {
+                    const float2 t = Texel<SA>::ld2(prev, ps);
+                    pv[u] = make_float4(t.x, t.y, 0.f, 0.f);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kWB; ++u) {
+            if (!ok[u]) continue;
+            const int g = src[u] >> 8, t = src[u] & 255;
+            if (CH == 4) {
+                float4 v = o.irr[g][t];
+                v.w = 1.0f;
+                if (!A.firstFrame) {
+                    v.x = mixf(v.x, pv[u].x, A.hysteresis);
+                    v.y = mixf(v.y, pv[u].y, A.hysteresis);
+                    v.z = mixf(v.z, pv[u].z, A.hysteresis);
+                }
+                Texel<SA>::st4(out, dst[u], v);
+            } else {
+                float2 v = o.dep[g][t];
+                if (!A.firstFrame) {
+                    v.x = mixf(v.x, pv[u].x, A.hysteresis);
+                    v.y = mixf(v.y, pv[u].y, A.hysteresis);
+                }
+                Texel<SA>::st2(out, dst[u], v);
+            }
         }
     }
 }
 
-__device__ __forceinline__ void mfma_store_chunk(const UpdateArgs& A, int r0, int p0, int tpadI, int tpadD,
-                                                 const float* __restrict__ dirs0, const MfmaPrefetch& f,
-                                                 MfmaStage& st, int* sMis) {
-    const int tid = threadIdx.x;
-    const int qd = tpadD / 4, qi = tpadI / 4;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int i = tid + k * 256;
-        if (qd && i < kKC * qd) *reinterpret_cast<float4*>(&st.wd[i / qd][(i % qd) * 4]) = f.wd[k];
-    }
-    if (qi && tid < kKC * qi) *reinterpret_cast<float4*>(&st.wi[tid / qi][(tid % qi) * 4]) = f.wi;
-    const int g = tid / kKC, rr = tid % kKC, r = r0 + rr, p = p0 + g;
-    float d0 = 0.f, d1 = 0.f;
-    if (p < A.P && r < A.R) {
-        float dist = fminf(A.maxDistance, f.dd.w - 0.01f);  // ProbeUpdate.glsl:75-78
-        if (dist == -1.0f) dist = A.maxDistance;
-        d0 = dist;
-        d1 = dist * dist;
-        if (f.dd.x != dirs0[r * 4 + 0] || f.dd.y != dirs0[r * 4 + 1] || f.dd.z != dirs0[r * 4 + 2]) sMis[g] = 1;
-    }
-    st.bd[rr][g * 2 + 0] = d0;
-    st.bd[rr][g * 2 + 1] = d1;
-    st.bi[rr][g * 3 + 0] = f.rc.x;
-    st.bi[rr][g * 3 + 1] = f.rc.y;
-    st.bi[rr][g * 3 + 2] = f.rc.z;
-}
-
-// Writes the kMG finished interiors staged in LDS as full bordered tiles (the border copy of
-// BorderUpdate.glsl when fuseBorder), blended with the previous atlas (ProbeUpdate.glsl:144-145).
 template <class SA>
 __device__ __forceinline__ void write_tiles(const UpdateArgs& A, const MfmaOut& o, int p0, const SA* __restrict__ prevIrr,
                                             const SA* __restrict__ prevDepth, SA* __restrict__ outIrr,
                                             SA* __restrict__ outDepth) {
-    const int tid = threadIdx.x;
-    const int pxy = A.px * A.py;
-    for (int map = 0; map < 2; ++map) {
-        if ((map == 0 && !A.doIrr) || (map == 1 && !A.doDepth)) continue;
-        const int n = map == 0 ? A.ni : A.nd, pw = n + 2;
-        const int total = kMG * pw * pw;
-        for (int i = tid; i < total; i += 256) {
-            const int bx = i % pw, g = (i / pw) % kMG, by = i / (pw * kMG);
-            const int p = p0 + g;
-            if (p >= A.P) continue;
-            const bool interior = bx >= 1 && bx <= n && by >= 1 && by <= n;
-            if (!interior && !A.fuseBorder) continue;
-            int sx, sy;
-            border_source(bx, by, n, sx, sy);
-            const int src = (sy - 1) * n + (sx - 1);
-            const int ox = (p % pxy) * pw + 1, oy = (p / pxy) * pw + 1;
-            if (map == 0) {
-                float4 v = o.irr[g][src];
-                v.w = 1.0f;
-                if (!A.firstFrame) {
-                    const float4 pv = Texel<SA>::ld4(prevIrr, (size_t)(oy + sy) * A.Wi + ox + sx);
-                    v.x = mixf(v.x, pv.x, A.hysteresis);
-                    v.y = mixf(v.y, pv.y, A.hysteresis);
-                    v.z = mixf(v.z, pv.z, A.hysteresis);
-                }
-                Texel<SA>::st4(outIrr, (size_t)(oy + by) * A.Wi + ox + bx, v);
-            } else {
-                float2 v = o.dep[g][src];
-                if (!A.firstFrame) {
-                    const float2 pv = Texel<SA>::ld2(prevDepth, (size_t)(oy + sy) * A.Wd + ox + sx);
-                    v.x = mixf(v.x, pv.x, A.hysteresis);
-                    v.y = mixf(v.y, pv.y, A.hysteresis);
-                }
-                Texel<SA>::st2(outDepth, (size_t)(oy + by) * A.Wd + ox + bx, v);
-            }
-        }
-    }
-}
-
-template <class SR, class SA>
-__global__ __launch_bounds__(256) void probe_update_mfma_kernel(
-    UpdateArgs A, const SR* __restrict__ radiance, const SR* __restrict__ dirDist, const SA* __restrict__ prevIrr,
-    const SA* __restrict__ prevDepth, SA* __restrict__ outIrr, SA* __restrict__ outDepth, const float* __restrict__ dirs0,
-    const float* __restrict__ Wi, const float* __restrict__ Si, const float* __restrict__ Wd,
-    const float* __restrict__ Sd, uint8_t* __restrict__ mismatch, int tpadI, int tpadD) {
-    __shared__ MfmaLds lds;
-    __shared__ int sMis[kMG];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int p0 = blockIdx.x * kMG;
-    const int MtD = A.doDepth ? tpadD / 16 : 0, MtI = A.doIrr ? tpadI / 16 : 0;
-    const int tpI = A.doIrr ? tpadI : 0, tpD = A.doDepth ? tpadD : 0;
-    if (tid < kMG) sMis[tid] = 0;
-    floatx4 accD[4][2], accI[3];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) accD[j][0] = accD[j][1] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int j = 0; j < 3; ++j) accI[j] = floatx4{0.f, 0.f, 0.f, 0.f};
-
-    const int nChunks = (A.R + kKC - 1) / kKC;
-    MfmaPrefetch f;
-    mfma_load_chunk<SR>(A, 0, p0, tpI, tpD, radiance, dirDist, Wi, Wd, f);
-    __syncthreads();  // sMis initialised
-    mfma_store_chunk(A, 0, p0, tpI, tpD, dirs0, f, lds.st[0], sMis);
-    __syncthreads();
-    const int col = lane & 15;
-    for (int c = 0; c < nChunks; ++c) {
-        const bool more = c + 1 < nChunks;
-        if (more) mfma_load_chunk<SR>(A, (c + 1) * kKC, p0, tpI, tpD, radiance, dirDist, Wi, Wd, f);
-        const MfmaStage& st = lds.st[c & 1];
-#pragma unroll
-        for (int ks = 0; ks < kKC / 4; ++ks) {
-            const int k = ks * 4 + (lane >> 4);
-            if (MtD) {
-                const float b0 = st.bd[k][col], b1 = st.bd[k][16 + col];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int m = wave + 4 * j;
-                    if (m < MtD) {
-                        const float a = st.wd[k][m * 16 + col];
-                        accD[j][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, accD[j][0], 0, 0, 0);
-                        accD[j][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, accD[j][1], 0, 0, 0);
-                    }
-                }
-            }
-            if (wave < MtI) {
-                const float a = st.wi[k][wave * 16 + col];
-                accI[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, st.bi[k][col], accI[0], 0, 0, 0);
-                accI[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, st.bi[k][16 + col], accI[1], 0, 0, 0);
-                accI[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, st.bi[k][32 + col], accI[2], 0, 0, 0);
-            }
-        }
-        if (more) mfma_store_chunk(A, (c + 1) * kKC, p0, tpI, tpD, dirs0, f, lds.st[(c + 1) & 1], sMis);
-        __syncthreads();
-    }
-    // the LDS now holds the finished interiors
-    const int Td = A.nd * A.nd, Ti = A.ni * A.ni;
-    const int rowq = (lane >> 4) * 4;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int m = wave + 4 * j;
-        if (m >= MtD) continue;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int t = m * 16 + rowq + v;
-            if (t >= Td) continue;
-            const float s = Sd[t];
-            const float sc = s > kFltEps ? 1.0f / (2.0f * s) : 1.0f;
-#pragma unroll
-            for (int nt = 0; nt < 2; ++nt) {
-                const int c = nt * 16 + col;
-                float* dst = reinterpret_cast<float*>(&lds.out.dep[c >> 1][t]);
-                dst[c & 1] = accD[j][nt][v] * sc;
-            }
-        }
-    }
-    if (wave < MtI) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int t = wave * 16 + rowq + v;
-            if (t >= Ti) continue;
-            const float s = Si[t];
-            const float sc = s > kFltEps ? 1.0f / (2.0f * s) : 1.0f;
-#pragma unroll
-            for (int nt = 0; nt < 3; ++nt) {
-                const int c = nt * 16 + col;
-                float* dst = reinterpret_cast<float*>(&lds.out.irr[c / 3][t]);
-                dst[c % 3] = powf(accI[nt][v] * sc, A.invGamma);
-            }
-        }
-    }
-    __syncthreads();
-    if (tid < kMG && p0 + tid < A.P) mismatch[p0 + tid] = (uint8_t)sMis[tid];
-    write_tiles<SA>(A, lds.out, p0, prevIrr, prevDepth, outIrr, outDepth);
+    if (A.doIrr) write_map<SA, 4>(A, o, p0, A.ni, A.Wi, prevIrr, outIrr);
+    if (A.doDepth) write_map<SA, 2>(A, o, p0, A.nd, A.Wd, prevDepth, outDepth);
 }
 
 // ---------------------------------------------------------------------------
-// Reduced-precision blend (LUXGI_UPDATE_HALF): the same contraction on the FP16 MFMA
-// v_mfma_f32_16x16x32_f16 with FP32 accumulation. Operand lanes (one k-step = 32 rays):
-// A[m][k] = W at texel m, ray k: lane l holds k = 8(l/16) .. 8(l/16)+7 of row l%16, one 16-byte
-// load from the texel-major FP16 weights; B[k][n] = ray data of probe n: lane l holds rays
-// 8(l/16) .. +7 of probe l%16; D[4(l/16)+v][l%16] as for the FP32 form. W is rounded to FP16 and
-// normalised by the sum of the rounded weights (a constant input stays constant); depth moments
-// enter as r/maxDistance and (r/maxDistance)^2, which keeps them in FP16's normal range.
+// Blend on the FP16 MFMA v_mfma_f32_16x16x32_f16 with FP32 accumulation, 16 probes per block,
+// 32 rays per k-step. Operand lanes: A[m][k] = W at texel m, ray k -- lane l holds rays
+// 8(l/16) .. 8(l/16)+7 of texel row l%16, one 16-byte load from the texel-major FP16 weights;
+// B[k][n] = ray data of probe n -- lane l holds the same 8 rays of probe l%16, one 16-byte LDS
+// read; D[4(l/16)+v][l%16] as for the FP32 MFMA. Depth moments enter as r/maxDistance and its
+// square (both in [0, 1]).
+//   SPLIT (the plain form): every operand x is carried as hi = fp16(x), lo = fp16(x - hi) and
+//     each product as hi*hi + hi*lo + lo*hi: about 22 significant bits per operand, the dropped
+//     lo*lo term and the FP16 subnormal floor sit below the FP32 rounding of the normalised sums
+//     because all operands are O(1). Normalised by the sum of hi + lo.
+//   !SPLIT (LUXGI_UPDATE_HALF): one FP16 rounding per operand, normalised by the sum of the
+//     rounded weights, so a constant input stays constant.
+// The ray data of each k-step is staged in LDS (double-buffered) by all 256 lanes; a probe whose
+// directions differ from ray row 0 is flagged for the per-probe fix-up pass.
 // ---------------------------------------------------------------------------
 typedef _Float16 half8v __attribute__((ext_vector_type(8)));
 constexpr int kHK = 32;  // rays per k-step
+constexpr int kHCh = 5;  // r, r^2, R, G, B
 
 __global__ __launch_bounds__(256) void weights_half_kernel(const float* __restrict__ dirs0, int R, int Rpad, int n,
-                                                           int tpad, int depth, float sharpness,
-                                                           _Float16* __restrict__ Wh, float* __restrict__ Sh) {
+                                                           int tpad, int depth, float sharpness, int split,
+                                                           _Float16* __restrict__ Whi, _Float16* __restrict__ Wlo,
+                                                           float* __restrict__ Sh) {
     __shared__ float part[16][17];
     const int tl = threadIdx.x & 15, j = threadIdx.x >> 4;
     const int t = blockIdx.x * 16 + tl;
@@ -653,8 +476,14 @@ __global__ __launch_bounds__(256) void weights_half_kernel(const float* __restri
             if (!(w >= kFltEps)) w = 0.0f;
         }
         const _Float16 h = (_Float16)w;
-        Wh[(size_t)t * Rpad + r] = h;
-        s += (float)h;
+        Whi[(size_t)t * Rpad + r] = h;
+        if (split) {
+            const _Float16 l = (_Float16)(w - (float)h);
+            Wlo[(size_t)t * Rpad + r] = l;
+            s += (float)h + (float)l;
+        } else {
+            s += (float)h;
+        }
     }
     part[j][tl] = s;
     __syncthreads();
@@ -665,79 +494,137 @@ __global__ __launch_bounds__(256) void weights_half_kernel(const float* __restri
     }
 }
 
-template <class SR, class SA>
+struct HalfStage {
+    _Float16 b[2][kHCh][kMG][kHK];  // [hi / lo][channel][probe][ray]
+};
+union HalfLds {
+    HalfStage st[2];
+    MfmaOut out;
+};
+
+// ray data of one k-step: thread t stages rays 2(t%16), 2(t%16)+1 of probe t/16
+struct HalfPrefetch {
+    float4 dd[2], rc[2];
+};
+
+template <class SR>
+__device__ __forceinline__ void half_load(const UpdateArgs& A, int k0, int p0, const SR* __restrict__ radiance,
+                                          const SR* __restrict__ dirDist, HalfPrefetch& f) {
+    const int g = threadIdx.x >> 4, rp = (threadIdx.x & 15) * 2, p = p0 + g;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int r = k0 + rp + q;
+        f.dd[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        f.rc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (p < A.P && r < A.R) {
+            const size_t e = (size_t)p * A.rayWidth + r;
+            f.dd[q] = Texel<SR>::ld4(dirDist, e);
+            if (A.doIrr) f.rc[q] = Texel<SR>::ld4(radiance, e);
+        }
+    }
+}
+
+template <bool SPLIT>
+__device__ __forceinline__ void half_store(const UpdateArgs& A, int k0, int p0, const float* __restrict__ dirs0,
+                                           const HalfPrefetch& f, HalfStage& st, int* sMis) {
+    const int g = threadIdx.x >> 4, rp = (threadIdx.x & 15) * 2, p = p0 + g;
+    const float invMax = 1.0f / A.maxDistance;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int r = k0 + rp + q;
+        float v[kHCh] = {0.f, 0.f, f.rc[q].x, f.rc[q].y, f.rc[q].z};
+        if (p < A.P && r < A.R) {
+            float dist = fminf(A.maxDistance, f.dd[q].w - 0.01f);  // ProbeUpdate.glsl:75-78
+            if (dist == -1.0f) dist = A.maxDistance;
+            v[0] = dist * invMax;
+            v[1] = v[0] * v[0];
+            if (f.dd[q].x != dirs0[r * 4 + 0] || f.dd[q].y != dirs0[r * 4 + 1] || f.dd[q].z != dirs0[r * 4 + 2])
+                sMis[g] = 1;
+        }
+#pragma unroll
+        for (int c = 0; c < kHCh; ++c) {
+            const _Float16 h = (_Float16)v[c];
+            st.b[0][c][g][rp + q] = h;
+            if (SPLIT) st.b[1][c][g][rp + q] = (_Float16)(v[c] - (float)h);
+        }
+    }
+}
+
+template <bool SPLIT>
+__device__ __forceinline__ floatx4 mfma_h(half8v ah, half8v al, half8v bh, half8v bl, floatx4 acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc, 0, 0, 0);
+    if (SPLIT) {
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+template <class SR, class SA, bool SPLIT>
 __global__ __launch_bounds__(256) void probe_update_mfma16_kernel(
     UpdateArgs A, const SR* __restrict__ radiance, const SR* __restrict__ dirDist, const SA* __restrict__ prevIrr,
     const SA* __restrict__ prevDepth, SA* __restrict__ outIrr, SA* __restrict__ outDepth, const float* __restrict__ dirs0,
-    const _Float16* __restrict__ Wih, const float* __restrict__ Sih, const _Float16* __restrict__ Wdh,
-    const float* __restrict__ Sdh, uint8_t* __restrict__ mismatch, int tpadI, int tpadD, int Rpad) {
-    __shared__ MfmaOut out;
+    const _Float16* __restrict__ Wih, const _Float16* __restrict__ Wil, const float* __restrict__ Sih,
+    const _Float16* __restrict__ Wdh, const _Float16* __restrict__ Wdl, const float* __restrict__ Sdh,
+    uint8_t* __restrict__ mismatch, int tpadI, int tpadD, int Rpad) {
+    __shared__ HalfLds lds;
     __shared__ int sMis[kMG];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int col = lane & 15, kq = lane >> 4;
-    const int p0 = blockIdx.x * kMG, p = p0 + col;
+    const int p0 = blockIdx.x * kMG;
     const int MtD = A.doDepth ? tpadD / 16 : 0, MtI = A.doIrr ? tpadI / 16 : 0;
-    const float invMax = 1.0f / A.maxDistance;
     if (tid < kMG) sMis[tid] = 0;
-    __syncthreads();
     floatx4 accD[4][2], accI[3];
 #pragma unroll
     for (int j = 0; j < 4; ++j) accD[j][0] = accD[j][1] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int j = 0; j < 3; ++j) accI[j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    int mis = 0;
-    for (int k0 = 0; k0 < Rpad; k0 += kHK) {
-        const int r0 = k0 + 8 * kq;
-        half8v bR, bR2, bCr, bCg, bCb;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int r = r0 + j;
-            float m1 = 0.f, m2 = 0.f, cr = 0.f, cg = 0.f, cb = 0.f;
-            if (p < A.P && r < A.R) {
-                const size_t e = (size_t)p * A.rayWidth + r;
-                if (A.doDepth) {
-                    const float4 dd = Texel<SR>::ld4(dirDist, e);
-                    float dist = fminf(A.maxDistance, dd.w - 0.01f);  // ProbeUpdate.glsl:75-78
-                    if (dist == -1.0f) dist = A.maxDistance;
-                    m1 = dist * invMax;
-                    m2 = m1 * m1;
-                    if (wave == 0 && (dd.x != dirs0[r * 4 + 0] || dd.y != dirs0[r * 4 + 1] || dd.z != dirs0[r * 4 + 2]))
-                        mis = 1;
-                }
-                if (A.doIrr) {
-                    const float4 c = Texel<SR>::ld4(radiance, e);
-                    cr = c.x;
-                    cg = c.y;
-                    cb = c.z;
-                    if (!A.doDepth && wave == 0) {
-                        const float4 dd = Texel<SR>::ld4(dirDist, e);
-                        if (dd.x != dirs0[r * 4 + 0] || dd.y != dirs0[r * 4 + 1] || dd.z != dirs0[r * 4 + 2]) mis = 1;
-                    }
-                }
+    const int nSteps = Rpad / kHK;
+    HalfPrefetch f;
+    half_load<SR>(A, 0, p0, radiance, dirDist, f);
+    __syncthreads();  // sMis initialised
+    half_store<SPLIT>(A, 0, p0, dirs0, f, lds.st[0], sMis);
+    __syncthreads();
+    for (int s = 0; s < nSteps; ++s) {
+        const int k0 = s * kHK;
+        const bool more = s + 1 < nSteps;
+        if (more) half_load<SR>(A, k0 + kHK, p0, radiance, dirDist, f);
+        const HalfStage& st = lds.st[s & 1];
+        const int ko = k0 + 8 * kq;
+        if (MtD) {
+            const half8v bh0 = *reinterpret_cast<const half8v*>(&st.b[0][0][col][8 * kq]);
+            const half8v bh1 = *reinterpret_cast<const half8v*>(&st.b[0][1][col][8 * kq]);
+            half8v bl0 = bh0, bl1 = bh1;
+            if (SPLIT) {
+                bl0 = *reinterpret_cast<const half8v*>(&st.b[1][0][col][8 * kq]);
+                bl1 = *reinterpret_cast<const half8v*>(&st.b[1][1][col][8 * kq]);
             }
-            bR[j] = (_Float16)m1;
-            bR2[j] = (_Float16)m2;
-            bCr[j] = (_Float16)cr;
-            bCg[j] = (_Float16)cg;
-            bCb[j] = (_Float16)cb;
-        }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int m = wave + 4 * j;
-            if (m < MtD) {
-                const half8v a = *reinterpret_cast<const half8v*>(Wdh + (size_t)(m * 16 + col) * Rpad + r0);
-                accD[j][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bR, accD[j][0], 0, 0, 0);
-                accD[j][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bR2, accD[j][1], 0, 0, 0);
+            for (int j = 0; j < 4; ++j) {
+                const int m = wave + 4 * j;
+                if (m < MtD) {
+                    const size_t off = (size_t)(m * 16 + col) * Rpad + ko;
+                    const half8v ah = *reinterpret_cast<const half8v*>(Wdh + off);
+                    const half8v al = SPLIT ? *reinterpret_cast<const half8v*>(Wdl + off) : ah;
+                    accD[j][0] = mfma_h<SPLIT>(ah, al, bh0, bl0, accD[j][0]);
+                    accD[j][1] = mfma_h<SPLIT>(ah, al, bh1, bl1, accD[j][1]);
+                }
             }
         }
         if (wave < MtI) {
-            const half8v a = *reinterpret_cast<const half8v*>(Wih + (size_t)(wave * 16 + col) * Rpad + r0);
-            accI[0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bCr, accI[0], 0, 0, 0);
-            accI[1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bCg, accI[1], 0, 0, 0);
-            accI[2] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bCb, accI[2], 0, 0, 0);
+            const size_t off = (size_t)(wave * 16 + col) * Rpad + ko;
+            const half8v ah = *reinterpret_cast<const half8v*>(Wih + off);
+            const half8v al = SPLIT ? *reinterpret_cast<const half8v*>(Wil + off) : ah;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const half8v bh = *reinterpret_cast<const half8v*>(&st.b[0][2 + c][col][8 * kq]);
+                const half8v bl = SPLIT ? *reinterpret_cast<const half8v*>(&st.b[1][2 + c][col][8 * kq]) : bh;
+                accI[c] = mfma_h<SPLIT>(ah, al, bh, bl, accI[c]);
+            }
         }
+        if (more) half_store<SPLIT>(A, k0 + kHK, p0, dirs0, f, lds.st[(s + 1) & 1], sMis);
+        __syncthreads();
     }
-    if (mis) atomicOr(&sMis[col], 1);
     // interiors: normalise by 1/(2 S), undo the depth scaling, gamma-encode (ProbeUpdate.glsl:133-141)
     const int Td = A.nd * A.nd, Ti = A.ni * A.ni;
     const int rowq = kq * 4;
@@ -752,7 +639,7 @@ __global__ __launch_bounds__(256) void probe_update_mfma16_kernel(
             if (t >= Td) continue;
             const float sm = Sdh[t];
             const float sc = sm > kFltEps ? 1.0f / (2.0f * sm) : 1.0f;
-            out.dep[col][t] = make_float2(accD[j][0][v] * sc * mScale[0], accD[j][1][v] * sc * mScale[1]);
+            lds.out.dep[col][t] = make_float2(accD[j][0][v] * sc * mScale[0], accD[j][1][v] * sc * mScale[1]);
         }
     }
     if (wave < MtI) {
@@ -762,13 +649,13 @@ __global__ __launch_bounds__(256) void probe_update_mfma16_kernel(
             if (t >= Ti) continue;
             const float sm = Sih[t];
             const float sc = sm > kFltEps ? 1.0f / (2.0f * sm) : 1.0f;
-            out.irr[col][t] = make_float4(powf(accI[0][v] * sc, A.invGamma), powf(accI[1][v] * sc, A.invGamma),
-                                          powf(accI[2][v] * sc, A.invGamma), 1.0f);
+            lds.out.irr[col][t] = make_float4(powf(accI[0][v] * sc, A.invGamma), powf(accI[1][v] * sc, A.invGamma),
+                                              powf(accI[2][v] * sc, A.invGamma), 1.0f);
         }
     }
     __syncthreads();
     if (tid < kMG && p0 + tid < A.P) mismatch[p0 + tid] = (uint8_t)sMis[tid];
-    write_tiles<SA>(A, out, p0, prevIrr, prevDepth, outIrr, outDepth);
+    write_tiles<SA>(A, lds.out, p0, prevIrr, prevDepth, outIrr, outDepth);
 }
 
 static int check_uniform(const LuxgiDDGIUniform* u) {
@@ -805,7 +692,7 @@ extern "C" int luxgi_ddgi_fill_atlas_sizes(LuxgiDDGIUniform* u) {
 struct UpdateWorkspace {
     float *dirs0, *Wi, *Si, *Wd, *Sd;
     uint8_t* mismatch;
-    _Float16 *Wih, *Wdh;
+    _Float16 *Wih, *Wil, *Wdh, *Wdl;  // hi / lo parts
     float *Sih, *Sdh;
 };
 
@@ -832,13 +719,17 @@ static size_t update_workspace_layout(const LuxgiDDGIUniform* u, char* base, Upd
     char* mm = take(P);
     const size_t Rp = (size_t)pad32((int)R);
     char* wih = take(Rp * Ti * 2);
+    char* wil = take(Rp * Ti * 2);
     char* sih = take(Ti * 4);
     char* wdh = take(Rp * Td * 2);
+    char* wdl = take(Rp * Td * 2);
     char* sdh = take(Td * 4);
     if (w) {
         w->Wih = reinterpret_cast<_Float16*>(wih);
+        w->Wil = reinterpret_cast<_Float16*>(wil);
         w->Sih = reinterpret_cast<float*>(sih);
         w->Wdh = reinterpret_cast<_Float16*>(wdh);
+        w->Wdl = reinterpret_cast<_Float16*>(wdl);
         w->Sdh = reinterpret_cast<float*>(sdh);
         w->dirs0 = reinterpret_cast<float*>(d0);
         w->Wi = reinterpret_cast<float*>(wi);
@@ -864,39 +755,31 @@ static int launch_update(const UpdateArgs& A, const LuxgiRayBuffers* rays, const
     const SA* pD = static_cast<const SA*>(prev ? prev->depth : nullptr);
     SA* oI = static_cast<SA*>(out->irradiance);
     SA* oD = static_cast<SA*>(out->depth);
-    const bool mfma = !A.forcePerProbe && A.ni * A.ni <= 64 && A.nd * A.nd <= 256;
+    const bool mfma = !A.forcePerProbe && A.ni * A.ni <= 64 && A.nd * A.nd <= 256 && A.maxDistance > 0.0f;
     if (mfma) {
         const int tpI = pad16(A.ni * A.ni), tpD = pad16(A.nd * A.nd);
         hipLaunchKernelGGL(row0_kernel<SR>, dim3((A.R + 255) / 256), dim3(256), 0, st, dd, A.R, w.dirs0);
         LUXGI_LAUNCH_CHECK();
-        if (A.half) {
-            const int Rp = pad32(A.R);
-            if (A.doIrr) {
-                hipLaunchKernelGGL(weights_half_kernel, dim3(tpI / 16), dim3(256), 0, st, w.dirs0, A.R, Rp, A.ni, tpI, 0,
-                                   A.sharpness, w.Wih, w.Sih);
-                LUXGI_LAUNCH_CHECK();
-            }
-            if (A.doDepth) {
-                hipLaunchKernelGGL(weights_half_kernel, dim3(tpD / 16), dim3(256), 0, st, w.dirs0, A.R, Rp, A.nd, tpD, 1,
-                                   A.sharpness, w.Wdh, w.Sdh);
-                LUXGI_LAUNCH_CHECK();
-            }
-            hipLaunchKernelGGL((probe_update_mfma16_kernel<SR, SA>), dim3((A.P + kMG - 1) / kMG), dim3(256), 0, st, A,
-                               rad, dd, pI, pD, oI, oD, w.dirs0, w.Wih, w.Sih, w.Wdh, w.Sdh, w.mismatch, tpI, tpD, Rp);
-        } else {
-            if (A.doIrr) {
-                hipLaunchKernelGGL(weights_kernel_tiled, dim3(tpI / 16), dim3(256), 0, st, w.dirs0, A.R, A.ni, tpI, 0,
-                                   A.sharpness, w.Wi, w.Si);
-                LUXGI_LAUNCH_CHECK();
-            }
-            if (A.doDepth) {
-                hipLaunchKernelGGL(weights_kernel_tiled, dim3(tpD / 16), dim3(256), 0, st, w.dirs0, A.R, A.nd, tpD, 1,
-                                   A.sharpness, w.Wd, w.Sd);
-                LUXGI_LAUNCH_CHECK();
-            }
-            hipLaunchKernelGGL((probe_update_mfma_kernel<SR, SA>), dim3((A.P + kMG - 1) / kMG), dim3(256), 0, st, A,
-                               rad, dd, pI, pD, oI, oD, w.dirs0, w.Wi, w.Si, w.Wd, w.Sd, w.mismatch, tpI, tpD);
+        const int Rp = pad32(A.R);
+        const int split = A.half ? 0 : 1;
+        if (A.doIrr) {
+            hipLaunchKernelGGL(weights_half_kernel, dim3(tpI / 16), dim3(256), 0, st, w.dirs0, A.R, Rp, A.ni, tpI, 0,
+                               A.sharpness, split, w.Wih, w.Wil, w.Sih);
+            LUXGI_LAUNCH_CHECK();
         }
+        if (A.doDepth) {
+            hipLaunchKernelGGL(weights_half_kernel, dim3(tpD / 16), dim3(256), 0, st, w.dirs0, A.R, Rp, A.nd, tpD, 1,
+                               A.sharpness, split, w.Wdh, w.Wdl, w.Sdh);
+            LUXGI_LAUNCH_CHECK();
+        }
+        if (split)
+            hipLaunchKernelGGL((probe_update_mfma16_kernel<SR, SA, true>), dim3((A.P + kMG - 1) / kMG), dim3(256), 0, st,
+                               A, rad, dd, pI, pD, oI, oD, w.dirs0, w.Wih, w.Wil, w.Sih, w.Wdh, w.Wdl, w.Sdh, w.mismatch,
+                               tpI, tpD, Rp);
+        else
+            hipLaunchKernelGGL((probe_update_mfma16_kernel<SR, SA, false>), dim3((A.P + kMG - 1) / kMG), dim3(256), 0,
+                               st, A, rad, dd, pI, pD, oI, oD, w.dirs0, w.Wih, w.Wil, w.Sih, w.Wdh, w.Wdl, w.Sdh,
+                               w.mismatch, tpI, tpD, Rp);
         LUXGI_LAUNCH_CHECK();
         // probes whose directions differ from row 0: reference arithmetic with their own directions
         UpdateArgs F = A;

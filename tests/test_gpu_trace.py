@@ -43,9 +43,10 @@ def test_mip_matches_oracle():
     np.testing.assert_allclose(got2, oracle.sdf_mip(s2, 96, max_distance=2.0), rtol=4e-7, atol=1e-7)
 
 
-def _gpu_trace(u, g, sdf, mip, atlas, params, count=False, storage=abi.STORAGE_FP32, offsets=None):
+def _gpu_trace(u, g, sdf, mip, atlas, params, count=False, storage=abi.STORAGE_FP32, offsets=None, sky=None,
+               layout=abi.LAYOUT_LINEAR):
     from luxgi_amd.ddgi import DeviceScene, DDGIVolume
-    ds = DeviceScene(g, sdf, mip, atlas)
+    ds = DeviceScene(g, sdf, mip, atlas, layout=layout, sky_faces=sky, sky_size=0 if sky is None else sky.shape[1])
     vol = DDGIVolume(u, ray_storage=storage, flags_buffer=True)
     if offsets is not None:
         vol.offsets.copy_(torch.from_numpy(offsets))
@@ -95,6 +96,26 @@ def test_trace_unorm8_atlas_and_offsets():
     gpu = _gpu_trace(u, g, sdf, mip, atlas, p, offsets=off)
     ref = _scenes.oracle_trace(u, g, sdf, mip, atlas, p, probe_offsets=off)
     _compare(u, g, gpu, ref, 2.0 / 64)
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_trace_sky_cubemap_matches_oracle(order):
+    """Misses sample the sky cube map (GISDFRays.comp:121-124); with the hit sort (order 0) the
+    march stores them, in ray order the shade pass does."""
+    prims, sdf, mip, g, atlas = _scenes.small_scene()
+    u = _scenes.small_uniform(rays=128, counts=(5, 4, 6))
+    p = _scenes.trace_params(rot_seed=17)
+    p.shadeOrder = order
+    sky = np.random.default_rng(8).uniform(0.0, 3.0, size=(6, 8, 8, 4)).astype(np.float32)
+    gpu = _gpu_trace(u, g, sdf, mip, atlas, p, sky=sky, layout=abi.LAYOUT_BRICK4)
+    ref = _scenes.oracle_trace(u, g, sdf, mip, atlas, p, sky_faces=sky, sky_size=8)
+    _compare(u, g, gpu, ref, 2.0 / 64)
+    miss = (gpu[2] == 0) & (ref[2] == 0)
+    assert miss.sum() > 50
+    # the fp32 ray direction may differ by 2e-6 + 4e-7 R (see _compare); through the cube map
+    # (size / 2 / |major axis| <= 6.9 texels per unit at size 8, texel steps <= 3) that is <= 1.5e-3
+    np.testing.assert_allclose(gpu[0][miss][:, :3], ref[0][miss][:, :3], atol=1.5e-3)
+    assert np.ptp(gpu[0][miss][:, 0]) > 0.5  # the misses saw different sky texels
 
 
 def test_trace_counts_match_oracle():
@@ -209,3 +230,27 @@ def test_trace_fp16_scaled_bricks_and_fp32_refine_vs_oracle():
         hit = (got[2] == 1) & (ref[2] == 1)
         errs.append(float(np.abs(got[1][..., 3] - ref[1][..., 3])[hit].mean()))
     assert errs[1] <= errs[0] * 1.01 + 1e-7, errs
+
+
+def test_trace_shade_order_is_bitwise_identical():
+    """Shading the front hits grouped by surface-atlas chunk (the default: the march writes the
+    misses and inside hits, a counting sort orders the rest) or every ray in ray order gives the
+    same outputs bit for bit; the counting trace (always in ray order) agrees too."""
+    from luxgi_amd.ddgi import DDGIVolume, DeviceScene
+    prims, sdf, mip, g, atlas = _scenes.small_scene()
+    u = _scenes.small_uniform(rays=96, counts=(7, 5, 6))
+    outs = []
+    for order, count in ((0, False), (1, False), (0, True)):
+        p = _scenes.trace_params(rot_seed=29)
+        p.shadeOrder = order
+        ds = DeviceScene(g, sdf, mip, atlas, layout=abi.LAYOUT_BRICK4)
+        vol = DDGIVolume(u, flags_buffer=True)
+        counters = torch.zeros(8, dtype=torch.int64, device="cuda") if count else None
+        for _ in range(2):  # the second call reuses the workspace: the bins must start from zero again
+            vol.trace_rays(ds, p, counters=counters)
+        torch.cuda.synchronize()
+        outs.append((vol.radiance.cpu().numpy(), vol.dir_dist.cpu().numpy(), vol.ray_flags.cpu().numpy()))
+    assert (outs[0][2] == 1).any() and (outs[0][2] == 0).any()
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)

@@ -137,3 +137,44 @@ def test_atlas_constant_colour_box():
     np.testing.assert_allclose(got, s[:, None] * colour[None, :].astype(np.float64), rtol=1e-6)
     assert np.all(s > 0) and np.all(s <= 1 + 1e-6)
     assert (np.abs(s - 1) < 1e-6).mean() > 0.5
+
+
+def _cube_texel_directions(size):
+    """Unit direction through the centre of every texel of a cube map, face order +x -x +y -y +z -z,
+    from the major-axis table of the Vulkan cube-map lookup (face, s_c, t_c)."""
+    out = np.zeros((6, size, size, 4), np.float32)
+    s = (np.arange(size) + 0.5) / size * 2.0 - 1.0  # s_c / m_a at texel centres
+    sc, tc = np.meshgrid(s, s)  # [row = t][col = s]
+    one = np.ones_like(sc)
+    dirs = [np.stack(v, -1) for v in (
+        (one, -tc, -sc),    # +x: s_c = -z, t_c = -y
+        (-one, -tc, sc),    # -x: s_c = +z, t_c = -y
+        (sc, one, tc),      # +y: s_c = +x, t_c = +z
+        (sc, -one, -tc),    # -y: s_c = +x, t_c = -z
+        (sc, -tc, one),     # +z: s_c = +x, t_c = -y
+        (-sc, -tc, -one),   # -z: s_c = -x, t_c = -y
+    )]
+    for f, d in enumerate(dirs):
+        out[f, ..., :3] = d / np.linalg.norm(d, axis=-1, keepdims=True)
+    return out
+
+
+def test_sky_cubemap_orientation():
+    """Misses return sky(dir) (GISDFRays.comp:121-124). A cube map whose texels hold their own
+    centre direction must return (close to) the ray direction itself: a face stored flipped or
+    transposed returns a direction far from it. The bound is the clamp-to-edge half texel at face
+    borders (~1/size) and the bilinear chord sag inside a face."""
+    res, size = 32, 16
+    sdf = np.ones((res, res, res), np.float32)  # empty space: every ray misses
+    mip = oracle.sdf_mip(sdf, res, max_distance=2.0)
+    g = abi.make_sdf_data(resolution=res)
+    prims = scene.scene_small(seed=1)
+    atlas = scene.build_atlas(prims, resolution=512, tile=32, threshold=1.05 * 2 / res)
+    faces = _cube_texel_directions(size)
+    u = abi.make_uniform((2, 2, 2), (-0.5, -0.5, -0.5), (1.0, 1.0, 1.0), 512)
+    rad, dd, fl, _ = _scenes.oracle_trace(u, g, sdf, mip, atlas, _scenes.trace_params(rot_seed=4),
+                                          sky_faces=faces, sky_size=size)
+    assert np.all(fl == 0)
+    err = np.linalg.norm(rad[..., :3] - dd[..., :3], axis=-1)
+    assert err.max() < 0.75 / size, err.max()
+    assert np.median(err) < 0.02, np.median(err)

@@ -376,6 +376,87 @@ int luxgi_sdf_build_guard(const float* sdf, int32_t width, int32_t height, int32
                           void* stream);
 
 /* ------------------------------------------------------------------------- */
+/* Surface atlas: chunk culling and surface-cache lighting                    */
+/* ------------------------------------------------------------------------- */
+
+/* Workspace bytes of luxgi_atlas_cull_chunks (fixed: two 40^3 uint arrays and the scan totals). */
+size_t luxgi_atlas_cull_workspace_size(void);
+
+/* SDFCulling.comp:36-101 (dispatch GlobalSurfaceAtlas.cpp:607-641): for each of the 40^3 chunks
+ * of chunkSize * [-20, 20)^3, the objects (data->objectsCount entries of `objects`) whose bounding
+ * sphere objectBounds touches the chunk box, in increasing index order. Writes chunks[40^3]
+ * (offset of the chunk's run [count, id0, id1, ...] in culledObjects, 0 = none) and
+ * culledObjects (data->culledObjectsCapacity entries; [0] = 1 + the sum of count + 1 over the
+ * non-empty chunks, the counter value the reference leaves). Runs are placed in chunk-address
+ * order (the reference's atomicAdd order is timing-dependent); an empty chunk, or one whose run
+ * would end past the capacity, gets offset 0 (the reference writes the chunk address into
+ * chunks[0] instead, SDFCulling.comp:60,69). */
+int luxgi_atlas_cull_chunks(const LuxgiSurfaceAtlasData* data, const LuxgiAtlasObject* objects, uint32_t* chunks,
+                            uint32_t* culledObjects, void* workspace, size_t workspaceBytes, void* stream);
+
+/* Light: Shaders/Common/Light.glsl:19-28 (host Scene/Component/Light.h:21-31), 64 bytes.
+ * type: 0 directional, 1 spot, 2 point (Light.glsl:13-15). */
+typedef struct LuxgiLight {
+    float color[4];
+    float position[4];
+    float direction[4];
+    float intensity;
+    float radius;
+    float type;
+    float angle;
+} LuxgiLight;
+
+#define LUXGI_MAX_LIGHTS 32         /* MAX_LIGHTS, Light.glsl:30 */
+#define LUXGI_SURFACE_EMISSIVE 0x1u /* CopyEmissive.frag:22-27: light = emissive        */
+#define LUXGI_SURFACE_DIRECT   0x2u /* SDFDeferredLight.frag:79-130: += per light        */
+#define LUXGI_SURFACE_INDIRECT 0x4u /* SDFAtlasIndirectLight.frag:44-68: += DDGI bounce  */
+
+/* Surface-cache G-buffer (GlobalSurfaceAtlas.cpp:407-410), [res][res] texels each:
+ * color    RGBA16F (albedo rgb, world x)               SDFDeferredColor.frag
+ * normal   RGBA16F (octahedral normal xy, world y, z)
+ * pbr      RGBA8 UNORM (metallic, roughness, ...)
+ * emissive RGBA16F (rgb; R11G11B10 in the reference) */
+typedef struct LuxgiSurfaceGBuffer {
+    const void* color;
+    const void* normal;
+    const void* pbr;
+    const void* emissive;
+    int32_t     resolution;
+    int32_t     pad;
+} LuxgiSurfaceGBuffer;
+
+/* SDFDeferredLight.frag:31-36 / SDFAtlasIndirectLight.frag:37-40 uniforms and the passes to run. */
+typedef struct LuxgiSurfaceLightParams {
+    float    cameraPosition[4]; /* xyz; w = shadow bias (GlobalSurfaceAtlas.cpp:99,946) */
+    float    indirectIntensity; /* IrradianceVolume intensity (GlobalSurfaceAtlas.cpp:1043) */
+    uint32_t passes;            /* LUXGI_SURFACE_* */
+    int32_t  numLights;         /* entries of `lights`, <= LUXGI_MAX_LIGHTS */
+    int32_t  maxSteps;          /* shadow-ray steps per cascade; 0 = 250 (SDFCommon.glsl:143) */
+} LuxgiSurfaceLightParams;
+
+/* One tile quad of the lighting draws (GlobalSurfaceAtlas.cpp:554-574). */
+typedef struct LuxgiSurfaceDraw {
+    uint32_t tile;      /* index into the tile buffer; each tile at most once per call */
+    uint32_t lightMask; /* bit l set: light l's draw list holds this tile (the host culling of
+                           GlobalSurfaceAtlas.cpp:913-940) */
+} LuxgiSurfaceDraw;
+
+/* Surface-cache lighting of the drawn tiles into lightCache ([res][res] RGBA, FP32 or FP16,
+ * alpha 1): emissive + sum of direct(light l) over the lights of the tile's mask + DDGI bounce,
+ * per LUXGI_SURFACE_* pass. Each draw covers its tile's full rect; the G-buffer is sampled
+ * bilinearly (repeat) at ((p + 0.5 - X) / Wt) * extends.zw + extends.xy, as the reference's tile
+ * quads do. The direct term traces a shadow ray through the global SDF (`sdf`, any layout and
+ * storage of the trace) from world + normal * bias with cascadeTraceStartBias 2
+ * (SDFDeferredLight.frag:104-112). The bounce needs `ddgi` and `probes` (nullable otherwise).
+ * `lights`, `tiles`, `draws` and every image are device memory; a draw naming a tile >= numTiles
+ * is skipped. Texels outside the drawn tiles are left untouched. */
+int luxgi_surface_light(const LuxgiSurfaceLightParams* params, const LuxgiLight* lights,
+                        const LuxgiAtlasTile* tiles, int32_t numTiles, const LuxgiSurfaceDraw* draws, int32_t numDraws,
+                        const LuxgiSurfaceGBuffer* gbuffer, const LuxgiGlobalSDFData* sdfData,
+                        const LuxgiSDFVolume* sdf, const LuxgiDDGIUniform* ddgi, const LuxgiProbeAtlas* probes,
+                        int32_t outStorage, void* lightCache, void* stream);
+
+/* ------------------------------------------------------------------------- */
 /* SVGF (reflection = RGB, shadow = scalar)                                   */
 /* ------------------------------------------------------------------------- */
 

@@ -31,6 +31,9 @@ SIGNATURES = {
     "luxgi_sdf_to_bricks": (C.c_int, [P, I, I, I, I, P, P, P]),
     "luxgi_sdf_build_guard": (C.c_int, [P, I, I, I, P, P]),
     "luxgi_atlas_pack": (C.c_int, [P, P, I, P, P]),
+    "luxgi_atlas_cull_workspace_size": (C.c_size_t, []),
+    "luxgi_atlas_cull_chunks": (C.c_int, [P, P, P, P, P, C.c_size_t, P]),
+    "luxgi_surface_light": (C.c_int, [P, P, P, I, P, I, P, P, P, P, P, I, P, P]),
     "luxgi_svgf_tile_count": (I, [I, I]),
     "luxgi_svgf_reflection_reproject": (C.c_int, [P, I, I, I, P, P, P, P, P, P, P, P, P]),
     "luxgi_svgf_reflection_atrous": (C.c_int, [P, I, I, I, P, P, P, P, P, P]),

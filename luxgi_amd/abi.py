@@ -145,9 +145,36 @@ class ShadowAtrousParams(C.Structure):
                 ("near_", C.c_float), ("far_", C.c_float)]
 
 
+class Light(C.Structure):  # Shaders/Common/Light.glsl:19-28
+    _fields_ = [("color", f4), ("position", f4), ("direction", f4), ("intensity", C.c_float),
+                ("radius", C.c_float), ("type", C.c_float), ("angle", C.c_float)]
+
+
+LIGHT_DIRECTIONAL, LIGHT_SPOT, LIGHT_POINT = 0.0, 1.0, 2.0  # Light.glsl:13-15
+MAX_LIGHTS = 32
+SURFACE_EMISSIVE = 0x1
+SURFACE_DIRECT = 0x2
+SURFACE_INDIRECT = 0x4
+
+
+class SurfaceGBuffer(C.Structure):
+    _fields_ = [("color", C.c_void_p), ("normal", C.c_void_p), ("pbr", C.c_void_p), ("emissive", C.c_void_p),
+                ("resolution", C.c_int32), ("pad", C.c_int32)]
+
+
+class SurfaceLightParams(C.Structure):
+    _fields_ = [("cameraPosition", f4), ("indirectIntensity", C.c_float), ("passes", C.c_uint32),
+                ("numLights", C.c_int32), ("maxSteps", C.c_int32)]
+
+
+class SurfaceDraw(C.Structure):
+    _fields_ = [("tile", C.c_uint32), ("lightMask", C.c_uint32)]
+
+
 _SIZES = {DDGIUniform: 96, GlobalSDFData: 96, SurfaceAtlasData: 32, AtlasObject: 128, AtlasTile: 96,
           TraceParams: 112, ReflectionReprojectParams: 168, ReflectionAtrousParams: 24,
-          ShadowReprojectParams: 80, ShadowAtrousParams: 32, SampleProbeParams: 80}
+          ShadowReprojectParams: 80, ShadowAtrousParams: 32, SampleProbeParams: 80, Light: 64,
+          SurfaceGBuffer: 40, SurfaceLightParams: 32, SurfaceDraw: 8}
 for _t, _n in _SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
 

@@ -297,4 +297,37 @@ __device__ __forceinline__ float sample_volume<float>(const float* __restrict__ 
     return W >= 2 ? sample3d_pairs(vol, W, H, D, u, v, w) : sample3d<float>(vol, W, H, D, u, v, w);
 }
 
+// ---------------------------------------------------------------------------
+// prefix sums
+// ---------------------------------------------------------------------------
+// inclusive prefix sum across the 64 lanes of a wave
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, (unsigned)o);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan of in[0, n) within blocks of BLOCK elements (out may alias in); the total of
+// block b goes to totals[b]. Launch ceil(n / BLOCK) blocks of BLOCK lanes.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void block_scan_kernel(const uint32_t* in, uint32_t* out,
+                                                           uint32_t* __restrict__ totals, int n) {
+    static_assert(BLOCK % 64 == 0 && BLOCK <= 1024, "whole waves, one block");
+    __shared__ uint32_t waveSum[BLOCK / 64];
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t v = i < n ? in[i] : 0u;
+    const uint32_t x = wave_inclusive_scan(v);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) waveSum[wave] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int k = 0; k < wave; ++k) before += waveSum[k];
+    if (i < n) out[i] = before + x - v;
+    if (threadIdx.x == BLOCK - 1) totals[blockIdx.x] = before + x;
+}
+
 }  // namespace luxgi

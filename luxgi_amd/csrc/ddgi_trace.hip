@@ -570,33 +570,6 @@ constexpr int kScanBlock = 1024;
 constexpr int kScanBlocks = (kChunkBins + kScanBlock - 1) / kScanBlock;
 static_assert(kScanBlocks <= 64, "the block totals are scanned by one wave");
 
-// inclusive prefix sum across the 64 lanes of a wave
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = (uint32_t)__shfl_up((int)x, (unsigned)o);
-        if (lane >= o) x += y;
-    }
-    return x;
-}
-
-// exclusive scan of the bins within blocks of kScanBlock; the block totals go to `totals`
-__global__ __launch_bounds__(kScanBlock) void scan_bins_kernel(uint32_t* __restrict__ bins,
-                                                               uint32_t* __restrict__ totals, int n) {
-    __shared__ uint32_t waveSum[kScanBlock / 64];
-    const int i = blockIdx.x * kScanBlock + threadIdx.x;
-    const uint32_t v = i < n ? bins[i] : 0u;
-    const uint32_t x = wave_inclusive_scan(v);
-    const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 63) waveSum[wave] = x;
-    __syncthreads();
-    uint32_t before = 0;
-    for (int k = 0; k < wave; ++k) before += waveSum[k];
-    if (i < n) bins[i] = before + x - v;
-    if (threadIdx.x == kScanBlock - 1) totals[blockIdx.x] = before + x;
-}
-
 // order[start of the chunk + slot] = ray for every front hit; block 0 also stores the hit count
 __global__ __launch_bounds__(256) void scatter_hits_kernel(const uint2* __restrict__ keys,
                                                            const uint32_t* __restrict__ starts,
@@ -688,7 +661,8 @@ static int launch_trace(const TraceArgs& A, const LuxgiSDFVolume* vol, const Lux
                            w.recA, w.recB, w.keys, w.bins, skyFaces, outRad, outDD, rayFlags, counters);
     LUXGI_LAUNCH_CHECK();
     if (A.sortHits) {
-        hipLaunchKernelGGL(scan_bins_kernel, dim3(kScanBlocks), dim3(kScanBlock), 0, st, w.bins, w.totals, kChunkBins);
+        hipLaunchKernelGGL(block_scan_kernel<kScanBlock>, dim3(kScanBlocks), dim3(kScanBlock), 0, st, w.bins, w.bins,
+                           w.totals, kChunkBins);
         LUXGI_LAUNCH_CHECK();
         hipLaunchKernelGGL(scatter_hits_kernel, dim3((int)((nRays + 255) / 256)), dim3(256), 0, st, w.keys, w.bins,
                            w.totals, w.totals + kScanBlocks, w.order, nRays);

@@ -1,0 +1,100 @@
+// sdf_trace.hpp -- tracyGlobalSDF (Shaders/SDF/SDFCommon.glsl:98-194) as a device function, for
+// the passes other than the probe-ray march that trace the global SDF for a yes/no or a hit
+// distance: the surface-cache shadow ray (SDFDeferredLight.frag:104-112) and the screen-space
+// shadow and reflection producers. Step rule, hit test and cascade walk are the probe-ray march's
+// (the trace kernel), with the per-cascade start offset dir * voxel * cascadeTraceStartBias.
+#pragma once
+#include "common.hpp"
+
+namespace luxgi {
+
+template <class SS>
+struct SdfView {
+    const SS* sdf;             // fine volume, LINEAR or BRICK4 layout
+    const SS* mip;             // coarse volume, LINEAR
+    const float* guard;        // optional brick guard (FP32 LINEAR volumes only)
+    const float* brickScale;   // optional per-brick scale (BRICK4)
+    int res, mipRes;
+};
+
+constexpr float kGlobalSdfWorldSize = 60000.0f;  // GLOBAL_SDF_WORLD_SIZE, SDFCommon.glsl:6
+
+struct SdfHitTime {
+    float time;  // -1 = no hit
+    float sdf;
+    int cascade;
+};
+
+template <class SS, bool BRICK>
+__device__ __forceinline__ float sdf_fine(const SdfView<SS>& v, int W, float u, float y, float z) {
+    if constexpr (BRICK) return sample3d_brick4<SS>(v.sdf, W, v.res, v.res, u, y, z, v.brickScale);
+    else return sample_volume<SS>(v.sdf, W, v.res, v.res, u, y, z);
+}
+
+template <class SS, bool BRICK>
+__device__ SdfHitTime trace_global_sdf(const SdfView<SS>& v, const LuxgiGlobalSDFData& g, f3 o, f3 d,
+                                       float maxDistance, float startBias, int maxSteps) {
+    SdfHitTime hit{-1.0f, 0.0f, 0};
+    const int C = (int)g.cascadesCount;
+    const float traceMax = fminf(maxDistance, g.cascadePosDistance[C - 1][3] * 2.0f);
+    const float chunkSizeDistance = 32.0f / g.resolution;   // GLOBAL_SDF_RASTERIZE_CHUNK_SIZE / resolution
+    const float chunkMarginDistance = 4.0f / g.resolution;  // GLOBAL_SDF_RASTERIZE_CHUNK_MARGIN / resolution
+    const int texW = v.res * C, mipW = v.mipRes * C;
+    const f3 traceEnd = o + d * traceMax;
+    float nextStart = 0.0f;
+    for (int cas = 0; cas < C && hit.time < 0.0f; ++cas) {
+        const f3 cpos = mk(g.cascadePosDistance[cas][0], g.cascadePosDistance[cas][1], g.cascadePosDistance[cas][2]);
+        const float half = g.cascadePosDistance[cas][3];
+        const float voxel = g.cascadeVoxelSize[cas];
+        const f3 wp = o + d * (voxel * startBias);
+        // lineHitAABB (SDFCommon.glsl:84-95) from the biased start to the unbiased end
+        const f3 dl = traceEnd - wp;
+        const f3 inv = mk(1.0f / dl.x, 1.0f / dl.y, 1.0f / dl.z);
+        const f3 en = (cpos - mk(half, half, half) - wp) * inv;
+        const f3 ex = (cpos + mk(half, half, half) - wp) * inv;
+        float ix = fmaxf(fminf(en.x, ex.x), fmaxf(fminf(en.y, ex.y), fminf(en.z, ex.z)));
+        float iy = fminf(fmaxf(en.x, ex.x), fminf(fmaxf(en.y, ex.y), fmaxf(en.z, ex.z)));
+        ix = clampf(ix, 0.0f, 1.0f) * traceMax;
+        iy = clampf(iy, 0.0f, 1.0f) * traceMax;
+        ix = fmaxf(ix, nextStart);
+        float stepTime = ix;
+        if (ix >= iy) stepTime = iy;
+        else nextStart = iy;
+        const float cmax = half * 2.0f;
+        for (int step = 0; step < maxSteps && stepTime < iy; ++step) {
+            const f3 p = wp + d * stepTime;
+            const float ux = clampf((p.x - cpos.x) / cmax + 0.5f, 0.0f, 1.0f);
+            const float uy = clampf((p.y - cpos.y) / cmax + 0.5f, 0.0f, 1.0f);
+            const float uz = clampf((p.z - cpos.z) / cmax + 0.5f, 0.0f, 1.0f);
+            const float tx = (float(cas) + ux) / float(C);
+            bool needFine = true;
+            if (v.guard) {  // a guard >= twice the margin proves the fine value is not used
+                const int gx = clampi(int(floorf(tx * float(texW) - 0.5f)), 0, texW - 1);
+                const int gy = clampi(int(floorf(uy * float(v.res) - 0.5f)), 0, v.res - 1);
+                const int gz = clampi(int(floorf(uz * float(v.res) - 0.5f)), 0, v.res - 1);
+                const float gv = v.guard[((size_t)(gz >> 2) * (v.res >> 2) + (gy >> 2)) * (texW >> 2) + (gx >> 2)];
+                needFine = !(gv >= chunkMarginDistance * 2.0f * (1.0f + 1e-5f));
+            }
+            // both lookups are issued together; the fine value is used only under the mip test
+            float sd = sample_volume<SS>(v.mip, mipW, v.mipRes, v.mipRes, tx, uy, uz);
+            const float st = needFine ? sdf_fine<SS, BRICK>(v, texW, tx, uy, uz) : 1.0f;
+            if (sd < chunkSizeDistance) {
+                if (needFine && st < chunkMarginDistance * 2.0f) sd = st;
+            } else {
+                sd = chunkSizeDistance;
+            }
+            sd *= cmax;
+            const float thick = voxel * 0.5f * clampf(stepTime / voxel, 0.0f, 1.0f);
+            if (sd < thick) {
+                hit.time = fmaxf(stepTime + sd - thick, 0.0f);
+                hit.sdf = sd;
+                hit.cascade = cas;
+                break;
+            }
+            stepTime += fmaxf(sd, voxel);
+        }
+    }
+    return hit;
+}
+
+}  // namespace luxgi

@@ -185,8 +185,78 @@ class Atlas:
     """Host-side surface atlas (numpy / ctypes arrays)."""
 
 
+def cull_chunks_np(bounds, chunk_size, capacity=None):
+    """Chunk culling of Shaders/SDF/SDFCulling.comp:36-101 with the runs in chunk-address order (the
+    layout luxgi_atlas_cull_chunks writes): every fp32 operation of boxIntersectsSphere
+    (AtlasCommon.glsl:34-38) in the written order, so the decisions match the GPU pass bit for bit.
+    bounds: (n, 4) sphere centre + radius. Returns (chunks[40^3], culled) with culled[0] the
+    reference's final counter (1 + sum of count + 1 over the non-empty chunks); a run that would
+    end past `capacity` (default: all fit) leaves its chunk at offset 0."""
+    R40 = abi.ATLAS_CHUNKS_RESOLUTION
+    b = np.asarray(bounds, dtype=np.float32).reshape(-1, 4)
+    cs = np.float32(chunk_size)
+    mn = (np.arange(R40, dtype=np.float32) - np.float32(R40 * 0.5)) * cs
+    mx = mn + cs
+    lists = {}
+    for i in range(b.shape[0]):
+        c, r = b[i, :3], b[i, 3]
+        ax = []
+        for a in range(3):  # candidate chunks per axis (a superset), then the exact fp32 test
+            ax.append(np.nonzero((mx >= c[a] - r - cs) & (mn <= c[a] + r + cs))[0])
+        if any(len(v) == 0 for v in ax):
+            continue
+        Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        d = [c[0] - np.minimum(np.maximum(c[0], mn[X]), mx[X]), c[1] - np.minimum(np.maximum(c[1], mn[Y]), mx[Y]),
+             c[2] - np.minimum(np.maximum(c[2], mn[Z]), mx[Z])]
+        d2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]
+        hit = np.sqrt(d2) <= r
+        for addr in ((Z * R40 + Y) * R40 + X)[hit].tolist():
+            lists.setdefault(addr, []).append(i)
+    chunks = np.zeros(R40 ** 3, dtype=np.uint32)
+    culled = [0]
+    cursor = 1
+    for addr in sorted(lists):
+        ids = lists[addr]
+        start, size = cursor, len(ids) + 1
+        cursor += size
+        if capacity is not None and start + size > capacity:
+            continue
+        chunks[addr] = start
+        culled.append(len(ids))
+        culled.extend(ids)
+    culled[0] = cursor
+    return chunks, np.asarray(culled, dtype=np.uint32)
+
+
+def _surface_normals(prim, pts, h=1e-5):
+    """Unit gradient of one primitive's distance at surface points (central differences)."""
+    g = np.zeros_like(pts)
+    one = prim[None] if prim.ndim == 0 else prim
+    for a in range(3):
+        e = np.zeros(3)
+        e[a] = h
+        g[:, a] = (prim_distance_np(one, pts + e) - prim_distance_np(one, pts - e)) / (2.0 * h)
+    return g / np.maximum(np.linalg.norm(g, axis=1, keepdims=True), 1e-30)
+
+
+def _oct_encode_np(n):
+    """octEncode (DDGICommon.glsl:65-72): unit vectors (m, 3) -> (m, 2) in [-1, 1]."""
+    l1 = np.abs(n).sum(axis=1, keepdims=True)
+    r = n[:, :2] / l1
+    neg = n[:, 2] < 0
+    sx = np.where(r[neg, 0] >= 0, 1.0, -1.0)
+    sy = np.where(r[neg, 1] >= 0, 1.0, -1.0)
+    r[neg] = np.stack([(1.0 - np.abs(r[neg, 1])) * sx, (1.0 - np.abs(r[neg, 0])) * sy], axis=1)
+    return r
+
+
 def build_atlas(prims, resolution=512, tile=32, color_storage=abi.STORAGE_FP32, seed=3, threshold=0.05,
-                gi_distance=2.0, emissive_fraction=0.02):
+                gi_distance=2.0, emissive_fraction=0.02, surface_gbuffer=False):
+    """Synthetic global surface atlas of the primitives: 6 tiles per object (the view frames of
+    GlobalSurfaceAtlas.cpp:249-277), tile depth, light-cache colour (albedo with emissive texels),
+    the object/tile buffers and the chunk lists. With surface_gbuffer, also the surface-cache
+    G-buffer of the lit tiles (GlobalSurfaceAtlas.cpp:407-410): world position, octahedral normal,
+    albedo, PBR (metallic, roughness) and emissive per texel, zero where a tile sees nothing."""
     n = len(prims)
     ext = prim_half_extents(prims)
     slot = tile + 1
@@ -202,6 +272,9 @@ def build_atlas(prims, resolution=512, tile=32, color_storage=abi.STORAGE_FP32, 
     k = (np.arange(tile) + 0.5) / (tile - 1)
     TU, TV = np.meshgrid(k, k, indexing="xy")  # [ky][kx]
     emissive_value = 1.0 if color_storage == abi.STORAGE_UNORM8 else 4.0
+    if surface_gbuffer:
+        gb = {k: np.zeros((resolution, resolution, 4), dtype=np.float32) for k in ("color", "normal", "pbr", "emissive")}
+        metallic = np.random.default_rng(seed + 1).uniform(0.0, 0.5, size=n)
     for i in range(n):
         p = prims[i]
         c = p["center"].astype(np.float64)
@@ -234,39 +307,26 @@ def build_atlas(prims, resolution=512, tile=32, color_storage=abi.STORAGE_FP32, 
                 tex = np.clip(tex, 0.0, 1.0)
             color[ty:ty + tile, tx:tx + tile, :3] = tex
             color[ty:ty + tile, tx:tx + tile, 3] = 1.0
-    # chunk table + culled runs (the culling of Shaders/SDF/SDFCulling.comp, done on the host)
-    R40 = abi.ATLAS_CHUNKS_RESOLUTION
-    cs = gi_distance / R40
-    lists = {}
-    lo_edges = (np.arange(R40) - R40 * 0.5) * cs
-    hi_edges = lo_edges + cs
-    lo_edges = lo_edges.copy()
-    hi_edges = hi_edges.copy()
-    lo_edges[0] = -np.inf
-    hi_edges[-1] = np.inf
-    for i in range(n):
-        c = prims[i]["center"].astype(np.float64)
-        rad = float(np.linalg.norm(ext[i])) + threshold
-        rng_ax = []
-        for a in range(3):
-            idx = np.nonzero((hi_edges >= c[a] - rad) & (lo_edges <= c[a] + rad))[0]
-            rng_ax.append(idx)
-        for z in rng_ax[2]:
-            dz = max(lo_edges[z] - c[2], 0.0, c[2] - hi_edges[z])
-            for y in rng_ax[1]:
-                dy = max(lo_edges[y] - c[1], 0.0, c[1] - hi_edges[y])
-                for x in rng_ax[0]:
-                    dx = max(lo_edges[x] - c[0], 0.0, c[0] - hi_edges[x])
-                    if dx * dx + dy * dy + dz * dz <= rad * rad:
-                        lists.setdefault(int(z * R40 * R40 + y * R40 + x), []).append(i)
-    chunks = np.zeros(R40 ** 3, dtype=np.uint32)
-    culled = [0]
-    for addr in sorted(lists):
-        ids = sorted(lists[addr])
-        chunks[addr] = len(culled)
-        culled.append(len(ids))
-        culled.extend(ids)
-    culled = np.asarray(culled, dtype=np.uint32)
+            if surface_gbuffer:
+                seen = dep < 1.0
+                if seen.any():
+                    tp = np.stack([X[seen], Y[seen], dep[seen] * ob[2]], axis=1)
+                    world = c[None, :] + tp @ R  # local = R^T tp; objects are translated, not rotated
+                    nrm = _surface_normals(prims[i:i + 1], world)
+                    oc = _oct_encode_np(nrm)
+                    sl = (slice(ty, ty + tile), slice(tx, tx + tile))
+                    blk = {k: v[sl] for k, v in gb.items()}
+                    albedo = alb[None, None, :] * np.ones((tile, tile, 1), np.float32)
+                    blk["color"][seen] = np.concatenate([albedo[seen], world[:, :1]], axis=1)
+                    blk["normal"][seen] = np.concatenate([oc, world[:, 1:3]], axis=1)
+                    blk["pbr"][seen] = [metallic[i], float(p["roughness"]), 0.0, 0.0]
+                    emv = np.zeros((tile, tile, 4), np.float32)
+                    emv[em] = [emissive_value, emissive_value, emissive_value, 0.0]
+                    blk["emissive"][seen] = emv[seen]
+    # chunk table + culled runs (Shaders/SDF/SDFCulling.comp, here on the host)
+    cs = gi_distance / abi.ATLAS_CHUNKS_RESOLUTION
+    bounds = np.array([list(objects[i].objectBounds) for i in range(n)], dtype=np.float32).reshape(-1, 4)
+    chunks, culled = cull_chunks_np(bounds, cs)
     data = abi.SurfaceAtlasData()
     data.cameraPos[:] = [0.0, 0.0, 0.0]
     data.chunkSize = cs
@@ -287,6 +347,12 @@ def build_atlas(prims, resolution=512, tile=32, color_storage=abi.STORAGE_FP32, 
     else:
         a.color = color
     a.n_objects = n
+    if surface_gbuffer:
+        # stored as the reference's formats: RGBA16F colour/normal/emissive, RGBA8 PBR
+        a.gb_color = gb["color"].astype(np.float16)
+        a.gb_normal = gb["normal"].astype(np.float16)
+        a.gb_emissive = gb["emissive"].astype(np.float16)
+        a.gb_pbr = np.round(np.clip(gb["pbr"], 0.0, 1.0) * 255.0).astype(np.uint8)
     return a
 
 

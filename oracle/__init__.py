@@ -71,6 +71,11 @@ def _declare(L):
     L.oracle_shadow_reproject_f64.argtypes = [P, I, I, P, P, P, P, P, P, P, P, I]
     L.oracle_shadow_atrous_f64.argtypes = [P, I, I, P, P, P, P, P, I]
     L.oracle_hardware_threads.restype = I
+    L.oracle_cull_chunks.argtypes = [C.c_float, P, I, C.c_uint32, P, P]
+    for s in ("f64", "f32"):
+        getattr(L, "oracle_surface_light_" + s).argtypes = [P, P, P, I, P, I, I, P, P, P, P, P, P, P, I, I, P, P, P,
+                                                            P, I]
+    L.oracle_brdf.argtypes = [P, P, C.c_double, C.c_double, P, P, P, P]
 
 
 def _p(a):
@@ -326,4 +331,48 @@ def texture_coord_from_direction(d, probe_index, width, height, side):
     dv = np.ascontiguousarray(d, dtype=np.float64)
     out = np.zeros(2)
     L.oracle_texture_coord_from_direction(_p(dv), probe_index, width, height, side, _p(out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Surface atlas: chunk culling and surface-cache lighting
+# ---------------------------------------------------------------------------
+def cull_chunks(chunk_size, objects, n, capacity):
+    """SDFCulling.comp:36-101 with runs in chunk-address order. objects: ctypes array of
+    LuxgiAtlasObject. Returns (chunks[40^3] uint32, culled[capacity] uint32)."""
+    chunks = np.zeros(40 ** 3, dtype=np.uint32)
+    culled = np.zeros(max(int(capacity), 1), dtype=np.uint32)
+    lib().oracle_cull_chunks(float(chunk_size), C.cast(objects, C.c_void_p) if n > 0 else None, int(n),
+                             int(capacity), _p(chunks), _p(culled))
+    return chunks, culled
+
+
+def surface_light(params, lights, tiles, draws, res, gb_color, gb_normal, gb_pbr, gb_emissive, sdf_data=None,
+                  sdf=None, mip=None, sdf_res=0, mip_res=0, u=None, irradiance=None, depth_atlas=None,
+                  precision="f64", nthreads=None, out=None):
+    """Surface-cache lighting of the drawn tiles. lights/tiles/draws: ctypes arrays; G-buffer
+    images float [res][res][4] (values as the GPU reads them). Returns [res][res][4]; texels
+    outside the draws keep `out`'s values (zeros by default)."""
+    dt = np.float64 if precision == "f64" else np.float32
+    if out is None:
+        out = np.zeros((res, res, 4), dtype=dt)
+    out = np.ascontiguousarray(out, dtype=dt)
+    imgs = [_f32(a) for a in (gb_color, gb_normal, gb_pbr, gb_emissive)]
+    sdf = None if sdf is None else _f32(sdf)
+    mip = None if mip is None else _f32(mip)
+    irr = None if irradiance is None else _f32(irradiance)
+    dep = None if depth_atlas is None else _f32(depth_atlas)
+    fn = getattr(lib(), "oracle_surface_light_" + precision)
+    fn(C.byref(params), C.cast(lights, C.c_void_p) if lights is not None else None, C.cast(tiles, C.c_void_p),
+       len(tiles), C.cast(draws, C.c_void_p), len(draws), int(res), *[_p(a) for a in imgs],
+       C.byref(sdf_data) if sdf_data is not None else None, _p(sdf), _p(mip), int(sdf_res), int(mip_res),
+       C.byref(u) if u is not None else None, _p(irr), _p(dep), _p(out), nthreads or threads())
+    return out
+
+
+def brdf(albedo, n, rough, metal, view, h, l):
+    """Raytraced/BRDF.glsl:65-79 (the 7-argument BRDF) at one point; returns rgb."""
+    out = np.zeros(3)
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (albedo, n, view, h, l)]
+    lib().oracle_brdf(_p(a[0]), _p(a[1]), float(rough), float(metal), _p(a[2]), _p(a[3]), _p(a[4]), _p(out))
     return out

@@ -348,11 +348,13 @@ template <class T> inline v2<T> line_hit_aabb(v3<T> a, v3<T> b, v3<T> mn, v3<T> 
     return {clampT(x, T(0), T(1)), clampT(y, T(0), T(1))};
 }
 
-// tracyGlobalSDF (SDFCommon.glsl:98-194), stepScale 1, needsHitNormal true,
-// cascadeTraceStartBias 0 (GISDFRays.comp:79-87).
+// tracyGlobalSDF (SDFCommon.glsl:98-194), stepScale 1; the probe rays use needsHitNormal true and
+// cascadeTraceStartBias 0 (GISDFRays.comp:79-87), the surface-cache shadow rays false and 2
+// (SDFDeferredLight.frag:104-111).
 template <class T>
 Hit<T> trace_global_sdf(const LuxgiGlobalSDFData& g, const float* sdf, const float* mip, int res, int mipRes,
-                        v3<T> origin, v3<T> dir, T maxDistance, int maxSteps, T hitEps, TraceCounts* cnt) {
+                        v3<T> origin, v3<T> dir, T maxDistance, int maxSteps, T hitEps, TraceCounts* cnt,
+                        T cascadeTraceStartBias = T(0), bool needsHitNormal = true) {
     Hit<T> hit;
     const int C = (int)g.cascadesCount;
     T traceMax = std::min(maxDistance, T(g.cascadePosDistance[C - 1][3]) * T(2));
@@ -365,7 +367,7 @@ Hit<T> trace_global_sdf(const LuxgiGlobalSDFData& g, const float* sdf, const flo
         T half = T(g.cascadePosDistance[c][3]);
         T voxel = T(g.cascadeVoxelSize[c]);
         T voxelHalf = voxel * T(0.5f);
-        v3<T> wpos = origin;  // + dir * (voxel * 0)
+        v3<T> wpos = origin + dir * (voxel * cascadeTraceStartBias);
         v2<T> is = line_hit_aabb(wpos, traceEnd, cpos - mk3(half, half, half), cpos + mk3(half, half, half));
         is.x *= traceMax;
         is.y *= traceMax;
@@ -396,6 +398,7 @@ Hit<T> trace_global_sdf(const LuxgiGlobalSDFData& g, const float* sdf, const flo
                 hit.time = std::max(stepTime + sd - thick, T(0));
                 hit.cascade = c;
                 hit.sdf = sd;
+                if (!needsHitNormal) break;
                 T o = T(1.0f) / T(g.resolution);
                 auto s = [&](T a, T b, T cc) { return sample3d<T>(sdf, res * C, res, res, a, b, cc); };
                 T xp = s(tuv.x + o, tuv.y, tuv.z), xn = s(tuv.x - o, tuv.y, tuv.z);
@@ -1144,6 +1147,181 @@ inline void sample_probe_pixel(const LuxgiDDGIUniform& u, const LuxgiSampleProbe
 // ---------------------------------------------------------------------------
 // threading helper: split [0, n) over `threads` std::threads
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Surface-atlas chunk culling (Shaders/SDF/SDFCulling.comp:36-101)
+// ---------------------------------------------------------------------------
+// boxIntersectsSphere (AtlasCommon.glsl:34-38) in fp32, operations in the written order:
+// chunkMin = (vec3(chunkCoord) - 20) * chunkSize, chunkMax = chunkMin + chunkSize
+// (SDFCulling.comp:42-43), distance(centre, clamp(centre, min, max)) <= radius.
+inline bool chunk_touches_sphere(float cs, int cx, int cy, int cz, const float* s) {
+    const int c[3] = {cx, cy, cz};
+    float d2 = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        volatile float mn = (float(c[a]) - 20.0f) * cs;  // volatile: one rounding per operation
+        volatile float mx = mn + cs;
+        volatile float d = s[a] - std::min(std::max(s[a], (float)mn), (float)mx);
+        volatile float sq = d * d;
+        d2 = d2 + sq;
+    }
+    return std::sqrt(d2) <= s[3];
+}
+
+// Lists in chunk-address order: chunks[addr] = offset of [count, ids...] in culled, 0 when the
+// chunk touches no object or its run would end past `capacity`; culled[0] = 1 + sum over the
+// non-empty chunks of (count + 1), the counter value SDFCulling.comp:64-65 leaves.
+inline void cull_chunks(float chunkSize, const float* bounds, int stride, int n, uint32_t capacity, uint32_t* chunks,
+                        uint32_t* culled) {
+    uint32_t cursor = 1;
+    std::vector<uint32_t> ids;
+    for (int addr = 0; addr < 40 * 40 * 40; ++addr) {
+        const int cx = addr % 40, cy = (addr / 40) % 40, cz = addr / 1600;
+        ids.clear();
+        for (int i = 0; i < n; ++i)
+            if (chunk_touches_sphere(chunkSize, cx, cy, cz, bounds + (size_t)i * stride)) ids.push_back((uint32_t)i);
+        chunks[addr] = 0;
+        if (ids.empty()) continue;
+        const uint32_t start = cursor, size = (uint32_t)ids.size() + 1;
+        cursor += size;
+        if ((unsigned long long)start + size > capacity) continue;
+        chunks[addr] = start;
+        culled[start] = (uint32_t)ids.size();
+        for (size_t k = 0; k < ids.size(); ++k) culled[start + 1 + k] = ids[k];
+    }
+    culled[0] = cursor;
+}
+
+// ---------------------------------------------------------------------------
+// Surface-cache lighting: CopyEmissive.frag:22-27, SDFDeferredLight.frag:41-130,
+// SDFAtlasIndirectLight.frag:44-68; one texel of one tile draw. The reference
+// draws each tile as a quad over its full rect (GlobalSurfaceAtlas.cpp:554-574)
+// while the tile UV maps onto the rect minus the padding (extends.zw,
+// GlobalSurfaceAtlas.cpp:272-276), so the G-buffer is sampled with bilinear
+// filtering at atlasUV = ((p + 0.5 - X) / Wt) * extends.zw + extends.xy.
+// Passes blend: emissive copy (blend none), + direct per light, + indirect
+// (GlobalSurfaceAtlas.cpp:886-1085).
+// ---------------------------------------------------------------------------
+// Raytraced/BRDF.glsl:12-79, the 7-argument BRDF
+template <class T>
+inline v3<T> brdf_ggx(v3<T> albedo, v3<T> n, T rough, T metal, v3<T> view, v3<T> h, v3<T> l) {
+    const T eps = T(0.00001f);  // BRDF.glsl:9
+    const v3<T> F0{mixT(T(0.04f), albedo.x, metal), mixT(T(0.04f), albedo.y, metal), mixT(T(0.04f), albedo.z, metal)};
+    const T cosLi = std::max(T(0), dot(n, l)), cosLh = std::max(T(0), dot(n, h)), NdotV = std::max(T(0), dot(n, view));
+    const T c = clampT(T(1) - std::max(dot(h, view), T(0)), T(0), T(1));
+    const T c5 = c * c * c * c * c;
+    const v3<T> F{F0.x + (T(1) - F0.x) * c5, F0.y + (T(1) - F0.y) * c5, F0.z + (T(1) - F0.z) * c5};
+    const T alpha = rough * rough, a2 = alpha * alpha;
+    const T denom = cosLh * cosLh * (a2 - T(1)) + T(1);
+    const T D = a2 / (T(kPi) * denom * denom);
+    const T r = rough + T(1), k = r * r / T(8);
+    const T G = cosLi / (cosLi * (T(1) - k) + k) * (NdotV / (NdotV * (T(1) - k) + k));
+    const T spec = D * G / std::max(eps, T(4) * cosLi * NdotV);
+    const T km = T(1) - metal;
+    return {(T(1) - F.x) * km * albedo.x / T(kPi) + F.x * spec, (T(1) - F.y) * km * albedo.y / T(kPi) + F.y * spec,
+            (T(1) - F.z) * km * albedo.z / T(kPi) + F.z * spec};
+}
+
+// fetchLight (SDFDeferredLight.frag:41-76); Light.glsl:13-15 types 0 directional, 1 spot, 2 point
+template <class T>
+inline void fetch_light(const LuxgiLight& L, v3<T> wp, v3<T>& Wi, T& tmax, T& atten) {
+    Wi = {0, 0, 0};
+    tmax = T(kGlobalSDFWorldSize);
+    atten = T(1);
+    const v3<T> lp{T(L.position[0]), T(L.position[1]), T(L.position[2])};
+    const v3<T> ld{T(L.direction[0]), T(L.direction[1]), T(L.direction[2])};
+    if (L.type == 0.0f) {
+        Wi = {-ld.x, -ld.y, -ld.z};
+    } else if (L.type == 2.0f) {
+        const v3<T> dv = lp - wp;
+        const T dist = length(dv);
+        Wi = normalize(dv);
+        atten = T(L.radius) / (dist * dist + T(1));
+        tmax = dist;
+    } else if (L.type == 1.0f) {
+        const v3<T> dv = lp - wp;
+        const T cutoff = T(1) - T(L.angle);
+        Wi = normalize(dv);
+        const T dist = length(dv);
+        const T theta = dot(Wi, ld);
+        const T e = cutoff - cutoff * T(0.9f);
+        atten = (theta - cutoff) / e;
+        atten *= T(L.radius) / (dist * dist + T(1));
+        atten = clampT(atten, T(0), T(1));
+        tmax = dist;
+    }
+}
+
+// One texel (px, py) of a tile draw. gb*: float RGBA [res][res] images (the G-buffer texels
+// already converted from their storage); sdf/mip: the LINEAR FP32 global SDF; irr/dep: the probe
+// atlases (float RGBA / RG), or null for no indirect pass.
+template <class T>
+inline void surface_light_texel(const LuxgiSurfaceLightParams& prm, const LuxgiLight* lights, uint32_t lightMask,
+                                const LuxgiAtlasTile& tile, int res, int px, int py, const float* gbColor,
+                                const float* gbNormal, const float* gbPbr, const float* gbEmissive,
+                                const LuxgiGlobalSDFData& g, const float* sdf, const float* mip, int sdfRes,
+                                int mipRes, const LuxgiDDGIUniform* u, const float* irr, const float* dep, T out[4]) {
+    const int X = (int)std::lrint(tile.extends[0] * (float)res), Y = (int)std::lrint(tile.extends[1] * (float)res);
+    const int Wt = (int)std::lrint(tile.extends[2] * (float)res) + 1;
+    const int Ht = (int)std::lrint(tile.extends[3] * (float)res) + 1;
+    const T au = ((T(px) + T(0.5) - T(X)) / T(Wt)) * T(tile.extends[2]) + T(tile.extends[0]);
+    const T av = ((T(py) + T(0.5) - T(Y)) / T(Ht)) * T(tile.extends[3]) + T(tile.extends[1]);
+    T acc[3] = {0, 0, 0};
+    if (prm.passes & LUXGI_SURFACE_EMISSIVE) {
+        T e[4];
+        sample2d_repeat<T>(gbEmissive, res, res, 4, au, av, e);
+        acc[0] = e[0]; acc[1] = e[1]; acc[2] = e[2];
+    }
+    const int nl = (prm.passes & LUXGI_SURFACE_DIRECT) ? prm.numLights : 0;
+    const bool indirect = (prm.passes & LUXGI_SURFACE_INDIRECT) && u && irr && dep;
+    if (nl > 0 || indirect) {
+        T c[4], nt[4], pbr[4];
+        sample2d_repeat<T>(gbColor, res, res, 4, au, av, c);
+        sample2d_repeat<T>(gbNormal, res, res, 4, au, av, nt);
+        sample2d_repeat<T>(gbPbr, res, res, 4, au, av, pbr);
+        const v3<T> n = octahedral_to_direction<T>(nt[0], nt[1]);
+        const v3<T> wp{c[3], nt[2], nt[3]};
+        const v3<T> cam{T(prm.cameraPosition[0]), T(prm.cameraPosition[1]), T(prm.cameraPosition[2])};
+        const T sb = T(prm.cameraPosition[3]);
+        const v3<T> view = normalize(cam - wp);
+        const int steps = prm.maxSteps > 0 ? prm.maxSteps : 250;
+        for (int l = 0; l < nl; ++l) {
+            if (!((lightMask >> l) & 1u)) continue;
+            v3<T> L;
+            T dist, atten;
+            fetch_light<T>(lights[l], wp, L, dist, atten);
+            const T NoL = dot(n, L);
+            const T bias = T(2) * sb * clampT(T(1) - NoL, T(0), T(1)) + sb;
+            T shadow = T(1);
+            if (NoL > T(0)) {
+                if (atten > T(0)) {
+                    const Hit<T> h = trace_global_sdf<T>(g, sdf, mip, sdfRes, mipRes, wp + n * sb, L, dist - bias,
+                                                         steps, T(0), nullptr, T(2), false);
+                    shadow = h.time >= T(0) ? T(0) : T(1);
+                }
+            } else {
+                shadow = T(0);
+            }
+            const T inten = std::pow(T(lights[l].intensity), T(1.4f)) + T(0.1f);
+            const v3<T> Lh = normalize(L + view);
+            const T cosLi = std::max(T(0), dot(n, L));
+            const v3<T> b = brdf_ggx<T>({c[0], c[1], c[2]}, n, pbr[1], pbr[0], view, Lh, L);
+            const T s = cosLi * shadow * atten;
+            acc[0] += b.x * T(lights[l].color[0]) * inten * s;
+            acc[1] += b.y * T(lights[l].color[1]) * inten * s;
+            acc[2] += b.z * T(lights[l].color[2]) * inten * s;
+        }
+        if (indirect) {
+            const T m = pbr[0];
+            const T al[3] = {std::min(c[0], T(0.9f)), std::min(c[1], T(0.9f)), std::min(c[2], T(0.9f))};
+            const v3<T> ir = sample_irradiance<T>(*u, wp, n, view, irr, dep);
+            const T I = T(prm.indirectIntensity);
+            acc[0] += I * ((al[0] - al[0] * m) / T(kPi)) * ir.x;
+            acc[1] += I * ((al[1] - al[1] * m) / T(kPi)) * ir.y;
+            acc[2] += I * ((al[2] - al[2] * m) / T(kPi)) * ir.z;
+        }
+    }
+    out[0] = acc[0]; out[1] = acc[1]; out[2] = acc[2]; out[3] = T(1);
+}
+
 template <class F> inline void parallel_for(int n, int threads, F f) {
     if (threads <= 1 || n < 2) {
         for (int i = 0; i < n; ++i) f(i);

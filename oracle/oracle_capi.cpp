@@ -331,3 +331,59 @@ extern "C" void oracle_texture_coord_from_direction(const double* dir, int probe
     out[0] = r.x;
     out[1] = r.y;
 }
+
+// SDFCulling.comp:36-101 over `objects` (LuxgiAtlasObject, objectBounds at float 0 of each 32-float record)
+extern "C" void oracle_cull_chunks(float chunkSize, const LuxgiAtlasObject* objects, int n, uint32_t capacity,
+                                   uint32_t* chunks, uint32_t* culled) {
+    cull_chunks(chunkSize, n > 0 ? objects[0].objectBounds : nullptr, (int)(sizeof(LuxgiAtlasObject) / sizeof(float)),
+                n, capacity, chunks, culled);
+}
+
+// Surface-cache lighting of the drawn tiles into out ([res][res][4], only drawn texels written).
+template <class T>
+static void surface_light_impl(const LuxgiSurfaceLightParams* prm, const LuxgiLight* lights, const LuxgiAtlasTile* tiles,
+                               int numTiles, const LuxgiSurfaceDraw* draws, int numDraws, int res, const float* gbColor,
+                               const float* gbNormal, const float* gbPbr, const float* gbEmissive,
+                               const LuxgiGlobalSDFData* g, const float* sdf, const float* mip, int sdfRes, int mipRes,
+                               const LuxgiDDGIUniform* u, const float* irr, const float* dep, T* out, int threads) {
+    parallel_for(numDraws, threads, [&](int d) {
+        const LuxgiSurfaceDraw dr = draws[d];
+        if (dr.tile >= (uint32_t)numTiles) return;
+        const LuxgiAtlasTile& t = tiles[dr.tile];
+        const int X = (int)std::lrint(t.extends[0] * (float)res), Y = (int)std::lrint(t.extends[1] * (float)res);
+        const int Wt = (int)std::lrint(t.extends[2] * (float)res) + 1, Ht = (int)std::lrint(t.extends[3] * (float)res) + 1;
+        for (int py = Y; py < Y + Ht; ++py)
+            for (int px = X; px < X + Wt; ++px) {
+                if (px < 0 || py < 0 || px >= res || py >= res) continue;
+                surface_light_texel<T>(*prm, lights, dr.lightMask, t, res, px, py, gbColor, gbNormal, gbPbr, gbEmissive,
+                                       *g, sdf, mip, sdfRes, mipRes, u, irr, dep, out + ((size_t)py * res + px) * 4);
+            }
+    });
+}
+extern "C" void oracle_surface_light_f64(const LuxgiSurfaceLightParams* prm, const LuxgiLight* lights,
+                                         const LuxgiAtlasTile* tiles, int numTiles, const LuxgiSurfaceDraw* draws,
+                                         int numDraws, int res, const float* gbColor, const float* gbNormal,
+                                         const float* gbPbr, const float* gbEmissive, const LuxgiGlobalSDFData* g,
+                                         const float* sdf, const float* mip, int sdfRes, int mipRes,
+                                         const LuxgiDDGIUniform* u, const float* irr, const float* dep, double* out,
+                                         int threads) {
+    surface_light_impl<double>(prm, lights, tiles, numTiles, draws, numDraws, res, gbColor, gbNormal, gbPbr, gbEmissive,
+                               g, sdf, mip, sdfRes, mipRes, u, irr, dep, out, threads);
+}
+extern "C" void oracle_surface_light_f32(const LuxgiSurfaceLightParams* prm, const LuxgiLight* lights,
+                                         const LuxgiAtlasTile* tiles, int numTiles, const LuxgiSurfaceDraw* draws,
+                                         int numDraws, int res, const float* gbColor, const float* gbNormal,
+                                         const float* gbPbr, const float* gbEmissive, const LuxgiGlobalSDFData* g,
+                                         const float* sdf, const float* mip, int sdfRes, int mipRes,
+                                         const LuxgiDDGIUniform* u, const float* irr, const float* dep, float* out,
+                                         int threads) {
+    surface_light_impl<float>(prm, lights, tiles, numTiles, draws, numDraws, res, gbColor, gbNormal, gbPbr, gbEmissive,
+                              g, sdf, mip, sdfRes, mipRes, u, irr, dep, out, threads);
+}
+// Raytraced/BRDF.glsl:65-79 at one set of inputs (unit test hook)
+extern "C" void oracle_brdf(const double* albedo, const double* n, double rough, double metal, const double* view,
+                            const double* h, const double* l, double* out) {
+    v3<double> r = brdf_ggx<double>({albedo[0], albedo[1], albedo[2]}, {n[0], n[1], n[2]}, rough, metal,
+                                    {view[0], view[1], view[2]}, {h[0], h[1], h[2]}, {l[0], l[1], l[2]});
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}

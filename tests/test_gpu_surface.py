@@ -1,0 +1,150 @@
+"""GPU parity of the surface-atlas passes: luxgi_atlas_cull_chunks against the oracle bit for bit,
+luxgi_surface_light against the FP64 oracle.
+
+Tolerances (lighting). The shadow term is a chain of threshold decisions (the hit test of each
+march step), so a texel whose decision flips between fp32 and fp64 differs by a whole light
+term; parity is stated per texel population:
+  * >= 99% of drawn texels agree to 1e-3 relative + 1e-4 absolute (fp32 bilinear weights, the
+    BRDF's cancellation in 1 - F, the bounce's pow);
+  * the emissive-only pass agrees to 1e-6 relative on every texel;
+  * every texel outside the draws keeps its previous value."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from luxgi_amd import abi, scene
+from tests import _scenes
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _objects(bounds):
+    objs = (abi.AtlasObject * max(len(bounds), 1))()
+    for i, b in enumerate(bounds):
+        objs[i].objectBounds[:] = [float(v) for v in b]
+    return objs
+
+
+@pytest.mark.parametrize("n,capacity", [(300, 10 ** 6), (300, 5000), (0, 16), (1, 1)])
+def test_cull_matches_oracle(n, capacity):
+    from luxgi_amd.ddgi import to_device
+    from luxgi_amd.surface import cull_chunks
+    rng = np.random.default_rng(n + capacity)
+    b = np.concatenate([rng.uniform(-1.1, 1.1, (n, 3)), rng.uniform(0.01, 0.3, (n, 1))], axis=1).astype(np.float32)
+    objs = _objects(b)
+    data = abi.SurfaceAtlasData()
+    data.chunkSize = 0.05
+    ch, cu = cull_chunks(data, to_device(objs), n, capacity)
+    torch.cuda.synchronize()
+    want_ch, want_cu = oracle.cull_chunks(0.05, objs, n, capacity)
+    got_ch, got_cu = ch.cpu().numpy().view(np.uint32), cu.cpu().numpy().view(np.uint32)
+    assert np.array_equal(got_ch, want_ch)
+    assert got_cu[0] == want_cu[0]
+    used = int(min(want_cu[0], capacity))
+    assert np.array_equal(got_cu[1:used], want_cu[1:used])
+
+
+def test_cull_rejects_bad_arguments():
+    from luxgi_amd._lib import lib
+    data = abi.SurfaceAtlasData()
+    data.chunkSize = 0.0
+    data.culledObjectsCapacity = 4
+    buf = torch.zeros(64000 + 4, dtype=torch.int32, device="cuda")
+    ws = torch.empty(lib().luxgi_atlas_cull_workspace_size(), dtype=torch.uint8, device="cuda")
+    p = C.c_void_p(buf.data_ptr())
+    assert lib().luxgi_atlas_cull_chunks(C.byref(data), None, p, p, C.c_void_p(ws.data_ptr()), ws.numel(), None) < 0
+    data.chunkSize = 0.05
+    assert lib().luxgi_atlas_cull_chunks(C.byref(data), None, p, p, C.c_void_p(ws.data_ptr()), 16, None) < 0
+
+
+def _lights():
+    arr = (abi.Light * 3)()
+    arr[0].color[:] = [1.0, 0.9, 0.8, 1.0]
+    arr[0].direction[:] = [0.3, -0.5, -0.81, 0.0]
+    arr[0].intensity, arr[0].type = 2.0, abi.LIGHT_DIRECTIONAL
+    arr[1].color[:] = [0.2, 0.6, 1.0, 1.0]
+    arr[1].position[:] = [0.1, 0.4, 0.2, 1.0]
+    arr[1].intensity, arr[1].radius, arr[1].type = 1.5, 1.2, abi.LIGHT_POINT
+    arr[2].color[:] = [1.0, 0.3, 0.3, 1.0]
+    arr[2].position[:] = [-0.5, -0.2, 0.6, 1.0]
+    arr[2].direction[:] = [0.5, 0.1, -0.86, 0.0]
+    arr[2].intensity, arr[2].radius, arr[2].type, arr[2].angle = 3.0, 1.5, abi.LIGHT_SPOT, 0.3
+    return arr
+
+
+def _probe_atlases(u, seed=3):
+    rng = np.random.default_rng(seed)
+    irr = rng.uniform(0.2, 0.9, (u.irradianceTextureHeight, u.irradianceTextureWidth, 4)).astype(np.float32)
+    dep = np.zeros((u.depthTextureHeight, u.depthTextureWidth, 2), np.float32)
+    dep[..., 0] = rng.uniform(0.05, 0.6, dep.shape[:2])
+    dep[..., 1] = dep[..., 0] ** 2 + rng.uniform(0.0, 0.02, dep.shape[:2])
+    return irr, dep
+
+
+@pytest.mark.parametrize("form", ["linear", "brick_guard", "fp16_scaled"])
+def test_surface_light_matches_oracle(form):
+    from luxgi_amd.ddgi import DDGIVolume, DeviceScene
+    from luxgi_amd.surface import SurfaceCache
+    prims, sdf, mip, g, _ = _scenes.small_scene()
+    res = 64
+    atlas = scene.build_atlas(prims, resolution=512, tile=32, seed=3, threshold=1.05 * 2 / res, surface_gbuffer=True)
+    kw = dict(linear={}, brick_guard=dict(layout=abi.LAYOUT_BRICK4, guard=True),
+              fp16_scaled=dict(layout=abi.LAYOUT_BRICK4, sdf_storage=abi.STORAGE_FP16, brick_scale=True))[form]
+    ds = DeviceScene(g, sdf, mip, atlas, **kw)
+    u = _scenes.small_uniform(rays=64, counts=(5, 4, 6))
+    vol = DDGIVolume(u)
+    irr, dep = _probe_atlases(u)
+    vol.irradiance[vol.ping_pong].copy_(torch.from_numpy(irr).reshape(vol.irradiance[vol.ping_pong].shape))
+    vol.depth[vol.ping_pong].copy_(torch.from_numpy(dep).reshape(vol.depth[vol.ping_pong].shape))
+    sc = SurfaceCache(atlas, out_storage=abi.STORAGE_FP32)
+    lights = _lights()
+    sc.set_lights(lights)
+    sentinel = -7.0
+    sc.light_cache.fill_(sentinel)
+    passes = abi.SURFACE_EMISSIVE | abi.SURFACE_DIRECT | abi.SURFACE_INDIRECT
+    cam = (0.2, 0.3, 1.8)
+    sc.light(ds, passes=passes, camera=cam, shadow_bias=0.03, volume=vol, intensity=1.3)
+    torch.cuda.synchronize()
+    got = sc.light_cache.cpu().numpy()
+    p = abi.SurfaceLightParams()
+    p.cameraPosition[:] = [cam[0], cam[1], cam[2], 0.03]
+    p.indirectIntensity, p.passes, p.numLights = 1.3, passes, 3
+    sdf_ref = sdf if form != "fp16_scaled" else sdf  # the FP16 form is compared with the FP32 volume
+    want = oracle.surface_light(p, lights, atlas.tiles, sc.draws_host, res=512,
+                                gb_color=atlas.gb_color.astype(np.float32), gb_normal=atlas.gb_normal.astype(np.float32),
+                                gb_pbr=atlas.gb_pbr.astype(np.float32) / 255.0,
+                                gb_emissive=atlas.gb_emissive.astype(np.float32), sdf_data=g, sdf=sdf_ref, mip=mip,
+                                sdf_res=64, mip_res=16, u=u, irradiance=irr, depth_atlas=dep,
+                                out=np.full((512, 512, 4), sentinel))
+    drawn = want[..., 3] == 1.0
+    assert drawn.sum() > 40000
+    assert np.all(got[~drawn] == sentinel)
+    err = np.abs(got[..., :3] - want[..., :3])
+    ok = np.all(err <= 1e-3 * np.abs(want[..., :3]) + 1e-4, axis=-1)
+    frac = ok[drawn].mean()
+    assert frac >= (0.99 if form != "fp16_scaled" else 0.97), frac
+    assert np.all(got[drawn][:, 3] == 1.0)
+    lit = np.abs(want[..., :3]).sum(-1) > 1e-3
+    assert lit[drawn].mean() > 0.3
+
+
+def test_surface_emissive_pass_is_exact():
+    from luxgi_amd.ddgi import DeviceScene
+    from luxgi_amd.surface import SurfaceCache
+    prims, sdf, mip, g, _ = _scenes.small_scene()
+    atlas = scene.build_atlas(prims, resolution=512, tile=32, seed=3, threshold=0.05, surface_gbuffer=True)
+    ds = DeviceScene(g, sdf, mip, atlas)
+    sc = SurfaceCache(atlas, out_storage=abi.STORAGE_FP32)
+    sc.set_lights((abi.Light * 0)())
+    sc.light(ds, passes=abi.SURFACE_EMISSIVE)
+    torch.cuda.synchronize()
+    p = abi.SurfaceLightParams()
+    p.passes = abi.SURFACE_EMISSIVE
+    z = np.zeros((512, 512, 4), np.float32)
+    want = oracle.surface_light(p, None, atlas.tiles, sc.draws_host, 512, z, z, z,
+                                atlas.gb_emissive.astype(np.float32))
+    np.testing.assert_allclose(sc.light_cache.cpu().numpy(), want, rtol=1e-6, atol=1e-7)
+    assert want[..., 0].max() > 1.0  # emissive texels are present

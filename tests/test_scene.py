@@ -37,8 +37,8 @@ def test_atlas_tables():
     prims = scene.scene_small(seed=1)
     a = scene.build_atlas(prims, resolution=512, tile=32, threshold=0.05)
     assert a.data.objectsCount == 11
-    # chunk runs are [count, ids...] with slot 0 reserved
-    assert a.culled[0] == 0
+    # chunk runs are [count, ids...]; slot 0 holds the culling counter (1 + all run sizes)
+    assert a.culled[0] == len(a.culled)
     starts = a.chunks[a.chunks > 0]
     for s in np.unique(starts)[:50]:
         n = a.culled[s]

@@ -12,7 +12,9 @@ Reported beside the step (not in value): the reduced-precision form of the same
 frame (FP16 SDF bricks + FP32 hit re-evaluation, FP16-MFMA blend with one rounding per
 operand, RGBA16F SVGF; reduced_precision), and per-pixel probe sampling of the updated
 volume at 1920x1080 (probe_sample_ms; bytes = G-buffer depth + normal + RGBA32F
-output per pixel + both atlases once).
+output per pixel + both atlases once), and the surface-atlas passes (surface_atlas): chunk
+culling of the objects and the relight of every atlas tile (emissive copy, one directional
+light with an SDF shadow ray per lit texel, DDGI bounce) into an RGBA16F light cache.
 value = rays / DDGI-update time (Grays/s). All timing uses HIP events on the
 stream the kernels are launched on (torch's current stream), W warm-up steps
 then exactly K timed steps between device synchronisations.
@@ -49,6 +51,7 @@ def parse():
     ap.add_argument("--svgf-iterations", type=int, default=5)
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision form")
+    ap.add_argument("--no-surface", action="store_true", help="skip the surface-atlas passes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, launch a plain copy kernel (256 MiB read + 256 MiB written, 3 times) "
                          "whose rocprofv3 FETCH_SIZE / WRITE_SIZE scale the trace counters to bytes")
@@ -84,7 +87,7 @@ def main():
     g = abi.make_sdf_data(resolution=res)
     log(f"[bench] sdf {res}^3 baked", q)
     atlas = scene.build_atlas(prims, resolution=2048, tile=16, color_storage=abi.STORAGE_UNORM8, seed=3,
-                              threshold=1.05 * 2.0 / res)
+                              threshold=1.05 * 2.0 / res, surface_gbuffer=not args.no_surface)
     log("[bench] atlas 2048^2 built", q)
     ds = DeviceScene(g, sdf, mip, atlas, layout=abi.LAYOUT_BRICK4)
     counts = (32, 16, 32)
@@ -156,6 +159,47 @@ def main():
     # G-buffer depth (4 B) + octahedral normal (8 B) + RGBA32F output (16 B) per pixel, atlases read once
     atlas_bytes = sum(t.numel() * t.element_size() for t in (vol.current_irradiance, vol.current_depth))
     sample_bytes = W * H * (4 + 8 + 16) + atlas_bytes
+
+    # ---------------- surface atlas passes (reported beside the step) ----------------
+    # chunk culling of the 2,000 objects, then the surface-cache relight of all 12,000 tiles
+    # (16x16 texels) with one directional light (SDF shadow ray per lit texel), the emissive
+    # copy and the DDGI bounce from the updated volume, into an RGBA16F light cache
+    surf = None
+    if not args.no_surface:
+        from luxgi_amd.surface import SurfaceCache, cull_chunks
+        cap = int(atlas.culled.size) + 64
+        for _ in range(2):
+            cull_chunks(atlas.data, ds.objects, atlas.n_objects, cap)
+        cev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        cev[0].record()
+        for _ in range(args.steps):
+            cull_chunks(atlas.data, ds.objects, atlas.n_objects, cap)
+        cev[1].record()
+        sc = SurfaceCache(atlas, out_storage=abi.STORAGE_FP16)
+        sun = (abi.Light * 1)()
+        sun[0].color[:] = [1.0, 0.95, 0.9, 1.0]
+        dsun = np.array([0.3, -0.8, -0.52]) / np.linalg.norm([0.3, -0.8, -0.52])
+        sun[0].direction[:] = [float(dsun[0]), float(dsun[1]), float(dsun[2]), 0.0]
+        sun[0].intensity, sun[0].type = 2.0, abi.LIGHT_DIRECTIONAL
+        sc.set_lights(sun)
+        passes = abi.SURFACE_EMISSIVE | abi.SURFACE_DIRECT | abi.SURFACE_INDIRECT
+        for _ in range(2):
+            sc.light(ds, passes=passes, camera=eye, shadow_bias=2.0 * 2.0 / res, volume=vol)
+        sev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        sev[0].record()
+        for _ in range(args.steps):
+            sc.light(ds, passes=passes, camera=eye, shadow_bias=2.0 * 2.0 / res, volume=vol)
+        sev[1].record()
+        torch.cuda.synchronize()
+        light_ms = sev[0].elapsed_time(sev[1]) / args.steps
+        texels = sc.n_draws * 16 * 16
+        lit = int(sum(1 for k in range(sc.n_draws) if sc.draws_host[k].lightMask))
+        surf = {"cull_ms": cev[0].elapsed_time(cev[1]) / args.steps, "surface_light_ms": light_ms,
+                "tiles": sc.n_draws, "tiles_with_light": lit, "texels": texels,
+                "gtexels_per_s": texels / (light_ms * 1e-3) / 1e9,
+                "passes": "emissive + 1 directional light with SDF shadow rays + DDGI bounce", "out": "RGBA16F"}
+        log(f"[bench] surface atlas: cull {surf['cull_ms']:.3f} ms, relight {light_ms:.3f} ms", q)
+        del sc
 
     # ---------------- reduced-precision form (reported beside the step, not in value) ----------------
     # FP16 SDF bricks with per-brick scale and FP32 re-evaluation at hits, FP16-MFMA blend (one
@@ -254,11 +298,13 @@ def main():
         "probe_sample_ms": sample_ms,
         "reduced_precision": mixed,
         "probe_sample_gbs": sample_bytes / (sample_ms * 1e-3) / 1e9,
+        "surface_atlas": surf,
         "trace_grays_per_s": rays_per_frame / trace_s / 1e9,
         "trace_counts": {"lookups": n_lookup, "normals": n_normal, "objects": c[3], "tile_gathers": n_tile,
                          "rays": n_rays, "steps": c[6], "hits": c[7]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "trace_kernel",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "luxgi_ddgi_trace_sdf launch sequence (march + hit sort + shade)",
                      "algorithmic_bytes_per_launch": trace_bytes},
         "cpu_baseline": cpu,
     }

@@ -228,14 +228,23 @@ def cull_chunks_np(bounds, chunk_size, capacity=None):
     return chunks, np.asarray(culled, dtype=np.uint32)
 
 
-def _surface_normals(prim, pts, h=1e-5):
-    """Unit gradient of one primitive's distance at surface points (central differences)."""
-    g = np.zeros_like(pts)
-    one = prim[None] if prim.ndim == 0 else prim
-    for a in range(3):
-        e = np.zeros(3)
-        e[a] = h
-        g[:, a] = (prim_distance_np(one, pts + e) - prim_distance_np(one, pts - e)) / (2.0 * h)
+def _surface_normals(prim, pts):
+    """Outward unit normal (the distance gradient) of one primitive at points on its surface."""
+    q = pts - prim["center"].astype(np.float64)
+    t = int(prim["type"])
+    sz = prim["size"].astype(np.float64)
+    if t == PRIM_BOX:  # the face whose slab holds the point: the largest |q_a| - s_a
+        rows = np.arange(len(q))
+        a = np.argmax(np.abs(q) - sz[None, :], axis=1)
+        g = np.zeros_like(q)
+        g[rows, a] = np.where(q[rows, a] >= 0.0, 1.0, -1.0)
+    elif t == PRIM_SPHERE:
+        g = q
+    else:
+        c = np.zeros_like(q)  # closest point of the capsule's segment
+        ax = int(sz[2])
+        c[:, ax] = np.clip(q[:, ax], -sz[1], sz[1])
+        g = q - c
     return g / np.maximum(np.linalg.norm(g, axis=1, keepdims=True), 1e-30)
 
 
@@ -312,7 +321,7 @@ def build_atlas(prims, resolution=512, tile=32, color_storage=abi.STORAGE_FP32, 
                 if seen.any():
                     tp = np.stack([X[seen], Y[seen], dep[seen] * ob[2]], axis=1)
                     world = c[None, :] + tp @ R  # local = R^T tp; objects are translated, not rotated
-                    nrm = _surface_normals(prims[i:i + 1], world)
+                    nrm = _surface_normals(p, world)
                     oc = _oct_encode_np(nrm)
                     sl = (slice(ty, ty + tile), slice(tx, tx + tile))
                     blk = {k: v[sl] for k, v in gb.items()}

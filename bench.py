@@ -14,7 +14,10 @@ operand, RGBA16F SVGF; reduced_precision), and per-pixel probe sampling of the u
 volume at 1920x1080 (probe_sample_ms; bytes = G-buffer depth + normal + RGBA32F
 output per pixel + both atlases once), and the surface-atlas passes (surface_atlas): chunk
 culling of the objects and the relight of every atlas tile (emissive copy, one directional
-light with an SDF shadow ray per lit texel, DDGI bounce) into an RGBA16F light cache.
+light with an SDF shadow ray per lit texel, DDGI bounce) into an RGBA16F light cache, and
+the SDF soft-shadow mask at 1920x1080 (sdf_shadow_mask_ms: one blue-noise disk ray per pixel) and
+the SDF reflection rays at 1920x1080 (sdf_reflection_ms: one GGX-sampled ray per pixel with the
+surface-atlas lookup at its hit, RGBA16F out).
 value = rays / DDGI-update time (Grays/s). All timing uses HIP events on the
 stream the kernels are launched on (torch's current stream), W warm-up steps
 then exactly K timed steps between device synchronisations.
@@ -201,6 +204,54 @@ def main():
         log(f"[bench] surface atlas: cull {surf['cull_ms']:.3f} ms, relight {light_ms:.3f} ms", q)
         del sc
 
+    # ---------------- SDF soft-shadow mask at 1080p (the shadow SVGF's input; reported beside the step) ----
+    # one ray per pixel toward a blue-noise point of a sun disk; synthetic blue-noise tables of the
+    # engine's shapes (a permutation per Sobol dimension, random scrambling-ranking bytes)
+    from luxgi_amd.producers import BlueNoiseTables, sdf_shadow_mask
+    nrng = np.random.default_rng(21)
+    noise = BlueNoiseTables(np.stack([nrng.permutation(256) for _ in range(4)], axis=1).astype(np.uint8),
+                            nrng.integers(0, 256, (128, 128, 4), dtype=np.uint8))
+    sp = abi.SdfShadowParams()
+    sp.light.color[:] = [1.0, 1.0, 1.0, 1.0]
+    sp.light.direction[:] = [0.3, -0.8, -0.52, 0.02]
+    sp.light.intensity, sp.light.type = 1.0, abi.LIGHT_DIRECTIONAL
+    sp.viewProjInv = abi.mat4_colmajor(VPI)
+    sp.shadowBias = 2.0 * 2.0 / res
+    smask = torch.zeros(((H + 3) // 4, (W + 7) // 8), dtype=torch.int32, device="cuda")
+    for f in range(2):
+        sp.numFrames = f
+        sdf_shadow_mask(sp, noise, cur["depth"], cur["normals"], ds, mask=smask)
+    mev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    mev[0].record()
+    for f in range(args.steps):
+        sp.numFrames = f
+        sdf_shadow_mask(sp, noise, cur["depth"], cur["normals"], ds, mask=smask)
+    mev[1].record()
+    torch.cuda.synchronize()
+    shadow_ms = mev[0].elapsed_time(mev[1]) / args.steps
+    log(f"[bench] sdf shadow mask 1080p {shadow_ms:.3f} ms", q)
+    # SDF reflection rays at 1080p (the reflection SVGF's noisy input): one GGX-sampled ray per pixel
+    # (the G-buffer's roughness lies in (0.05, 0.45)), coloured by the surface-atlas lookup or the sky
+    from luxgi_amd.producers import sdf_reflection
+    rp = abi.SdfReflectionParams()
+    rp.trim, rp.roughDDGIIntensity = 1.0, 1.0
+    rp.cameraPosition[:] = [float(eye[0]), float(eye[1]), float(eye[2]), 1.0]
+    rp.viewProjInv = abi.mat4_colmajor(VPI)
+    rp.skyColor[:] = [0.6, 0.7, 0.9, 1.0]
+    refl_out = torch.zeros((H, W, 4), dtype=torch.float16, device="cuda")
+    for f in range(2):
+        rp.numFrames = f
+        sdf_reflection(rp, noise, cur, ds, refl_out)
+    rev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    rev[0].record()
+    for f in range(args.steps):
+        rp.numFrames = f
+        sdf_reflection(rp, noise, cur, ds, refl_out)
+    rev[1].record()
+    torch.cuda.synchronize()
+    refl_ms = rev[0].elapsed_time(rev[1]) / args.steps
+    log(f"[bench] sdf reflection 1080p {refl_ms:.3f} ms", q)
+
     # ---------------- reduced-precision form (reported beside the step, not in value) ----------------
     # FP16 SDF bricks with per-brick scale and FP32 re-evaluation at hits, FP16-MFMA blend (one
     # rounding per operand, FP32 accumulation), SVGF with RGBA16F intermediates. Ray buffers stay
@@ -299,6 +350,8 @@ def main():
         "reduced_precision": mixed,
         "probe_sample_gbs": sample_bytes / (sample_ms * 1e-3) / 1e9,
         "surface_atlas": surf,
+        "sdf_shadow_mask_ms": shadow_ms,
+        "sdf_reflection_ms": refl_ms,
         "trace_grays_per_s": rays_per_frame / trace_s / 1e9,
         "trace_counts": {"lookups": n_lookup, "normals": n_normal, "objects": c[3], "tile_gathers": n_tile,
                          "rays": n_rays, "steps": c[6], "hits": c[7]},

@@ -577,6 +577,76 @@ int luxgi_svgf_shadow_atrous(const LuxgiShadowAtrousParams* params,
 /* Library version string. */
 const char* luxgi_version(void);
 
+/* ------------------------------------------------------------------------- */
+/* Noisy-signal producers traced through the global SDF                       */
+/* ------------------------------------------------------------------------- */
+
+/* Blue-noise tables of sampleBlueNoise (Raytraced/BlueNoise.glsl:8-18), RGBA8 texels as the
+ * engine loads them (Engine/Noise/BlueNoise.h:15-26): sobol_256_4d ([256]) and
+ * scrambling_ranking_128x128_2d_1spp ([128][128]). Device memory. */
+typedef struct LuxgiBlueNoise {
+    const uint8_t* sobol;
+    const uint8_t* scrambling;
+} LuxgiBlueNoise;
+
+/* SDFShadow.comp UBO (:24-31) plus the march step cap. 144 bytes. */
+typedef struct LuxgiSdfShadowParams {
+    LuxgiLight light;           /* light.direction.w = light source radius (soft shadows) */
+    float      viewProjInv[16];
+    uint32_t   numFrames;       /* blue-noise sample index (the frame counter) */
+    float      shadowBias;
+    int32_t    maxSteps;        /* 0 = 250 (SDFCommon.glsl:143) */
+    int32_t    pad;
+} LuxgiSdfShadowParams;
+
+/* SDFShadow.comp:113-157 (dispatch RaytracedShadow.cpp:340-356): one soft-shadow ray per pixel of
+ * the G-buffer (depth R32F, normals RGBA32F with the octahedral normal in xy) toward the light's
+ * disk point picked by the blue-noise sampler, traced through the global SDF. mask holds one
+ * R32UI word per 8x4 pixel block ([ceil(H/4)][ceil(W/8)]), bit x + 8y set where the ray reaches
+ * the light; a word is written only when its block's first pixel is not sky (depth 1), as in
+ * the reference. Pixels past the image edge read the G-buffer with repeat addressing (the
+ * default sampler) at their unwrapped screen position, as the reference's invocations do. */
+int luxgi_sdf_shadow_mask(const LuxgiSdfShadowParams* params, const LuxgiBlueNoise* noise, int32_t width,
+                          int32_t height, const float* depth, const float* normals,
+                          const LuxgiGlobalSDFData* sdfData, const LuxgiSDFVolume* sdf, uint32_t* mask,
+                          void* stream);
+
+/* SDFReflection.comp push constants (:64-76, 112 bytes) followed by options that extend the reference. 144 bytes. */
+typedef struct LuxgiSdfReflectionParams {
+    float    bias;                /* unused by the reference's shader, kept for the layout */
+    float    trim;                /* scales the blue-noise sample of the GGX lobe */
+    float    intensity;           /* unused by the reference's shader */
+    float    roughDDGIIntensity;  /* scale of the DDGI estimate for rough pixels */
+    uint32_t numLights;           /* unused by the reference's shader */
+    uint32_t numFrames;           /* blue-noise sample index */
+    uint32_t sampleGI;            /* unused by the reference's shader */
+    uint32_t approximateWithDDGI; /* 1: roughness > 0.45 reads the probes instead of tracing */
+    float    cameraPosition[4];
+    float    viewProjInv[16];
+    /* --- extensions --- */
+    int32_t  maxSteps;            /* 0 = 250 (SDFCommon.glsl:143) */
+    int32_t  pad[3];
+    float    skyColor[4];         /* sky when no cube map is bound (the reference binds a unit cube) */
+} LuxgiSdfReflectionParams;
+
+/* Workspace bytes of luxgi_sdf_reflection for an atlas of `objectCapacity` objects. */
+size_t luxgi_sdf_reflection_workspace_size(int32_t objectCapacity);
+
+/* SDFReflection.comp:116-162 (dispatch RaytracedReflection.cpp:307-344): per non-sky pixel of the
+ * G-buffer (normals and pbr RGBA32F, depth R32F; the colour image is not read), a reflection ray
+ * through the global SDF from world + normal * voxel: the mirror direction below roughness 0.05,
+ * a GGX half vector from the blue-noise sampler up to 0.45 (or beyond, unless
+ * approximateWithDDGI), and for approximateWithDDGI above 0.45 the DDGI irradiance in the mirror
+ * direction instead of a ray. A ray's colour is the surface-atlas lookup at the hit (normal -R,
+ * as the trace's `sampleGlobalSurfaceAtlas`: rgb / max(a, 1e-4), a), or the sky. Writes
+ * out [H][W] RGBA (FP32 or FP16) at non-sky pixels only; the SVGF reprojection reads its alpha as
+ * the ray length. `ddgi` / `probes` may be null unless approximateWithDDGI is 1. */
+int luxgi_sdf_reflection(const LuxgiSdfReflectionParams* params, const LuxgiBlueNoise* noise, int32_t width,
+                         int32_t height, const LuxgiGBuffer* gbuffer, const LuxgiGlobalSDFData* sdfData,
+                         const LuxgiSDFVolume* sdf, const LuxgiSurfaceAtlas* atlas, const LuxgiSkybox* sky,
+                         const LuxgiDDGIUniform* ddgi, const LuxgiProbeAtlas* probes, int32_t outStorage, void* out,
+                         void* workspace, size_t workspaceBytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

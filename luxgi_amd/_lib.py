@@ -34,6 +34,9 @@ SIGNATURES = {
     "luxgi_atlas_cull_workspace_size": (C.c_size_t, []),
     "luxgi_atlas_cull_chunks": (C.c_int, [P, P, P, P, P, C.c_size_t, P]),
     "luxgi_surface_light": (C.c_int, [P, P, P, I, P, I, P, P, P, P, P, I, P, P]),
+    "luxgi_sdf_shadow_mask": (C.c_int, [P, P, I, I, P, P, P, P, P, P]),
+    "luxgi_sdf_reflection_workspace_size": (C.c_size_t, [I]),
+    "luxgi_sdf_reflection": (C.c_int, [P, P, I, I, P, P, P, P, P, P, P, I, P, P, C.c_size_t, P]),
     "luxgi_svgf_tile_count": (I, [I, I]),
     "luxgi_svgf_reflection_reproject": (C.c_int, [P, I, I, I, P, P, P, P, P, P, P, P, P]),
     "luxgi_svgf_reflection_atrous": (C.c_int, [P, I, I, I, P, P, P, P, P, P]),

@@ -171,10 +171,27 @@ class SurfaceDraw(C.Structure):
     _fields_ = [("tile", C.c_uint32), ("lightMask", C.c_uint32)]
 
 
+class BlueNoise(C.Structure):  # Raytraced/BlueNoise.glsl tables (RGBA8)
+    _fields_ = [("sobol", C.c_void_p), ("scrambling", C.c_void_p)]
+
+
+class SdfShadowParams(C.Structure):  # Shaders/SDF/SDFShadow.comp:24-31
+    _fields_ = [("light", Light), ("viewProjInv", f16), ("numFrames", C.c_uint32), ("shadowBias", C.c_float),
+                ("maxSteps", C.c_int32), ("pad", C.c_int32)]
+
+
+class SdfReflectionParams(C.Structure):  # Shaders/SDF/SDFReflection.comp:64-76 + extensions
+    _fields_ = [("bias", C.c_float), ("trim", C.c_float), ("intensity", C.c_float), ("roughDDGIIntensity", C.c_float),
+                ("numLights", C.c_uint32), ("numFrames", C.c_uint32), ("sampleGI", C.c_uint32),
+                ("approximateWithDDGI", C.c_uint32), ("cameraPosition", f4), ("viewProjInv", f16),
+                ("maxSteps", C.c_int32), ("pad", C.c_int32 * 3), ("skyColor", f4)]
+
+
 _SIZES = {DDGIUniform: 96, GlobalSDFData: 96, SurfaceAtlasData: 32, AtlasObject: 128, AtlasTile: 96,
           TraceParams: 112, ReflectionReprojectParams: 168, ReflectionAtrousParams: 24,
           ShadowReprojectParams: 80, ShadowAtrousParams: 32, SampleProbeParams: 80, Light: 64,
-          SurfaceGBuffer: 40, SurfaceLightParams: 32, SurfaceDraw: 8}
+          SurfaceGBuffer: 40, SurfaceLightParams: 32, SurfaceDraw: 8, BlueNoise: 16, SdfShadowParams: 144,
+          SdfReflectionParams: 144}
 for _t, _n in _SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
 

@@ -97,4 +97,37 @@ __device__ SdfHitTime trace_global_sdf(const SdfView<SS>& v, const LuxgiGlobalSD
     return hit;
 }
 
+// Validation of a global SDF for the tracers of this header: LuxgiStatus (LUXGI_OK when usable).
+inline int sdf_volume_status(const LuxgiGlobalSDFData* g, const LuxgiSDFVolume* vol) {
+    if (!g || !vol || !vol->sdf || !vol->mip) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (g->cascadesCount < 1 || g->cascadesCount > 4) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (vol->resolution <= 0 || vol->mipResolution <= 0) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (vol->storage != LUXGI_STORAGE_FP32 && vol->storage != LUXGI_STORAGE_FP16) return LUXGI_ERR_UNSUPPORTED;
+    if (vol->layout != LUXGI_LAYOUT_LINEAR && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (vol->layout == LUXGI_LAYOUT_BRICK4 && (vol->resolution & 3)) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (vol->brickScale && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (vol->guard && ((vol->resolution & 3) || vol->storage != LUXGI_STORAGE_FP32)) return LUXGI_ERR_INVALID_ARGUMENT;
+    // the samplers address the volumes with 32-bit byte offsets
+    const size_t es = vol->storage == LUXGI_STORAGE_FP16 ? 2 : 4;
+    const size_t C = g->cascadesCount, r = (size_t)vol->resolution, m = (size_t)vol->mipResolution;
+    const size_t fine = vol->layout == LUXGI_LAYOUT_BRICK4 ? C * (r / 4) * (r / 4) * (r / 4) * kBrickElems * es
+                                                           : C * r * r * r * es;
+    if (fine >= (1ull << 32) || C * m * m * m * es >= (1ull << 32)) return LUXGI_ERR_UNSUPPORTED;
+    return LUXGI_OK;
+}
+
+template <class SS>
+inline SdfView<SS> sdf_view(const LuxgiSDFVolume* vol) {
+    SdfView<SS> v{};
+    if (vol) {
+        v.sdf = static_cast<const SS*>(vol->sdf);
+        v.mip = static_cast<const SS*>(vol->mip);
+        v.guard = vol->guard;
+        v.brickScale = vol->brickScale;
+        v.res = vol->resolution;
+        v.mipRes = vol->mipResolution;
+    }
+    return v;
+}
+
 }  // namespace luxgi

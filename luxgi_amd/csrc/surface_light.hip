@@ -182,15 +182,7 @@ template <class SS, bool BRICK, class SA>
 static void launch_surface(const SurfaceArgs& A, const LuxgiLight* lights, const LuxgiAtlasTile* tiles,
                            int numTiles, const LuxgiSurfaceDraw* draws, int numDraws, const LuxgiSurfaceGBuffer* gb,
                            const LuxgiSDFVolume* vol, const LuxgiProbeAtlas* probes, void* out, hipStream_t st) {
-    SdfView<SS> v{};
-    if (vol) {
-        v.sdf = static_cast<const SS*>(vol->sdf);
-        v.mip = static_cast<const SS*>(vol->mip);
-        v.guard = vol->guard;
-        v.brickScale = vol->brickScale;
-        v.res = vol->resolution;
-        v.mipRes = vol->mipResolution;
-    }
+    const SdfView<SS> v = sdf_view<SS>(vol);
     hipLaunchKernelGGL((surface_light_kernel<SS, BRICK, SA>), dim3(numDraws), dim3(256), 0, st, A, lights, tiles,
                        numTiles, draws, static_cast<const __half*>(gb->color), static_cast<const __half*>(gb->normal),
                        static_cast<const Unorm8*>(gb->pbr), static_cast<const __half*>(gb->emissive), v,
@@ -228,21 +220,9 @@ extern "C" int luxgi_surface_light(const LuxgiSurfaceLightParams* params, const 
     if ((direct || indirect) && (!gb->color || !gb->normal || !gb->pbr)) return LUXGI_ERR_INVALID_ARGUMENT;
     if (params->numLights < 0 || params->numLights > LUXGI_MAX_LIGHTS) return LUXGI_ERR_INVALID_ARGUMENT;
     if (direct) {
-        if (!lights || !sdfData || !vol || !vol->sdf || !vol->mip) return LUXGI_ERR_INVALID_ARGUMENT;
-        if (sdfData->cascadesCount < 1 || sdfData->cascadesCount > 4) return LUXGI_ERR_INVALID_ARGUMENT;
-        if (vol->resolution <= 0 || vol->mipResolution <= 0) return LUXGI_ERR_INVALID_ARGUMENT;
-        if (vol->storage != LUXGI_STORAGE_FP32 && vol->storage != LUXGI_STORAGE_FP16) return LUXGI_ERR_UNSUPPORTED;
-        if (vol->layout != LUXGI_LAYOUT_LINEAR && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
-        if (vol->layout == LUXGI_LAYOUT_BRICK4 && (vol->resolution & 3)) return LUXGI_ERR_INVALID_ARGUMENT;
-        if (vol->brickScale && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
-        if (vol->guard && ((vol->resolution & 3) || vol->storage != LUXGI_STORAGE_FP32)) return LUXGI_ERR_INVALID_ARGUMENT;
-        {  // the samplers address the volumes with 32-bit byte offsets
-            const size_t es = vol->storage == LUXGI_STORAGE_FP16 ? 2 : 4;
-            const size_t C = sdfData->cascadesCount, r = (size_t)vol->resolution, m = (size_t)vol->mipResolution;
-            const size_t fine = vol->layout == LUXGI_LAYOUT_BRICK4 ? C * (r / 4) * (r / 4) * (r / 4) * kBrickElems * es
-                                                                   : C * r * r * r * es;
-            if (fine >= (1ull << 32) || C * m * m * m * es >= (1ull << 32)) return LUXGI_ERR_UNSUPPORTED;
-        }
+        if (!lights) return LUXGI_ERR_INVALID_ARGUMENT;
+        const int st = sdf_volume_status(sdfData, vol);
+        if (st != LUXGI_OK) return st;
     }
     if (indirect) {
         if (!ddgi || !probes || !probes->irradiance || !probes->depth) return LUXGI_ERR_INVALID_ARGUMENT;

@@ -76,6 +76,11 @@ def _declare(L):
         getattr(L, "oracle_surface_light_" + s).argtypes = [P, P, P, I, P, I, I, P, P, P, P, P, P, P, I, I, P, P, P,
                                                             P, I]
     L.oracle_brdf.argtypes = [P, P, C.c_double, C.c_double, P, P, P, P]
+    L.oracle_shadow_mask_f64.argtypes = [P, P, P, I, I, P, P, P, P, P, I, I, P, I]
+    L.oracle_blue_noise.argtypes = [P, P, I, I, I, I]
+    L.oracle_blue_noise.restype = C.c_float
+    L.oracle_sdf_reflection_f64.argtypes = [P, P, P, I, I, P, P, P, P, P, P, I, I, P, P, P, P, P, P, P, P, I, P, P, P,
+                                            P, I]
 
 
 def _p(a):
@@ -376,3 +381,49 @@ def brdf(albedo, n, rough, metal, view, h, l):
     a = [np.ascontiguousarray(x, dtype=np.float64) for x in (albedo, n, view, h, l)]
     lib().oracle_brdf(_p(a[0]), _p(a[1]), float(rough), float(metal), _p(a[2]), _p(a[3]), _p(a[4]), _p(out))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Noisy-signal producers
+# ---------------------------------------------------------------------------
+def blue_noise(sobol, scrambling, x, y, frame, dim):
+    """sampleBlueNoise (Raytraced/BlueNoise.glsl:8-18); tables uint8 [256][4] and [128][128][4]."""
+    so = np.ascontiguousarray(sobol, dtype=np.uint8)
+    sc = np.ascontiguousarray(scrambling, dtype=np.uint8)
+    return lib().oracle_blue_noise(_p(so), _p(sc), int(x), int(y), int(frame), int(dim))
+
+
+def shadow_mask(params, sobol, scrambling, W, H, depth, normals, sdf_data, sdf, mip, res, mip_res, mask=None,
+                nthreads=None):
+    """SDFShadow.comp over the image; returns the R32UI mask [ceil(H/4)][ceil(W/8)] (words whose block's
+    first pixel is sky keep `mask`'s value, 0 by default)."""
+    mw, mh = (W + 7) // 8, (H + 3) // 4
+    out = np.zeros((mh, mw), dtype=np.uint32) if mask is None else np.ascontiguousarray(mask, dtype=np.uint32).copy()
+    so = np.ascontiguousarray(sobol, dtype=np.uint8)
+    sc = np.ascontiguousarray(scrambling, dtype=np.uint8)
+    lib().oracle_shadow_mask_f64(C.byref(params), _p(so), _p(sc), int(W), int(H), _p(_f32(depth)), _p(_f32(normals)),
+                                 C.byref(sdf_data), _p(_f32(sdf)), _p(_f32(mip)), int(res), int(mip_res), _p(out),
+                                 nthreads or threads())
+    return out
+
+
+def sdf_reflection(params, sobol, scrambling, W, H, normals, pbr, depth, sdf_data, sdf, mip, res, mip_res, atlas,
+                   sky_faces=None, sky_size=0, u=None, irradiance=None, depth_atlas=None, out=None, nthreads=None):
+    """SDFReflection.comp over the image; atlas: scene.Atlas (colour as the GPU reads it in color_f32).
+    Returns [H][W][4] float64; sky pixels keep `out`'s value (zeros by default)."""
+    o = np.zeros((H, W, 4)) if out is None else np.ascontiguousarray(out, dtype=np.float64).copy()
+    so = np.ascontiguousarray(sobol, dtype=np.uint8)
+    sc = np.ascontiguousarray(scrambling, dtype=np.uint8)
+    chunks = np.ascontiguousarray(atlas.chunks, dtype=np.uint32)
+    culled = np.ascontiguousarray(atlas.culled, dtype=np.uint32)
+    adep, acol = _f32(atlas.depth), _f32(atlas.color_f32)
+    sky = None if sky_faces is None else _f32(sky_faces)
+    irr = None if irradiance is None else _f32(irradiance)
+    dep = None if depth_atlas is None else _f32(depth_atlas)
+    lib().oracle_sdf_reflection_f64(C.byref(params), _p(so), _p(sc), int(W), int(H), _p(_f32(normals)), _p(_f32(pbr)),
+                                    _p(_f32(depth)), C.byref(sdf_data), _p(_f32(sdf)), _p(_f32(mip)), int(res),
+                                    int(mip_res), C.byref(atlas.data), _p(chunks), _p(culled),
+                                    C.cast(atlas.objects, C.c_void_p), C.cast(atlas.tiles, C.c_void_p), _p(adep),
+                                    _p(acol), _p(sky), int(sky_size), C.byref(u) if u is not None else None, _p(irr),
+                                    _p(dep), _p(o), nthreads or threads())
+    return o

@@ -1322,6 +1322,201 @@ inline void surface_light_texel(const LuxgiSurfaceLightParams& prm, const LuxgiL
     out[0] = acc[0]; out[1] = acc[1]; out[2] = acc[2]; out[3] = T(1);
 }
 
+
+// ---------------------------------------------------------------------------
+// Noisy-signal producers: SDF soft-shadow mask (Shaders/SDF/SDFShadow.comp:36-157)
+// ---------------------------------------------------------------------------
+// sampleBlueNoise (Raytraced/BlueNoise.glsl:8-18) on RGBA8 tables: sobol [256], scrambling [128][128].
+// A UNORM8 texel v reads as v/255; int(clamp(v/255 * 256, 0, 255)) = v.
+inline float blue_noise(const uint8_t* sobol, const uint8_t* scr, int x, int y, int frame, int dim) {
+    x %= 128; y %= 128; frame %= 256; dim %= 4;
+    auto byte = [](uint8_t v) { return (int)std::min(std::max((float)v / 255.0f * 256.0f, 0.0f), 255.0f); };
+    const uint8_t* s = scr + ((size_t)y * 128 + x) * 4;
+    const int ranked = frame ^ byte(s[2]);
+    int value = byte(sobol[(size_t)ranked * 4 + dim]);
+    value = value ^ byte(s[dim % 2]);
+    return (0.5f + (float)value) / 256.0f;
+}
+
+// fetchLight with softShadow = true (SDFShadow.comp:43-109)
+template <class T>
+inline void fetch_light_soft(const LuxgiLight& L, v3<T> wp, v3<T> n, T rx, T ry, v3<T>& Wi, T& tmax, T& atten) {
+    v3<T> ld{0, 0, 0};
+    T radius = 0;
+    tmax = 0;
+    atten = 0;
+    const v3<T> lp{T(L.position[0]), T(L.position[1]), T(L.position[2])};
+    if (L.type == 0.0f) {
+        ld = {-T(L.direction[0]), -T(L.direction[1]), -T(L.direction[2])};
+        tmax = T(kGlobalSDFWorldSize);
+        radius = T(L.direction[3]);
+        atten = T(1);
+    } else if (L.type == 2.0f) {
+        const v3<T> dv = lp - wp;
+        const T dist = length(dv);
+        ld = normalize(dv);
+        atten = T(1);
+        tmax = dist;
+        radius = T(L.direction[3]) / dist;
+    } else if (L.type == 1.0f) {
+        const v3<T> dv = lp - wp;
+        const T cutoff = T(1) - T(L.angle);
+        ld = normalize(dv);
+        const T dist = length(dv);
+        const T theta = dot(ld, v3<T>{T(L.direction[0]), T(L.direction[1]), T(L.direction[2])});
+        const T e = cutoff - cutoff * T(0.9f);
+        atten = (theta - cutoff) / e;
+        atten *= T(L.radius) / (dist * dist + T(1));
+        atten = clampT(atten, T(0), T(1));
+        tmax = dist;
+        radius = T(L.direction[3]) / dist;
+    }
+    const v3<T> t = normalize(v3<T>{-ld.z, T(0), ld.x});  // cross(lightDir, (0, 1, 0))
+    const v3<T> b = normalize(v3<T>{t.y * ld.z - t.z * ld.y, t.z * ld.x - t.x * ld.z, t.x * ld.y - t.y * ld.x});
+    const T pr = radius * std::sqrt(rx);
+    const T pa = ry * T(2) * T(kPi);
+    Wi = normalize(ld + t * (pr * std::cos(pa)) + b * (pr * std::sin(pa)));
+    atten *= clampT(dot(n, Wi), T(0), T(1));
+}
+
+// Visibility bit of pixel (x, y) (x, y may pass the image edge: the G-buffer is read with repeat
+// addressing at the wrapped pixel, the world position comes from the unwrapped screen position).
+// depth R32F [H][W], normals RGBA32F [H][W] (octahedral normal in xy). Returns 0/1; *sky = depth == 1.
+template <class T>
+inline uint32_t shadow_visibility(const LuxgiSdfShadowParams& p, const uint8_t* sobol, const uint8_t* scr, int W, int H,
+                                  int x, int y, const float* depth, const float* normals, const LuxgiGlobalSDFData& g,
+                                  const float* sdf, const float* mip, int res, int mipRes, bool* sky) {
+    const size_t e = (size_t)(y % H) * W + (x % W);
+    const float d = depth[e];
+    *sky = d == 1.0f;
+    if (*sky) return 0;
+    const T tu = (T(x) + T(0.5)) / T(W), tv = (T(y) + T(0.5)) / T(H);
+    const v3<T> wp = world_from_depth<T>(tu, tv, T(d), p.viewProjInv);
+    const v3<T> n = octahedral_to_direction<T>(T(normals[e * 4]), T(normals[e * 4 + 1]));
+    const T rx = T(blue_noise(sobol, scr, x, y, (int)p.numFrames, 0));
+    const T ry = T(blue_noise(sobol, scr, x, y, (int)p.numFrames, 1));
+    v3<T> Wi;
+    T tmax, atten;
+    fetch_light_soft<T>(p.light, wp, n, rx, ry, Wi, tmax, atten);
+    const T NoL = dot(n, Wi);
+    if (!(NoL > T(0)) || !(atten > T(0))) return 0;
+    const T sb = T(p.shadowBias);
+    const T bias = T(2) * sb * clampT(T(1) - NoL, T(0), T(1)) + sb;
+    const Hit<T> h = trace_global_sdf<T>(g, sdf, mip, res, mipRes, wp + n * sb, Wi, tmax - bias,
+                                         p.maxSteps > 0 ? p.maxSteps : 250, T(0), nullptr, T(0), false);
+    return h.time >= T(0) ? 0u : 1u;
+}
+
+// ---------------------------------------------------------------------------
+// Noisy-signal producers: SDF reflection (Shaders/SDF/SDFReflection.comp:78-162)
+// ---------------------------------------------------------------------------
+// texture(samplerCube, d) with the alpha channel (the sky read of trace(), SDFReflection.comp:113)
+template <class T> inline v4<T> sample_cube4(const float* faces, int size, v3<T> d) {
+    T ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    int face;
+    T sc, tc, ma;
+    if (ax >= ay && ax >= az) { ma = ax; face = d.x >= 0 ? 0 : 1; sc = d.x >= 0 ? -d.z : d.z; tc = -d.y; }
+    else if (ay >= az) { ma = ay; face = d.y >= 0 ? 2 : 3; sc = d.x; tc = d.y >= 0 ? d.z : -d.z; }
+    else { ma = az; face = d.z >= 0 ? 4 : 5; sc = d.z >= 0 ? d.x : -d.x; tc = -d.y; }
+    T s = T(0.5) * (sc / ma + T(1)), t = T(0.5) * (tc / ma + T(1));
+    T x = s * T(size) - T(0.5), y = t * T(size) - T(0.5);
+    T fx0 = std::floor(x), fy0 = std::floor(y);
+    T a = x - fx0, b = y - fy0;
+    int x0 = (int)fx0, y0 = (int)fy0;
+    int x1 = std::min(std::max(x0 + 1, 0), size - 1), y1 = std::min(std::max(y0 + 1, 0), size - 1);
+    x0 = std::min(std::max(x0, 0), size - 1); y0 = std::min(std::max(y0, 0), size - 1);
+    const float* f = faces + (size_t)face * size * size * 4;
+    T r[4];
+    for (int c = 0; c < 4; ++c) {
+        T t00 = T(f[((size_t)y0 * size + x0) * 4 + c]), t10 = T(f[((size_t)y0 * size + x1) * 4 + c]);
+        T t01 = T(f[((size_t)y1 * size + x0) * 4 + c]), t11 = T(f[((size_t)y1 * size + x1) * 4 + c]);
+        r[c] = (t00 * (T(1) - a) + t10 * a) * (T(1) - b) + (t01 * (T(1) - a) + t11 * a) * b;
+    }
+    return {r[0], r[1], r[2], r[3]};
+}
+
+// importanceSampleGGX (Raytraced/BRDF.glsl:176-200), the half vector
+template <class T> inline v3<T> importance_sample_ggx(T ex, T ey, v3<T> N, T roughness) {
+    const T a = roughness * roughness, m2 = a * a;
+    const T phi = T(2) * T(kPi) * ex;
+    const T cosT = std::sqrt((T(1) - ey) / (T(1) + (m2 - T(1)) * ey));
+    const T sinT = std::sqrt(T(1) - cosT * cosT);
+    const v3<T> Hl{std::cos(phi) * sinT, std::sin(phi) * sinT, cosT};
+    const v3<T> up = std::fabs(N.z) < T(0.999f) ? v3<T>{0, 0, 1} : v3<T>{1, 0, 0};
+    const v3<T> t = normalize(v3<T>{up.y * N.z - up.z * N.y, up.z * N.x - up.x * N.z, up.x * N.y - up.y * N.x});
+    const v3<T> b{N.y * t.z - N.z * t.y, N.z * t.x - N.x * t.z, N.x * t.y - N.y * t.x};
+    return normalize(t * Hl.x + b * Hl.y + N * Hl.z);
+}
+
+template <class T> inline v3<T> reflect3(v3<T> I, v3<T> N) { return I - N * (T(2) * dot(N, I)); }
+
+struct ReflectionInputs {
+    const LuxgiSdfReflectionParams* p;
+    const uint8_t *sobol, *scr;
+    int W, H;
+    const float *normals, *pbr, *depth;       // G-buffer (RGBA32F normals and PBR, R32F depth)
+    const LuxgiGlobalSDFData* g;
+    const float *sdf, *mip;
+    int res, mipRes;
+    const LuxgiSurfaceAtlasData* ad;
+    const uint32_t *chunks, *culled;
+    const LuxgiAtlasObject* objects;
+    const LuxgiAtlasTile* tiles;
+    const float *atlasDepth, *atlasColor;   // float [res][res] and RGBA [res][res][4]
+    const float* skyFaces;
+    int skySize;
+    const LuxgiDDGIUniform* u;
+    const float *irr, *dep;
+};
+
+// One pixel; returns false for sky (the texel is not written).
+template <class T>
+inline bool reflection_pixel(const ReflectionInputs& in, int x, int y, T out[4]) {
+    const LuxgiSdfReflectionParams& p = *in.p;
+    const size_t e = (size_t)y * in.W + x;
+    const float d = in.depth[e];
+    if (d == 1.0f) return false;
+    const T tu = (T(x) + T(0.5)) / T(in.W), tv = (T(y) + T(0.5)) / T(in.H);
+    v3<T> wp = world_from_depth<T>(tu, tv, T(d), p.viewProjInv);
+    const T roughness = T(in.pbr[e * 4 + 1]);
+    const v3<T> n = octahedral_to_direction<T>(T(in.normals[e * 4]), T(in.normals[e * 4 + 1]));
+    const v3<T> cam{T(p.cameraPosition[0]), T(p.cameraPosition[1]), T(p.cameraPosition[2])};
+    const v3<T> Wo = normalize(cam - wp);
+    wp = wp + n * T(in.g->cascadeVoxelSize[0]);
+    v4<T> col{0, 0, 0, 0};
+    v3<T> R;
+    bool traceRay = true;
+    const v3<T> mWo{-Wo.x, -Wo.y, -Wo.z};
+    if (roughness < T(0.05f)) {
+        R = reflect3<T>(mWo, n);
+    } else if (roughness > T(0.45f) && p.approximateWithDDGI == 1u) {
+        R = reflect3<T>(mWo, n);
+        traceRay = false;
+        const v3<T> ir = sample_irradiance<T>(*in.u, wp, R, Wo, in.irr, in.dep);
+        col = {T(p.roughDDGIIntensity) * ir.x, T(p.roughDDGIIntensity) * ir.y, T(p.roughDDGIIntensity) * ir.z, T(0)};
+    } else {
+        const T ex = T(blue_noise(in.sobol, in.scr, x, y, (int)p.numFrames, 0)) * T(p.trim);
+        const T ey = T(blue_noise(in.sobol, in.scr, x, y, (int)p.numFrames, 1)) * T(p.trim);
+        R = reflect3<T>(mWo, importance_sample_ggx<T>(ex, ey, n, roughness));
+    }
+    if (traceRay) {
+        const int steps = p.maxSteps > 0 ? p.maxSteps : 250;
+        const Hit<T> h = trace_global_sdf<T>(*in.g, in.sdf, in.mip, in.res, in.mipRes, wp, R, T(kGlobalSDFWorldSize),
+                                             steps, T(0), nullptr, T(0), false);
+        if (h.time >= T(0)) {
+            const T thr = T(in.g->cascadeVoxelSize[h.cascade]) * T(1.05f);
+            col = sample_surface_atlas<T>(*in.ad, in.chunks, in.culled, in.objects, in.tiles, in.atlasDepth,
+                                          in.atlasColor, wp + R * h.time, v3<T>{-R.x, -R.y, -R.z}, thr, nullptr);
+        } else if (in.skyFaces) {
+            col = sample_cube4<T>(in.skyFaces, in.skySize, R);
+        } else {
+            col = {T(p.skyColor[0]), T(p.skyColor[1]), T(p.skyColor[2]), T(p.skyColor[3])};
+        }
+    }
+    out[0] = col.x; out[1] = col.y; out[2] = col.z; out[3] = col.w;
+    return true;
+}
+
 template <class F> inline void parallel_for(int n, int threads, F f) {
     if (threads <= 1 || n < 2) {
         for (int i = 0; i < n; ++i) f(i);

@@ -387,3 +387,53 @@ extern "C" void oracle_brdf(const double* albedo, const double* n, double rough,
                                     {view[0], view[1], view[2]}, {h[0], h[1], h[2]}, {l[0], l[1], l[2]});
     out[0] = r.x; out[1] = r.y; out[2] = r.z;
 }
+
+// SDFShadow.comp:113-157 over the image: mask [ceil(H/4)][ceil(W/8)]; a word is written only when its
+// block's first pixel is not sky (other words keep their value).
+template <class T>
+static void shadow_mask_impl(const LuxgiSdfShadowParams* p, const uint8_t* sobol, const uint8_t* scr, int W, int H,
+                             const float* depth, const float* normals, const LuxgiGlobalSDFData* g, const float* sdf,
+                             const float* mip, int res, int mipRes, uint32_t* mask, int threads) {
+    const int mw = (W + 7) / 8, mh = (H + 3) / 4;
+    parallel_for(mh, threads, [&](int wy) {
+        for (int wx = 0; wx < mw; ++wx) {
+            uint32_t word = 0;
+            bool firstSky = true;
+            for (int k = 0; k < 32; ++k) {
+                bool sky;
+                const uint32_t v = shadow_visibility<T>(*p, sobol, scr, W, H, wx * 8 + (k & 7), wy * 4 + (k >> 3), depth,
+                                                        normals, *g, sdf, mip, res, mipRes, &sky);
+                if (k == 0) firstSky = sky;
+                word |= v << k;
+            }
+            if (!firstSky) mask[(size_t)wy * mw + wx] = word;
+        }
+    });
+}
+extern "C" void oracle_shadow_mask_f64(const LuxgiSdfShadowParams* p, const uint8_t* sobol, const uint8_t* scr, int W,
+                                       int H, const float* depth, const float* normals, const LuxgiGlobalSDFData* g,
+                                       const float* sdf, const float* mip, int res, int mipRes, uint32_t* mask,
+                                       int threads) {
+    shadow_mask_impl<double>(p, sobol, scr, W, H, depth, normals, g, sdf, mip, res, mipRes, mask, threads);
+}
+extern "C" float oracle_blue_noise(const uint8_t* sobol, const uint8_t* scr, int x, int y, int frame, int dim) {
+    return blue_noise(sobol, scr, x, y, frame, dim);
+}
+
+// SDFReflection.comp:116-162 over the image: out [H][W][4] (sky pixels keep their value).
+extern "C" void oracle_sdf_reflection_f64(const LuxgiSdfReflectionParams* p, const uint8_t* sobol, const uint8_t* scr,
+                                          int W, int H, const float* normals, const float* pbr, const float* depth,
+                                          const LuxgiGlobalSDFData* g, const float* sdf, const float* mip, int res,
+                                          int mipRes, const LuxgiSurfaceAtlasData* ad, const uint32_t* chunks,
+                                          const uint32_t* culled, const LuxgiAtlasObject* objects,
+                                          const LuxgiAtlasTile* tiles, const float* atlasDepth,
+                                          const float* atlasColor, const float* skyFaces, int skySize,
+                                          const LuxgiDDGIUniform* u, const float* irr, const float* dep, double* out,
+                                          int threads) {
+    ReflectionInputs in{p,   sobol,  scr,   W,     H,       normals,    pbr,        depth, g,       sdf,  mip,
+                        res, mipRes, ad,    chunks, culled, objects,    tiles,      atlasDepth,    atlasColor,
+                        skyFaces, skySize, u, irr, dep};
+    parallel_for(H, threads, [&](int y) {
+        for (int x = 0; x < W; ++x) reflection_pixel<double>(in, x, y, out + ((size_t)y * W + x) * 4);
+    });
+}

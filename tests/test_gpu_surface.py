@@ -70,7 +70,7 @@ def _lights():
     arr[1].intensity, arr[1].radius, arr[1].type = 1.5, 1.2, abi.LIGHT_POINT
     arr[2].color[:] = [1.0, 0.3, 0.3, 1.0]
     arr[2].position[:] = [-0.5, -0.2, 0.6, 1.0]
-    arr[2].direction[:] = [0.5, 0.1, -0.86, 0.0]
+    arr[2].direction[:] = [-0.5, -0.1, 0.86, 0.0]  # surface-to-light along the cone axis (SDFDeferredLight.frag:68)
     arr[2].intensity, arr[2].radius, arr[2].type, arr[2].angle = 3.0, 1.5, abi.LIGHT_SPOT, 0.3
     return arr
 

@@ -17,7 +17,8 @@ culling of the objects and the relight of every atlas tile (emissive copy, one d
 light with an SDF shadow ray per lit texel, DDGI bounce) into an RGBA16F light cache, and
 the SDF soft-shadow mask at 1920x1080 (sdf_shadow_mask_ms: one blue-noise disk ray per pixel) and
 the SDF reflection rays at 1920x1080 (sdf_reflection_ms: one GGX-sampled ray per pixel with the
-surface-atlas lookup at its hit, RGBA16F out).
+surface-atlas lookup at its hit, RGBA16F out), and the global SDF build from mesh SDFs (sdf_build:
+2,000 instances of three baked 32^3 meshes rasterized into the 256^3 cascade).
 value = rays / DDGI-update time (Grays/s). All timing uses HIP events on the
 stream the kernels are launched on (torch's current stream), W warm-up steps
 then exactly K timed steps between device synchronisations.
@@ -252,6 +253,31 @@ def main():
     refl_ms = rev[0].elapsed_time(rev[1]) / args.steps
     log(f"[bench] sdf reflection 1080p {refl_ms:.3f} ms", q)
 
+    # ---------------- global SDF build from mesh SDFs (reported beside the step) ----------------
+    # the 2,000 primitives as instances of three baked 32^3 unit meshes (+2 mips), rasterized into
+    # all 512 chunks of the 256^3 cascade (luxgi_sdf_rasterize; FP32 volume, as the trace reads it)
+    from luxgi_amd import meshsdf
+    msh, mobj, mbounds, mwhich = meshsdf.primitive_instances(prims)
+    cas0 = meshsdf.Cascade(0, (0.0, 0.0, 0.0), 1.0, res)
+    mlists = meshsdf.chunk_lists(cas0, mbounds)
+    builder = meshsdf.GlobalSDFBuilder(msh, mobj, len(prims), which=mwhich)
+    built = torch.empty((res, res, res), dtype=torch.float32, device="cuda")
+    for _ in range(2):
+        builder.rasterize(cas0, mlists, built)
+    bev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    bev[0].record()
+    for _ in range(args.steps):
+        builder.rasterize(cas0, mlists, built)
+    bev[1].record()
+    torch.cuda.synchronize()
+    build_ms = bev[0].elapsed_time(bev[1]) / args.steps
+    near = torch.abs(sdf) < 0.02
+    build_err = float(torch.abs(built - sdf)[near].max())
+    sdf_build = {"rasterize_ms": build_ms, "objects": len(prims), "chunk_object_pairs": int(mlists[1][-1]),
+                 "voxels": res ** 3, "max_abs_diff_vs_analytic_near_surface": build_err}
+    log(f"[bench] global SDF rasterize {build_ms:.3f} ms", q)
+    del builder, built
+
     # ---------------- reduced-precision form (reported beside the step, not in value) ----------------
     # FP16 SDF bricks with per-brick scale and FP32 re-evaluation at hits, FP16-MFMA blend (one
     # rounding per operand, FP32 accumulation), SVGF with RGBA16F intermediates. Ray buffers stay
@@ -352,6 +378,7 @@ def main():
         "surface_atlas": surf,
         "sdf_shadow_mask_ms": shadow_ms,
         "sdf_reflection_ms": refl_ms,
+        "sdf_build": sdf_build,
         "trace_grays_per_s": rays_per_frame / trace_s / 1e9,
         "trace_counts": {"lookups": n_lookup, "normals": n_normal, "objects": c[3], "tile_gathers": n_tile,
                          "rays": n_rays, "steps": c[6], "hits": c[7]},

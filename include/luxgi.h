@@ -349,6 +349,56 @@ int luxgi_sdf_build_mip(const float* sdf, int32_t resolution, int32_t cascades, 
                         float maxDistance, int32_t floodPasses,
                         float* mip, float* temp, void* stream);
 
+/* ObjectRasterizeData: Shaders/SDF/ObjectRasterizeData.glsl (std430; host GlobalDistanceField.cpp:102-112,
+ * filled by chunkCalculate :486-504). 176 bytes. */
+typedef struct LuxgiObjectRasterizeData {
+    float worldToVolume[16];        /* column-major */
+    float volumeToWorld[16];
+    float volumeToUVWMul[3];
+    float mipOffset;                /* mesh-SDF mip level read (min(cascade, 2)) */
+    float volumeToUVWAdd[3];
+    float decodeMul;                /* distance = (texel * 2 - 1) * decodeMul */
+    float volumeLocalBoundsExtent[3];
+    float decodeAdd;
+} LuxgiObjectRasterizeData;
+
+/* ModelsRasterizeData UBO (SDFRasterizeModel.glsl:16-24; host GlobalDistanceField.cpp:756-761). 48 bytes. */
+typedef struct LuxgiRasterizeParams {
+    float   cascadeCoordToPosMul[3];
+    float   maxDistance;            /* the cascade's full extent */
+    float   cascadeCoordToPosAdd[3];
+    int32_t cascadeResolution;
+    int32_t cascadeIndex;
+    int32_t pad[3];
+} LuxgiRasterizeParams;
+
+/* One object's mesh SDF (SDFBaker.cpp:158-200): RGBA-less R16F texels [z][y][x] per mip level,
+ * stored as FP16 bits in a shared pool; the engine's Texture3D (repeat addressing, linear filter).
+ * 72 bytes. */
+typedef struct LuxgiMeshSDF {
+    uint64_t offset[3];   /* element offset of mip levels 0 .. mipCount-1 in the pool */
+    int32_t  size[3][3];  /* x, y, z of each level */
+    int32_t  mipCount;    /* 1 .. 3 */
+    int32_t  pad[2];
+} LuxgiMeshSDF;
+
+/* SDFRasterizeModel.glsl:42-63 with GlobalSDFClear.comp:27-33 (dispatch GlobalDistanceField.cpp:700-816):
+ * for each listed 32^3 chunk of one cascade, every voxel gets clamp(min over the chunk's objects of
+ * distanceToModelSDF (SDFCommon.glsl:38-62) / maxDistance, -1, 1), or 1 for a chunk without objects.
+ * The objects of a chunk are taken in list order in batches of 28 (GLOBAL_SDF_RASTERIZE_MODEL_MAX_COUNT):
+ * from the second batch on the running distance restarts from the stored value, as the reference's
+ * READ_DISTANCE layers do (so FP16 storage rounds between batches). chunkOrigins [numChunks][4] holds
+ * the voxel coordinates of each chunk's first voxel within the cascade; chunkObjectStart
+ * [numChunks + 1] and chunkObjects form the per-chunk object lists (indices into objects and meshes;
+ * out-of-range entries are skipped). sdf is the [res][res][cascades*res] global SDF (FP32 or FP16);
+ * voxels outside the cascade are not written. The reference drops a full chunk's first 28 objects
+ * when it overflows (the reference assigns the next layer into the full one, GlobalDistanceField.cpp:521-525);
+ * here every listed object is used. */
+int luxgi_sdf_rasterize(const LuxgiRasterizeParams* params, const LuxgiObjectRasterizeData* objects,
+                        const LuxgiMeshSDF* meshes, const uint16_t* pool, int32_t numObjects,
+                        const int32_t* chunkOrigins, const uint32_t* chunkObjectStart, const uint32_t* chunkObjects,
+                        int32_t numChunks, int32_t cascades, int32_t storage, void* sdf, void* stream);
+
 /* Bytes of a LUXGI_LAYOUT_BRICK4 volume of width x height x depth voxels (multiples of 4) in
  * `storage` (FP32 or FP16): (W/4)(H/4)(D/4) bricks of 128 elements. 0 for invalid arguments. */
 size_t luxgi_sdf_bricks_bytes(int32_t width, int32_t height, int32_t depth, int32_t storage);

@@ -27,6 +27,7 @@ SIGNATURES = {
     "luxgi_ddgi_relocate_classify": (C.c_int, [P, P, P, P, P, P, P]),
     "luxgi_ddgi_sample_probes": (C.c_int, [P, P, P, I, I, P, P, I, P, P]),
     "luxgi_sdf_build_mip": (C.c_int, [P, I, I, I, C.c_float, I, P, P, P]),
+    "luxgi_sdf_rasterize": (C.c_int, [P, P, P, P, I, P, P, P, I, I, I, P, P]),
     "luxgi_sdf_bricks_bytes": (C.c_size_t, [I, I, I, I]),
     "luxgi_sdf_to_bricks": (C.c_int, [P, I, I, I, I, P, P, P]),
     "luxgi_sdf_build_guard": (C.c_int, [P, I, I, I, P, P]),

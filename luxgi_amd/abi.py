@@ -187,11 +187,28 @@ class SdfReflectionParams(C.Structure):  # Shaders/SDF/SDFReflection.comp:64-76 
                 ("maxSteps", C.c_int32), ("pad", C.c_int32 * 3), ("skyColor", f4)]
 
 
+class ObjectRasterizeData(C.Structure):  # Shaders/SDF/ObjectRasterizeData.glsl
+    _fields_ = [("worldToVolume", f16), ("volumeToWorld", f16), ("volumeToUVWMul", C.c_float * 3),
+                ("mipOffset", C.c_float), ("volumeToUVWAdd", C.c_float * 3), ("decodeMul", C.c_float),
+                ("volumeLocalBoundsExtent", C.c_float * 3), ("decodeAdd", C.c_float)]
+
+
+class RasterizeParams(C.Structure):  # SDFRasterizeModel.glsl:16-24
+    _fields_ = [("cascadeCoordToPosMul", C.c_float * 3), ("maxDistance", C.c_float),
+                ("cascadeCoordToPosAdd", C.c_float * 3), ("cascadeResolution", C.c_int32),
+                ("cascadeIndex", C.c_int32), ("pad", C.c_int32 * 3)]
+
+
+class MeshSDF(C.Structure):
+    _fields_ = [("offset", C.c_uint64 * 3), ("size", (C.c_int32 * 3) * 3), ("mipCount", C.c_int32),
+                ("pad", C.c_int32 * 2)]
+
+
 _SIZES = {DDGIUniform: 96, GlobalSDFData: 96, SurfaceAtlasData: 32, AtlasObject: 128, AtlasTile: 96,
           TraceParams: 112, ReflectionReprojectParams: 168, ReflectionAtrousParams: 24,
           ShadowReprojectParams: 80, ShadowAtrousParams: 32, SampleProbeParams: 80, Light: 64,
           SurfaceGBuffer: 40, SurfaceLightParams: 32, SurfaceDraw: 8, BlueNoise: 16, SdfShadowParams: 144,
-          SdfReflectionParams: 144}
+          SdfReflectionParams: 144, ObjectRasterizeData: 176, RasterizeParams: 48, MeshSDF: 72}
 for _t, _n in _SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
 

@@ -79,6 +79,10 @@ def _declare(L):
     L.oracle_shadow_mask_f64.argtypes = [P, P, P, I, I, P, P, P, P, P, I, I, P, I]
     L.oracle_blue_noise.argtypes = [P, P, I, I, I, I]
     L.oracle_blue_noise.restype = C.c_float
+    for s_ in ("f64", "f32"):
+        getattr(L, "oracle_sdf_rasterize_" + s_).argtypes = [P, P, P, P, I, P, P, P, I, I, I, P, I]
+    L.oracle_round_to_half.argtypes = [C.c_float]
+    L.oracle_round_to_half.restype = C.c_float
     L.oracle_sdf_reflection_f64.argtypes = [P, P, P, I, I, P, P, P, P, P, P, I, I, P, P, P, P, P, P, P, P, I, P, P, P,
                                             P, I]
 
@@ -427,3 +431,28 @@ def sdf_reflection(params, sobol, scrambling, W, H, normals, pbr, depth, sdf_dat
                                     _p(acol), _p(sky), int(sky_size), C.byref(u) if u is not None else None, _p(irr),
                                     _p(dep), _p(o), nthreads or threads())
     return o
+
+
+# ---------------------------------------------------------------------------
+# Global SDF build from mesh SDFs
+# ---------------------------------------------------------------------------
+def sdf_rasterize(params, objects, meshes, pool, num_objects, origins, start, objs, cascades=1, half=False, out=None,
+                  precision="f64", nthreads=None):
+    """SDFRasterizeModel over listed chunks; objects / meshes ctypes arrays, pool uint16, origins int32
+    [n][4], start uint32 [n + 1], objs uint32. Returns the float volume [R][R][cascades R] (`out`'s
+    values outside the listed chunks)."""
+    R = params.cascadeResolution
+    vol = np.ones((R, R, cascades * R), np.float32) if out is None else np.ascontiguousarray(out, np.float32).copy()
+    pool = np.ascontiguousarray(pool, dtype=np.uint16)
+    origins = np.ascontiguousarray(origins, dtype=np.int32)
+    start = np.ascontiguousarray(start, dtype=np.uint32)
+    objs = np.ascontiguousarray(objs if len(objs) else np.zeros(1, np.uint32), dtype=np.uint32)
+    fn = getattr(lib(), "oracle_sdf_rasterize_" + precision)
+    fn(C.byref(params), C.cast(objects, C.c_void_p), C.cast(meshes, C.c_void_p), _p(pool), int(num_objects),
+       _p(origins), _p(start), _p(objs), int(len(start) - 1), int(cascades), 1 if half else 0, _p(vol),
+       nthreads or threads())
+    return vol
+
+
+def round_to_half(f):
+    return lib().oracle_round_to_half(float(f))

@@ -1517,6 +1517,95 @@ inline bool reflection_pixel(const ReflectionInputs& in, int x, int y, T out[4])
     return true;
 }
 
+
+// ---------------------------------------------------------------------------
+// Global SDF build from mesh SDFs (Shaders/SDF/SDFRasterizeModel.glsl:42-63, GlobalSDFClear.comp:27-33,
+// distanceToModelSDF SDFCommon.glsl:38-62)
+// ---------------------------------------------------------------------------
+// float -> nearest half (ties to even) -> float, the rounding of an R16F imageStore
+inline float round_to_half(float f) {
+    if (!(std::fabs(f) < 65520.0f)) return f;  // not reached by clamped distances
+    if (f == 0.0f) return f;
+    int e;
+    std::frexp(f, &e);                      // |f| in [2^(e-1), 2^e)
+    const int ulpExp = std::max(e - 11, -24);  // 11 significant bits, subnormals below 2^-14
+    return (float)std::nearbyint((double)f * std::ldexp(1.0, -ulpExp)) * (float)std::ldexp(1.0, ulpExp);
+}
+
+inline float half_bits_to_float(uint16_t h) {
+    const int s = h >> 15, e = (h >> 10) & 31, m = h & 1023;
+    float v;
+    if (e == 0) v = std::ldexp((float)m, -24);
+    else if (e == 31) v = m ? NAN : INFINITY;
+    else v = std::ldexp((float)(m | 1024), e - 25);
+    return s ? -v : v;
+}
+
+// textureLod of an R16F mesh SDF level: trilinear, repeat addressing, level clamped to the chain
+template <class T>
+inline T mesh_sample(const uint16_t* pool, const LuxgiMeshSDF& m, int level, v3<T> uvw) {
+    level = std::max(0, std::min(level, m.mipCount - 1));
+    const int W = m.size[level][0], H = m.size[level][1], D = m.size[level][2];
+    const uint16_t* v = pool + m.offset[level];
+    const T x = uvw.x * T(W) - T(0.5), y = uvw.y * T(H) - T(0.5), z = uvw.z * T(D) - T(0.5);
+    const T fx = std::floor(x), fy = std::floor(y), fz = std::floor(z);
+    const T ax = x - fx, ay = y - fy, az = z - fz;
+    auto wrap = [](long long i, int n) { long long r = i % n; return (int)(r < 0 ? r + n : r); };
+    const int x0 = wrap((long long)fx, W), x1 = wrap((long long)fx + 1, W);
+    const int y0 = wrap((long long)fy, H), y1 = wrap((long long)fy + 1, H);
+    const int z0 = wrap((long long)fz, D), z1 = wrap((long long)fz + 1, D);
+    auto t = [&](int xi, int yi, int zi) { return T(half_bits_to_float(v[((size_t)zi * H + yi) * W + xi])); };
+    const T c00 = t(x0, y0, z0) * (T(1) - ax) + t(x1, y0, z0) * ax, c10 = t(x0, y1, z0) * (T(1) - ax) + t(x1, y1, z0) * ax;
+    const T c01 = t(x0, y0, z1) * (T(1) - ax) + t(x1, y0, z1) * ax, c11 = t(x0, y1, z1) * (T(1) - ax) + t(x1, y1, z1) * ax;
+    return (c00 * (T(1) - ay) + c10 * ay) * (T(1) - az) + (c01 * (T(1) - ay) + c11 * ay) * az;
+}
+
+template <class T>
+inline T distance_to_model_sdf(T minDistance, const LuxgiObjectRasterizeData& o, const LuxgiMeshSDF& m,
+                               const uint16_t* pool, v3<T> wp) {
+    const v3<T> vp = mat4_mul<T>(o.worldToVolume, wp, T(1));
+    const v3<T> uvw{vp.x * T(o.volumeToUVWMul[0]) + T(o.volumeToUVWAdd[0]), vp.y * T(o.volumeToUVWMul[1]) + T(o.volumeToUVWAdd[1]),
+                    vp.z * T(o.volumeToUVWMul[2]) + T(o.volumeToUVWAdd[2])};
+    const v3<T> e{T(o.volumeLocalBoundsExtent[0]), T(o.volumeLocalBoundsExtent[1]), T(o.volumeLocalBoundsExtent[2])};
+    const v3<T> vc{clampT(vp.x, -e.x, e.x), clampT(vp.y, -e.y, e.y), clampT(vp.z, -e.z, e.z)};
+    const v3<T> wc = mat4_mul<T>(o.volumeToWorld, vc, T(1));
+    T dv = length(wp - wc);
+    if (dv < T(0.01f)) dv = length(vp - vc);
+    dv = std::max(dv, T(0));
+    if (minDistance <= dv) return dv;
+    const T sd = (mesh_sample<T>(pool, m, (int)o.mipOffset, uvw) * T(2) - T(1)) * T(o.decodeMul);
+    T r = combine_distance_to_sdf<T>(sd, dv);
+    if (dv > T(0)) r = std::max(dv, r);
+    return r;
+}
+
+// One voxel of a listed chunk: objects in list order, batches of 28 restarting from the stored value.
+template <class T>
+inline T rasterize_voxel(const LuxgiRasterizeParams& p, const LuxgiObjectRasterizeData* objects,
+                         const LuxgiMeshSDF* meshes, const uint16_t* pool, int numObjects, const uint32_t* list,
+                         uint32_t n, int vx, int vy, int vz, bool half) {
+    const v3<T> wp{T(vx) * T(p.cascadeCoordToPosMul[0]) + T(p.cascadeCoordToPosAdd[0]),
+                   T(vy) * T(p.cascadeCoordToPosMul[1]) + T(p.cascadeCoordToPosAdd[1]),
+                   T(vz) * T(p.cascadeCoordToPosMul[2]) + T(p.cascadeCoordToPosAdd[2])};
+    const T maxD = T(p.maxDistance);
+    T minDistance = maxD, stored = T(1);
+    for (uint32_t b = 0; b < n; b += 28) {
+        if (b != 0) minDistance = maxD * stored;
+        for (uint32_t k = b; k < std::min<uint32_t>(n, b + 28); ++k) {
+            const uint32_t id = list[k];
+            if (id >= (uint32_t)numObjects) continue;
+            const LuxgiMeshSDF& m = meshes[id];
+            bool ok = m.mipCount >= 1 && m.mipCount <= 3;
+            for (int l = 0; ok && l < m.mipCount; ++l) ok = m.size[l][0] > 0 && m.size[l][1] > 0 && m.size[l][2] > 0;
+            if (!ok) continue;
+            minDistance = std::min(minDistance, distance_to_model_sdf<T>(minDistance, objects[id], m, pool, wp));
+        }
+        stored = clampT(minDistance / maxD, T(-1), T(1));
+        if (half) stored = T(round_to_half((float)stored));
+    }
+    return stored;
+}
+
 template <class F> inline void parallel_for(int n, int threads, F f) {
     if (threads <= 1 || n < 2) {
         for (int i = 0; i < n; ++i) f(i);

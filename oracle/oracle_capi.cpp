@@ -437,3 +437,40 @@ extern "C" void oracle_sdf_reflection_f64(const LuxgiSdfReflectionParams* p, con
         for (int x = 0; x < W; ++x) reflection_pixel<double>(in, x, y, out + ((size_t)y * W + x) * 4);
     });
 }
+
+// SDFRasterizeModel over the listed chunks of one cascade into a float volume [R][R][cascades*R]
+// (values as stored: FP16-rounded when `half`).
+template <class T>
+static void sdf_rasterize_impl(const LuxgiRasterizeParams* p, const LuxgiObjectRasterizeData* objects,
+                               const LuxgiMeshSDF* meshes, const uint16_t* pool, int numObjects, const int32_t* origins,
+                               const uint32_t* start, const uint32_t* list, int numChunks, int cascades, int half,
+                               float* out, int threads) {
+    const int R = p->cascadeResolution, W = R * cascades;
+    parallel_for(numChunks, threads, [&](int c) {
+        const int* o = origins + (size_t)c * 4;
+        for (int z = 0; z < 32; ++z)
+            for (int y = 0; y < 32; ++y)
+                for (int x = 0; x < 32; ++x) {
+                    const int vx = o[0] + x, vy = o[1] + y, vz = o[2] + z;
+                    if (vx < 0 || vy < 0 || vz < 0 || vx >= R || vy >= R || vz >= R) continue;
+                    const T v = rasterize_voxel<T>(*p, objects, meshes, pool, numObjects, list + start[c],
+                                                   start[c + 1] - start[c], vx, vy, vz, half != 0);
+                    out[((size_t)vz * R + vy) * W + (size_t)p->cascadeIndex * R + vx] = (float)v;
+                }
+    });
+}
+extern "C" void oracle_sdf_rasterize_f64(const LuxgiRasterizeParams* p, const LuxgiObjectRasterizeData* objects,
+                                         const LuxgiMeshSDF* meshes, const uint16_t* pool, int numObjects,
+                                         const int32_t* origins, const uint32_t* start, const uint32_t* list,
+                                         int numChunks, int cascades, int half, float* out, int threads) {
+    sdf_rasterize_impl<double>(p, objects, meshes, pool, numObjects, origins, start, list, numChunks, cascades, half,
+                               out, threads);
+}
+extern "C" void oracle_sdf_rasterize_f32(const LuxgiRasterizeParams* p, const LuxgiObjectRasterizeData* objects,
+                                         const LuxgiMeshSDF* meshes, const uint16_t* pool, int numObjects,
+                                         const int32_t* origins, const uint32_t* start, const uint32_t* list,
+                                         int numChunks, int cascades, int half, float* out, int threads) {
+    sdf_rasterize_impl<float>(p, objects, meshes, pool, numObjects, origins, start, list, numChunks, cascades, half,
+                              out, threads);
+}
+extern "C" float oracle_round_to_half(float f) { return round_to_half(f); }

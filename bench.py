@@ -271,10 +271,18 @@ def main():
     bev[1].record()
     torch.cuda.synchronize()
     build_ms = bev[0].elapsed_time(bev[1]) / args.steps
-    near = torch.abs(sdf) < 0.02
-    build_err = float(torch.abs(built - sdf)[near].max())
+    # against the analytic SDF on the voxels within one voxel of a surface: the instances scale unit
+    # meshes non-uniformly (distances scaled by one factor, as the engine does) and a chunk is only
+    # rasterized with the objects whose bounds + 4 voxels reach it (other chunks clear to 1), so
+    # report the distribution in voxels and the cleared fraction rather than a maximum
+    vox = 1.0 / res  # one voxel in the stored (distance / 2) units
+    near = torch.abs(sdf) < vox
+    d_near = (torch.abs(built - sdf)[near] / vox).float()
     sdf_build = {"rasterize_ms": build_ms, "objects": len(prims), "chunk_object_pairs": int(mlists[1][-1]),
-                 "voxels": res ** 3, "max_abs_diff_vs_analytic_near_surface": build_err}
+                 "voxels": res ** 3,
+                 "near_surface_abs_diff_voxels": {"median": float(d_near.median()),
+                                                  "p99": float(torch.quantile(d_near[:1 << 24], 0.99))},
+                 "near_surface_cleared_fraction": float((built[near] >= 1.0).float().mean())}
     log(f"[bench] global SDF rasterize {build_ms:.3f} ms", q)
     del builder, built
 

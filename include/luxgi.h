@@ -697,6 +697,45 @@ int luxgi_sdf_reflection(const LuxgiSdfReflectionParams* params, const LuxgiBlue
                          const LuxgiDDGIUniform* ddgi, const LuxgiProbeAtlas* probes, int32_t outStorage, void* out,
                          void* workspace, size_t workspaceBytes, void* stream);
 
+/* ------------------------------------------------------------------------- */
+/* Learned a-trous edge-stopping weights (no counterpart in the reference)     */
+/* ------------------------------------------------------------------------- */
+
+#define LUXGI_LEARNED_ATROUS_PARAMS 81 /* W1 [10][6], b1 [10], w2 [10], b2 */
+
+/* One pass of the learned filter: the parameters of Reflection/DenoiseAtrous.comp plus the level. */
+typedef struct LuxgiLearnedAtrousParams {
+    int32_t radius;      /* 0 .. 2 */
+    int32_t stepSize;    /* 1 << iteration */
+    float   phiColor;
+    float   sigmaDepth;
+    float   level;       /* network input: iteration / (iterations - 1) */
+    int32_t pad[3];
+} LuxgiLearnedAtrousParams;
+
+/* The reflection a-trous pass (Reflection/DenoiseAtrous.comp:63-134: 3x3 variance at the centre,
+ * sky outputs 0, roughness outside (0.05, 0.45) passes through) with each tap's edge-stopping x
+ * kernel weight replaced by exp(MLP(features)), the MLP of the 81 `weights` (one hidden layer of 10
+ * ReLU units over the tap's luminance difference / phi, depth term, normal term, kernel weight, level
+ * and the centre's standard deviation). in / out [H][W] RGBA32F (rgb, variance); gbuffer normals and
+ * pbr as for luxgi_svgf_reflection_atrous. sumW (nullable, [H][W]) keeps 1 + the tap weights for
+ * luxgi_svgf_learned_atrous_backward. Every pixel is evaluated (no tile map). */
+int luxgi_svgf_learned_atrous(const LuxgiLearnedAtrousParams* params, const float* weights, int32_t width,
+                              int32_t height, const LuxgiGBuffer* gbuffer, const float* in, float* out, float* sumW,
+                              void* stream);
+
+/* Backward of one learned pass: from dL/d(out rgb) (gradOut [H][W][4], alpha ignored) writes
+ * dL/d(in rgb) into gradIn ([H][W][4], alpha 0) and adds dL/d(weights) into gradWeights [81]. The
+ * weights' dependence on the input colour (through luminance and variance) is held fixed. */
+int luxgi_svgf_learned_atrous_backward(const LuxgiLearnedAtrousParams* params, const float* weights, int32_t width,
+                                       int32_t height, const LuxgiGBuffer* gbuffer, const float* in, const float* out,
+                                       const float* sumW, const float* gradOut, float* gradIn, float* gradWeights,
+                                       void* stream);
+
+/* One Adam step on n parameters (step counts from 1); clears `grad`. */
+int luxgi_adam_step(float* weights, float* grad, float* m, float* v, int32_t n, float lr, float beta1, float beta2,
+                    float eps, int32_t step, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

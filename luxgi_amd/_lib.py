@@ -44,6 +44,9 @@ SIGNATURES = {
     "luxgi_svgf_reflection_denoise": (C.c_int, [P, P, I, I, I, I, P, P, P, P, P, P, P, I, P, P, P, P, P]),
     "luxgi_svgf_shadow_reproject": (C.c_int, [P, I, I, I, P, P, P, P, P, P, P, P, P]),
     "luxgi_svgf_shadow_atrous": (C.c_int, [P, I, I, I, P, P, P, P, P, P]),
+    "luxgi_svgf_learned_atrous": (C.c_int, [P, P, I, I, P, P, P, P, P]),
+    "luxgi_svgf_learned_atrous_backward": (C.c_int, [P, P, I, I, P, P, P, P, P, P, P, P]),
+    "luxgi_adam_step": (C.c_int, [P, P, P, P, I, C.c_float, C.c_float, C.c_float, C.c_float, I, P]),
     "luxgi_scene_bake_sdf": (C.c_int, [P, I, I, P, C.c_float, P, P]),
     "luxgi_scene_gbuffer": (C.c_int, [P, I, P, P, I, P, P, I, I, U, P, P, P, P, P, P, P]),
 }

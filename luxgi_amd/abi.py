@@ -204,11 +204,19 @@ class MeshSDF(C.Structure):
                 ("pad", C.c_int32 * 2)]
 
 
+class LearnedAtrousParams(C.Structure):  # include/luxgi.h LuxgiLearnedAtrousParams
+    _fields_ = [("radius", C.c_int32), ("stepSize", C.c_int32), ("phiColor", C.c_float), ("sigmaDepth", C.c_float),
+                ("level", C.c_float), ("pad", C.c_int32 * 3)]
+
+
+LEARNED_ATROUS_PARAMS = 81
+
 _SIZES = {DDGIUniform: 96, GlobalSDFData: 96, SurfaceAtlasData: 32, AtlasObject: 128, AtlasTile: 96,
           TraceParams: 112, ReflectionReprojectParams: 168, ReflectionAtrousParams: 24,
           ShadowReprojectParams: 80, ShadowAtrousParams: 32, SampleProbeParams: 80, Light: 64,
           SurfaceGBuffer: 40, SurfaceLightParams: 32, SurfaceDraw: 8, BlueNoise: 16, SdfShadowParams: 144,
-          SdfReflectionParams: 144, ObjectRasterizeData: 176, RasterizeParams: 48, MeshSDF: 72}
+          SdfReflectionParams: 144, ObjectRasterizeData: 176, RasterizeParams: 48, MeshSDF: 72,
+          LearnedAtrousParams: 32}
 for _t, _n in _SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
 

@@ -81,6 +81,8 @@ def _declare(L):
     L.oracle_blue_noise.restype = C.c_float
     for s_ in ("f64", "f32"):
         getattr(L, "oracle_sdf_rasterize_" + s_).argtypes = [P, P, P, P, I, P, P, P, I, I, I, P, I]
+    L.oracle_learned_atrous_forward.argtypes = [P, P, I, I, P, P, P, P, P, P]
+    L.oracle_learned_atrous_backward.argtypes = [P, P, I, I, P, P, P, P, P, P]
     L.oracle_round_to_half.argtypes = [C.c_float]
     L.oracle_round_to_half.restype = C.c_float
     L.oracle_sdf_reflection_f64.argtypes = [P, P, P, I, I, P, P, P, P, P, P, I, I, P, P, P, P, P, P, P, P, I, P, P, P,
@@ -456,3 +458,31 @@ def sdf_rasterize(params, objects, meshes, pool, num_objects, origins, start, ob
 
 def round_to_half(f):
     return lib().oracle_round_to_half(float(f))
+
+
+# ---------------------------------------------------------------------------
+# Learned a-trous pass
+# ---------------------------------------------------------------------------
+def learned_atrous_forward(params, weights, normals, pbr, image, color=None):
+    """One pass; `image` gives the weights' inputs, `color` (default: image) the blended values.
+    Returns (out [H][W][4] float64, sumW [H][W])."""
+    H, W = image.shape[:2]
+    out = np.zeros((H, W, 4))
+    sw = np.zeros((H, W))
+    wts = np.ascontiguousarray(weights, dtype=np.float64)
+    col = image if color is None else color
+    lib().oracle_learned_atrous_forward(C.byref(params), _p(wts), W, H, _p(_f32(normals)), _p(_f32(pbr)),
+                                        _p(_f32(image)), _p(_f32(col)), _p(out), _p(sw))
+    return out, sw
+
+
+def learned_atrous_backward(params, weights, normals, pbr, image, grad_out):
+    """Returns (dL/d image rgb [H][W][4], dL/d weights [81])."""
+    H, W = image.shape[:2]
+    gin = np.zeros((H, W, 4))
+    gp = np.zeros(81)
+    wts = np.ascontiguousarray(weights, dtype=np.float64)
+    g = np.ascontiguousarray(grad_out, dtype=np.float64)
+    lib().oracle_learned_atrous_backward(C.byref(params), _p(wts), W, H, _p(_f32(normals)), _p(_f32(pbr)),
+                                         _p(_f32(image)), _p(g), _p(gin), _p(gp))
+    return gin, gp

@@ -1606,6 +1606,154 @@ inline T rasterize_voxel(const LuxgiRasterizeParams& p, const LuxgiObjectRasteri
     return stored;
 }
 
+
+// ---------------------------------------------------------------------------
+// Learned a-trous pass (the reflection a-trous of Reflection/DenoiseAtrous.comp:63-134 with an MLP
+// tap weight; no counterpart in the reference) and its backward. `guide` gives the quantities the
+// weights are computed from (luminance, variance, mode), `color` the values that are blended: the
+// product path passes the same image for both; the finite-difference check perturbs `color` only.
+// ---------------------------------------------------------------------------
+struct LearnedAtrousImage {
+    int W, H;
+    const float *normals, *pbr;   // RGBA32F G-buffer images
+    const float *guide, *color;   // RGBA32F (rgb, variance)
+};
+
+template <class T> inline T la_lum(T r, T g, T b) { return std::max(r * T(0.299) + g * T(0.587) + b * T(0.114), T(0.0001)); }
+
+inline int la_mode_of(const LearnedAtrousImage& im, size_t q) {
+    const float z = im.pbr[q * 4 + 3], r = im.pbr[q * 4 + 1];
+    if (z == 1.0f) return 2;
+    return (r <= 0.05f || r >= 0.45f) ? 1 : 0;
+}
+
+template <class T>
+inline T la_center_var(const LearnedAtrousImage& im, int x, int y) {
+    const T kv[2][2] = {{T(0.25), T(0.125)}, {T(0.125), T(0.0625)}};
+    T v = 0;
+    for (int yy = -1; yy <= 1; ++yy)
+        for (int xx = -1; xx <= 1; ++xx) {
+            const int px = x + xx, py = y + yy;
+            if (px >= 0 && py >= 0 && px < im.W && py < im.H) v += T(im.guide[((size_t)py * im.W + px) * 4 + 3]) * kv[std::abs(xx)][std::abs(yy)];
+        }
+    return v;
+}
+
+// tap (xx, yy) of centre (x, y): the MLP input
+template <class T>
+inline void la_tap_features(const LearnedAtrousImage& im, const LuxgiLearnedAtrousParams& p, int x, int y, int xx,
+                            int yy, T f[6]) {
+    const size_t c = (size_t)y * im.W + x, q = (size_t)(y + yy * p.stepSize) * im.W + (x + xx * p.stepSize);
+    const T var = la_center_var<T>(im, x, y);
+    const T phi = T(p.phiColor) * std::sqrt(std::max(T(0), var)) + T(1e-10);
+    const T lc = la_lum<T>(T(im.guide[c * 4]), T(im.guide[c * 4 + 1]), T(im.guide[c * 4 + 2]));
+    const T lq = la_lum<T>(T(im.guide[q * 4]), T(im.guide[q * 4 + 1]), T(im.guide[q * 4 + 2]));
+    const v3<T> nc = octahedral_to_direction<T>(T(im.normals[c * 4]), T(im.normals[c * 4 + 1]));
+    const v3<T> nq = octahedral_to_direction<T>(T(im.normals[q * 4]), T(im.normals[q * 4 + 1]));
+    const T kw[3] = {T(1), T(2) / T(3), T(1) / T(6)};
+    f[0] = std::min(std::fabs(lc - lq) / phi, T(16));
+    f[1] = std::exp(-std::fabs(T(im.pbr[c * 4 + 3]) - T(im.pbr[q * 4 + 3])) / T(p.sigmaDepth));
+    f[2] = std::max(T(0), dot(nc, nq));
+    f[3] = kw[std::abs(xx)] * kw[std::abs(yy)];
+    f[4] = T(p.level);
+    f[5] = std::min(std::sqrt(std::max(var, T(0))), T(4));
+}
+
+// MLP forward: returns exp(clamp(o)), h the hidden activations, *clamped whether o was clamped
+template <class T>
+inline T la_mlp_eval(const T* P, const T f[6], T h[10], bool* clamped) {
+    const T* W1 = P;
+    const T* b1 = P + 60;
+    const T* w2 = P + 70;
+    T o = P[80];
+    for (int j = 0; j < 10; ++j) {
+        T a = b1[j];
+        for (int i = 0; i < 6; ++i) a += W1[j * 6 + i] * f[i];
+        h[j] = std::max(a, T(0));
+        o += w2[j] * h[j];
+    }
+    *clamped = o < T(-16) || o > T(8);
+    return std::exp(clampT(o, T(-16), T(8)));
+}
+
+template <class T>
+inline bool la_tap_inside(const LearnedAtrousImage& im, const LuxgiLearnedAtrousParams& p, int x, int y, int xx, int yy) {
+    const int px = x + xx * p.stepSize, py = y + yy * p.stepSize;
+    return px >= 0 && py >= 0 && px < im.W && py < im.H && !(xx == 0 && yy == 0);
+}
+
+// forward of one pixel: out[4], *sumW
+template <class T>
+inline void la_forward_pixel(const LearnedAtrousImage& im, const LuxgiLearnedAtrousParams& p, const T* P, int x, int y,
+                             T out[4], T* sumW) {
+    const size_t e = (size_t)y * im.W + x;
+    const int mode = la_mode_of(im, e);
+    *sumW = T(1);
+    if (mode == 2) { out[0] = out[1] = out[2] = out[3] = 0; return; }
+    for (int k = 0; k < 4; ++k) out[k] = T(im.color[e * 4 + k]);
+    if (mode == 1) return;
+    T s[4] = {out[0], out[1], out[2], out[3]}, sw = 1;
+    for (int yy = -p.radius; yy <= p.radius; ++yy)
+        for (int xx = -p.radius; xx <= p.radius; ++xx) {
+            if (!la_tap_inside<T>(im, p, x, y, xx, yy)) continue;
+            T f[6], h[10];
+            bool cl;
+            la_tap_features<T>(im, p, x, y, xx, yy, f);
+            const T w = la_mlp_eval<T>(P, f, h, &cl);
+            const size_t q = (size_t)(y + yy * p.stepSize) * im.W + (x + xx * p.stepSize);
+            sw += w;
+            for (int k = 0; k < 3; ++k) s[k] += w * T(im.color[q * 4 + k]);
+            s[3] += w * w * T(im.color[q * 4 + 3]);
+        }
+    for (int k = 0; k < 3; ++k) out[k] = s[k] / sw;
+    out[3] = s[3] / (sw * sw);
+    *sumW = sw;
+}
+
+// backward over the image (scatter form): gIn [H][W][4] (rgb), gP [81] accumulated
+template <class T>
+inline void la_backward(const LearnedAtrousImage& im, const LuxgiLearnedAtrousParams& p, const T* P, const T* gOut,
+                        T* gIn, T* gP) {
+    for (size_t i = 0; i < (size_t)im.W * im.H * 4; ++i) gIn[i] = 0;
+    for (int y = 0; y < im.H; ++y)
+        for (int x = 0; x < im.W; ++x) {
+            const size_t e = (size_t)y * im.W + x;
+            const int mode = la_mode_of(im, e);
+            if (mode == 2) continue;
+            if (mode == 1) {
+                for (int k = 0; k < 3; ++k) gIn[e * 4 + k] += gOut[e * 4 + k];
+                continue;
+            }
+            T out[4], sw;
+            la_forward_pixel<T>(im, p, P, x, y, out, &sw);
+            for (int k = 0; k < 3; ++k) gIn[e * 4 + k] += gOut[e * 4 + k] / sw;
+            for (int yy = -p.radius; yy <= p.radius; ++yy)
+                for (int xx = -p.radius; xx <= p.radius; ++xx) {
+                    if (!la_tap_inside<T>(im, p, x, y, xx, yy)) continue;
+                    T f[6], h[10];
+                    bool cl;
+                    la_tap_features<T>(im, p, x, y, xx, yy, f);
+                    const T w = la_mlp_eval<T>(P, f, h, &cl);
+                    const size_t q = (size_t)(y + yy * p.stepSize) * im.W + (x + xx * p.stepSize);
+                    T dw = 0;
+                    for (int k = 0; k < 3; ++k) {
+                        gIn[q * 4 + k] += gOut[e * 4 + k] * w / sw;
+                        dw += gOut[e * 4 + k] * (T(im.color[q * 4 + k]) - out[k]);
+                    }
+                    dw /= sw;
+                    if (cl) continue;
+                    const T dO = dw * w;
+                    gP[80] += dO;
+                    for (int j = 0; j < 10; ++j) {
+                        gP[70 + j] += dO * h[j];
+                        const T dh = h[j] > T(0) ? dO * P[70 + j] : T(0);
+                        gP[60 + j] += dh;
+                        for (int i = 0; i < 6; ++i) gP[j * 6 + i] += dh * f[i];
+                    }
+                }
+        }
+}
+
 template <class F> inline void parallel_for(int n, int threads, F f) {
     if (threads <= 1 || n < 2) {
         for (int i = 0; i < n; ++i) f(i);

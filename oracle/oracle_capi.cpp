@@ -474,3 +474,21 @@ extern "C" void oracle_sdf_rasterize_f32(const LuxgiRasterizeParams* p, const Lu
                               out, threads);
 }
 extern "C" float oracle_round_to_half(float f) { return round_to_half(f); }
+
+// Learned a-trous pass (FP64): forward over the image and the backward.
+extern "C" void oracle_learned_atrous_forward(const LuxgiLearnedAtrousParams* p, const double* P, int W, int H,
+                                              const float* normals, const float* pbr, const float* guide,
+                                              const float* color, double* out, double* sumW) {
+    LearnedAtrousImage im{W, H, normals, pbr, guide, color};
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t e = (size_t)y * W + x;
+            la_forward_pixel<double>(im, *p, P, x, y, out + e * 4, sumW + e);
+        }
+}
+extern "C" void oracle_learned_atrous_backward(const LuxgiLearnedAtrousParams* p, const double* P, int W, int H,
+                                               const float* normals, const float* pbr, const float* image,
+                                               const double* gOut, double* gIn, double* gP) {
+    LearnedAtrousImage im{W, H, normals, pbr, image, image};
+    la_backward<double>(im, *p, P, gOut, gIn, gP);
+}

@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision form")
     ap.add_argument("--no-surface", action="store_true", help="skip the surface-atlas passes")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched SVGF measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, launch a plain copy kernel (256 MiB read + 256 MiB written, 3 times) "
                          "whose rocprofv3 FETCH_SIZE / WRITE_SIZE scale the trace counters to bytes")
@@ -66,6 +67,84 @@ def log(msg, quiet=False):
     if not quiet:
         print(msg, file=sys.stderr, flush=True)
 
+
+
+def bench_svgf_batched(args, prims_dev, n_prims, sdf_data, sdf, res, q):
+    """Per-frame vs batched vs graph-replayed reflection denoise on two batches of uneven frames."""
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from luxgi_amd import scene
+    from luxgi_amd.autotune import Autotuner, tune_svgf_batch
+    from luxgi_amd.batch import ReflectionDenoiseBatch
+    from luxgi_amd.svgf import ReflectionDenoiser, make_gbuffer
+    workloads = {
+        "16 frames 1920x1080..640x360": [(1920, 1080), (1280, 720), (960, 540), (640, 360)] * 4,
+        "64 frames 480x270..160x90": [(480, 270), (320, 180), (240, 135), (160, 90)] * 16,
+    }
+    out = {}
+    tuner = Autotuner(path=os.path.join(tempfile.mkdtemp(), "autotune.json"))
+    for name, sizes in workloads.items():
+        frames = []
+        for k, (w, h) in enumerate(sizes):
+            VP, VPp, eye, delta = scene.camera_pair(w, h, seed=4 + k, eye=(0.0, 0.3, 2.4))
+            VPI = np.linalg.inv(VP)
+            cur, noisy, _ = make_gbuffer(prims_dev, n_prims, sdf_data, sdf, res, VPI, VPp, w, h, seed=1 + k,
+                                         with_mask=False)
+            prev, _, _ = make_gbuffer(prims_dev, n_prims, sdf_data, sdf, res, np.linalg.inv(VPp), VPp, w, h,
+                                      seed=100 + k, with_mask=False)
+            frames.append((cur, prev, noisy, (VPI, VPp, eye, delta)))
+        singles = []
+        for (w, h), (cur, prev, noisy, cam) in zip(sizes, frames):
+            d = ReflectionDenoiser(w, h, iterations=4)
+            d.set_camera(*cam)
+            singles.append(d)
+        batch = ReflectionDenoiseBatch(sizes, iterations=4)
+        graphed = ReflectionDenoiseBatch(sizes, iterations=4)
+        for b in (batch, graphed):
+            for k, (cur, prev, noisy, cam) in enumerate(frames):
+                b.set_frame(k, cur, prev, noisy, *cam)
+        choice = tune_svgf_batch(batch, tuner)
+        graphed.grid_blocks, graphed.order = batch.grid_blocks, batch.order
+        graphed.capture()
+
+        def per_frame():
+            for d, (cur, prev, noisy, _) in zip(singles, frames):
+                d.denoise(cur, prev, noisy)
+
+        def timed(fn, reps):
+            for _ in range(2):
+                fn()
+            torch.cuda.synchronize()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            t0 = time.perf_counter()
+            ev[0].record()
+            for _ in range(reps):
+                fn()
+            ev[1].record()
+            torch.cuda.synchronize()
+            return ev[0].elapsed_time(ev[1]) / reps, (time.perf_counter() - t0) * 1e3 / reps
+
+        reps = max(args.steps, 5)
+        t_single = timed(per_frame, reps)
+        t_batch = timed(batch.denoise, reps)
+        t_graph = timed(graphed.denoise, reps)
+        # the batched outputs equal the per-frame ones (same inputs, same state sequence)
+        out[name] = {"frames": len(sizes), "pixels": int(sum(w * h for w, h in sizes)),
+                     "per_frame_calls_ms": t_single[0], "batched_ms": t_batch[0], "graph_replay_ms": t_graph[0],
+                     "per_frame_calls_wall_ms": t_single[1], "batched_wall_ms": t_batch[1],
+                     "graph_replay_wall_ms": t_graph[1],
+                     "launches": {"per_frame": 5 * len(sizes), "batched": 5, "graph": 1},
+                     "speedup_batched_vs_per_frame": t_single[0] / t_batch[0],
+                     "speedup_graph_vs_per_frame": t_single[0] / t_graph[0],
+                     "schedule": {"gridBlocks": choice[0], "order": choice[1],
+                                  "candidates_ms": tuner.last_timings or None}}
+        log(f"[bench] svgf batched {name}: per-frame {t_single[0]:.3f} ms, batched {t_batch[0]:.3f} ms, "
+            f"graph {t_graph[0]:.3f} ms (schedule {choice})", q)
+        del singles, batch, graphed, frames
+    return out
 
 def main():
     args = parse()
@@ -286,6 +365,11 @@ def main():
     log(f"[bench] global SDF rasterize {build_ms:.3f} ms", q)
     del builder, built
 
+    # ---------------- batched SVGF over frames of uneven sizes (reported beside the step) ----------------
+    # the same reflection denoise (reprojection + 4 a-trous) for many frames at once: per-frame calls
+    # (1 + 4 launches each) against the batched persistent launches (5 in all) and their graph replay;
+    # the persistent grid and unit order chosen by the autotuner
+    svgf_batched = None if args.no_batch else bench_svgf_batched(args, prims_dev, len(prims), g, sdf, res, q)
     # ---------------- reduced-precision form (reported beside the step, not in value) ----------------
     # FP16 SDF bricks with per-brick scale and FP32 re-evaluation at hits, FP16-MFMA blend (one
     # rounding per operand, FP32 accumulation), SVGF with RGBA16F intermediates. Ray buffers stay
@@ -387,6 +471,7 @@ def main():
         "sdf_shadow_mask_ms": shadow_ms,
         "sdf_reflection_ms": refl_ms,
         "sdf_build": sdf_build,
+        "svgf_batched": svgf_batched,
         "trace_grays_per_s": rays_per_frame / trace_s / 1e9,
         "trace_counts": {"lookups": n_lookup, "normals": n_normal, "objects": c[3], "tile_gathers": n_tile,
                          "rays": n_rays, "steps": c[6], "hits": c[7]},

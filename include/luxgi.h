@@ -698,6 +698,53 @@ int luxgi_sdf_reflection(const LuxgiSdfReflectionParams* params, const LuxgiBlue
                          void* workspace, size_t workspaceBytes, void* stream);
 
 /* ------------------------------------------------------------------------- */
+/* Batched reflection denoise (no counterpart in the reference)               */
+/* ------------------------------------------------------------------------- */
+
+/* Largest batch: frames per luxgi_svgf_reflection_denoise_batched call. */
+#define LUXGI_MAX_DENOISE_BATCH 1024
+
+/* One frame of a batch: the arguments of luxgi_svgf_reflection_denoise for it. The array of items
+ * lives in device memory, so a captured hipGraph replays with whatever the caller copied there
+ * before the replay (new cameras, noisy inputs, the flipped pingPong, even other sizes). */
+typedef struct LuxgiReflectionDenoiseItem {
+    LuxgiReflectionReprojectParams reproject;  /* 168 B */
+    LuxgiReflectionAtrousParams filter;        /* a-trous parameters; stepSize is set per iteration */
+    int32_t width, height;
+    int32_t pingPong;                          /* moments[pingPong] written, moments[1 - pingPong] read */
+    int32_t blockStart;                        /* first 16x16 work unit; set by luxgi_svgf_batch_prepare */
+    LuxgiGBuffer cur, prev;
+    const void* noisy;
+    void* outputs[2];                          /* accumulation target, history colour */
+    void* moments[2];
+    void* atrous[2];                           /* the a-trous ping-pong pair */
+    uint8_t* tileMap;                          /* luxgi_svgf_tile_count(width, height) bytes */
+} LuxgiReflectionDenoiseItem;                  /* 336 B */
+
+typedef struct LuxgiDenoiseBatchHeader {
+    int32_t count;        /* items */
+    int32_t totalBlocks;  /* 16x16 work units over all items */
+    int32_t pad[2];
+} LuxgiDenoiseBatchHeader;
+
+/* Host-side: checks `count` host items (sizes, radius, pointers), sets each blockStart and fills
+ * the header. The caller then copies items and header to device memory. No GPU call. */
+int luxgi_svgf_batch_prepare(LuxgiReflectionDenoiseItem* items, int32_t count, LuxgiDenoiseBatchHeader* header);
+
+/* denoise::system (RaytracedReflection.cpp:496-520) for every item of a batch of frames of any
+ * sizes: the reprojection of all frames in one persistent launch, then each a-trous iteration of
+ * all frames in one persistent launch (1 + iterations launches in all, whatever the count). Each
+ * item's outputs are identical to luxgi_svgf_reflection_denoise on it. header and items are device
+ * pointers (prepared as above); gridBlocks (> 0) is the grid and `order` how blocks walk the work
+ * units: 0 persistent and strided (block b takes b, b + grid, ...), 1 persistent with contiguous
+ * runs, 2 one unit per block (no loop; gridBlocks must be at least the header's totalBlocks when
+ * the launch runs, units beyond the grid are not processed). *result as for
+ * luxgi_svgf_reflection_denoise (the same for every item). */
+int luxgi_svgf_reflection_denoise_batched(const LuxgiDenoiseBatchHeader* header, const LuxgiReflectionDenoiseItem* items,
+                                          int32_t iterations, int32_t storage, int32_t gridBlocks, int32_t order,
+                                          int32_t* result, void* stream);
+
+/* ------------------------------------------------------------------------- */
 /* Learned a-trous edge-stopping weights (no counterpart in the reference)     */
 /* ------------------------------------------------------------------------- */
 

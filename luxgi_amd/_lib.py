@@ -44,6 +44,8 @@ SIGNATURES = {
     "luxgi_svgf_reflection_denoise": (C.c_int, [P, P, I, I, I, I, P, P, P, P, P, P, P, I, P, P, P, P, P]),
     "luxgi_svgf_shadow_reproject": (C.c_int, [P, I, I, I, P, P, P, P, P, P, P, P, P]),
     "luxgi_svgf_shadow_atrous": (C.c_int, [P, I, I, I, P, P, P, P, P, P]),
+    "luxgi_svgf_batch_prepare": (C.c_int, [P, I, P]),
+    "luxgi_svgf_reflection_denoise_batched": (C.c_int, [P, P, I, I, I, I, P, P]),
     "luxgi_svgf_learned_atrous": (C.c_int, [P, P, I, I, P, P, P, P, P]),
     "luxgi_svgf_learned_atrous_backward": (C.c_int, [P, P, I, I, P, P, P, P, P, P, P, P]),
     "luxgi_adam_step": (C.c_int, [P, P, P, P, I, C.c_float, C.c_float, C.c_float, C.c_float, I, P]),

@@ -211,12 +211,25 @@ class LearnedAtrousParams(C.Structure):  # include/luxgi.h LuxgiLearnedAtrousPar
 
 LEARNED_ATROUS_PARAMS = 81
 
+MAX_DENOISE_BATCH = 1024
+
+
+class ReflectionDenoiseItem(C.Structure):  # include/luxgi.h LuxgiReflectionDenoiseItem
+    _fields_ = [("reproject", ReflectionReprojectParams), ("filter", ReflectionAtrousParams), ("width", C.c_int32),
+                ("height", C.c_int32), ("pingPong", C.c_int32), ("blockStart", C.c_int32), ("cur", GBuffer),
+                ("prev", GBuffer), ("noisy", C.c_void_p), ("outputs", C.c_void_p * 2), ("moments", C.c_void_p * 2),
+                ("atrous", C.c_void_p * 2), ("tileMap", C.c_void_p)]
+
+
+class DenoiseBatchHeader(C.Structure):
+    _fields_ = [("count", C.c_int32), ("totalBlocks", C.c_int32), ("pad", C.c_int32 * 2)]
+
 _SIZES = {DDGIUniform: 96, GlobalSDFData: 96, SurfaceAtlasData: 32, AtlasObject: 128, AtlasTile: 96,
           TraceParams: 112, ReflectionReprojectParams: 168, ReflectionAtrousParams: 24,
           ShadowReprojectParams: 80, ShadowAtrousParams: 32, SampleProbeParams: 80, Light: 64,
           SurfaceGBuffer: 40, SurfaceLightParams: 32, SurfaceDraw: 8, BlueNoise: 16, SdfShadowParams: 144,
           SdfReflectionParams: 144, ObjectRasterizeData: 176, RasterizeParams: 48, MeshSDF: 72,
-          LearnedAtrousParams: 32}
+          LearnedAtrousParams: 32, ReflectionDenoiseItem: 336, DenoiseBatchHeader: 16}
 for _t, _n in _SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
 

@@ -1,0 +1,175 @@
+"""Batched reflection denoise over frames of uneven sizes, and its graph capture (libluxgi.so).
+
+ReflectionDenoiseBatch runs raytraced_reflection::denoise::system (Engine/Raytrace/
+RaytracedReflection.cpp:496-520) for many frames at once through
+luxgi_svgf_reflection_denoise_batched: the reprojection of every frame in one persistent launch,
+then each a-trous iteration of every frame in one more. The frames' descriptors
+(LuxgiReflectionDenoiseItem) live in device memory and the kernels read them at run time, so the
+1 + iterations launches are captured once (capture()) and replayed every frame after the
+descriptors are refreshed by one host-to-device copy from pinned memory. Each frame's outputs are
+identical to luxgi_svgf_reflection_denoise on that frame.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import abi
+from ._lib import check, lib
+from .ddgi import stream_handle
+
+
+def default_grid_blocks():
+    """Four persistent blocks per compute unit."""
+    return 4 * torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+
+
+class ReflectionDenoiseBatch:
+    """sizes: list of (width, height), one per frame. Per-frame state (accumulation targets,
+    moments, a-trous pair, tile map) is allocated here; the G-buffers and noisy inputs are the
+    caller's, set per frame with set_frame(). Schedule: order 2 (default) launches one block per
+    16x16 unit (the grid follows the unit total); orders 0 / 1 run a persistent grid of
+    grid_blocks blocks (default four per compute unit)."""
+
+    def __init__(self, sizes, storage=abi.STORAGE_FP32, iterations=4, alpha=0.01, moments_alpha=0.2, radius=1,
+                 phi_color=10.0, phi_normal=32.0, sigma_depth=1.0, grid_blocks=None, order=2):
+        n = len(sizes)
+        if not 0 < n <= abi.MAX_DENOISE_BATCH:
+            raise ValueError(f"1 .. {abi.MAX_DENOISE_BATCH} frames per batch")
+        self.n = n
+        self.sizes = [(int(w), int(h)) for w, h in sizes]
+        self.storage = storage
+        self.iterations = int(iterations)
+        self.grid_blocks = int(grid_blocks or default_grid_blocks())
+        self.order = int(order)
+        dt = torch.float16 if storage == abi.STORAGE_FP16 else torch.float32
+        self.items = (abi.ReflectionDenoiseItem * n)()
+        self.header = abi.DenoiseBatchHeader()
+        self.outputs, self.moments, self.atrous, self.tile_maps = [], [], [], []
+        self._inputs = [None] * n
+        for k, (w, h) in enumerate(self.sizes):
+            z = lambda: torch.zeros((h, w, 4), dtype=dt, device="cuda")  # noqa: E731
+            self.outputs.append([z(), z()])
+            self.moments.append([z(), z()])
+            self.atrous.append([torch.ones((h, w, 4), dtype=dt, device="cuda") for _ in range(2)])
+            self.tile_maps.append(torch.zeros(max(lib().luxgi_svgf_tile_count(w, h), 1), dtype=torch.uint8,
+                                              device="cuda"))
+            it = self.items[k]
+            it.width, it.height = w, h
+            it.reproject.alpha = alpha
+            it.reproject.momentsAlpha = moments_alpha
+            it.reproject.approximateWithDDGI = 1
+            it.filter = abi.ReflectionAtrousParams(radius, 0, phi_color, phi_normal, sigma_depth, 1)
+            for j in range(2):
+                it.outputs[j] = self.outputs[k][j].data_ptr()
+                it.moments[j] = self.moments[k][j].data_ptr()
+                it.atrous[j] = self.atrous[k][j].data_ptr()
+            it.tileMap = self.tile_maps[k].data_ptr()
+        nbytes = C.sizeof(self.items) + C.sizeof(self.header)
+        self._hdr_off = C.sizeof(self.items)
+        self.device = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        self._staging = [torch.empty(nbytes, dtype=torch.uint8, pin_memory=True) for _ in range(2)]
+        self._staged = [None, None]  # event recorded after each staging buffer's copy
+        self._next = 0
+        self.ping_pong = 0
+        self.graph = None
+        self.result_index = -1 if self.iterations == 0 else (1 if self.iterations & 1 else 0)
+
+    def set_frame(self, k, cur, prev, noisy, view_proj_inv, prev_view_proj, camera_pos, camera_delta):
+        """Frame k's inputs: G-buffer dicts of device tensors (color, normals, pbr, depth), the noisy
+        signal, and the camera of ReflectionDenoiser.set_camera."""
+        w, h = self.sizes[k]
+        if noisy.shape[:2] != (h, w) or cur["depth"].shape != (h, w) or prev["depth"].shape != (h, w):
+            raise ValueError(f"frame {k}: inputs must be {h}x{w}")
+        it = self.items[k]
+        it.reproject.viewProjInv = abi.mat4_colmajor(view_proj_inv)
+        it.reproject.prevViewProj = abi.mat4_colmajor(prev_view_proj)
+        it.reproject.cameraPos[:] = [float(camera_pos[0]), float(camera_pos[1]), float(camera_pos[2]), 1.0]
+        it.reproject.cameraDelta[:] = [float(v) for v in camera_delta]
+        it.cur = abi.GBuffer(cur["color"].data_ptr(), cur["normals"].data_ptr(), cur["pbr"].data_ptr(),
+                             cur["depth"].data_ptr())
+        it.prev = abi.GBuffer(prev["color"].data_ptr(), prev["normals"].data_ptr(), prev["pbr"].data_ptr(),
+                              prev["depth"].data_ptr())
+        it.noisy = noisy.data_ptr()
+        self._inputs[k] = (cur, prev, noisy)  # keep the caller's tensors alive while they are referenced
+
+    def upload(self):
+        """Checks the descriptors, sets blockStart and the header (luxgi_svgf_batch_prepare) and copies
+        them to device memory on the current stream, through a pinned double buffer."""
+        if any(x is None for x in self._inputs):
+            raise RuntimeError("set_frame() every frame before the first upload")
+        for k in range(self.n):
+            self.items[k].pingPong = self.ping_pong
+        check(lib().luxgi_svgf_batch_prepare(C.byref(self.items), self.n, C.byref(self.header)),
+              "luxgi_svgf_batch_prepare")
+        j = self._next
+        self._next = 1 - j
+        if self._staged[j] is not None:
+            self._staged[j].synchronize()  # the copy that last read this staging buffer is done
+        host = self._staging[j].numpy()
+        host[:self._hdr_off] = np.frombuffer(memoryview(self.items).cast("B"), dtype=np.uint8)
+        host[self._hdr_off:] = np.frombuffer(memoryview(self.header).cast("B"), dtype=np.uint8)
+        self.device.copy_(self._staging[j], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        self._staged[j] = ev
+
+    def _grid(self):
+        return max(int(self.header.totalBlocks), 1) if self.order == 2 else self.grid_blocks
+
+    def launch(self):
+        """The 1 + iterations launches over the uploaded descriptors."""
+        res = C.c_int32(-2)
+        base = self.device.data_ptr()
+        check(lib().luxgi_svgf_reflection_denoise_batched(C.c_void_p(base + self._hdr_off), C.c_void_p(base),
+                                                          self.iterations, self.storage, self._grid(), self.order,
+                                                          C.byref(res), stream_handle()),
+              "luxgi_svgf_reflection_denoise_batched")
+        self.result_index = res.value
+
+    def capture(self):
+        """Captures launch() into a CUDA/HIP graph (after one eager warm-up run, whose effect on the
+        temporal state is undone); denoise() then replays it."""
+        self.upload()
+        saved = self.state()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            self.launch()  # warm-up outside the capture, as torch asks
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.restore(saved)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.launch()
+        self.graph = g
+        self._graph_grid = self._grid()
+
+    def denoise(self):
+        """One frame of every item (after set_frame): upload, launch (or graph replay), flip the
+        moments ping-pong. Returns the output tensor of each frame."""
+        self.upload()
+        if self.graph is not None and self.order == 2 and self._grid() > self._graph_grid:
+            self.capture()  # the frames grew past the captured one-unit-per-block grid
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self.launch()
+        self.ping_pong = 1 - self.ping_pong
+        return self.results()
+
+    def results(self):
+        r = self.result_index
+        return [self.outputs[k][0] if r < 0 else self.atrous[k][r] for k in range(self.n)]
+
+    def state(self):
+        """The temporal state (for a caller that must run extra launches without disturbing it)."""
+        return [t.clone() for k in range(self.n) for t in (*self.outputs[k], *self.moments[k])]
+
+    def restore(self, saved):
+        i = 0
+        for k in range(self.n):
+            for t in (*self.outputs[k], *self.moments[k]):
+                t.copy_(saved[i])
+                i += 1

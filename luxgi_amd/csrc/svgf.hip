@@ -71,10 +71,15 @@ struct ReflArgs {
     LuxgiReflectionReprojectParams pc;
     int W, H, tilesX;
 };
+// the same with the parameters left where they are (the batched kernels' descriptors)
+struct ReflArgsRef {
+    const LuxgiReflectionReprojectParams& pc;
+    int W, H, tilesX;
+};
 
 // reproject, RGB overload (Reprojection.glsl:242-405)
-template <class S>
-__device__ bool reproject_rgb(const ReflArgs& A, int cx, int cy, float depth, float rayLength, const GBuf& cur,
+template <class S, class RA>
+__device__ bool reproject_rgb(const RA& A, int cx, int cy, float depth, float rayLength, const GBuf& cur,
                               const GBuf& prev, const S* histOut, const S* histMom, f3& hc, float2& hm, float& hl) {
     const int W = A.W, H = A.H;
     const float sx = float(W), sy = float(H);
@@ -178,18 +183,27 @@ __device__ bool reproject_rgb(const ReflArgs& A, int cx, int cy, float depth, fl
     return valid;
 }
 
-// 16x16 pixels per block; LDS halo of 8 for the 17x17 statistics.
-template <class S>
-__global__ __launch_bounds__(256) void reflection_reproject_kernel(ReflArgs A, GBuf cur, GBuf prev,
-                                                                   const S* __restrict__ noisy,
-                                                                   const S* __restrict__ histOut,
-                                                                   const S* __restrict__ histMom, S* __restrict__ outColor,
-                                                                   S* __restrict__ outMom, uint8_t* __restrict__ tileMap) {
-    __shared__ float3 tile[32][33];
-    __shared__ float3 h1[32][16], h2[32][16];
-    __shared__ int vote[4];
+// LDS of one reprojection block
+struct ReprojectShared {
+    float3 tile[32][33];
+    float3 h1[32][16], h2[32][16];
+    int vote[4];
+};
+
+// One 16x16-pixel block (block coordinates bX, bY) of the reprojection; LDS halo of 8 for the
+// 17x17 statistics. Every lane of the block calls it.
+template <class S, class RA>
+__device__ __forceinline__ void reproject_block(const RA& A, const GBuf& cur, const GBuf& prev,
+                                                const S* __restrict__ noisy, const S* __restrict__ histOut,
+                                                const S* __restrict__ histMom, S* __restrict__ outColor,
+                                                S* __restrict__ outMom, uint8_t* __restrict__ tileMap, int bX, int bY,
+                                                ReprojectShared& sm) {
+    auto& tile = sm.tile;
+    auto& h1 = sm.h1;
+    auto& h2 = sm.h2;
+    auto& vote = sm.vote;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int bx = blockIdx.x * 16, by = blockIdx.y * 16;
+    const int bx = bX * 16, by = bY * 16;
     const int W = A.W, H = A.H;
     if (threadIdx.x < 4) vote[threadIdx.x] = 0;
     for (int i = threadIdx.x; i < 32 * 32; i += 256) {
@@ -225,7 +239,7 @@ __global__ __launch_bounds__(256) void reflection_reproject_kernel(ReflArgs A, G
             f3 hc;
             float2 hm;
             float hl;
-            const bool ok = reproject_rgb<S>(A, x, y, depth, cr.w, cur, prev, histOut, histMom, hc, hm, hl);
+            const bool ok = reproject_rgb<S, RA>(A, x, y, depth, cr.w, cur, prev, histOut, histMom, hc, hm, hl);
             hl = fminf(32.0f, ok ? hl + 1.0f : 1.0f);
             if (ok) {  // neighborhoodStandardDeviation + clipAABB (Reflection/DenoiseReprojection.comp:78-114)
                 const float inv = 1.0f / 289.0f;
@@ -256,9 +270,19 @@ __global__ __launch_bounds__(256) void reflection_reproject_kernel(ReflArgs A, G
     }
     __syncthreads();
     if (threadIdx.x < 4) {
-        const int tX = blockIdx.x * 2 + (threadIdx.x & 1), tY = blockIdx.y * 2 + (threadIdx.x >> 1);
+        const int tX = bX * 2 + (threadIdx.x & 1), tY = bY * 2 + (threadIdx.x >> 1);
         if (tX * 8 < W && tY * 8 < H) tileMap[tY * A.tilesX + tX] = (uint8_t)vote[threadIdx.x];
     }
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void reflection_reproject_kernel(ReflArgs A, GBuf cur, GBuf prev,
+                                                                   const S* __restrict__ noisy,
+                                                                   const S* __restrict__ histOut,
+                                                                   const S* __restrict__ histMom, S* __restrict__ outColor,
+                                                                   S* __restrict__ outMom, uint8_t* __restrict__ tileMap) {
+    __shared__ ReprojectShared sm;
+    reproject_block<S>(A, cur, prev, noisy, histOut, histMom, outColor, outMom, tileMap, blockIdx.x, blockIdx.y, sm);
 }
 
 // computeEdgeStoppingWeight (EdgeStop.glsl:24-40) with the per-pixel constants folded:
@@ -286,7 +310,7 @@ struct AtrousArgs {
 };
 
 // squarings that evaluate x^phiNormal exactly as repeated products, or -1
-static inline int normal_squarings(float phiN) {
+__host__ __device__ inline int normal_squarings(float phiN) {
     for (int k = 0; k <= 6; ++k)
         if (phiN == float(1 << k)) return k;
     return -1;
@@ -340,18 +364,15 @@ __device__ __forceinline__ float load_roughness(const float4* pbr, size_t q) {
 // Direct form, one 8x8 tile per 64-lane block: every tap read from global memory. Used when the
 // footprint radius * stepSize exceeds the LDS halo.
 template <class S>
-__global__ __launch_bounds__(64) void reflection_atrous_kernel(AtrousArgs A, const float4* __restrict__ normals,
-                                                               const float4* __restrict__ pbr,
-                                                               const uint8_t* __restrict__ tileMap,
-                                                               const S* __restrict__ in, S* __restrict__ out,
-                                                               S* __restrict__ hist) {
-    const int x = blockIdx.x * 8 + (threadIdx.x & 7), y = blockIdx.y * 8 + (threadIdx.x >> 3);
+__device__ __forceinline__ void atrous_direct_pixel(const AtrousArgs& A, const float4* __restrict__ normals,
+                                                    const float4* __restrict__ pbr, const uint8_t* __restrict__ tileMap,
+                                                    const S* __restrict__ in, S* __restrict__ out, S* __restrict__ hist,
+                                                    int x, int y) {
     const int W = A.W, H = A.H;
-    if (x >= W || y >= H) return;
     const size_t e = (size_t)y * W + x;
     const float4 c = Texel<S>::ld4(in, e);
     float4 o = c;
-    if (tileMap[blockIdx.y * A.tilesX + blockIdx.x]) {
+    if (tileMap[(y >> 3) * A.tilesX + (x >> 3)]) {
         const float kv[2][2] = {{1.0f / 4.0f, 1.0f / 8.0f}, {1.0f / 8.0f, 1.0f / 16.0f}};
         float var = 0.0f;  // computeVarianceCenter (:41-61)
         for (int yy = -1; yy <= 1; ++yy)
@@ -373,6 +394,17 @@ __global__ __launch_bounds__(64) void reflection_atrous_kernel(AtrousArgs A, con
     if (hist) Texel<S>::st4(hist, e, o);
 }
 
+template <class S>
+__global__ __launch_bounds__(64) void reflection_atrous_kernel(AtrousArgs A, const float4* __restrict__ normals,
+                                                               const float4* __restrict__ pbr,
+                                                               const uint8_t* __restrict__ tileMap,
+                                                               const S* __restrict__ in, S* __restrict__ out,
+                                                               S* __restrict__ hist) {
+    const int x = blockIdx.x * 8 + (threadIdx.x & 7), y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    if (x >= A.W || y >= A.H) return;
+    atrous_direct_pixel<S>(A, normals, pbr, tileMap, in, out, hist, x, y);
+}
+
 // LDS-tiled form: a 16x16-pixel block (2x2 tiles) stages its input and guides (decoded normal,
 // linear depth) with a halo of max(radius * stepSize, 1) pixels once, so each tap is an LDS read
 // and each normal is decoded once per block instead of once per tap.
@@ -385,12 +417,12 @@ struct AtrousRegion {
 };
 
 // Block-uniform: does any of the block's (up to) 2x2 tiles need filtering?
-__device__ __forceinline__ bool block_has_filter_tile(const AtrousArgs& A, const uint8_t* tileMap) {
+__device__ __forceinline__ bool block_has_filter_tile(const AtrousArgs& A, const uint8_t* tileMap, int bX, int bY) {
     const int tilesY = (A.H + 7) / 8;
     bool any = false;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const int tX = blockIdx.x * 2 + (i & 1), tY = blockIdx.y * 2 + (i >> 1);
+        const int tX = bX * 2 + (i & 1), tY = bY * 2 + (i >> 1);
         if (tX < A.tilesX && tY < tilesY && tileMap[tY * A.tilesX + tX]) any = true;
     }
     return any;
@@ -398,12 +430,12 @@ __device__ __forceinline__ bool block_has_filter_tile(const AtrousArgs& A, const
 
 template <class S, class V, class Load>
 __device__ __forceinline__ AtrousRegion stage_region(const AtrousArgs& A, const float4* normals, const float4* pbr,
-                                                     V* sIn, float4* sGuide, Load load) {
+                                                     V* sIn, float4* sGuide, Load load, int bX, int bY) {
     AtrousRegion r;
     r.h = max(A.radius * A.stepSize, 1);
     r.side = kAtrousBlock + 2 * r.h;
-    r.ox = blockIdx.x * kAtrousBlock - r.h;
-    r.oy = blockIdx.y * kAtrousBlock - r.h;
+    r.ox = bX * kAtrousBlock - r.h;
+    r.oy = bY * kAtrousBlock - r.h;
     for (int i = threadIdx.x; i < r.side * r.side; i += 256) {
         const int ly = i / r.side, lx = i - ly * r.side;
         const int gx = r.ox + lx, gy = r.oy + ly;
@@ -422,20 +454,19 @@ __device__ __forceinline__ AtrousRegion stage_region(const AtrousArgs& A, const 
     return r;
 }
 
+// One 16x16-pixel block (block coordinates bX, bY) of the LDS-tiled form; every lane of the block
+// calls it (the staging has a barrier).
 template <class S>
-__global__ __launch_bounds__(256) void reflection_atrous_lds_kernel(AtrousArgs A, const float4* __restrict__ normals,
-                                                                    const float4* __restrict__ pbr,
-                                                                    const uint8_t* __restrict__ tileMap,
-                                                                    const S* __restrict__ in, S* __restrict__ out,
-                                                                    S* __restrict__ hist) {
-    __shared__ float4 sIn[kAtrousMaxSide * kAtrousMaxSide];
-    __shared__ float4 sGuide[kAtrousMaxSide * kAtrousMaxSide];
+__device__ __forceinline__ void atrous_lds_block(const AtrousArgs& A, const float4* __restrict__ normals,
+                                                 const float4* __restrict__ pbr, const uint8_t* __restrict__ tileMap,
+                                                 const S* __restrict__ in, S* __restrict__ out, S* __restrict__ hist,
+                                                 int bX, int bY, float4* sIn, float4* sGuide) {
     const int W = A.W, H = A.H;
     const int tx = threadIdx.x & (kAtrousBlock - 1), ty = threadIdx.x / kAtrousBlock;
-    const int x = blockIdx.x * kAtrousBlock + tx, y = blockIdx.y * kAtrousBlock + ty;
+    const int x = bX * kAtrousBlock + tx, y = bY * kAtrousBlock + ty;
     const bool inImage = x < W && y < H;
     const size_t e = (size_t)y * W + x;
-    if (!block_has_filter_tile(A, tileMap)) {  // copy tiles only (DenoiseCopyTiles.comp:21-25)
+    if (!block_has_filter_tile(A, tileMap, bX, bY)) {  // copy tiles only (DenoiseCopyTiles.comp:21-25)
         if (inImage) {
             const float4 c = Texel<S>::ld4(in, e);
             Texel<S>::st4(out, e, c);
@@ -444,7 +475,7 @@ __global__ __launch_bounds__(256) void reflection_atrous_lds_kernel(AtrousArgs A
         return;
     }
     const AtrousRegion r = stage_region<S>(A, normals, pbr, sIn, sGuide,
-                                           [&](size_t q) { return Texel<S>::ld4(in, q); });
+                                           [&](size_t q) { return Texel<S>::ld4(in, q); }, bX, bY);
     if (!inImage) return;
     const int lc = (ty + r.h) * r.side + (tx + r.h);
     const float4 c = sIn[lc];
@@ -471,6 +502,17 @@ __global__ __launch_bounds__(256) void reflection_atrous_lds_kernel(AtrousArgs A
     }
     Texel<S>::st4(out, e, o);
     if (hist) Texel<S>::st4(hist, e, o);
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void reflection_atrous_lds_kernel(AtrousArgs A, const float4* __restrict__ normals,
+                                                                    const float4* __restrict__ pbr,
+                                                                    const uint8_t* __restrict__ tileMap,
+                                                                    const S* __restrict__ in, S* __restrict__ out,
+                                                                    S* __restrict__ hist) {
+    __shared__ float4 sIn[kAtrousMaxSide * kAtrousMaxSide];
+    __shared__ float4 sGuide[kAtrousMaxSide * kAtrousMaxSide];
+    atrous_lds_block<S>(A, normals, pbr, tileMap, in, out, hist, blockIdx.x, blockIdx.y, sIn, sGuide);
 }
 
 // ---------------------------------------------------------------------------
@@ -718,7 +760,7 @@ __global__ __launch_bounds__(256) void shadow_atrous_lds_kernel(AtrousArgs A, co
     const int x = blockIdx.x * kAtrousBlock + tx, y = blockIdx.y * kAtrousBlock + ty;
     const bool inImage = x < W && y < H;
     const size_t e = (size_t)y * W + x;
-    if (!block_has_filter_tile(A, tileMap)) {
+    if (!block_has_filter_tile(A, tileMap, blockIdx.x, blockIdx.y)) {
         if (inImage) {
             Texel<S>::st2(out, e, make_float2(0.f, 0.f));
             if (hist) Texel<S>::st2(hist, e, make_float2(0.f, 0.f));
@@ -726,7 +768,7 @@ __global__ __launch_bounds__(256) void shadow_atrous_lds_kernel(AtrousArgs A, co
         return;
     }
     const AtrousRegion r = stage_region<S>(A, normals, pbr, sIn, sGuide,
-                                           [&](size_t q) { return Texel<S>::ld2(in, q); });
+                                           [&](size_t q) { return Texel<S>::ld2(in, q); }, blockIdx.x, blockIdx.y);
     if (!inImage) return;
     float2 o = make_float2(0.f, 0.f);
     if (tileMap[(y >> 3) * A.tilesX + (x >> 3)]) {
@@ -755,10 +797,141 @@ __global__ __launch_bounds__(256) void shadow_atrous_lds_kernel(AtrousArgs A, co
     if (hist) Texel<S>::st2(hist, e, o);
 }
 
-static GBuf to_gbuf(const LuxgiGBuffer* g) {
+__host__ __device__ inline GBuf to_gbuf(const LuxgiGBuffer* g) {
     return GBuf{static_cast<const float4*>(static_cast<const void*>(g->color)),
                 static_cast<const float4*>(static_cast<const void*>(g->normals)),
                 static_cast<const float4*>(static_cast<const void*>(g->pbr)), g->depth};
+}
+
+// ---------------------------------------------------------------------------
+// Batched reflection denoise. The work of a pass over all items is the list of their 16x16-pixel
+// units (item k owns [blockStart_k, blockStart_k + units_k)); a persistent grid walks it, each
+// block finds its unit's item by binary search over blockStart and runs the same block body as the
+// single-frame kernels, so every item's output is identical to the single-frame path. Items, count
+// and unit total are read from device memory at run time, so a captured graph follows them.
+// ---------------------------------------------------------------------------
+using Item = LuxgiReflectionDenoiseItem;
+
+__device__ __forceinline__ int batch_item_of(const Item* items, int count, int w) {
+    int lo = 0, hi = count - 1;  // the last item whose blockStart <= w
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].blockStart <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+struct BatchWalk {
+    int begin, end, stride;
+};
+
+// order 0: block b takes units b, b + grid, ...; order 1: a contiguous run of ceil(total / grid)
+__device__ __forceinline__ BatchWalk batch_walk(int total, int order) {
+    BatchWalk w;
+    if (order == 1) {
+        const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+        w.begin = (int)blockIdx.x * per;
+        w.end = min(total, w.begin + per);
+        w.stride = 1;
+    } else {
+        w.begin = blockIdx.x;
+        w.end = total;
+        w.stride = gridDim.x;
+    }
+    return w;
+}
+
+// the unit's item and its block coordinates; false when the unit lies outside the item (a header
+// that does not match the items), which the whole block skips together
+__device__ __forceinline__ bool batch_unit(const Item* items, int count, int w, const Item*& it, int& bX, int& bY) {
+    // the item index is block-uniform: say so, so that its fields are read with scalar loads
+    it = &items[__builtin_amdgcn_readfirstlane(batch_item_of(items, count, w))];
+    const int W = it->width, H = it->height;
+    if (W <= 0 || H <= 0) return false;
+    const int nbx = (W + 15) / 16, local = w - it->blockStart;
+    if (local < 0 || local >= nbx * ((H + 15) / 16)) return false;
+    bX = local % nbx;
+    bY = local / nbx;
+    return true;
+}
+
+// one unit of the batched reprojection (every lane of the block calls it)
+template <class S>
+__device__ __forceinline__ void batch_reproject_unit(const Item* __restrict__ items, int count, int w,
+                                                     ReprojectShared& sm) {
+    const Item* it;
+    int bX, bY;
+    if (!batch_unit(items, count, w, it, bX, bY)) return;
+    const int pp = it->pingPong & 1;
+    // the parameters stay in memory (scalar loads where used) rather than copied into registers
+    const ReflArgsRef A{it->reproject, it->width, it->height, (it->width + 7) / 8};
+    __syncthreads();  // the previous unit's LDS reads are done
+    reproject_block<S>(A, to_gbuf(&it->cur), to_gbuf(&it->prev), static_cast<const S*>(it->noisy),
+                       static_cast<const S*>(it->outputs[1]), static_cast<const S*>(it->moments[1 - pp]),
+                       static_cast<S*>(it->outputs[0]), static_cast<S*>(it->moments[pp]), it->tileMap, bX, bY, sm);
+}
+
+// ONE: one unit per block (unit = blockIdx.x; the grid covers the units, order 2), without the
+// persistent loop, which costs the body ~50 registers
+template <class S, bool ONE>
+__global__ __launch_bounds__(256) void batch_reproject_kernel(const LuxgiDenoiseBatchHeader* __restrict__ hdr,
+                                                              const Item* __restrict__ items, int order) {
+    __shared__ ReprojectShared sm;
+    const int count = hdr->count, total = hdr->totalBlocks;
+    if (count <= 0 || count > LUXGI_MAX_DENOISE_BATCH) return;
+    if (ONE) {
+        if ((int)blockIdx.x < total) batch_reproject_unit<S>(items, count, blockIdx.x, sm);
+        return;
+    }
+    const BatchWalk wk = batch_walk(total, order);
+    for (int w = wk.begin; w < wk.end; w += wk.stride) batch_reproject_unit<S>(items, count, w, sm);
+}
+
+// iteration i of every item: input outputs[0] (i = 0) or atrous[readIdx], output atrous[writeIdx],
+// the history copy into outputs[1] after iteration 1 (RaytracedReflection.cpp:419-494)
+template <class S>
+__device__ __forceinline__ void batch_atrous_unit(const Item* __restrict__ items, int count, int w, int iteration,
+                                                  int readIdx, int writeIdx, float4* sIn, float4* sGuide) {
+    {
+        const Item* it;
+        int bX, bY;
+        if (!batch_unit(items, count, w, it, bX, bY)) return;
+        const LuxgiReflectionAtrousParams& f = it->filter;
+        const int radius = min(max(f.radius, 0), 2);
+        const AtrousArgs A{it->width, it->height, (it->width + 7) / 8, radius, 1 << iteration, f.phiColor, f.phiNormal,
+                           f.sigmaDepth, 0.0f, normal_squarings(f.phiNormal)};
+        const S* in = static_cast<const S*>(iteration == 0 ? it->outputs[0] : it->atrous[readIdx]);
+        S* out = static_cast<S*>(it->atrous[writeIdx]);
+        S* hist = iteration == 1 ? static_cast<S*>(it->outputs[1]) : nullptr;
+        const float4* nrm = static_cast<const float4*>(static_cast<const void*>(it->cur.normals));
+        const float4* pbr = static_cast<const float4*>(static_cast<const void*>(it->cur.pbr));
+        if (A.stepSize <= kAtrousMaxHalo && A.radius * A.stepSize <= kAtrousMaxHalo) {
+            __syncthreads();  // the previous unit's LDS reads are done
+            atrous_lds_block<S>(A, nrm, pbr, it->tileMap, in, out, hist, bX, bY, sIn, sGuide);
+        } else {
+            const int x = bX * 16 + (threadIdx.x & 15), y = bY * 16 + (threadIdx.x >> 4);
+            if (x < A.W && y < A.H) atrous_direct_pixel<S>(A, nrm, pbr, it->tileMap, in, out, hist, x, y);
+        }
+    }
+}
+
+template <class S, bool ONE>
+__global__ __launch_bounds__(256) void batch_atrous_kernel(const LuxgiDenoiseBatchHeader* __restrict__ hdr,
+                                                           const Item* __restrict__ items, int iteration, int readIdx,
+                                                           int writeIdx, int order) {
+    __shared__ float4 sIn[kAtrousMaxSide * kAtrousMaxSide];
+    __shared__ float4 sGuide[kAtrousMaxSide * kAtrousMaxSide];
+    const int count = hdr->count, total = hdr->totalBlocks;
+    if (count <= 0 || count > LUXGI_MAX_DENOISE_BATCH) return;
+    if (ONE) {
+        if ((int)blockIdx.x < total)
+            batch_atrous_unit<S>(items, count, blockIdx.x, iteration, readIdx, writeIdx, sIn, sGuide);
+        return;
+    }
+    const BatchWalk wk = batch_walk(total, order);
+    for (int w = wk.begin; w < wk.end; w += wk.stride)
+        batch_atrous_unit<S>(items, count, w, iteration, readIdx, writeIdx, sIn, sGuide);
 }
 static bool gbuf_ok(const LuxgiGBuffer* g) { return g && g->color && g->normals && g->pbr && g->depth; }
 
@@ -919,5 +1092,66 @@ extern "C" int luxgi_svgf_shadow_atrous(const LuxgiShadowAtrousParams* p, int32_
         launch_atrous<__half>(A, shadow_atrous_lds_kernel<__half>, shadow_atrous_kernel<__half>, st, nrm, pbr, tileMap,
                               input, output, historyOut);
     LUXGI_LAUNCH_CHECK();
+    return LUXGI_OK;
+}
+
+extern "C" int luxgi_svgf_batch_prepare(LuxgiReflectionDenoiseItem* items, int32_t count,
+                                        LuxgiDenoiseBatchHeader* header) {
+    if (!items || !header || count <= 0 || count > LUXGI_MAX_DENOISE_BATCH) return LUXGI_ERR_INVALID_ARGUMENT;
+    long long total = 0;
+    for (int k = 0; k < count; ++k) {
+        LuxgiReflectionDenoiseItem& it = items[k];
+        const int W = it.width, H = it.height;
+        if (W <= 0 || H <= 0 || (long long)W * H >= (1ll << 31)) return LUXGI_ERR_INVALID_ARGUMENT;
+        if (it.filter.radius < 0 || it.filter.radius > 2) return LUXGI_ERR_UNSUPPORTED;
+        if (!gbuf_ok(&it.cur) || !gbuf_ok(&it.prev) || !it.noisy || !it.tileMap) return LUXGI_ERR_INVALID_ARGUMENT;
+        for (int j = 0; j < 2; ++j)
+            if (!it.outputs[j] || !it.moments[j] || !it.atrous[j]) return LUXGI_ERR_INVALID_ARGUMENT;
+        it.blockStart = (int32_t)total;
+        total += (long long)((W + 15) / 16) * ((H + 15) / 16);
+        if (total >= (1ll << 31)) return LUXGI_ERR_INVALID_ARGUMENT;
+    }
+    header->count = count;
+    header->totalBlocks = (int32_t)total;
+    header->pad[0] = header->pad[1] = 0;
+    return LUXGI_OK;
+}
+
+template <class S>
+static void launch_batched(const LuxgiDenoiseBatchHeader* hdr, const LuxgiReflectionDenoiseItem* items, int iterations,
+                           int grid, int order, hipStream_t st) {
+    if (order == 2)
+        hipLaunchKernelGGL((batch_reproject_kernel<S, true>), dim3(grid), dim3(256), 0, st, hdr, items, order);
+    else
+        hipLaunchKernelGGL((batch_reproject_kernel<S, false>), dim3(grid), dim3(256), 0, st, hdr, items, order);
+    bool pp = false;
+    for (int i = 0; i < iterations; ++i) {
+        const int readIdx = pp ? 1 : 0, writeIdx = pp ? 0 : 1;
+        if (order == 2)
+            hipLaunchKernelGGL((batch_atrous_kernel<S, true>), dim3(grid), dim3(256), 0, st, hdr, items, i, readIdx,
+                               writeIdx, order);
+        else
+            hipLaunchKernelGGL((batch_atrous_kernel<S, false>), dim3(grid), dim3(256), 0, st, hdr, items, i, readIdx,
+                               writeIdx, order);
+        pp = !pp;
+    }
+}
+
+extern "C" int luxgi_svgf_reflection_denoise_batched(const LuxgiDenoiseBatchHeader* header,
+                                                     const LuxgiReflectionDenoiseItem* items, int32_t iterations,
+                                                     int32_t storage, int32_t gridBlocks, int32_t order, int32_t* result,
+                                                     void* stream) {
+    if (!header || !items || iterations < 0 || iterations > 16 || gridBlocks <= 0 || gridBlocks > (1 << 20) ||
+        order < 0 || order > 2)
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    hipStream_t st = as_stream(stream);
+    if (storage == LUXGI_STORAGE_FP32)
+        launch_batched<float>(header, items, iterations, gridBlocks, order, st);
+    else if (storage == LUXGI_STORAGE_FP16)
+        launch_batched<__half>(header, items, iterations, gridBlocks, order, st);
+    else
+        return LUXGI_ERR_UNSUPPORTED;
+    LUXGI_LAUNCH_CHECK();
+    if (result) *result = iterations > 0 ? ((iterations & 1) ? 1 : 0) : -1;
     return LUXGI_OK;
 }

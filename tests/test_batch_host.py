@@ -1,0 +1,65 @@
+"""Host side of the batched reflection denoise (CPU): luxgi_svgf_batch_prepare's unit offsets,
+header and argument checks (it makes no GPU call), and the descriptor layout."""
+import ctypes as C
+
+import pytest
+
+from luxgi_amd import abi
+
+
+def _items(sizes):
+    items = (abi.ReflectionDenoiseItem * len(sizes))()
+    fake = 0x1000  # never dereferenced by the host-side preparation
+    for k, (w, h) in enumerate(sizes):
+        it = items[k]
+        it.width, it.height = w, h
+        it.filter = abi.ReflectionAtrousParams(1, 0, 10.0, 32.0, 1.0, 1)
+        it.cur = abi.GBuffer(fake, fake, fake, fake)
+        it.prev = abi.GBuffer(fake, fake, fake, fake)
+        it.noisy = fake
+        it.tileMap = fake
+        for j in range(2):
+            it.outputs[j] = it.moments[j] = it.atrous[j] = fake
+    return items
+
+
+def test_prepare_sets_unit_offsets_and_header():
+    from luxgi_amd._lib import lib
+    sizes = [(77, 45), (8, 8), (1920, 1080), (1, 1), (17, 16)]
+    items = _items(sizes)
+    hdr = abi.DenoiseBatchHeader()
+    assert lib().luxgi_svgf_batch_prepare(C.byref(items), len(sizes), C.byref(hdr)) == 0
+    units = [((w + 15) // 16) * ((h + 15) // 16) for w, h in sizes]
+    start = 0
+    for k, n in enumerate(units):
+        assert items[k].blockStart == start
+        start += n
+    assert hdr.count == len(sizes) and hdr.totalBlocks == sum(units) == 15 + 1 + 8160 + 1 + 2
+
+
+@pytest.mark.parametrize("field", ["width", "radius", "noisy", "atrous1", "cur.depth", "count"])
+def test_prepare_rejects_bad_items(field):
+    from luxgi_amd._lib import lib
+    items = _items([(64, 32), (16, 16)])
+    n = 2
+    if field == "width":
+        items[1].width = 0
+    elif field == "radius":
+        items[0].filter.radius = 3
+    elif field == "noisy":
+        items[1].noisy = None
+    elif field == "atrous1":
+        items[0].atrous[1] = None
+    elif field == "cur.depth":
+        items[1].cur.depth = None
+    else:
+        n = abi.MAX_DENOISE_BATCH + 1
+    hdr = abi.DenoiseBatchHeader()
+    assert lib().luxgi_svgf_batch_prepare(C.byref(items), n, C.byref(hdr)) < 0
+
+
+def test_descriptor_layout_matches_the_header():
+    assert C.sizeof(abi.ReflectionDenoiseItem) == 336
+    assert abi.ReflectionDenoiseItem.cur.offset == 208
+    assert abi.ReflectionDenoiseItem.tileMap.offset == 328
+    assert C.sizeof(abi.DenoiseBatchHeader) == 16

@@ -134,6 +134,139 @@ __device__ __forceinline__ uint32_t hist_add(uint32_t* __restrict__ bins, uint32
 #endif
 }
 
+// State of one ray's sphere trace (SDFCommon.glsl:98-171), advanced one step at a time.
+struct MarchRay {
+    f3 o, d, traceEnd;
+    f3 cpos;             // the current cascade's centre
+    float half, voxel;   // and its half extent and voxel size
+    float stepTime, iy, nextStart;
+    int cas, step;
+};
+
+// lineHitAABB (SDFCommon.glsl:84-95) and the entry into cascade r.cas (:121-141)
+__device__ __forceinline__ void march_enter(const TraceArgs& A, MarchRay& r, float traceMax) {
+    const int cas = r.cas;
+    r.cpos = mk(A.g.cascadePosDistance[cas][0], A.g.cascadePosDistance[cas][1], A.g.cascadePosDistance[cas][2]);
+    r.half = A.g.cascadePosDistance[cas][3];
+    r.voxel = A.g.cascadeVoxelSize[cas];
+    const float half = r.half;
+    const f3 dl = r.traceEnd - r.o;
+    const f3 inv = mk(1.0f / dl.x, 1.0f / dl.y, 1.0f / dl.z);
+    const f3 en = (r.cpos - mk(half, half, half) - r.o) * inv;
+    const f3 ex = (r.cpos + mk(half, half, half) - r.o) * inv;
+    float ix = fmaxf(fminf(en.x, ex.x), fmaxf(fminf(en.y, ex.y), fminf(en.z, ex.z)));
+    float iy = fminf(fmaxf(en.x, ex.x), fminf(fmaxf(en.y, ex.y), fmaxf(en.z, ex.z)));
+    ix = clampf(ix, 0.0f, 1.0f) * traceMax;
+    iy = clampf(iy, 0.0f, 1.0f) * traceMax;
+    ix = fmaxf(ix, r.nextStart);
+    r.stepTime = ix;
+    if (ix >= iy) r.stepTime = iy;
+    else r.nextStart = iy;
+    r.iy = iy;
+    r.step = 0;
+}
+
+__device__ __forceinline__ void march_begin(const TraceArgs& A, const float* __restrict__ probeOffsets, int probe,
+                                            int ray, float traceMax, MarchRay& r) {
+    ray_setup(A, probeOffsets, probe, ray, r.o, r.d);
+    r.traceEnd = r.o + r.d * traceMax;
+    r.nextStart = 0.0f;
+    r.cas = 0;
+    march_enter(A, r, traceMax);
+}
+
+// One step of cascade r.cas (the body of SDFCommon.glsl:143-170). Returns true on a hit and fills
+// the hit record; otherwise advances stepTime.
+template <class SS, bool BRICK, bool COUNT>
+__device__ __forceinline__ bool march_step(const TraceArgs& A, const SS* __restrict__ sdf, const SS* __restrict__ mip,
+                                           const float* __restrict__ guard, MarchRay& r, float4& ra, float2& rb,
+                                           unsigned long long& cMip, unsigned long long& cFine) {
+    const int C = (int)A.g.cascadesCount;
+    const float chunkSizeDistance = 32.0f / A.g.resolution;
+    const float chunkMarginDistance = 4.0f / A.g.resolution;
+    const int texW = A.res * C, mipW = A.mipRes * C;
+    const float cmax = r.half * 2.0f;
+    const float stepTime = r.stepTime;
+    const f3 p = r.o + r.d * stepTime;
+    const float ux = clampf((p.x - r.cpos.x) / cmax + 0.5f, 0.0f, 1.0f);
+    const float uy = clampf((p.y - r.cpos.y) / cmax + 0.5f, 0.0f, 1.0f);
+    const float uz = clampf((p.z - r.cpos.z) / cmax + 0.5f, 0.0f, 1.0f);
+    const float tx = (float(r.cas) + ux) / float(C);
+    // The fine sample is used only where the mip distance is below the chunk size and the fine
+    // distance below twice the chunk margin (SDFCommon.glsl:150-162). A brick guard >= that margin
+    // proves the second test fails for every voxel of the footprint (trilinear filtering is a
+    // convex combination), so the fine lookup is skipped; the factor 1 + 1e-5 covers fp32
+    // rounding of the filter.
+    bool needFine = true;
+    if (guard) {
+        const int gx = clampi(int(floorf(tx * float(texW) - 0.5f)), 0, texW - 1);
+        const int gy = clampi(int(floorf(uy * float(A.res) - 0.5f)), 0, A.res - 1);
+        const int gz = clampi(int(floorf(uz * float(A.res) - 0.5f)), 0, A.res - 1);
+        const float gv = guard[((size_t)(gz >> 2) * (A.res >> 2) + (gy >> 2)) * (texW >> 2) + (gx >> 2)];
+        needFine = !(gv >= chunkMarginDistance * 2.0f * (1.0f + 1e-5f));
+    }
+    const float sm = sample_volume<SS>(mip, mipW, A.mipRes, A.mipRes, tx, uy, uz);
+    float sd = sm;
+    float st = 1.0f;
+    if (needFine) st = fine_lookup<SS, BRICK>(sdf, A.brickScale, texW, A.res, A.res, tx, uy, uz);
+    if (COUNT) cMip++;
+    if (sd < chunkSizeDistance) {
+        if (COUNT) cFine++;
+        if (needFine && st < chunkMarginDistance * 2.0f) sd = st;
+    } else {
+        sd = chunkSizeDistance;
+    }
+    sd *= cmax;
+    const float voxel = r.voxel;
+    const float thick = A.hitEps > 0.0f ? A.hitEps : voxel * 0.5f * clampf(stepTime / voxel, 0.0f, 1.0f);
+    if (sd < thick) {
+        if (A.refine) {  // FP32 re-evaluation of the hit step, same selection rule
+            const float f32 = refine_lookup(A, texW, A.res, A.res, tx, uy, uz);
+            const float s32 = sm < chunkSizeDistance ? (f32 < chunkMarginDistance * 2.0f ? f32 : sm)
+                                                     : chunkSizeDistance;
+            sd = s32 * cmax;
+        }
+        ra = make_float4(fmaxf(stepTime + sd - thick, 0.0f), tx, uy, uz);
+        rb = make_float2(sd, float(r.cas));
+        return true;
+    }
+    r.stepTime = stepTime + fmaxf(sd, voxel);
+    return false;
+}
+
+// What a finished ray leaves: its hit record and, with the hit sort, either its final result (no
+// atlas lookup: miss or inside hit) or its atlas chunk (returned; kNoChunk otherwise).
+template <class SR>
+__device__ __forceinline__ uint32_t march_finish(const TraceArgs& A, const MarchRay& r, float4 ra, float2 rb,
+                                                 int probe, int ray, float4* __restrict__ recA,
+                                                 float2* __restrict__ recB, const float* __restrict__ skyFaces,
+                                                 SR* __restrict__ outRad, SR* __restrict__ outDD,
+                                                 uint8_t* __restrict__ flags) {
+    const size_t rid = (size_t)probe * A.R + ray;
+    recA[rid] = ra;
+    recB[rid] = rb;
+    uint32_t key = kNoChunk;
+    if (A.sortHits) {
+        const bool inside = ra.x >= 0.0f && rb.x <= 0.0f && ra.x <= A.g.cascadeVoxelSize[0];
+        if (ra.x >= 0.0f && !inside) {  // atlas chunk of a front hit (the lookup of sample_atlas)
+            const f3 hp = r.o + r.d * ra.x;
+            const float pc[3] = {hp.x, hp.y, hp.z};
+            int cc[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                cc[i] = clampi(int(floorf(pc[i] / A.ad.chunkSize + float(kChunksRes) * 0.5f)), 0, kChunksRes - 1);
+            key = (uint32_t)((cc[2] * kChunksRes + cc[1]) * kChunksRes + cc[0]);
+        } else {
+            store_nonhit<SR>(A, skyFaces, outRad, outDD, flags, (size_t)probe * A.rayWidth + ray, r.d, inside);
+        }
+    }
+    return key;
+}
+
+__device__ __forceinline__ float trace_max_distance(const TraceArgs& A) {
+    return fminf(kWorldSize, A.g.cascadePosDistance[(int)A.g.cascadesCount - 1][3] * 2.0f);
+}
+
 // Sphere trace of one ray per lane (SDFCommon.glsl:98-171). Writes the hit record; with the hit
 // sort it also writes the final result of the rays without an atlas lookup and the (chunk, slot)
 // of the others.
@@ -155,99 +288,25 @@ __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __res
     const bool valid = probe < A.P && ray < A.R;
     const size_t rid = (size_t)probe * A.R + ray;
     if (valid) {
-        f3 o, d;
-        ray_setup(A, probeOffsets, probe, ray, o, d);
         const int C = (int)A.g.cascadesCount;
-        const float traceMax = fminf(kWorldSize, A.g.cascadePosDistance[C - 1][3] * 2.0f);
-        const float chunkSizeDistance = 32.0f / A.g.resolution;
-        const float chunkMarginDistance = 4.0f / A.g.resolution;
-        const int texW = A.res * C, mipW = A.mipRes * C;
-        const f3 traceEnd = o + d * traceMax;
-        float nextStart = 0.0f;
+        const float traceMax = trace_max_distance(A);
+        MarchRay r;
+        march_begin(A, probeOffsets, probe, ray, traceMax, r);
         float4 ra = make_float4(-1.0f, 0.f, 0.f, 0.f);
         float2 rb = make_float2(0.f, 0.f);
-        for (int cas = 0; cas < C && ra.x < 0.0f; ++cas) {
-            const f3 cpos = mk(A.g.cascadePosDistance[cas][0], A.g.cascadePosDistance[cas][1], A.g.cascadePosDistance[cas][2]);
-            const float half = A.g.cascadePosDistance[cas][3];
-            const float voxel = A.g.cascadeVoxelSize[cas];
-            // lineHitAABB (SDFCommon.glsl:84-95) and the cascade entry (:121-141)
-            const f3 dl = traceEnd - o;
-            const f3 inv = mk(1.0f / dl.x, 1.0f / dl.y, 1.0f / dl.z);
-            const f3 en = (cpos - mk(half, half, half) - o) * inv;
-            const f3 ex = (cpos + mk(half, half, half) - o) * inv;
-            float ix = fmaxf(fminf(en.x, ex.x), fmaxf(fminf(en.y, ex.y), fminf(en.z, ex.z)));
-            float iy = fminf(fmaxf(en.x, ex.x), fminf(fmaxf(en.y, ex.y), fmaxf(en.z, ex.z)));
-            ix = clampf(ix, 0.0f, 1.0f) * traceMax;
-            iy = clampf(iy, 0.0f, 1.0f) * traceMax;
-            ix = fmaxf(ix, nextStart);
-            float stepTime = ix;
-            if (ix >= iy) stepTime = iy;
-            else nextStart = iy;
-            const float cmax = half * 2.0f;
+        for (;;) {
             int step = 0;
-            for (; step < A.maxSteps && stepTime < iy; ++step) {
-                const f3 p = o + d * stepTime;
-                const float ux = clampf((p.x - cpos.x) / cmax + 0.5f, 0.0f, 1.0f);
-                const float uy = clampf((p.y - cpos.y) / cmax + 0.5f, 0.0f, 1.0f);
-                const float uz = clampf((p.z - cpos.z) / cmax + 0.5f, 0.0f, 1.0f);
-                const float tx = (float(cas) + ux) / float(C);
-                // The fine sample is used only where the mip distance is below the chunk size
-                // and the fine distance below twice the chunk margin (SDFCommon.glsl:150-162).
-                // A brick guard >= that margin proves the second test fails for every voxel of
-                // the footprint (trilinear filtering is a convex combination), so the fine
-                // lookup is skipped; the factor 1 + 1e-5 covers fp32 rounding of the filter.
-                bool needFine = true;
-                if (guard) {
-                    const int gx = clampi(int(floorf(tx * float(texW) - 0.5f)), 0, texW - 1);
-                    const int gy = clampi(int(floorf(uy * float(A.res) - 0.5f)), 0, A.res - 1);
-                    const int gz = clampi(int(floorf(uz * float(A.res) - 0.5f)), 0, A.res - 1);
-                    const float gv = guard[((size_t)(gz >> 2) * (A.res >> 2) + (gy >> 2)) * (texW >> 2) + (gx >> 2)];
-                    needFine = !(gv >= chunkMarginDistance * 2.0f * (1.0f + 1e-5f));
-                }
-                const float sm = sample_volume<SS>(mip, mipW, A.mipRes, A.mipRes, tx, uy, uz);
-                float sd = sm;
-                float st = 1.0f;
-                if (needFine) st = fine_lookup<SS, BRICK>(sdf, A.brickScale, texW, A.res, A.res, tx, uy, uz);
-                if (COUNT) cMip++;
-                if (sd < chunkSizeDistance) {
-                    if (COUNT) cFine++;
-                    if (needFine && st < chunkMarginDistance * 2.0f) sd = st;
-                } else {
-                    sd = chunkSizeDistance;
-                }
-                sd *= cmax;
-                const float thick = A.hitEps > 0.0f ? A.hitEps : voxel * 0.5f * clampf(stepTime / voxel, 0.0f, 1.0f);
-                if (sd < thick) {
-                    if (A.refine) {  // FP32 re-evaluation of the hit step, same selection rule
-                        const float f32 = refine_lookup(A, texW, A.res, A.res, tx, uy, uz);
-                        const float s32 = sm < chunkSizeDistance ? (f32 < chunkMarginDistance * 2.0f ? f32 : sm)
-                                                                 : chunkSizeDistance;
-                        sd = s32 * cmax;
-                    }
-                    ra = make_float4(fmaxf(stepTime + sd - thick, 0.0f), tx, uy, uz);
-                    rb = make_float2(sd, float(cas));
+            bool hit = false;
+            for (; step < A.maxSteps && r.stepTime < r.iy; ++step)
+                if (march_step<SS, BRICK, COUNT>(A, sdf, mip, guard, r, ra, rb, cMip, cFine)) {
+                    hit = true;
                     break;
                 }
-                stepTime += fmaxf(sd, voxel);
-            }
             if (COUNT) cSteps += step;
+            if (hit || ++r.cas >= C) break;
+            march_enter(A, r, traceMax);
         }
-        recA[rid] = ra;
-        recB[rid] = rb;
-        if (A.sortHits) {
-            const bool inside = ra.x >= 0.0f && rb.x <= 0.0f && ra.x <= A.g.cascadeVoxelSize[0];
-            if (ra.x >= 0.0f && !inside) {  // atlas chunk of a front hit (the lookup of sample_atlas)
-                const f3 hp = o + d * ra.x;
-                const float pc[3] = {hp.x, hp.y, hp.z};
-                int cc[3];
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    cc[i] = clampi(int(floorf(pc[i] / A.ad.chunkSize + float(kChunksRes) * 0.5f)), 0, kChunksRes - 1);
-                key = (uint32_t)((cc[2] * kChunksRes + cc[1]) * kChunksRes + cc[0]);
-            } else {
-                store_nonhit<SR>(A, skyFaces, outRad, outDD, flags, (size_t)probe * A.rayWidth + ray, d, inside);
-            }
-        }
+        key = march_finish<SR>(A, r, ra, rb, probe, ray, recA, recB, skyFaces, outRad, outDD, flags);
     }
     if (A.sortHits) {  // chunk histogram; each hit keeps its slot within its chunk
         const uint32_t slot = hist_add(bins, key, key != kNoChunk);

@@ -232,7 +232,8 @@ def test_trace_fp16_scaled_bricks_and_fp32_refine_vs_oracle():
     assert errs[1] <= errs[0] * 1.01 + 1e-7, errs
 
 
-def test_trace_shade_order_is_bitwise_identical():
+@pytest.mark.parametrize("layout,guard", [(abi.LAYOUT_BRICK4, False), (abi.LAYOUT_LINEAR, True)])
+def test_trace_shade_order_is_bitwise_identical(layout, guard):
     """Shading the front hits grouped by surface-atlas chunk (the default: the march writes the
     misses and inside hits, a counting sort orders the rest) or every ray in ray order gives the
     same outputs bit for bit; the counting trace (always in ray order) agrees too."""
@@ -243,7 +244,7 @@ def test_trace_shade_order_is_bitwise_identical():
     for order, count in ((0, False), (1, False), (0, True)):
         p = _scenes.trace_params(rot_seed=29)
         p.shadeOrder = order
-        ds = DeviceScene(g, sdf, mip, atlas, layout=abi.LAYOUT_BRICK4)
+        ds = DeviceScene(g, sdf, mip, atlas, layout=layout, guard=guard)
         vol = DDGIVolume(u, flags_buffer=True)
         counters = torch.zeros(8, dtype=torch.int64, device="cuda") if count else None
         for _ in range(2):  # the second call reuses the workspace: the bins must start from zero again

@@ -34,6 +34,7 @@ probes) on a 32x16x2 slab of the same volume (1,024 probes x 256 rays: trace +
 blend), on the host cores, scaled to Grays/s.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -242,6 +243,46 @@ def main():
     # G-buffer depth (4 B) + octahedral normal (8 B) + RGBA32F output (16 B) per pixel, atlases read once
     atlas_bytes = sum(t.numel() * t.element_size() for t in (vol.current_irradiance, vol.current_depth))
     sample_bytes = W * H * (4 + 8 + 16) + atlas_bytes
+
+    # ---------------- learned a-trous chain at 1080p (reported beside the step) ----------------
+    # 4 passes of the learned edge-stopping filter on the accumulated reflection signal, against the
+    # same 4 passes of the reference a-trous (tile map from the frame's reprojection)
+    from luxgi_amd._lib import lib as _L
+    from luxgi_amd.ddgi import ptr, stream_handle
+    from luxgi_amd.learned import LearnedAtrous
+    from luxgi_amd.svgf import gbuffer_struct
+    la = LearnedAtrous(W, H, iterations=4)
+    la_in = dn.outputs[0].float().contiguous()
+    gbs = gbuffer_struct(cur)
+    ref_bufs = [torch.empty_like(la_in), torch.empty_like(la_in)]
+
+    def ref_chain():
+        src = la_in
+        for i in range(4):
+            ap = abi.ReflectionAtrousParams(1, 1 << i, 10.0, 32.0, 1.0, 1)
+            _L().luxgi_svgf_reflection_atrous(ctypes.byref(ap), W, H, abi.STORAGE_FP32, ctypes.byref(gbs),
+                                              ptr(dn.tile_map), ptr(src), ptr(ref_bufs[i & 1]), None,
+                                              stream_handle())
+            src = ref_bufs[i & 1]
+
+    lev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    for _ in range(2):
+        la.forward(cur, la_in)
+        ref_chain()
+    lev[0].record()
+    for _ in range(args.steps):
+        la.forward(cur, la_in)
+    lev[1].record()
+    for _ in range(args.steps):
+        ref_chain()
+    lev[2].record()
+    torch.cuda.synchronize()
+    learned = {"learned_4_passes_ms": lev[0].elapsed_time(lev[1]) / args.steps,
+               "reference_4_passes_ms": lev[1].elapsed_time(lev[2]) / args.steps,
+               "mlp": "6-10-1 ReLU, 81 parameters, one per tap", "image": [W, H]}
+    log(f"[bench] learned a-trous 1080p x4 {learned['learned_4_passes_ms']:.3f} ms "
+        f"(reference {learned['reference_4_passes_ms']:.3f} ms)", q)
+    del la, la_in, ref_bufs
 
     # ---------------- surface atlas passes (reported beside the step) ----------------
     # chunk culling of the 2,000 objects, then the surface-cache relight of all 12,000 tiles
@@ -465,6 +506,7 @@ def main():
         "probe_update_ms": float(np.mean(t_upd)),
         "svgf_ms": float(np.mean(t_svgf)),
         "probe_sample_ms": sample_ms,
+        "learned_atrous": learned,
         "reduced_precision": mixed,
         "probe_sample_gbs": sample_bytes / (sample_ms * 1e-3) / 1e9,
         "surface_atlas": surf,

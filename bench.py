@@ -147,6 +147,52 @@ def bench_svgf_batched(args, prims_dev, n_prims, sdf_data, sdf, res, q):
         del singles, batch, graphed, frames
     return out
 
+
+def bench_ddgi_batched(args, ds, q):
+    """Eight probe volumes of uneven probe and ray counts over the bench scene: per-volume updates on
+    one stream, on four streams, and one graph replay per frame (orientations read from device
+    memory)."""
+    import numpy as np
+    import torch
+
+    from luxgi_amd import abi, scene
+    from luxgi_amd.batch import DDGIBatch
+    shapes = [((8, 8, 8), 128), ((16, 8, 16), 128), ((8, 4, 8), 256), ((12, 6, 12), 192), ((6, 6, 6), 64),
+              ((10, 5, 10), 96), ((4, 4, 4), 256), ((14, 7, 14), 128)]
+    unis = [abi.make_uniform(c, (-1.0, -1.0, -1.0), [2.0 / (n - 1) for n in c], r, irr_oct=8, depth_oct=16)
+            for c, r in shapes]
+    serial = DDGIBatch(unis, ds, streams=0)
+    eager = DDGIBatch(unis, ds)
+    graphed = DDGIBatch(unis, ds)
+    rng = np.random.default_rng(99)
+    frames = [[scene.random_rotation(int(s)) for s in rng.integers(0, 2 ** 31, size=len(unis))]
+              for _ in range(args.warmup + args.steps + 1)]
+    for b in (serial, eager, graphed):
+        b.update(frames[0])
+    graphed.capture()
+    for i in range(1, args.warmup + 1):
+        for b in (serial, eager, graphed):
+            b.update(frames[i])
+    torch.cuda.synchronize()
+    out = {"volumes": len(unis), "rays": int(sum(u.num_probes * u.raysPerProbe for u in unis)),
+           "shapes": [[list(c), r] for c, r in shapes]}
+    for name, b in (("per_volume_one_stream_ms", serial), ("per_volume_ms", eager), ("graph_replay_ms", graphed)):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for i in range(args.warmup + 1, args.warmup + 1 + args.steps):
+            b.update(frames[i])
+        ev[1].record()
+        torch.cuda.synchronize()
+        out[name] = ev[0].elapsed_time(ev[1]) / args.steps
+    same = all(torch.equal(a.current_irradiance, b.current_irradiance) and
+               torch.equal(a.current_irradiance, c.current_irradiance)
+               for a, b, c in zip(eager.volumes, graphed.volumes, serial.volumes))
+    out["graph_equals_per_volume"] = bool(same)
+    out["speedup_graph_vs_one_stream"] = out["per_volume_one_stream_ms"] / out["graph_replay_ms"]
+    log(f"[bench] ddgi batched {len(unis)} volumes: one stream {out['per_volume_one_stream_ms']:.3f} ms, "
+        f"4 streams {out['per_volume_ms']:.3f} ms, graph {out['graph_replay_ms']:.3f} ms", q)
+    return out
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -411,6 +457,7 @@ def main():
     # (1 + 4 launches each) against the batched persistent launches (5 in all) and their graph replay;
     # the persistent grid and unit order chosen by the autotuner
     svgf_batched = None if args.no_batch else bench_svgf_batched(args, prims_dev, len(prims), g, sdf, res, q)
+    ddgi_batched = None if args.no_batch else bench_ddgi_batched(args, ds, q)
     # ---------------- reduced-precision form (reported beside the step, not in value) ----------------
     # FP16 SDF bricks with per-brick scale and FP32 re-evaluation at hits, FP16-MFMA blend (one
     # rounding per operand, FP32 accumulation), SVGF with RGBA16F intermediates. Ray buffers stay
@@ -514,6 +561,7 @@ def main():
         "sdf_reflection_ms": refl_ms,
         "sdf_build": sdf_build,
         "svgf_batched": svgf_batched,
+        "ddgi_batched": ddgi_batched,
         "trace_grays_per_s": rays_per_frame / trace_s / 1e9,
         "trace_counts": {"lookups": n_lookup, "normals": n_normal, "objects": c[3], "tile_gathers": n_tile,
                          "rays": n_rays, "steps": c[6], "hits": c[7]},

@@ -126,6 +126,9 @@ typedef struct LuxgiTraceParams {
                                workspace), 1 = in ray order; the outputs are identical */
     int32_t  pad1;
     float    skyColor[4];   /* sky radiance when no cubemap is bound (reference binds a unit cube) */
+    const float* deviceOrientation; /* optional device array of 16 floats (column-major): read in place of
+                                       randomOrientation when the kernels run, so a captured graph follows
+                                       the orientation copied there before each replay */
 } LuxgiTraceParams;
 
 /* ------------------------------------------------------------------------- */

@@ -77,7 +77,7 @@ class TraceParams(C.Structure):
         ("randomOrientation", f16), ("numFrames", C.c_uint32), ("infiniteBounces", C.c_uint32),
         ("numLights", C.c_int32), ("intensity", C.c_float),
         ("maxSteps", C.c_int32), ("hitEpsilon", C.c_float), ("shadeOrder", C.c_int32), ("pad1", C.c_int32),
-        ("skyColor", f4),
+        ("skyColor", f4), ("deviceOrientation", C.c_void_p),
     ]
 
 
@@ -225,7 +225,7 @@ class DenoiseBatchHeader(C.Structure):
     _fields_ = [("count", C.c_int32), ("totalBlocks", C.c_int32), ("pad", C.c_int32 * 2)]
 
 _SIZES = {DDGIUniform: 96, GlobalSDFData: 96, SurfaceAtlasData: 32, AtlasObject: 128, AtlasTile: 96,
-          TraceParams: 112, ReflectionReprojectParams: 168, ReflectionAtrousParams: 24,
+          TraceParams: 120, ReflectionReprojectParams: 168, ReflectionAtrousParams: 24,
           ShadowReprojectParams: 80, ShadowAtrousParams: 32, SampleProbeParams: 80, Light: 64,
           SurfaceGBuffer: 40, SurfaceLightParams: 32, SurfaceDraw: 8, BlueNoise: 16, SdfShadowParams: 144,
           SdfReflectionParams: 144, ObjectRasterizeData: 176, RasterizeParams: 48, MeshSDF: 72,

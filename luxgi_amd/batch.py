@@ -1,4 +1,5 @@
-"""Batched reflection denoise over frames of uneven sizes, and its graph capture (libluxgi.so).
+"""Batched reflection denoise over frames of uneven sizes, graph-replayed probe-volume updates
+(libluxgi.so).
 
 ReflectionDenoiseBatch runs raytraced_reflection::denoise::system (Engine/Raytrace/
 RaytracedReflection.cpp:496-520) for many frames at once through
@@ -8,6 +9,9 @@ then each a-trous iteration of every frame in one more. The frames' descriptors
 1 + iterations launches are captured once (capture()) and replayed every frame after the
 descriptors are refreshed by one host-to-device copy from pinned memory. Each frame's outputs are
 identical to luxgi_svgf_reflection_denoise on that frame.
+
+DDGIBatch updates several probe volumes per frame by replaying a captured graph of their trace and
+blend launches; the traces read the per-frame ray orientations from device memory.
 """
 import ctypes as C
 
@@ -173,3 +177,105 @@ class ReflectionDenoiseBatch:
             for t in (*self.outputs[k], *self.moments[k]):
                 t.copy_(saved[i])
                 i += 1
+
+
+class DDGIBatch:
+    """Several probe volumes of any probe and ray counts over one scene, updated each frame (trace,
+    blend with the fused border, flip; DDGIRenderer.cpp:217-413 per volume) by one graph replay.
+
+    Per frame only the volumes' ray orientations change (the ping-pong parity alternates and the
+    first frame is special). So the first frame runs eagerly, capture() records one graph per
+    parity, and update() copies the new orientations into a device array that the traces read
+    at run time (LuxgiTraceParams.deviceOrientation) before replaying the parity's graph."""
+
+    def __init__(self, uniforms, scene, trace_params=None, fuse_border=True, streams=4):
+        from .ddgi import DDGIVolume, default_trace_params
+        self.volumes = [DDGIVolume(u) for u in uniforms]
+        self.scene = scene
+        self.fuse_border = fuse_border
+        # the volumes are independent: volume i runs on side stream i % streams (default priority),
+        # forked from and joined back to the current stream, so small volumes overlap on the GPU
+        # (and become parallel branches of the captured graph)
+        self.streams = [torch.cuda.Stream() for _ in range(max(int(streams), 0))]
+        k = len(self.volumes)
+        self.orientation = torch.zeros((k, 16), dtype=torch.float32, device="cuda")
+        self._staging = [torch.zeros((k, 16), dtype=torch.float32, pin_memory=True) for _ in range(2)]
+        self._staged = [None, None]
+        self._next = 0
+        self.params = []
+        for i in range(k):
+            p = trace_params[i] if trace_params is not None else default_trace_params(np.eye(3))
+            p.deviceOrientation = self.orientation[i].data_ptr()
+            self.params.append(p)
+        self.graphs = [None, None]
+
+    @property
+    def frames(self):
+        return self.volumes[0].frames
+
+    def _upload(self, rotations):
+        """rotations: one 3x3 (or 4x4) orientation per volume -> the device array, column-major."""
+        j = self._next
+        self._next = 1 - j
+        if self._staged[j] is not None:
+            self._staged[j].synchronize()
+        host = self._staging[j].numpy()
+        for i, r in enumerate(rotations):
+            m = np.eye(4)
+            r = np.asarray(r, dtype=np.float64)
+            m[:r.shape[0], :r.shape[1]] = r
+            host[i] = m.T.reshape(-1)  # column-major
+        self.orientation.copy_(self._staging[j], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        self._staged[j] = ev
+
+    def _run_volume(self, vol, p):
+        vol.trace_rays(self.scene, p)
+        vol.probe_update(fuse_border=self.fuse_border)
+        if not self.fuse_border:
+            vol.border_update()
+
+    def _run(self):
+        if not self.streams:
+            for vol, p in zip(self.volumes, self.params):
+                self._run_volume(vol, p)
+            return
+        main = torch.cuda.current_stream()
+        for st in self.streams:
+            st.wait_stream(main)
+        for i, (vol, p) in enumerate(zip(self.volumes, self.params)):
+            with torch.cuda.stream(self.streams[i % len(self.streams)]):
+                self._run_volume(vol, p)
+        for st in self.streams:
+            main.wait_stream(st)
+
+    def update(self, rotations):
+        """One frame of every volume with the given orientations."""
+        if len(rotations) != len(self.volumes):
+            raise ValueError("one orientation per volume")
+        self._upload(rotations)
+        parity = self.volumes[0].ping_pong
+        if self.frames > 0 and self.graphs[parity] is not None:
+            self.graphs[parity].replay()
+        else:
+            self._run()
+        for vol in self.volumes:
+            vol.end_frame()
+
+    def capture(self):
+        """Records the frame for both ping-pong parities (after at least one eager frame, which also
+        allocates the trace workspaces). The volumes' state is left as it was."""
+        if self.frames == 0:
+            raise RuntimeError("run one frame with update() before capture()")
+        torch.cuda.synchronize()
+        keep = [(v.ping_pong, v.frames) for v in self.volumes]
+        for parity in (0, 1):
+            for v in self.volumes:
+                v.ping_pong = parity
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._run()
+            self.graphs[parity] = g
+        for v, (pp, fr) in zip(self.volumes, keep):
+            v.ping_pong, v.frames = pp, fr

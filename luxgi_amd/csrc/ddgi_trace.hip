@@ -27,6 +27,7 @@ struct TraceArgs {
     LuxgiGlobalSDFData g;
     LuxgiSurfaceAtlasData ad;
     float rot[9];  // column-major mat3 of randomOrientation
+    const float* rotDev;  // or, when set, the 4x4 column-major orientation in device memory
     float sky[3];
     int res, mipRes, maxSteps, skySize, P, R, rayWidth;
     float hitEps;
@@ -89,9 +90,13 @@ __device__ __forceinline__ void ray_setup(const TraceArgs& A, const float* __res
         origin = origin + mk(o.x, o.y, o.z);
     }
     const f3 sf = spherical_fibonacci(float(ray), float(A.R));
-    dir = normalize3(mk(A.rot[0] * sf.x + A.rot[3] * sf.y + A.rot[6] * sf.z,
-                        A.rot[1] * sf.x + A.rot[4] * sf.y + A.rot[7] * sf.z,
-                        A.rot[2] * sf.x + A.rot[5] * sf.y + A.rot[8] * sf.z));
+    float m[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) m[c * 3 + r] = A.rotDev ? A.rotDev[c * 4 + r] : A.rot[c * 3 + r];
+    dir = normalize3(mk(m[0] * sf.x + m[3] * sf.y + m[6] * sf.z, m[1] * sf.x + m[4] * sf.y + m[7] * sf.z,
+                        m[2] * sf.x + m[5] * sf.y + m[8] * sf.z));
 }
 
 // fine-volume lookup in the volume's layout
@@ -575,6 +580,7 @@ static int trace_entry(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, c
     A.ad = at->data;
     for (int c = 0; c < 3; ++c)
         for (int r = 0; r < 3; ++r) A.rot[c * 3 + r] = p->randomOrientation[c * 4 + r];
+    A.rotDev = p->deviceOrientation;
     for (int i = 0; i < 3; ++i) A.sky[i] = p->skyColor[i];
     A.res = vol->resolution;
     A.mipRes = vol->mipResolution;

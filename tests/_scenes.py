@@ -44,3 +44,22 @@ def oracle_trace(u, g, sdf, mip, atlas, params, **kw):
 
 def np_f32(x):
     return np.ascontiguousarray(x, dtype=np.float32)
+
+
+def cascaded_sdf(prims, res, halves, center=(0.0, 0.0, 0.0)):
+    """A global SDF clipmap of len(halves) cascades around `center` (GlobalDistanceField.cpp:656-666):
+    cascade c covers center +- halves[c] at res^3, normalised by its own extent 2 halves[c], packed
+    along x ([res][res][C res]); the coarse mip of each cascade with maxDistance 2 halves[c]
+    (:823-846). Returns (sdf, mip, GlobalSDFData)."""
+    C = len(halves)
+    sdf = np.concatenate([scene.bake_sdf_numpy(prims, res, center=center, half=h) for h in halves], axis=2)
+    mip = None
+    for c, h in enumerate(halves):
+        mip = oracle.sdf_mip(sdf, res, cascades=C, cascade=c, max_distance=2.0 * h, mip=mip)
+    g = abi.make_sdf_data(center=center, half_extent=halves[0], resolution=res)
+    for c, h in enumerate(halves):
+        g.cascadePosDistance[c][0], g.cascadePosDistance[c][1], g.cascadePosDistance[c][2] = center
+        g.cascadePosDistance[c][3] = h
+        g.cascadeVoxelSize[c] = 2.0 * h / res
+    g.cascadesCount = C
+    return np.ascontiguousarray(sdf), mip, g

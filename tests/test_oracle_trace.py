@@ -178,3 +178,56 @@ def test_sky_cubemap_orientation():
     err = np.linalg.norm(rad[..., :3] - dd[..., :3], axis=-1)
     assert err.max() < 0.75 / size, err.max()
     assert np.median(err) < 0.02, np.median(err)
+
+
+def test_trace_analytic_spheres_across_cascades():
+    """Three cascades (half extents 0.5, 1, 2 at 32^3 each) and two spheres: one inside cascade 0,
+    one reachable only through cascade 2. Rays that reach a sphere hit it within 1.6 voxels of the
+    cascade holding the hit point (the march moves out cascade by cascade, SDFCommon.glsl:121-141);
+    rays that pass both spheres by more than that cascade's 1.5 voxels miss."""
+    res = 32
+    halves = (0.5, 1.0, 2.0)
+    prims = np.zeros(2, dtype=scene.PRIM_DTYPE)
+    prims["type"] = scene.PRIM_SPHERE
+    prims["center"][0] = (0.22, 0.08, -0.1)
+    prims["size"][0] = (0.12, 0, 0)
+    prims["center"][1] = (1.35, 0.0, 0.05)
+    prims["size"][1] = (0.35, 0, 0)
+    prims["albedo"] = (0.5, 0.5, 0.5)
+    sdf, mip, g = _scenes.cascaded_sdf(prims, res, halves)
+    atlas = scene.build_atlas(prims, resolution=128, tile=16, threshold=1.05 * 2 / res)
+    u = abi.make_uniform((3, 3, 3), (-0.3, -0.3, -0.3), (0.3, 0.3, 0.3), 128)
+    rad, dd, fl, _ = _scenes.oracle_trace(u, g, sdf, mip, atlas, _scenes.trace_params(rot_seed=4))
+
+    def cascade_voxel(p):
+        for h in halves:
+            if np.all(np.abs(p) <= h):
+                return 2.0 * h / res
+        return None
+
+    checked = [0, 0]
+    for pi in range(u.num_probes):
+        o = oracle.probe_location(u, pi)
+        for k in range(u.raysPerProbe):
+            d = dd[pi, k, :3].astype(np.float64)
+            best, which, line = None, None, []
+            for s in range(2):
+                c, r = prims["center"][s].astype(np.float64), float(prims["size"][s][0])
+                oc = o - c
+                b = oc @ d
+                disc = b * b - (oc @ oc - r * r)
+                line.append((np.sqrt(max(oc @ oc - b * b, 0.0)) if b < 0 else np.linalg.norm(oc)) - r)
+                if disc >= 0 and -b - np.sqrt(disc) > 0:
+                    t = -b - np.sqrt(disc)
+                    if best is None or t < best:
+                        best, which = t, s
+            if best is not None and best < 3.5:
+                vox = cascade_voxel(o + d * best)
+                if vox is None or line[which] > -vox:
+                    continue  # grazing, or outside the clipmap
+                assert fl[pi, k] == 1, (pi, k)
+                assert abs(dd[pi, k, 3] - best) <= 1.6 * vox, (pi, k, dd[pi, k, 3], best, vox)
+                checked[which] += 1
+            elif min(line) > 1.5 * 2.0 * halves[-1] / res:
+                assert fl[pi, k] == 0, (pi, k)
+    assert checked[0] > 20 and checked[1] > 20, checked

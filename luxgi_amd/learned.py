@@ -145,3 +145,78 @@ class LearnedAtrous:
 
     def weights_numpy(self):
         return self.weights.detach().cpu().numpy().astype(np.float64)
+
+
+def reference_chain(gbuffer, inp, tile_map, iterations=4, radius=1, phi_color=10.0, phi_normal=32.0,
+                    sigma_depth=1.0):
+    """The reference's a-trous chain on an FP32 (rgb, variance) image: `iterations` passes of
+    luxgi_svgf_reflection_atrous with step 1 << i and the reprojection's tile map."""
+    H, W = inp.shape[:2]
+    gb = gbuffer_struct(gbuffer)
+    bufs = [torch.empty_like(inp), torch.empty_like(inp)]
+    src = inp.contiguous()
+    for i in range(iterations):
+        ap = abi.ReflectionAtrousParams(radius, 1 << i, phi_color, phi_normal, sigma_depth, 1)
+        check(lib().luxgi_svgf_reflection_atrous(C.byref(ap), W, H, abi.STORAGE_FP32, C.byref(gb), ptr(tile_map),
+                                                 ptr(src), ptr(bufs[i & 1]), None, stream_handle()),
+              "luxgi_svgf_reflection_atrous")
+        src = bufs[i & 1]
+    return src
+
+
+def masked_mse(x, target, mask):
+    m = mask.to(torch.float32)
+    return float((((x[..., :3] - target[..., :3]) ** 2).sum(-1) * m).sum() / m.sum().clamp_min(1.0))
+
+
+class LearnedReflectionDenoiser:
+    """The reflection denoise (denoise::system, RaytracedReflection.cpp:496-520) with the learned
+    a-trous chain in place of the reference's, used only when validate() has shown it closer to the
+    converged target than the reference's chain on the caller's samples; otherwise (and before any
+    validation) every frame goes through the reference's path unchanged. The learned chain keeps the
+    reference's temporal structure: reprojection, then the passes, with the output of pass 1 copied
+    back as the next frame's history."""
+
+    def __init__(self, width, height, weights=None, iterations=4, alpha=0.01, moments_alpha=0.2, radius=1,
+                 phi_color=10.0, phi_normal=32.0, sigma_depth=1.0):
+        from .svgf import ReflectionDenoiser
+        self.W, self.H = width, height
+        self.plain = ReflectionDenoiser(width, height, iterations=iterations, alpha=alpha,
+                                        moments_alpha=moments_alpha, radius=radius, phi_color=phi_color,
+                                        phi_normal=phi_normal, sigma_depth=sigma_depth)
+        self.learned = LearnedAtrous(width, height, weights=weights, iterations=iterations, radius=radius,
+                                     phi_color=phi_color, sigma_depth=sigma_depth)
+        self.chain = dict(iterations=iterations, radius=radius, phi_color=phi_color, phi_normal=phi_normal,
+                          sigma_depth=sigma_depth)
+        self.use_learned = False
+        self.validation = None
+
+    def set_camera(self, view_proj_inv, prev_view_proj, camera_pos, camera_delta):
+        self.plain.set_camera(view_proj_inv, prev_view_proj, camera_pos, camera_delta)
+
+    def validate(self, samples):
+        """samples: (gbuffer, reprojected input, tile map, converged target, mask) tuples. Chooses the
+        learned chain iff its mean squared error over the samples is below the reference chain's."""
+        lm, pm = [], []
+        for gb, inp, tiles, target, mask in samples:
+            lm.append(masked_mse(self.learned.forward(gb, inp), target, mask))
+            pm.append(masked_mse(reference_chain(gb, inp, tiles, **self.chain), target, mask))
+        self.validation = {"learned_mse": float(np.mean(lm)), "reference_mse": float(np.mean(pm))}
+        self.use_learned = self.validation["learned_mse"] < self.validation["reference_mse"]
+        return self.validation
+
+    def denoise(self, cur, prev, noisy):
+        """One frame; returns the output tensor."""
+        if not self.use_learned:
+            return self.plain.denoise(cur, prev, noisy)
+        d = self.plain
+        check(lib().luxgi_svgf_reflection_reproject(
+            C.byref(d.rp), self.W, self.H, abi.STORAGE_FP32, C.byref(gbuffer_struct(cur)), C.byref(gbuffer_struct(prev)),
+            ptr(noisy), ptr(d.outputs[1]), ptr(d.moments[1 - d.ping_pong]), ptr(d.outputs[0]),
+            ptr(d.moments[d.ping_pong]), ptr(d.tile_map), stream_handle()), "luxgi_svgf_reflection_reproject")
+        out = self.learned.forward(cur, d.outputs[0])
+        if self.learned.iterations > 1:
+            d.outputs[1].copy_(self.learned.images[1])  # the history copy after iteration 1 (:488-491)
+        d.ping_pong = 1 - d.ping_pong
+        d.result = out
+        return out

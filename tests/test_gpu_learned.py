@@ -258,3 +258,59 @@ def test_learned_chain_beats_the_reference_chain_on_a_held_out_view():
     assert last < first
     assert learned < 0.9 * ref, (learned, ref)
     assert learned < raw
+
+    # the denoiser that uses the learned chain only when it validates better than the reference's
+    from luxgi_amd.learned import LearnedReflectionDenoiser
+    samples = [(gB, seqB[k][0], seqB[k][1], tB, mB) for k in frames_b]
+    good = LearnedReflectionDenoiser(W, H, weights=la.weights_numpy())
+    assert not good.use_learned  # nothing validated yet: the reference path
+    v = good.validate(samples)
+    assert good.use_learned and v["learned_mse"] < v["reference_mse"], v
+    blur = np.zeros(81)
+    blur[80] = 8.0  # every tap weighted e^8: a box blur over each pass's footprint
+    bad = LearnedReflectionDenoiser(W, H, weights=blur, radius=2)
+    v = bad.validate(samples)
+    assert bad.use_learned == (v["learned_mse"] < v["reference_mse"]), v
+
+
+def test_learned_denoiser_falls_back_to_the_reference_path_and_keeps_its_temporal_structure():
+    """Unvalidated: every frame equals ReflectionDenoiser's bit for bit. Learned: each frame equals
+    the reprojection followed by the learned chain, with pass 1's output as the next history."""
+    from luxgi_amd._lib import lib
+    from luxgi_amd.ddgi import ptr, stream_handle
+    from luxgi_amd.learned import LearnedAtrous, LearnedReflectionDenoiser
+    from luxgi_amd.svgf import ReflectionDenoiser, gbuffer_struct
+    from tests import _svgf_inputs
+    W, H = 64, 48
+    g, prev, noisy, _, _, _, cam = _svgf_inputs.make_pair(W, H, 3)
+    dev = lambda d: {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in d.items()}  # noqa: E731
+    g, prev = dev(g), dev(prev)
+    frames = [torch.from_numpy(noisy * (1.0 + 0.2 * f)).cuda() for f in range(3)]
+    ld = LearnedReflectionDenoiser(W, H)
+    ref = ReflectionDenoiser(W, H, iterations=4)
+    for d in (ld, ref):
+        d.set_camera(cam["VPI"], cam["VPp"], cam["eye"], cam["delta"])
+    for f in frames:
+        assert torch.equal(ld.denoise(g, prev, f), ref.denoise(g, prev, f))
+    # learned path against a hand-run reprojection + chain
+    ld2 = LearnedReflectionDenoiser(W, H)
+    ld2.set_camera(cam["VPI"], cam["VPp"], cam["eye"], cam["delta"])
+    ld2.use_learned = True
+    la = LearnedAtrous(W, H)
+    rp = ld2.plain.rp
+    hist_c = torch.zeros((H, W, 4), device="cuda")
+    mom = [torch.zeros((H, W, 4), device="cuda"), torch.zeros((H, W, 4), device="cuda")]
+    tiles = torch.zeros(lib().luxgi_svgf_tile_count(W, H), dtype=torch.uint8, device="cuda")
+    pp = 0
+    for f in frames:
+        got = ld2.denoise(g, prev, f).clone()
+        oc = torch.zeros((H, W, 4), device="cuda")
+        assert lib().luxgi_svgf_reflection_reproject(C.byref(rp), W, H, abi.STORAGE_FP32, C.byref(gbuffer_struct(g)),
+                                                     C.byref(gbuffer_struct(prev)), ptr(f), ptr(hist_c),
+                                                     ptr(mom[1 - pp]), ptr(oc), ptr(mom[pp]), ptr(tiles),
+                                                     stream_handle()) == 0
+        want = la.forward(g, oc).clone()
+        hist_c = la.images[1].clone()
+        pp = 1 - pp
+        torch.cuda.synchronize()
+        assert torch.equal(got, want)

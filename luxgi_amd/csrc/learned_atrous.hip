@@ -154,6 +154,89 @@ __global__ __launch_bounds__(256) void learned_atrous_kernel(LaArgs A, const flo
     if (sumW) sumW[e] = sw;
 }
 
+// LDS-tiled form of the same pass for footprints radius * stepSize <= 8: a 16x16 block stages its
+// input and guides (decoded normal, linear depth) with a halo of max(radius * stepSize, 1) once,
+// so every tap, and the 3x3 variance, is an LDS read. The arithmetic is the direct kernel's, in
+// the same order (the compiler may contract a multiply-add differently, so the two agree to
+// rounding, not necessarily bit for bit).
+constexpr int kLaBlock = 16, kLaMaxHalo = 8, kLaMaxSide = kLaBlock + 2 * kLaMaxHalo;
+
+__global__ __launch_bounds__(256) void learned_atrous_lds_kernel(LaArgs A, const float* __restrict__ params,
+                                                                 const float4* __restrict__ normals,
+                                                                 const float4* __restrict__ pbr,
+                                                                 const float4* __restrict__ in,
+                                                                 float4* __restrict__ out, float* __restrict__ sumW) {
+    __shared__ float sP[kLaParams];
+    __shared__ float4 sIn[kLaMaxSide * kLaMaxSide];
+    __shared__ float4 sGuide[kLaMaxSide * kLaMaxSide];  // normal xyz, linear depth
+    const int h = max(A.radius * A.stepSize, 1), side = kLaBlock + 2 * h;
+    const int ox = blockIdx.x * kLaBlock - h, oy = blockIdx.y * kLaBlock - h;
+    for (int i = threadIdx.x; i < kLaParams; i += blockDim.x) sP[i] = params[i];
+    for (int i = threadIdx.x; i < side * side; i += blockDim.x) {
+        const int ly = i / side, lx = i - ly * side;
+        const int gx = ox + lx, gy = oy + ly;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f), gd = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (la_inside(gx, gy, A.W, A.H)) {
+            const LaPixel q = la_load(in, normals, pbr, (size_t)gy * A.W + gx);
+            v = q.c;
+            gd = make_float4(q.n.x, q.n.y, q.n.z, q.z);
+        }
+        sIn[i] = v;
+        sGuide[i] = gd;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int x = blockIdx.x * kLaBlock + tx, y = blockIdx.y * kLaBlock + ty;
+    if (x >= A.W || y >= A.H) return;
+    const size_t e = (size_t)y * A.W + x;
+    const int lc = (ty + h) * side + (tx + h);
+    LaPixel p;
+    p.c = sIn[lc];
+    const float4 gc = sGuide[lc];
+    p.n = mk(gc.x, gc.y, gc.z);
+    p.z = gc.w;
+    p.rough = pbr[e].y;
+    const int mode = la_mode(p);
+    float4 o = p.c;
+    float sw = 1.0f;
+    if (mode == 2) {
+        o = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else if (mode == 0) {
+        const float kv[2][2] = {{1.0f / 4.0f, 1.0f / 8.0f}, {1.0f / 8.0f, 1.0f / 16.0f}};
+        float var = 0.0f;  // out-of-image texels are staged as 0, as la_phi skips them
+        for (int yy = -1; yy <= 1; ++yy)
+            for (int xx = -1; xx <= 1; ++xx)
+                if (la_inside(x + xx, y + yy, A.W, A.H)) var += sIn[lc + yy * side + xx].w * kv[abs(xx)][abs(yy)];
+        const float phi = A.phiColor * sqrtf(fmaxf(0.0f, var)) + 1e-10f;
+        const float lp = la_luminance(p.c.x, p.c.y, p.c.z), invPhi = 1.0f / phi;
+        const float sdv = fminf(sqrtf(fmaxf(var, 0.0f)), 4.0f);
+        float4 s = p.c;
+        for (int yy = -A.radius; yy <= A.radius; ++yy)
+            for (int xx = -A.radius; xx <= A.radius; ++xx) {
+                const int px = x + xx * A.stepSize, py = y + yy * A.stepSize;
+                if (!la_inside(px, py, A.W, A.H) || (xx == 0 && yy == 0)) continue;
+                const int l = (py - oy) * side + (px - ox);
+                LaPixel q;
+                q.c = sIn[l];
+                const float4 gq = sGuide[l];
+                q.n = mk(gq.x, gq.y, gq.z);
+                q.z = gq.w;
+                const LaFeatures F = la_features(A, p, lp, invPhi, sdv, q, xx, yy);
+                float hh[kLaHidden];
+                bool cl;
+                const float w = la_mlp(sP, F, hh, cl);
+                sw += w;
+                s.x += w * q.c.x;
+                s.y += w * q.c.y;
+                s.z += w * q.c.z;
+                s.w += w * w * q.c.w;
+            }
+        o = make_float4(s.x / sw, s.y / sw, s.z / sw, s.w / (sw * sw));
+    }
+    out[e] = o;
+    if (sumW) sumW[e] = sw;
+}
+
 // Backward of one pass. Per pixel x: as a centre, the parameter gradient of its taps; as a tap of
 // its neighbours (and as its own centre), dL/d(in rgb).
 __global__ __launch_bounds__(256) void learned_atrous_backward_kernel(
@@ -275,9 +358,16 @@ extern "C" int luxgi_svgf_learned_atrous(const LuxgiLearnedAtrousParams* p, cons
     const int rc = la_args(p, W, H, gb, A);
     if (rc != LUXGI_OK) return rc;
     if (!weights || !in || !out || in == out) return LUXGI_ERR_INVALID_ARGUMENT;
-    hipLaunchKernelGGL(learned_atrous_kernel, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, as_stream(stream), A,
-                       weights, reinterpret_cast<const float4*>(gb->normals), reinterpret_cast<const float4*>(gb->pbr),
-                       reinterpret_cast<const float4*>(in), reinterpret_cast<float4*>(out), sumW);
+    if (A.stepSize <= kLaMaxHalo && A.radius * A.stepSize <= kLaMaxHalo)
+        hipLaunchKernelGGL(learned_atrous_lds_kernel, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0,
+                           as_stream(stream), A, weights, reinterpret_cast<const float4*>(gb->normals),
+                           reinterpret_cast<const float4*>(gb->pbr), reinterpret_cast<const float4*>(in),
+                           reinterpret_cast<float4*>(out), sumW);
+    else
+        hipLaunchKernelGGL(learned_atrous_kernel, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, as_stream(stream),
+                           A, weights, reinterpret_cast<const float4*>(gb->normals),
+                           reinterpret_cast<const float4*>(gb->pbr), reinterpret_cast<const float4*>(in),
+                           reinterpret_cast<float4*>(out), sumW);
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }

@@ -43,8 +43,9 @@ def _forward(p, wts, gbd, img):
     return tw, ti, out, sw
 
 
+# footprints radius * step <= 8 run the LDS-tiled kernel, larger ones the direct kernel
 @pytest.mark.parametrize("W,H,radius,step,seed", [(77, 45, 1, 1, 0), (128, 64, 2, 2, 1), (40, 33, 1, 8, 2),
-                                                  (9, 7, 2, 1, 3)])
+                                                  (9, 7, 2, 1, 3), (50, 40, 1, 16, 4), (60, 44, 2, 8, 5)])
 def test_forward_matches_oracle(W, H, radius, step, seed):
     nrm, pbr, img = _inputs(W, H, seed=seed)
     wts = _random_weights(seed)

@@ -150,8 +150,8 @@ def bench_svgf_batched(args, prims_dev, n_prims, sdf_data, sdf, res, q):
 
 def bench_ddgi_batched(args, ds, q):
     """Eight probe volumes of uneven probe and ray counts over the bench scene: per-volume updates on
-    one stream, on four streams, and one graph replay per frame (orientations read from device
-    memory)."""
+    one stream and on four streams, all volumes traced in one launch sequence (blends on four
+    streams), and that frame as one graph replay (orientations read from device memory)."""
     import numpy as np
     import torch
 
@@ -161,22 +161,24 @@ def bench_ddgi_batched(args, ds, q):
               ((10, 5, 10), 96), ((4, 4, 4), 256), ((14, 7, 14), 128)]
     unis = [abi.make_uniform(c, (-1.0, -1.0, -1.0), [2.0 / (n - 1) for n in c], r, irr_oct=8, depth_oct=16)
             for c, r in shapes]
-    serial = DDGIBatch(unis, ds, streams=0)
+    serial = DDGIBatch(unis, ds, streams=0, trace="per_volume")
+    streamed = DDGIBatch(unis, ds, trace="per_volume")
     eager = DDGIBatch(unis, ds)
     graphed = DDGIBatch(unis, ds)
     rng = np.random.default_rng(99)
     frames = [[scene.random_rotation(int(s)) for s in rng.integers(0, 2 ** 31, size=len(unis))]
               for _ in range(args.warmup + args.steps + 1)]
-    for b in (serial, eager, graphed):
+    for b in (serial, streamed, eager, graphed):
         b.update(frames[0])
     graphed.capture()
     for i in range(1, args.warmup + 1):
-        for b in (serial, eager, graphed):
+        for b in (serial, streamed, eager, graphed):
             b.update(frames[i])
     torch.cuda.synchronize()
     out = {"volumes": len(unis), "rays": int(sum(u.num_probes * u.raysPerProbe for u in unis)),
            "shapes": [[list(c), r] for c, r in shapes]}
-    for name, b in (("per_volume_one_stream_ms", serial), ("per_volume_ms", eager), ("graph_replay_ms", graphed)):
+    for name, b in (("per_volume_one_stream_ms", serial), ("per_volume_4_streams_ms", streamed),
+                    ("one_trace_launch_ms", eager), ("graph_replay_ms", graphed)):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         ev[0].record()
         for i in range(args.warmup + 1, args.warmup + 1 + args.steps):
@@ -185,12 +187,14 @@ def bench_ddgi_batched(args, ds, q):
         torch.cuda.synchronize()
         out[name] = ev[0].elapsed_time(ev[1]) / args.steps
     same = all(torch.equal(a.current_irradiance, b.current_irradiance) and
-               torch.equal(a.current_irradiance, c.current_irradiance)
-               for a, b, c in zip(eager.volumes, graphed.volumes, serial.volumes))
+               torch.equal(a.current_irradiance, c.current_irradiance) and
+               torch.equal(a.current_irradiance, d.current_irradiance)
+               for a, b, c, d in zip(eager.volumes, graphed.volumes, serial.volumes, streamed.volumes))
     out["graph_equals_per_volume"] = bool(same)
     out["speedup_graph_vs_one_stream"] = out["per_volume_one_stream_ms"] / out["graph_replay_ms"]
     log(f"[bench] ddgi batched {len(unis)} volumes: one stream {out['per_volume_one_stream_ms']:.3f} ms, "
-        f"4 streams {out['per_volume_ms']:.3f} ms, graph {out['graph_replay_ms']:.3f} ms", q)
+        f"4 streams {out['per_volume_4_streams_ms']:.3f} ms, one trace launch {out['one_trace_launch_ms']:.3f} ms, "
+        f"graph {out['graph_replay_ms']:.3f} ms", q)
     return out
 
 def main():

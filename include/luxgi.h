@@ -701,6 +701,44 @@ int luxgi_sdf_reflection(const LuxgiSdfReflectionParams* params, const LuxgiBlue
                          void* workspace, size_t workspaceBytes, void* stream);
 
 /* ------------------------------------------------------------------------- */
+/* Several probe volumes in one trace (no counterpart in the reference)        */
+/* ------------------------------------------------------------------------- */
+
+#define LUXGI_MAX_TRACE_BATCH 256
+
+/* One probe volume of a batched trace: the per-volume arguments of luxgi_ddgi_trace_sdf. */
+typedef struct LuxgiTraceVolume {
+    LuxgiDDGIUniform uniform;
+    float    orientation[16];          /* randomOrientation of this volume (column-major) */
+    const float* deviceOrientation;    /* optional, as LuxgiTraceParams.deviceOrientation */
+    const float* probeOffsets;         /* optional float4[P] */
+    LuxgiRayBuffers rays;
+    uint8_t* rayFlags;                 /* optional uint8[P*width] */
+    int64_t  rayStart;                 /* first ray in the batch's flat ray list; set by
+                                          luxgi_ddgi_trace_batch_prepare */
+} LuxgiTraceVolume;                    /* 224 B */
+
+/* Host-side: checks `count` host descriptors (grid, ray counts against the ray buffers, one ray
+ * storage `rayStorage` for all), sets each rayStart and returns the total ray count. No GPU call. */
+int luxgi_ddgi_trace_batch_prepare(LuxgiTraceVolume* volumes, int32_t count, int32_t rayStorage,
+                                   int64_t* totalRays);
+
+/* Workspace bytes of luxgi_ddgi_trace_sdf_batched for `totalRays` rays. */
+size_t luxgi_ddgi_trace_batch_workspace_size(int64_t totalRays, int32_t objectCapacity);
+
+/* luxgi_ddgi_trace_sdf for several volumes over one scene in one launch sequence: the march, the
+ * hit sort and the shade each run once over the flat list of all volumes' rays, each ray finding
+ * its volume by binary search over rayStart. Every volume's outputs are identical to a
+ * luxgi_ddgi_trace_sdf call on it. `volumes` is a device array of `count` prepared descriptors
+ * (read when the kernels run); `params` gives the shared settings (maxSteps, hitEpsilon,
+ * shadeOrder, skyColor; its orientations are ignored). */
+int luxgi_ddgi_trace_sdf_batched(const LuxgiTraceVolume* volumes, int32_t count, int64_t totalRays,
+                                 int32_t rayStorage, const LuxgiGlobalSDFData* sdfData, const LuxgiSDFVolume* volume,
+                                 const LuxgiSurfaceAtlas* atlas, const LuxgiSkybox* sky,
+                                 const LuxgiTraceParams* params, void* workspace, size_t workspaceBytes,
+                                 void* stream);
+
+/* ------------------------------------------------------------------------- */
 /* Batched reflection denoise (no counterpart in the reference)               */
 /* ------------------------------------------------------------------------- */
 

@@ -211,6 +211,15 @@ class LearnedAtrousParams(C.Structure):  # include/luxgi.h LuxgiLearnedAtrousPar
 
 LEARNED_ATROUS_PARAMS = 81
 
+MAX_TRACE_BATCH = 256
+
+
+class TraceVolume(C.Structure):  # include/luxgi.h LuxgiTraceVolume
+    _fields_ = [("uniform", DDGIUniform), ("orientation", C.c_float * 16), ("deviceOrientation", C.c_void_p),
+                ("probeOffsets", C.c_void_p), ("rays", RayBuffers), ("rayFlags", C.c_void_p),
+                ("rayStart", C.c_int64)]
+
+
 MAX_DENOISE_BATCH = 1024
 
 
@@ -229,7 +238,8 @@ _SIZES = {DDGIUniform: 96, GlobalSDFData: 96, SurfaceAtlasData: 32, AtlasObject:
           ShadowReprojectParams: 80, ShadowAtrousParams: 32, SampleProbeParams: 80, Light: 64,
           SurfaceGBuffer: 40, SurfaceLightParams: 32, SurfaceDraw: 8, BlueNoise: 16, SdfShadowParams: 144,
           SdfReflectionParams: 144, ObjectRasterizeData: 176, RasterizeParams: 48, MeshSDF: 72,
-          LearnedAtrousParams: 32, ReflectionDenoiseItem: 336, DenoiseBatchHeader: 16}
+          LearnedAtrousParams: 32, ReflectionDenoiseItem: 336, DenoiseBatchHeader: 16,
+          TraceVolume: 224}
 for _t, _n in _SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
 

@@ -186,13 +186,21 @@ class DDGIBatch:
     Per frame only the volumes' ray orientations change (the ping-pong parity alternates and the
     first frame is special). So the first frame runs eagerly, capture() records one graph per
     parity, and update() copies the new orientations into a device array that the traces read
-    at run time (LuxgiTraceParams.deviceOrientation) before replaying the parity's graph."""
+    at run time (deviceOrientation) before replaying the parity's graph.
 
-    def __init__(self, uniforms, scene, trace_params=None, fuse_border=True, streams=4):
+    trace="batched" (default) traces all volumes in one launch sequence over their flat ray list
+    (luxgi_ddgi_trace_sdf_batched); trace="per_volume" calls luxgi_ddgi_trace_sdf per volume. The
+    blends run per volume; with `streams` > 0 the per-volume work is spread over that many side
+    streams (default priority), forked from and joined to the current stream."""
+
+    def __init__(self, uniforms, scene, trace_params=None, fuse_border=True, streams=4, trace="batched"):
         from .ddgi import DDGIVolume, default_trace_params
+        if trace not in ("batched", "per_volume"):
+            raise ValueError("trace is 'batched' or 'per_volume'")
         self.volumes = [DDGIVolume(u) for u in uniforms]
         self.scene = scene
         self.fuse_border = fuse_border
+        self.trace_mode = trace
         # the volumes are independent: volume i runs on side stream i % streams (default priority),
         # forked from and joined back to the current stream, so small volumes overlap on the GPU
         # (and become parallel branches of the captured graph)
@@ -208,6 +216,26 @@ class DDGIBatch:
             p.deviceOrientation = self.orientation[i].data_ptr()
             self.params.append(p)
         self.graphs = [None, None]
+        self.descriptors = None
+        if trace == "batched":
+            if k > abi.MAX_TRACE_BATCH:
+                raise ValueError(f"at most {abi.MAX_TRACE_BATCH} volumes per batched trace")
+            d = (abi.TraceVolume * k)()
+            for i, vol in enumerate(self.volumes):
+                d[i].uniform = vol.u
+                d[i].deviceOrientation = self.orientation[i].data_ptr()
+                d[i].rays = vol.rays
+            total = C.c_int64(0)
+            check(lib().luxgi_ddgi_trace_batch_prepare(C.byref(d), k, self.volumes[0].ray_storage, C.byref(total)),
+                  "luxgi_ddgi_trace_batch_prepare")
+            self.total_rays = total.value
+            # the descriptors do not change from frame to frame (the orientations are read through
+            # deviceOrientation), so they are copied to the device once
+            self.descriptors = torch.from_numpy(
+                np.frombuffer(memoryview(d).cast("B"), dtype=np.uint8).copy()).cuda()
+            ws = lib().luxgi_ddgi_trace_batch_workspace_size(self.total_rays, scene.n_objects)
+            self.trace_workspace = torch.empty(max(ws, 256), dtype=torch.uint8, device="cuda")
+            self.shared_params = self.params[0]
 
     @property
     def frames(self):
@@ -230,13 +258,24 @@ class DDGIBatch:
         ev.record()
         self._staged[j] = ev
 
+    def _trace_all(self):
+        sc = self.scene
+        check(lib().luxgi_ddgi_trace_sdf_batched(
+            C.c_void_p(self.descriptors.data_ptr()), len(self.volumes), self.total_rays, self.volumes[0].ray_storage,
+            C.byref(sc.sdf_data), C.byref(sc.volume), C.byref(sc.atlas), C.byref(sc.skybox),
+            C.byref(self.shared_params), C.c_void_p(self.trace_workspace.data_ptr()), self.trace_workspace.numel(),
+            stream_handle()), "luxgi_ddgi_trace_sdf_batched")
+
     def _run_volume(self, vol, p):
-        vol.trace_rays(self.scene, p)
+        if self.trace_mode == "per_volume":
+            vol.trace_rays(self.scene, p)
         vol.probe_update(fuse_border=self.fuse_border)
         if not self.fuse_border:
             vol.border_update()
 
     def _run(self):
+        if self.trace_mode == "batched":
+            self._trace_all()
         if not self.streams:
             for vol, p in zip(self.volumes, self.params):
                 self._run_volume(vol, p)

@@ -23,11 +23,8 @@ namespace luxgi {
 constexpr float kWorldSize = 60000.0f;  // GLOBAL_SDF_WORLD_SIZE, SDFCommon.glsl:6
 
 struct TraceArgs {
-    LuxgiDDGIUniform u;
     LuxgiGlobalSDFData g;
     LuxgiSurfaceAtlasData ad;
-    float rot[9];  // column-major mat3 of randomOrientation
-    const float* rotDev;  // or, when set, the 4x4 column-major orientation in device memory
     float sky[3];
     int res, mipRes, maxSteps, skySize, P, R, rayWidth;
     float hitEps;
@@ -36,6 +33,10 @@ struct TraceArgs {
     const float* refine;      // optional FP32 fine volume re-evaluated at hits
     int refineLayout;
     int sortHits;             // shade the hit rays grouped by surface-atlas chunk
+    LuxgiTraceVolume vol;     // the volume of a single-volume call
+    const LuxgiTraceVolume* vols;  // the volumes of a batched call (device memory), their count
+    int nVols;                     // and the total of their rays
+    long long totalRays;
 };
 
 constexpr int kChunkBins = kChunksRes * kChunksRes * kChunksRes;
@@ -78,25 +79,38 @@ __device__ __forceinline__ long long xcd_block(long long b, long long nb) {
     return (b % kXcds) * (nb / kXcds) + b / kXcds;
 }
 
-__device__ __forceinline__ void ray_setup(const TraceArgs& A, const float* __restrict__ probeOffsets, int probe,
-                                          int ray, f3& origin, f3& dir) {
+__device__ __forceinline__ void ray_setup(const TraceArgs& A, const LuxgiTraceVolume& v, int probe, int ray,
+                                          f3& origin, f3& dir) {
     // probeLocation (DDGICommon.glsl:106-114) + GISDFRays.comp:72-73
-    const int px = A.u.probeCounts[0], pxy = A.u.probeCounts[0] * A.u.probeCounts[1];
+    const LuxgiDDGIUniform& u = v.uniform;
+    const int px = u.probeCounts[0], pxy = u.probeCounts[0] * u.probeCounts[1];
     const int gx = probe % px, gy = (probe % pxy) / px, gz = probe / pxy;
-    origin = mk(A.u.step[0] * float(gx) + A.u.startPosition[0], A.u.step[1] * float(gy) + A.u.startPosition[1],
-                A.u.step[2] * float(gz) + A.u.startPosition[2]);
-    if (probeOffsets) {
-        const float4 o = reinterpret_cast<const float4*>(probeOffsets)[probe];
+    origin = mk(u.step[0] * float(gx) + u.startPosition[0], u.step[1] * float(gy) + u.startPosition[1],
+                u.step[2] * float(gz) + u.startPosition[2]);
+    if (v.probeOffsets) {
+        const float4 o = reinterpret_cast<const float4*>(v.probeOffsets)[probe];
         origin = origin + mk(o.x, o.y, o.z);
     }
-    const f3 sf = spherical_fibonacci(float(ray), float(A.R));
+    const f3 sf = spherical_fibonacci(float(ray), float(u.raysPerProbe));
+    const float* rot = v.deviceOrientation ? v.deviceOrientation : v.orientation;  // column-major 4x4
     float m[9];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int r = 0; r < 3; ++r) m[c * 3 + r] = A.rotDev ? A.rotDev[c * 4 + r] : A.rot[c * 3 + r];
+        for (int r = 0; r < 3; ++r) m[c * 3 + r] = rot[c * 4 + r];
     dir = normalize3(mk(m[0] * sf.x + m[3] * sf.y + m[6] * sf.z, m[1] * sf.x + m[4] * sf.y + m[7] * sf.z,
                         m[2] * sf.x + m[5] * sf.y + m[8] * sf.z));
+}
+
+// the volume of flat ray `gid` in a batch: the last whose rayStart <= gid
+__device__ __forceinline__ int trace_volume_of(const LuxgiTraceVolume* __restrict__ vols, int n, long long gid) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (vols[mid].rayStart <= gid) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 // fine-volume lookup in the volume's layout
@@ -171,9 +185,9 @@ __device__ __forceinline__ void march_enter(const TraceArgs& A, MarchRay& r, flo
     r.step = 0;
 }
 
-__device__ __forceinline__ void march_begin(const TraceArgs& A, const float* __restrict__ probeOffsets, int probe,
-                                            int ray, float traceMax, MarchRay& r) {
-    ray_setup(A, probeOffsets, probe, ray, r.o, r.d);
+__device__ __forceinline__ void march_begin(const TraceArgs& A, const LuxgiTraceVolume& v, int probe, int ray,
+                                            float traceMax, MarchRay& r) {
+    ray_setup(A, v, probe, ray, r.o, r.d);
     r.traceEnd = r.o + r.d * traceMax;
     r.nextStart = 0.0f;
     r.cas = 0;
@@ -242,12 +256,10 @@ __device__ __forceinline__ bool march_step(const TraceArgs& A, const SS* __restr
 // What a finished ray leaves: its hit record and, with the hit sort, either its final result (no
 // atlas lookup: miss or inside hit) or its atlas chunk (returned; kNoChunk otherwise).
 template <class SR>
-__device__ __forceinline__ uint32_t march_finish(const TraceArgs& A, const MarchRay& r, float4 ra, float2 rb,
-                                                 int probe, int ray, float4* __restrict__ recA,
-                                                 float2* __restrict__ recB, const float* __restrict__ skyFaces,
-                                                 SR* __restrict__ outRad, SR* __restrict__ outDD,
-                                                 uint8_t* __restrict__ flags) {
-    const size_t rid = (size_t)probe * A.R + ray;
+__device__ __forceinline__ uint32_t march_finish(const TraceArgs& A, const LuxgiTraceVolume& v, const MarchRay& r,
+                                                 float4 ra, float2 rb, int probe, int ray, size_t rid,
+                                                 float4* __restrict__ recA, float2* __restrict__ recB,
+                                                 const float* __restrict__ skyFaces) {
     recA[rid] = ra;
     recB[rid] = rb;
     uint32_t key = kNoChunk;
@@ -262,7 +274,8 @@ __device__ __forceinline__ uint32_t march_finish(const TraceArgs& A, const March
                 cc[i] = clampi(int(floorf(pc[i] / A.ad.chunkSize + float(kChunksRes) * 0.5f)), 0, kChunksRes - 1);
             key = (uint32_t)((cc[2] * kChunksRes + cc[1]) * kChunksRes + cc[0]);
         } else {
-            store_nonhit<SR>(A, skyFaces, outRad, outDD, flags, (size_t)probe * A.rayWidth + ray, r.d, inside);
+            store_nonhit<SR>(A, skyFaces, static_cast<SR*>(v.rays.radiance), static_cast<SR*>(v.rays.directionDistance),
+                             v.rayFlags, (size_t)probe * v.rays.width + ray, r.d, inside);
         }
     }
     return key;
@@ -272,46 +285,70 @@ __device__ __forceinline__ float trace_max_distance(const TraceArgs& A) {
     return fminf(kWorldSize, A.g.cascadePosDistance[(int)A.g.cascadesCount - 1][3] * 2.0f);
 }
 
-// Sphere trace of one ray per lane (SDFCommon.glsl:98-171). Writes the hit record; with the hit
-// sort it also writes the final result of the rays without an atlas lookup and the (chunk, slot)
-// of the others.
+// The sphere trace of one ray (SDFCommon.glsl:98-171) through its cascades; returns its hit chunk
+// (kNoChunk when it needs no atlas lookup).
 template <class SS, class SR, bool COUNT, bool BRICK>
+__device__ __forceinline__ uint32_t march_ray(const TraceArgs& A, const LuxgiTraceVolume& v, int probe, int ray,
+                                              size_t rid, const SS* __restrict__ sdf, const SS* __restrict__ mip,
+                                              const float* __restrict__ guard, float4* __restrict__ recA,
+                                              float2* __restrict__ recB, const float* __restrict__ skyFaces,
+                                              unsigned long long& cMip, unsigned long long& cFine,
+                                              unsigned long long& cSteps) {
+    const int C = (int)A.g.cascadesCount;
+    const float traceMax = trace_max_distance(A);
+    MarchRay r;
+    march_begin(A, v, probe, ray, traceMax, r);
+    float4 ra = make_float4(-1.0f, 0.f, 0.f, 0.f);
+    float2 rb = make_float2(0.f, 0.f);
+    for (;;) {
+        int step = 0;
+        bool hit = false;
+        for (; step < A.maxSteps && r.stepTime < r.iy; ++step)
+            if (march_step<SS, BRICK, COUNT>(A, sdf, mip, guard, r, ra, rb, cMip, cFine)) {
+                hit = true;
+                break;
+            }
+        if (COUNT) cSteps += step;
+        if (hit || ++r.cas >= C) break;
+        march_enter(A, r, traceMax);
+    }
+    return march_finish<SR>(A, v, r, ra, rb, probe, ray, rid, recA, recB, skyFaces);
+}
+
+// Sphere trace of one ray per lane. Writes the hit record; with the hit sort it also writes the
+// final result of the rays without an atlas lookup and the (chunk, slot) of the others. Single
+// volume: 64 rays of one probe per wave (rays padded to a multiple of 64 per probe). BATCH: the
+// flat ray list of all volumes, each ray finding its volume by rayStart.
+template <class SS, class SR, bool COUNT, bool BRICK, bool BATCH>
 __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __restrict__ sdf, const SS* __restrict__ mip,
-                                                    const float* __restrict__ guard,
-                                                    const float* __restrict__ probeOffsets, float4* __restrict__ recA,
+                                                    const float* __restrict__ guard, float4* __restrict__ recA,
                                                     float2* __restrict__ recB, uint2* __restrict__ hitKeys,
                                                     uint32_t* __restrict__ bins, const float* __restrict__ skyFaces,
-                                                    SR* __restrict__ outRad, SR* __restrict__ outDD,
-                                                    uint8_t* __restrict__ flags,
                                                     unsigned long long* __restrict__ counters) {
-    const int Rp = (A.R + 63) & ~63;
     const long long gb = xcd_block(blockIdx.x, gridDim.x);
     const long long gid = gb * blockDim.x + threadIdx.x;
-    const int probe = int(gid / Rp), ray = int(gid % Rp);
     unsigned long long cMip = 0, cFine = 0, cSteps = 0;
     uint32_t key = kNoChunk;
-    const bool valid = probe < A.P && ray < A.R;
-    const size_t rid = (size_t)probe * A.R + ray;
-    if (valid) {
-        const int C = (int)A.g.cascadesCount;
-        const float traceMax = trace_max_distance(A);
-        MarchRay r;
-        march_begin(A, probeOffsets, probe, ray, traceMax, r);
-        float4 ra = make_float4(-1.0f, 0.f, 0.f, 0.f);
-        float2 rb = make_float2(0.f, 0.f);
-        for (;;) {
-            int step = 0;
-            bool hit = false;
-            for (; step < A.maxSteps && r.stepTime < r.iy; ++step)
-                if (march_step<SS, BRICK, COUNT>(A, sdf, mip, guard, r, ra, rb, cMip, cFine)) {
-                    hit = true;
-                    break;
-                }
-            if (COUNT) cSteps += step;
-            if (hit || ++r.cas >= C) break;
-            march_enter(A, r, traceMax);
+    bool valid;
+    size_t rid;
+    if constexpr (BATCH) {
+        valid = gid < A.totalRays;
+        rid = (size_t)gid;
+        if (valid) {
+            const LuxgiTraceVolume& v = A.vols[trace_volume_of(A.vols, A.nVols, gid)];
+            const int R = v.uniform.raysPerProbe;
+            const long long local = gid - v.rayStart;
+            key = march_ray<SS, SR, COUNT, BRICK>(A, v, int(local / R), int(local % R), rid, sdf, mip, guard, recA,
+                                                  recB, skyFaces, cMip, cFine, cSteps);
         }
-        key = march_finish<SR>(A, r, ra, rb, probe, ray, recA, recB, skyFaces, outRad, outDD, flags);
+    } else {
+        const int Rp = (A.R + 63) & ~63;
+        const int probe = int(gid / Rp), ray = int(gid % Rp);
+        valid = probe < A.P && ray < A.R;
+        rid = (size_t)probe * A.R + ray;
+        if (valid)
+            key = march_ray<SS, SR, COUNT, BRICK>(A, A.vol, probe, ray, rid, sdf, mip, guard, recA, recB, skyFaces,
+                                                  cMip, cFine, cSteps);
     }
     if (A.sortHits) {  // chunk histogram; each hit keeps its slot within its chunk
         const uint32_t slot = hist_add(bins, key, key != kNoChunk);
@@ -334,8 +371,64 @@ __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __res
 }
 
 // Normal, surface-atlas lookup and ray-result encode (SDFCommon.glsl:172-184,
-// GISDFRays.comp:89-127) for one ray per lane, 64 rays of a probe per wave.
+// GISDFRays.comp:89-127) for one ray.
 template <class SS, class SC, class SR, bool COUNT>
+__device__ __forceinline__ void shade_ray(const TraceArgs& A, const LuxgiTraceVolume& v, int probe, int ray,
+                                          size_t rid, const SS* __restrict__ sdf, const uint32_t* __restrict__ chunks,
+                                          const uint32_t* __restrict__ culled,
+                                          const LuxgiAtlasObject* __restrict__ objects,
+                                          const float* __restrict__ objInv, const LuxgiAtlasTile* __restrict__ tiles,
+                                          const SC* __restrict__ color, const float* __restrict__ atlasDepth,
+                                          const float* __restrict__ skyFaces, const float4* __restrict__ recA,
+                                          const float2* __restrict__ recB, Counts& c) {
+    SR* outRad = static_cast<SR*>(v.rays.radiance);
+    SR* outDD = static_cast<SR*>(v.rays.directionDistance);
+    uint8_t* flags = v.rayFlags;
+    f3 origin, dir;
+    ray_setup(A, v, probe, ray, origin, dir);
+    const float4 ra = recA[rid];
+    const float hitTime = ra.x;
+    const size_t e = (size_t)probe * v.rays.width + ray;
+    if (hitTime >= 0.0f) {
+        const float2 rb = recB[rid];
+        const float hitSdf = rb.x;
+        const int hitCascade = int(rb.y);
+        if (COUNT) c.hits++;
+        if (hitSdf <= 0.0f && hitTime <= A.g.cascadeVoxelSize[0]) {
+            store_nonhit<SR>(A, skyFaces, outRad, outDD, flags, e, dir, true);
+        } else {
+            const int C = (int)A.g.cascadesCount;
+            const int texW = A.res * C;
+            const float o = 1.0f / A.g.resolution;
+            const float tx = ra.y, uy = ra.z, uz = ra.w;
+            auto nl = [&](float u_, float v_, float w_) {
+                if (A.refine) return refine_lookup(A, texW, A.res, A.res, u_, v_, w_);
+                return A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, u_, v_, w_, A.brickScale)
+                                    : sample_volume<SS>(sdf, texW, A.res, A.res, u_, v_, w_);
+            };
+            const float xp = nl(tx + o, uy, uz), xn = nl(tx - o, uy, uz);
+            const float yp = nl(tx, uy + o, uz), yn = nl(tx, uy - o, uz);
+            const float zp = nl(tx, uy, uz + o), zn = nl(tx, uy, uz - o);
+            const f3 n = normalize3(mk(xp - xn, yp - yn, zp - zn));
+            if (COUNT) c.normals++;
+            const f3 hp = origin + dir * hitTime;
+            const float thr = A.g.cascadeVoxelSize[hitCascade] * 1.05f;
+            const float4 s = sample_atlas<SC, COUNT>(A.ad, chunks, culled, objects, objInv, tiles, color, atlasDepth, hp,
+                                                     n, thr, c);
+            const float dist = fmaxf(hitTime + A.g.cascadeVoxelSize[hitCascade] * 0.5f, 0.0f);
+            Texel<SR>::st4(outRad, e, make_float4(s.x, s.y, s.z, 0.f));
+            Texel<SR>::st4(outDD, e, make_float4(dir.x, dir.y, dir.z, dist));
+            if (flags) flags[e] = 1;
+        }
+    } else {
+        store_nonhit<SR>(A, skyFaces, outRad, outDD, flags, e, dir, false);
+    }
+}
+
+// One ray per lane: the sorted front hits (`order`, grouped by atlas chunk, so the object and tile
+// loops of a wave run in step), or every ray (single volume: 64 rays of a probe per wave; BATCH:
+// the flat ray list).
+template <class SS, class SC, class SR, bool COUNT, bool BATCH>
 __global__ __launch_bounds__(256) void shade_kernel(TraceArgs A, const SS* __restrict__ sdf,
                                                     const uint32_t* __restrict__ chunks,
                                                     const uint32_t* __restrict__ culled,
@@ -344,67 +437,41 @@ __global__ __launch_bounds__(256) void shade_kernel(TraceArgs A, const SS* __res
                                                     const LuxgiAtlasTile* __restrict__ tiles,
                                                     const SC* __restrict__ color, const float* __restrict__ atlasDepth,
                                                     const float* __restrict__ skyFaces,
-                                                    const float* __restrict__ probeOffsets,
                                                     const float4* __restrict__ recA, const float2* __restrict__ recB,
-                                                    SR* __restrict__ outRad, SR* __restrict__ outDD,
-                                                    uint8_t* __restrict__ flags, const uint32_t* __restrict__ order,
+                                                    const uint32_t* __restrict__ order,
                                                     const uint32_t* __restrict__ hitCount,
                                                     unsigned long long* __restrict__ counters) {
     const long long gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    int probe, ray;
-    if (order) {  // the front hits grouped by atlas chunk: the object and tile loops of a wave run in step
-        const long long n = *hitCount;
-        const uint32_t rid = gid < n ? order[gid] : 0u;
-        probe = gid < n ? int(rid / (uint32_t)A.R) : A.P;
-        ray = int(rid % (uint32_t)A.R);
+    Counts c = {0, 0, 0, 0, 0, 0, 0};
+    bool active;
+    if (BATCH || order) {  // a flat ray id
+        long long rid = -1;
+        if (order) {
+            if (gid < (long long)*hitCount) rid = order[gid];
+        } else if (gid < A.totalRays) {
+            rid = gid;
+        }
+        active = rid >= 0;
+        if (active) {
+            if constexpr (BATCH) {
+                const LuxgiTraceVolume& v = A.vols[trace_volume_of(A.vols, A.nVols, rid)];
+                const int R = v.uniform.raysPerProbe;
+                const long long local = rid - v.rayStart;
+                shade_ray<SS, SC, SR, COUNT>(A, v, int(local / R), int(local % R), (size_t)rid, sdf, chunks, culled,
+                                             objects, objInv, tiles, color, atlasDepth, skyFaces, recA, recB, c);
+            } else {
+                shade_ray<SS, SC, SR, COUNT>(A, A.vol, int(rid / A.R), int(rid % A.R), (size_t)rid, sdf, chunks,
+                                             culled, objects, objInv, tiles, color, atlasDepth, skyFaces, recA, recB,
+                                             c);
+            }
+        }
     } else {
         const int Rp = (A.R + 63) & ~63;
-        probe = int(gid / Rp);
-        ray = int(gid % Rp);
-    }
-    const bool active = probe < A.P && ray < A.R;
-    Counts c = {0, 0, 0, 0, 0, 0, 0};
-    if (active) {
-        f3 origin, dir;
-        ray_setup(A, probeOffsets, probe, ray, origin, dir);
-        const size_t rid = (size_t)probe * A.R + ray;
-        const float4 ra = recA[rid];
-        const float hitTime = ra.x;
-        const size_t e = (size_t)probe * A.rayWidth + ray;
-        if (hitTime >= 0.0f) {
-            const float2 rb = recB[rid];
-            const float hitSdf = rb.x;
-            const int hitCascade = int(rb.y);
-            if (COUNT) c.hits++;
-            if (hitSdf <= 0.0f && hitTime <= A.g.cascadeVoxelSize[0]) {
-                store_nonhit<SR>(A, skyFaces, outRad, outDD, flags, e, dir, true);
-            } else {
-                const int C = (int)A.g.cascadesCount;
-                const int texW = A.res * C;
-                const float o = 1.0f / A.g.resolution;
-                const float tx = ra.y, uy = ra.z, uz = ra.w;
-                auto nl = [&](float u_, float v_, float w_) {
-                    if (A.refine) return refine_lookup(A, texW, A.res, A.res, u_, v_, w_);
-                    return A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, u_, v_, w_, A.brickScale)
-                                        : sample_volume<SS>(sdf, texW, A.res, A.res, u_, v_, w_);
-                };
-                const float xp = nl(tx + o, uy, uz), xn = nl(tx - o, uy, uz);
-                const float yp = nl(tx, uy + o, uz), yn = nl(tx, uy - o, uz);
-                const float zp = nl(tx, uy, uz + o), zn = nl(tx, uy, uz - o);
-                const f3 n = normalize3(mk(xp - xn, yp - yn, zp - zn));
-                if (COUNT) c.normals++;
-                const f3 hp = origin + dir * hitTime;
-                const float thr = A.g.cascadeVoxelSize[hitCascade] * 1.05f;
-                const float4 s = sample_atlas<SC, COUNT>(A.ad, chunks, culled, objects, objInv, tiles, color,
-                                                         atlasDepth, hp, n, thr, c);
-                const float dist = fmaxf(hitTime + A.g.cascadeVoxelSize[hitCascade] * 0.5f, 0.0f);
-                Texel<SR>::st4(outRad, e, make_float4(s.x, s.y, s.z, 0.f));
-                Texel<SR>::st4(outDD, e, make_float4(dir.x, dir.y, dir.z, dist));
-                if (flags) flags[e] = 1;
-            }
-        } else {
-            store_nonhit<SR>(A, skyFaces, outRad, outDD, flags, e, dir, false);
-        }
+        const int probe = int(gid / Rp), ray = int(gid % Rp);
+        active = probe < A.P && ray < A.R;
+        if (active)
+            shade_ray<SS, SC, SR, COUNT>(A, A.vol, probe, ray, (size_t)probe * A.R + ray, sdf, chunks, culled, objects,
+                                         objInv, tiles, color, atlasDepth, skyFaces, recA, recB, c);
     }
     if (COUNT) {
         __shared__ unsigned long long sc[8];
@@ -501,27 +568,24 @@ extern "C" size_t luxgi_ddgi_trace_workspace_size(const LuxgiDDGIUniform* u, int
     return trace_workspace_layout(P * R, objectCapacity, nullptr, nullptr);
 }
 
-template <class SS, class SC, class SR, bool COUNT>
+template <class SS, class SC, class SR, bool COUNT, bool BATCH>
 static int launch_trace(const TraceArgs& A, const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at,
-                        const LuxgiSkybox* sky, const float* probeOffsets, const LuxgiRayBuffers* rays,
-                        uint8_t* rayFlags, const TraceWorkspace& w, unsigned long long* counters, hipStream_t st) {
-    const long long lanes = (long long)A.P * ((A.R + 63) & ~63);
-    const long long nRays = (long long)A.P * A.R;
-    SR* outRad = static_cast<SR*>(rays->radiance);
-    SR* outDD = static_cast<SR*>(rays->directionDistance);
+                        const LuxgiSkybox* sky, const TraceWorkspace& w, unsigned long long* counters, hipStream_t st) {
+    const long long nRays = BATCH ? A.totalRays : (long long)A.P * A.R;
+    const long long lanes = BATCH ? nRays : (long long)A.P * ((A.R + 63) & ~63);
     const float* skyFaces = sky && sky->faces ? sky->faces : nullptr;
     if (A.sortHits) {
         hipLaunchKernelGGL(zero_bins_kernel, dim3((kChunkBins + 255) / 256), dim3(256), 0, st, w.bins, kChunkBins);
         LUXGI_LAUNCH_CHECK();
     }
     if (A.fineLayout == LUXGI_LAYOUT_BRICK4)
-        hipLaunchKernelGGL((march_kernel<SS, SR, COUNT, true>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st, A,
-                           static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), vol->guard, probeOffsets,
-                           w.recA, w.recB, w.keys, w.bins, skyFaces, outRad, outDD, rayFlags, counters);
+        hipLaunchKernelGGL((march_kernel<SS, SR, COUNT, true, BATCH>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st,
+                           A, static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), vol->guard, w.recA,
+                           w.recB, w.keys, w.bins, skyFaces, counters);
     else
-        hipLaunchKernelGGL((march_kernel<SS, SR, COUNT, false>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st, A,
-                           static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), vol->guard, probeOffsets,
-                           w.recA, w.recB, w.keys, w.bins, skyFaces, outRad, outDD, rayFlags, counters);
+        hipLaunchKernelGGL((march_kernel<SS, SR, COUNT, false, BATCH>), dim3((int)((lanes + 255) / 256)), dim3(256), 0,
+                           st, A, static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), vol->guard,
+                           w.recA, w.recB, w.keys, w.bins, skyFaces, counters);
     LUXGI_LAUNCH_CHECK();
     if (A.sortHits) {
         hipLaunchKernelGGL(block_scan_kernel<kScanBlock>, dim3(kScanBlocks), dim3(kScanBlock), 0, st, w.bins, w.bins,
@@ -533,35 +597,33 @@ static int launch_trace(const TraceArgs& A, const LuxgiSDFVolume* vol, const Lux
     }
     // sorted: one lane per ray covers every possible hit count; the lanes past the count exit
     const long long shadeLanes = A.sortHits ? nRays : lanes;
-    hipLaunchKernelGGL((shade_kernel<SS, SC, SR, COUNT>), dim3((int)((shadeLanes + 255) / 256)), dim3(256), 0, st, A,
-                       static_cast<const SS*>(vol->sdf), at->chunks, at->culledObjects, at->objects, w.inv, at->tiles,
-                       static_cast<const SC*>(at->color), at->depth, skyFaces, probeOffsets, w.recA, w.recB, outRad,
-                       outDD, rayFlags, A.sortHits ? w.order : nullptr, w.totals + kScanBlocks, counters);
+    hipLaunchKernelGGL((shade_kernel<SS, SC, SR, COUNT, BATCH>), dim3((int)((shadeLanes + 255) / 256)), dim3(256), 0, st,
+                       A, static_cast<const SS*>(vol->sdf), at->chunks, at->culledObjects, at->objects, w.inv,
+                       at->tiles, static_cast<const SC*>(at->color), at->depth, skyFaces, w.recA, w.recB,
+                       A.sortHits ? w.order : nullptr, w.totals + kScanBlocks, counters);
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }
 
-template <class SS, class SR, bool COUNT>
+template <class SS, class SR, bool COUNT, bool BATCH>
 static int dispatch_color(const TraceArgs& A, const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at,
-                          const LuxgiSkybox* sky, const float* off, const LuxgiRayBuffers* rays, uint8_t* fl,
-                          const TraceWorkspace& w, unsigned long long* cnt, hipStream_t st) {
+                          const LuxgiSkybox* sky, const TraceWorkspace& w, unsigned long long* cnt, hipStream_t st) {
     if (at->layout == LUXGI_LAYOUT_BRICK4)
-        return launch_trace<SS, PackedAtlas, SR, COUNT>(A, vol, at, sky, off, rays, fl, w, cnt, st);
+        return launch_trace<SS, PackedAtlas, SR, COUNT, BATCH>(A, vol, at, sky, w, cnt, st);
     switch (at->colorStorage) {
-        case LUXGI_STORAGE_FP32: return launch_trace<SS, float, SR, COUNT>(A, vol, at, sky, off, rays, fl, w, cnt, st);
-        case LUXGI_STORAGE_FP16: return launch_trace<SS, __half, SR, COUNT>(A, vol, at, sky, off, rays, fl, w, cnt, st);
-        case LUXGI_STORAGE_UNORM8: return launch_trace<SS, Unorm8, SR, COUNT>(A, vol, at, sky, off, rays, fl, w, cnt, st);
+        case LUXGI_STORAGE_FP32: return launch_trace<SS, float, SR, COUNT, BATCH>(A, vol, at, sky, w, cnt, st);
+        case LUXGI_STORAGE_FP16: return launch_trace<SS, __half, SR, COUNT, BATCH>(A, vol, at, sky, w, cnt, st);
+        case LUXGI_STORAGE_UNORM8: return launch_trace<SS, Unorm8, SR, COUNT, BATCH>(A, vol, at, sky, w, cnt, st);
         default: return LUXGI_ERR_UNSUPPORTED;
     }
 }
 
-template <bool COUNT>
-static int trace_entry(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, const LuxgiSDFVolume* vol,
-                       const LuxgiSurfaceAtlas* at, const LuxgiSkybox* sky, const LuxgiTraceParams* p,
-                       const float* probeOffsets, const LuxgiRayBuffers* rays, uint8_t* rayFlags, void* workspace,
-                       size_t workspaceBytes, unsigned long long* counters, void* stream) {
-    if (!u || !g || !vol || !at || !p || !rays) return LUXGI_ERR_INVALID_ARGUMENT;
-    if (!vol->sdf || !vol->mip || !rays->radiance || !rays->directionDistance) return LUXGI_ERR_INVALID_ARGUMENT;
+// The scene arguments shared by the single and batched traces: checks them and fills A's scene
+// part and shared settings.
+static int trace_scene_args(const LuxgiGlobalSDFData* g, const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at,
+                            const LuxgiSkybox* sky, const LuxgiTraceParams* p, bool count, TraceArgs& A) {
+    if (!g || !vol || !at || !p) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (!vol->sdf || !vol->mip) return LUXGI_ERR_INVALID_ARGUMENT;
     if (!at->chunks || !at->culledObjects || !at->objects || !at->tiles || !at->color) return LUXGI_ERR_INVALID_ARGUMENT;
     if (at->layout == LUXGI_LAYOUT_LINEAR) {
         if (!at->depth) return LUXGI_ERR_INVALID_ARGUMENT;
@@ -574,28 +636,20 @@ static int trace_entry(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, c
     if (vol->resolution <= 0 || vol->mipResolution <= 0) return LUXGI_ERR_INVALID_ARGUMENT;
     if (at->data.resolution == 0 || at->objectCapacity < 0) return LUXGI_ERR_INVALID_ARGUMENT;
     if (sky && sky->faces && sky->size <= 0) return LUXGI_ERR_INVALID_ARGUMENT;
-    TraceArgs A;
-    A.u = *u;
     A.g = *g;
     A.ad = at->data;
-    for (int c = 0; c < 3; ++c)
-        for (int r = 0; r < 3; ++r) A.rot[c * 3 + r] = p->randomOrientation[c * 4 + r];
-    A.rotDev = p->deviceOrientation;
     for (int i = 0; i < 3; ++i) A.sky[i] = p->skyColor[i];
     A.res = vol->resolution;
     A.mipRes = vol->mipResolution;
     A.maxSteps = p->maxSteps > 0 ? p->maxSteps : 250;
     A.skySize = sky ? sky->size : 0;
-    A.P = u->probeCounts[0] * u->probeCounts[1] * u->probeCounts[2];
-    A.R = u->raysPerProbe;
-    A.rayWidth = rays->width;
     A.hitEps = p->hitEpsilon;
     A.fineLayout = vol->layout;
     A.brickScale = vol->brickScale;
     A.refine = vol->refine;
     A.refineLayout = vol->refineLayout;
     if (p->shadeOrder != 0 && p->shadeOrder != 1) return LUXGI_ERR_INVALID_ARGUMENT;
-    A.sortHits = !COUNT && p->shadeOrder == 0;  // the counting run shades in ray order
+    A.sortHits = !count && p->shadeOrder == 0;  // the counting run shades in ray order
     if (vol->brickScale && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
     if (vol->refine && ((vol->refineLayout != LUXGI_LAYOUT_LINEAR && vol->refineLayout != LUXGI_LAYOUT_BRICK4) ||
                         (vol->refineLayout == LUXGI_LAYOUT_BRICK4 && (vol->resolution & 3))))
@@ -604,8 +658,6 @@ static int trace_entry(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, c
     // the guard bounds FP32 voxel values; FP16 storage would round below them
     if (vol->guard && ((vol->resolution & 3) || vol->storage != LUXGI_STORAGE_FP32)) return LUXGI_ERR_INVALID_ARGUMENT;
     if (vol->layout == LUXGI_LAYOUT_BRICK4 && (vol->resolution & 3)) return LUXGI_ERR_INVALID_ARGUMENT;
-    if (A.P <= 0 || A.R <= 0 || A.R > rays->width || rays->height < A.P) return LUXGI_ERR_INVALID_ARGUMENT;
-    if ((long long)A.P * A.R >= (1ll << 31)) return LUXGI_ERR_UNSUPPORTED;
     {  // the samplers address the volumes with 32-bit byte offsets
         const size_t es = vol->storage == LUXGI_STORAGE_FP16 ? 2 : 4;
         const size_t C = g->cascadesCount, r = (size_t)vol->resolution, m = (size_t)vol->mipResolution;
@@ -616,23 +668,109 @@ static int trace_entry(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, c
         if (fine >= (1ull << 32) || C * m * m * m * es >= (1ull << 32) || (vol->refine && ref >= (1ull << 32)))
             return LUXGI_ERR_UNSUPPORTED;
     }
-    const size_t need = luxgi_ddgi_trace_workspace_size(u, at->objectCapacity);
+    A.vols = nullptr;
+    A.nVols = 0;
+    A.totalRays = 0;
+    return LUXGI_OK;
+}
+
+template <bool COUNT, bool BATCH>
+static int trace_launch(const TraceArgs& A, const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at,
+                        const LuxgiSkybox* sky, int rayStorage, size_t rays, void* workspace, size_t workspaceBytes,
+                        unsigned long long* counters, hipStream_t st) {
+    const size_t need = trace_workspace_layout(rays, at->objectCapacity, nullptr, nullptr);
     if (!workspace || workspaceBytes < need) return LUXGI_ERR_WORKSPACE_TOO_SMALL;
     TraceWorkspace w;
-    trace_workspace_layout((size_t)A.P * A.R, at->objectCapacity, static_cast<char*>(workspace), &w);
-    hipStream_t st = as_stream(stream);
+    trace_workspace_layout(rays, at->objectCapacity, static_cast<char*>(workspace), &w);
     hipLaunchKernelGGL(trace_prepare_kernel, dim3(std::max(1, (at->objectCapacity + 255) / 256)), dim3(256), 0, st,
                        at->objects, at->objectCapacity, w.inv);
     LUXGI_LAUNCH_CHECK();
-    if (vol->storage == LUXGI_STORAGE_FP32 && rays->storage == LUXGI_STORAGE_FP32)
-        return dispatch_color<float, float, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, w, counters, st);
-    if (vol->storage == LUXGI_STORAGE_FP32 && rays->storage == LUXGI_STORAGE_FP16)
-        return dispatch_color<float, __half, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, w, counters, st);
-    if (vol->storage == LUXGI_STORAGE_FP16 && rays->storage == LUXGI_STORAGE_FP32)
-        return dispatch_color<__half, float, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, w, counters, st);
-    if (vol->storage == LUXGI_STORAGE_FP16 && rays->storage == LUXGI_STORAGE_FP16)
-        return dispatch_color<__half, __half, COUNT>(A, vol, at, sky, probeOffsets, rays, rayFlags, w, counters, st);
+    if (vol->storage == LUXGI_STORAGE_FP32 && rayStorage == LUXGI_STORAGE_FP32)
+        return dispatch_color<float, float, COUNT, BATCH>(A, vol, at, sky, w, counters, st);
+    if (vol->storage == LUXGI_STORAGE_FP32 && rayStorage == LUXGI_STORAGE_FP16)
+        return dispatch_color<float, __half, COUNT, BATCH>(A, vol, at, sky, w, counters, st);
+    if (vol->storage == LUXGI_STORAGE_FP16 && rayStorage == LUXGI_STORAGE_FP32)
+        return dispatch_color<__half, float, COUNT, BATCH>(A, vol, at, sky, w, counters, st);
+    if (vol->storage == LUXGI_STORAGE_FP16 && rayStorage == LUXGI_STORAGE_FP16)
+        return dispatch_color<__half, __half, COUNT, BATCH>(A, vol, at, sky, w, counters, st);
     return LUXGI_ERR_UNSUPPORTED;
+}
+
+// checks one volume's own arguments (grid and ray buffers)
+static int trace_volume_ok(const LuxgiDDGIUniform* u, const LuxgiRayBuffers* rays) {
+    if (!u || !rays || !rays->radiance || !rays->directionDistance) return LUXGI_ERR_INVALID_ARGUMENT;
+    const long long P = (long long)u->probeCounts[0] * u->probeCounts[1] * u->probeCounts[2];
+    if (u->probeCounts[0] <= 0 || u->probeCounts[1] <= 0 || u->probeCounts[2] <= 0) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (u->raysPerProbe <= 0 || u->raysPerProbe > rays->width || rays->height < P) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (P * u->raysPerProbe >= (1ll << 31)) return LUXGI_ERR_UNSUPPORTED;
+    if (rays->storage != LUXGI_STORAGE_FP32 && rays->storage != LUXGI_STORAGE_FP16) return LUXGI_ERR_UNSUPPORTED;
+    return LUXGI_OK;
+}
+
+template <bool COUNT>
+static int trace_entry(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, const LuxgiSDFVolume* vol,
+                       const LuxgiSurfaceAtlas* at, const LuxgiSkybox* sky, const LuxgiTraceParams* p,
+                       const float* probeOffsets, const LuxgiRayBuffers* rays, uint8_t* rayFlags, void* workspace,
+                       size_t workspaceBytes, unsigned long long* counters, void* stream) {
+    int rc = trace_volume_ok(u, rays);
+    if (rc != LUXGI_OK) return rc;
+    TraceArgs A;
+    rc = trace_scene_args(g, vol, at, sky, p, COUNT, A);
+    if (rc != LUXGI_OK) return rc;
+    A.P = u->probeCounts[0] * u->probeCounts[1] * u->probeCounts[2];
+    A.R = u->raysPerProbe;
+    A.rayWidth = rays->width;
+    A.vol.uniform = *u;
+    for (int i = 0; i < 16; ++i) A.vol.orientation[i] = p->randomOrientation[i];
+    A.vol.deviceOrientation = p->deviceOrientation;
+    A.vol.probeOffsets = probeOffsets;
+    A.vol.rays = *rays;
+    A.vol.rayFlags = rayFlags;
+    A.vol.rayStart = 0;
+    return trace_launch<COUNT, false>(A, vol, at, sky, rays->storage, (size_t)A.P * A.R, workspace, workspaceBytes,
+                                      counters, as_stream(stream));
+}
+
+extern "C" int luxgi_ddgi_trace_batch_prepare(LuxgiTraceVolume* volumes, int32_t count, int32_t rayStorage,
+                                              int64_t* totalRays) {
+    if (!volumes || !totalRays || count <= 0 || count > LUXGI_MAX_TRACE_BATCH) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (rayStorage != LUXGI_STORAGE_FP32 && rayStorage != LUXGI_STORAGE_FP16) return LUXGI_ERR_UNSUPPORTED;
+    long long total = 0;
+    for (int k = 0; k < count; ++k) {
+        LuxgiTraceVolume& v = volumes[k];
+        const int rc = trace_volume_ok(&v.uniform, &v.rays);
+        if (rc != LUXGI_OK) return rc;
+        if (v.rays.storage != rayStorage) return LUXGI_ERR_INVALID_ARGUMENT;
+        v.rayStart = total;
+        total += (long long)v.uniform.probeCounts[0] * v.uniform.probeCounts[1] * v.uniform.probeCounts[2] *
+                 v.uniform.raysPerProbe;
+        if (total >= (1ll << 31)) return LUXGI_ERR_UNSUPPORTED;
+    }
+    *totalRays = total;
+    return LUXGI_OK;
+}
+
+extern "C" size_t luxgi_ddgi_trace_batch_workspace_size(int64_t totalRays, int32_t objectCapacity) {
+    if (totalRays <= 0 || totalRays >= (1ll << 31)) return 0;
+    return trace_workspace_layout((size_t)totalRays, objectCapacity, nullptr, nullptr);
+}
+
+extern "C" int luxgi_ddgi_trace_sdf_batched(const LuxgiTraceVolume* volumes, int32_t count, int64_t totalRays,
+                                            int32_t rayStorage, const LuxgiGlobalSDFData* g,
+                                            const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at,
+                                            const LuxgiSkybox* sky, const LuxgiTraceParams* p, void* workspace,
+                                            size_t workspaceBytes, void* stream) {
+    if (!volumes || count <= 0 || count > LUXGI_MAX_TRACE_BATCH || totalRays <= 0 || totalRays >= (1ll << 31))
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    TraceArgs A;
+    const int rc = trace_scene_args(g, vol, at, sky, p, false, A);
+    if (rc != LUXGI_OK) return rc;
+    A.P = A.R = A.rayWidth = 0;
+    A.vols = volumes;
+    A.nVols = count;
+    A.totalRays = totalRays;
+    return trace_launch<false, true>(A, vol, at, sky, rayStorage, (size_t)totalRays, workspace, workspaceBytes,
+                                     nullptr, as_stream(stream));
 }
 
 extern "C" int luxgi_ddgi_trace_sdf(const LuxgiDDGIUniform* u, const LuxgiGlobalSDFData* g, const LuxgiSDFVolume* vol,

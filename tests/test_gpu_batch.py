@@ -125,11 +125,13 @@ def test_autotuner_chooses_and_caches(tmp_path):
 # ---------------------------------------------------------------------------------------------
 # Several probe volumes per graph replay
 # ---------------------------------------------------------------------------------------------
-def test_ddgi_batch_graph_replay_matches_per_volume_updates():
+@pytest.mark.parametrize("trace,streams", [("batched", 4), ("per_volume", 4), ("batched", 0)])
+def test_ddgi_batch_graph_replay_matches_per_volume_updates(trace, streams):
     """Three volumes of different probe and ray counts over the small scene, five frames with new
     random orientations each: the graph-replayed batch (orientations read from device memory) and
     the eager batch equal independent per-volume updates (orientations as host parameters), bit
-    for bit, frame after frame."""
+    for bit, frame after frame, with the volumes traced in one launch over their flat ray list or
+    one call per volume."""
     from luxgi_amd import scene
     from luxgi_amd.batch import DDGIBatch
     from luxgi_amd.ddgi import DDGIVolume, DeviceScene, default_trace_params
@@ -138,8 +140,8 @@ def test_ddgi_batch_graph_replay_matches_per_volume_updates():
     ds = DeviceScene(g, sdf, mip, atlas, layout=abi.LAYOUT_BRICK4)
     unis = [_scenes.small_uniform(rays=64, counts=(5, 4, 6)), _scenes.small_uniform(rays=96, counts=(3, 3, 3)),
             _scenes.small_uniform(rays=128, counts=(6, 2, 4))]
-    graphed = DDGIBatch(unis, ds)
-    eager = DDGIBatch(unis, ds)
+    graphed = DDGIBatch(unis, ds, streams=streams, trace=trace)
+    eager = DDGIBatch(unis, ds, streams=streams, trace=trace)
     singles = [DDGIVolume(u) for u in unis]
     rng = np.random.default_rng(5)
     for frame in range(5):

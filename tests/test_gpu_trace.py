@@ -282,3 +282,65 @@ def test_mip_of_each_cascade_matches_oracle():
     for c, h in enumerate(halves):
         got = build_mip(t, 32, max_distance=2.0 * h, cascades=3, cascade=c).cpu().numpy()
         np.testing.assert_allclose(got[:, :, c * 8:(c + 1) * 8], mip[:, :, c * 8:(c + 1) * 8], rtol=4e-7, atol=1e-7)
+
+
+@pytest.mark.parametrize("order,ray_storage", [(0, abi.STORAGE_FP32), (1, abi.STORAGE_FP32), (0, abi.STORAGE_FP16)])
+def test_batched_trace_equals_per_volume_traces(order, ray_storage):
+    """luxgi_ddgi_trace_sdf_batched over three volumes (different grids and ray counts, one with
+    relocation offsets, host and device orientations) writes exactly what one luxgi_ddgi_trace_sdf
+    call per volume writes: radiance, direction + distance and ray class."""
+    from luxgi_amd._lib import lib
+    from luxgi_amd.ddgi import DDGIVolume, DeviceScene, ptr, stream_handle
+    prims, sdf, mip, g, atlas = _scenes.small_scene()
+    ds = DeviceScene(g, sdf, mip, atlas, layout=abi.LAYOUT_BRICK4)
+    unis = [_scenes.small_uniform(rays=64, counts=(5, 4, 6)), _scenes.small_uniform(rays=96, counts=(3, 3, 3)),
+            _scenes.small_uniform(rays=128, counts=(6, 2, 4))]
+    rng = np.random.default_rng(order)
+    rots = [scene.random_rotation(int(s)) for s in rng.integers(0, 2 ** 31, size=3)]
+    singles, batched = [], []
+    for k, u in enumerate(unis):
+        for lst in (singles, batched):
+            v = DDGIVolume(u, ray_storage=ray_storage, flags_buffer=True)
+            if k == 1:
+                v.offsets.copy_(torch.from_numpy(rng.uniform(-0.05, 0.05, (u.num_probes, 4)).astype(np.float32)))
+            lst.append(v)
+        singles[k].offsets.copy_(batched[k].offsets)
+    def m4(r):  # the 4x4 orientation (random_rotation's upper 3x3 in an identity frame)
+        m = np.eye(4)
+        m[:3, :3] = np.asarray(r)[:3, :3]
+        return m
+
+    dev_rot = torch.from_numpy(np.ascontiguousarray(m4(rots[2]).T.reshape(-1), np.float32)).cuda()
+    for k, v in enumerate(singles):
+        p = _scenes.trace_params(rot_seed=3)
+        p.randomOrientation = abi.mat4_colmajor(m4(rots[k]))
+        p.shadeOrder = order
+        v.trace_rays(ds, p, use_offsets=(k == 1))
+    d = (abi.TraceVolume * 3)()
+    for k, v in enumerate(batched):
+        d[k].uniform = v.u
+        if k == 2:
+            d[k].deviceOrientation = dev_rot.data_ptr()
+        else:
+            d[k].orientation = abi.mat4_colmajor(m4(rots[k]))
+        d[k].probeOffsets = v.offsets.data_ptr() if k == 1 else None
+        d[k].rays = v.rays
+        d[k].rayFlags = v.ray_flags.data_ptr()
+    total = C.c_int64(0)
+    assert lib().luxgi_ddgi_trace_batch_prepare(C.byref(d), 3, ray_storage, C.byref(total)) == 0
+    assert total.value == sum(u.num_probes * u.raysPerProbe for u in unis)
+    dd = torch.from_numpy(np.frombuffer(memoryview(d).cast("B"), dtype=np.uint8).copy()).cuda()
+    ws = torch.empty(lib().luxgi_ddgi_trace_batch_workspace_size(total.value, ds.n_objects), dtype=torch.uint8,
+                     device="cuda")
+    p = _scenes.trace_params(rot_seed=3)
+    p.shadeOrder = order
+    assert lib().luxgi_ddgi_trace_sdf_batched(ptr(dd), 3, total.value, ray_storage, C.byref(ds.sdf_data),
+                                              C.byref(ds.volume), C.byref(ds.atlas), C.byref(ds.skybox), C.byref(p),
+                                              ptr(ws), ws.numel(), stream_handle()) == 0
+    torch.cuda.synchronize()
+    for k in range(3):
+        print(f"order {order} storage {ray_storage} volume {k}", flush=True)
+        assert torch.equal(batched[k].radiance, singles[k].radiance)
+        assert torch.equal(batched[k].dir_dist, singles[k].dir_dist)
+        assert torch.equal(batched[k].ray_flags, singles[k].ray_flags)
+    assert (singles[0].ray_flags == 1).any() and (singles[0].ray_flags == 0).any()

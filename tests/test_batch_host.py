@@ -63,3 +63,47 @@ def test_descriptor_layout_matches_the_header():
     assert abi.ReflectionDenoiseItem.cur.offset == 208
     assert abi.ReflectionDenoiseItem.tileMap.offset == 328
     assert C.sizeof(abi.DenoiseBatchHeader) == 16
+
+
+def _trace_volumes(shapes, storage=abi.STORAGE_FP32):
+    vols = (abi.TraceVolume * len(shapes))()
+    for k, (counts, rays) in enumerate(shapes):
+        u = abi.make_uniform(counts, (-1.0, -1.0, -1.0), [0.5, 0.5, 0.5], rays)
+        vols[k].uniform = u
+        P = counts[0] * counts[1] * counts[2]
+        vols[k].rays = abi.RayBuffers(0x1000, 0x2000, rays, P, storage, 0)  # never dereferenced here
+    return vols
+
+
+def test_trace_batch_prepare_sets_ray_starts():
+    from luxgi_amd._lib import lib
+    shapes = [((5, 4, 6), 64), ((3, 3, 3), 96), ((6, 2, 4), 128)]
+    vols = _trace_volumes(shapes)
+    total = C.c_int64(-1)
+    assert lib().luxgi_ddgi_trace_batch_prepare(C.byref(vols), 3, abi.STORAGE_FP32, C.byref(total)) == 0
+    starts = [0, 120 * 64, 120 * 64 + 27 * 96]
+    assert [vols[k].rayStart for k in range(3)] == starts
+    assert total.value == starts[-1] + 48 * 128
+    assert lib().luxgi_ddgi_trace_batch_workspace_size(total.value, 10) > 0
+    assert lib().luxgi_ddgi_trace_batch_workspace_size(0, 10) == 0
+
+
+@pytest.mark.parametrize("case", ["storage", "width", "height", "null", "count", "counts"])
+def test_trace_batch_prepare_rejects_bad_volumes(case):
+    from luxgi_amd._lib import lib
+    vols = _trace_volumes([((4, 4, 4), 64), ((2, 2, 2), 32)])
+    n, storage = 2, abi.STORAGE_FP32
+    if case == "storage":
+        vols[1].rays.storage = abi.STORAGE_FP16
+    elif case == "width":
+        vols[0].rays.width = 63
+    elif case == "height":
+        vols[1].rays.height = 7
+    elif case == "null":
+        vols[0].rays.radiance = None
+    elif case == "count":
+        n = abi.MAX_TRACE_BATCH + 1
+    else:
+        vols[1].uniform.probeCounts[2] = 0
+    total = C.c_int64(0)
+    assert lib().luxgi_ddgi_trace_batch_prepare(C.byref(vols), n, storage, C.byref(total)) < 0

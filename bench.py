@@ -201,7 +201,7 @@ def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (probe volumes would shard by "
-              "probe z-slices, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "probe z-slices); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

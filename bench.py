@@ -200,7 +200,7 @@ def bench_ddgi_batched(args, ds, q):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (probe volumes would shard by "
+        print("bench.py measures one GPU: there is no multi-GPU path (probe volumes would shard by "
               "probe z-slices); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import numpy as np

@@ -315,7 +315,7 @@ def main():
                                               stream_handle())
             src = ref_bufs[i & 1]
 
-    lev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    lev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
     for _ in range(2):
         la.forward(cur, la_in)
         ref_chain()
@@ -327,12 +327,24 @@ def main():
         ref_chain()
     lev[2].record()
     torch.cuda.synchronize()
+    # one training step (4 passes forward, 4 backward with the parameter-gradient reduction, Adam)
+    from luxgi_amd.learned import filtered_mask
+    la_mask = filtered_mask(cur)
+    la_target = la_in.clone()
+    for _ in range(2):
+        la.train_step(cur, la_in, la_target, la_mask, lr=1e-4)
+    lev[4].record()
+    for _ in range(args.steps):
+        la.train_step(cur, la_in, la_target, la_mask, lr=1e-4)
+    lev[5].record()
+    torch.cuda.synchronize()
     learned = {"learned_4_passes_ms": lev[0].elapsed_time(lev[1]) / args.steps,
                "reference_4_passes_ms": lev[1].elapsed_time(lev[2]) / args.steps,
+               "train_step_ms": lev[4].elapsed_time(lev[5]) / args.steps,
                "mlp": "6-10-1 ReLU, 81 parameters, one per tap", "image": [W, H]}
     log(f"[bench] learned a-trous 1080p x4 {learned['learned_4_passes_ms']:.3f} ms "
-        f"(reference {learned['reference_4_passes_ms']:.3f} ms)", q)
-    del la, la_in, ref_bufs
+        f"(reference {learned['reference_4_passes_ms']:.3f} ms), train step {learned['train_step_ms']:.3f} ms", q)
+    del la, la_in, ref_bufs, la_target
 
     # ---------------- surface atlas passes (reported beside the step) ----------------
     # chunk culling of the 2,000 objects, then the surface-cache relight of all 12,000 tiles

@@ -806,18 +806,21 @@ typedef struct LuxgiLearnedAtrousParams {
  * kernel weight replaced by exp(MLP(features)), the MLP of the 81 `weights` (one hidden layer of 10
  * ReLU units over the tap's luminance difference / phi, depth term, normal term, kernel weight, level
  * and the centre's standard deviation). in / out [H][W] RGBA32F (rgb, variance); gbuffer normals and
- * pbr as for luxgi_svgf_reflection_atrous. sumW (nullable, [H][W]) keeps 1 + the tap weights for
- * luxgi_svgf_learned_atrous_backward. Every pixel is evaluated (no tile map). */
+ * pbr as for luxgi_svgf_reflection_atrous. record (nullable, [H][W][4] floats) keeps what
+ * luxgi_svgf_learned_atrous_backward needs per pixel: (1 + the tap weights, luminance, 1 / phi,
+ * min(sqrt(variance), 4)) of a filtered pixel, (1, 0, 0, 0) of the others. Footprints
+ * radius * stepSize <= 8 are staged in LDS. */
 int luxgi_svgf_learned_atrous(const LuxgiLearnedAtrousParams* params, const float* weights, int32_t width,
-                              int32_t height, const LuxgiGBuffer* gbuffer, const float* in, float* out, float* sumW,
+                              int32_t height, const LuxgiGBuffer* gbuffer, const float* in, float* out, float* record,
                               void* stream);
 
 /* Backward of one learned pass: from dL/d(out rgb) (gradOut [H][W][4], alpha ignored) writes
  * dL/d(in rgb) into gradIn ([H][W][4], alpha 0) and adds dL/d(weights) into gradWeights [81]. The
- * weights' dependence on the input colour (through luminance and variance) is held fixed. */
+ * weights' dependence on the input colour (through luminance and variance) is held fixed; `record`
+ * is the forward's. */
 int luxgi_svgf_learned_atrous_backward(const LuxgiLearnedAtrousParams* params, const float* weights, int32_t width,
                                        int32_t height, const LuxgiGBuffer* gbuffer, const float* in, const float* out,
-                                       const float* sumW, const float* gradOut, float* gradIn, float* gradWeights,
+                                       const float* record, const float* gradOut, float* gradIn, float* gradWeights,
                                        void* stream);
 
 /* One Adam step on n parameters (step counts from 1); clears `grad`. */

@@ -39,6 +39,11 @@ struct LaPixel {
     float rough;
 };
 
+__device__ __forceinline__ f3 load_normal_oct(const float4* __restrict__ normals, size_t q) {
+    const float2 o = reinterpret_cast<const float2*>(normals)[2 * q];
+    return oct_decode(o.x, o.y);
+}
+
 __device__ __forceinline__ LaPixel la_load(const float4* __restrict__ in, const float4* __restrict__ normals,
                                            const float4* __restrict__ pbr, size_t q) {
     LaPixel p;
@@ -110,12 +115,13 @@ __device__ __forceinline__ void la_stage_params(const float* __restrict__ params
     __syncthreads();
 }
 
-// one learned a-trous pass; sumW (nullable) keeps 1 + the tap weights per pixel for the backward
+// one learned a-trous pass; `record` (nullable) keeps per pixel what the backward needs: (1 + the
+// tap weights, luminance, 1 / phi, min(sqrt(var), 4)) of a filtered pixel, (1, 0, 0, 0) otherwise
 __global__ __launch_bounds__(256) void learned_atrous_kernel(LaArgs A, const float* __restrict__ params,
                                                              const float4* __restrict__ normals,
                                                              const float4* __restrict__ pbr,
                                                              const float4* __restrict__ in, float4* __restrict__ out,
-                                                             float* __restrict__ sumW) {
+                                                             float4* __restrict__ record) {
     __shared__ float sP[kLaParams];
     la_stage_params(params, sP);
     const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
@@ -124,14 +130,15 @@ __global__ __launch_bounds__(256) void learned_atrous_kernel(LaArgs A, const flo
     const LaPixel p = la_load(in, normals, pbr, e);
     const int mode = la_mode(p);
     float4 o = p.c;
-    float sw = 1.0f;
+    float sw = 1.0f, lp = 0.0f, invPhi = 0.0f, sdv = 0.0f;
     if (mode == 2) {
         o = make_float4(0.f, 0.f, 0.f, 0.f);
     } else if (mode == 0) {
         const float var = la_phi(A, in, x, y);
         const float phi = A.phiColor * sqrtf(fmaxf(0.0f, var)) + 1e-10f;
-        const float lp = la_luminance(p.c.x, p.c.y, p.c.z), invPhi = 1.0f / phi;
-        const float sdv = fminf(sqrtf(fmaxf(var, 0.0f)), 4.0f);
+        lp = la_luminance(p.c.x, p.c.y, p.c.z);
+        invPhi = 1.0f / phi;
+        sdv = fminf(sqrtf(fmaxf(var, 0.0f)), 4.0f);
         float4 s = p.c;
         for (int yy = -A.radius; yy <= A.radius; ++yy)
             for (int xx = -A.radius; xx <= A.radius; ++xx) {
@@ -151,7 +158,7 @@ __global__ __launch_bounds__(256) void learned_atrous_kernel(LaArgs A, const flo
         o = make_float4(s.x / sw, s.y / sw, s.z / sw, s.w / (sw * sw));
     }
     out[e] = o;
-    if (sumW) sumW[e] = sw;
+    if (record) record[e] = make_float4(sw, lp, invPhi, sdv);
 }
 
 // LDS-tiled form of the same pass for footprints radius * stepSize <= 8: a 16x16 block stages its
@@ -165,7 +172,8 @@ __global__ __launch_bounds__(256) void learned_atrous_lds_kernel(LaArgs A, const
                                                                  const float4* __restrict__ normals,
                                                                  const float4* __restrict__ pbr,
                                                                  const float4* __restrict__ in,
-                                                                 float4* __restrict__ out, float* __restrict__ sumW) {
+                                                                 float4* __restrict__ out,
+                                                                 float4* __restrict__ record) {
     __shared__ float sP[kLaParams];
     __shared__ float4 sIn[kLaMaxSide * kLaMaxSide];
     __shared__ float4 sGuide[kLaMaxSide * kLaMaxSide];  // normal xyz, linear depth
@@ -198,7 +206,7 @@ __global__ __launch_bounds__(256) void learned_atrous_lds_kernel(LaArgs A, const
     p.rough = pbr[e].y;
     const int mode = la_mode(p);
     float4 o = p.c;
-    float sw = 1.0f;
+    float sw = 1.0f, lp = 0.0f, invPhi = 0.0f, sdv = 0.0f;
     if (mode == 2) {
         o = make_float4(0.f, 0.f, 0.f, 0.f);
     } else if (mode == 0) {
@@ -208,8 +216,9 @@ __global__ __launch_bounds__(256) void learned_atrous_lds_kernel(LaArgs A, const
             for (int xx = -1; xx <= 1; ++xx)
                 if (la_inside(x + xx, y + yy, A.W, A.H)) var += sIn[lc + yy * side + xx].w * kv[abs(xx)][abs(yy)];
         const float phi = A.phiColor * sqrtf(fmaxf(0.0f, var)) + 1e-10f;
-        const float lp = la_luminance(p.c.x, p.c.y, p.c.z), invPhi = 1.0f / phi;
-        const float sdv = fminf(sqrtf(fmaxf(var, 0.0f)), 4.0f);
+        lp = la_luminance(p.c.x, p.c.y, p.c.z);
+        invPhi = 1.0f / phi;
+        sdv = fminf(sqrtf(fmaxf(var, 0.0f)), 4.0f);
         float4 s = p.c;
         for (int yy = -A.radius; yy <= A.radius; ++yy)
             for (int xx = -A.radius; xx <= A.radius; ++xx) {
@@ -234,14 +243,15 @@ __global__ __launch_bounds__(256) void learned_atrous_lds_kernel(LaArgs A, const
         o = make_float4(s.x / sw, s.y / sw, s.z / sw, s.w / (sw * sw));
     }
     out[e] = o;
-    if (sumW) sumW[e] = sw;
+    if (record) record[e] = make_float4(sw, lp, invPhi, sdv);
 }
 
 // Backward of one pass. Per pixel x: as a centre, the parameter gradient of its taps; as a tap of
-// its neighbours (and as its own centre), dL/d(in rgb).
+// its neighbours (and as its own centre), dL/d(in rgb). The centres' weight sums, luminances,
+// 1 / phi and deviations come from the forward's records (no variance re-evaluation).
 __global__ __launch_bounds__(256) void learned_atrous_backward_kernel(
     LaArgs A, const float* __restrict__ params, const float4* __restrict__ normals, const float4* __restrict__ pbr,
-    const float4* __restrict__ in, const float4* __restrict__ out, const float* __restrict__ sumW,
+    const float4* __restrict__ in, const float4* __restrict__ out, const float4* __restrict__ record,
     const float4* __restrict__ gOut, float4* __restrict__ gIn, float* __restrict__ gParams) {
     __shared__ float sP[kLaParams];
     __shared__ float sG[4][kLaParams];
@@ -260,13 +270,10 @@ __global__ __launch_bounds__(256) void learned_atrous_backward_kernel(
         if (mode == 1) gi = make_float3(g.x, g.y, g.z);  // pass-through
         if (mode == 0) {
             // centre: out = (c + sum w c_q) / W -> d out / d c_p = 1 / W (weights held fixed)
-            const float Wp = sumW[e];
+            const float4 rec = record[e];
+            const float Wp = rec.x, lp = rec.y, invPhi = rec.z, sdv = rec.w;
             gi = make_float3(g.x / Wp, g.y / Wp, g.z / Wp);
             const float4 op = out[e];
-            const float var = la_phi(A, in, x, y);
-            const float phi = A.phiColor * sqrtf(fmaxf(0.0f, var)) + 1e-10f;
-            const float lp = la_luminance(p.c.x, p.c.y, p.c.z), invPhi = 1.0f / phi;
-            const float sdv = fminf(sqrtf(fmaxf(var, 0.0f)), 4.0f);
             for (int yy = -A.radius; yy <= A.radius; ++yy)
                 for (int xx = -A.radius; xx <= A.radius; ++xx) {
                     const int px = x + xx * A.stepSize, py = y + yy * A.stepSize;
@@ -301,17 +308,18 @@ __global__ __launch_bounds__(256) void learned_atrous_backward_kernel(
                 const int cx = x - xx * A.stepSize, cy = y - yy * A.stepSize;
                 if (!la_inside(cx, cy, A.W, A.H)) continue;
                 const size_t ce = (size_t)cy * A.W + cx;
-                const LaPixel c = la_load(in, normals, pbr, ce);
+                LaPixel c;  // the centre's guides (its colour enters through the record's luminance)
+                c.n = load_normal_oct(normals, ce);
+                const float4 m = pbr[ce];
+                c.z = m.w;
+                c.rough = m.y;
                 if (la_mode(c) != 0) continue;
-                const float var = la_phi(A, in, cx, cy);
-                const float phi = A.phiColor * sqrtf(fmaxf(0.0f, var)) + 1e-10f;
-                const float lc = la_luminance(c.c.x, c.c.y, c.c.z);
-                const float sdv = fminf(sqrtf(fmaxf(var, 0.0f)), 4.0f);
+                const float4 rec = record[ce];
                 // x is the tap (xx, yy) of the centre c
-                const LaFeatures F = la_features(A, c, lc, 1.0f / phi, sdv, p, xx, yy);
+                const LaFeatures F = la_features(A, c, rec.y, rec.z, rec.w, p, xx, yy);
                 float h[kLaHidden];
                 bool cl;
-                const float w = la_mlp(sP, F, h, cl) / sumW[ce];
+                const float w = la_mlp(sP, F, h, cl) / rec.x;
                 const float4 gc = gOut[ce];
                 gi.x += gc.x * w;
                 gi.y += gc.y * w;
@@ -352,7 +360,7 @@ static int la_args(const LuxgiLearnedAtrousParams* p, int32_t W, int32_t H, cons
 }
 
 extern "C" int luxgi_svgf_learned_atrous(const LuxgiLearnedAtrousParams* p, const float* weights, int32_t W,
-                                         int32_t H, const LuxgiGBuffer* gb, const float* in, float* out, float* sumW,
+                                         int32_t H, const LuxgiGBuffer* gb, const float* in, float* out, float* record,
                                          void* stream) {
     LaArgs A;
     const int rc = la_args(p, W, H, gb, A);
@@ -362,29 +370,30 @@ extern "C" int luxgi_svgf_learned_atrous(const LuxgiLearnedAtrousParams* p, cons
         hipLaunchKernelGGL(learned_atrous_lds_kernel, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0,
                            as_stream(stream), A, weights, reinterpret_cast<const float4*>(gb->normals),
                            reinterpret_cast<const float4*>(gb->pbr), reinterpret_cast<const float4*>(in),
-                           reinterpret_cast<float4*>(out), sumW);
+                           reinterpret_cast<float4*>(out), reinterpret_cast<float4*>(record));
     else
         hipLaunchKernelGGL(learned_atrous_kernel, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, as_stream(stream),
                            A, weights, reinterpret_cast<const float4*>(gb->normals),
                            reinterpret_cast<const float4*>(gb->pbr), reinterpret_cast<const float4*>(in),
-                           reinterpret_cast<float4*>(out), sumW);
+                           reinterpret_cast<float4*>(out), reinterpret_cast<float4*>(record));
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }
 
 extern "C" int luxgi_svgf_learned_atrous_backward(const LuxgiLearnedAtrousParams* p, const float* weights, int32_t W,
                                                   int32_t H, const LuxgiGBuffer* gb, const float* in, const float* out,
-                                                  const float* sumW, const float* gradOut, float* gradIn,
+                                                  const float* record, const float* gradOut, float* gradIn,
                                                   float* gradWeights, void* stream) {
     LaArgs A;
     const int rc = la_args(p, W, H, gb, A);
     if (rc != LUXGI_OK) return rc;
-    if (!weights || !in || !out || !sumW || !gradOut || !gradIn || !gradWeights) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (!weights || !in || !out || !record || !gradOut || !gradIn || !gradWeights) return LUXGI_ERR_INVALID_ARGUMENT;
     if (gradIn == gradOut || gradIn == in) return LUXGI_ERR_INVALID_ARGUMENT;
     hipLaunchKernelGGL(learned_atrous_backward_kernel, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0,
                        as_stream(stream), A, weights, reinterpret_cast<const float4*>(gb->normals),
                        reinterpret_cast<const float4*>(gb->pbr), reinterpret_cast<const float4*>(in),
-                       reinterpret_cast<const float4*>(out), sumW, reinterpret_cast<const float4*>(gradOut),
+                       reinterpret_cast<const float4*>(out), reinterpret_cast<const float4*>(record),
+                       reinterpret_cast<const float4*>(gradOut),
                        reinterpret_cast<float4*>(gradIn), gradWeights);
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;

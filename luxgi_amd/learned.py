@@ -5,7 +5,7 @@ driven by denoise::system, Engine/Raytrace/RaytracedReflection.cpp:419-494: `ite
 of step 1 << i) with each tap's weight exp(-|dl|/phi - wZ) * wNormal * k (EdgeStop.glsl:24-40)
 replaced by exp(MLP(features)), one MLP (81 parameters) shared by the passes and told the pass by
 its `level` input. Training runs on the GPU: luxgi_svgf_learned_atrous keeps each pixel's weight
-sum, luxgi_svgf_learned_atrous_backward returns dL/d(input) and accumulates dL/d(parameters),
+sum and centre quantities, luxgi_svgf_learned_atrous_backward returns dL/d(input) and accumulates dL/d(parameters),
 luxgi_adam_step updates them. The loss is the mean squared error of the chain's output against a
 converged target over the pixels the filter touches (non-sky, roughness in (0.05, 0.45)).
 
@@ -89,7 +89,8 @@ class LearnedAtrous:
             self.params.append(p)
         z = lambda: torch.zeros((height, width, 4), dtype=torch.float32, device="cuda")  # noqa: E731
         self.images = [z() for _ in range(iterations)]  # outputs of the passes
-        self.sum_w = [torch.zeros((height, width), dtype=torch.float32, device="cuda") for _ in range(iterations)]
+        # per pass and pixel: (1 + tap weights, luminance, 1 / phi, min(sqrt(var), 4)) for the backward
+        self.records = [z() for _ in range(iterations)]
         self.grads = [z(), z()]
         self._inputs = [None] * iterations
 
@@ -103,7 +104,7 @@ class LearnedAtrous:
         for i, p in enumerate(self.params):
             self._inputs[i] = src
             check(lib().luxgi_svgf_learned_atrous(C.byref(p), ptr(self.weights), self.W, self.H, C.byref(gb), ptr(src),
-                                                  ptr(self.images[i]), ptr(self.sum_w[i]), stream_handle()),
+                                                  ptr(self.images[i]), ptr(self.records[i]), stream_handle()),
                   "luxgi_svgf_learned_atrous")
             src = self.images[i]
         return src
@@ -119,7 +120,7 @@ class LearnedAtrous:
                 dst = self.grads[1 - (i & 1)]
             check(lib().luxgi_svgf_learned_atrous_backward(
                 C.byref(self.params[i]), ptr(self.weights), self.W, self.H, C.byref(gb), ptr(self._inputs[i]),
-                ptr(self.images[i]), ptr(self.sum_w[i]), ptr(g), ptr(dst), ptr(self.grad), stream_handle()),
+                ptr(self.images[i]), ptr(self.records[i]), ptr(g), ptr(dst), ptr(self.grad), stream_handle()),
                 "luxgi_svgf_learned_atrous_backward")
             g = dst
         return g

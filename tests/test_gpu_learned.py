@@ -36,7 +36,7 @@ def _forward(p, wts, gbd, img):
     tw = torch.from_numpy(np.asarray(wts, np.float32)).cuda()
     ti = torch.from_numpy(img).cuda()
     out = torch.full((H, W, 4), -7.0, device="cuda")
-    sw = torch.zeros((H, W), device="cuda")
+    sw = torch.zeros((H, W, 4), device="cuda")
     rc = lib().luxgi_svgf_learned_atrous(C.byref(p), ptr(tw), W, H, C.byref(gbuffer_struct(gbd)), ptr(ti), ptr(out),
                                          ptr(sw), stream_handle())
     assert rc == 0
@@ -54,7 +54,7 @@ def test_forward_matches_oracle(W, H, radius, step, seed):
     torch.cuda.synchronize()
     want, want_sw = oracle.learned_atrous_forward(p, wts, nrm, pbr, img)
     np.testing.assert_allclose(out.cpu().numpy(), want, rtol=1e-4, atol=1e-5)
-    np.testing.assert_allclose(sw.cpu().numpy(), want_sw, rtol=1e-4)
+    np.testing.assert_allclose(sw.cpu().numpy()[..., 0], want_sw, rtol=1e-4)
 
 
 def test_reference_like_weights_reproduce_the_reference_pass():

@@ -517,7 +517,7 @@ def main():
                  "speedup_ddgi_vs_fp32": ddgi_ms / (m_trace + m_upd),
                  "speedup_step_vs_fp32": (ddgi_ms + float(np.mean(t_svgf))) / (m_trace + m_upd + m_svgf)}
         log(f"[bench] reduced precision: trace {m_trace:.3f} ms, update {m_upd:.3f} ms, svgf {m_svgf:.3f} ms", q)
-        del ds16, vol16, dn16
+        del vol16, dn16  # ds16 stays for the error report beside the CPU baseline
 
     # ---------------- counting run (algorithmic bytes of the trace) ----------------
     counters = torch.zeros(8, dtype=torch.int64, device="cuda")
@@ -542,9 +542,20 @@ def main():
         del src, dst
 
     # ---------------- CPU baseline (bounded sample, FP32 transcription) ----------------
+    # and the error of the plain and reduced-precision forms against the FP64 oracle: one first
+    # frame (zero history) of fresh volumes, compared on the two lowest probe layers
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(u, g, sdf, mip, res, atlas, rotations[-1], args.cpu_threads, q)
+        first = {}
+        forms = [("fp32", ds, False)] + ([] if mixed is None else [("reduced", ds16, True)])
+        for name, scene_, half in forms:
+            v = DDGIVolume(u)
+            v.trace_rays(scene_, default_trace_params(rotations[-1]))
+            v.probe_update(fuse_border=True, half=half)
+            torch.cuda.synchronize()
+            first[name] = (v.irradiance[1].cpu().numpy(), v.depth[1].cpu().numpy())
+            del v
+        cpu = cpu_baseline(u, g, sdf, mip, res, atlas, rotations[-1], args.cpu_threads, q, first)
 
     out = {
         "metric": "DDGI update Grays/s (32x16x32 probes x 256 rays) + 1080p SVGF ms",
@@ -590,7 +601,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(u_full, g, sdf_t, mip_t, res, atlas, rotation, threads, quiet):
+def cpu_baseline(u_full, g, sdf_t, mip_t, res, atlas, rotation, threads, quiet, first=None):
     import numpy as np
 
     import oracle
@@ -628,11 +639,48 @@ def cpu_baseline(u_full, g, sdf_t, mip_t, res, atlas, rotation, threads, quiet):
         u, tt, tu = run(nz)
     rays = u.num_probes * u.raysPerProbe
     log(f"[bench] cpu baseline: {u.num_probes} probes, trace {tt:.2f}s update {tu:.2f}s on {n} threads", quiet)
-    return {"value": rays / (tt + tu) / 1e9, "unit": "Grays/s", "cores": n, "kind": "port",
-            "sample": f"FP32 C++ transcription, {u.num_probes} probes (32x16x{nz} slab of the same volume) x "
-                      f"{u.raysPerProbe} rays: trace {tt:.2f} s + blend {tu:.2f} s on {n} threads "
-                      f"(host reports {hw} hardware threads)",
-            "trace_s": tt, "update_s": tu}
+    result = {"value": rays / (tt + tu) / 1e9, "unit": "Grays/s", "cores": n, "kind": "port",
+              "sample": f"FP32 C++ transcription, {u.num_probes} probes (32x16x{nz} slab of the same volume) x "
+                        f"{u.raysPerProbe} rays: trace {tt:.2f} s + blend {tu:.2f} s on {n} threads "
+                        f"(host reports {hw} hardware threads)",
+              "trace_s": tt, "update_s": tu}
+    if first:
+        result["error_vs_fp64"] = precision_report(u_full, g, sdf, mip, res, atlas, p, n, first)
+        for name, e in result["error_vs_fp64"].items():
+            log(f"[bench] {name} vs FP64: irradiance max rel {e['irradiance_max_rel']:.2e} "
+                f"(p99.9 {e['irradiance_p999_rel']:.2e}), depth max abs {e['depth_max_abs']:.2e}", quiet)
+    return result
+
+
+def precision_report(u_full, g, sdf, mip, res, atlas, p, n, first):
+    """Error of first-frame GPU atlases against the FP64 oracle on the two lowest probe layers (the
+    same probes and rays: a 32x16x2 volume with the full volume's start and spacing). Those layers
+    fill the first 2 (side + 2) + 1 rows of both atlases (the atlas border row 0, then a tile row
+    of side + 2 per layer); relative irradiance errors are taken against max(|ref|, 1e-2)."""
+    import numpy as np
+
+    import oracle
+    from luxgi_amd import abi
+    counts = (u_full.probeCounts[0], u_full.probeCounts[1], 2)
+    u = abi.make_uniform(counts, tuple(u_full.startPosition[:3]), tuple(u_full.step[:3]), u_full.raysPerProbe,
+                         irr_oct=8, depth_oct=16, hysteresis=0.98)
+    rad, dd, fl, _ = oracle.trace(u, g, sdf, mip, res, (res + 3) // 4, atlas.data, atlas.chunks, atlas.culled,
+                                  atlas.objects, atlas.tiles, atlas.depth, atlas.color_f32, p, nthreads=n)
+    z_i = np.zeros((u.irradianceTextureHeight, u.irradianceTextureWidth, 4), np.float32)
+    z_d = np.zeros((u.depthTextureHeight, u.depthTextureWidth, 2), np.float32)
+    ref_i, ref_d = oracle.probe_update(u, rad, dd, z_i, z_d, 1, 7, nthreads=n)
+    hi, hd = 2 * (8 + 2) + 1, 2 * (16 + 2) + 1  # border row 0 and the tiles of the two layers
+    out = {}
+    for name, (gi, gd) in first.items():
+        ri = ref_i[:hi, :, :3]
+        di = np.abs(gi[:hi, :, :3].astype(np.float64) - ri)
+        rel = di / np.maximum(np.abs(ri), 1e-2)
+        dd_ = np.abs(gd[:hd].astype(np.float64) - ref_d[:hd])
+        out[name] = {"irradiance_max_rel": float(rel.max()), "irradiance_p999_rel": float(np.quantile(rel, 0.999)),
+                     "irradiance_mean_abs": float(di.mean()), "depth_max_abs": float(dd_.max()),
+                     "depth_mean_abs": float(dd_.mean()), "texels": int(ri.shape[0] * ri.shape[1]),
+                     "rays_compared": int(u.num_probes * u.raysPerProbe)}
+    return out
 
 
 if __name__ == "__main__":

@@ -164,3 +164,22 @@ def test_ddgi_batch_graph_replay_matches_per_volume_updates(trace, streams):
                 assert torch.equal(other.current_depth, vol.current_depth)
                 assert torch.equal(other.radiance, vol.radiance)
     assert graphed.graphs[0] is not None and graphed.graphs[1] is not None
+
+
+def test_autotuner_trace_shade_order(tmp_path):
+    """tune_trace_shade_order times both shading orders of the trace (identical outputs), sets the
+    fastest in the parameters and caches it."""
+    from luxgi_amd.autotune import Autotuner, tune_trace_shade_order
+    from luxgi_amd.ddgi import DDGIVolume, DeviceScene
+    from tests import _scenes
+    prims, sdf, mip, g, atlas = _scenes.small_scene()
+    ds = DeviceScene(g, sdf, mip, atlas, layout=abi.LAYOUT_BRICK4)
+    vol = DDGIVolume(_scenes.small_uniform(rays=64, counts=(5, 4, 6)))
+    p = _scenes.trace_params(rot_seed=2)
+    tuner = Autotuner(path=str(tmp_path / "tune.json"))
+    choice = tune_trace_shade_order(vol, ds, p, tuner)
+    assert choice in (0, 1) and p.shadeOrder == choice
+    assert set(tuner.last_timings) == {"0", "1"}
+    p2 = _scenes.trace_params(rot_seed=2)
+    assert tune_trace_shade_order(vol, ds, p2, Autotuner(path=str(tmp_path / "tune.json"))) == choice
+    assert p2.shadeOrder == choice

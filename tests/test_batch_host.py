@@ -107,3 +107,24 @@ def test_trace_batch_prepare_rejects_bad_volumes(case):
         vols[1].uniform.probeCounts[2] = 0
     total = C.c_int64(0)
     assert lib().luxgi_ddgi_trace_batch_prepare(C.byref(vols), n, storage, C.byref(total)) < 0
+
+
+def test_batched_entry_points_reject_bad_arguments_before_any_launch():
+    """Argument checks that return before touching the GPU (null descriptors, grid, order, counts)."""
+    from luxgi_amd._lib import lib
+    L = lib()
+    res = C.c_int32(0)
+    fake = C.c_void_p(0x1000)
+    assert L.luxgi_svgf_reflection_denoise_batched(None, fake, 4, abi.STORAGE_FP32, 64, 0, C.byref(res), None) < 0
+    assert L.luxgi_svgf_reflection_denoise_batched(fake, fake, 4, abi.STORAGE_FP32, 0, 0, C.byref(res), None) < 0
+    assert L.luxgi_svgf_reflection_denoise_batched(fake, fake, 4, abi.STORAGE_FP32, 64, 3, C.byref(res), None) < 0
+    assert L.luxgi_svgf_reflection_denoise_batched(fake, fake, 17, abi.STORAGE_FP32, 64, 0, C.byref(res), None) < 0
+    p = abi.TraceParams()
+    assert L.luxgi_ddgi_trace_sdf_batched(None, 1, 100, abi.STORAGE_FP32, None, None, None, None, C.byref(p), None, 0,
+                                          None) < 0
+    assert L.luxgi_ddgi_trace_sdf_batched(fake, 0, 100, abi.STORAGE_FP32, None, None, None, None, C.byref(p), None, 0,
+                                          None) < 0
+    assert L.luxgi_ddgi_trace_sdf_batched(fake, 1, 0, abi.STORAGE_FP32, None, None, None, None, C.byref(p), None, 0,
+                                          None) < 0
+    assert L.luxgi_ddgi_trace_sdf_batched(fake, 1, 100, abi.STORAGE_FP32, None, None, None, None, C.byref(p), None, 0,
+                                          None) < 0  # no scene

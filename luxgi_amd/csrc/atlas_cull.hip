@@ -31,11 +31,16 @@ static_assert(kCullScanBlocks <= 64, "the block totals are scanned by one wave")
 
 // boxIntersectsSphere (AtlasCommon.glsl:34-38) against the box [lo, hi] of chunks lo..hi (inclusive
 // chunk coordinates per axis), with the fp32 operations in the order written there and none
-// contracted, so the oracle repeats the decision bit for bit. A box of several chunks uses the
-// bounds of its outer chunks: its distance is never above that of a chunk inside it (each step
-// is monotone), so it is a conservative pre-test for the chunks it holds.
-__device__ __forceinline__ float chunk_lo(int c, float cs) { return __fmul_rn(float(c) - 20.0f, cs); }
+// contracted (contraction is off in these functions; HIP's __fmul_rn and __fadd_rn are plain
+// operations that could otherwise fuse), so the oracle repeats the decision bit for bit. A box of
+// several chunks uses the bounds of its outer chunks: its distance is never above that of a chunk
+// inside it (each step is monotone), so it is a conservative pre-test for the chunks it holds.
+__device__ __forceinline__ float chunk_lo(int c, float cs) {
+#pragma clang fp contract(off)
+    return __fmul_rn(float(c) - 20.0f, cs);
+}
 __device__ __forceinline__ bool box_touches(float cs, int3 lo, int3 hi, float4 s) {
+#pragma clang fp contract(off)
     const float mnx = chunk_lo(lo.x, cs), mny = chunk_lo(lo.y, cs), mnz = chunk_lo(lo.z, cs);
     const float mxx = __fadd_rn(chunk_lo(hi.x, cs), cs), mxy = __fadd_rn(chunk_lo(hi.y, cs), cs),
                 mxz = __fadd_rn(chunk_lo(hi.z, cs), cs);

@@ -260,7 +260,15 @@ __device__ __forceinline__ void reproject_block(const RA& A, const GBuf& cur, co
             const float alphaM = ok ? fmaxf(A.pc.momentsAlpha, 1.0f / maxAcc) : 1.0f;
             const float l = luminance(cr.x, cr.y, cr.z);
             const float mm0 = mixf(hm.x, l, alphaM), mm1 = mixf(hm.y, l * l, alphaM);
-            const float var = fmaxf(0.0f, mm1 - mm0 * mm0);
+            // the square rounded on its own (contraction into a fused multiply-subtract is off
+            // here): a single sample's moments (m2 = l*l rounded, m1 = l) then give exactly 0, as
+            // in exact arithmetic, instead of the rounding residual of l*l, which would open the
+            // luminance edge-stopping of freshly disoccluded pixels to ~1e-3
+            float var;
+            {
+#pragma clang fp contract(off)
+                var = fmaxf(0.0f, mm1 - mm0 * mm0);
+            }
             oc = make_float4(mixf(hc.x, cr.x, alpha), mixf(hc.y, cr.y, alpha), mixf(hc.z, cr.z, alpha), var);
             om = make_float4(mm0, mm1, hl, 0.0f);
             if (!(rough <= 0.05f || rough >= 0.45f)) vote[(ty >> 3) * 2 + (tx >> 3)] = 1;
@@ -669,7 +677,10 @@ __global__ __launch_bounds__(256) void shadow_reproject_kernel(ShadowArgs A, GBu
             om.x = mixf(hm.x, vis, alphaM);
             om.y = mixf(hm.y, vis * vis, alphaM);
             ov.x = mixf(hv, vis, alpha);
-            ov.y = fmaxf(0.0f, om.y - om.x * om.x);
+            {
+#pragma clang fp contract(off)
+                ov.y = fmaxf(0.0f, om.y - om.x * om.x);  // as in the reflection reprojection
+            }
             if (ov.x > 0.0f) vote[(ty >> 3) * 2 + (tx >> 3)] = 1;
         }
         Texel<S>::st4(outMom, (size_t)y * W + x, make_float4(om.x, om.y, hl, 0.0f));

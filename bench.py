@@ -18,7 +18,11 @@ light with an SDF shadow ray per lit texel, DDGI bounce) into an RGBA16F light c
 the SDF soft-shadow mask at 1920x1080 (sdf_shadow_mask_ms: one blue-noise disk ray per pixel) and
 the SDF reflection rays at 1920x1080 (sdf_reflection_ms: one GGX-sampled ray per pixel with the
 surface-atlas lookup at its hit, RGBA16F out), and the global SDF build from mesh SDFs (sdf_build:
-2,000 instances of three baked 32^3 meshes rasterized into the 256^3 cascade).
+2,000 instances of three baked 32^3 meshes rasterized into the 256^3 cascade), the learned a-trous
+chain at 1920x1080 (learned_atrous: 4 passes, and one training step), the batched forms
+(svgf_batched: 16 and 64 frames of uneven sizes per-frame / batched / graph-replayed, schedule
+from the autotuner; ddgi_batched: eight uneven probe volumes one after another, on four streams,
+in one trace launch sequence and as one graph replay).
 value = rays / DDGI-update time (Grays/s). All timing uses HIP events on the
 stream the kernels are launched on (torch's current stream), W warm-up steps
 then exactly K timed steps between device synchronisations.
@@ -30,8 +34,10 @@ gathers x 4 texels x (4 B depth + 4 B colour) + rays x 32 B of output) / the
 mean trace-launch time. traffic = HBM bytes per launch measured with rocprofv3
 FETCH_SIZE / WRITE_SIZE (profiles/trace_hbm_traffic.json), null when absent.
 cpu_baseline: the FP32 CPU transcription (oracle/liboracle.so, std::thread over
-probes) on a 32x16x2 slab of the same volume (1,024 probes x 256 rays: trace +
-blend), on the host cores, scaled to Grays/s.
+probes) on a 32x16xk slab of the same volume (trace + blend, k sized to ~10 s of
+CPU work), on the host cores, scaled to Grays/s; with it (error_vs_fp64) the error
+of one first frame of the plain and reduced-precision forms against the FP64
+oracle on the two lowest probe layers.
 """
 import argparse
 import ctypes

@@ -344,3 +344,35 @@ def test_batched_trace_equals_per_volume_traces(order, ray_storage):
         assert torch.equal(batched[k].dir_dist, singles[k].dir_dist)
         assert torch.equal(batched[k].ray_flags, singles[k].ray_flags)
     assert (singles[0].ray_flags == 1).any() and (singles[0].ray_flags == 0).any()
+
+
+def test_trace_through_a_global_sdf_built_from_mesh_sdfs():
+    """End to end on the GPU: the small scene as instances of baked mesh SDFs, rasterized into the
+    global SDF (luxgi_sdf_rasterize), its coarse mip (luxgi_sdf_build_mip), then the probe trace;
+    against the oracle's rasterization, mip and trace of the same instances. Same tolerances as
+    the other trace tests (the FP32 rasterization is within 2e-5 of FP64, so only rays within
+    rounding of a threshold can change class)."""
+    from luxgi_amd import meshsdf
+    from luxgi_amd.ddgi import build_mip
+    prims, _, _, _, atlas = _scenes.small_scene()
+    res = 64
+    msh, objs, bounds, which = meshsdf.primitive_instances(prims)
+    cas = meshsdf.Cascade(0, (0.0, 0.0, 0.0), 1.0, res)
+    lists = meshsdf.chunk_lists(cas, bounds)
+    b = meshsdf.GlobalSDFBuilder(msh, objs, len(prims), which=which)
+    vol = torch.empty((res, res, res), dtype=torch.float32, device="cuda")
+    b.rasterize(cas, lists, vol)
+    gmip = build_mip(vol, res, max_distance=2.0)
+    torch.cuda.synchronize()
+    gsdf = vol.cpu().numpy()
+    osdf = oracle.sdf_rasterize(cas.params(), objs, b.pool.entries, b.pool.pool, len(prims), *lists)
+    np.testing.assert_allclose(gsdf, osdf, atol=2e-5)
+    omip = oracle.sdf_mip(osdf.astype(np.float32), res, max_distance=2.0)
+    np.testing.assert_allclose(gmip.cpu().numpy(), omip, atol=3e-5)
+    g = abi.make_sdf_data(resolution=res)
+    u = _scenes.small_uniform(rays=96, counts=(5, 4, 6))
+    p = _scenes.trace_params(rot_seed=31)
+    gpu = _gpu_trace(u, g, gsdf, gmip.cpu().numpy(), atlas, p, layout=abi.LAYOUT_BRICK4)
+    ref = _scenes.oracle_trace(u, g, osdf.astype(np.float32), omip, atlas, p)
+    _compare(u, g, gpu, ref, 2.0 / res)
+    assert (gpu[2] == 1).mean() > 0.3

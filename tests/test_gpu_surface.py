@@ -148,3 +148,39 @@ def test_surface_emissive_pass_is_exact():
                                 atlas.gb_emissive.astype(np.float32))
     np.testing.assert_allclose(sc.light_cache.cpu().numpy(), want, rtol=1e-6, atol=1e-7)
     assert want[..., 0].max() > 1.0  # emissive texels are present
+
+
+def test_relit_surface_cache_feeds_the_probe_trace():
+    """End to end on the GPU: the surface-cache relight (emissive + three lights with SDF shadow
+    rays) writes the RGBA16F light cache, which the probe trace then reads as the atlas colour
+    (the engine's frame order: GlobalSurfaceAtlas lighting, then GISDFRays); against the oracle
+    trace reading the same light cache. Trace tolerances as in the other trace tests."""
+    import copy
+
+    from luxgi_amd.ddgi import DDGIVolume, DeviceScene
+    from luxgi_amd.surface import SurfaceCache
+    from tests.test_gpu_trace import _compare
+    prims, sdf, mip, g, _ = _scenes.small_scene()
+    res = 64
+    atlas = scene.build_atlas(prims, resolution=512, tile=32, seed=3, threshold=1.05 * 2 / res, surface_gbuffer=True)
+    ds = DeviceScene(g, sdf, mip, atlas)
+    sc = SurfaceCache(atlas, out_storage=abi.STORAGE_FP16)
+    sc.set_lights(_lights())
+    sc.light(ds, passes=abi.SURFACE_EMISSIVE | abi.SURFACE_DIRECT, camera=(0.1, 0.3, 0.75), shadow_bias=0.03)
+    torch.cuda.synchronize()
+    lit = sc.light_cache
+    assert float(lit.float().abs().sum()) > 0
+    ds.atlas.color = lit.data_ptr()
+    ds.atlas.colorStorage = abi.STORAGE_FP16
+    u = _scenes.small_uniform(rays=96, counts=(5, 4, 6))
+    p = _scenes.trace_params(rot_seed=13)
+    vol = DDGIVolume(u, flags_buffer=True)
+    vol.trace_rays(ds, p)
+    torch.cuda.synchronize()
+    got = (vol.radiance.float().cpu().numpy(), vol.dir_dist.float().cpu().numpy(), vol.ray_flags.cpu().numpy(), None)
+    host = copy.copy(atlas)
+    host.color_f32 = np.ascontiguousarray(lit.float().cpu().numpy())
+    ref = _scenes.oracle_trace(u, g, sdf, mip, host, p)
+    _compare(u, g, got, ref, 2.0 / res)
+    hits = got[2] == 1
+    assert hits.mean() > 0.3 and np.abs(got[0][hits][:, :3]).max() > 0.05  # lit surfaces seen by the rays

@@ -206,12 +206,21 @@ typedef struct LuxgiRayBuffers {
 } LuxgiRayBuffers;
 
 /* One ping-pong side of the probe atlases (DDGIRenderer.cpp:179-209):
- * irradiance RGBA, depth RG, sizes taken from the uniform. */
+ * irradiance RGBA, depth RG, sizes taken from the uniform.
+ * probeOffsets / probeStates are the output of luxgi_ddgi_relocate_classify and are read by the
+ * atlas consumers (luxgi_ddgi_sample_probes, the bounce of luxgi_surface_light, the DDGI branch
+ * of luxgi_sdf_reflection): a probe sits at its grid position + offset, and an inactive probe
+ * (state 1) takes no weight; a point whose contributing probes are all inactive gets 0, as in
+ * the published classification rule (Majercik et al., JCGT 2021). NULL = the reference's
+ * sampleIrradiance (DDGICommon.glsl:163-233): grid positions, every probe active. The update and
+ * border passes ignore both. */
 typedef struct LuxgiProbeAtlas {
     void*   irradiance;
     void*   depth;
     int32_t storage;  /* FP32 or FP16 */
     int32_t pad;
+    const float*   probeOffsets; /* optional float4[P] (xyz used), as written by relocation */
+    const uint8_t* probeStates;  /* optional uint8[P]: 0 active, 1 inactive */
 } LuxgiProbeAtlas;
 
 /* Probe-update flags */

@@ -109,7 +109,8 @@ class RayBuffers(C.Structure):
 
 
 class ProbeAtlas(C.Structure):
-    _fields_ = [("irradiance", C.c_void_p), ("depth", C.c_void_p), ("storage", C.c_int32), ("pad", C.c_int32)]
+    _fields_ = [("irradiance", C.c_void_p), ("depth", C.c_void_p), ("storage", C.c_int32), ("pad", C.c_int32),
+                ("probeOffsets", C.c_void_p), ("probeStates", C.c_void_p)]
 
 
 class RelocateParams(C.Structure):

@@ -78,10 +78,23 @@ __device__ __forceinline__ float2 bilinear2(const S* __restrict__ img, int W, in
                        (t00.y * (1.0f - ax) + t10.y * ax) * (1.0f - ay) + (t01.y * (1.0f - ax) + t11.y * ax) * ay);
 }
 
-// sampleIrradiance (DDGICommon.glsl:163-233)
+// Relocation and classification read by the consumers (LuxgiProbeAtlas.probeOffsets/probeStates);
+// null pointers give the reference's grid positions with every probe active.
+struct ProbeReloc {
+    const float4* offsets;
+    const uint8_t* states;
+};
+__host__ __device__ inline ProbeReloc probe_reloc(const LuxgiProbeAtlas* a) {
+    return a ? ProbeReloc{reinterpret_cast<const float4*>(a->probeOffsets), a->probeStates} : ProbeReloc{nullptr, nullptr};
+}
+
+// sampleIrradiance (DDGICommon.glsl:163-233). With relocation (rl.offsets) each neighbour probe is
+// read at its grid position + offset (the trilinear weights stay on the grid); with classification
+// (rl.states) inactive neighbours are skipped, and a point with no weighted neighbour gets 0 -- the
+// irradiance gather of the published rule (Majercik et al., JCGT 2021). The reference has neither.
 template <class SA>
 __device__ f3 sample_irradiance(const LuxgiDDGIUniform& u, const AtlasGeom& irrG, const AtlasGeom& depG, f3 P, f3 N,
-                                f3 Wo, const SA* __restrict__ irr, const SA* __restrict__ dep) {
+                                f3 Wo, const SA* __restrict__ irr, const SA* __restrict__ dep, ProbeReloc rl = {}) {
     const f3 start = mk(u.startPosition[0], u.startPosition[1], u.startPosition[2]);
     const f3 stp = mk(u.step[0], u.step[1], u.step[2]);
     const int bx = clampi(int((P.x - start.x) / stp.x), 0, u.probeCounts[0] - 1);
@@ -99,14 +112,19 @@ __device__ f3 sample_irradiance(const LuxgiDDGIUniform& u, const AtlasGeom& irrG
         const int ox = i & 1, oy = (i >> 1) & 1, oz = (i >> 2) & 1;
         const int gx = min(bx + ox, u.probeCounts[0] - 1), gy = min(by + oy, u.probeCounts[1] - 1),
                   gz = min(bz + oz, u.probeCounts[2] - 1);
-        const f3 probePos = mk(stp.x * float(gx) + start.x, stp.y * float(gy) + start.y, stp.z * float(gz) + start.z);
+        const int probeIdx = gx + gy * u.probeCounts[0] + gz * u.probeCounts[0] * u.probeCounts[1];
+        if (rl.states && rl.states[probeIdx] != 0) continue;
+        f3 probePos = mk(stp.x * float(gx) + start.x, stp.y * float(gy) + start.y, stp.z * float(gz) + start.z);
+        if (rl.offsets) {
+            const float4 o = rl.offsets[probeIdx];
+            probePos = probePos + mk(o.x, o.y, o.z);
+        }
         const float tri = (ox ? alpha.x : 1.0f - alpha.x) * (oy ? alpha.y : 1.0f - alpha.y) *
                           (oz ? alpha.z : 1.0f - alpha.z);
         float w = 1.0f;
         const f3 dirToProbe = fast_normalize(probePos - P);
         const float sb = fmaxf(0.0001f, (dot3(dirToProbe, N) + 1.0f) * 0.5f);
         w *= sb * sb + 0.2f;
-        const int probeIdx = gx + gy * u.probeCounts[0] + gz * u.probeCounts[0] * u.probeCounts[1];
         const f3 probeToPoint = P - probePos + vBias;
         // textureCoordFromDirection(-normalize(-probeToPoint)) == the direction of probeToPoint
         float2 tc = atlas_coord(probeToPoint, probeIdx, depG);
@@ -127,6 +145,7 @@ __device__ f3 sample_irradiance(const LuxgiDDGIUniform& u, const AtlasGeom& irrG
         sum = sum + pi * w;
         sumW += w;
     }
+    if (sumW == 0.0f) return mk(0.f, 0.f, 0.f);  // every weighted neighbour inactive
     f3 net = sum * (1.0f / sumW);
     net = net * net;
     return net * (2.0f * kPiF);

@@ -18,6 +18,7 @@ struct SampleArgs {
     LuxgiDDGIUniform u;
     LuxgiSampleProbeParams pc;
     AtlasGeom irr, dep;
+    ProbeReloc rl;
     int W, H;
 };
 
@@ -36,7 +37,7 @@ __global__ __launch_bounds__(256) void sample_probes_kernel(SampleArgs A, const 
         const float2 on = reinterpret_cast<const float2*>(normals)[2 * e];
         const f3 N = oct_decode(on.x, on.y);
         const f3 Wo = fast_normalize(mk(A.pc.cameraPosition[0], A.pc.cameraPosition[1], A.pc.cameraPosition[2]) - P);
-        const f3 ir = sample_irradiance<SA>(A.u, A.irr, A.dep, P, N, Wo, irr, dep);
+        const f3 ir = sample_irradiance<SA>(A.u, A.irr, A.dep, P, N, Wo, irr, dep, A.rl);
         o = make_float4(ir.x, ir.y, ir.z, 1.0f);
     }
     Texel<SO>::st4(out, e, o);
@@ -69,6 +70,7 @@ extern "C" int luxgi_ddgi_sample_probes(const LuxgiDDGIUniform* u, const LuxgiPr
     A.pc = *params;
     A.irr = atlas_geom(u->irradianceTextureWidth, u->irradianceTextureHeight, u->irradianceProbeSideLength);
     A.dep = atlas_geom(u->depthTextureWidth, u->depthTextureHeight, u->depthProbeSideLength);
+    A.rl = probe_reloc(atlas);
     A.W = W;
     A.H = H;
     hipStream_t st = as_stream(stream);

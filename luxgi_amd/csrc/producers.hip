@@ -126,6 +126,7 @@ struct ReflArgsP {
     LuxgiSurfaceAtlasData ad;
     LuxgiDDGIUniform u;
     AtlasGeom irrG, depG;
+    ProbeReloc rl;
     int W, H, maxSteps, skySize, outHalf, ddgi;
     float sky[4];
 };
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(256) void reflection_kernel(ReflArgsP A, const floa
         R = reflect3(mk(-Wo.x, -Wo.y, -Wo.z), n);
         traceRay = false;
         if (A.ddgi) {
-            const f3 ir = sample_irradiance<SA>(A.u, A.irrG, A.depG, wp, R, Wo, irr, dep);
+            const f3 ir = sample_irradiance<SA>(A.u, A.irrG, A.depG, wp, R, Wo, irr, dep, A.rl);
             col = make_float4(A.p.roughDDGIIntensity * ir.x, A.p.roughDDGIIntensity * ir.y,
                               A.p.roughDDGIIntensity * ir.z, 0.0f);
         }
@@ -350,6 +351,7 @@ extern "C" int luxgi_sdf_reflection(const LuxgiSdfReflectionParams* params, cons
     }
     const float* skyFaces = sky && sky->faces ? sky->faces : nullptr;
     const LuxgiProbeAtlas* pr = useDdgi ? probes : nullptr;
+    A.rl = probe_reloc(pr);
     const bool half = sdf->storage == LUXGI_STORAGE_FP16, brick = sdf->layout == LUXGI_LAYOUT_BRICK4;
     int rc;
     if (half && brick) rc = launch_reflection_sc<__half, true>(A, gb, noise, sdf, at, inv, skyFaces, pr, out, s);

@@ -26,6 +26,7 @@ struct SurfaceArgs {
     LuxgiGlobalSDFData g;
     LuxgiDDGIUniform u;
     AtlasGeom irrG, depG;
+    ProbeReloc rl;
     int res;       // atlas resolution
     int sdfRes, mipRes, maxSteps;
     int indirect;  // probe atlases bound and the indirect pass requested
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(256) void surface_light_kernel(SurfaceArgs A, const
                 const float m = pbr.x;
                 const f3 alb = mk(fminf(c.x, 0.9f), fminf(c.y, 0.9f), fminf(c.z, 0.9f));
                 const f3 diff = mk((alb.x - alb.x * m) / kPiF, (alb.y - alb.y * m) / kPiF, (alb.z - alb.z * m) / kPiF);
-                const f3 ir = sample_irradiance<SA>(A.u, A.irrG, A.depG, wp, n, view, irr, dep);
+                const f3 ir = sample_irradiance<SA>(A.u, A.irrG, A.depG, wp, n, view, irr, dep, A.rl);
                 acc = acc + mk(A.p.indirectIntensity * diff.x * ir.x, A.p.indirectIntensity * diff.y * ir.y,
                                A.p.indirectIntensity * diff.z * ir.z);
             }
@@ -246,6 +247,7 @@ extern "C" int luxgi_surface_light(const LuxgiSurfaceLightParams* params, const 
     hipStream_t st = as_stream(stream);
     const LuxgiSDFVolume* v = direct ? vol : nullptr;
     const LuxgiProbeAtlas* pr = indirect ? probes : nullptr;
+    A.rl = probe_reloc(pr);
     const bool half = v && v->storage == LUXGI_STORAGE_FP16, brick = v && v->layout == LUXGI_LAYOUT_BRICK4;
     if (half && brick) launch_surface_sa<__half, true>(A, lights, tiles, numTiles, draws, numDraws, gb, v, pr, lightCache, st);
     else if (half) launch_surface_sa<__half, false>(A, lights, tiles, numTiles, draws, numDraws, gb, v, pr, lightCache, st);

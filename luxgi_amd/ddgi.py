@@ -152,14 +152,26 @@ class DDGIVolume:
         self.ray_flags = torch.zeros((P, self.ray_width), dtype=torch.uint8, device="cuda") if flags_buffer else None
         self.offsets = torch.zeros((P, 4), dtype=torch.float32, device="cuda")
         self.states = torch.zeros(P, dtype=torch.uint8, device="cuda")
+        self.relocation = False
         ws = lib().luxgi_ddgi_probe_update_workspace_size(C.byref(uniform))
         self.workspace = torch.empty(max(ws, 256), dtype=torch.uint8, device="cuda")
         self.trace_workspace = None
         self.frames = 0
         self.ping_pong = 0
 
+    def enable_relocation(self, on=True):
+        """Relocation + classification in effect: the trace starts rays at the offset probe positions
+        and the consumers (sample_probes, the surface-cache bounce and the reflection rays' DDGI
+        branch, through the atlas descriptors) read the offsets and skip inactive probes."""
+        self.relocation = bool(on)
+        for a in self.atlas:
+            a.probeOffsets = self.offsets.data_ptr() if on else None
+            a.probeStates = self.states.data_ptr() if on else None
+
     # trace_rays::system (DDGIRenderer.cpp:217-330)
-    def trace_rays(self, scene, params, use_offsets=False, counters=None):
+    def trace_rays(self, scene, params, use_offsets=None, counters=None):
+        if use_offsets is None:
+            use_offsets = self.relocation
         off = ptr(self.offsets) if use_offsets else None
         need = lib().luxgi_ddgi_trace_workspace_size(C.byref(self.u), scene.n_objects)
         if self.trace_workspace is None or self.trace_workspace.numel() < need:

@@ -66,7 +66,7 @@ def _declare(L):
                                                     P, P, P, P, I]
         getattr(L, "oracle_reflection_reproject_" + s).argtypes = [P, I, I, P, P, P, P, P, P, P, P, I]
         getattr(L, "oracle_reflection_atrous_" + s).argtypes = [P, I, I, P, P, P, P, P, I]
-        getattr(L, "oracle_sample_probes_" + s).argtypes = [P, P, I, I, P, P, P, P, P, I]
+        getattr(L, "oracle_sample_probes_" + s).argtypes = [P, P, I, I, P, P, P, P, P, P, P, I]
     L.oracle_sdf_mip.argtypes = [P, I, I, I, C.c_float, I, P, P]
     L.oracle_shadow_reproject_f64.argtypes = [P, I, I, P, P, P, P, P, P, P, P, I]
     L.oracle_shadow_atrous_f64.argtypes = [P, I, I, P, P, P, P, P, I]
@@ -74,7 +74,7 @@ def _declare(L):
     L.oracle_cull_chunks.argtypes = [C.c_float, P, I, C.c_uint32, P, P]
     for s in ("f64", "f32"):
         getattr(L, "oracle_surface_light_" + s).argtypes = [P, P, P, I, P, I, I, P, P, P, P, P, P, P, I, I, P, P, P,
-                                                            P, I]
+                                                            P, P, P, I]
     L.oracle_brdf.argtypes = [P, P, C.c_double, C.c_double, P, P, P, P]
     L.oracle_shadow_mask_f64.argtypes = [P, P, P, I, I, P, P, P, P, P, I, I, P, I]
     L.oracle_blue_noise.argtypes = [P, P, I, I, I, I]
@@ -86,7 +86,7 @@ def _declare(L):
     L.oracle_round_to_half.argtypes = [C.c_float]
     L.oracle_round_to_half.restype = C.c_float
     L.oracle_sdf_reflection_f64.argtypes = [P, P, P, I, I, P, P, P, P, P, P, I, I, P, P, P, P, P, P, P, P, I, P, P, P,
-                                            P, I]
+                                            P, P, P, I]
 
 
 def _p(a):
@@ -324,14 +324,23 @@ def relocate(u, dir_dist, flags, params, offsets):
 # ---------------------------------------------------------------------------
 # probe sampling (SampleProbe.comp + sampleIrradiance)
 # ---------------------------------------------------------------------------
-def sample_probes(u, params, W, H, depth, normals, irradiance, depth_atlas, precision="f64", nthreads=None):
-    """Returns [H][W][4] (irradiance, 1) per pixel, 0 on sky."""
+def _reloc(offsets, states):
+    off = None if offsets is None else np.ascontiguousarray(offsets, dtype=np.float32).reshape(-1, 4)
+    st = None if states is None else np.ascontiguousarray(states, dtype=np.uint8)
+    return off, st
+
+
+def sample_probes(u, params, W, H, depth, normals, irradiance, depth_atlas, precision="f64", nthreads=None,
+                  offsets=None, states=None):
+    """Returns [H][W][4] (irradiance, 1) per pixel, 0 on sky. offsets (float4 per probe) / states
+    (uint8 per probe) apply relocation / classification."""
     dt = np.float64 if precision == "f64" else np.float32
     depth, normals = _f32(depth), _f32(normals)
     irradiance, depth_atlas = _f32(irradiance), _f32(depth_atlas)
     out = np.zeros((H, W, 4), dtype=dt)
+    off, st = _reloc(offsets, states)
     getattr(lib(), "oracle_sample_probes_" + precision)(C.byref(u), C.byref(params), W, H, _p(depth), _p(normals),
-                                                        _p(irradiance), _p(depth_atlas), _p(out),
+                                                        _p(irradiance), _p(depth_atlas), _p(off), _p(st), _p(out),
                                                         nthreads or threads())
     return out
 
@@ -360,7 +369,7 @@ def cull_chunks(chunk_size, objects, n, capacity):
 
 def surface_light(params, lights, tiles, draws, res, gb_color, gb_normal, gb_pbr, gb_emissive, sdf_data=None,
                   sdf=None, mip=None, sdf_res=0, mip_res=0, u=None, irradiance=None, depth_atlas=None,
-                  precision="f64", nthreads=None, out=None):
+                  precision="f64", nthreads=None, out=None, offsets=None, states=None):
     """Surface-cache lighting of the drawn tiles. lights/tiles/draws: ctypes arrays; G-buffer
     images float [res][res][4] (values as the GPU reads them). Returns [res][res][4]; texels
     outside the draws keep `out`'s values (zeros by default)."""
@@ -373,11 +382,12 @@ def surface_light(params, lights, tiles, draws, res, gb_color, gb_normal, gb_pbr
     mip = None if mip is None else _f32(mip)
     irr = None if irradiance is None else _f32(irradiance)
     dep = None if depth_atlas is None else _f32(depth_atlas)
+    off, st = _reloc(offsets, states)
     fn = getattr(lib(), "oracle_surface_light_" + precision)
     fn(C.byref(params), C.cast(lights, C.c_void_p) if lights is not None else None, C.cast(tiles, C.c_void_p),
        len(tiles), C.cast(draws, C.c_void_p), len(draws), int(res), *[_p(a) for a in imgs],
        C.byref(sdf_data) if sdf_data is not None else None, _p(sdf), _p(mip), int(sdf_res), int(mip_res),
-       C.byref(u) if u is not None else None, _p(irr), _p(dep), _p(out), nthreads or threads())
+       C.byref(u) if u is not None else None, _p(irr), _p(dep), _p(off), _p(st), _p(out), nthreads or threads())
     return out
 
 
@@ -414,7 +424,8 @@ def shadow_mask(params, sobol, scrambling, W, H, depth, normals, sdf_data, sdf, 
 
 
 def sdf_reflection(params, sobol, scrambling, W, H, normals, pbr, depth, sdf_data, sdf, mip, res, mip_res, atlas,
-                   sky_faces=None, sky_size=0, u=None, irradiance=None, depth_atlas=None, out=None, nthreads=None):
+                   sky_faces=None, sky_size=0, u=None, irradiance=None, depth_atlas=None, out=None, nthreads=None,
+                   offsets=None, states=None):
     """SDFReflection.comp over the image; atlas: scene.Atlas (colour as the GPU reads it in color_f32).
     Returns [H][W][4] float64; sky pixels keep `out`'s value (zeros by default)."""
     o = np.zeros((H, W, 4)) if out is None else np.ascontiguousarray(out, dtype=np.float64).copy()
@@ -426,12 +437,13 @@ def sdf_reflection(params, sobol, scrambling, W, H, normals, pbr, depth, sdf_dat
     sky = None if sky_faces is None else _f32(sky_faces)
     irr = None if irradiance is None else _f32(irradiance)
     dep = None if depth_atlas is None else _f32(depth_atlas)
+    off, st = _reloc(offsets, states)
     lib().oracle_sdf_reflection_f64(C.byref(params), _p(so), _p(sc), int(W), int(H), _p(_f32(normals)), _p(_f32(pbr)),
                                     _p(_f32(depth)), C.byref(sdf_data), _p(_f32(sdf)), _p(_f32(mip)), int(res),
                                     int(mip_res), C.byref(atlas.data), _p(chunks), _p(culled),
                                     C.cast(atlas.objects, C.c_void_p), C.cast(atlas.tiles, C.c_void_p), _p(adep),
                                     _p(acol), _p(sky), int(sky_size), C.byref(u) if u is not None else None, _p(irr),
-                                    _p(dep), _p(o), nthreads or threads())
+                                    _p(dep), _p(off), _p(st), _p(o), nthreads or threads())
     return o
 
 

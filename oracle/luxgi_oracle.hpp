@@ -1070,9 +1070,13 @@ template <class T> inline v2<T> texture_coord_from_direction(v3<T> dir, int prob
 }
 
 // sampleIrradiance (DDGICommon.glsl:163-233); irr = float4 atlas, dep = float2 atlas.
+// offsets (float4 per probe) / states (uint8 per probe, 1 = inactive), both optional, extend it with
+// relocation and classification as the published irradiance gather does (Majercik et al., JCGT
+// 2021; the reference has neither): neighbour probes at grid position + offset, trilinear weights
+// on the grid, inactive neighbours skipped, 0 when no neighbour carries weight.
 template <class T>
 inline v3<T> sample_irradiance(const LuxgiDDGIUniform& u, v3<T> P, v3<T> N, v3<T> Wo, const float* irr,
-                               const float* dep) {
+                               const float* dep, const float* offsets = nullptr, const uint8_t* states = nullptr) {
     const v3<T> start{T(u.startPosition[0]), T(u.startPosition[1]), T(u.startPosition[2])};
     const v3<T> stp{T(u.step[0]), T(u.step[1]), T(u.step[2])};
     int base[3];
@@ -1087,14 +1091,19 @@ inline v3<T> sample_irradiance(const LuxgiDDGIUniform& u, v3<T> P, v3<T> N, v3<T
         const int off[3] = {i & 1, (i >> 1) & 1, (i >> 2) & 1};
         int g[3];
         for (int k = 0; k < 3; ++k) g[k] = std::min(std::max(base[k] + off[k], 0), u.probeCounts[k] - 1);
-        const v3<T> probePos{stp.x * T(g[0]) + start.x, stp.y * T(g[1]) + start.y, stp.z * T(g[2]) + start.z};
+        const int probeIdx = grid_to_probe_index(u, g[0], g[1], g[2]);
+        if (states && states[probeIdx] != 0) continue;
+        v3<T> probePos{stp.x * T(g[0]) + start.x, stp.y * T(g[1]) + start.y, stp.z * T(g[2]) + start.z};
+        if (offsets) {
+            const float* o = offsets + (size_t)probeIdx * 4;
+            probePos = probePos + v3<T>{T(o[0]), T(o[1]), T(o[2])};
+        }
         const v3<T> tri{off[0] ? alpha.x : T(1) - alpha.x, off[1] ? alpha.y : T(1) - alpha.y,
                         off[2] ? alpha.z : T(1) - alpha.z};
         T w = 1;
         const v3<T> dirToProbe = normalize(probePos - P);
         const T sb = gmax(T(0.0001), (dot(dirToProbe, N) + T(1)) * T(0.5));
         w *= sb * sb + T(0.2);
-        const int probeIdx = grid_to_probe_index(u, g[0], g[1], g[2]);
         const v3<T> vBias = (N + Wo * T(3)) * T(u.normalBias);
         const v3<T> probeToPoint = P - probePos + vBias;
         const v3<T> dir = normalize(probeToPoint * T(-1));
@@ -1122,6 +1131,7 @@ inline v3<T> sample_irradiance(const LuxgiDDGIUniform& u, v3<T> P, v3<T> N, v3<T
         sum = sum + pi * w;
         sumW += w;
     }
+    if (sumW == T(0)) return {0, 0, 0};
     v3<T> net = sum * (T(1) / sumW);
     net = net * net;
     return net * T(2 * kPi);
@@ -1131,7 +1141,7 @@ inline v3<T> sample_irradiance(const LuxgiDDGIUniform& u, v3<T> P, v3<T> N, v3<T
 template <class T>
 inline void sample_probe_pixel(const LuxgiDDGIUniform& u, const LuxgiSampleProbeParams& pc, int W, int H, int x,
                                int y, const float* depth, const float* normals, const float* irr, const float* dep,
-                               T* out) {
+                               T* out, const float* offsets = nullptr, const uint8_t* states = nullptr) {
     const T d = T(depth[(size_t)y * W + x]);
     if (d == T(1)) { out[0] = out[1] = out[2] = out[3] = 0; return; }
     const T tu = (T(x) + T(0.5)) / T(W), tv = (T(y) + T(0.5)) / T(H);
@@ -1140,7 +1150,7 @@ inline void sample_probe_pixel(const LuxgiDDGIUniform& u, const LuxgiSampleProbe
     const v3<T> N = octahedral_to_direction<T>(T(nn[0]), T(nn[1]));
     const v3<T> cam{T(pc.cameraPosition[0]), T(pc.cameraPosition[1]), T(pc.cameraPosition[2])};
     const v3<T> Wo = normalize(cam - P);
-    const v3<T> e = sample_irradiance<T>(u, P, N, Wo, irr, dep);
+    const v3<T> e = sample_irradiance<T>(u, P, N, Wo, irr, dep, offsets, states);
     out[0] = e.x; out[1] = e.y; out[2] = e.z; out[3] = T(1);
 }
 
@@ -1258,7 +1268,8 @@ inline void surface_light_texel(const LuxgiSurfaceLightParams& prm, const LuxgiL
                                 const LuxgiAtlasTile& tile, int res, int px, int py, const float* gbColor,
                                 const float* gbNormal, const float* gbPbr, const float* gbEmissive,
                                 const LuxgiGlobalSDFData& g, const float* sdf, const float* mip, int sdfRes,
-                                int mipRes, const LuxgiDDGIUniform* u, const float* irr, const float* dep, T out[4]) {
+                                int mipRes, const LuxgiDDGIUniform* u, const float* irr, const float* dep, T out[4],
+                                const float* offsets = nullptr, const uint8_t* states = nullptr) {
     const int X = (int)std::lrint(tile.extends[0] * (float)res), Y = (int)std::lrint(tile.extends[1] * (float)res);
     const int Wt = (int)std::lrint(tile.extends[2] * (float)res) + 1;
     const int Ht = (int)std::lrint(tile.extends[3] * (float)res) + 1;
@@ -1312,7 +1323,7 @@ inline void surface_light_texel(const LuxgiSurfaceLightParams& prm, const LuxgiL
         if (indirect) {
             const T m = pbr[0];
             const T al[3] = {std::min(c[0], T(0.9f)), std::min(c[1], T(0.9f)), std::min(c[2], T(0.9f))};
-            const v3<T> ir = sample_irradiance<T>(*u, wp, n, view, irr, dep);
+            const v3<T> ir = sample_irradiance<T>(*u, wp, n, view, irr, dep, offsets, states);
             const T I = T(prm.indirectIntensity);
             acc[0] += I * ((al[0] - al[0] * m) / T(kPi)) * ir.x;
             acc[1] += I * ((al[1] - al[1] * m) / T(kPi)) * ir.y;
@@ -1467,6 +1478,8 @@ struct ReflectionInputs {
     int skySize;
     const LuxgiDDGIUniform* u;
     const float *irr, *dep;
+    const float* offsets = nullptr;     // optional relocation / classification of the probes
+    const uint8_t* states = nullptr;
 };
 
 // One pixel; returns false for sky (the texel is not written).
@@ -1492,7 +1505,7 @@ inline bool reflection_pixel(const ReflectionInputs& in, int x, int y, T out[4])
     } else if (roughness > T(0.45f) && p.approximateWithDDGI == 1u) {
         R = reflect3<T>(mWo, n);
         traceRay = false;
-        const v3<T> ir = sample_irradiance<T>(*in.u, wp, R, Wo, in.irr, in.dep);
+        const v3<T> ir = sample_irradiance<T>(*in.u, wp, R, Wo, in.irr, in.dep, in.offsets, in.states);
         col = {T(p.roughDDGIIntensity) * ir.x, T(p.roughDDGIIntensity) * ir.y, T(p.roughDDGIIntensity) * ir.z, T(0)};
     } else {
         const T ex = T(blue_noise(in.sobol, in.scr, x, y, (int)p.numFrames, 0)) * T(p.trim);

@@ -308,22 +308,23 @@ extern "C" void oracle_relocate(const LuxgiDDGIUniform* u, const float* dirDist,
 
 template <class T>
 static void sample_probes_impl(const LuxgiDDGIUniform* u, const LuxgiSampleProbeParams* pc, int W, int H,
-                               const float* depth, const float* normals, const float* irr, const float* dep, T* out,
-                               int threads) {
+                               const float* depth, const float* normals, const float* irr, const float* dep,
+                               const float* offsets, const uint8_t* states, T* out, int threads) {
     parallel_for(H, threads, [&](int y) {
         for (int x = 0; x < W; ++x)
-            sample_probe_pixel<T>(*u, *pc, W, H, x, y, depth, normals, irr, dep, out + ((size_t)y * W + x) * 4);
+            sample_probe_pixel<T>(*u, *pc, W, H, x, y, depth, normals, irr, dep, out + ((size_t)y * W + x) * 4,
+                                  offsets, states);
     });
 }
 extern "C" void oracle_sample_probes_f64(const LuxgiDDGIUniform* u, const LuxgiSampleProbeParams* pc, int W, int H,
                                          const float* depth, const float* normals, const float* irr, const float* dep,
-                                         double* out, int threads) {
-    sample_probes_impl<double>(u, pc, W, H, depth, normals, irr, dep, out, threads);
+                                         const float* offsets, const uint8_t* states, double* out, int threads) {
+    sample_probes_impl<double>(u, pc, W, H, depth, normals, irr, dep, offsets, states, out, threads);
 }
 extern "C" void oracle_sample_probes_f32(const LuxgiDDGIUniform* u, const LuxgiSampleProbeParams* pc, int W, int H,
                                          const float* depth, const float* normals, const float* irr, const float* dep,
-                                         float* out, int threads) {
-    sample_probes_impl<float>(u, pc, W, H, depth, normals, irr, dep, out, threads);
+                                         const float* offsets, const uint8_t* states, float* out, int threads) {
+    sample_probes_impl<float>(u, pc, W, H, depth, normals, irr, dep, offsets, states, out, threads);
 }
 extern "C" void oracle_texture_coord_from_direction(const double* dir, int probeIndex, int width, int height, int side,
                                                     double* out) {
@@ -345,7 +346,8 @@ static void surface_light_impl(const LuxgiSurfaceLightParams* prm, const LuxgiLi
                                int numTiles, const LuxgiSurfaceDraw* draws, int numDraws, int res, const float* gbColor,
                                const float* gbNormal, const float* gbPbr, const float* gbEmissive,
                                const LuxgiGlobalSDFData* g, const float* sdf, const float* mip, int sdfRes, int mipRes,
-                               const LuxgiDDGIUniform* u, const float* irr, const float* dep, T* out, int threads) {
+                               const LuxgiDDGIUniform* u, const float* irr, const float* dep, const float* offsets,
+                               const uint8_t* states, T* out, int threads) {
     parallel_for(numDraws, threads, [&](int d) {
         const LuxgiSurfaceDraw dr = draws[d];
         if (dr.tile >= (uint32_t)numTiles) return;
@@ -356,7 +358,8 @@ static void surface_light_impl(const LuxgiSurfaceLightParams* prm, const LuxgiLi
             for (int px = X; px < X + Wt; ++px) {
                 if (px < 0 || py < 0 || px >= res || py >= res) continue;
                 surface_light_texel<T>(*prm, lights, dr.lightMask, t, res, px, py, gbColor, gbNormal, gbPbr, gbEmissive,
-                                       *g, sdf, mip, sdfRes, mipRes, u, irr, dep, out + ((size_t)py * res + px) * 4);
+                                       *g, sdf, mip, sdfRes, mipRes, u, irr, dep, out + ((size_t)py * res + px) * 4,
+                                       offsets, states);
             }
     });
 }
@@ -365,20 +368,20 @@ extern "C" void oracle_surface_light_f64(const LuxgiSurfaceLightParams* prm, con
                                          int numDraws, int res, const float* gbColor, const float* gbNormal,
                                          const float* gbPbr, const float* gbEmissive, const LuxgiGlobalSDFData* g,
                                          const float* sdf, const float* mip, int sdfRes, int mipRes,
-                                         const LuxgiDDGIUniform* u, const float* irr, const float* dep, double* out,
-                                         int threads) {
+                                         const LuxgiDDGIUniform* u, const float* irr, const float* dep,
+                                         const float* offsets, const uint8_t* states, double* out, int threads) {
     surface_light_impl<double>(prm, lights, tiles, numTiles, draws, numDraws, res, gbColor, gbNormal, gbPbr, gbEmissive,
-                               g, sdf, mip, sdfRes, mipRes, u, irr, dep, out, threads);
+                            g, sdf, mip, sdfRes, mipRes, u, irr, dep, offsets, states, out, threads);
 }
 extern "C" void oracle_surface_light_f32(const LuxgiSurfaceLightParams* prm, const LuxgiLight* lights,
                                          const LuxgiAtlasTile* tiles, int numTiles, const LuxgiSurfaceDraw* draws,
                                          int numDraws, int res, const float* gbColor, const float* gbNormal,
                                          const float* gbPbr, const float* gbEmissive, const LuxgiGlobalSDFData* g,
                                          const float* sdf, const float* mip, int sdfRes, int mipRes,
-                                         const LuxgiDDGIUniform* u, const float* irr, const float* dep, float* out,
-                                         int threads) {
+                                         const LuxgiDDGIUniform* u, const float* irr, const float* dep,
+                                         const float* offsets, const uint8_t* states, float* out, int threads) {
     surface_light_impl<float>(prm, lights, tiles, numTiles, draws, numDraws, res, gbColor, gbNormal, gbPbr, gbEmissive,
-                              g, sdf, mip, sdfRes, mipRes, u, irr, dep, out, threads);
+                            g, sdf, mip, sdfRes, mipRes, u, irr, dep, offsets, states, out, threads);
 }
 // Raytraced/BRDF.glsl:65-79 at one set of inputs (unit test hook)
 extern "C" void oracle_brdf(const double* albedo, const double* n, double rough, double metal, const double* view,
@@ -428,11 +431,11 @@ extern "C" void oracle_sdf_reflection_f64(const LuxgiSdfReflectionParams* p, con
                                           const uint32_t* culled, const LuxgiAtlasObject* objects,
                                           const LuxgiAtlasTile* tiles, const float* atlasDepth,
                                           const float* atlasColor, const float* skyFaces, int skySize,
-                                          const LuxgiDDGIUniform* u, const float* irr, const float* dep, double* out,
-                                          int threads) {
+                                          const LuxgiDDGIUniform* u, const float* irr, const float* dep,
+                                          const float* offsets, const uint8_t* states, double* out, int threads) {
     ReflectionInputs in{p,   sobol,  scr,   W,     H,       normals,    pbr,        depth, g,       sdf,  mip,
                         res, mipRes, ad,    chunks, culled, objects,    tiles,      atlasDepth,    atlasColor,
-                        skyFaces, skySize, u, irr, dep};
+                        skyFaces, skySize, u, irr, dep, offsets, states};
     parallel_for(H, threads, [&](int y) {
         for (int x = 0; x < W; ++x) reflection_pixel<double>(in, x, y, out + ((size_t)y * W + x) * 4);
     });

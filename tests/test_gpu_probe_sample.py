@@ -78,3 +78,52 @@ def test_sample_probes_after_update_fp16_output():
     o16 = vol.sample_probes(depth, normals, cam["eye"], cam["VPI"], out_storage=abi.STORAGE_FP16).cpu().numpy()
     assert np.isfinite(o32).all() and o32[..., :3].max() > 0
     np.testing.assert_array_equal(o16.astype(np.float32), o32.astype(np.float16).astype(np.float32))
+
+
+@pytest.mark.parametrize("storage", [abi.STORAGE_FP32, abi.STORAGE_FP16])
+def test_sample_probes_relocated_and_classified_matches_oracle(storage):
+    """Probe offsets (up to 0.4 of the spacing) and 30% inactive probes in the atlas descriptor:
+    the GPU gather reads neighbours at grid + offset and skips inactive ones, as the oracle does
+    (tests/test_oracle_probe_sample.py pins that rule). Same tolerances as above; pixels whose 8
+    neighbours are all inactive are exactly 0 on both sides."""
+    from luxgi_amd.ddgi import DDGIVolume
+    u = _scenes.small_uniform(rays=64, counts=(6, 5, 7))
+    irr, dep = _atlases(u, 8)
+    W, H = 96, 64
+    g, prev, noisy, hist, hmom, mask, cam = _svgf_inputs.make_pair(W, H, 8)
+    rng = np.random.default_rng(8)
+    off = np.zeros((u.num_probes, 4), np.float32)
+    off[:, :3] = rng.uniform(-0.4, 0.4, (u.num_probes, 3)) * np.array([u.step[0], u.step[1], u.step[2]])
+    states = (rng.uniform(size=u.num_probes) < 0.3).astype(np.uint8)
+    vol = DDGIVolume(u, atlas_storage=storage)
+    tdt = torch.float16 if storage == abi.STORAGE_FP16 else torch.float32
+    vol.irradiance[vol.ping_pong].copy_(torch.from_numpy(irr).to(tdt))
+    vol.depth[vol.ping_pong].copy_(torch.from_numpy(dep).to(tdt))
+    vol.offsets.copy_(torch.from_numpy(off))
+    vol.states.copy_(torch.from_numpy(states))
+    vol.enable_relocation(True)
+    if storage == abi.STORAGE_FP16:
+        irr = irr.astype(np.float16).astype(np.float32)
+        dep = dep.astype(np.float16).astype(np.float32)
+    out = vol.sample_probes(torch.from_numpy(g["depth"]).cuda(), torch.from_numpy(g["normals"]).cuda(), cam["eye"],
+                            cam["VPI"])
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().astype(np.float64)
+    p = abi.SampleProbeParams()
+    p.viewProjInv = abi.mat4_colmajor(cam["VPI"])
+    p.cameraPosition[:] = [*cam["eye"], 1.0]
+    ref = oracle.sample_probes(u, p, W, H, g["depth"], g["normals"], irr, dep, offsets=off, states=states)
+    base = oracle.sample_probes(u, p, W, H, g["depth"], g["normals"], irr, dep)
+    sky = g["depth"] == 1.0
+    a, b = got[~sky][:, :3], ref[~sky][:, :3]
+    assert np.abs(b - base[~sky][:, :3]).max() > 1e-2          # the relocation changes the answer
+    zero = np.all(b == 0, axis=1)
+    assert np.all(a[zero] == 0)
+    err = np.abs(a - b)
+    assert (err <= 1e-5 + 1e-3 * np.abs(b)).mean() >= 0.995
+    assert np.all(err <= 1e-4 + 1e-2 * np.abs(b)), float((err / (np.abs(b) + 1e-6)).max())
+    vol.enable_relocation(False)
+    out0 = vol.sample_probes(torch.from_numpy(g["depth"]).cuda(), torch.from_numpy(g["normals"]).cuda(),
+                             cam["eye"], cam["VPI"]).cpu().numpy()
+    e0 = np.abs(out0[~sky][:, :3] - base[~sky][:, :3])
+    assert (e0 <= 1e-5 + 1e-3 * np.abs(base[~sky][:, :3])).mean() >= 0.995

@@ -107,3 +107,85 @@ def test_point_on_probe_reads_that_probe_only():
                 probe = x + y * px + gz * px * py
                 want = 2 * np.pi * (float(np.float32(vals[probe])) ** g) ** 2
                 assert abs(out[y, x, 0] - want) < 2e-4 * want, (x, y, gz, out[y, x, 0], want)
+
+
+# ---------------------------------------------------------------------------------------------
+# relocation + classification read by the consumers (offsets / states; no reference counterpart,
+# the published gather: neighbour probes at grid + offset, inactive neighbours skipped)
+# ---------------------------------------------------------------------------------------------
+def _cube_setup(depth_mean):
+    """2x2x2 probes at the corners of the unit cube, one pixel at its centre with normal +z and
+    normalBias 0; probe p's irradiance tile holds c_p, its depth tile (mean, mean^2)."""
+    u = abi.make_uniform((2, 2, 2), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 64, irr_oct=4, depth_oct=6)
+    u.normalBias = 0.0
+    c = np.array([0.15, 0.25, 0.35, 0.45, 0.55, 0.65, 0.75, 0.85], np.float32)
+    irr = np.zeros((u.irradianceTextureHeight, u.irradianceTextureWidth, 4), np.float32)
+    dep = np.zeros((u.depthTextureHeight, u.depthTextureWidth, 2), np.float32)
+    for p in range(8):
+        col, row = p % 4, p // 4
+        n = u.irradianceProbeSideLength
+        irr[row * (n + 2) + 1: row * (n + 2) + n + 3, col * (n + 2) + 1: col * (n + 2) + n + 3, :3] = c[p]
+        n = u.depthProbeSideLength
+        dep[row * (n + 2) + 1: row * (n + 2) + n + 3, col * (n + 2) + 1: col * (n + 2) + n + 3, 0] = depth_mean
+        dep[row * (n + 2) + 1: row * (n + 2) + n + 3, col * (n + 2) + 1: col * (n + 2) + n + 3, 1] = depth_mean ** 2
+    p = abi.SampleProbeParams()
+    m = np.eye(4, dtype=np.float32)
+    m[:3, 3] = [0.5, 0.5, 0.0]           # pixel centre, NDC (0, 0, 0.5) -> world (0.5, 0.5, 0.5)
+    p.viewProjInv = abi.mat4_colmajor(m)
+    p.cameraPosition[:] = [0.5, 0.5, 3.0, 1.0]
+    depth = np.full((1, 1), 0.5, np.float32)
+    normals = np.zeros((1, 1, 4), np.float32)
+    return u, c, irr, dep, p, depth, normals
+
+
+def _gather(c, weights, u):
+    """(sum w c^(gamma/2) / sum w)^2 * 2 pi."""
+    g = float(np.float32(u.ddgiGamma * 0.5))
+    cw = sum(w * float(c[i]) ** g for i, w in weights.items())
+    return 2 * np.pi * (cw / sum(weights.values())) ** 2
+
+
+def test_inactive_probes_take_no_weight():
+    """Every probe visible (depth mean 100). From the cube centre the direction to a top probe
+    has n.d = 1/sqrt(3), to a bottom probe -1/sqrt(3): weights ((1 +- 1/sqrt 3)/2)^2 + 0.2, times
+    the trilinear 1/8. Inactive probes drop out of both sums; all inactive gives 0."""
+    u, c, irr, dep, p, depth, normals = _cube_setup(100.0)
+    s3 = 1.0 / np.sqrt(3.0)
+    w_top, w_bot = ((1 + s3) / 2) ** 2 + 0.2, ((1 - s3) / 2) ** 2 + 0.2
+    wt = {i: (w_top if i >= 4 else w_bot) / 8 for i in range(8)}
+    out = oracle.sample_probes(u, p, 1, 1, depth, normals, irr, dep)
+    assert abs(out[0, 0, 0] - _gather(c, wt, u)) < 1e-12 * out[0, 0, 0]
+    for inactive in ([0], [3, 4], [4, 5, 6, 7], list(range(7))):
+        st = np.zeros(8, np.uint8)
+        st[inactive] = 1
+        out = oracle.sample_probes(u, p, 1, 1, depth, normals, irr, dep, states=st)
+        want = _gather(c, {i: w for i, w in wt.items() if i not in inactive}, u)
+        assert abs(out[0, 0, 0] - want) < 1e-12 * want, (inactive, out[0, 0, 0], want)
+    out = oracle.sample_probes(u, p, 1, 1, depth, normals, irr, dep, states=np.ones(8, np.uint8))
+    assert np.all(out[0, 0, :3] == 0) and out[0, 0, 3] == 1.0
+
+
+def test_relocated_probe_passes_the_visibility_test():
+    """Depth mean 0.7 everywhere, zero variance: from the cube centre (distance sqrt(3)/2 = 0.866
+    to every grid probe) every probe is occluded, weight 1e-6 crushed to 2.5e-17, so the result is
+    the plain average. Moving probe 7 by (-1/4, -1/4, -1/4) puts it 0.433 from the point: it alone
+    passes the test, weight (((1 + 1/sqrt 3)/2)^2 + 0.2) / 8, and the result is its colour."""
+    u, c, irr, dep, p, depth, normals = _cube_setup(0.7)
+    out = oracle.sample_probes(u, p, 1, 1, depth, normals, irr, dep)
+    want = _gather(c, {i: 1.0 for i in range(8)}, u)
+    assert abs(out[0, 0, 0] - want) < 1e-9 * want
+    off = np.zeros((8, 4), np.float32)
+    off[7, :3] = -0.25
+    out = oracle.sample_probes(u, p, 1, 1, depth, normals, irr, dep, offsets=off)
+    s3 = 1.0 / np.sqrt(3.0)
+    w7 = (((1 + s3) / 2) ** 2 + 0.2) / 8
+    tiny = (1e-6 ** 3 / 0.04) / 8
+    want = _gather(c, {**{i: tiny for i in range(7)}, 7: w7}, u)
+    assert abs(out[0, 0, 0] - want) < 1e-12 * want, (out[0, 0, 0], want)
+    assert abs(out[0, 0, 0] - 2 * np.pi * float(c[7]) ** 5) < 1e-9 * out[0, 0, 0]
+    # the same offset on an inactive probe changes nothing: the visible probe is skipped
+    st = np.zeros(8, np.uint8)
+    st[7] = 1
+    out = oracle.sample_probes(u, p, 1, 1, depth, normals, irr, dep, offsets=off, states=st)
+    want = _gather(c, {i: 1.0 for i in range(7)}, u)
+    assert abs(out[0, 0, 0] - want) < 1e-9 * want

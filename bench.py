@@ -64,6 +64,9 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision form")
     ap.add_argument("--no-surface", action="store_true", help="skip the surface-atlas passes")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched SVGF measurement")
+    ap.add_argument("--step-only", action="store_true",
+                    help="time the step (DDGI update + SVGF) and the counting run only: no side passes, batched "
+                         "forms, reduced-precision form or CPU baseline (for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, launch a plain copy kernel (256 MiB read + 256 MiB written, 3 times) "
                          "whose rocprofv3 FETCH_SIZE / WRITE_SIZE scale the trace counters to bytes")
@@ -285,199 +288,204 @@ def main():
     log(f"[bench] timed {args.steps} steps: trace {t_trace.mean():.3f} ms, update {t_upd.mean():.3f} ms, "
         f"svgf {t_svgf.mean():.3f} ms, wall/step {ms_step:.3f} ms", q)
 
-    # ---------------- probe sampling at 1080p (sample_probe pass; reported beside the step) ----------------
-    pev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    irr_out = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
-    for _ in range(2):
-        vol.sample_probes(cur["depth"], cur["normals"], eye, VPI, out=irr_out)
-    pev[0].record()
-    for _ in range(args.steps):
-        vol.sample_probes(cur["depth"], cur["normals"], eye, VPI, out=irr_out)
-    pev[1].record()
-    torch.cuda.synchronize()
-    sample_ms = pev[0].elapsed_time(pev[1]) / args.steps
-    # G-buffer depth (4 B) + octahedral normal (8 B) + RGBA32F output (16 B) per pixel, atlases read once
-    atlas_bytes = sum(t.numel() * t.element_size() for t in (vol.current_irradiance, vol.current_depth))
-    sample_bytes = W * H * (4 + 8 + 16) + atlas_bytes
-
-    # ---------------- learned a-trous chain at 1080p (reported beside the step) ----------------
-    # 4 passes of the learned edge-stopping filter on the accumulated reflection signal, against the
-    # same 4 passes of the reference a-trous (tile map from the frame's reprojection)
-    from luxgi_amd._lib import lib as _L
-    from luxgi_amd.ddgi import ptr, stream_handle
-    from luxgi_amd.learned import LearnedAtrous
-    from luxgi_amd.svgf import gbuffer_struct
-    la = LearnedAtrous(W, H, iterations=4)
-    la_in = dn.outputs[0].float().contiguous()
-    gbs = gbuffer_struct(cur)
-    ref_bufs = [torch.empty_like(la_in), torch.empty_like(la_in)]
-
-    def ref_chain():
-        src = la_in
-        for i in range(4):
-            ap = abi.ReflectionAtrousParams(1, 1 << i, 10.0, 32.0, 1.0, 1)
-            _L().luxgi_svgf_reflection_atrous(ctypes.byref(ap), W, H, abi.STORAGE_FP32, ctypes.byref(gbs),
-                                              ptr(dn.tile_map), ptr(src), ptr(ref_bufs[i & 1]), None,
-                                              stream_handle())
-            src = ref_bufs[i & 1]
-
-    lev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
-    for _ in range(2):
-        la.forward(cur, la_in)
-        ref_chain()
-    lev[0].record()
-    for _ in range(args.steps):
-        la.forward(cur, la_in)
-    lev[1].record()
-    for _ in range(args.steps):
-        ref_chain()
-    lev[2].record()
-    torch.cuda.synchronize()
-    # one training step (4 passes forward, 4 backward with the parameter-gradient reduction, Adam)
-    from luxgi_amd.learned import filtered_mask
-    la_mask = filtered_mask(cur)
-    la_target = la_in.clone()
-    for _ in range(2):
-        la.train_step(cur, la_in, la_target, la_mask, lr=1e-4)
-    lev[4].record()
-    for _ in range(args.steps):
-        la.train_step(cur, la_in, la_target, la_mask, lr=1e-4)
-    lev[5].record()
-    torch.cuda.synchronize()
-    learned = {"learned_4_passes_ms": lev[0].elapsed_time(lev[1]) / args.steps,
-               "reference_4_passes_ms": lev[1].elapsed_time(lev[2]) / args.steps,
-               "train_step_ms": lev[4].elapsed_time(lev[5]) / args.steps,
-               "mlp": "6-10-1 ReLU, 81 parameters, one per tap", "image": [W, H]}
-    log(f"[bench] learned a-trous 1080p x4 {learned['learned_4_passes_ms']:.3f} ms "
-        f"(reference {learned['reference_4_passes_ms']:.3f} ms), train step {learned['train_step_ms']:.3f} ms", q)
-    del la, la_in, ref_bufs, la_target
-
-    # ---------------- surface atlas passes (reported beside the step) ----------------
-    # chunk culling of the 2,000 objects, then the surface-cache relight of all 12,000 tiles
-    # (16x16 texels) with one directional light (SDF shadow ray per lit texel), the emissive
-    # copy and the DDGI bounce from the updated volume, into an RGBA16F light cache
-    surf = None
-    if not args.no_surface:
-        from luxgi_amd.surface import SurfaceCache, cull_chunks
-        cap = int(atlas.culled.size) + 64
+    # the passes reported beside the step (skipped with --step-only, e.g. for profiler runs)
+    sample_ms = sample_bytes = learned = surf = shadow_ms = refl_ms = sdf_build = None
+    if not args.step_only:
+        # ---------------- probe sampling at 1080p (sample_probe pass; reported beside the step) ----------------
+        pev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        irr_out = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
         for _ in range(2):
-            cull_chunks(atlas.data, ds.objects, atlas.n_objects, cap)
-        cev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        cev[0].record()
+            vol.sample_probes(cur["depth"], cur["normals"], eye, VPI, out=irr_out)
+        pev[0].record()
         for _ in range(args.steps):
-            cull_chunks(atlas.data, ds.objects, atlas.n_objects, cap)
-        cev[1].record()
-        sc = SurfaceCache(atlas, out_storage=abi.STORAGE_FP16)
-        sun = (abi.Light * 1)()
-        sun[0].color[:] = [1.0, 0.95, 0.9, 1.0]
-        dsun = np.array([0.3, -0.8, -0.52]) / np.linalg.norm([0.3, -0.8, -0.52])
-        sun[0].direction[:] = [float(dsun[0]), float(dsun[1]), float(dsun[2]), 0.0]
-        sun[0].intensity, sun[0].type = 2.0, abi.LIGHT_DIRECTIONAL
-        sc.set_lights(sun)
-        passes = abi.SURFACE_EMISSIVE | abi.SURFACE_DIRECT | abi.SURFACE_INDIRECT
-        for _ in range(2):
-            sc.light(ds, passes=passes, camera=eye, shadow_bias=2.0 * 2.0 / res, volume=vol)
-        sev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        sev[0].record()
-        for _ in range(args.steps):
-            sc.light(ds, passes=passes, camera=eye, shadow_bias=2.0 * 2.0 / res, volume=vol)
-        sev[1].record()
+            vol.sample_probes(cur["depth"], cur["normals"], eye, VPI, out=irr_out)
+        pev[1].record()
         torch.cuda.synchronize()
-        light_ms = sev[0].elapsed_time(sev[1]) / args.steps
-        texels = sc.n_draws * 16 * 16
-        lit = int(sum(1 for k in range(sc.n_draws) if sc.draws_host[k].lightMask))
-        surf = {"cull_ms": cev[0].elapsed_time(cev[1]) / args.steps, "surface_light_ms": light_ms,
-                "tiles": sc.n_draws, "tiles_with_light": lit, "texels": texels,
-                "gtexels_per_s": texels / (light_ms * 1e-3) / 1e9,
-                "passes": "emissive + 1 directional light with SDF shadow rays + DDGI bounce", "out": "RGBA16F"}
-        log(f"[bench] surface atlas: cull {surf['cull_ms']:.3f} ms, relight {light_ms:.3f} ms", q)
-        del sc
+        sample_ms = pev[0].elapsed_time(pev[1]) / args.steps
+        # G-buffer depth (4 B) + octahedral normal (8 B) + RGBA32F output (16 B) per pixel, atlases read once
+        atlas_bytes = sum(t.numel() * t.element_size() for t in (vol.current_irradiance, vol.current_depth))
+        sample_bytes = W * H * (4 + 8 + 16) + atlas_bytes
 
-    # ---------------- SDF soft-shadow mask at 1080p (the shadow SVGF's input; reported beside the step) ----
-    # one ray per pixel toward a blue-noise point of a sun disk; synthetic blue-noise tables of the
-    # engine's shapes (a permutation per Sobol dimension, random scrambling-ranking bytes)
-    from luxgi_amd.producers import BlueNoiseTables, sdf_shadow_mask
-    nrng = np.random.default_rng(21)
-    noise = BlueNoiseTables(np.stack([nrng.permutation(256) for _ in range(4)], axis=1).astype(np.uint8),
-                            nrng.integers(0, 256, (128, 128, 4), dtype=np.uint8))
-    sp = abi.SdfShadowParams()
-    sp.light.color[:] = [1.0, 1.0, 1.0, 1.0]
-    sp.light.direction[:] = [0.3, -0.8, -0.52, 0.02]
-    sp.light.intensity, sp.light.type = 1.0, abi.LIGHT_DIRECTIONAL
-    sp.viewProjInv = abi.mat4_colmajor(VPI)
-    sp.shadowBias = 2.0 * 2.0 / res
-    smask = torch.zeros(((H + 3) // 4, (W + 7) // 8), dtype=torch.int32, device="cuda")
-    for f in range(2):
-        sp.numFrames = f
-        sdf_shadow_mask(sp, noise, cur["depth"], cur["normals"], ds, mask=smask)
-    mev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    mev[0].record()
-    for f in range(args.steps):
-        sp.numFrames = f
-        sdf_shadow_mask(sp, noise, cur["depth"], cur["normals"], ds, mask=smask)
-    mev[1].record()
-    torch.cuda.synchronize()
-    shadow_ms = mev[0].elapsed_time(mev[1]) / args.steps
-    log(f"[bench] sdf shadow mask 1080p {shadow_ms:.3f} ms", q)
-    # SDF reflection rays at 1080p (the reflection SVGF's noisy input): one GGX-sampled ray per pixel
-    # (the G-buffer's roughness lies in (0.05, 0.45)), coloured by the surface-atlas lookup or the sky
-    from luxgi_amd.producers import sdf_reflection
-    rp = abi.SdfReflectionParams()
-    rp.trim, rp.roughDDGIIntensity = 1.0, 1.0
-    rp.cameraPosition[:] = [float(eye[0]), float(eye[1]), float(eye[2]), 1.0]
-    rp.viewProjInv = abi.mat4_colmajor(VPI)
-    rp.skyColor[:] = [0.6, 0.7, 0.9, 1.0]
-    refl_out = torch.zeros((H, W, 4), dtype=torch.float16, device="cuda")
-    for f in range(2):
-        rp.numFrames = f
-        sdf_reflection(rp, noise, cur, ds, refl_out)
-    rev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    rev[0].record()
-    for f in range(args.steps):
-        rp.numFrames = f
-        sdf_reflection(rp, noise, cur, ds, refl_out)
-    rev[1].record()
-    torch.cuda.synchronize()
-    refl_ms = rev[0].elapsed_time(rev[1]) / args.steps
-    log(f"[bench] sdf reflection 1080p {refl_ms:.3f} ms", q)
+        # ---------------- learned a-trous chain at 1080p (reported beside the step) ----------------
+        # 4 passes of the learned edge-stopping filter on the accumulated reflection signal, against the
+        # same 4 passes of the reference a-trous (tile map from the frame's reprojection)
+        from luxgi_amd._lib import lib as _L
+        from luxgi_amd.ddgi import ptr, stream_handle
+        from luxgi_amd.learned import LearnedAtrous
+        from luxgi_amd.svgf import gbuffer_struct
+        la = LearnedAtrous(W, H, iterations=4)
+        la_in = dn.outputs[0].float().contiguous()
+        gbs = gbuffer_struct(cur)
+        ref_bufs = [torch.empty_like(la_in), torch.empty_like(la_in)]
 
-    # ---------------- global SDF build from mesh SDFs (reported beside the step) ----------------
-    # the 2,000 primitives as instances of three baked 32^3 unit meshes (+2 mips), rasterized into
-    # all 512 chunks of the 256^3 cascade (luxgi_sdf_rasterize; FP32 volume, as the trace reads it)
-    from luxgi_amd import meshsdf
-    msh, mobj, mbounds, mwhich = meshsdf.primitive_instances(prims)
-    cas0 = meshsdf.Cascade(0, (0.0, 0.0, 0.0), 1.0, res)
-    mlists = meshsdf.chunk_lists(cas0, mbounds)
-    builder = meshsdf.GlobalSDFBuilder(msh, mobj, len(prims), which=mwhich)
-    built = torch.empty((res, res, res), dtype=torch.float32, device="cuda")
-    for _ in range(2):
-        builder.rasterize(cas0, mlists, built)
-    bev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    bev[0].record()
-    for _ in range(args.steps):
-        builder.rasterize(cas0, mlists, built)
-    bev[1].record()
-    torch.cuda.synchronize()
-    build_ms = bev[0].elapsed_time(bev[1]) / args.steps
-    # against the analytic SDF on the voxels within one voxel of a surface: the instances scale unit
-    # meshes non-uniformly (distances scaled by one factor, as the engine does) and a chunk is only
-    # rasterized with the objects whose bounds + 4 voxels reach it (other chunks clear to 1), so
-    # report the distribution in voxels and the cleared fraction rather than a maximum
-    vox = 1.0 / res  # one voxel in the stored (distance / 2) units
-    near = torch.abs(sdf) < vox
-    d_near = (torch.abs(built - sdf)[near] / vox).float()
-    sdf_build = {"rasterize_ms": build_ms, "objects": len(prims), "chunk_object_pairs": int(mlists[1][-1]),
-                 "voxels": res ** 3,
-                 "near_surface_abs_diff_voxels": {"median": float(d_near.median()),
-                                                  "p99": float(torch.quantile(d_near[:1 << 24], 0.99))},
-                 "near_surface_cleared_fraction": float((built[near] >= 1.0).float().mean())}
-    log(f"[bench] global SDF rasterize {build_ms:.3f} ms", q)
-    del builder, built
+        def ref_chain():
+            src = la_in
+            for i in range(4):
+                ap = abi.ReflectionAtrousParams(1, 1 << i, 10.0, 32.0, 1.0, 1)
+                _L().luxgi_svgf_reflection_atrous(ctypes.byref(ap), W, H, abi.STORAGE_FP32, ctypes.byref(gbs),
+                                                  ptr(dn.tile_map), ptr(src), ptr(ref_bufs[i & 1]), None,
+                                                  stream_handle())
+                src = ref_bufs[i & 1]
+
+        lev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        for _ in range(2):
+            la.forward(cur, la_in)
+            ref_chain()
+        lev[0].record()
+        for _ in range(args.steps):
+            la.forward(cur, la_in)
+        lev[1].record()
+        for _ in range(args.steps):
+            ref_chain()
+        lev[2].record()
+        torch.cuda.synchronize()
+        # one training step (4 passes forward, 4 backward with the parameter-gradient reduction, Adam)
+        from luxgi_amd.learned import filtered_mask
+        la_mask = filtered_mask(cur)
+        la_target = la_in.clone()
+        for _ in range(2):
+            la.train_step(cur, la_in, la_target, la_mask, lr=1e-4)
+        lev[4].record()
+        for _ in range(args.steps):
+            la.train_step(cur, la_in, la_target, la_mask, lr=1e-4)
+        lev[5].record()
+        torch.cuda.synchronize()
+        learned = {"learned_4_passes_ms": lev[0].elapsed_time(lev[1]) / args.steps,
+                   "reference_4_passes_ms": lev[1].elapsed_time(lev[2]) / args.steps,
+                   "train_step_ms": lev[4].elapsed_time(lev[5]) / args.steps,
+                   "mlp": "6-10-1 ReLU, 81 parameters, one per tap", "image": [W, H]}
+        log(f"[bench] learned a-trous 1080p x4 {learned['learned_4_passes_ms']:.3f} ms "
+            f"(reference {learned['reference_4_passes_ms']:.3f} ms), train step {learned['train_step_ms']:.3f} ms", q)
+        del la, la_in, ref_bufs, la_target
+
+        # ---------------- surface atlas passes (reported beside the step) ----------------
+        # chunk culling of the 2,000 objects, then the surface-cache relight of all 12,000 tiles
+        # (16x16 texels) with one directional light (SDF shadow ray per lit texel), the emissive
+        # copy and the DDGI bounce from the updated volume, into an RGBA16F light cache
+        surf = None
+        if not args.no_surface:
+            from luxgi_amd.surface import SurfaceCache, cull_chunks
+            cap = int(atlas.culled.size) + 64
+            for _ in range(2):
+                cull_chunks(atlas.data, ds.objects, atlas.n_objects, cap)
+            cev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            cev[0].record()
+            for _ in range(args.steps):
+                cull_chunks(atlas.data, ds.objects, atlas.n_objects, cap)
+            cev[1].record()
+            sc = SurfaceCache(atlas, out_storage=abi.STORAGE_FP16)
+            sun = (abi.Light * 1)()
+            sun[0].color[:] = [1.0, 0.95, 0.9, 1.0]
+            dsun = np.array([0.3, -0.8, -0.52]) / np.linalg.norm([0.3, -0.8, -0.52])
+            sun[0].direction[:] = [float(dsun[0]), float(dsun[1]), float(dsun[2]), 0.0]
+            sun[0].intensity, sun[0].type = 2.0, abi.LIGHT_DIRECTIONAL
+            sc.set_lights(sun)
+            passes = abi.SURFACE_EMISSIVE | abi.SURFACE_DIRECT | abi.SURFACE_INDIRECT
+            for _ in range(2):
+                sc.light(ds, passes=passes, camera=eye, shadow_bias=2.0 * 2.0 / res, volume=vol)
+            sev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            sev[0].record()
+            for _ in range(args.steps):
+                sc.light(ds, passes=passes, camera=eye, shadow_bias=2.0 * 2.0 / res, volume=vol)
+            sev[1].record()
+            torch.cuda.synchronize()
+            light_ms = sev[0].elapsed_time(sev[1]) / args.steps
+            texels = sc.n_draws * 16 * 16
+            lit = int(sum(1 for k in range(sc.n_draws) if sc.draws_host[k].lightMask))
+            surf = {"cull_ms": cev[0].elapsed_time(cev[1]) / args.steps, "surface_light_ms": light_ms,
+                    "tiles": sc.n_draws, "tiles_with_light": lit, "texels": texels,
+                    "gtexels_per_s": texels / (light_ms * 1e-3) / 1e9,
+                    "passes": "emissive + 1 directional light with SDF shadow rays + DDGI bounce", "out": "RGBA16F"}
+            log(f"[bench] surface atlas: cull {surf['cull_ms']:.3f} ms, relight {light_ms:.3f} ms", q)
+            del sc
+
+        # ---------------- SDF soft-shadow mask at 1080p (the shadow SVGF's input; reported beside the step) ----
+        # one ray per pixel toward a blue-noise point of a sun disk; synthetic blue-noise tables of the
+        # engine's shapes (a permutation per Sobol dimension, random scrambling-ranking bytes)
+        from luxgi_amd.producers import BlueNoiseTables, sdf_shadow_mask
+        nrng = np.random.default_rng(21)
+        noise = BlueNoiseTables(np.stack([nrng.permutation(256) for _ in range(4)], axis=1).astype(np.uint8),
+                                nrng.integers(0, 256, (128, 128, 4), dtype=np.uint8))
+        sp = abi.SdfShadowParams()
+        sp.light.color[:] = [1.0, 1.0, 1.0, 1.0]
+        sp.light.direction[:] = [0.3, -0.8, -0.52, 0.02]
+        sp.light.intensity, sp.light.type = 1.0, abi.LIGHT_DIRECTIONAL
+        sp.viewProjInv = abi.mat4_colmajor(VPI)
+        sp.shadowBias = 2.0 * 2.0 / res
+        smask = torch.zeros(((H + 3) // 4, (W + 7) // 8), dtype=torch.int32, device="cuda")
+        for f in range(2):
+            sp.numFrames = f
+            sdf_shadow_mask(sp, noise, cur["depth"], cur["normals"], ds, mask=smask)
+        mev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        mev[0].record()
+        for f in range(args.steps):
+            sp.numFrames = f
+            sdf_shadow_mask(sp, noise, cur["depth"], cur["normals"], ds, mask=smask)
+        mev[1].record()
+        torch.cuda.synchronize()
+        shadow_ms = mev[0].elapsed_time(mev[1]) / args.steps
+        log(f"[bench] sdf shadow mask 1080p {shadow_ms:.3f} ms", q)
+        # SDF reflection rays at 1080p (the reflection SVGF's noisy input): one GGX-sampled ray per pixel
+        # (the G-buffer's roughness lies in (0.05, 0.45)), coloured by the surface-atlas lookup or the sky
+        from luxgi_amd.producers import sdf_reflection
+        rp = abi.SdfReflectionParams()
+        rp.trim, rp.roughDDGIIntensity = 1.0, 1.0
+        rp.cameraPosition[:] = [float(eye[0]), float(eye[1]), float(eye[2]), 1.0]
+        rp.viewProjInv = abi.mat4_colmajor(VPI)
+        rp.skyColor[:] = [0.6, 0.7, 0.9, 1.0]
+        refl_out = torch.zeros((H, W, 4), dtype=torch.float16, device="cuda")
+        for f in range(2):
+            rp.numFrames = f
+            sdf_reflection(rp, noise, cur, ds, refl_out)
+        rev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        rev[0].record()
+        for f in range(args.steps):
+            rp.numFrames = f
+            sdf_reflection(rp, noise, cur, ds, refl_out)
+        rev[1].record()
+        torch.cuda.synchronize()
+        refl_ms = rev[0].elapsed_time(rev[1]) / args.steps
+        log(f"[bench] sdf reflection 1080p {refl_ms:.3f} ms", q)
+
+        # ---------------- global SDF build from mesh SDFs (reported beside the step) ----------------
+        # the 2,000 primitives as instances of three baked 32^3 unit meshes (+2 mips), rasterized into
+        # all 512 chunks of the 256^3 cascade (luxgi_sdf_rasterize; FP32 volume, as the trace reads it)
+        from luxgi_amd import meshsdf
+        msh, mobj, mbounds, mwhich = meshsdf.primitive_instances(prims)
+        cas0 = meshsdf.Cascade(0, (0.0, 0.0, 0.0), 1.0, res)
+        mlists = meshsdf.chunk_lists(cas0, mbounds)
+        builder = meshsdf.GlobalSDFBuilder(msh, mobj, len(prims), which=mwhich)
+        built = torch.empty((res, res, res), dtype=torch.float32, device="cuda")
+        for _ in range(2):
+            builder.rasterize(cas0, mlists, built)
+        bev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        bev[0].record()
+        for _ in range(args.steps):
+            builder.rasterize(cas0, mlists, built)
+        bev[1].record()
+        torch.cuda.synchronize()
+        build_ms = bev[0].elapsed_time(bev[1]) / args.steps
+        # against the analytic SDF on the voxels within one voxel of a surface: the instances scale unit
+        # meshes non-uniformly (distances scaled by one factor, as the engine does) and a chunk is only
+        # rasterized with the objects whose bounds + 4 voxels reach it (other chunks clear to 1), so
+        # report the distribution in voxels and the cleared fraction rather than a maximum
+        vox = 1.0 / res  # one voxel in the stored (distance / 2) units
+        near = torch.abs(sdf) < vox
+        d_near = (torch.abs(built - sdf)[near] / vox).float()
+        sdf_build = {"rasterize_ms": build_ms, "objects": len(prims), "chunk_object_pairs": int(mlists[1][-1]),
+                     "voxels": res ** 3,
+                     "near_surface_abs_diff_voxels": {"median": float(d_near.median()),
+                                                      "p99": float(torch.quantile(d_near[:1 << 24], 0.99))},
+                     "near_surface_cleared_fraction": float((built[near] >= 1.0).float().mean())}
+        log(f"[bench] global SDF rasterize {build_ms:.3f} ms", q)
+        del builder, built
 
     # ---------------- batched SVGF over frames of uneven sizes (reported beside the step) ----------------
     # the same reflection denoise (reprojection + 4 a-trous) for many frames at once: per-frame calls
     # (1 + 4 launches each) against the batched persistent launches (5 in all) and their graph replay;
     # the persistent grid and unit order chosen by the autotuner
+    if args.step_only:
+        args.no_batch = args.no_mixed = args.no_cpu = True
     svgf_batched = None if args.no_batch else bench_svgf_batched(args, prims_dev, len(prims), g, sdf, res, q)
     ddgi_batched = None if args.no_batch else bench_ddgi_batched(args, ds, q)
     # ---------------- reduced-precision form (reported beside the step, not in value) ----------------
@@ -588,7 +596,7 @@ def main():
         "probe_sample_ms": sample_ms,
         "learned_atrous": learned,
         "reduced_precision": mixed,
-        "probe_sample_gbs": sample_bytes / (sample_ms * 1e-3) / 1e9,
+        "probe_sample_gbs": None if sample_ms is None else sample_bytes / (sample_ms * 1e-3) / 1e9,
         "surface_atlas": surf,
         "sdf_shadow_mask_ms": shadow_ms,
         "sdf_reflection_ms": refl_ms,

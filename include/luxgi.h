@@ -563,7 +563,7 @@ typedef struct LuxgiShadowReprojectParams {
     float pad[2];
 } LuxgiShadowReprojectParams;
 
-/* Shadow/DenoiseAtrous.comp push constants (:239-249). */
+/* Shadow/DenoiseAtrous.comp push constants (:29-39). */
 typedef struct LuxgiShadowAtrousParams {
     int32_t radius;
     int32_t stepSize;
@@ -628,13 +628,31 @@ int luxgi_svgf_shadow_reproject(const LuxgiShadowReprojectParams* params,
                                 void* outColor, void* outMoments, uint8_t* tileMap,
                                 void* stream);
 
-/* Shadow/DenoiseAtrous.comp:275-339 fused with Shadow/DenoiseCopyTiles.comp:356-360
+/* Shadow/DenoiseAtrous.comp:65-128 fused with Shadow/DenoiseCopyTiles.comp:18-22
  * (copy tiles write 0). */
 int luxgi_svgf_shadow_atrous(const LuxgiShadowAtrousParams* params,
                              int32_t width, int32_t height, int32_t storage,
                              const LuxgiGBuffer* cur, const uint8_t* tileMap,
                              const void* input, void* output, void* historyOut,
                              void* stream);
+
+/* raytraced_shadow::denoise::system (RaytracedShadow.cpp:543-609) for one frame: the reprojection
+ * (accumulation, :418-462) followed by `iterations` a-trous passes (atrousFilter, :464-541).
+ *   output0/output1   RG outputs[0] (reprojection result) and outputs[1] (history colour), [H][W]
+ *   moments0/1        RGBA currentMoments[2]; the pass writes moments[pingPong] and reads
+ *                     moments[1 - pingPong] (the caller flips pingPong after the call, :565)
+ *   atrous0/1         RG a-trous ping-pong images; iteration i reads atrous[i % 2 == 1 ? 1 : 0]
+ *                     (output0 for i = 0) and writes the other
+ *   ap->power         applied on the last iteration only (:510); stepSize is set per pass
+ * The output of iteration 1 is copied into output1 (:532-535). *result receives the index of
+ * the atrous image holding the final output (-1 with iterations = 0: output0). */
+int luxgi_svgf_shadow_denoise(const LuxgiShadowReprojectParams* reprojectParams,
+                              const LuxgiShadowAtrousParams* atrousParams, int32_t iterations,
+                              int32_t width, int32_t height, int32_t storage,
+                              const LuxgiGBuffer* cur, const LuxgiGBuffer* prev, const uint32_t* mask,
+                              void* output0, void* output1, void* moments0, void* moments1, int32_t pingPong,
+                              void* atrous0, void* atrous1, uint8_t* tileMap, int32_t* result,
+                              void* stream);
 
 /* Library version string. */
 const char* luxgi_version(void);

@@ -44,6 +44,7 @@ SIGNATURES = {
     "luxgi_svgf_reflection_denoise": (C.c_int, [P, P, I, I, I, I, P, P, P, P, P, P, P, I, P, P, P, P, P]),
     "luxgi_svgf_shadow_reproject": (C.c_int, [P, I, I, I, P, P, P, P, P, P, P, P, P]),
     "luxgi_svgf_shadow_atrous": (C.c_int, [P, I, I, I, P, P, P, P, P, P]),
+    "luxgi_svgf_shadow_denoise": (C.c_int, [P, P, I, I, I, I, P, P, P, P, P, P, P, I, P, P, P, P, P]),
     "luxgi_ddgi_trace_batch_prepare": (C.c_int, [P, I, I, P]),
     "luxgi_ddgi_trace_batch_workspace_size": (C.c_size_t, [C.c_int64, I]),
     "luxgi_ddgi_trace_sdf_batched": (C.c_int, [P, I, C.c_int64, I, P, P, P, P, P, P, C.c_size_t, P]),

@@ -3,8 +3,8 @@
 // Reference (paths under Code/Maple/src/):
 //   Shaders/Reflection/DenoiseReprojection.comp:78-218, DenoiseAtrous.comp:41-134,
 //   DenoiseCopyTiles.comp:21-25, DenoiseReset.comp:23-32 (RGB reflection)
-//   Shaders/Shadow/DenoiseReprojection.comp:71-210, DenoiseAtrous.comp:251-339,
-//   DenoiseCopyTiles.comp:356-360 (scalar shadow visibility)
+//   Shaders/Shadow/DenoiseReprojection.comp:71-210, DenoiseAtrous.comp:43-128,
+//   DenoiseCopyTiles.comp:18-22 (scalar shadow visibility)
 //   Shaders/Raytraced/Reprojection.glsl:12-405, EdgeStop.glsl:7-40
 //   host drivers Engine/Raytrace/RaytracedReflection.cpp:346-520, RaytracedShadow.cpp:406-560
 //
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(256) void shadow_reproject_kernel(ShadowArgs A, GBu
     }
 }
 
-// Shadow/DenoiseAtrous.comp:275-339 for one filtered pixel (same tap contract and fixed sum order
+// Shadow/DenoiseAtrous.comp:65-128 for one filtered pixel (same tap contract and fixed sum order
 // as reflection_filter).
 template <class Tap>
 __device__ __forceinline__ float2 shadow_filter(const AtrousArgs& A, int x, int y, float2 c, float var, f3 cn, float cd,
@@ -1103,6 +1103,40 @@ extern "C" int luxgi_svgf_shadow_atrous(const LuxgiShadowAtrousParams* p, int32_
         launch_atrous<__half>(A, shadow_atrous_lds_kernel<__half>, shadow_atrous_kernel<__half>, st, nrm, pbr, tileMap,
                               input, output, historyOut);
     LUXGI_LAUNCH_CHECK();
+    return LUXGI_OK;
+}
+
+extern "C" int luxgi_svgf_shadow_denoise(const LuxgiShadowReprojectParams* rp, const LuxgiShadowAtrousParams* ap,
+                                         int32_t iterations, int32_t W, int32_t H, int32_t storage,
+                                         const LuxgiGBuffer* cur, const LuxgiGBuffer* prev, const uint32_t* mask,
+                                         void* output0, void* output1, void* moments0, void* moments1,
+                                         int32_t pingPong, void* atrous0, void* atrous1, uint8_t* tileMap,
+                                         int32_t* result, void* stream) {
+    if (!ap || iterations < 0 || !atrous0 || !atrous1 || !moments0 || !moments1 || !output0 || !output1)
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    void* mom[2] = {moments0, moments1};
+    void* atr[2] = {atrous0, atrous1};
+    // accumulation (RaytracedShadow.cpp:418-462): outColor = outputs[0], history = outputs[1],
+    // moments written to currentMoments[pingPong], read from currentMoments[1 - pingPong]
+    int rc = luxgi_svgf_shadow_reproject(rp, W, H, storage, cur, prev, mask, output1, mom[1 - (pingPong & 1)], output0,
+                                         mom[pingPong & 1], tileMap, stream);
+    if (rc) return rc;
+    // atrousFilter (RaytracedShadow.cpp:464-541): power only on the last iteration (:510),
+    // the output of iteration 1 copied into the history colour (:532-535)
+    bool pp = false;
+    int writeIdx = 1;
+    for (int i = 0; i < iterations; ++i) {
+        const int readIdx = pp ? 1 : 0;
+        writeIdx = pp ? 0 : 1;
+        LuxgiShadowAtrousParams a = *ap;
+        a.stepSize = 1 << i;
+        a.power = i == iterations - 1 ? ap->power : 0.0f;
+        rc = luxgi_svgf_shadow_atrous(&a, W, H, storage, cur, tileMap, i == 0 ? output0 : atr[readIdx], atr[writeIdx],
+                                      i == 1 ? output1 : nullptr, stream);
+        if (rc) return rc;
+        pp = !pp;
+    }
+    if (result) *result = iterations > 0 ? writeIdx : -1;
     return LUXGI_OK;
 }
 

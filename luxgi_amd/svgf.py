@@ -76,36 +76,23 @@ class ShadowDenoiser:
         self.rp = abi.ShadowReprojectParams()
         self.rp.alpha = alpha
         self.rp.momentsAlpha = moments_alpha
-        self.ap = abi.ShadowAtrousParams(radius, 0, phi_visibility, phi_normal, sigma_depth, power, near, far)
+        self.ap = abi.ShadowAtrousParams(radius, 1, phi_visibility, phi_normal, sigma_depth, power, near, far)
+        self.result = None
 
     def set_camera(self, view_proj_inv):
         self.rp.viewProjInv = abi.mat4_colmajor(view_proj_inv)
 
     def denoise(self, cur, prev, mask):
-        """denoise::system (RaytracedShadow.cpp:543-609) for one frame."""
-        st = stream_handle()
-        pp = self.ping_pong
-        check(lib().luxgi_svgf_shadow_reproject(C.byref(self.rp), self.W, self.H, self.storage,
-                                                C.byref(gbuffer_struct(cur)), C.byref(gbuffer_struct(prev)), ptr(mask),
-                                                ptr(self.outputs[1]), ptr(self.moments[1 - pp]), ptr(self.outputs[0]),
-                                                ptr(self.moments[pp]), ptr(self.tile_map), st),
-              "luxgi_svgf_shadow_reproject")
-        ping = False
-        write = 1
-        for i in range(self.iterations):
-            read = 1 if ping else 0
-            write = 0 if ping else 1
-            a = abi.ShadowAtrousParams(self.ap.radius, 1 << i, self.ap.phiVisibility, self.ap.phiNormal,
-                                       self.ap.sigmaDepth, self.power if i == self.iterations - 1 else 0.0,
-                                       self.ap.near_, self.ap.far_)
-            src = self.outputs[0] if i == 0 else self.atrous[read]
-            hist = self.outputs[1] if i == 1 else None
-            check(lib().luxgi_svgf_shadow_atrous(C.byref(a), self.W, self.H, self.storage, C.byref(gbuffer_struct(cur)),
-                                                 ptr(self.tile_map), ptr(src), ptr(self.atrous[write]), ptr(hist), st),
-                  "luxgi_svgf_shadow_atrous")
-            ping = not ping
+        """denoise::system (RaytracedShadow.cpp:543-609) for one frame; returns the output tensor."""
+        res = C.c_int32(-1)
+        check(lib().luxgi_svgf_shadow_denoise(
+            C.byref(self.rp), C.byref(self.ap), self.iterations, self.W, self.H, self.storage,
+            C.byref(gbuffer_struct(cur)), C.byref(gbuffer_struct(prev)), ptr(mask), ptr(self.outputs[0]),
+            ptr(self.outputs[1]), ptr(self.moments[0]), ptr(self.moments[1]), self.ping_pong, ptr(self.atrous[0]),
+            ptr(self.atrous[1]), ptr(self.tile_map), C.byref(res), stream_handle()), "luxgi_svgf_shadow_denoise")
         self.ping_pong = 1 - self.ping_pong
-        return self.atrous[write] if self.iterations > 0 else self.outputs[0]
+        self.result = self.outputs[0] if res.value < 0 else self.atrous[res.value]
+        return self.result
 
 
 def make_gbuffer(prims_dev, n_prims, sdf_data, sdf, res, view_proj_inv, prev_view_proj, width, height, seed,

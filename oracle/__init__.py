@@ -498,3 +498,48 @@ def learned_atrous_backward(params, weights, normals, pbr, image, grad_out):
     lib().oracle_learned_atrous_backward(C.byref(params), _p(wts), W, H, _p(_f32(normals)), _p(_f32(pbr)),
                                          _p(_f32(image)), _p(g), _p(gin), _p(gp))
     return gin, gp
+
+
+# ---------------------------------------------------------------------------
+# Error statistics of a GPU result against the FP64 oracle
+# ---------------------------------------------------------------------------
+def relative_error_report(got, ref, floor=1e-2, exclude=None):
+    """Error of `got` against the FP64 reference `ref` (same shape, last axis = channels).
+
+    normwise_rel  ||got - ref||_2 / ||ref||_2 over every element: the relative error that the
+                  1e-3 irradiance bound is stated in (it weighs a texel by its energy, as a
+                  converged probe's lighting does);
+    max_rel       max |got - ref| / max(|ref|, floor) over elements (the per-texel worst case);
+    p99_rel, p999_rel  the 99th and 99.9th percentiles of the same;
+    mean_abs      mean |got - ref|;
+    max_rel_excl  max_rel over the elements not in `exclude` (a boolean mask over the leading
+                  axes, e.g. texels of probes whose rays changed class against FP64)."""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    d = np.abs(got - ref)
+    rel = d / np.maximum(np.abs(ref), floor)
+    nref = float(np.linalg.norm(ref))
+    out = {"normwise_rel": float(np.linalg.norm(got - ref) / nref) if nref > 0 else float(np.linalg.norm(got - ref)),
+           "max_rel": float(rel.max()) if rel.size else 0.0,
+           "p99_rel": float(np.quantile(rel, 0.99)) if rel.size else 0.0,
+           "p999_rel": float(np.quantile(rel, 0.999)) if rel.size else 0.0,
+           "mean_abs": float(d.mean()) if d.size else 0.0,
+           "elements": int(d.size)}
+    if exclude is not None:
+        keep = ~np.asarray(exclude, dtype=bool)
+        out["max_rel_excl"] = float(rel[keep].max()) if keep.any() else 0.0
+        out["excluded"] = int((~keep).sum())
+    return out
+
+
+def probe_texel_mask(u, probes, side, height, width):
+    """Boolean [height][width] mask of the bordered tiles of `probes` (indices) in an atlas of
+    octahedral side `side` (DDGIRenderer.cpp:182-185 layout: probe p at column p % (px*py),
+    row p / (px*py), tile (side + 2)^2 starting one texel in from the atlas edge)."""
+    m = np.zeros((height, width), dtype=bool)
+    pxy = u.probeCounts[0] * u.probeCounts[1]
+    t = side + 2
+    for p in np.asarray(probes).ravel():
+        cx, cy = int(p) % pxy, int(p) // pxy
+        m[1 + cy * t:1 + (cy + 1) * t, 1 + cx * t:1 + (cx + 1) * t] = True
+    return m

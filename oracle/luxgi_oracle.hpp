@@ -959,7 +959,7 @@ bool shadow_reproject_pixel(const LuxgiShadowReprojectParams& pc, int W, int H, 
     return depth != T(1) && ov[0] > T(0);
 }
 
-// Shadow/DenoiseAtrous.comp:275-339 for one pixel of a denoise tile.
+// Shadow/DenoiseAtrous.comp:65-128 for one pixel of a denoise tile.
 template <class T>
 void shadow_atrous_pixel(const LuxgiShadowAtrousParams& pc, int W, int H, int x, int y, Img normals, Img pbr, Img in,
                          T* out) {

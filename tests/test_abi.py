@@ -72,3 +72,50 @@ def test_brick_layout_sizes_and_rejections():
     assert L.luxgi_atlas_pack(None, None, 2048, None, None) < 0
     dummy = C.c_void_p(16)
     assert L.luxgi_atlas_pack(dummy, dummy, 2046, dummy, None) < 0
+
+
+def _declarations():
+    """name -> (return type, [parameter types]) of every function the headers declare."""
+    out = {}
+    for h in ("luxgi.h", "luxgi_scene.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"#[^\n]*", "", text)
+        for m in re.finditer(r"([A-Za-z_][\w\s\*]*?)\b(luxgi_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", text):
+            ret, name, params = m.group(1).strip(), m.group(2), m.group(3).strip()
+            plist = [] if params in ("", "void") else [p.strip() for p in params.split(",")]
+            types = []
+            for p in plist:
+                p = re.sub(r"\[[^\]]*\]", "*", p)  # float rot[16] -> pointer
+                p = re.sub(r"\b[A-Za-z_]\w*$", "", p).strip() if not p.endswith("*") else p
+                types.append(p)
+            out[name] = (ret.split()[-1] if ret else ret, types)
+    return out
+
+
+def _ctype_of(c_decl):
+    """The ctypes type a C parameter/return type must bind to."""
+    if "*" in c_decl:
+        return "ptr"
+    base = c_decl.replace("const", "").strip()
+    return {"int": C.c_int, "int32_t": C.c_int32, "uint32_t": C.c_uint32, "size_t": C.c_size_t,
+            "int64_t": C.c_int64, "float": C.c_float, "uint64_t": C.c_uint64}[base]
+
+
+def test_ctypes_argument_lists_match_the_header():
+    decls = _declarations()
+    assert set(decls) == set(_lib.SIGNATURES), set(decls) ^ set(_lib.SIGNATURES)
+    for name, (ret, params) in decls.items():
+        res, args = _lib.SIGNATURES[name]
+        assert len(args) == len(params), (name, len(args), len(params), params)
+        for i, (c_t, py_t) in enumerate(zip(params, args)):
+            want = _ctype_of(c_t)
+            if want == "ptr":
+                assert py_t in (C.c_void_p, C.c_char_p), (name, i, c_t, py_t)
+            else:
+                assert C.sizeof(py_t) == C.sizeof(want) and (py_t in (C.c_float, C.c_double)) == \
+                    (want in (C.c_float, C.c_double)), (name, i, c_t, py_t)
+        if ret.endswith("*"):
+            assert res in (C.c_char_p, C.c_void_p), name
+        else:
+            assert C.sizeof(res) == C.sizeof(_ctype_of(ret)), (name, ret, res)

@@ -162,6 +162,69 @@ def test_shadow_reproject_and_atrous(W, H, seed):
         x = got
 
 
+def test_shadow_denoise_frames():
+    """Three frames of the shadow denoise::system (RaytracedShadow.cpp:543-609) through the whole-frame
+    entry point: moments ping-pong, the history colour copied after a-trous iteration 1 (:532-535) and
+    power applied on the last iteration only (:510), against the oracle chain."""
+    from luxgi_amd.svgf import ShadowDenoiser
+    W, H = 88, 52
+    g, prev, noisy, hist, hmom, mask, cam = _svgf_inputs.make_pair(W, H, 6)
+    dn = ShadowDenoiser(W, H, iterations=4, far=1000.0)
+    dn.set_camera(cam["VPI"])
+    dg, dp = _dev(g), _dev(prev)
+    o_out = [np.zeros((H, W, 2), np.float32), np.zeros((H, W, 2), np.float32)]
+    o_mom = [np.zeros((H, W, 4), np.float32), np.zeros((H, W, 4), np.float32)]
+    pp = 0
+    rng = np.random.default_rng(11)
+    for frame in range(3):
+        m = (mask ^ rng.integers(0, 2 ** 32, mask.shape, dtype=np.uint64).astype(np.uint32) & 0x01010101).astype(
+            np.uint32)
+        got = dn.denoise(dg, dp, torch.from_numpy(m.view(np.int32)).cuda()).cpu().numpy()
+        c, mo, t = oracle.shadow_reproject(dn.rp, W, H, g, prev, m, o_out[1], o_mom[1 - pp])
+        o_out[0] = c.astype(np.float32)
+        o_mom[pp] = mo.astype(np.float32)
+        x = o_out[0]
+        for i in range(4):
+            ap = abi.ShadowAtrousParams(1, 1 << i, 10.0, 32.0, 1.0, 1.2 if i == 3 else 0.0, 0.1, 1000.0)
+            x = oracle.shadow_atrous(ap, W, H, g["normals"], g["pbr"], t, x).astype(np.float32)
+            if i == 1:
+                o_out[1] = x.copy()
+        assert np.array_equal(dn.tile_map.cpu().numpy(), t)
+        f = _frac_close(got, x, 2e-3, 2e-3)
+        assert f >= 0.995, (frame, f)
+        # the history copy holds iteration 1's output, before the last pass's power
+        f_h = _frac_close(dn.outputs[1].cpu().numpy(), o_out[1], 2e-3, 2e-3)
+        assert f_h >= 0.995, (frame, f_h)
+        pp = 1 - pp
+        assert dn.ping_pong == pp
+        # feed the GPU state back into the oracle so that rare threshold flips do not accumulate
+        o_out[1] = dn.outputs[1].cpu().numpy()
+        o_mom = [dn.moments[0].cpu().numpy(), dn.moments[1].cpu().numpy()]
+
+
+def test_shadow_denoise_power_only_on_last_pass():
+    """With power = 1 the whole-frame output equals the chain without power; with power 1.2 it is that
+    output to the power 1.2 on filtered pixels (the last pass applies pow to the filtered value)."""
+    from luxgi_amd.svgf import ShadowDenoiser
+    W, H = 64, 40
+    g, prev, noisy, hist, hmom, mask, cam = _svgf_inputs.make_pair(W, H, 8)
+    outs = []
+    for power in (1.0, 1.2):
+        dn = ShadowDenoiser(W, H, iterations=3, power=power, far=1000.0)
+        dn.set_camera(cam["VPI"])
+        outs.append(dn.denoise(_dev(g), _dev(prev), torch.from_numpy(mask.view(np.int32)).cuda()).cpu().numpy())
+        tiles = dn.tile_map.cpu().numpy()
+    filt = np.zeros((H, W), bool)
+    for ty in range((H + 7) // 8):
+        for tx in range((W + 7) // 8):
+            if tiles[ty * ((W + 7) // 8) + tx]:
+                filt[ty * 8:ty * 8 + 8, tx * 8:tx * 8 + 8] = True
+    sel = filt & (outs[0][..., 0] > 0)
+    assert sel.sum() > 100
+    np.testing.assert_allclose(outs[1][sel, 0], outs[0][sel, 0] ** 1.2, rtol=2e-5, atol=1e-6)
+    np.testing.assert_array_equal(outs[1][..., 1], outs[0][..., 1])  # variance untouched by power
+
+
 def test_reflection_fp16_storage_close_to_fp32():
     from luxgi_amd.svgf import ReflectionDenoiser
     W, H = 64, 40

@@ -144,7 +144,8 @@ typedef struct LuxgiSDFVolume {
     int32_t     resolution;
     int32_t     mipResolution;
     int32_t     storage;  /* LuxgiStorage: FP32 or FP16 */
-    int32_t     layout;   /* LuxgiLayout of `sdf` (the mip is always linear) */
+    int32_t     layout;   /* LuxgiLayout of `sdf`; the mip is LINEAR, except with LUXGI_LAYOUT_CELL8, where
+                             the mip is in CELL8 records as well */
     const float* guard;   /* optional [D/4][H/4][W/4] lower bounds of `sdf` per 4x4x4 brick and its
                              +1 apron (luxgi_sdf_build_guard); lets the march skip fine lookups whose
                              value the reference would not use. NULL = no skipping. */
@@ -161,13 +162,23 @@ typedef struct LuxgiSDFVolume {
 typedef enum LuxgiLayout {
     LUXGI_LAYOUT_LINEAR = 0, /* SDF: [z][y][x], the reference texture's texel order.
                                 Atlas: separate colour [res][res] and depth [res][res] images. */
-    LUXGI_LAYOUT_BRICK4 = 1  /* SDF: 4x4x4 bricks (x-fastest brick order), each stored with a +1 apron
+    LUXGI_LAYOUT_BRICK4 = 1, /* SDF: 4x4x4 bricks (x-fastest brick order), each stored with a +1 apron
                                 as 5x5x5 voxels (element k*25 + j*5 + i = voxel (4bx+i, 4by+j, 4bz+k),
                                 clamped to the volume) padded to 128 elements; FP32 or FP16, every
                                 dimension a multiple of 4 (luxgi_sdf_to_bricks, luxgi_sdf_bricks_bytes).
                                 Atlas: one 8-byte texel (RGBA8 colour, FP32 depth) per texel in 4x4-texel
                                 blocks of 128 bytes, blocks row-major; UNORM8 colour, res a multiple of 4
                                 (luxgi_atlas_pack); `color` points at it and `depth` is unused. */
+    LUXGI_LAYOUT_CELL8 = 2   /* SDF only, for the fine volume AND the coarse mip of an LuxgiSDFVolume:
+                                one record per voxel cell (x, y, z) holding the 8 corner voxels of the
+                                trilinear footprint whose lower corner it is, corner (i, j, k) at element
+                                i + 2j + 4k = voxel (x+i, y+j, z+k) clamped to the volume (32 B FP32 or
+                                16 B FP16 per record), so a lookup is one aligned record. Records are
+                                grouped in bricks of 8x8x8 cells (x-fastest brick order, 512 records per
+                                brick) and Morton-ordered inside a brick (bit 3b+a of the index = bit b
+                                of the cell's coordinate a, a = x, y, z). Any dimensions
+                                (luxgi_sdf_to_cells, luxgi_sdf_cells_bytes). Lookups return what the
+                                LINEAR layout returns, bit for bit. */
 } LuxgiLayout;
 
 /* Surface atlas + object/tile/chunk buffers read by the hit lookup
@@ -423,6 +434,16 @@ size_t luxgi_sdf_bricks_bytes(int32_t width, int32_t height, int32_t depth, int3
  * exact. No counterpart in the reference, whose Vulkan 3D image is tiled by the driver. */
 int luxgi_sdf_to_bricks(const float* src, int32_t width, int32_t height, int32_t depth, int32_t storage, void* dst,
                         float* brickScale, void* stream);
+
+/* Bytes of a width x height x depth volume in LUXGI_LAYOUT_CELL8 (storage FP32 or FP16; 0 on bad
+ * arguments or when the records would not be addressable with 32-bit record indices). */
+size_t luxgi_sdf_cells_bytes(int32_t width, int32_t height, int32_t depth, int32_t storage);
+
+/* Converts a LINEAR FP32 volume ([depth][height][width]) into LUXGI_LAYOUT_CELL8 records, rounded
+ * to FP16 (nearest) when storage is FP16. `dst` holds luxgi_sdf_cells_bytes() bytes; the padding
+ * records of partial bricks are written as zeros. Used for both the fine volume and the mip. */
+int luxgi_sdf_to_cells(const float* src, int32_t width, int32_t height, int32_t depth, int32_t storage, void* dst,
+                       void* stream);
 
 /* Packs a UNORM8 colour atlas (RGBA8 [res][res]) and its FP32 depth ([res][res]) into the
  * LUXGI_LAYOUT_BRICK4 atlas layout (res a multiple of 4; `packed` holds res*res*8 bytes). No

@@ -31,6 +31,8 @@ SIGNATURES = {
     "luxgi_sdf_bricks_bytes": (C.c_size_t, [I, I, I, I]),
     "luxgi_sdf_to_bricks": (C.c_int, [P, I, I, I, I, P, P, P]),
     "luxgi_sdf_build_guard": (C.c_int, [P, I, I, I, P, P]),
+    "luxgi_sdf_cells_bytes": (C.c_size_t, [I, I, I, I]),
+    "luxgi_sdf_to_cells": (C.c_int, [P, I, I, I, I, P, P]),
     "luxgi_atlas_pack": (C.c_int, [P, P, I, P, P]),
     "luxgi_atlas_cull_workspace_size": (C.c_size_t, []),
     "luxgi_atlas_cull_chunks": (C.c_int, [P, P, P, P, P, C.c_size_t, P]),

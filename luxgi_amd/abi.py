@@ -83,6 +83,7 @@ class TraceParams(C.Structure):
 
 LAYOUT_LINEAR = 0
 LAYOUT_BRICK4 = 1
+LAYOUT_CELL8 = 2
 
 
 class SDFVolume(C.Structure):

@@ -285,6 +285,73 @@ __device__ __forceinline__ float sample3d_brick4(const S* __restrict__ vol, int 
     return scale ? r * scale[b] : r;  // a power of two: exact
 }
 
+// Cell-record volume (LUXGI_LAYOUT_CELL8): one record of the 8 corner voxels per cell, 8x8x8-cell
+// bricks in x-fastest order, Morton order inside a brick.
+__host__ __device__ __forceinline__ uint32_t cell8_spread(uint32_t v) {  // bits 0,1,2 -> 0,3,6
+    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4);
+}
+__host__ __device__ __forceinline__ uint32_t cell8_compact(uint32_t m) {  // bits 0,3,6 -> 0,1,2
+    return (m & 1u) | ((m >> 2) & 2u) | ((m >> 4) & 4u);
+}
+inline size_t cell8_records(int W, int H, int D) {
+    return (size_t)((W + 7) / 8) * ((H + 7) / 8) * ((D + 7) / 8) * 512;
+}
+__device__ __forceinline__ uint32_t cell8_index(int x, int y, int z, uint32_t nbx, uint32_t nby) {
+    const uint32_t b = ((uint32_t)(z >> 3) * nby + (uint32_t)(y >> 3)) * nbx + (uint32_t)(x >> 3);
+    return (b << 9) | cell8_spread((uint32_t)x & 7u) | (cell8_spread((uint32_t)y & 7u) << 1) |
+           (cell8_spread((uint32_t)z & 7u) << 2);
+}
+
+template <class S> struct CellRecord;
+template <> struct CellRecord<float> {  // 32 B: two 16-byte loads of one 32-byte-aligned record
+    static __device__ __forceinline__ void load(const float* __restrict__ vol, uint32_t rec, float (&v)[8]) {
+        const float4* p = reinterpret_cast<const float4*>(vol) + (size_t)rec * 2;
+        const float4 a = p[0], b = p[1];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    }
+};
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+template <> struct CellRecord<__half> {  // 16 B: one load
+    static __device__ __forceinline__ void load(const __half* __restrict__ vol, uint32_t rec, float (&v)[8]) {
+        const half8_t h = reinterpret_cast<const half8_t*>(vol)[rec];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = float(h[k]);
+    }
+};
+
+// texture(sampler3D) linear + clamp-to-edge from cell records: the same corners and the same
+// arithmetic as sample3d, so the same result bit for bit. A lower corner at -1 (the first half
+// voxel) reads cell 0 and takes its lower corner for both; the upper clamp is in the record.
+template <class S>
+__device__ __forceinline__ float sample3d_cell8(const S* __restrict__ vol, int W, int H, int D, float u, float v,
+                                                float w) {
+    float x = u * float(W) - 0.5f, y = v * float(H) - 0.5f, z = w * float(D) - 0.5f;
+    float fx0 = floorf(x), fy0 = floorf(y), fz0 = floorf(z);
+    float ax = x - fx0, ay = y - fy0, az = z - fz0;
+    const int x0 = int(fx0), y0 = int(fy0), z0 = int(fz0);
+    const bool lx = x0 < 0, ly = y0 < 0, lz = z0 < 0;
+    const uint32_t rec = cell8_index(clampi(x0, 0, W - 1), clampi(y0, 0, H - 1), clampi(z0, 0, D - 1),
+                                     (uint32_t)(W + 7) >> 3, (uint32_t)(H + 7) >> 3);
+    float r[8];
+    CellRecord<S>::load(vol, rec, r);
+    const float v000 = r[0];
+    const float v100 = lx ? r[0] : r[1];
+    const float v010 = ly ? r[0] : r[2];
+    const float v110 = ly ? v100 : (lx ? r[2] : r[3]);
+    const float v001 = lz ? r[0] : r[4];
+    const float v101 = lz ? v100 : (lx ? r[4] : r[5]);
+    const float v011 = lz ? v010 : (ly ? r[4] : r[6]);
+    const float v111 = lz ? v110 : (ly ? (lx ? r[4] : r[5]) : (lx ? r[6] : r[7]));
+    float c00 = v000 * (1.0f - ax) + v100 * ax;
+    float c10 = v010 * (1.0f - ax) + v110 * ax;
+    float c01 = v001 * (1.0f - ax) + v101 * ax;
+    float c11 = v011 * (1.0f - ax) + v111 * ax;
+    float c0 = c00 * (1.0f - ay) + c10 * ay;
+    float c1 = c01 * (1.0f - ay) + c11 * ay;
+    return c0 * (1.0f - az) + c1 * az;
+}
+
 // volume lookup used by the march: pair loads for FP32 volumes of width >= 2
 template <class S>
 __device__ __forceinline__ float sample_volume(const S* __restrict__ vol, int W, int H, int D, float u, float v,

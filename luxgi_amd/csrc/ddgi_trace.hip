@@ -14,6 +14,7 @@
 // The march (march_kernel) and the per-hit work (shade_kernel) are separate
 // launches, so the march loop carries only the registers it needs.
 #include "atlas_sample.hpp"
+#include "sdf_trace.hpp"
 
 #include <algorithm>
 #include <type_traits>
@@ -113,18 +114,28 @@ __device__ __forceinline__ int trace_volume_of(const LuxgiTraceVolume* __restric
     return lo;
 }
 
-// fine-volume lookup in the volume's layout
-template <class SS, bool BRICK>
+// fine-volume lookup in the volume's layout (FL: LuxgiLayout)
+template <class SS, int FL>
 __device__ __forceinline__ float fine_lookup(const SS* __restrict__ sdf, const float* __restrict__ scale, int W, int H,
                                              int D, float u, float v, float w) {
-    if constexpr (BRICK) return sample3d_brick4<SS>(sdf, W, H, D, u, v, w, scale);
+    if constexpr (FL == LUXGI_LAYOUT_BRICK4) return sample3d_brick4<SS>(sdf, W, H, D, u, v, w, scale);
+    else if constexpr (FL == LUXGI_LAYOUT_CELL8) return sample3d_cell8<SS>(sdf, W, H, D, u, v, w);
     else return sample_volume<SS>(sdf, W, H, D, u, v, w);
+}
+
+// coarse-mip lookup: CELL8 records with a CELL8 volume, linear otherwise
+template <class SS, int FL>
+__device__ __forceinline__ float mip_lookup(const SS* __restrict__ mip, int W, int H, int D, float u, float v,
+                                            float w) {
+    if constexpr (FL == LUXGI_LAYOUT_CELL8) return sample3d_cell8<SS>(mip, W, H, D, u, v, w);
+    else return sample_volume<SS>(mip, W, H, D, u, v, w);
 }
 
 // the FP32 re-evaluation volume, in its own layout
 __device__ __forceinline__ float refine_lookup(const TraceArgs& A, int W, int H, int D, float u, float v, float w) {
-    return A.refineLayout == LUXGI_LAYOUT_BRICK4 ? sample3d_brick4<float>(A.refine, W, H, D, u, v, w)
-                                                 : sample_volume<float>(A.refine, W, H, D, u, v, w);
+    if (A.refineLayout == LUXGI_LAYOUT_BRICK4) return sample3d_brick4<float>(A.refine, W, H, D, u, v, w);
+    if (A.refineLayout == LUXGI_LAYOUT_CELL8) return sample3d_cell8<float>(A.refine, W, H, D, u, v, w);
+    return sample_volume<float>(A.refine, W, H, D, u, v, w);
 }
 
 // Result of a ray without an atlas lookup (GISDFRays.comp:93-96,121-124): an inside hit stores
@@ -196,7 +207,7 @@ __device__ __forceinline__ void march_begin(const TraceArgs& A, const LuxgiTrace
 
 // One step of cascade r.cas (the body of SDFCommon.glsl:143-170). Returns true on a hit and fills
 // the hit record; otherwise advances stepTime.
-template <class SS, bool BRICK, bool COUNT>
+template <class SS, int FL, bool COUNT>
 __device__ __forceinline__ bool march_step(const TraceArgs& A, const SS* __restrict__ sdf, const SS* __restrict__ mip,
                                            const float* __restrict__ guard, MarchRay& r, float4& ra, float2& rb,
                                            unsigned long long& cMip, unsigned long long& cFine) {
@@ -224,10 +235,10 @@ __device__ __forceinline__ bool march_step(const TraceArgs& A, const SS* __restr
         const float gv = guard[((size_t)(gz >> 2) * (A.res >> 2) + (gy >> 2)) * (texW >> 2) + (gx >> 2)];
         needFine = !(gv >= chunkMarginDistance * 2.0f * (1.0f + 1e-5f));
     }
-    const float sm = sample_volume<SS>(mip, mipW, A.mipRes, A.mipRes, tx, uy, uz);
+    const float sm = mip_lookup<SS, FL>(mip, mipW, A.mipRes, A.mipRes, tx, uy, uz);
     float sd = sm;
     float st = 1.0f;
-    if (needFine) st = fine_lookup<SS, BRICK>(sdf, A.brickScale, texW, A.res, A.res, tx, uy, uz);
+    if (needFine) st = fine_lookup<SS, FL>(sdf, A.brickScale, texW, A.res, A.res, tx, uy, uz);
     if (COUNT) cMip++;
     if (sd < chunkSizeDistance) {
         if (COUNT) cFine++;
@@ -287,7 +298,7 @@ __device__ __forceinline__ float trace_max_distance(const TraceArgs& A) {
 
 // The sphere trace of one ray (SDFCommon.glsl:98-171) through its cascades; returns its hit chunk
 // (kNoChunk when it needs no atlas lookup).
-template <class SS, class SR, bool COUNT, bool BRICK>
+template <class SS, class SR, bool COUNT, int FL>
 __device__ __forceinline__ uint32_t march_ray(const TraceArgs& A, const LuxgiTraceVolume& v, int probe, int ray,
                                               size_t rid, const SS* __restrict__ sdf, const SS* __restrict__ mip,
                                               const float* __restrict__ guard, float4* __restrict__ recA,
@@ -304,7 +315,7 @@ __device__ __forceinline__ uint32_t march_ray(const TraceArgs& A, const LuxgiTra
         int step = 0;
         bool hit = false;
         for (; step < A.maxSteps && r.stepTime < r.iy; ++step)
-            if (march_step<SS, BRICK, COUNT>(A, sdf, mip, guard, r, ra, rb, cMip, cFine)) {
+            if (march_step<SS, FL, COUNT>(A, sdf, mip, guard, r, ra, rb, cMip, cFine)) {
                 hit = true;
                 break;
             }
@@ -319,7 +330,7 @@ __device__ __forceinline__ uint32_t march_ray(const TraceArgs& A, const LuxgiTra
 // final result of the rays without an atlas lookup and the (chunk, slot) of the others. Single
 // volume: 64 rays of one probe per wave (rays padded to a multiple of 64 per probe). BATCH: the
 // flat ray list of all volumes, each ray finding its volume by rayStart.
-template <class SS, class SR, bool COUNT, bool BRICK, bool BATCH>
+template <class SS, class SR, bool COUNT, int FL, bool BATCH>
 __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __restrict__ sdf, const SS* __restrict__ mip,
                                                     const float* __restrict__ guard, float4* __restrict__ recA,
                                                     float2* __restrict__ recB, uint2* __restrict__ hitKeys,
@@ -338,7 +349,7 @@ __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __res
             const LuxgiTraceVolume& v = A.vols[trace_volume_of(A.vols, A.nVols, gid)];
             const int R = v.uniform.raysPerProbe;
             const long long local = gid - v.rayStart;
-            key = march_ray<SS, SR, COUNT, BRICK>(A, v, int(local / R), int(local % R), rid, sdf, mip, guard, recA,
+            key = march_ray<SS, SR, COUNT, FL>(A, v, int(local / R), int(local % R), rid, sdf, mip, guard, recA,
                                                   recB, skyFaces, cMip, cFine, cSteps);
         }
     } else {
@@ -347,7 +358,7 @@ __global__ __launch_bounds__(256) void march_kernel(TraceArgs A, const SS* __res
         valid = probe < A.P && ray < A.R;
         rid = (size_t)probe * A.R + ray;
         if (valid)
-            key = march_ray<SS, SR, COUNT, BRICK>(A, A.vol, probe, ray, rid, sdf, mip, guard, recA, recB, skyFaces,
+            key = march_ray<SS, SR, COUNT, FL>(A, A.vol, probe, ray, rid, sdf, mip, guard, recA, recB, skyFaces,
                                                   cMip, cFine, cSteps);
     }
     if (A.sortHits) {  // chunk histogram; each hit keeps its slot within its chunk
@@ -403,8 +414,10 @@ __device__ __forceinline__ void shade_ray(const TraceArgs& A, const LuxgiTraceVo
             const float tx = ra.y, uy = ra.z, uz = ra.w;
             auto nl = [&](float u_, float v_, float w_) {
                 if (A.refine) return refine_lookup(A, texW, A.res, A.res, u_, v_, w_);
-                return A.fineLayout ? sample3d_brick4<SS>(sdf, texW, A.res, A.res, u_, v_, w_, A.brickScale)
-                                    : sample_volume<SS>(sdf, texW, A.res, A.res, u_, v_, w_);
+                if (A.fineLayout == LUXGI_LAYOUT_BRICK4)
+                    return sample3d_brick4<SS>(sdf, texW, A.res, A.res, u_, v_, w_, A.brickScale);
+                if (A.fineLayout == LUXGI_LAYOUT_CELL8) return sample3d_cell8<SS>(sdf, texW, A.res, A.res, u_, v_, w_);
+                return sample_volume<SS>(sdf, texW, A.res, A.res, u_, v_, w_);
             };
             const float xp = nl(tx + o, uy, uz), xn = nl(tx - o, uy, uz);
             const float yp = nl(tx, uy + o, uz), yn = nl(tx, uy - o, uz);
@@ -578,14 +591,18 @@ static int launch_trace(const TraceArgs& A, const LuxgiSDFVolume* vol, const Lux
         hipLaunchKernelGGL(zero_bins_kernel, dim3((kChunkBins + 255) / 256), dim3(256), 0, st, w.bins, kChunkBins);
         LUXGI_LAUNCH_CHECK();
     }
+    const dim3 mgrid((int)((lanes + 255) / 256));
+    const SS* vs = static_cast<const SS*>(vol->sdf);
+    const SS* vm = static_cast<const SS*>(vol->mip);
     if (A.fineLayout == LUXGI_LAYOUT_BRICK4)
-        hipLaunchKernelGGL((march_kernel<SS, SR, COUNT, true, BATCH>), dim3((int)((lanes + 255) / 256)), dim3(256), 0, st,
-                           A, static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), vol->guard, w.recA,
-                           w.recB, w.keys, w.bins, skyFaces, counters);
+        hipLaunchKernelGGL((march_kernel<SS, SR, COUNT, LUXGI_LAYOUT_BRICK4, BATCH>), mgrid, dim3(256), 0, st, A, vs, vm,
+                           vol->guard, w.recA, w.recB, w.keys, w.bins, skyFaces, counters);
+    else if (A.fineLayout == LUXGI_LAYOUT_CELL8)
+        hipLaunchKernelGGL((march_kernel<SS, SR, COUNT, LUXGI_LAYOUT_CELL8, BATCH>), mgrid, dim3(256), 0, st, A, vs, vm,
+                           vol->guard, w.recA, w.recB, w.keys, w.bins, skyFaces, counters);
     else
-        hipLaunchKernelGGL((march_kernel<SS, SR, COUNT, false, BATCH>), dim3((int)((lanes + 255) / 256)), dim3(256), 0,
-                           st, A, static_cast<const SS*>(vol->sdf), static_cast<const SS*>(vol->mip), vol->guard,
-                           w.recA, w.recB, w.keys, w.bins, skyFaces, counters);
+        hipLaunchKernelGGL((march_kernel<SS, SR, COUNT, LUXGI_LAYOUT_LINEAR, BATCH>), mgrid, dim3(256), 0, st, A, vs, vm,
+                           vol->guard, w.recA, w.recB, w.keys, w.bins, skyFaces, counters);
     LUXGI_LAUNCH_CHECK();
     if (A.sortHits) {
         hipLaunchKernelGGL(block_scan_kernel<kScanBlock>, dim3(kScanBlocks), dim3(kScanBlock), 0, st, w.bins, w.bins,
@@ -650,24 +667,8 @@ static int trace_scene_args(const LuxgiGlobalSDFData* g, const LuxgiSDFVolume* v
     A.refineLayout = vol->refineLayout;
     if (p->shadeOrder != 0 && p->shadeOrder != 1) return LUXGI_ERR_INVALID_ARGUMENT;
     A.sortHits = !count && p->shadeOrder == 0;  // the counting run shades in ray order
-    if (vol->brickScale && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
-    if (vol->refine && ((vol->refineLayout != LUXGI_LAYOUT_LINEAR && vol->refineLayout != LUXGI_LAYOUT_BRICK4) ||
-                        (vol->refineLayout == LUXGI_LAYOUT_BRICK4 && (vol->resolution & 3))))
-        return LUXGI_ERR_INVALID_ARGUMENT;
-    if (vol->layout != LUXGI_LAYOUT_LINEAR && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
-    // the guard bounds FP32 voxel values; FP16 storage would round below them
-    if (vol->guard && ((vol->resolution & 3) || vol->storage != LUXGI_STORAGE_FP32)) return LUXGI_ERR_INVALID_ARGUMENT;
-    if (vol->layout == LUXGI_LAYOUT_BRICK4 && (vol->resolution & 3)) return LUXGI_ERR_INVALID_ARGUMENT;
-    {  // the samplers address the volumes with 32-bit byte offsets
-        const size_t es = vol->storage == LUXGI_STORAGE_FP16 ? 2 : 4;
-        const size_t C = g->cascadesCount, r = (size_t)vol->resolution, m = (size_t)vol->mipResolution;
-        const size_t fine = vol->layout == LUXGI_LAYOUT_BRICK4 ? C * (r / 4) * (r / 4) * (r / 4) * kBrickElems * es
-                                                               : C * r * r * r * es;
-        const size_t ref = vol->refineLayout == LUXGI_LAYOUT_BRICK4 ? C * (r / 4) * (r / 4) * (r / 4) * kBrickElems * 4
-                                                                    : C * r * r * r * 4;
-        if (fine >= (1ull << 32) || C * m * m * m * es >= (1ull << 32) || (vol->refine && ref >= (1ull << 32)))
-            return LUXGI_ERR_UNSUPPORTED;
-    }
+    const int rc = sdf_volume_status(g, vol);
+    if (rc != LUXGI_OK) return rc;
     A.vols = nullptr;
     A.nVols = 0;
     A.totalRays = 0;

@@ -80,7 +80,7 @@ __device__ __forceinline__ void fetch_light_soft(const LuxgiLight& L, f3 wp, f3 
     atten *= clampf(dot3(n, Wi), 0.0f, 1.0f);
 }
 
-template <class SS, bool BRICK>
+template <class SS, int FL>
 __global__ __launch_bounds__(256) void shadow_mask_kernel(ShadowArgs A, const float* __restrict__ depth,
                                                           const float* __restrict__ normals,
                                                           const uint8_t* __restrict__ sobol,
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256) void shadow_mask_kernel(ShadowArgs A, const fl
         if (NoL > 0.0f && atten > 0.0f) {
             const float sb = A.p.shadowBias;
             const float bias = 2.0f * sb * clampf(1.0f - NoL, 0.0f, 1.0f) + sb;
-            const SdfHitTime h = trace_global_sdf<SS, BRICK>(sdf, A.g, wp + n * sb, Wi, tmax - bias, 0.0f, A.maxSteps);
+            const SdfHitTime h = trace_global_sdf<SS, FL>(sdf, A.g, wp + n * sb, Wi, tmax - bias, 0.0f, A.maxSteps);
             result = h.time >= 0.0f ? 0u : 1u;
         }
     }
@@ -146,7 +146,7 @@ __device__ __forceinline__ f3 importance_sample_ggx(float ex, float ey, f3 N, fl
 
 __device__ __forceinline__ f3 reflect3(f3 I, f3 N) { return I - N * (2.0f * dot3(N, I)); }
 
-template <class SS, bool BRICK, class SC, class SA>
+template <class SS, int FL, class SC, class SA>
 __global__ __launch_bounds__(256) void reflection_kernel(ReflArgsP A, const float* __restrict__ normals,
                                                          const float* __restrict__ pbr, const float* __restrict__ depth,
                                                          const uint8_t* __restrict__ sobol,
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void reflection_kernel(ReflArgsP A, const floa
         R = reflect3(mk(-Wo.x, -Wo.y, -Wo.z), Hh);
     }
     if (traceRay) {
-        const SdfHitTime h = trace_global_sdf<SS, BRICK>(sdf, A.g, wp, R, kGlobalSdfWorldSize, 0.0f, A.maxSteps);
+        const SdfHitTime h = trace_global_sdf<SS, FL>(sdf, A.g, wp, R, kGlobalSdfWorldSize, 0.0f, A.maxSteps);
         if (h.time >= 0.0f) {
             Counts c{};
             const f3 hp = wp + R * h.time;
@@ -210,10 +210,10 @@ __global__ __launch_bounds__(256) void reflection_kernel(ReflArgsP A, const floa
 
 using namespace luxgi;
 
-template <class SS, bool BRICK>
+template <class SS, int FL>
 static void launch_shadow(const ShadowArgs& A, const float* depth, const float* normals, const LuxgiBlueNoise* nz,
                           const LuxgiSDFVolume* vol, uint32_t* mask, hipStream_t st) {
-    hipLaunchKernelGGL((shadow_mask_kernel<SS, BRICK>), dim3((A.mw + 1) / 2, (A.mh + 3) / 4), dim3(256), 0, st, A, depth,
+    hipLaunchKernelGGL((shadow_mask_kernel<SS, FL>), dim3((A.mw + 1) / 2, (A.mh + 3) / 4), dim3(256), 0, st, A, depth,
                        normals, nz->sobol, nz->scrambling, sdf_view<SS>(vol), mask);
 }
 
@@ -235,11 +235,20 @@ extern "C" int luxgi_sdf_shadow_mask(const LuxgiSdfShadowParams* params, const L
     A.mh = (height + 3) / 4;
     A.maxSteps = params->maxSteps > 0 ? params->maxSteps : 250;
     hipStream_t s = as_stream(stream);
-    const bool half = sdf->storage == LUXGI_STORAGE_FP16, brick = sdf->layout == LUXGI_LAYOUT_BRICK4;
-    if (half && brick) launch_shadow<__half, true>(A, depth, normals, noise, sdf, mask, s);
-    else if (half) launch_shadow<__half, false>(A, depth, normals, noise, sdf, mask, s);
-    else if (brick) launch_shadow<float, true>(A, depth, normals, noise, sdf, mask, s);
-    else launch_shadow<float, false>(A, depth, normals, noise, sdf, mask, s);
+    const bool half = sdf->storage == LUXGI_STORAGE_FP16;
+    switch (sdf->layout) {
+        case LUXGI_LAYOUT_BRICK4:
+            if (half) launch_shadow<__half, LUXGI_LAYOUT_BRICK4>(A, depth, normals, noise, sdf, mask, s);
+            else launch_shadow<float, LUXGI_LAYOUT_BRICK4>(A, depth, normals, noise, sdf, mask, s);
+            break;
+        case LUXGI_LAYOUT_CELL8:
+            if (half) launch_shadow<__half, LUXGI_LAYOUT_CELL8>(A, depth, normals, noise, sdf, mask, s);
+            else launch_shadow<float, LUXGI_LAYOUT_CELL8>(A, depth, normals, noise, sdf, mask, s);
+            break;
+        default:
+            if (half) launch_shadow<__half, LUXGI_LAYOUT_LINEAR>(A, depth, normals, noise, sdf, mask, s);
+            else launch_shadow<float, LUXGI_LAYOUT_LINEAR>(A, depth, normals, noise, sdf, mask, s);
+    }
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }
@@ -256,39 +265,39 @@ __global__ void reflection_prepare_kernel(const LuxgiAtlasObject* __restrict__ o
     object_inverse_kernel_body(objects, n, inv);
 }
 
-template <class SS, bool BRICK, class SC, class SA>
+template <class SS, int FL, class SC, class SA>
 static void launch_reflection(const ReflArgsP& A, const LuxgiGBuffer* gb, const LuxgiBlueNoise* nz,
                               const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at, const float* inv,
                               const float* skyFaces, const LuxgiProbeAtlas* probes, void* out, hipStream_t st) {
-    hipLaunchKernelGGL((reflection_kernel<SS, BRICK, SC, SA>), dim3((A.W + 15) / 16, (A.H + 15) / 16), dim3(256), 0, st,
+    hipLaunchKernelGGL((reflection_kernel<SS, FL, SC, SA>), dim3((A.W + 15) / 16, (A.H + 15) / 16), dim3(256), 0, st,
                        A, gb->normals, gb->pbr, gb->depth, nz->sobol, nz->scrambling, sdf_view<SS>(vol), at->chunks,
                        at->culledObjects, at->objects, inv, at->tiles, static_cast<const SC*>(at->color), at->depth,
                        skyFaces, probes ? static_cast<const SA*>(probes->irradiance) : nullptr,
                        probes ? static_cast<const SA*>(probes->depth) : nullptr, out);
 }
 
-template <class SS, bool BRICK, class SC>
+template <class SS, int FL, class SC>
 static void launch_reflection_sa(const ReflArgsP& A, const LuxgiGBuffer* gb, const LuxgiBlueNoise* nz,
                                  const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at, const float* inv,
                                  const float* sky, const LuxgiProbeAtlas* pr, void* out, hipStream_t st) {
     if (pr && pr->storage == LUXGI_STORAGE_FP16)
-        launch_reflection<SS, BRICK, SC, __half>(A, gb, nz, vol, at, inv, sky, pr, out, st);
+        launch_reflection<SS, FL, SC, __half>(A, gb, nz, vol, at, inv, sky, pr, out, st);
     else
-        launch_reflection<SS, BRICK, SC, float>(A, gb, nz, vol, at, inv, sky, pr, out, st);
+        launch_reflection<SS, FL, SC, float>(A, gb, nz, vol, at, inv, sky, pr, out, st);
 }
 
-template <class SS, bool BRICK>
+template <class SS, int FL>
 static int launch_reflection_sc(const ReflArgsP& A, const LuxgiGBuffer* gb, const LuxgiBlueNoise* nz,
                                 const LuxgiSDFVolume* vol, const LuxgiSurfaceAtlas* at, const float* inv,
                                 const float* sky, const LuxgiProbeAtlas* pr, void* out, hipStream_t st) {
     if (at->layout == LUXGI_LAYOUT_BRICK4) {
-        launch_reflection_sa<SS, BRICK, PackedAtlas>(A, gb, nz, vol, at, inv, sky, pr, out, st);
+        launch_reflection_sa<SS, FL, PackedAtlas>(A, gb, nz, vol, at, inv, sky, pr, out, st);
         return LUXGI_OK;
     }
     switch (at->colorStorage) {
-        case LUXGI_STORAGE_FP32: launch_reflection_sa<SS, BRICK, float>(A, gb, nz, vol, at, inv, sky, pr, out, st); break;
-        case LUXGI_STORAGE_FP16: launch_reflection_sa<SS, BRICK, __half>(A, gb, nz, vol, at, inv, sky, pr, out, st); break;
-        case LUXGI_STORAGE_UNORM8: launch_reflection_sa<SS, BRICK, Unorm8>(A, gb, nz, vol, at, inv, sky, pr, out, st); break;
+        case LUXGI_STORAGE_FP32: launch_reflection_sa<SS, FL, float>(A, gb, nz, vol, at, inv, sky, pr, out, st); break;
+        case LUXGI_STORAGE_FP16: launch_reflection_sa<SS, FL, __half>(A, gb, nz, vol, at, inv, sky, pr, out, st); break;
+        case LUXGI_STORAGE_UNORM8: launch_reflection_sa<SS, FL, Unorm8>(A, gb, nz, vol, at, inv, sky, pr, out, st); break;
         default: return LUXGI_ERR_UNSUPPORTED;
     }
     return LUXGI_OK;
@@ -352,12 +361,21 @@ extern "C" int luxgi_sdf_reflection(const LuxgiSdfReflectionParams* params, cons
     const float* skyFaces = sky && sky->faces ? sky->faces : nullptr;
     const LuxgiProbeAtlas* pr = useDdgi ? probes : nullptr;
     A.rl = probe_reloc(pr);
-    const bool half = sdf->storage == LUXGI_STORAGE_FP16, brick = sdf->layout == LUXGI_LAYOUT_BRICK4;
+    const bool half = sdf->storage == LUXGI_STORAGE_FP16;
     int rc;
-    if (half && brick) rc = launch_reflection_sc<__half, true>(A, gb, noise, sdf, at, inv, skyFaces, pr, out, s);
-    else if (half) rc = launch_reflection_sc<__half, false>(A, gb, noise, sdf, at, inv, skyFaces, pr, out, s);
-    else if (brick) rc = launch_reflection_sc<float, true>(A, gb, noise, sdf, at, inv, skyFaces, pr, out, s);
-    else rc = launch_reflection_sc<float, false>(A, gb, noise, sdf, at, inv, skyFaces, pr, out, s);
+    switch (sdf->layout) {
+        case LUXGI_LAYOUT_BRICK4:
+            rc = half ? launch_reflection_sc<__half, LUXGI_LAYOUT_BRICK4>(A, gb, noise, sdf, at, inv, skyFaces, pr, out, s)
+                      : launch_reflection_sc<float, LUXGI_LAYOUT_BRICK4>(A, gb, noise, sdf, at, inv, skyFaces, pr, out, s);
+            break;
+        case LUXGI_LAYOUT_CELL8:
+            rc = half ? launch_reflection_sc<__half, LUXGI_LAYOUT_CELL8>(A, gb, noise, sdf, at, inv, skyFaces, pr, out, s)
+                      : launch_reflection_sc<float, LUXGI_LAYOUT_CELL8>(A, gb, noise, sdf, at, inv, skyFaces, pr, out, s);
+            break;
+        default:
+            rc = half ? launch_reflection_sc<__half, LUXGI_LAYOUT_LINEAR>(A, gb, noise, sdf, at, inv, skyFaces, pr, out, s)
+                      : launch_reflection_sc<float, LUXGI_LAYOUT_LINEAR>(A, gb, noise, sdf, at, inv, skyFaces, pr, out, s);
+    }
     if (rc != LUXGI_OK) return rc;
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;

@@ -70,6 +70,28 @@ __global__ void to_bricks_kernel(const float* __restrict__ src, int W, int H, in
     Texel<D_>::st1(dst, (size_t)b * kBrickElems + t, v);
 }
 
+// One thread per record of LUXGI_LAYOUT_CELL8: record r of brick b holds the 8 corner voxels of cell
+// (x, y, z) = 8 * brick + the Morton-decoded r, clamped to the volume; padding records are zero.
+template <class D_>
+__global__ void to_cells_kernel(const float* __restrict__ src, int W, int H, int D, uint32_t nbx, uint32_t nby,
+                                uint32_t records, D_* __restrict__ dst) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= records) return;
+    const uint32_t b = r >> 9, m = r & 511u;
+    const uint32_t bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int x = int(8 * bx + cell8_compact(m)), y = int(8 * by + cell8_compact(m >> 1)),
+              z = int(8 * bz + cell8_compact(m >> 2));
+    float v[8];
+    const bool inside = x < W && y < H && z < D;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int xi = min(x + (k & 1), W - 1), yj = min(y + ((k >> 1) & 1), H - 1), zl = min(z + (k >> 2), D - 1);
+        v[k] = inside ? src[((size_t)zl * H + yj) * W + xi] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) Texel<D_>::st1(dst, (size_t)r * 8 + k, v[k]);
+}
+
 __global__ void guard_kernel(const float* __restrict__ src, int W, int H, int D, float* __restrict__ guard) {
     const int nbx = W >> 2, nby = H >> 2, nbz = D >> 2;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -113,6 +135,31 @@ extern "C" int luxgi_sdf_to_bricks(const float* src, int32_t W, int32_t H, int32
     else
         hipLaunchKernelGGL(to_bricks_kernel<__half>, grid, dim3(kBrickElems), 0, as_stream(stream), src, W, H, D,
                            static_cast<__half*>(dst), brickScale);
+    LUXGI_LAUNCH_CHECK();
+    return LUXGI_OK;
+}
+
+extern "C" size_t luxgi_sdf_cells_bytes(int32_t W, int32_t H, int32_t D, int32_t storage) {
+    if (W <= 0 || H <= 0 || D <= 0) return 0;
+    if (storage != LUXGI_STORAGE_FP32 && storage != LUXGI_STORAGE_FP16) return 0;
+    const size_t records = cell8_records(W, H, D);
+    if (records >= (1ull << 32)) return 0;
+    return records * 8 * (storage == LUXGI_STORAGE_FP16 ? 2 : 4);
+}
+
+extern "C" int luxgi_sdf_to_cells(const float* src, int32_t W, int32_t H, int32_t D, int32_t storage, void* dst,
+                                  void* stream) {
+    const size_t bytes = luxgi_sdf_cells_bytes(W, H, D, storage);
+    if (!src || !dst || bytes == 0) return LUXGI_ERR_INVALID_ARGUMENT;
+    const uint32_t records = (uint32_t)cell8_records(W, H, D);
+    const uint32_t nbx = (uint32_t)(W + 7) / 8, nby = (uint32_t)(H + 7) / 8;
+    const dim3 grid((records + 255) / 256);
+    if (storage == LUXGI_STORAGE_FP32)
+        hipLaunchKernelGGL(to_cells_kernel<float>, grid, dim3(256), 0, as_stream(stream), src, W, H, D, nbx, nby, records,
+                           static_cast<float*>(dst));
+    else
+        hipLaunchKernelGGL(to_cells_kernel<__half>, grid, dim3(256), 0, as_stream(stream), src, W, H, D, nbx, nby,
+                           records, static_cast<__half*>(dst));
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }

@@ -10,8 +10,8 @@ namespace luxgi {
 
 template <class SS>
 struct SdfView {
-    const SS* sdf;             // fine volume, LINEAR or BRICK4 layout
-    const SS* mip;             // coarse volume, LINEAR
+    const SS* sdf;             // fine volume, LINEAR, BRICK4 or CELL8 layout
+    const SS* mip;             // coarse volume, LINEAR (CELL8 with a CELL8 fine volume)
     const float* guard;        // optional brick guard (FP32 LINEAR volumes only)
     const float* brickScale;   // optional per-brick scale (BRICK4)
     int res, mipRes;
@@ -25,13 +25,20 @@ struct SdfHitTime {
     int cascade;
 };
 
-template <class SS, bool BRICK>
+// fine and coarse lookups in the volume's layout (FL: LuxgiLayout; CELL8 holds both in cell records)
+template <class SS, int FL>
 __device__ __forceinline__ float sdf_fine(const SdfView<SS>& v, int W, float u, float y, float z) {
-    if constexpr (BRICK) return sample3d_brick4<SS>(v.sdf, W, v.res, v.res, u, y, z, v.brickScale);
+    if constexpr (FL == LUXGI_LAYOUT_BRICK4) return sample3d_brick4<SS>(v.sdf, W, v.res, v.res, u, y, z, v.brickScale);
+    else if constexpr (FL == LUXGI_LAYOUT_CELL8) return sample3d_cell8<SS>(v.sdf, W, v.res, v.res, u, y, z);
     else return sample_volume<SS>(v.sdf, W, v.res, v.res, u, y, z);
 }
+template <class SS, int FL>
+__device__ __forceinline__ float sdf_coarse(const SdfView<SS>& v, int W, float u, float y, float z) {
+    if constexpr (FL == LUXGI_LAYOUT_CELL8) return sample3d_cell8<SS>(v.mip, W, v.mipRes, v.mipRes, u, y, z);
+    else return sample_volume<SS>(v.mip, W, v.mipRes, v.mipRes, u, y, z);
+}
 
-template <class SS, bool BRICK>
+template <class SS, int FL>
 __device__ SdfHitTime trace_global_sdf(const SdfView<SS>& v, const LuxgiGlobalSDFData& g, f3 o, f3 d,
                                        float maxDistance, float startBias, int maxSteps) {
     SdfHitTime hit{-1.0f, 0.0f, 0};
@@ -76,8 +83,8 @@ __device__ SdfHitTime trace_global_sdf(const SdfView<SS>& v, const LuxgiGlobalSD
                 needFine = !(gv >= chunkMarginDistance * 2.0f * (1.0f + 1e-5f));
             }
             // both lookups are issued together; the fine value is used only under the mip test
-            float sd = sample_volume<SS>(v.mip, mipW, v.mipRes, v.mipRes, tx, uy, uz);
-            const float st = needFine ? sdf_fine<SS, BRICK>(v, texW, tx, uy, uz) : 1.0f;
+            float sd = sdf_coarse<SS, FL>(v, mipW, tx, uy, uz);
+            const float st = needFine ? sdf_fine<SS, FL>(v, texW, tx, uy, uz) : 1.0f;
             if (sd < chunkSizeDistance) {
                 if (needFine && st < chunkMarginDistance * 2.0f) sd = st;
             } else {
@@ -103,16 +110,31 @@ inline int sdf_volume_status(const LuxgiGlobalSDFData* g, const LuxgiSDFVolume* 
     if (g->cascadesCount < 1 || g->cascadesCount > 4) return LUXGI_ERR_INVALID_ARGUMENT;
     if (vol->resolution <= 0 || vol->mipResolution <= 0) return LUXGI_ERR_INVALID_ARGUMENT;
     if (vol->storage != LUXGI_STORAGE_FP32 && vol->storage != LUXGI_STORAGE_FP16) return LUXGI_ERR_UNSUPPORTED;
-    if (vol->layout != LUXGI_LAYOUT_LINEAR && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (vol->layout != LUXGI_LAYOUT_LINEAR && vol->layout != LUXGI_LAYOUT_BRICK4 && vol->layout != LUXGI_LAYOUT_CELL8)
+        return LUXGI_ERR_INVALID_ARGUMENT;
     if (vol->layout == LUXGI_LAYOUT_BRICK4 && (vol->resolution & 3)) return LUXGI_ERR_INVALID_ARGUMENT;
     if (vol->brickScale && vol->layout != LUXGI_LAYOUT_BRICK4) return LUXGI_ERR_INVALID_ARGUMENT;
+    // the guard bounds FP32 voxel values; FP16 storage would round below them
     if (vol->guard && ((vol->resolution & 3) || vol->storage != LUXGI_STORAGE_FP32)) return LUXGI_ERR_INVALID_ARGUMENT;
-    // the samplers address the volumes with 32-bit byte offsets
+    if (vol->refine) {
+        const int rl = vol->refineLayout;
+        if (rl != LUXGI_LAYOUT_LINEAR && rl != LUXGI_LAYOUT_BRICK4 && rl != LUXGI_LAYOUT_CELL8)
+            return LUXGI_ERR_INVALID_ARGUMENT;
+        if (rl == LUXGI_LAYOUT_BRICK4 && (vol->resolution & 3)) return LUXGI_ERR_INVALID_ARGUMENT;
+    }
+    // the linear and brick samplers address the volumes with 32-bit byte offsets, the cell-record
+    // sampler with 32-bit record indices
     const size_t es = vol->storage == LUXGI_STORAGE_FP16 ? 2 : 4;
-    const size_t C = g->cascadesCount, r = (size_t)vol->resolution, m = (size_t)vol->mipResolution;
-    const size_t fine = vol->layout == LUXGI_LAYOUT_BRICK4 ? C * (r / 4) * (r / 4) * (r / 4) * kBrickElems * es
-                                                           : C * r * r * r * es;
-    if (fine >= (1ull << 32) || C * m * m * m * es >= (1ull << 32)) return LUXGI_ERR_UNSUPPORTED;
+    const int C = (int)g->cascadesCount, r = vol->resolution, m = vol->mipResolution;
+    auto too_big = [&](int layout, size_t elem) {
+        if (layout == LUXGI_LAYOUT_CELL8)
+            return cell8_records(C * r, r, r) >= (1ull << 32) || cell8_records(C * m, m, m) >= (1ull << 32);
+        const size_t R = (size_t)r;
+        const size_t fine = layout == LUXGI_LAYOUT_BRICK4 ? C * (R / 4) * (R / 4) * (R / 4) * kBrickElems * elem
+                                                          : C * R * R * R * elem;
+        return fine >= (1ull << 32) || (size_t)C * m * m * m * elem >= (1ull << 32);
+    };
+    if (too_big(vol->layout, es) || (vol->refine && too_big(vol->refineLayout, 4))) return LUXGI_ERR_UNSUPPORTED;
     return LUXGI_OK;
 }
 

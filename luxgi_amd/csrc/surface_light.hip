@@ -96,7 +96,7 @@ __device__ __forceinline__ void fetch_light(const LuxgiLight& L, f3 wp, f3& Wi, 
     }
 }
 
-template <class SS, bool BRICK, class SA>
+template <class SS, int FL, class SA>
 __global__ __launch_bounds__(256) void surface_light_kernel(SurfaceArgs A, const LuxgiLight* __restrict__ lights,
                                                             const LuxgiAtlasTile* __restrict__ tiles, int numTiles,
                                                             const LuxgiSurfaceDraw* __restrict__ draws,
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void surface_light_kernel(SurfaceArgs A, const
                 if (NoL > 0.0f) {
                     shadow = 1.0f;
                     if (atten > 0.0f) {
-                                const SdfHitTime h = trace_global_sdf<SS, BRICK>(sdf, A.g, wp + n * sb, L, dist - bias,
+                                const SdfHitTime h = trace_global_sdf<SS, FL>(sdf, A.g, wp + n * sb, L, dist - bias,
                                                                          2.0f, A.maxSteps);
                         shadow = h.time >= 0.0f ? 0.0f : 1.0f;
                     }
@@ -179,26 +179,26 @@ __global__ __launch_bounds__(256) void surface_light_kernel(SurfaceArgs A, const
 
 using namespace luxgi;
 
-template <class SS, bool BRICK, class SA>
+template <class SS, int FL, class SA>
 static void launch_surface(const SurfaceArgs& A, const LuxgiLight* lights, const LuxgiAtlasTile* tiles,
                            int numTiles, const LuxgiSurfaceDraw* draws, int numDraws, const LuxgiSurfaceGBuffer* gb,
                            const LuxgiSDFVolume* vol, const LuxgiProbeAtlas* probes, void* out, hipStream_t st) {
     const SdfView<SS> v = sdf_view<SS>(vol);
-    hipLaunchKernelGGL((surface_light_kernel<SS, BRICK, SA>), dim3(numDraws), dim3(256), 0, st, A, lights, tiles,
+    hipLaunchKernelGGL((surface_light_kernel<SS, FL, SA>), dim3(numDraws), dim3(256), 0, st, A, lights, tiles,
                        numTiles, draws, static_cast<const __half*>(gb->color), static_cast<const __half*>(gb->normal),
                        static_cast<const Unorm8*>(gb->pbr), static_cast<const __half*>(gb->emissive), v,
                        probes ? static_cast<const SA*>(probes->irradiance) : nullptr,
                        probes ? static_cast<const SA*>(probes->depth) : nullptr, out);
 }
 
-template <class SS, bool BRICK>
+template <class SS, int FL>
 static void launch_surface_sa(const SurfaceArgs& A, const LuxgiLight* lights, const LuxgiAtlasTile* tiles,
                               int numTiles, const LuxgiSurfaceDraw* draws, int numDraws, const LuxgiSurfaceGBuffer* gb,
                               const LuxgiSDFVolume* vol, const LuxgiProbeAtlas* probes, void* out, hipStream_t st) {
     if (probes && probes->storage == LUXGI_STORAGE_FP16)
-        launch_surface<SS, BRICK, __half>(A, lights, tiles, numTiles, draws, numDraws, gb, vol, probes, out, st);
+        launch_surface<SS, FL, __half>(A, lights, tiles, numTiles, draws, numDraws, gb, vol, probes, out, st);
     else
-        launch_surface<SS, BRICK, float>(A, lights, tiles, numTiles, draws, numDraws, gb, vol, probes, out, st);
+        launch_surface<SS, FL, float>(A, lights, tiles, numTiles, draws, numDraws, gb, vol, probes, out, st);
 }
 
 extern "C" int luxgi_surface_light(const LuxgiSurfaceLightParams* params, const LuxgiLight* lights,
@@ -248,11 +248,21 @@ extern "C" int luxgi_surface_light(const LuxgiSurfaceLightParams* params, const 
     const LuxgiSDFVolume* v = direct ? vol : nullptr;
     const LuxgiProbeAtlas* pr = indirect ? probes : nullptr;
     A.rl = probe_reloc(pr);
-    const bool half = v && v->storage == LUXGI_STORAGE_FP16, brick = v && v->layout == LUXGI_LAYOUT_BRICK4;
-    if (half && brick) launch_surface_sa<__half, true>(A, lights, tiles, numTiles, draws, numDraws, gb, v, pr, lightCache, st);
-    else if (half) launch_surface_sa<__half, false>(A, lights, tiles, numTiles, draws, numDraws, gb, v, pr, lightCache, st);
-    else if (brick) launch_surface_sa<float, true>(A, lights, tiles, numTiles, draws, numDraws, gb, v, pr, lightCache, st);
-    else launch_surface_sa<float, false>(A, lights, tiles, numTiles, draws, numDraws, gb, v, pr, lightCache, st);
+    const bool half = v && v->storage == LUXGI_STORAGE_FP16;
+    const int layout = v ? v->layout : LUXGI_LAYOUT_LINEAR;
+#define LUXGI_SURFACE_LAUNCH(SS_, FL_) \
+    launch_surface_sa<SS_, FL_>(A, lights, tiles, numTiles, draws, numDraws, gb, v, pr, lightCache, st)
+    if (layout == LUXGI_LAYOUT_BRICK4) {
+        if (half) LUXGI_SURFACE_LAUNCH(__half, LUXGI_LAYOUT_BRICK4);
+        else LUXGI_SURFACE_LAUNCH(float, LUXGI_LAYOUT_BRICK4);
+    } else if (layout == LUXGI_LAYOUT_CELL8) {
+        if (half) LUXGI_SURFACE_LAUNCH(__half, LUXGI_LAYOUT_CELL8);
+        else LUXGI_SURFACE_LAUNCH(float, LUXGI_LAYOUT_CELL8);
+    } else {
+        if (half) LUXGI_SURFACE_LAUNCH(__half, LUXGI_LAYOUT_LINEAR);
+        else LUXGI_SURFACE_LAUNCH(float, LUXGI_LAYOUT_LINEAR);
+    }
+#undef LUXGI_SURFACE_LAUNCH
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }

@@ -76,10 +76,30 @@ class DeviceScene:
                                             stream_handle()), "luxgi_sdf_to_bricks")
             return out
 
+        def cells(vol, W, H, D, storage, dt):
+            nbytes = lib().luxgi_sdf_cells_bytes(W, H, D, storage)
+            if nbytes == 0:
+                raise ValueError("volume too large for LAYOUT_CELL8")
+            out = torch.empty(nbytes // (2 if storage == abi.STORAGE_FP16 else 4), dtype=dt, device="cuda")
+            check(lib().luxgi_sdf_to_cells(ptr(vol), W, H, D, storage, ptr(out), stream_handle()),
+                  "luxgi_sdf_to_cells")
+            return out
+
         if brick_scale and layout != abi.LAYOUT_BRICK4:
             raise ValueError("a per-brick scale needs LAYOUT_BRICK4")
         if refine:  # FP32 distances for the hits, in the same layout
-            self.refine = bricks(abi.STORAGE_FP32, torch.float32) if layout == abi.LAYOUT_BRICK4 else src
+            if layout == abi.LAYOUT_BRICK4:
+                self.refine = bricks(abi.STORAGE_FP32, torch.float32)
+            elif layout == abi.LAYOUT_CELL8:
+                self.refine = cells(src, self.res * C_, self.res, self.res, abi.STORAGE_FP32, torch.float32)
+            else:
+                self.refine = src
+        if layout == abi.LAYOUT_CELL8:
+            # fine volume and coarse mip both as 8-corner cell records (one aligned record per lookup)
+            mip32 = self.mip.float()
+            self.sdf = cells(src, self.res * C_, self.res, self.res, sdf_storage, sdt)
+            self.mip = cells(mip32, self.mip_res * C_, self.mip_res, self.mip_res, sdf_storage, sdt)
+            del mip32
         if layout == abi.LAYOUT_BRICK4:
             # the engine would keep its SDF in this layout; here it is converted once
             if brick_scale:

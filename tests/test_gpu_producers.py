@@ -48,7 +48,8 @@ def _light(kind):
 # point and spot lights its interior (camera inside)
 @pytest.mark.parametrize("kind,W,H,layout,eye", [("sun", 256, 160, abi.LAYOUT_BRICK4, (0.1, 0.3, 2.2)),
                                                   ("point", 250, 158, abi.LAYOUT_LINEAR, (0.1, 0.3, 0.75)),
-                                                  ("spot", 200, 120, abi.LAYOUT_BRICK4, (0.2, 0.4, 0.7))])
+                                                  ("spot", 200, 120, abi.LAYOUT_BRICK4, (0.2, 0.4, 0.7)),
+                                                  ("sun", 160, 96, abi.LAYOUT_CELL8, (0.1, 0.3, 2.2))])
 def test_shadow_mask_matches_oracle(kind, W, H, layout, eye):
     from luxgi_amd.ddgi import DeviceScene, to_device
     from luxgi_amd.producers import BlueNoiseTables, sdf_shadow_mask
@@ -89,7 +90,8 @@ def test_shadow_mask_rejects_bad_arguments():
     assert lib().luxgi_sdf_shadow_mask(C.byref(p), None, 8, 8, None, None, None, None, None, None) < 0
 
 
-@pytest.mark.parametrize("layout,atlas_layout", [(abi.LAYOUT_BRICK4, abi.LAYOUT_LINEAR), (abi.LAYOUT_LINEAR, abi.LAYOUT_BRICK4)])
+@pytest.mark.parametrize("layout,atlas_layout", [(abi.LAYOUT_BRICK4, abi.LAYOUT_LINEAR), (abi.LAYOUT_LINEAR, abi.LAYOUT_BRICK4),
+                                                 (abi.LAYOUT_CELL8, abi.LAYOUT_BRICK4)])
 def test_sdf_reflection_matches_oracle(layout, atlas_layout):
     """All three branches (mirror, GGX lobe, DDGI) against the FP64 oracle: >= 99% of non-sky pixels
     agree to 2e-3 (1 + |v|) on every channel (a GGX ray through fp32 trigonometry may graze past

@@ -84,7 +84,7 @@ def _probe_atlases(u, seed=3):
     return irr, dep
 
 
-@pytest.mark.parametrize("form", ["linear", "brick_guard", "fp16_scaled"])
+@pytest.mark.parametrize("form", ["linear", "brick_guard", "fp16_scaled", "cell8"])
 def test_surface_light_matches_oracle(form):
     from luxgi_amd.ddgi import DDGIVolume, DeviceScene
     from luxgi_amd.surface import SurfaceCache
@@ -92,7 +92,8 @@ def test_surface_light_matches_oracle(form):
     res = 64
     atlas = scene.build_atlas(prims, resolution=512, tile=32, seed=3, threshold=1.05 * 2 / res, surface_gbuffer=True)
     kw = dict(linear={}, brick_guard=dict(layout=abi.LAYOUT_BRICK4, guard=True),
-              fp16_scaled=dict(layout=abi.LAYOUT_BRICK4, sdf_storage=abi.STORAGE_FP16, brick_scale=True))[form]
+              fp16_scaled=dict(layout=abi.LAYOUT_BRICK4, sdf_storage=abi.STORAGE_FP16, brick_scale=True),
+              cell8=dict(layout=abi.LAYOUT_CELL8))[form]
     ds = DeviceScene(g, sdf, mip, atlas, **kw)
     u = _scenes.small_uniform(rays=64, counts=(5, 4, 6))
     vol = DDGIVolume(u)

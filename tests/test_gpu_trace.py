@@ -137,15 +137,16 @@ def test_trace_counts_match_oracle():
 
 
 def test_trace_brick_layout_is_bitwise_identical():
-    """LUXGI_LAYOUT_BRICK4 only changes where voxels live and the brick guard only skips fine
-    lookups whose value is not used: outputs equal the plain linear trace's bit for bit."""
+    """LUXGI_LAYOUT_BRICK4 and LUXGI_LAYOUT_CELL8 only change where voxels live and the brick guard
+    only skips fine lookups whose value is not used: outputs equal the plain linear trace's bit for
+    bit."""
     from luxgi_amd.ddgi import DDGIVolume, DeviceScene
     prims, sdf, mip, g, atlas = _scenes.small_scene()
     u = _scenes.small_uniform(rays=128, counts=(6, 5, 4))
     p = _scenes.trace_params(rot_seed=21)
     outs = []
     for layout, guard in ((abi.LAYOUT_LINEAR, False), (abi.LAYOUT_BRICK4, False), (abi.LAYOUT_BRICK4, True),
-                          (abi.LAYOUT_LINEAR, True)):
+                          (abi.LAYOUT_LINEAR, True), (abi.LAYOUT_CELL8, False), (abi.LAYOUT_CELL8, True)):
         ds = DeviceScene(g, sdf, mip, atlas, layout=layout, guard=guard)
         vol = DDGIVolume(u, flags_buffer=True)
         vol.trace_rays(ds, p)
@@ -154,6 +155,45 @@ def test_trace_brick_layout_is_bitwise_identical():
     for other in outs[1:]:
         for a, b in zip(outs[0], other):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("shape,storage", [((13, 7, 9), abi.STORAGE_FP32), ((24, 16, 8), abi.STORAGE_FP16),
+                                           ((1, 1, 1), abi.STORAGE_FP32)])
+def test_cell_records_hold_the_clamped_corners(shape, storage):
+    """luxgi_sdf_to_cells: record of cell (x, y, z) = the 8 voxels (x+i, y+j, z+k) clamped to the
+    volume, at brick (x/8, y/8, z/8) x-fastest, Morton slot (bit 3b+a = bit b of coordinate a)."""
+    from luxgi_amd import _lib
+    from luxgi_amd.ddgi import ptr, stream_handle
+    W, H, D = shape
+    vol = np.random.default_rng(4).uniform(-1, 1, (D, H, W)).astype(np.float32)
+    nbytes = _lib.lib().luxgi_sdf_cells_bytes(W, H, D, storage)
+    nbx, nby, nbz = (W + 7) // 8, (H + 7) // 8, (D + 7) // 8
+    assert nbytes == nbx * nby * nbz * 512 * 8 * (2 if storage == abi.STORAGE_FP16 else 4)
+    dt = torch.float16 if storage == abi.STORAGE_FP16 else torch.float32
+    out = torch.full((nbytes // (2 if storage == abi.STORAGE_FP16 else 4),), 7.0, dtype=dt, device="cuda")
+    src = torch.from_numpy(vol).cuda()
+    assert _lib.lib().luxgi_sdf_to_cells(ptr(src), W, H, D, storage, ptr(out), stream_handle()) == 0
+    torch.cuda.synchronize()
+    rec = out.float().cpu().numpy().reshape(-1, 8)
+    ref = vol.astype(np.float16).astype(np.float32) if storage == abi.STORAGE_FP16 else vol
+
+    def spread(v):
+        return (v & 1) | ((v & 2) << 2) | ((v & 4) << 4)
+    for z in range(D):
+        for y in range(H):
+            for x in range(W):
+                idx = (((z >> 3) * nby + (y >> 3)) * nbx + (x >> 3)) * 512 + \
+                    (spread(x & 7) | (spread(y & 7) << 1) | (spread(z & 7) << 2))
+                want = [ref[min(z + k, D - 1), min(y + j, H - 1), min(x + i, W - 1)]
+                        for k in (0, 1) for j in (0, 1) for i in (0, 1)]
+                assert np.array_equal(rec[idx], np.array(want, np.float32)), (x, y, z)
+    written = np.zeros(rec.shape[0], bool)
+    for z in range(D):
+        for y in range(H):
+            for x in range(W):
+                written[(((z >> 3) * nby + (y >> 3)) * nbx + (x >> 3)) * 512 +
+                        (spread(x & 7) | (spread(y & 7) << 1) | (spread(z & 7) << 2))] = True
+    assert np.all(rec[~written] == 0)  # padding records of partial bricks
 
 
 def test_trace_fp16_rays():
@@ -194,14 +234,15 @@ def test_trace_fp16_sdf_bricks_match_linear_and_oracle():
     u = _scenes.small_uniform(rays=64)
     p = _scenes.trace_params(rot_seed=17)
     outs = []
-    for layout in (abi.LAYOUT_LINEAR, abi.LAYOUT_BRICK4):
+    for layout in (abi.LAYOUT_LINEAR, abi.LAYOUT_BRICK4, abi.LAYOUT_CELL8):
         ds = DeviceScene(g, sdf, mip, atlas, sdf_storage=abi.STORAGE_FP16, layout=layout)
         vol = DDGIVolume(u, flags_buffer=True)
         vol.trace_rays(ds, p)
         torch.cuda.synchronize()
         outs.append((vol.radiance.cpu().numpy(), vol.dir_dist.cpu().numpy(), vol.ray_flags.cpu().numpy()))
-    for a, b in zip(*outs):
-        assert np.array_equal(a, b)
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)
     s16 = sdf.astype(np.float16).astype(np.float32)
     m16 = mip.astype(np.float16).astype(np.float32)
     ref = _scenes.oracle_trace(u, g, s16, m16, atlas, p)
@@ -257,7 +298,7 @@ def test_trace_shade_order_is_bitwise_identical(layout, guard):
             assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("layout", [abi.LAYOUT_LINEAR, abi.LAYOUT_BRICK4])
+@pytest.mark.parametrize("layout", [abi.LAYOUT_LINEAR, abi.LAYOUT_BRICK4, abi.LAYOUT_CELL8])
 def test_trace_multi_cascade_matches_oracle(layout):
     """A three-cascade clipmap (half extents 0.5, 1, 2 at 32^3, packed along x) of the small scene;
     probes inside and outside cascade 0, so rays start in different cascades and move out through

@@ -311,6 +311,23 @@ int luxgi_ddgi_probe_update(const LuxgiDDGIUniform* uniform,
                             void* workspace, size_t workspaceBytes,
                             void* stream);
 
+/* luxgi_ddgi_probe_update without a workspace argument and without any allocation. No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The workspace lives in
+ * a static device array of LUXGI_UPDATE_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends; volumes whose luxgi_ddgi_probe_update_workspace_size() exceeds it are rejected with
+ * LUXGI_ERR_WORKSPACE_TOO_SMALL (256 rays, 16/16 oct and up to 2M probes fit). All calls on one
+ * GPU share that array, so calls that overlap on one GPU (different streams, different host
+ * threads) race; calls ordered on one stream do not. Results equal luxgi_ddgi_probe_update's. */
+#define LUXGI_UPDATE_NOALLOC_SCRATCH_BYTES (4u << 20)
+int luxgi_ddgi_probe_update_noalloc(const LuxgiDDGIUniform* uniform,
+                                    const LuxgiRayBuffers* rays,
+                                    const LuxgiProbeAtlas* prev,
+                                    const LuxgiProbeAtlas* out,
+                                    uint32_t firstFrame,
+                                    uint32_t flags,
+                                    void* stream);
+
 /* IrradianceBorderUpdate.comp / DepthBorderUpdate.comp (Shaders/DDGI/BorderUpdate.glsl:136-156),
  * dispatched at DDGIRenderer.cpp:420-438. In-place on `atlas`.
  * which: LUXGI_UPDATE_IRRADIANCE and/or LUXGI_UPDATE_DEPTH. */

@@ -23,6 +23,7 @@ SIGNATURES = {
     "luxgi_ddgi_trace_sdf_count": (C.c_int, [P, P, P, P, P, P, P, P, P, P, C.c_size_t, P, P]),
     "luxgi_ddgi_probe_update_workspace_size": (C.c_size_t, [P]),
     "luxgi_ddgi_probe_update": (C.c_int, [P, P, P, P, U, U, P, C.c_size_t, P]),
+    "luxgi_ddgi_probe_update_noalloc": (C.c_int, [P, P, P, P, U, U, P]),
     "luxgi_ddgi_border_update": (C.c_int, [P, P, U, P]),
     "luxgi_ddgi_relocate_classify": (C.c_int, [P, P, P, P, P, P, P]),
     "luxgi_ddgi_sample_probes": (C.c_int, [P, P, P, I, I, P, P, I, P, P]),

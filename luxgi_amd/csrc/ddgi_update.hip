@@ -20,6 +20,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace luxgi {
 
@@ -808,6 +809,46 @@ static int launch_update(const UpdateArgs& A, const LuxgiRayBuffers* rays, const
                        st, A, rad, dd, pI, pD, oI, oD, w.Wi, w.Wd, nullptr);
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
+}
+
+// Scratch of luxgi_ddgi_probe_update_noalloc: the same workspace layout as the caller-provided
+// one, held in a file-scope device array. Every GPU holds its own copy at its own address, found
+// once per device with hipGetSymbolAddress; host code never passes the array's name to a kernel
+// (on the host it is a stand-in, not a device address). Every part of the workspace is written
+// by the update's own kernels before it is read, so nothing needs clearing between calls.
+static __device__ __align__(256) char scratch[LUXGI_UPDATE_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratchLock;
+void* g_scratchPtr[kMaxScratchDevices] = {};
+
+// the scratch array's device address on the current device (found on the first call there)
+int scratch_address(void** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return LUXGI_ERR_HIP;
+    if (dev < 0 || dev >= kMaxScratchDevices) return LUXGI_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> g(g_scratchLock);
+    if (!g_scratchPtr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return LUXGI_ERR_HIP;
+        g_scratchPtr[dev] = p;
+    }
+    *out = g_scratchPtr[dev];
+    return LUXGI_OK;
+}
+}  // namespace
+
+extern "C" int luxgi_ddgi_probe_update_noalloc(const LuxgiDDGIUniform* u, const LuxgiRayBuffers* rays,
+                                               const LuxgiProbeAtlas* prev, const LuxgiProbeAtlas* out,
+                                               uint32_t firstFrame, uint32_t flags, void* stream) {
+    if (!u) return LUXGI_ERR_INVALID_ARGUMENT;
+    const size_t need = update_workspace_layout(u, nullptr, nullptr);
+    if (need > (size_t)LUXGI_UPDATE_NOALLOC_SCRATCH_BYTES) return LUXGI_ERR_WORKSPACE_TOO_SMALL;
+    void* ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc) return rc;
+    return luxgi_ddgi_probe_update(u, rays, prev, out, firstFrame, flags, ws, need, stream);
 }
 
 extern "C" int luxgi_ddgi_probe_update(const LuxgiDDGIUniform* u, const LuxgiRayBuffers* rays,

@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision form")
     ap.add_argument("--no-surface", action="store_true", help="skip the surface-atlas passes")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched SVGF measurement")
+    ap.add_argument("--sdf-layout", choices=["cell8", "brick4", "linear"], default="cell8",
+                    help="SDF layout of both forms (cell8: one 8-corner record per lookup)")
     ap.add_argument("--step-only", action="store_true",
                     help="time the step (DDGI update + SVGF) and the counting run only: no side passes, batched "
                          "forms, reduced-precision form or CPU baseline (for profiler runs)")
@@ -232,7 +234,8 @@ def main():
     atlas = scene.build_atlas(prims, resolution=2048, tile=16, color_storage=abi.STORAGE_UNORM8, seed=3,
                               threshold=1.05 * 2.0 / res, surface_gbuffer=not args.no_surface)
     log("[bench] atlas 2048^2 built", q)
-    ds = DeviceScene(g, sdf, mip, atlas, layout=abi.LAYOUT_BRICK4)
+    layout = {"cell8": abi.LAYOUT_CELL8, "brick4": abi.LAYOUT_BRICK4, "linear": abi.LAYOUT_LINEAR}[args.sdf_layout]
+    ds = DeviceScene(g, sdf, mip, atlas, layout=layout)
     counts = (32, 16, 32)
     step = [2.0 / (c - 1) for c in counts]
     u = abi.make_uniform(counts, (-1.0, -1.0, -1.0), step, 256, irr_oct=8, depth_oct=16, hysteresis=0.98)
@@ -494,7 +497,8 @@ def main():
     # FP32: RGBA16F distances lose the depth moments of near hits to the cancellation in t - 0.01.
     mixed = None
     if not args.no_mixed:
-        ds16 = DeviceScene(g, sdf, mip, atlas, sdf_storage=abi.STORAGE_FP16, layout=abi.LAYOUT_BRICK4, brick_scale=True,
+        ds16 = DeviceScene(g, sdf, mip, atlas, sdf_storage=abi.STORAGE_FP16, layout=layout,
+                           brick_scale=layout == abi.LAYOUT_BRICK4,
                            refine=True)
         vol16 = DDGIVolume(u)
         dn16 = ReflectionDenoiser(W, H, storage=abi.STORAGE_FP16, iterations=args.svgf_iterations)

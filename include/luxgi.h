@@ -48,7 +48,8 @@ typedef enum LuxgiStatus {
 typedef enum LuxgiStorage {
     LUXGI_STORAGE_FP32 = 0,   /* RGBA32F / RG32F / R32F: the plain FP32 form      */
     LUXGI_STORAGE_FP16 = 1,   /* RGBA16F / RG16F / R16F: the reference's formats  */
-    LUXGI_STORAGE_UNORM8 = 2  /* RGBA8 unorm (surface-atlas colour only)          */
+    LUXGI_STORAGE_UNORM8 = 2, /* RGBA8 unorm (surface-atlas colour only)          */
+    LUXGI_STORAGE_BF16 = 3    /* R16 bfloat16 (SDF volume and mip only)           */
 } LuxgiStorage;
 
 /* ------------------------------------------------------------------------- */
@@ -143,7 +144,7 @@ typedef struct LuxgiSDFVolume {
     const void* mip;      /* R32F or R16F, [mipRes][mipRes][cascades*mipRes] */
     int32_t     resolution;
     int32_t     mipResolution;
-    int32_t     storage;  /* LuxgiStorage: FP32 or FP16 */
+    int32_t     storage;  /* LuxgiStorage: FP32, FP16 or BF16 */
     int32_t     layout;   /* LuxgiLayout of `sdf`; the mip is LINEAR, except with LUXGI_LAYOUT_CELL8, where
                              the mip is in CELL8 records as well */
     const float* guard;   /* optional [D/4][H/4][W/4] lower bounds of `sdf` per 4x4x4 brick and its
@@ -175,8 +176,7 @@ typedef enum LuxgiLayout {
                                 i + 2j + 4k = voxel (x+i, y+j, z+k) clamped to the volume (32 B FP32 or
                                 16 B FP16 per record), so a lookup is one aligned record. Records are
                                 grouped in bricks of 8x8x8 cells (x-fastest brick order, 512 records per
-                                brick) and Morton-ordered inside a brick (bit 3b+a of the index = bit b
-                                of the cell's coordinate a, a = x, y, z). Any dimensions
+                                brick, record (z%8)*64 + (y%8)*8 + x%8 inside its brick). Any dimensions
                                 (luxgi_sdf_to_cells, luxgi_sdf_cells_bytes). Lookups return what the
                                 LINEAR layout returns, bit for bit. */
 } LuxgiLayout;
@@ -241,6 +241,8 @@ typedef struct LuxgiProbeAtlas {
 #define LUXGI_UPDATE_PER_PROBE    0x8u  /* force the per-probe weight path (no shared-direction GEMM) */
 #define LUXGI_UPDATE_HALF        0x10u  /* reduced precision: weights and ray data rounded to FP16 for the
                                            FP16 MFMA contraction, FP32 accumulation and epilogue */
+#define LUXGI_UPDATE_BF16        0x20u  /* with LUXGI_UPDATE_HALF: the operands rounded to BF16 for the BF16 MFMA
+                                           (v_mfma_f32_16x16x32_bf16) instead of FP16 */
 
 /* ------------------------------------------------------------------------- */
 /* DDGI                                                                        */

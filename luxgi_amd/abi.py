@@ -17,12 +17,14 @@ LUXGI_OK = 0
 STORAGE_FP32 = 0
 STORAGE_FP16 = 1
 STORAGE_UNORM8 = 2
+STORAGE_BF16 = 3
 
 UPDATE_IRRADIANCE = 0x1
 UPDATE_DEPTH = 0x2
 UPDATE_FUSE_BORDER = 0x4
 UPDATE_PER_PROBE = 0x8
 UPDATE_HALF = 0x10
+UPDATE_BF16 = 0x20
 
 GLOBAL_SDF_WORLD_SIZE = 60000.0  # Shaders/SDF/SDFCommon.glsl:6
 ATLAS_CHUNKS_RESOLUTION = 40     # Shaders/SDF/AtlasCommon.glsl:5

@@ -109,6 +109,13 @@ template <> struct Texel<__half> {
     static __device__ __forceinline__ void st1(__half* p, size_t i, float v) { p[i] = __float2half_rn(v); }
 };
 
+// BF16 scalar voxels (SDF volumes only): the upper half of an FP32, rounded to nearest even
+template <> struct Texel<__bf16> {
+    static __device__ __forceinline__ float ld1(const __bf16* p, size_t i) { return float(p[i]); }
+    static __device__ __forceinline__ float cvt(__bf16 v) { return float(v); }
+    static __device__ __forceinline__ void st1(__bf16* p, size_t i, float v) { p[i] = (__bf16)v; }
+};
+
 // RGBA8 unorm colour (surface atlas only)
 struct Unorm8 {};
 template <> struct Texel<Unorm8> {
@@ -245,6 +252,11 @@ template <> struct VoxelPair<__half> {
     typedef half2_a2 T;
     static __device__ __forceinline__ float2 get(T p) { return make_float2(float(p.x), float(p.y)); }
 };
+typedef __bf16 bf16x2_a2 __attribute__((ext_vector_type(2), aligned(2)));
+template <> struct VoxelPair<__bf16> {
+    typedef bf16x2_a2 T;
+    static __device__ __forceinline__ float2 get(T p) { return make_float2(float(p.x), float(p.y)); }
+};
 
 __device__ __forceinline__ uint32_t brick_index(int x, int y, int z, uint32_t nbx, uint32_t nby) {
     return ((uint32_t)(z >> 2) * nby + (uint32_t)(y >> 2)) * nbx + (uint32_t)(x >> 2);
@@ -286,20 +298,13 @@ __device__ __forceinline__ float sample3d_brick4(const S* __restrict__ vol, int 
 }
 
 // Cell-record volume (LUXGI_LAYOUT_CELL8): one record of the 8 corner voxels per cell, 8x8x8-cell
-// bricks in x-fastest order, Morton order inside a brick.
-__host__ __device__ __forceinline__ uint32_t cell8_spread(uint32_t v) {  // bits 0,1,2 -> 0,3,6
-    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4);
-}
-__host__ __device__ __forceinline__ uint32_t cell8_compact(uint32_t m) {  // bits 0,3,6 -> 0,1,2
-    return (m & 1u) | ((m >> 2) & 2u) | ((m >> 4) & 4u);
-}
+// bricks in x-fastest order, cells x-fastest inside a brick.
 inline size_t cell8_records(int W, int H, int D) {
     return (size_t)((W + 7) / 8) * ((H + 7) / 8) * ((D + 7) / 8) * 512;
 }
 __device__ __forceinline__ uint32_t cell8_index(int x, int y, int z, uint32_t nbx, uint32_t nby) {
     const uint32_t b = ((uint32_t)(z >> 3) * nby + (uint32_t)(y >> 3)) * nbx + (uint32_t)(x >> 3);
-    return (b << 9) | cell8_spread((uint32_t)x & 7u) | (cell8_spread((uint32_t)y & 7u) << 1) |
-           (cell8_spread((uint32_t)z & 7u) << 2);
+    return (b << 9) | (((uint32_t)z & 7u) << 6) | (((uint32_t)y & 7u) << 3) | ((uint32_t)x & 7u);
 }
 
 template <class S> struct CellRecord;
@@ -312,6 +317,17 @@ template <> struct CellRecord<float> {  // 32 B: two 16-byte loads of one 32-byt
     }
 };
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+template <> struct CellRecord<__bf16> {  // 16 B: one load; a BF16 is the upper half of an FP32
+    static __device__ __forceinline__ void load(const __bf16* __restrict__ vol, uint32_t rec, float (&v)[8]) {
+        const uint4 q = reinterpret_cast<const uint4*>(vol)[rec];
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[2 * k] = __uint_as_float(w[k] << 16);
+            v[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+        }
+    }
+};
 template <> struct CellRecord<__half> {  // 16 B: one load
     static __device__ __forceinline__ void load(const __half* __restrict__ vol, uint32_t rec, float (&v)[8]) {
         const half8_t h = reinterpret_cast<const half8_t*>(vol)[rec];

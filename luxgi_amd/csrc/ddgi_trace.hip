@@ -30,6 +30,7 @@ struct TraceArgs {
     int res, mipRes, maxSteps, skySize, P, R, rayWidth;
     float hitEps;
     int fineLayout;  // LuxgiLayout of the fine volume
+    float invCascades;        // 1 / cascadesCount
     const float* brickScale;  // optional per-brick scale of a BRICK4 fine volume
     const float* refine;      // optional FP32 fine volume re-evaluated at hits
     int refineLayout;
@@ -169,6 +170,7 @@ struct MarchRay {
     f3 o, d, traceEnd;
     f3 cpos;             // the current cascade's centre
     float half, voxel;   // and its half extent and voxel size
+    float invExtent, invVoxel;  // 1 / (2 half) and 1 / voxel (see march_step)
     float stepTime, iy, nextStart;
     int cas, step;
 };
@@ -179,6 +181,8 @@ __device__ __forceinline__ void march_enter(const TraceArgs& A, MarchRay& r, flo
     r.cpos = mk(A.g.cascadePosDistance[cas][0], A.g.cascadePosDistance[cas][1], A.g.cascadePosDistance[cas][2]);
     r.half = A.g.cascadePosDistance[cas][3];
     r.voxel = A.g.cascadeVoxelSize[cas];
+    r.invExtent = 1.0f / (r.half * 2.0f);
+    r.invVoxel = 1.0f / r.voxel;
     const float half = r.half;
     const f3 dl = r.traceEnd - r.o;
     const f3 inv = mk(1.0f / dl.x, 1.0f / dl.y, 1.0f / dl.z);
@@ -215,13 +219,18 @@ __device__ __forceinline__ bool march_step(const TraceArgs& A, const SS* __restr
     const float chunkSizeDistance = 32.0f / A.g.resolution;
     const float chunkMarginDistance = 4.0f / A.g.resolution;
     const int texW = A.res * C, mipW = A.mipRes * C;
+    // The cascade UV (SDFCommon.glsl:64-71) divides by the cascade extent and the cascade count, and
+    // the hit thickness (:166-167) by the voxel size; those divisors are fixed per cascade, so the
+    // divisions are multiplications by their reciprocals: identical results when the divisor is a
+    // power of two (extents of 2^k, one, two or four cascades), within one ulp otherwise (GLSL's
+    // own division is within 2.5 ulp).
     const float cmax = r.half * 2.0f;
     const float stepTime = r.stepTime;
     const f3 p = r.o + r.d * stepTime;
-    const float ux = clampf((p.x - r.cpos.x) / cmax + 0.5f, 0.0f, 1.0f);
-    const float uy = clampf((p.y - r.cpos.y) / cmax + 0.5f, 0.0f, 1.0f);
-    const float uz = clampf((p.z - r.cpos.z) / cmax + 0.5f, 0.0f, 1.0f);
-    const float tx = (float(r.cas) + ux) / float(C);
+    const float ux = clampf((p.x - r.cpos.x) * r.invExtent + 0.5f, 0.0f, 1.0f);
+    const float uy = clampf((p.y - r.cpos.y) * r.invExtent + 0.5f, 0.0f, 1.0f);
+    const float uz = clampf((p.z - r.cpos.z) * r.invExtent + 0.5f, 0.0f, 1.0f);
+    const float tx = (float(r.cas) + ux) * A.invCascades;
     // The fine sample is used only where the mip distance is below the chunk size and the fine
     // distance below twice the chunk margin (SDFCommon.glsl:150-162). A brick guard >= that margin
     // proves the second test fails for every voxel of the footprint (trilinear filtering is a
@@ -248,7 +257,7 @@ __device__ __forceinline__ bool march_step(const TraceArgs& A, const SS* __restr
     }
     sd *= cmax;
     const float voxel = r.voxel;
-    const float thick = A.hitEps > 0.0f ? A.hitEps : voxel * 0.5f * clampf(stepTime / voxel, 0.0f, 1.0f);
+    const float thick = A.hitEps > 0.0f ? A.hitEps : voxel * 0.5f * clampf(stepTime * r.invVoxel, 0.0f, 1.0f);
     if (sd < thick) {
         if (A.refine) {  // FP32 re-evaluation of the hit step, same selection rule
             const float f32 = refine_lookup(A, texW, A.res, A.res, tx, uy, uz);
@@ -662,6 +671,7 @@ static int trace_scene_args(const LuxgiGlobalSDFData* g, const LuxgiSDFVolume* v
     A.skySize = sky ? sky->size : 0;
     A.hitEps = p->hitEpsilon;
     A.fineLayout = vol->layout;
+    A.invCascades = 1.0f / float(g->cascadesCount);
     A.brickScale = vol->brickScale;
     A.refine = vol->refine;
     A.refineLayout = vol->refineLayout;
@@ -694,6 +704,10 @@ static int trace_launch(const TraceArgs& A, const LuxgiSDFVolume* vol, const Lux
         return dispatch_color<__half, float, COUNT, BATCH>(A, vol, at, sky, w, counters, st);
     if (vol->storage == LUXGI_STORAGE_FP16 && rayStorage == LUXGI_STORAGE_FP16)
         return dispatch_color<__half, __half, COUNT, BATCH>(A, vol, at, sky, w, counters, st);
+    if (vol->storage == LUXGI_STORAGE_BF16 && rayStorage == LUXGI_STORAGE_FP32)
+        return dispatch_color<__bf16, float, COUNT, BATCH>(A, vol, at, sky, w, counters, st);
+    if (vol->storage == LUXGI_STORAGE_BF16 && rayStorage == LUXGI_STORAGE_FP16)
+        return dispatch_color<__bf16, __half, COUNT, BATCH>(A, vol, at, sky, w, counters, st);
     return LUXGI_ERR_UNSUPPORTED;
 }
 

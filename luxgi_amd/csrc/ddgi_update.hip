@@ -34,7 +34,7 @@ struct UpdateArgs {
     int ni, nd;
     int Wi, Wd;
     float maxDistance, sharpness, hysteresis, invGamma;
-    int firstFrame, doIrr, doDepth, fuseBorder, forcePerProbe, half;
+    int firstFrame, doIrr, doDepth, fuseBorder, forcePerProbe, half, bf16;
 };
 
 // Shared weights W[r][t] (thresholded at FLT_EPS as ProbeUpdate.glsl:93) and
@@ -337,11 +337,11 @@ __global__ void border_kernel(SA* __restrict__ atlas, int W, int n, int pxy, int
 // MFMA form. With shared ray directions the blend of a block of 16 probes is
 //   depth: C[Td x 16] = W_d[Td x R] . B_r[R x 16]  and  W_d . B_r2   (probe g, r and r^2)
 //   irr:   C[Ti x 16] = W_i[Ti x R] . B_c[R x 16]  for c = R, G, B
-// contracted on the FP16 MFMA (probe_update_mfma16_kernel below). The epilogue normalises by
-// 1/(2 S[t]), applies the gamma, stages the interiors in LDS and writes each probe's full
-// bordered tile with the hysteresis blend. A probe whose staged directions differ from ray
-// row 0 is flagged; probe_update_kernel then recomputes the flagged probes with their own
-// directions.
+// contracted on the 16x16x32 MFMA (probe_blend_map_kernel below), one map per launch. The
+// epilogue normalises by 1/(2 S[t]), applies the gamma, stages the interiors in LDS and writes
+// each probe's full bordered tile with the hysteresis blend. A probe whose staged directions
+// differ from ray row 0 is flagged; probe_update_kernel then recomputes the flagged probes with
+// their own directions.
 // ---------------------------------------------------------------------------
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr int kMG = 16;    // probes per block
@@ -363,105 +363,33 @@ __device__ __forceinline__ void border_source(int bx, int by, int n, int& sx, in
     else { sx = bx == 0 ? n : 1; sy = by == 0 ? n : 1; }
 }
 
-struct MfmaOut {
-    float2 dep[kMG][256];
-    float4 irr[kMG][64];
-};
-
-// Writes the kMG finished interiors staged in LDS as full bordered tiles (the border copy of
-// BorderUpdate.glsl when fuseBorder), blended with the previous atlas (ProbeUpdate.glsl:144-145).
-// Consecutive lanes cover consecutive texels of an atlas row (the 16 probes' tiles are adjacent),
-// and each lane keeps kWB previous-atlas loads in flight before it blends and stores.
-constexpr int kWB = 4;
-
-template <class SA, int CH>
-__device__ __forceinline__ void write_map(const UpdateArgs& A, const MfmaOut& o, int p0, int n, int W,
-                                          const SA* __restrict__ prev, SA* __restrict__ out) {
-    const int pw = n + 2, pxy = A.px * A.py;
-    const int total = kMG * pw * pw;
-    for (int i0 = threadIdx.x; i0 < total; i0 += 256 * kWB) {
-        float4 pv[kWB];
-        size_t dst[kWB];
-        int src[kWB];
-        bool ok[kWB];
-#pragma unroll
-        for (int u = 0; u < kWB; ++u) {
-            const int i = i0 + u * 256;
-            const int bx = i % pw, g = (i / pw) % kMG, by = i / (pw * kMG);
-            const int p = p0 + g;
-            const bool interior = bx >= 1 && bx <= n && by >= 1 && by <= n;
-            ok[u] = i < total && p < A.P && (interior || A.fuseBorder);
-            int sx = 0, sy = 0;
-            border_source(bx, by, n, sx, sy);
-            src[u] = g * 256 + (sy - 1) * n + (sx - 1);
-            const int ox = (p % pxy) * pw + 1, oy = (p / pxy) * pw + 1;
-            dst[u] = (size_t)(oy + by) * W + ox + bx;
-            pv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ok[u] && !A.firstFrame) {
-                const size_t ps = (size_t)(oy + sy) * W + ox + sx;
-                if (CH == 4) pv[u] = Texel<SA>::ld4(prev, ps);
-                else {
-                    const float2 t = Texel<SA>::ld2(prev, ps);
-                    pv[u] = make_float4(t.x, t.y, 0.f, 0.f);
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kWB; ++u) {
-            if (!ok[u]) continue;
-            const int g = src[u] >> 8, t = src[u] & 255;
-            if (CH == 4) {
-                float4 v = o.irr[g][t];
-                v.w = 1.0f;
-                if (!A.firstFrame) {
-                    v.x = mixf(v.x, pv[u].x, A.hysteresis);
-                    v.y = mixf(v.y, pv[u].y, A.hysteresis);
-                    v.z = mixf(v.z, pv[u].z, A.hysteresis);
-                }
-                Texel<SA>::st4(out, dst[u], v);
-            } else {
-                float2 v = o.dep[g][t];
-                if (!A.firstFrame) {
-                    v.x = mixf(v.x, pv[u].x, A.hysteresis);
-                    v.y = mixf(v.y, pv[u].y, A.hysteresis);
-                }
-                Texel<SA>::st2(out, dst[u], v);
-            }
-        }
-    }
-}
-
-template <class SA>
-__device__ __forceinline__ void write_tiles(const UpdateArgs& A, const MfmaOut& o, int p0, const SA* __restrict__ prevIrr,
-                                            const SA* __restrict__ prevDepth, SA* __restrict__ outIrr,
-                                            SA* __restrict__ outDepth) {
-    if (A.doIrr) write_map<SA, 4>(A, o, p0, A.ni, A.Wi, prevIrr, outIrr);
-    if (A.doDepth) write_map<SA, 2>(A, o, p0, A.nd, A.Wd, prevDepth, outDepth);
-}
-
 // ---------------------------------------------------------------------------
-// Blend on the FP16 MFMA v_mfma_f32_16x16x32_f16 with FP32 accumulation, 16 probes per block,
-// 32 rays per k-step. Operand lanes: A[m][k] = W at texel m, ray k -- lane l holds rays
-// 8(l/16) .. 8(l/16)+7 of texel row l%16, one 16-byte load from the texel-major FP16 weights;
+// Blend on the 16x16x32 MFMA with FP32 accumulation, 16 probes per block, 32 rays per k-step,
+// operands in FP16 (v_mfma_f32_16x16x32_f16) or, in the reduced-precision form on request, BF16
+// (v_mfma_f32_16x16x32_bf16). Operand lanes: A[m][k] = W at texel m, ray k -- lane l holds rays
+// 8(l/16) .. 8(l/16)+7 of texel row l%16, one 16-byte load from the texel-major weights;
 // B[k][n] = ray data of probe n -- lane l holds the same 8 rays of probe l%16, one 16-byte LDS
-// read; D[4(l/16)+v][l%16] as for the FP32 MFMA. Depth moments enter as r/maxDistance and its
-// square (both in [0, 1]).
+// read; D[4(l/16)+v][l%16]. Depth moments enter as r/maxDistance and its square (both in [0, 1]).
 //   SPLIT (the plain form): every operand x is carried as hi = fp16(x), lo = fp16(x - hi) and
 //     each product as hi*hi + hi*lo + lo*hi: about 22 significant bits per operand, the dropped
 //     lo*lo term and the FP16 subnormal floor sit below the FP32 rounding of the normalised sums
 //     because all operands are O(1). Normalised by the sum of hi + lo.
-//   !SPLIT (LUXGI_UPDATE_HALF): one FP16 rounding per operand, normalised by the sum of the
-//     rounded weights, so a constant input stays constant.
+//   !SPLIT (LUXGI_UPDATE_HALF): one FP16 (or BF16) rounding per operand, normalised by the sum
+//     of the rounded weights, so a constant input stays constant.
 // The ray data of each k-step is staged in LDS (double-buffered) by all 256 lanes; a probe whose
 // directions differ from ray row 0 is flagged for the per-probe fix-up pass.
 // ---------------------------------------------------------------------------
 typedef _Float16 half8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+template <class T> struct Lp8;
+template <> struct Lp8<_Float16> { typedef half8v V; };
+template <> struct Lp8<__bf16> { typedef bf16x8v V; };
 constexpr int kHK = 32;  // rays per k-step
-constexpr int kHCh = 5;  // r, r^2, R, G, B
 
-__global__ __launch_bounds__(256) void weights_half_kernel(const float* __restrict__ dirs0, int R, int Rpad, int n,
+template <class T>
+__global__ __launch_bounds__(256) void weights_lowp_kernel(const float* __restrict__ dirs0, int R, int Rpad, int n,
                                                            int tpad, int depth, float sharpness, int split,
-                                                           _Float16* __restrict__ Whi, _Float16* __restrict__ Wlo,
+                                                           T* __restrict__ Whi, T* __restrict__ Wlo,
                                                            float* __restrict__ Sh) {
     __shared__ float part[16][17];
     const int tl = threadIdx.x & 15, j = threadIdx.x >> 4;
@@ -476,10 +404,10 @@ __global__ __launch_bounds__(256) void weights_half_kernel(const float* __restri
             if (depth) w = powf(w, sharpness);
             if (!(w >= kFltEps)) w = 0.0f;
         }
-        const _Float16 h = (_Float16)w;
+        const T h = (T)w;
         Whi[(size_t)t * Rpad + r] = h;
         if (split) {
-            const _Float16 l = (_Float16)(w - (float)h);
+            const T l = (T)(w - (float)h);
             Wlo[(size_t)t * Rpad + r] = l;
             s += (float)h + (float)l;
         } else {
@@ -495,168 +423,259 @@ __global__ __launch_bounds__(256) void weights_half_kernel(const float* __restri
     }
 }
 
-struct HalfStage {
-    _Float16 b[2][kHCh][kMG][kHK];  // [hi / lo][channel][probe][ray]
+__device__ __forceinline__ floatx4 mfma32(half8v a, half8v b, floatx4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc, 0, 0, 0);
+}
+__device__ __forceinline__ floatx4 mfma32(bf16x8v a, bf16x8v b, floatx4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+}
+template <bool SPLIT, class V>
+__device__ __forceinline__ floatx4 mfma_lp(V ah, V al, V bh, V bl, floatx4 acc) {
+    acc = mfma32(ah, bh, acc);
+    if (SPLIT) {
+        acc = mfma32(ah, bl, acc);
+        acc = mfma32(al, bh, acc);
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------------------
+// The same contraction with one map per launch (MAP 0: depth moments r, r^2; MAP 1: irradiance
+// R, G, B), as the reference dispatches DepthProbeUpdate.comp and IrradianceProbeUpdate.comp
+// separately (DDGIRenderer.cpp:347-375). Each launch holds only its own map's accumulators and
+// stages only its own output tile in LDS (depth 32 KB, irradiance 16 KB), so four 256-lane blocks
+// fit a CU and the 1,024 blocks of a 16,384-probe volume are resident at once. The depth launch
+// reads only the direction/distance rays and checks the directions against row 0; the irradiance
+// launch reads only the radiance rays (and the directions too when the depth map is not
+// updated in the same call, CHECK).
+// ---------------------------------------------------------------------------
+constexpr int kWB = 4;  // previous-atlas loads each lane keeps in flight in the tile writer
+
+template <int CH, class T>
+struct MapStage {
+    T b[2][CH][kMG][kHK];  // [hi / lo][channel][probe][ray]
 };
-union HalfLds {
-    HalfStage st[2];
-    MfmaOut out;
+template <int MAP>
+struct MapOut;
+template <>
+struct MapOut<0> {
+    float2 v[kMG][256];
+    __device__ __forceinline__ float4 get(int g, int t) const { return make_float4(v[g][t].x, v[g][t].y, 0.f, 0.f); }
+};
+template <>
+struct MapOut<1> {
+    float4 v[kMG][64];
+    __device__ __forceinline__ float4 get(int g, int t) const { return v[g][t]; }
+};
+template <int MAP, class T>
+union MapLds {
+    MapStage<MAP == 0 ? 2 : 3, T> st[2];
+    MapOut<MAP> out;
 };
 
-// ray data of one k-step: thread t stages rays 2(t%16), 2(t%16)+1 of probe t/16
-struct HalfPrefetch {
-    float4 dd[2], rc[2];
-};
-
-template <class SR>
-__device__ __forceinline__ void half_load(const UpdateArgs& A, int k0, int p0, const SR* __restrict__ radiance,
-                                          const SR* __restrict__ dirDist, HalfPrefetch& f) {
-    const int g = threadIdx.x >> 4, rp = (threadIdx.x & 15) * 2, p = p0 + g;
+// Bordered tiles of the kMG staged interiors of one map, blended with the previous atlas
+// (ProbeUpdate.glsl:144-145) and, when fuseBorder, with the border copy of BorderUpdate.glsl.
+// Consecutive lanes cover consecutive texels of an atlas row (the 16 probes' tiles are adjacent),
+// and each lane keeps kWB previous-atlas loads in flight before it blends and stores.
+template <class SA, int CH, class Staged>
+__device__ __forceinline__ void write_staged(const UpdateArgs& A, const Staged& o, int p0, int n, int W,
+                                             const SA* __restrict__ prev, SA* __restrict__ out) {
+    const int pw = n + 2, pxy = A.px * A.py;
+    const int total = kMG * pw * pw;
+    const int tstride = CH == 4 ? 64 : 256;
+    for (int i0 = threadIdx.x; i0 < total; i0 += 256 * kWB) {
+        float4 pv[kWB];
+        size_t dst[kWB];
+        int src[kWB];
+        bool ok[kWB];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int r = k0 + rp + q;
-        f.dd[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-        f.rc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (p < A.P && r < A.R) {
-            const size_t e = (size_t)p * A.rayWidth + r;
-            f.dd[q] = Texel<SR>::ld4(dirDist, e);
-            if (A.doIrr) f.rc[q] = Texel<SR>::ld4(radiance, e);
+        for (int u = 0; u < kWB; ++u) {
+            const int i = i0 + u * 256;
+            const int bx = i % pw, g = (i / pw) % kMG, by = i / (pw * kMG);
+            const int p = p0 + g;
+            const bool interior = bx >= 1 && bx <= n && by >= 1 && by <= n;
+            ok[u] = i < total && p < A.P && (interior || A.fuseBorder);
+            int sx = 0, sy = 0;
+            border_source(bx, by, n, sx, sy);
+            src[u] = g * tstride + (sy - 1) * n + (sx - 1);
+            const int ox = (p % pxy) * pw + 1, oy = (p / pxy) * pw + 1;
+            dst[u] = (size_t)(oy + by) * W + ox + bx;
+            pv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (ok[u] && !A.firstFrame) {
+                const size_t ps = (size_t)(oy + sy) * W + ox + sx;
+                if (CH == 4) pv[u] = Texel<SA>::ld4(prev, ps);
+                else {
+                    const float2 t = Texel<SA>::ld2(prev, ps);
+                    pv[u] = make_float4(t.x, t.y, 0.f, 0.f);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kWB; ++u) {
+            if (!ok[u]) continue;
+            const int g = src[u] / tstride, t = src[u] % tstride;
+            float4 v = o.get(g, t);
+            if (CH == 4) {
+                v.w = 1.0f;
+                if (!A.firstFrame) {
+                    v.x = mixf(v.x, pv[u].x, A.hysteresis);
+                    v.y = mixf(v.y, pv[u].y, A.hysteresis);
+                    v.z = mixf(v.z, pv[u].z, A.hysteresis);
+                }
+                Texel<SA>::st4(out, dst[u], v);
+            } else {
+                if (!A.firstFrame) {
+                    v.x = mixf(v.x, pv[u].x, A.hysteresis);
+                    v.y = mixf(v.y, pv[u].y, A.hysteresis);
+                }
+                Texel<SA>::st2(out, dst[u], make_float2(v.x, v.y));
+            }
         }
     }
 }
 
-template <bool SPLIT>
-__device__ __forceinline__ void half_store(const UpdateArgs& A, int k0, int p0, const float* __restrict__ dirs0,
-                                           const HalfPrefetch& f, HalfStage& st, int* sMis) {
+// one k-step's ray data of this map: thread t stages rays 2(t%16), 2(t%16)+1 of probe t/16
+struct MapPrefetch {
+    float4 a[2];  // dd (depth) or radiance (irradiance)
+    float4 d[2];  // dd, irradiance with CHECK only
+};
+
+template <int MAP, bool CHECK, class SR>
+__device__ __forceinline__ void map_load(const UpdateArgs& A, int k0, int p0, const SR* __restrict__ radiance,
+                                         const SR* __restrict__ dirDist, MapPrefetch& f) {
+    const int g = threadIdx.x >> 4, rp = (threadIdx.x & 15) * 2, p = p0 + g;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int r = k0 + rp + q;
+        f.a[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (CHECK) f.d[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (p < A.P && r < A.R) {
+            const size_t e = (size_t)p * A.rayWidth + r;
+            f.a[q] = Texel<SR>::ld4(MAP == 0 ? dirDist : radiance, e);
+            if (CHECK) f.d[q] = Texel<SR>::ld4(dirDist, e);
+        }
+    }
+}
+
+template <int MAP, bool CHECK, bool SPLIT, class T>
+__device__ __forceinline__ void map_store(const UpdateArgs& A, int k0, int p0, const float* __restrict__ dirs0,
+                                          const MapPrefetch& f, MapStage<MAP == 0 ? 2 : 3, T>& st, int* sMis) {
+    constexpr int CH = MAP == 0 ? 2 : 3;
     const int g = threadIdx.x >> 4, rp = (threadIdx.x & 15) * 2, p = p0 + g;
     const float invMax = 1.0f / A.maxDistance;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int r = k0 + rp + q;
-        float v[kHCh] = {0.f, 0.f, f.rc[q].x, f.rc[q].y, f.rc[q].z};
+        float v[CH];
+        if (MAP == 0) {
+            v[0] = 0.0f;
+            v[1] = 0.0f;
+        } else {
+            v[0] = f.a[q].x;
+            v[1] = f.a[q].y;
+            v[CH - 1] = f.a[q].z;
+        }
         if (p < A.P && r < A.R) {
-            float dist = fminf(A.maxDistance, f.dd[q].w - 0.01f);  // ProbeUpdate.glsl:75-78
-            if (dist == -1.0f) dist = A.maxDistance;
-            v[0] = dist * invMax;
-            v[1] = v[0] * v[0];
-            if (f.dd[q].x != dirs0[r * 4 + 0] || f.dd[q].y != dirs0[r * 4 + 1] || f.dd[q].z != dirs0[r * 4 + 2])
-                sMis[g] = 1;
+            if (MAP == 0) {
+                float dist = fminf(A.maxDistance, f.a[q].w - 0.01f);  // ProbeUpdate.glsl:75-78
+                if (dist == -1.0f) dist = A.maxDistance;
+                v[0] = dist * invMax;
+                v[1] = v[0] * v[0];
+            }
+            if (MAP == 0 || CHECK) {
+                const float4 d = MAP == 0 ? f.a[q] : f.d[q];
+                if (d.x != dirs0[r * 4 + 0] || d.y != dirs0[r * 4 + 1] || d.z != dirs0[r * 4 + 2]) sMis[g] = 1;
+            }
         }
 #pragma unroll
-        for (int c = 0; c < kHCh; ++c) {
-            const _Float16 h = (_Float16)v[c];
+        for (int c = 0; c < CH; ++c) {
+            const T h = (T)v[c];
             st.b[0][c][g][rp + q] = h;
-            if (SPLIT) st.b[1][c][g][rp + q] = (_Float16)(v[c] - (float)h);
+            if (SPLIT) st.b[1][c][g][rp + q] = (T)(v[c] - (float)h);
         }
     }
 }
 
-template <bool SPLIT>
-__device__ __forceinline__ floatx4 mfma_h(half8v ah, half8v al, half8v bh, half8v bl, floatx4 acc) {
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc, 0, 0, 0);
-    if (SPLIT) {
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc, 0, 0, 0);
-    }
-    return acc;
-}
-
-template <class SR, class SA, bool SPLIT>
-__global__ __launch_bounds__(256) void probe_update_mfma16_kernel(
-    UpdateArgs A, const SR* __restrict__ radiance, const SR* __restrict__ dirDist, const SA* __restrict__ prevIrr,
-    const SA* __restrict__ prevDepth, SA* __restrict__ outIrr, SA* __restrict__ outDepth, const float* __restrict__ dirs0,
-    const _Float16* __restrict__ Wih, const _Float16* __restrict__ Wil, const float* __restrict__ Sih,
-    const _Float16* __restrict__ Wdh, const _Float16* __restrict__ Wdl, const float* __restrict__ Sdh,
-    uint8_t* __restrict__ mismatch, int tpadI, int tpadD, int Rpad) {
-    __shared__ HalfLds lds;
+template <class SR, class SA, class T, bool SPLIT, int MAP, bool CHECK>
+__global__ __launch_bounds__(256, 4) void probe_blend_map_kernel(
+    UpdateArgs A, const SR* __restrict__ radiance, const SR* __restrict__ dirDist, const SA* __restrict__ prev,
+    SA* __restrict__ out, const float* __restrict__ dirs0, const T* __restrict__ Wh, const T* __restrict__ Wl,
+    const float* __restrict__ Sh, uint8_t* __restrict__ mismatch, int tpad, int Rpad) {
+    typedef typename Lp8<T>::V V;
+    constexpr int CH = MAP == 0 ? 2 : 3;
+    constexpr int NJ = MAP == 0 ? 4 : 1;  // m-tiles per wave: depth up to 16 tiles, irradiance up to 4
+    __shared__ MapLds<MAP, T> lds;
     __shared__ int sMis[kMG];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int col = lane & 15, kq = lane >> 4;
     const int p0 = blockIdx.x * kMG;
-    const int MtD = A.doDepth ? tpadD / 16 : 0, MtI = A.doIrr ? tpadI / 16 : 0;
+    const int Mt = tpad / 16;
     if (tid < kMG) sMis[tid] = 0;
-    floatx4 accD[4][2], accI[3];
+    floatx4 acc[NJ][CH];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) accD[j][0] = accD[j][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < NJ; ++j)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) accI[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < CH; ++c) acc[j][c] = floatx4{0.f, 0.f, 0.f, 0.f};
     const int nSteps = Rpad / kHK;
-    HalfPrefetch f;
-    half_load<SR>(A, 0, p0, radiance, dirDist, f);
+    MapPrefetch f;
+    map_load<MAP, CHECK, SR>(A, 0, p0, radiance, dirDist, f);
     __syncthreads();  // sMis initialised
-    half_store<SPLIT>(A, 0, p0, dirs0, f, lds.st[0], sMis);
+    map_store<MAP, CHECK, SPLIT, T>(A, 0, p0, dirs0, f, lds.st[0], sMis);
     __syncthreads();
     for (int s = 0; s < nSteps; ++s) {
         const int k0 = s * kHK;
         const bool more = s + 1 < nSteps;
-        if (more) half_load<SR>(A, k0 + kHK, p0, radiance, dirDist, f);
-        const HalfStage& st = lds.st[s & 1];
+        if (more) map_load<MAP, CHECK, SR>(A, k0 + kHK, p0, radiance, dirDist, f);
+        const MapStage<CH, T>& st = lds.st[s & 1];
         const int ko = k0 + 8 * kq;
-        if (MtD) {
-            const half8v bh0 = *reinterpret_cast<const half8v*>(&st.b[0][0][col][8 * kq]);
-            const half8v bh1 = *reinterpret_cast<const half8v*>(&st.b[0][1][col][8 * kq]);
-            half8v bl0 = bh0, bl1 = bh1;
-            if (SPLIT) {
-                bl0 = *reinterpret_cast<const half8v*>(&st.b[1][0][col][8 * kq]);
-                bl1 = *reinterpret_cast<const half8v*>(&st.b[1][1][col][8 * kq]);
-            }
+        V bh[CH], bl[CH];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int m = wave + 4 * j;
-                if (m < MtD) {
-                    const size_t off = (size_t)(m * 16 + col) * Rpad + ko;
-                    const half8v ah = *reinterpret_cast<const half8v*>(Wdh + off);
-                    const half8v al = SPLIT ? *reinterpret_cast<const half8v*>(Wdl + off) : ah;
-                    accD[j][0] = mfma_h<SPLIT>(ah, al, bh0, bl0, accD[j][0]);
-                    accD[j][1] = mfma_h<SPLIT>(ah, al, bh1, bl1, accD[j][1]);
-                }
+        for (int c = 0; c < CH; ++c) {
+            bh[c] = *reinterpret_cast<const V*>(&st.b[0][c][col][8 * kq]);
+            bl[c] = SPLIT ? *reinterpret_cast<const V*>(&st.b[1][c][col][8 * kq]) : bh[c];
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int m = wave + 4 * j;
+            if (m < Mt) {
+                const size_t off = (size_t)(m * 16 + col) * Rpad + ko;
+                const V ah = *reinterpret_cast<const V*>(Wh + off);
+                const V al = SPLIT ? *reinterpret_cast<const V*>(Wl + off) : ah;
+#pragma unroll
+                for (int c = 0; c < CH; ++c) acc[j][c] = mfma_lp<SPLIT>(ah, al, bh[c], bl[c], acc[j][c]);
             }
         }
-        if (wave < MtI) {
-            const size_t off = (size_t)(wave * 16 + col) * Rpad + ko;
-            const half8v ah = *reinterpret_cast<const half8v*>(Wih + off);
-            const half8v al = SPLIT ? *reinterpret_cast<const half8v*>(Wil + off) : ah;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const half8v bh = *reinterpret_cast<const half8v*>(&st.b[0][2 + c][col][8 * kq]);
-                const half8v bl = SPLIT ? *reinterpret_cast<const half8v*>(&st.b[1][2 + c][col][8 * kq]) : bh;
-                accI[c] = mfma_h<SPLIT>(ah, al, bh, bl, accI[c]);
-            }
-        }
-        if (more) half_store<SPLIT>(A, k0 + kHK, p0, dirs0, f, lds.st[(s + 1) & 1], sMis);
+        if (more) map_store<MAP, CHECK, SPLIT, T>(A, k0 + kHK, p0, dirs0, f, lds.st[(s + 1) & 1], sMis);
         __syncthreads();
     }
-    // interiors: normalise by 1/(2 S), undo the depth scaling, gamma-encode (ProbeUpdate.glsl:133-141)
-    const int Td = A.nd * A.nd, Ti = A.ni * A.ni;
+    // interiors: normalise by 1/(2 S), undo the depth scaling or gamma-encode (ProbeUpdate.glsl:133-141)
+    const int n = MAP == 0 ? A.nd : A.ni;
+    const int nT = n * n;
     const int rowq = kq * 4;
-    const float mScale[2] = {A.maxDistance, A.maxDistance * A.maxDistance};
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NJ; ++j) {
         const int m = wave + 4 * j;
-        if (m >= MtD) continue;
+        if (m >= Mt) continue;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int t = m * 16 + rowq + v;
-            if (t >= Td) continue;
-            const float sm = Sdh[t];
+            if (t >= nT) continue;
+            const float sm = Sh[t];
             const float sc = sm > kFltEps ? 1.0f / (2.0f * sm) : 1.0f;
-            lds.out.dep[col][t] = make_float2(accD[j][0][v] * sc * mScale[0], accD[j][1][v] * sc * mScale[1]);
-        }
-    }
-    if (wave < MtI) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int t = wave * 16 + rowq + v;
-            if (t >= Ti) continue;
-            const float sm = Sih[t];
-            const float sc = sm > kFltEps ? 1.0f / (2.0f * sm) : 1.0f;
-            lds.out.irr[col][t] = make_float4(powf(accI[0][v] * sc, A.invGamma), powf(accI[1][v] * sc, A.invGamma),
-                                              powf(accI[2][v] * sc, A.invGamma), 1.0f);
+            if constexpr (MAP == 0) {
+                lds.out.v[col][t] = make_float2(acc[j][0][v] * sc * A.maxDistance,
+                                                acc[j][1][v] * sc * (A.maxDistance * A.maxDistance));
+            } else {
+                lds.out.v[col][t] = make_float4(powf(acc[j][0][v] * sc, A.invGamma),
+                                                powf(acc[j][1][v] * sc, A.invGamma),
+                                                powf(acc[j][CH - 1][v] * sc, A.invGamma), 1.0f);
+            }
         }
     }
     __syncthreads();
-    if (tid < kMG && p0 + tid < A.P) mismatch[p0 + tid] = (uint8_t)sMis[tid];
-    write_tiles<SA>(A, lds.out, p0, prevIrr, prevDepth, outIrr, outDepth);
+    if ((MAP == 0 || CHECK) && tid < kMG && p0 + tid < A.P) mismatch[p0 + tid] = (uint8_t)sMis[tid];
+    if constexpr (MAP == 0) write_staged<SA, 2>(A, lds.out, p0, A.nd, A.Wd, prev, out);
+    else write_staged<SA, 4>(A, lds.out, p0, A.ni, A.Wi, prev, out);
 }
 
 static int check_uniform(const LuxgiDDGIUniform* u) {
@@ -747,6 +766,43 @@ extern "C" size_t luxgi_ddgi_probe_update_workspace_size(const LuxgiDDGIUniform*
     return update_workspace_layout(u, nullptr, nullptr);
 }
 
+// weights of both maps and the per-map MFMA blends with operand type T (FP16 or BF16)
+template <class SR, class SA, class T, bool SPLIT>
+static int launch_maps(const UpdateArgs& A, const SR* rad, const SR* dd, const SA* pI, const SA* pD, SA* oI, SA* oD,
+                       const UpdateWorkspace& w, int tpI, int tpD, int Rp, hipStream_t st) {
+    T* Wih = reinterpret_cast<T*>(w.Wih);
+    T* Wil = reinterpret_cast<T*>(w.Wil);
+    T* Wdh = reinterpret_cast<T*>(w.Wdh);
+    T* Wdl = reinterpret_cast<T*>(w.Wdl);
+    if (A.doIrr) {
+        hipLaunchKernelGGL(weights_lowp_kernel<T>, dim3(tpI / 16), dim3(256), 0, st, w.dirs0, A.R, Rp, A.ni, tpI, 0,
+                           A.sharpness, SPLIT ? 1 : 0, Wih, Wil, w.Sih);
+        LUXGI_LAUNCH_CHECK();
+    }
+    if (A.doDepth) {
+        hipLaunchKernelGGL(weights_lowp_kernel<T>, dim3(tpD / 16), dim3(256), 0, st, w.dirs0, A.R, Rp, A.nd, tpD, 1,
+                           A.sharpness, SPLIT ? 1 : 0, Wdh, Wdl, w.Sdh);
+        LUXGI_LAUNCH_CHECK();
+    }
+    const dim3 grid((A.P + kMG - 1) / kMG);
+    if (A.doDepth) {
+        hipLaunchKernelGGL((probe_blend_map_kernel<SR, SA, T, SPLIT, 0, false>), grid, dim3(256), 0, st, A, rad, dd, pD,
+                           oD, w.dirs0, Wdh, Wdl, w.Sdh, w.mismatch, tpD, Rp);
+        LUXGI_LAUNCH_CHECK();
+    }
+    if (A.doIrr) {
+        // the directions were checked by the depth launch; alone, this launch checks them
+        if (A.doDepth)
+            hipLaunchKernelGGL((probe_blend_map_kernel<SR, SA, T, SPLIT, 1, false>), grid, dim3(256), 0, st, A, rad, dd,
+                               pI, oI, w.dirs0, Wih, Wil, w.Sih, w.mismatch, tpI, Rp);
+        else
+            hipLaunchKernelGGL((probe_blend_map_kernel<SR, SA, T, SPLIT, 1, true>), grid, dim3(256), 0, st, A, rad, dd,
+                               pI, oI, w.dirs0, Wih, Wil, w.Sih, w.mismatch, tpI, Rp);
+        LUXGI_LAUNCH_CHECK();
+    }
+    return LUXGI_OK;
+}
+
 template <class SR, class SA>
 static int launch_update(const UpdateArgs& A, const LuxgiRayBuffers* rays, const LuxgiProbeAtlas* prev,
                          const LuxgiProbeAtlas* out, const UpdateWorkspace& w, hipStream_t st) {
@@ -762,26 +818,11 @@ static int launch_update(const UpdateArgs& A, const LuxgiRayBuffers* rays, const
         hipLaunchKernelGGL(row0_kernel<SR>, dim3((A.R + 255) / 256), dim3(256), 0, st, dd, A.R, w.dirs0);
         LUXGI_LAUNCH_CHECK();
         const int Rp = pad32(A.R);
-        const int split = A.half ? 0 : 1;
-        if (A.doIrr) {
-            hipLaunchKernelGGL(weights_half_kernel, dim3(tpI / 16), dim3(256), 0, st, w.dirs0, A.R, Rp, A.ni, tpI, 0,
-                               A.sharpness, split, w.Wih, w.Wil, w.Sih);
-            LUXGI_LAUNCH_CHECK();
-        }
-        if (A.doDepth) {
-            hipLaunchKernelGGL(weights_half_kernel, dim3(tpD / 16), dim3(256), 0, st, w.dirs0, A.R, Rp, A.nd, tpD, 1,
-                               A.sharpness, split, w.Wdh, w.Wdl, w.Sdh);
-            LUXGI_LAUNCH_CHECK();
-        }
-        if (split)
-            hipLaunchKernelGGL((probe_update_mfma16_kernel<SR, SA, true>), dim3((A.P + kMG - 1) / kMG), dim3(256), 0, st,
-                               A, rad, dd, pI, pD, oI, oD, w.dirs0, w.Wih, w.Wil, w.Sih, w.Wdh, w.Wdl, w.Sdh, w.mismatch,
-                               tpI, tpD, Rp);
-        else
-            hipLaunchKernelGGL((probe_update_mfma16_kernel<SR, SA, false>), dim3((A.P + kMG - 1) / kMG), dim3(256), 0,
-                               st, A, rad, dd, pI, pD, oI, oD, w.dirs0, w.Wih, w.Wil, w.Sih, w.Wdh, w.Wdl, w.Sdh,
-                               w.mismatch, tpI, tpD, Rp);
-        LUXGI_LAUNCH_CHECK();
+        int rc;
+        if (A.bf16) rc = launch_maps<SR, SA, __bf16, false>(A, rad, dd, pI, pD, oI, oD, w, tpI, tpD, Rp, st);
+        else if (A.half) rc = launch_maps<SR, SA, _Float16, false>(A, rad, dd, pI, pD, oI, oD, w, tpI, tpD, Rp, st);
+        else rc = launch_maps<SR, SA, _Float16, true>(A, rad, dd, pI, pD, oI, oD, w, tpI, tpD, Rp, st);
+        if (rc != LUXGI_OK) return rc;
         // probes whose directions differ from row 0: reference arithmetic with their own directions
         UpdateArgs F = A;
         F.forcePerProbe = 1;
@@ -878,7 +919,9 @@ extern "C" int luxgi_ddgi_probe_update(const LuxgiDDGIUniform* u, const LuxgiRay
     A.fuseBorder = (flags & LUXGI_UPDATE_FUSE_BORDER) != 0;
     A.forcePerProbe = (flags & LUXGI_UPDATE_PER_PROBE) != 0;
     A.half = (flags & LUXGI_UPDATE_HALF) != 0;
+    A.bf16 = (flags & LUXGI_UPDATE_BF16) != 0;
     if (A.half && !(u->maxDistance > 0.0f)) return LUXGI_ERR_INVALID_ARGUMENT;
+    if (A.bf16 && !A.half) return LUXGI_ERR_INVALID_ARGUMENT;  // BF16 operands are a reduced-precision form
     if (A.R <= 0 || A.R > rays->width || rays->height < A.P) return LUXGI_ERR_INVALID_ARGUMENT;
     if (A.ni > kMaxSide || A.nd > kMaxSide) return LUXGI_ERR_UNSUPPORTED;
     if (!A.doIrr && !A.doDepth) return LUXGI_OK;

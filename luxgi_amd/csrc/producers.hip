@@ -226,6 +226,7 @@ extern "C" int luxgi_sdf_shadow_mask(const LuxgiSdfShadowParams* params, const L
     if (width <= 0 || height <= 0 || (long long)width * height >= (1ll << 31)) return LUXGI_ERR_INVALID_ARGUMENT;
     const int st = sdf_volume_status(sdfData, sdf);
     if (st != LUXGI_OK) return st;
+    if (sdf->storage == LUXGI_STORAGE_BF16) return LUXGI_ERR_UNSUPPORTED;  // BF16: the probe-ray trace only
     ShadowArgs A;
     A.p = *params;
     A.g = *sdfData;
@@ -315,6 +316,7 @@ extern "C" int luxgi_sdf_reflection(const LuxgiSdfReflectionParams* params, cons
     if (outStorage != LUXGI_STORAGE_FP32 && outStorage != LUXGI_STORAGE_FP16) return LUXGI_ERR_UNSUPPORTED;
     const int st = sdf_volume_status(sdfData, sdf);
     if (st != LUXGI_OK) return st;
+    if (sdf->storage == LUXGI_STORAGE_BF16) return LUXGI_ERR_UNSUPPORTED;  // BF16: the probe-ray trace only
     if (!at->chunks || !at->culledObjects || !at->objects || !at->tiles || !at->color || at->objectCapacity < 0)
         return LUXGI_ERR_INVALID_ARGUMENT;
     if (at->layout == LUXGI_LAYOUT_LINEAR) {

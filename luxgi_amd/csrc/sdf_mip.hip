@@ -71,7 +71,7 @@ __global__ void to_bricks_kernel(const float* __restrict__ src, int W, int H, in
 }
 
 // One thread per record of LUXGI_LAYOUT_CELL8: record r of brick b holds the 8 corner voxels of cell
-// (x, y, z) = 8 * brick + the Morton-decoded r, clamped to the volume; padding records are zero.
+// (x, y, z) = 8 * brick + (r % 8, r / 8 % 8, r / 64 % 8), clamped to the volume; padding records are zero.
 template <class D_>
 __global__ void to_cells_kernel(const float* __restrict__ src, int W, int H, int D, uint32_t nbx, uint32_t nby,
                                 uint32_t records, D_* __restrict__ dst) {
@@ -79,8 +79,7 @@ __global__ void to_cells_kernel(const float* __restrict__ src, int W, int H, int
     if (r >= records) return;
     const uint32_t b = r >> 9, m = r & 511u;
     const uint32_t bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
-    const int x = int(8 * bx + cell8_compact(m)), y = int(8 * by + cell8_compact(m >> 1)),
-              z = int(8 * bz + cell8_compact(m >> 2));
+    const int x = int(8 * bx + (m & 7u)), y = int(8 * by + ((m >> 3) & 7u)), z = int(8 * bz + (m >> 6));
     float v[8];
     const bool inside = x < W && y < H && z < D;
 #pragma unroll
@@ -118,8 +117,8 @@ extern "C" int luxgi_sdf_build_guard(const float* sdf, int32_t W, int32_t H, int
 
 extern "C" size_t luxgi_sdf_bricks_bytes(int32_t W, int32_t H, int32_t D, int32_t storage) {
     if (W <= 0 || H <= 0 || D <= 0 || (W & 3) || (H & 3) || (D & 3)) return 0;
-    if (storage != LUXGI_STORAGE_FP32 && storage != LUXGI_STORAGE_FP16) return 0;
-    const size_t es = storage == LUXGI_STORAGE_FP16 ? 2 : 4;
+    if (storage != LUXGI_STORAGE_FP32 && storage != LUXGI_STORAGE_FP16 && storage != LUXGI_STORAGE_BF16) return 0;
+    const size_t es = storage == LUXGI_STORAGE_FP32 ? 4 : 2;
     return (size_t)(W >> 2) * (H >> 2) * (D >> 2) * kBrickElems * es;
 }
 
@@ -132,19 +131,22 @@ extern "C" int luxgi_sdf_to_bricks(const float* src, int32_t W, int32_t H, int32
     if (storage == LUXGI_STORAGE_FP32)
         hipLaunchKernelGGL(to_bricks_kernel<float>, grid, dim3(kBrickElems), 0, as_stream(stream), src, W, H, D,
                            static_cast<float*>(dst), brickScale);
-    else
+    else if (storage == LUXGI_STORAGE_FP16)
         hipLaunchKernelGGL(to_bricks_kernel<__half>, grid, dim3(kBrickElems), 0, as_stream(stream), src, W, H, D,
                            static_cast<__half*>(dst), brickScale);
+    else
+        hipLaunchKernelGGL(to_bricks_kernel<__bf16>, grid, dim3(kBrickElems), 0, as_stream(stream), src, W, H, D,
+                           static_cast<__bf16*>(dst), brickScale);
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }
 
 extern "C" size_t luxgi_sdf_cells_bytes(int32_t W, int32_t H, int32_t D, int32_t storage) {
     if (W <= 0 || H <= 0 || D <= 0) return 0;
-    if (storage != LUXGI_STORAGE_FP32 && storage != LUXGI_STORAGE_FP16) return 0;
+    if (storage != LUXGI_STORAGE_FP32 && storage != LUXGI_STORAGE_FP16 && storage != LUXGI_STORAGE_BF16) return 0;
     const size_t records = cell8_records(W, H, D);
     if (records >= (1ull << 32)) return 0;
-    return records * 8 * (storage == LUXGI_STORAGE_FP16 ? 2 : 4);
+    return records * 8 * (storage == LUXGI_STORAGE_FP32 ? 4 : 2);
 }
 
 extern "C" int luxgi_sdf_to_cells(const float* src, int32_t W, int32_t H, int32_t D, int32_t storage, void* dst,
@@ -157,9 +159,12 @@ extern "C" int luxgi_sdf_to_cells(const float* src, int32_t W, int32_t H, int32_
     if (storage == LUXGI_STORAGE_FP32)
         hipLaunchKernelGGL(to_cells_kernel<float>, grid, dim3(256), 0, as_stream(stream), src, W, H, D, nbx, nby, records,
                            static_cast<float*>(dst));
-    else
+    else if (storage == LUXGI_STORAGE_FP16)
         hipLaunchKernelGGL(to_cells_kernel<__half>, grid, dim3(256), 0, as_stream(stream), src, W, H, D, nbx, nby,
                            records, static_cast<__half*>(dst));
+    else
+        hipLaunchKernelGGL(to_cells_kernel<__bf16>, grid, dim3(256), 0, as_stream(stream), src, W, H, D, nbx, nby,
+                           records, static_cast<__bf16*>(dst));
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }

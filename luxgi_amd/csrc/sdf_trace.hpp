@@ -68,12 +68,15 @@ __device__ SdfHitTime trace_global_sdf(const SdfView<SS>& v, const LuxgiGlobalSD
         if (ix >= iy) stepTime = iy;
         else nextStart = iy;
         const float cmax = half * 2.0f;
+        // divisions by the per-cascade constants as multiplications by their reciprocals, as in
+        // the probe-ray march (exact for power-of-two divisors, within one ulp otherwise)
+        const float invExtent = 1.0f / cmax, invVoxel = 1.0f / voxel, invC = 1.0f / float(C);
         for (int step = 0; step < maxSteps && stepTime < iy; ++step) {
             const f3 p = wp + d * stepTime;
-            const float ux = clampf((p.x - cpos.x) / cmax + 0.5f, 0.0f, 1.0f);
-            const float uy = clampf((p.y - cpos.y) / cmax + 0.5f, 0.0f, 1.0f);
-            const float uz = clampf((p.z - cpos.z) / cmax + 0.5f, 0.0f, 1.0f);
-            const float tx = (float(cas) + ux) / float(C);
+            const float ux = clampf((p.x - cpos.x) * invExtent + 0.5f, 0.0f, 1.0f);
+            const float uy = clampf((p.y - cpos.y) * invExtent + 0.5f, 0.0f, 1.0f);
+            const float uz = clampf((p.z - cpos.z) * invExtent + 0.5f, 0.0f, 1.0f);
+            const float tx = (float(cas) + ux) * invC;
             bool needFine = true;
             if (v.guard) {  // a guard >= twice the margin proves the fine value is not used
                 const int gx = clampi(int(floorf(tx * float(texW) - 0.5f)), 0, texW - 1);
@@ -91,7 +94,7 @@ __device__ SdfHitTime trace_global_sdf(const SdfView<SS>& v, const LuxgiGlobalSD
                 sd = chunkSizeDistance;
             }
             sd *= cmax;
-            const float thick = voxel * 0.5f * clampf(stepTime / voxel, 0.0f, 1.0f);
+            const float thick = voxel * 0.5f * clampf(stepTime * invVoxel, 0.0f, 1.0f);
             if (sd < thick) {
                 hit.time = fmaxf(stepTime + sd - thick, 0.0f);
                 hit.sdf = sd;
@@ -109,7 +112,8 @@ inline int sdf_volume_status(const LuxgiGlobalSDFData* g, const LuxgiSDFVolume* 
     if (!g || !vol || !vol->sdf || !vol->mip) return LUXGI_ERR_INVALID_ARGUMENT;
     if (g->cascadesCount < 1 || g->cascadesCount > 4) return LUXGI_ERR_INVALID_ARGUMENT;
     if (vol->resolution <= 0 || vol->mipResolution <= 0) return LUXGI_ERR_INVALID_ARGUMENT;
-    if (vol->storage != LUXGI_STORAGE_FP32 && vol->storage != LUXGI_STORAGE_FP16) return LUXGI_ERR_UNSUPPORTED;
+    if (vol->storage != LUXGI_STORAGE_FP32 && vol->storage != LUXGI_STORAGE_FP16 && vol->storage != LUXGI_STORAGE_BF16)
+        return LUXGI_ERR_UNSUPPORTED;
     if (vol->layout != LUXGI_LAYOUT_LINEAR && vol->layout != LUXGI_LAYOUT_BRICK4 && vol->layout != LUXGI_LAYOUT_CELL8)
         return LUXGI_ERR_INVALID_ARGUMENT;
     if (vol->layout == LUXGI_LAYOUT_BRICK4 && (vol->resolution & 3)) return LUXGI_ERR_INVALID_ARGUMENT;
@@ -124,7 +128,7 @@ inline int sdf_volume_status(const LuxgiGlobalSDFData* g, const LuxgiSDFVolume* 
     }
     // the linear and brick samplers address the volumes with 32-bit byte offsets, the cell-record
     // sampler with 32-bit record indices
-    const size_t es = vol->storage == LUXGI_STORAGE_FP16 ? 2 : 4;
+    const size_t es = vol->storage == LUXGI_STORAGE_FP32 ? 4 : 2;
     const int C = (int)g->cascadesCount, r = vol->resolution, m = vol->mipResolution;
     auto too_big = [&](int layout, size_t elem) {
         if (layout == LUXGI_LAYOUT_CELL8)

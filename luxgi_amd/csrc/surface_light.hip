@@ -224,6 +224,7 @@ extern "C" int luxgi_surface_light(const LuxgiSurfaceLightParams* params, const 
         if (!lights) return LUXGI_ERR_INVALID_ARGUMENT;
         const int st = sdf_volume_status(sdfData, vol);
         if (st != LUXGI_OK) return st;
+        if (vol->storage == LUXGI_STORAGE_BF16) return LUXGI_ERR_UNSUPPORTED;  // BF16: the probe-ray trace only
     }
     if (indirect) {
         if (!ddgi || !probes || !probes->irradiance || !probes->depth) return LUXGI_ERR_INVALID_ARGUMENT;

@@ -50,10 +50,10 @@ class DeviceScene:
         self.sdf_data = sdf_data
         self.res = int(sdf_data.resolution)
         self.mip_res = (self.res + 3) // 4
-        sdt = torch.float16 if sdf_storage == abi.STORAGE_FP16 else torch.float32
+        sdt = {abi.STORAGE_FP16: torch.float16, abi.STORAGE_BF16: torch.bfloat16}.get(sdf_storage, torch.float32)
         # the FP32 volume the stored forms derive from
         src = sdf.cuda().float() if torch.is_tensor(sdf) else to_device(sdf, torch.float32)
-        self.sdf = src if sdt == torch.float32 else src.half()
+        self.sdf = src if sdt == torch.float32 else src.to(sdt)
         self.mip = (mip.cuda() if torch.is_tensor(mip) else to_device(mip, torch.float32)).to(sdt)
         self.guard = None
         if guard:
@@ -71,7 +71,7 @@ class DeviceScene:
             nbytes = lib().luxgi_sdf_bricks_bytes(self.res * C_, self.res, self.res, storage)
             if nbytes == 0:
                 raise ValueError("LAYOUT_BRICK4 needs dimensions that are multiples of 4")
-            out = torch.empty(nbytes // (2 if storage == abi.STORAGE_FP16 else 4), dtype=dt, device="cuda")
+            out = torch.empty(nbytes // (4 if storage == abi.STORAGE_FP32 else 2), dtype=dt, device="cuda")
             check(lib().luxgi_sdf_to_bricks(ptr(src), self.res * C_, self.res, self.res, storage, ptr(out), ptr(scale),
                                             stream_handle()), "luxgi_sdf_to_bricks")
             return out
@@ -80,7 +80,7 @@ class DeviceScene:
             nbytes = lib().luxgi_sdf_cells_bytes(W, H, D, storage)
             if nbytes == 0:
                 raise ValueError("volume too large for LAYOUT_CELL8")
-            out = torch.empty(nbytes // (2 if storage == abi.STORAGE_FP16 else 4), dtype=dt, device="cuda")
+            out = torch.empty(nbytes // (4 if storage == abi.STORAGE_FP32 else 2), dtype=dt, device="cuda")
             check(lib().luxgi_sdf_to_cells(ptr(vol), W, H, D, storage, ptr(out), stream_handle()),
                   "luxgi_sdf_to_cells")
             return out
@@ -205,7 +205,9 @@ class DDGIVolume:
             check(lib().luxgi_ddgi_trace_sdf_count(*args, ptr(counters), stream_handle()), "luxgi_ddgi_trace_sdf_count")
 
     # probe_update::system (DDGIRenderer.cpp:377-413), border fused by default
-    def probe_update(self, fuse_border=True, per_probe=False, half=False):
+    def probe_update(self, fuse_border=True, per_probe=False, half=False, bf16=False):
+        """half: the reduced-precision blend (one rounding per operand); bf16: its operands in BF16
+        (needs half)."""
         flags = abi.UPDATE_IRRADIANCE | abi.UPDATE_DEPTH
         if fuse_border:
             flags |= abi.UPDATE_FUSE_BORDER
@@ -213,6 +215,8 @@ class DDGIVolume:
             flags |= abi.UPDATE_PER_PROBE
         if half:
             flags |= abi.UPDATE_HALF
+        if bf16:
+            flags |= abi.UPDATE_BF16
         write = 1 - self.ping_pong
         check(lib().luxgi_ddgi_probe_update(C.byref(self.u), C.byref(self.rays), C.byref(self.atlas[self.ping_pong]),
                                             C.byref(self.atlas[write]), 1 if self.frames == 0 else 0, flags,

@@ -119,3 +119,19 @@ def test_ctypes_argument_lists_match_the_header():
             assert res in (C.c_char_p, C.c_void_p), name
         else:
             assert C.sizeof(res) == C.sizeof(_ctype_of(ret)), (name, ret, res)
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="libluxgi.so not built")
+def test_bf16_blend_needs_the_reduced_form():
+    """LUXGI_UPDATE_BF16 without LUXGI_UPDATE_HALF is rejected (-1) before the workspace check (-4),
+    so before any launch."""
+    L = _lib.lib()
+    u = abi.make_uniform((2, 2, 2), (-1, -1, -1), (0.3, 0.4, 0.5), 64, max_distance=1.5)
+    dummy = C.c_void_p(256)
+    rays = abi.RayBuffers(dummy, dummy, 64, u.num_probes, abi.STORAGE_FP32, 0)
+    atl = abi.ProbeAtlas(dummy, dummy, abi.STORAGE_FP32, 0)
+    flags = abi.UPDATE_IRRADIANCE | abi.UPDATE_DEPTH
+    assert L.luxgi_ddgi_probe_update(C.byref(u), C.byref(rays), C.byref(atl), C.byref(atl), 1,
+                                     flags | abi.UPDATE_BF16, None, 0, None) == -1
+    assert L.luxgi_ddgi_probe_update(C.byref(u), C.byref(rays), C.byref(atl), C.byref(atl), 1,
+                                     flags | abi.UPDATE_BF16 | abi.UPDATE_HALF, None, 0, None) == -4

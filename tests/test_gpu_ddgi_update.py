@@ -184,3 +184,31 @@ def test_probe_update_half_constant_inputs_exact():
                         for p in range(P)])
     np.testing.assert_allclose(dep_int[..., 0], r / 2, rtol=2e-6)
     np.testing.assert_allclose(dep_int[..., 1], r * r / 2, rtol=2e-6)
+
+
+@pytest.mark.parametrize("counts,rays,ni,nd", CASES[:3])
+def test_probe_update_bf16_mfma_error_bound(counts, rays, ni, nd):
+    """Reduced-precision blend with BF16 operands (LUXGI_UPDATE_HALF | LUXGI_UPDATE_BF16, BF16 MFMA,
+    FP32 accumulation) against the FP64 oracle on the FP32 rays. A BF16 rounding is at most 2^-9
+    relative, so each product carries at most 2 x 2^-9; normalised by the sum of the rounded weights,
+    every texel stays within 1e-2 relative (+1e-4), the mean relative error within 1.5e-3 and the
+    normwise error within 2e-3. Borders and padding as in the FP32 form, bit for bit."""
+    u = abi.make_uniform(counts, (-1, -1, -1), (0.3, 0.4, 0.5), rays, irr_oct=ni, depth_oct=nd, hysteresis=0.97,
+                         max_distance=1.5)
+    rad, dd = _rays(u, seed=rays + 11)
+    rng = np.random.default_rng(3)
+    prev_i = rng.uniform(0, 1, (u.irradianceTextureHeight, u.irradianceTextureWidth, 4)).astype(np.float32)
+    prev_d = rng.uniform(0, 1, (u.depthTextureHeight, u.depthTextureWidth, 2)).astype(np.float32)
+    flags = abi.UPDATE_IRRADIANCE | abi.UPDATE_DEPTH | abi.UPDATE_FUSE_BORDER
+    for first in (1, 0):
+        gi, gd = _run_gpu(u, rad, dd, prev_i, prev_d, first, flags | abi.UPDATE_HALF | abi.UPDATE_BF16)
+        oi, od = oracle.probe_update(u, rad, dd, prev_i, prev_d, first, flags)
+        for g, o in ((gi, oi), (gd, od)):
+            np.testing.assert_allclose(g, o, rtol=1e-2, atol=1e-4)
+            rel = np.abs(g - o) / (np.abs(o) + 1e-6)
+            assert float(rel.mean()) <= 1.5e-3, float(rel.mean())
+            assert oracle.relative_error_report(g, o)["normwise_rel"] <= 2e-3
+        for atlas, side in ((gi, ni), (gd, nd)):
+            ref = oracle.border_update(atlas.astype(np.float64), side, *counts).astype(np.float32)
+            assert np.array_equal(ref, atlas)
+

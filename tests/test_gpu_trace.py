@@ -57,7 +57,7 @@ def _gpu_trace(u, g, sdf, mip, atlas, params, count=False, storage=abi.STORAGE_F
     return out + ((counters.cpu().numpy() if count else None),)
 
 
-def _compare(u, g, gpu, ref, voxel):
+def _compare(u, g, gpu, ref, voxel, good_fraction=0.98):
     grad, gdd, gfl = gpu[:3]
     orad, odd, ofl, _ = ref
     n = gfl.size
@@ -69,7 +69,7 @@ def _compare(u, g, gpu, ref, voxel):
     t_ok = np.abs(gdd[..., 3] - odd[..., 3]) <= 1e-4 * (1 + np.abs(odd[..., 3]))
     r_ok = np.all(np.abs(grad[..., :3] - orad[..., :3]) <= 2e-3, axis=-1)
     good = same & t_ok & r_ok
-    assert good.mean() >= 0.98, (good.mean(), n)
+    assert good.mean() >= good_fraction, (good.mean(), n)
     assert np.all(np.abs(gdd[..., 3] - odd[..., 3])[same] <= voxel + 1e-4 * (1 + np.abs(odd[..., 3][same])))
     assert np.all(grad[..., 3] == 0)
 
@@ -161,7 +161,7 @@ def test_trace_brick_layout_is_bitwise_identical():
                                            ((1, 1, 1), abi.STORAGE_FP32)])
 def test_cell_records_hold_the_clamped_corners(shape, storage):
     """luxgi_sdf_to_cells: record of cell (x, y, z) = the 8 voxels (x+i, y+j, z+k) clamped to the
-    volume, at brick (x/8, y/8, z/8) x-fastest, Morton slot (bit 3b+a = bit b of coordinate a)."""
+    volume, at brick (x/8, y/8, z/8) x-fastest, slot (z%8)*64 + (y%8)*8 + x%8 inside the brick."""
     from luxgi_amd import _lib
     from luxgi_amd.ddgi import ptr, stream_handle
     W, H, D = shape
@@ -177,13 +177,12 @@ def test_cell_records_hold_the_clamped_corners(shape, storage):
     rec = out.float().cpu().numpy().reshape(-1, 8)
     ref = vol.astype(np.float16).astype(np.float32) if storage == abi.STORAGE_FP16 else vol
 
-    def spread(v):
-        return (v & 1) | ((v & 2) << 2) | ((v & 4) << 4)
+    def slot(x, y, z):
+        return ((z & 7) << 6) | ((y & 7) << 3) | (x & 7)
     for z in range(D):
         for y in range(H):
             for x in range(W):
-                idx = (((z >> 3) * nby + (y >> 3)) * nbx + (x >> 3)) * 512 + \
-                    (spread(x & 7) | (spread(y & 7) << 1) | (spread(z & 7) << 2))
+                idx = (((z >> 3) * nby + (y >> 3)) * nbx + (x >> 3)) * 512 + slot(x, y, z)
                 want = [ref[min(z + k, D - 1), min(y + j, H - 1), min(x + i, W - 1)]
                         for k in (0, 1) for j in (0, 1) for i in (0, 1)]
                 assert np.array_equal(rec[idx], np.array(want, np.float32)), (x, y, z)
@@ -191,8 +190,7 @@ def test_cell_records_hold_the_clamped_corners(shape, storage):
     for z in range(D):
         for y in range(H):
             for x in range(W):
-                written[(((z >> 3) * nby + (y >> 3)) * nbx + (x >> 3)) * 512 +
-                        (spread(x & 7) | (spread(y & 7) << 1) | (spread(z & 7) << 2))] = True
+                written[(((z >> 3) * nby + (y >> 3)) * nbx + (x >> 3)) * 512 + slot(x, y, z)] = True
     assert np.all(rec[~written] == 0)  # padding records of partial bricks
 
 
@@ -247,6 +245,46 @@ def test_trace_fp16_sdf_bricks_match_linear_and_oracle():
     m16 = mip.astype(np.float16).astype(np.float32)
     ref = _scenes.oracle_trace(u, g, s16, m16, atlas, p)
     _compare(u, g, outs[1] + (None,), ref, 2.0 / 64)
+
+
+def _bf16_round(a):
+    """float32 -> nearest bfloat16 (ties to even), returned as float32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def test_trace_bf16_sdf_layouts_match_each_other_and_oracle():
+    """BF16 SDF storage (the reduced-precision form's other 16-bit format): the linear, brick and
+    cell-record layouts give the same rays bit for bit, and they match the oracle run on the same
+    BF16-rounded volume (same population statistics as FP32); with the FP32 re-evaluation at hits
+    they match the FP64 oracle on the FP32 volume within the population bounds that BF16 rounding
+    allows."""
+    from luxgi_amd.ddgi import DDGIVolume, DeviceScene
+    prims, sdf, mip, g, atlas = _scenes.small_scene()
+    u = _scenes.small_uniform(rays=64)
+    p = _scenes.trace_params(rot_seed=29)
+    outs = []
+    for layout in (abi.LAYOUT_LINEAR, abi.LAYOUT_BRICK4, abi.LAYOUT_CELL8):
+        ds = DeviceScene(g, sdf, mip, atlas, sdf_storage=abi.STORAGE_BF16, layout=layout)
+        vol = DDGIVolume(u, flags_buffer=True)
+        vol.trace_rays(ds, p)
+        torch.cuda.synchronize()
+        outs.append((vol.radiance.cpu().numpy(), vol.dir_dist.cpu().numpy(), vol.ray_flags.cpu().numpy()))
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)
+    ref = _scenes.oracle_trace(u, g, _bf16_round(sdf), _bf16_round(mip), atlas, p)
+    _compare(u, g, outs[2] + (None,), ref, 2.0 / 64)
+    ds = DeviceScene(g, sdf, mip, atlas, sdf_storage=abi.STORAGE_BF16, layout=abi.LAYOUT_CELL8, refine=True)
+    vol = DDGIVolume(u, flags_buffer=True)
+    vol.trace_rays(ds, p)
+    torch.cuda.synchronize()
+    got = (vol.radiance.cpu().numpy(), vol.dir_dist.cpu().numpy(), vol.ray_flags.cpu().numpy(), None)
+    # Against FP64 on the FP32 volume, BF16's 2^-9 rounding moves the step at which 3.8% of the rays
+    # detect their hit (the FP64 march on the BF16-rounded volume agrees with the one on the FP32
+    # volume for 96.2% of the rays, every class the same); those rays stay within one voxel.
+    _compare(u, g, got, _scenes.oracle_trace(u, g, sdf, mip, atlas, p), 2.0 / 64, good_fraction=0.95)
 
 
 def test_trace_fp16_scaled_bricks_and_fp32_refine_vs_oracle():

@@ -125,14 +125,15 @@ def padding_box(bmin, bmax):
 
 def bake_primitive(prim, min_res=32, max_res=128, texels_per_meter=3.0):
     """Mesh SDF of one synthetic primitive in its local frame (centred at the origin), with the
-    baker's grid (texel i at min + i * size / (n - 1), SDFBaker.cpp:108-110), resolution rule
-    (extent / targetTexelPerMeter clamped to [32, 128], :58-64), encoding and mips."""
+    baker's grid over the padded box (texel i at min + i * size / (n - 1), SDFBaker.cpp:108-110),
+    resolution rule on the unpadded mesh box (its extent / targetTexelPerMeter clamped to
+    [32, 128] and truncated, :55-62), encoding and mips."""
     p = prim.copy()
     p["center"] = (0.0, 0.0, 0.0)
     ext = scene.prim_half_extents(np.array([p]))[0].astype(np.float64)
     bmin, bmax = padding_box(-ext, ext)
     size = bmax - bmin
-    dims = np.clip(size / texels_per_meter, min_res, max_res).astype(np.int64)
+    dims = np.clip((2.0 * ext) / texels_per_meter, min_res, max_res).astype(np.int64)
     axes = [np.arange(n) * (size[a] / (n - 1)) + bmin[a] for a, n in enumerate(dims)]
     Z, Y, X = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
     d = scene.prim_distance_np(np.array([p]), np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)).reshape(Z.shape)

@@ -153,3 +153,17 @@ def test_chunk_range():
     assert list(lo) == [1, 4, 3] and list(hi) == [1, 4, 3]
     lo, hi = meshsdf.chunk_range(cas, (-1.0, -1.0, -1.0), (-0.95, -0.95, -0.95))
     assert list(lo) == [0, 0, 0]  # -0.23 truncates toward zero
+
+
+def test_baked_resolution_follows_the_unpadded_mesh_box():
+    """SDFBaker.cpp:55-62 takes the resolution from the mesh's own bounding box (extent / texels
+    per metre, clamped to [32, 128], truncated); the 5% / 0.1 m padding only spaces the texels. A
+    120 m long box gives 40 texels along x (the padded 132 m would give 44), and thin axes clamp
+    to 32."""
+    p = np.zeros(1, dtype=scene.PRIM_DTYPE)[0]
+    p["type"] = scene.PRIM_BOX
+    p["size"] = (60.0, 0.5, 0.5)
+    m = meshsdf.bake_primitive(p)
+    assert m.levels[0].shape == (32, 32, 40)
+    bmin, bmax = meshsdf.padding_box((-60.0, -0.5, -0.5), (60.0, 0.5, 0.5))
+    assert np.allclose(m.aabb_min, bmin) and np.allclose(m.aabb_max, bmax)

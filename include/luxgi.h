@@ -330,6 +330,49 @@ int luxgi_ddgi_probe_update_noalloc(const LuxgiDDGIUniform* uniform,
                                     uint32_t flags,
                                     void* stream);
 
+/* Probe update of several volumes (each its own grid, ray count and atlases) in one launch per
+ * map: the per-volume sequence of luxgi_ddgi_probe_update (DDGIRenderer.cpp:347-375 per volume)
+ * with the volumes' 16-probe groups laid end to end in one grid. Results equal a
+ * luxgi_ddgi_probe_update call per volume with the same flags, bit for bit. Every volume must take
+ * the shared-direction MFMA path (irradiance oct^2 <= 64, depth oct^2 <= 256, maxDistance > 0,
+ * no LUXGI_UPDATE_PER_PROBE) and all volumes store rays alike and atlases alike. */
+#define LUXGI_MAX_UPDATE_BATCH 4096
+typedef struct LuxgiUpdateVolume {
+    LuxgiDDGIUniform uniform;
+    LuxgiRayBuffers  rays;
+    LuxgiProbeAtlas  prev;       /* read unless firstFrame */
+    LuxgiProbeAtlas  out;
+    uint32_t         firstFrame;
+    int32_t          pad;
+} LuxgiUpdateVolume;
+
+/* Launch shape of a prepared batch (filled by luxgi_ddgi_update_batch_prepare). */
+typedef struct LuxgiUpdateBatch {
+    int32_t count;            /* volumes */
+    int32_t groups;           /* 16-probe groups over all volumes */
+    int32_t flags;            /* LUXGI_UPDATE_* shared by the volumes */
+    int32_t rayStorage;
+    int32_t atlasStorage;
+    int32_t maxRays;          /* largest raysPerProbe */
+    int32_t maxWeightBlocks;  /* largest (texel tiles x ray chunks) of the weight pass */
+    int32_t pad;
+} LuxgiUpdateBatch;
+
+/* Bytes of the device descriptor array of a batch of `count` volumes (0 if count is out of range). */
+size_t luxgi_ddgi_update_batch_descriptor_bytes(int32_t count);
+/* Device workspace bytes of the batch: the volumes' update workspaces end to end, 256-aligned. */
+size_t luxgi_ddgi_update_batch_workspace_size(const LuxgiUpdateVolume* volumes, int32_t count);
+/* Host-only: checks every volume as luxgi_ddgi_probe_update does, carves the volumes' workspaces
+ * out of `workspace` (a device address, not dereferenced here) and writes the descriptors into
+ * `descriptors` (HOST memory of luxgi_ddgi_update_batch_descriptor_bytes(count) bytes), which the
+ * caller copies to device memory; fills `batch`. A batch whose workspace or descriptors are shared
+ * with another call races with it. */
+int luxgi_ddgi_update_batch_prepare(const LuxgiUpdateVolume* volumes, int32_t count, uint32_t flags,
+                                    void* workspace, size_t workspaceBytes,
+                                    void* descriptors, LuxgiUpdateBatch* batch);
+/* Runs a prepared batch; `descriptors` is the DEVICE copy of what prepare wrote. */
+int luxgi_ddgi_probe_update_batched(const LuxgiUpdateBatch* batch, const void* descriptors, void* stream);
+
 /* IrradianceBorderUpdate.comp / DepthBorderUpdate.comp (Shaders/DDGI/BorderUpdate.glsl:136-156),
  * dispatched at DDGIRenderer.cpp:420-438. In-place on `atlas`.
  * which: LUXGI_UPDATE_IRRADIANCE and/or LUXGI_UPDATE_DEPTH. */
@@ -782,8 +825,10 @@ typedef struct LuxgiTraceVolume {
     LuxgiRayBuffers rays;
     uint8_t* rayFlags;                 /* optional uint8[P*width] */
     int64_t  rayStart;                 /* first ray in the batch's flat ray list; set by
-                                          luxgi_ddgi_trace_batch_prepare */
-} LuxgiTraceVolume;                    /* 224 B */
+                                          luxgi_ddgi_trace_batch_prepare / luxgi_ddgi_trace_scenes_prepare */
+    int32_t  scene;                    /* several-scene batches: index of the volume's scene */
+    int32_t  pad;
+} LuxgiTraceVolume;                    /* 232 B */
 
 /* Host-side: checks `count` host descriptors (grid, ray counts against the ray buffers, one ray
  * storage `rayStorage` for all), sets each rayStart and returns the total ray count. No GPU call. */
@@ -804,6 +849,48 @@ int luxgi_ddgi_trace_sdf_batched(const LuxgiTraceVolume* volumes, int32_t count,
                                  const LuxgiSurfaceAtlas* atlas, const LuxgiSkybox* sky,
                                  const LuxgiTraceParams* params, void* workspace, size_t workspaceBytes,
                                  void* stream);
+
+/* Several probe volumes from several scenes in one launch sequence (no counterpart in the
+ * reference, which traces one volume per dispatch, DDGIRenderer.cpp:318-321): each volume names its
+ * scene (LuxgiTraceVolume.scene), each scene carries the scene arguments of luxgi_ddgi_trace_sdf.
+ * All scenes share one SDF storage and layout and one atlas layout and colour storage; the call's
+ * LuxgiTraceParams gives maxSteps and hitEpsilon (orientations are per volume); hits are shaded in
+ * ray order. Per volume the results are those of luxgi_ddgi_trace_sdf on its scene. */
+typedef struct LuxgiTraceScene {
+    LuxgiGlobalSDFData sdfData;
+    LuxgiSDFVolume     volume;
+    LuxgiSurfaceAtlas  atlas;
+    LuxgiSkybox        skybox;
+    float              skyColor[4];    /* as LuxgiTraceParams.skyColor */
+} LuxgiTraceScene;
+
+/* Filled by luxgi_ddgi_trace_scenes_prepare; passed (host memory) to the trace. */
+typedef struct LuxgiTraceSceneBatch {
+    int64_t totalRays;      /* flat rays including the padding that starts each volume on a multiple of 64 */
+    int64_t totalObjects;   /* atlas objects of all scenes */
+    int32_t sceneCount, volumeCount, maxObjects, rayStorage;
+    int32_t sdfStorage, sdfLayout, atlasLayout, colorStorage;
+} LuxgiTraceSceneBatch;
+
+/* Bytes of the device descriptors of `sceneCount` scenes. */
+size_t luxgi_ddgi_trace_scenes_descriptor_bytes(int32_t sceneCount);
+
+/* Host-side: checks the scenes and the volumes (scene indices, grids, ray buffers of storage
+ * `rayStorage`), sets each volume's rayStart (multiples of 64), writes the scenes' device descriptors
+ * into `descriptors` (host memory of luxgi_ddgi_trace_scenes_descriptor_bytes; the caller copies it
+ * to device memory) and fills `batch`. No GPU call. */
+int luxgi_ddgi_trace_scenes_prepare(const LuxgiTraceScene* scenes, int32_t sceneCount,
+                                    LuxgiTraceVolume* volumes, int32_t volumeCount, int32_t rayStorage,
+                                    void* descriptors, LuxgiTraceSceneBatch* batch);
+
+/* Workspace bytes of luxgi_ddgi_trace_sdf_scenes. */
+size_t luxgi_ddgi_trace_scenes_workspace_size(const LuxgiTraceSceneBatch* batch);
+
+/* The trace: `volumes` and `descriptors` in device memory (as prepared), `batch` and `params` on
+ * the host. Calls that overlap on one workspace race. */
+int luxgi_ddgi_trace_sdf_scenes(const LuxgiTraceSceneBatch* batch, const LuxgiTraceVolume* volumes,
+                                const void* descriptors, const LuxgiTraceParams* params,
+                                void* workspace, size_t workspaceBytes, void* stream);
 
 /* ------------------------------------------------------------------------- */
 /* Batched reflection denoise (no counterpart in the reference)               */
@@ -893,6 +980,50 @@ int luxgi_svgf_learned_atrous_backward(const LuxgiLearnedAtrousParams* params, c
 /* One Adam step on n parameters (step counts from 1); clears `grad`. */
 int luxgi_adam_step(float* weights, float* grad, float* m, float* v, int32_t n, float lr, float beta1, float beta2,
                     float eps, int32_t step, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* Per-pixel learned edge-stopping scales (no counterpart in the reference)   */
+/* ------------------------------------------------------------------------- */
+/* A 12 -> 32 -> 32 -> 5 ReLU MLP (FP16 weights, FP32 accumulation) maps per-pixel features of the
+ * pass-0 input, the reprojection's moments and the G-buffer to one scale per a-trous pass,
+ * s_i = exp(clamp(o_i, -4, 4)); pass i is the reflection a-trous pass (Reflection/DenoiseAtrous.comp
+ * :63-134) with the luminance term of every tap's weight (EdgeStop.glsl:24-40) multiplied by s_i(p)
+ * (luxgi_svgf_reflection_atrous_scaled). Zero output weights and biases give s = 1: the reference
+ * chain, bit for bit. Parameters (FP32 master copy, rounded to FP16 by the forward): W1 [32][12],
+ * b1 [32], W2 [32][32], b2 [32], W3 [5][32], b3 [5]. */
+#define LUXGI_LEARNED_PIXEL_PARAMS 1637
+#define LUXGI_LEARNED_PIXEL_FEATURES 12
+#define LUXGI_LEARNED_PIXEL_PASSES 5
+
+/* luxgi_svgf_reflection_atrous with lumScale[y*W+x] multiplying the luminance term of pixel
+ * (x, y)'s taps (lumScale of 1 everywhere = luxgi_svgf_reflection_atrous, bit for bit). */
+int luxgi_svgf_reflection_atrous_scaled(const LuxgiReflectionAtrousParams* params, int32_t width, int32_t height,
+                                        int32_t storage, const LuxgiGBuffer* cur, const uint8_t* tileMap,
+                                        const void* input, void* output, void* historyOut, const float* lumScale,
+                                        void* stream);
+
+/* The scales of the five passes, planar [5][H][W] FP32, from the pass-0 input `color` (RGBA32F:
+ * rgb + variance) and the reprojection's `moments` (RGBA32F: m1, m2, history length). `features`
+ * (optional, [H][W][12] FP32) receives the features for the backward. */
+int luxgi_learned_pixel_scales(const float* params, int32_t width, int32_t height, const LuxgiGBuffer* cur,
+                               const float* color, const float* moments, float* scales, float* features,
+                               void* stream);
+
+/* Training: the backward of one scaled pass (FP32 images) from dL/d(output rgb): dL/d(input rgb)
+ * (variance channel 0) and dL/d(lumScale). The weights' dependence on the input (luminance, the
+ * variance behind phi) is held fixed. gradIn must not alias input or gradOut. */
+size_t luxgi_learned_pixel_pass_backward_workspace_size(int32_t width, int32_t height);
+int luxgi_learned_pixel_pass_backward(const LuxgiReflectionAtrousParams* params, int32_t width, int32_t height,
+                                      const LuxgiGBuffer* cur, const uint8_t* tileMap, const float* input,
+                                      const float* lumScale, const float* gradOut, float* gradIn, float* gradScale,
+                                      void* workspace, size_t workspaceBytes, void* stream);
+
+/* Training: adds dL/d(parameters) into gradParams from dL/d(scales) ([5][H][W]), the forward's
+ * scales and features. Deterministic (per-block partial sums added in block order). */
+size_t luxgi_learned_pixel_mlp_backward_workspace_size(void);
+int luxgi_learned_pixel_mlp_backward(const float* params, int32_t width, int32_t height, const float* features,
+                                     const float* scales, const float* gradScales, float* gradParams,
+                                     void* workspace, size_t workspaceBytes, void* stream);
 
 #ifdef __cplusplus
 }

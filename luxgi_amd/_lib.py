@@ -24,6 +24,10 @@ SIGNATURES = {
     "luxgi_ddgi_probe_update_workspace_size": (C.c_size_t, [P]),
     "luxgi_ddgi_probe_update": (C.c_int, [P, P, P, P, U, U, P, C.c_size_t, P]),
     "luxgi_ddgi_probe_update_noalloc": (C.c_int, [P, P, P, P, U, U, P]),
+    "luxgi_ddgi_update_batch_descriptor_bytes": (C.c_size_t, [I]),
+    "luxgi_ddgi_update_batch_workspace_size": (C.c_size_t, [P, I]),
+    "luxgi_ddgi_update_batch_prepare": (C.c_int, [P, I, U, P, C.c_size_t, P, P]),
+    "luxgi_ddgi_probe_update_batched": (C.c_int, [P, P, P]),
     "luxgi_ddgi_border_update": (C.c_int, [P, P, U, P]),
     "luxgi_ddgi_relocate_classify": (C.c_int, [P, P, P, P, P, P, P]),
     "luxgi_ddgi_sample_probes": (C.c_int, [P, P, P, I, I, P, P, I, P, P]),
@@ -51,11 +55,21 @@ SIGNATURES = {
     "luxgi_ddgi_trace_batch_prepare": (C.c_int, [P, I, I, P]),
     "luxgi_ddgi_trace_batch_workspace_size": (C.c_size_t, [C.c_int64, I]),
     "luxgi_ddgi_trace_sdf_batched": (C.c_int, [P, I, C.c_int64, I, P, P, P, P, P, P, C.c_size_t, P]),
+    "luxgi_ddgi_trace_scenes_descriptor_bytes": (C.c_size_t, [I]),
+    "luxgi_ddgi_trace_scenes_prepare": (C.c_int, [P, I, P, I, I, P, P]),
+    "luxgi_ddgi_trace_scenes_workspace_size": (C.c_size_t, [P]),
+    "luxgi_ddgi_trace_sdf_scenes": (C.c_int, [P, P, P, P, P, C.c_size_t, P]),
     "luxgi_svgf_batch_prepare": (C.c_int, [P, I, P]),
     "luxgi_svgf_reflection_denoise_batched": (C.c_int, [P, P, I, I, I, I, P, P]),
     "luxgi_svgf_learned_atrous": (C.c_int, [P, P, I, I, P, P, P, P, P]),
     "luxgi_svgf_learned_atrous_backward": (C.c_int, [P, P, I, I, P, P, P, P, P, P, P, P]),
     "luxgi_adam_step": (C.c_int, [P, P, P, P, I, C.c_float, C.c_float, C.c_float, C.c_float, I, P]),
+    "luxgi_svgf_reflection_atrous_scaled": (C.c_int, [P, I, I, I, P, P, P, P, P, P, P]),
+    "luxgi_learned_pixel_scales": (C.c_int, [P, I, I, P, P, P, P, P, P]),
+    "luxgi_learned_pixel_pass_backward_workspace_size": (C.c_size_t, [I, I]),
+    "luxgi_learned_pixel_pass_backward": (C.c_int, [P, I, I, P, P, P, P, P, P, P, P, C.c_size_t, P]),
+    "luxgi_learned_pixel_mlp_backward_workspace_size": (C.c_size_t, []),
+    "luxgi_learned_pixel_mlp_backward": (C.c_int, [P, I, I, P, P, P, P, P, C.c_size_t, P]),
     "luxgi_scene_bake_sdf": (C.c_int, [P, I, I, P, C.c_float, P, P]),
     "luxgi_scene_gbuffer": (C.c_int, [P, I, P, P, I, P, P, I, I, U, P, P, P, P, P, P, P]),
 }

@@ -214,6 +214,9 @@ class LearnedAtrousParams(C.Structure):  # include/luxgi.h LuxgiLearnedAtrousPar
 
 
 LEARNED_ATROUS_PARAMS = 81
+LEARNED_PIXEL_PARAMS = 1637
+LEARNED_PIXEL_FEATURES = 12
+LEARNED_PIXEL_PASSES = 5
 
 MAX_TRACE_BATCH = 256
 
@@ -221,7 +224,33 @@ MAX_TRACE_BATCH = 256
 class TraceVolume(C.Structure):  # include/luxgi.h LuxgiTraceVolume
     _fields_ = [("uniform", DDGIUniform), ("orientation", C.c_float * 16), ("deviceOrientation", C.c_void_p),
                 ("probeOffsets", C.c_void_p), ("rays", RayBuffers), ("rayFlags", C.c_void_p),
-                ("rayStart", C.c_int64)]
+                ("rayStart", C.c_int64), ("scene", C.c_int32), ("pad", C.c_int32)]
+
+
+class TraceScene(C.Structure):  # include/luxgi.h LuxgiTraceScene
+    _fields_ = [("sdfData", GlobalSDFData), ("volume", SDFVolume), ("atlas", SurfaceAtlas), ("skybox", Skybox),
+                ("skyColor", C.c_float * 4)]
+
+
+class TraceSceneBatch(C.Structure):  # include/luxgi.h LuxgiTraceSceneBatch
+    _fields_ = [("totalRays", C.c_int64), ("totalObjects", C.c_int64), ("sceneCount", C.c_int32),
+                ("volumeCount", C.c_int32), ("maxObjects", C.c_int32), ("rayStorage", C.c_int32),
+                ("sdfStorage", C.c_int32), ("sdfLayout", C.c_int32), ("atlasLayout", C.c_int32),
+                ("colorStorage", C.c_int32)]
+
+
+MAX_UPDATE_BATCH = 4096
+
+
+class UpdateVolume(C.Structure):  # include/luxgi.h LuxgiUpdateVolume
+    _fields_ = [("uniform", DDGIUniform), ("rays", RayBuffers), ("prev", ProbeAtlas), ("out", ProbeAtlas),
+                ("firstFrame", C.c_uint32), ("pad", C.c_int32)]
+
+
+class UpdateBatch(C.Structure):  # include/luxgi.h LuxgiUpdateBatch
+    _fields_ = [("count", C.c_int32), ("groups", C.c_int32), ("flags", C.c_int32), ("rayStorage", C.c_int32),
+                ("atlasStorage", C.c_int32), ("maxRays", C.c_int32), ("maxWeightBlocks", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 MAX_DENOISE_BATCH = 1024
@@ -243,7 +272,7 @@ _SIZES = {DDGIUniform: 96, GlobalSDFData: 96, SurfaceAtlasData: 32, AtlasObject:
           SurfaceGBuffer: 40, SurfaceLightParams: 32, SurfaceDraw: 8, BlueNoise: 16, SdfShadowParams: 144,
           SdfReflectionParams: 144, ObjectRasterizeData: 176, RasterizeParams: 48, MeshSDF: 72,
           LearnedAtrousParams: 32, ReflectionDenoiseItem: 336, DenoiseBatchHeader: 16,
-          TraceVolume: 224}
+          TraceVolume: 232, TraceScene: 288, TraceSceneBatch: 48, UpdateVolume: 216, UpdateBatch: 32}
 for _t, _n in _SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
 

@@ -179,6 +179,12 @@ class ReflectionDenoiseBatch:
                 i += 1
 
 
+def _field(st, name):
+    """A ctypes struct field as a comparable value (arrays as tuples)."""
+    v = getattr(st, name)
+    return tuple(v) if isinstance(v, C.Array) else v
+
+
 class DDGIBatch:
     """Several probe volumes of any probe and ray counts over one scene, updated each frame (trace,
     blend with the fused border, flip; DDGIRenderer.cpp:217-413 per volume) by one graph replay.
@@ -189,16 +195,29 @@ class DDGIBatch:
     at run time (deviceOrientation) before replaying the parity's graph.
 
     trace="batched" (default) traces all volumes in one launch sequence over their flat ray list
-    (luxgi_ddgi_trace_sdf_batched); trace="per_volume" calls luxgi_ddgi_trace_sdf per volume. The
-    blends run per volume; with `streams` > 0 the per-volume work is spread over that many side
-    streams (default priority), forked from and joined to the current stream."""
+    (luxgi_ddgi_trace_sdf_batched); trace="per_volume" calls luxgi_ddgi_trace_sdf per volume.
+    blend="batched" blends all volumes in one launch per map (luxgi_ddgi_probe_update_batched);
+    blend="per_volume" calls luxgi_ddgi_probe_update per volume; "auto" (default) takes the batched
+    form when every volume fits it. With `streams` > 0 the per-volume work is spread over that many
+    side streams (default priority), forked from and joined to the current stream."""
 
-    def __init__(self, uniforms, scene, trace_params=None, fuse_border=True, streams=4, trace="batched"):
+    def __init__(self, uniforms, scene, trace_params=None, fuse_border=True, streams=4, trace="batched",
+                 scene_of=None, blend="auto"):
+        """scene: one DeviceScene for all volumes, or a list of them with scene_of[i] the scene of
+        volume i (volumes from several scenes; trace="batched" then traces them all in one launch
+        sequence with luxgi_ddgi_trace_sdf_scenes)."""
         from .ddgi import DDGIVolume, default_trace_params
         if trace not in ("batched", "per_volume"):
             raise ValueError("trace is 'batched' or 'per_volume'")
+        if blend not in ("auto", "batched", "per_volume"):
+            raise ValueError("blend is 'auto', 'batched' or 'per_volume'")
         self.volumes = [DDGIVolume(u) for u in uniforms]
-        self.scene = scene
+        self.scene_list = list(scene) if isinstance(scene, (list, tuple)) else None
+        if self.scene_list is not None:
+            if scene_of is None or len(scene_of) != len(self.volumes):
+                raise ValueError("scene_of names the scene of every volume")
+            self.scene_of = [int(i) for i in scene_of]
+        self.scene = scene if self.scene_list is None else None
         self.fuse_border = fuse_border
         self.trace_mode = trace
         # the volumes are independent: volume i runs on side stream i % streams (default priority),
@@ -220,22 +239,87 @@ class DDGIBatch:
         if trace == "batched":
             if k > abi.MAX_TRACE_BATCH:
                 raise ValueError(f"at most {abi.MAX_TRACE_BATCH} volumes per batched trace")
+            # one launch sequence takes its march settings from one LuxgiTraceParams: the
+            # volumes may differ in orientation only
+            shared = [f for f, _ in abi.TraceParams._fields_ if f not in ("randomOrientation", "deviceOrientation")]
+            for p in self.params[1:]:
+                if any(_field(p, f) != _field(self.params[0], f) for f in shared):
+                    raise ValueError("batched trace: the volumes' trace_params may differ in orientation only")
             d = (abi.TraceVolume * k)()
             for i, vol in enumerate(self.volumes):
                 d[i].uniform = vol.u
                 d[i].deviceOrientation = self.orientation[i].data_ptr()
                 d[i].rays = vol.rays
-            total = C.c_int64(0)
-            check(lib().luxgi_ddgi_trace_batch_prepare(C.byref(d), k, self.volumes[0].ray_storage, C.byref(total)),
-                  "luxgi_ddgi_trace_batch_prepare")
-            self.total_rays = total.value
+                d[i].scene = self.scene_of[i] if self.scene_list is not None else 0
+            if self.scene_list is None:
+                total = C.c_int64(0)
+                check(lib().luxgi_ddgi_trace_batch_prepare(C.byref(d), k, self.volumes[0].ray_storage,
+                                                           C.byref(total)), "luxgi_ddgi_trace_batch_prepare")
+                self.total_rays = total.value
+                ws = lib().luxgi_ddgi_trace_batch_workspace_size(self.total_rays, scene.n_objects)
+            else:
+                n = len(self.scene_list)
+                sc = (abi.TraceScene * n)()
+                for j, ds in enumerate(self.scene_list):
+                    sc[j].sdfData, sc[j].volume, sc[j].atlas, sc[j].skybox = ds.sdf_data, ds.volume, ds.atlas, ds.skybox
+                    sc[j].skyColor[:] = list(self.params[0].skyColor)
+                host = (C.c_uint8 * lib().luxgi_ddgi_trace_scenes_descriptor_bytes(n))()
+                self.scene_batch = abi.TraceSceneBatch()
+                check(lib().luxgi_ddgi_trace_scenes_prepare(C.byref(sc), n, C.byref(d), k, self.volumes[0].ray_storage,
+                                                            C.byref(host), C.byref(self.scene_batch)),
+                      "luxgi_ddgi_trace_scenes_prepare")
+                self.total_rays = self.scene_batch.totalRays
+                self.scene_descriptors = torch.from_numpy(np.frombuffer(host, dtype=np.uint8).copy()).cuda()
+                ws = lib().luxgi_ddgi_trace_scenes_workspace_size(C.byref(self.scene_batch))
             # the descriptors do not change from frame to frame (the orientations are read through
             # deviceOrientation), so they are copied to the device once
             self.descriptors = torch.from_numpy(
                 np.frombuffer(memoryview(d).cast("B"), dtype=np.uint8).copy()).cuda()
-            ws = lib().luxgi_ddgi_trace_batch_workspace_size(self.total_rays, scene.n_objects)
             self.trace_workspace = torch.empty(max(ws, 256), dtype=torch.uint8, device="cuda")
             self.shared_params = self.params[0]
+        fits = all(v.u.irradianceProbeSideLength ** 2 <= 64 and v.u.depthProbeSideLength ** 2 <= 256
+                   and v.u.maxDistance > 0 for v in self.volumes) and k <= abi.MAX_UPDATE_BATCH
+        if blend == "batched" and not fits:
+            raise ValueError("batched blend: every volume needs irradiance oct^2 <= 64, depth oct^2 <= 256, "
+                             "maxDistance > 0")
+        self.blend_mode = "batched" if blend == "batched" or (blend == "auto" and fits) else "per_volume"
+        if self.blend_mode == "batched":
+            self._prepare_blend()
+
+    def _prepare_blend(self):
+        """Descriptors of the batched blend for the first frame and for each ping-pong parity (the
+        atlases read and written swap with the parity), on the device; one shared workspace."""
+        k = len(self.volumes)
+        flags = abi.UPDATE_IRRADIANCE | abi.UPDATE_DEPTH | (abi.UPDATE_FUSE_BORDER if self.fuse_border else 0)
+        vols = (abi.UpdateVolume * k)()
+        for i, v in enumerate(self.volumes):
+            vols[i].uniform = v.u
+            vols[i].rays = v.rays
+        ws = lib().luxgi_ddgi_update_batch_workspace_size(C.byref(vols), k)
+        self.blend_workspace = torch.empty(max(ws, 256), dtype=torch.uint8, device="cuda")
+        nbytes = lib().luxgi_ddgi_update_batch_descriptor_bytes(k)
+        self.blend_sets = {}
+        for key in ("first", 0, 1):
+            parity = 0 if key == "first" else key
+            for i, v in enumerate(self.volumes):
+                vols[i].prev = v.atlas[parity]
+                vols[i].out = v.atlas[1 - parity]
+                vols[i].firstFrame = 1 if key == "first" else 0
+            host = (C.c_uint8 * nbytes)()
+            b = abi.UpdateBatch()
+            check(lib().luxgi_ddgi_update_batch_prepare(C.byref(vols), k, flags,
+                                                        C.c_void_p(self.blend_workspace.data_ptr()),
+                                                        self.blend_workspace.numel(), C.byref(host), C.byref(b)),
+                  "luxgi_ddgi_update_batch_prepare")
+            self.blend_sets[key] = (b, torch.from_numpy(np.frombuffer(host, dtype=np.uint8).copy()).cuda())
+
+    def _blend_all(self):
+        b, d = self.blend_sets["first" if self.frames == 0 else self.volumes[0].ping_pong]
+        check(lib().luxgi_ddgi_probe_update_batched(C.byref(b), C.c_void_p(d.data_ptr()), stream_handle()),
+              "luxgi_ddgi_probe_update_batched")
+        if not self.fuse_border:
+            for vol in self.volumes:
+                vol.border_update()
 
     @property
     def frames(self):
@@ -259,6 +343,13 @@ class DDGIBatch:
         self._staged[j] = ev
 
     def _trace_all(self):
+        if self.scene_list is not None:
+            check(lib().luxgi_ddgi_trace_sdf_scenes(
+                C.byref(self.scene_batch), C.c_void_p(self.descriptors.data_ptr()),
+                C.c_void_p(self.scene_descriptors.data_ptr()), C.byref(self.shared_params),
+                C.c_void_p(self.trace_workspace.data_ptr()), self.trace_workspace.numel(), stream_handle()),
+                "luxgi_ddgi_trace_sdf_scenes")
+            return
         sc = self.scene
         check(lib().luxgi_ddgi_trace_sdf_batched(
             C.c_void_p(self.descriptors.data_ptr()), len(self.volumes), self.total_rays, self.volumes[0].ray_storage,
@@ -266,26 +357,35 @@ class DDGIBatch:
             C.byref(self.shared_params), C.c_void_p(self.trace_workspace.data_ptr()), self.trace_workspace.numel(),
             stream_handle()), "luxgi_ddgi_trace_sdf_batched")
 
-    def _run_volume(self, vol, p):
+    def _run_volume(self, vol, p, i=0):
         if self.trace_mode == "per_volume":
-            vol.trace_rays(self.scene, p)
-        vol.probe_update(fuse_border=self.fuse_border)
-        if not self.fuse_border:
-            vol.border_update()
+            vol.trace_rays(self.scene if self.scene_list is None else self.scene_list[self.scene_of[i]], p)
+        if self.blend_mode == "per_volume":
+            vol.probe_update(fuse_border=self.fuse_border)
+            if not self.fuse_border:
+                vol.border_update()
 
     def _run(self):
         if self.trace_mode == "batched":
             self._trace_all()
+        if self.blend_mode == "batched":
+            if self.trace_mode == "per_volume":
+                self._run_streams()
+            self._blend_all()
+            return
+        self._run_streams()
+
+    def _run_streams(self):
         if not self.streams:
-            for vol, p in zip(self.volumes, self.params):
-                self._run_volume(vol, p)
+            for i, (vol, p) in enumerate(zip(self.volumes, self.params)):
+                self._run_volume(vol, p, i)
             return
         main = torch.cuda.current_stream()
         for st in self.streams:
             st.wait_stream(main)
         for i, (vol, p) in enumerate(zip(self.volumes, self.params)):
             with torch.cuda.stream(self.streams[i % len(self.streams)]):
-                self._run_volume(vol, p)
+                self._run_volume(vol, p, i)
         for st in self.streams:
             main.wait_stream(st)
 

@@ -307,6 +307,30 @@ __global__ __launch_bounds__(256) void probe_update_kernel(UpdateArgs A, const S
     }
 }
 
+// The fix-up after the shared-direction blend: per-probe reference arithmetic for the probes flagged
+// in `mismatch` (directions not those of ray row 0). A block covers 256 probes and leaves at once
+// when none of them is flagged, the usual case.
+template <class SR, class SA>
+__global__ __launch_bounds__(256) void fixup_kernel(UpdateArgs A, const SR* __restrict__ radiance,
+                                                    const SR* __restrict__ dirDist, const SA* __restrict__ prevIrr,
+                                                    const SA* __restrict__ prevDepth, SA* __restrict__ outIrr,
+                                                    SA* __restrict__ outDepth, const uint8_t* __restrict__ mismatch) {
+    __shared__ UpdateLds lds;
+    __shared__ int sShared[kProbesPerBlock];
+    const int base = blockIdx.x * 256;
+    const int p = base + (int)threadIdx.x;
+    if (!__syncthreads_or(p < A.P && mismatch[p] != 0)) return;  // uniform across the block
+    const int end = min(base + 256, A.P);
+    for (int q0 = base; q0 < end; q0 += kProbesPerBlock) {
+        bool any = false;
+        for (int g = 0; g < kProbesPerBlock; ++g) any |= q0 + g < A.P && mismatch[q0 + g] != 0;
+        if (!any) continue;  // uniform across the block
+        update_group<SR, SA>(A, q0, radiance, dirDist, prevIrr, prevDepth, outIrr, outDepth, nullptr, nullptr,
+                             mismatch, lds, sShared);
+        __syncthreads();  // the next group reuses the LDS
+    }
+}
+
 // Stand-alone border copy, in place (BorderUpdate.glsl:136-156).
 template <class SA, int CH>
 __global__ void border_kernel(SA* __restrict__ atlas, int W, int n, int pxy, int P) {
@@ -386,18 +410,23 @@ template <> struct Lp8<_Float16> { typedef half8v V; };
 template <> struct Lp8<__bf16> { typedef bf16x8v V; };
 constexpr int kHK = 32;  // rays per k-step
 
+// Shared weights of one map in the MFMA operand type T (hi part, and with `split` the lo part), texel
+// major [tpad][Rpad], from ray row 0's directions, and their sums in kWChunk-ray chunks,
+// Sp[chunk][tpad] (the blend adds the chunks in order). Block (tile, chunk): 16 texels x kWChunk rays.
+constexpr int kWChunk = 64;
+
 template <class T>
-__global__ __launch_bounds__(256) void weights_lowp_kernel(const float* __restrict__ dirs0, int R, int Rpad, int n,
-                                                           int tpad, int depth, float sharpness, int split,
-                                                           T* __restrict__ Whi, T* __restrict__ Wlo,
-                                                           float* __restrict__ Sh) {
+__device__ __forceinline__ void weights_block(int tile, int chunk, const float* __restrict__ dirs0, int R, int Rpad,
+                                              int n, int tpad, int depth, float sharpness, int split,
+                                              T* __restrict__ Whi, T* __restrict__ Wlo, float* __restrict__ Sp) {
     __shared__ float part[16][17];
     const int tl = threadIdx.x & 15, j = threadIdx.x >> 4;
-    const int t = blockIdx.x * 16 + tl;
+    const int t = tile * 16 + tl;
     const bool valid = t < n * n;
     const f3 td = valid ? texel_direction(t % n, t / n, n) : mk(0.f, 0.f, 1.f);
     float s = 0.0f;
-    for (int r = j; r < Rpad; r += 16) {
+    const int r1 = min(Rpad, (chunk + 1) * kWChunk);
+    for (int r = chunk * kWChunk + j; r < r1; r += 16) {
         float w = 0.0f;
         if (valid && r < R) {
             w = fmaxf(0.0f, dot3(td, mk(dirs0[r * 4 + 0], dirs0[r * 4 + 1], dirs0[r * 4 + 2])));
@@ -419,8 +448,16 @@ __global__ __launch_bounds__(256) void weights_lowp_kernel(const float* __restri
     if (j == 0) {
         float tot = 0.0f;
         for (int k = 0; k < 16; ++k) tot += part[k][tl];
-        Sh[t] = tot;
+        Sp[(size_t)chunk * tpad + t] = tot;
     }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void weights_lowp_kernel(const float* __restrict__ dirs0, int R, int Rpad, int n,
+                                                           int tpad, int depth, float sharpness, int split,
+                                                           T* __restrict__ Whi, T* __restrict__ Wlo,
+                                                           float* __restrict__ Sp) {
+    weights_block<T>(blockIdx.x, blockIdx.y, dirs0, R, Rpad, n, tpad, depth, sharpness, split, Whi, Wlo, Sp);
 }
 
 __device__ __forceinline__ floatx4 mfma32(half8v a, half8v b, floatx4 acc) {
@@ -596,19 +633,19 @@ __device__ __forceinline__ void map_store(const UpdateArgs& A, int k0, int p0, c
     }
 }
 
+// one block of 16 probes starting at p0 (the body of the single-volume and batched map kernels)
 template <class SR, class SA, class T, bool SPLIT, int MAP, bool CHECK>
-__global__ __launch_bounds__(256, 4) void probe_blend_map_kernel(
-    UpdateArgs A, const SR* __restrict__ radiance, const SR* __restrict__ dirDist, const SA* __restrict__ prev,
-    SA* __restrict__ out, const float* __restrict__ dirs0, const T* __restrict__ Wh, const T* __restrict__ Wl,
-    const float* __restrict__ Sh, uint8_t* __restrict__ mismatch, int tpad, int Rpad) {
+__device__ __forceinline__ void blend_map_block(const UpdateArgs& A, int p0, const SR* __restrict__ radiance,
+                                                const SR* __restrict__ dirDist, const SA* __restrict__ prev,
+                                                SA* __restrict__ out, const float* __restrict__ dirs0,
+                                                const T* __restrict__ Wh, const T* __restrict__ Wl,
+                                                const float* __restrict__ Sh, uint8_t* __restrict__ mismatch,
+                                                int tpad, int Rpad, MapLds<MAP, T>& lds, int* sMis) {
     typedef typename Lp8<T>::V V;
     constexpr int CH = MAP == 0 ? 2 : 3;
     constexpr int NJ = MAP == 0 ? 4 : 1;  // m-tiles per wave: depth up to 16 tiles, irradiance up to 4
-    __shared__ MapLds<MAP, T> lds;
-    __shared__ int sMis[kMG];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int col = lane & 15, kq = lane >> 4;
-    const int p0 = blockIdx.x * kMG;
     const int Mt = tpad / 16;
     if (tid < kMG) sMis[tid] = 0;
     floatx4 acc[NJ][CH];
@@ -660,7 +697,8 @@ __global__ __launch_bounds__(256, 4) void probe_blend_map_kernel(
         for (int v = 0; v < 4; ++v) {
             const int t = m * 16 + rowq + v;
             if (t >= nT) continue;
-            const float sm = Sh[t];
+            float sm = 0.0f;
+            for (int c = 0; c * kWChunk < Rpad; ++c) sm += Sh[(size_t)c * tpad + t];
             const float sc = sm > kFltEps ? 1.0f / (2.0f * sm) : 1.0f;
             if constexpr (MAP == 0) {
                 lds.out.v[col][t] = make_float2(acc[j][0][v] * sc * A.maxDistance,
@@ -676,6 +714,114 @@ __global__ __launch_bounds__(256, 4) void probe_blend_map_kernel(
     if ((MAP == 0 || CHECK) && tid < kMG && p0 + tid < A.P) mismatch[p0 + tid] = (uint8_t)sMis[tid];
     if constexpr (MAP == 0) write_staged<SA, 2>(A, lds.out, p0, A.nd, A.Wd, prev, out);
     else write_staged<SA, 4>(A, lds.out, p0, A.ni, A.Wi, prev, out);
+}
+
+template <class SR, class SA, class T, bool SPLIT, int MAP, bool CHECK>
+__global__ __launch_bounds__(256, 4) void probe_blend_map_kernel(
+    UpdateArgs A, const SR* __restrict__ radiance, const SR* __restrict__ dirDist, const SA* __restrict__ prev,
+    SA* __restrict__ out, const float* __restrict__ dirs0, const T* __restrict__ Wh, const T* __restrict__ Wl,
+    const float* __restrict__ Sh, uint8_t* __restrict__ mismatch, int tpad, int Rpad) {
+    __shared__ MapLds<MAP, T> lds;
+    __shared__ int sMis[kMG];
+    blend_map_block<SR, SA, T, SPLIT, MAP, CHECK>(A, blockIdx.x * kMG, radiance, dirDist, prev, out, dirs0, Wh, Wl, Sh,
+                                                  mismatch, tpad, Rpad, lds, sMis);
+}
+
+// ---------------------------------------------------------------------------
+// Several volumes in one launch per pass (no counterpart in the reference, which dispatches the
+// update per volume): each volume's arguments and workspace pointers in a device descriptor, the
+// blocks of all volumes' 16-probe groups in one grid (block -> volume by the groups' prefix,
+// uniform per block). Per volume the passes are those of luxgi_ddgi_probe_update.
+// ---------------------------------------------------------------------------
+struct UpdateVolumeDesc {
+    UpdateArgs A;
+    const void* rad;
+    const void* dd;
+    const void* prevI;
+    const void* prevD;
+    void* outI;
+    void* outD;
+    float* dirs0;
+    void *Wih, *Wil, *Wdh, *Wdl;
+    float *Sih, *Sdh;
+    uint8_t* mismatch;
+    int tpI, tpD, Rp;
+    int groupStart;  // first 16-probe group of this volume in the batch grid
+};
+
+__device__ __forceinline__ int batch_update_volume(const UpdateVolumeDesc* __restrict__ d, int n, int group) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (d[mid].groupStart <= group) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class SR>
+__global__ void row0_batched_kernel(const UpdateVolumeDesc* __restrict__ d) {
+    const UpdateVolumeDesc& D = d[blockIdx.y];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < D.A.R)
+        reinterpret_cast<float4*>(D.dirs0)[r] = Texel<SR>::ld4(static_cast<const SR*>(D.dd), (size_t)r);
+}
+
+// blockIdx.y: volume, blockIdx.z: map (0 irradiance, 1 depth), x: (16-texel tile, ray chunk)
+template <class T>
+__global__ __launch_bounds__(256) void weights_batched_kernel(const UpdateVolumeDesc* __restrict__ d, int split) {
+    const UpdateVolumeDesc& D = d[blockIdx.y];
+    const bool depth = blockIdx.z == 1;
+    if (depth ? !D.A.doDepth : !D.A.doIrr) return;
+    const int tp = depth ? D.tpD : D.tpI;
+    const int chunks = (D.Rp + kWChunk - 1) / kWChunk;
+    const int tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    if (tile * 16 >= tp) return;
+    weights_block<T>(tile, chunk, D.dirs0, D.A.R, D.Rp, depth ? D.A.nd : D.A.ni, tp, depth ? 1 : 0, D.A.sharpness,
+                     split, static_cast<T*>(depth ? D.Wdh : D.Wih), static_cast<T*>(depth ? D.Wdl : D.Wil),
+                     depth ? D.Sdh : D.Sih);
+}
+
+template <class SR, class SA, class T, bool SPLIT, int MAP, bool CHECK>
+__global__ __launch_bounds__(256, 4) void probe_blend_map_batched_kernel(const UpdateVolumeDesc* __restrict__ d, int n) {
+    __shared__ MapLds<MAP, T> lds;
+    __shared__ int sMis[kMG];
+    const UpdateVolumeDesc& D = d[batch_update_volume(d, n, blockIdx.x)];
+    const int p0 = (blockIdx.x - D.groupStart) * kMG;
+    if (p0 >= D.A.P) return;  // uniform per block
+    if constexpr (MAP == 0)
+        blend_map_block<SR, SA, T, SPLIT, 0, CHECK>(D.A, p0, static_cast<const SR*>(D.rad), static_cast<const SR*>(D.dd),
+                                                    static_cast<const SA*>(D.prevD), static_cast<SA*>(D.outD), D.dirs0,
+                                                    static_cast<const T*>(D.Wdh), static_cast<const T*>(D.Wdl), D.Sdh,
+                                                    D.mismatch, D.tpD, D.Rp, lds, sMis);
+    else
+        blend_map_block<SR, SA, T, SPLIT, 1, CHECK>(D.A, p0, static_cast<const SR*>(D.rad), static_cast<const SR*>(D.dd),
+                                                    static_cast<const SA*>(D.prevI), static_cast<SA*>(D.outI), D.dirs0,
+                                                    static_cast<const T*>(D.Wih), static_cast<const T*>(D.Wil), D.Sih,
+                                                    D.mismatch, D.tpI, D.Rp, lds, sMis);
+}
+
+// the per-probe fix-up of flagged probes (directions not those of row 0), 4 probes at a time
+template <class SR, class SA>
+__global__ __launch_bounds__(256) void fixup_batched_kernel(const UpdateVolumeDesc* __restrict__ d, int n) {
+    __shared__ UpdateLds lds;
+    __shared__ int sShared[kProbesPerBlock];
+    const UpdateVolumeDesc& D = d[batch_update_volume(d, n, blockIdx.x)];
+    const int p0 = (blockIdx.x - D.groupStart) * kMG;
+    const int tid = threadIdx.x;
+    if (!__syncthreads_or(tid < kMG && p0 + tid < D.A.P && D.mismatch[p0 + tid] != 0)) return;  // uniform
+    UpdateArgs F = D.A;
+    F.forcePerProbe = 1;
+    for (int q0 = p0; q0 < min(p0 + kMG, F.P); q0 += kProbesPerBlock) {
+        bool any = false;
+        for (int g = 0; g < kProbesPerBlock; ++g) any |= q0 + g < F.P && D.mismatch[q0 + g] != 0;
+        if (!any) continue;  // uniform across the block
+        update_group<SR, SA>(F, q0, static_cast<const SR*>(D.rad), static_cast<const SR*>(D.dd),
+                             static_cast<const SA*>(D.prevI), static_cast<const SA*>(D.prevD),
+                             static_cast<SA*>(D.outI), static_cast<SA*>(D.outD), nullptr, nullptr, D.mismatch, lds,
+                             sShared);
+        __syncthreads();  // the next group reuses the LDS
+    }
 }
 
 static int check_uniform(const LuxgiDDGIUniform* u) {
@@ -738,12 +884,13 @@ static size_t update_workspace_layout(const LuxgiDDGIUniform* u, char* base, Upd
     char* sd = take(Td * 4);
     char* mm = take(P);
     const size_t Rp = (size_t)pad32((int)R);
+    const size_t chunks = (Rp + kWChunk - 1) / kWChunk;
     char* wih = take(Rp * Ti * 2);
     char* wil = take(Rp * Ti * 2);
-    char* sih = take(Ti * 4);
+    char* sih = take(chunks * Ti * 4);
     char* wdh = take(Rp * Td * 2);
     char* wdl = take(Rp * Td * 2);
-    char* sdh = take(Td * 4);
+    char* sdh = take(chunks * Td * 4);
     if (w) {
         w->Wih = reinterpret_cast<_Float16*>(wih);
         w->Wil = reinterpret_cast<_Float16*>(wil);
@@ -775,12 +922,14 @@ static int launch_maps(const UpdateArgs& A, const SR* rad, const SR* dd, const S
     T* Wdh = reinterpret_cast<T*>(w.Wdh);
     T* Wdl = reinterpret_cast<T*>(w.Wdl);
     if (A.doIrr) {
-        hipLaunchKernelGGL(weights_lowp_kernel<T>, dim3(tpI / 16), dim3(256), 0, st, w.dirs0, A.R, Rp, A.ni, tpI, 0,
+        hipLaunchKernelGGL(weights_lowp_kernel<T>, dim3(tpI / 16, (Rp + kWChunk - 1) / kWChunk), dim3(256), 0, st,
+                           w.dirs0, A.R, Rp, A.ni, tpI, 0,
                            A.sharpness, SPLIT ? 1 : 0, Wih, Wil, w.Sih);
         LUXGI_LAUNCH_CHECK();
     }
     if (A.doDepth) {
-        hipLaunchKernelGGL(weights_lowp_kernel<T>, dim3(tpD / 16), dim3(256), 0, st, w.dirs0, A.R, Rp, A.nd, tpD, 1,
+        hipLaunchKernelGGL(weights_lowp_kernel<T>, dim3(tpD / 16, (Rp + kWChunk - 1) / kWChunk), dim3(256), 0, st,
+                           w.dirs0, A.R, Rp, A.nd, tpD, 1,
                            A.sharpness, SPLIT ? 1 : 0, Wdh, Wdl, w.Sdh);
         LUXGI_LAUNCH_CHECK();
     }
@@ -826,9 +975,8 @@ static int launch_update(const UpdateArgs& A, const LuxgiRayBuffers* rays, const
         // probes whose directions differ from row 0: reference arithmetic with their own directions
         UpdateArgs F = A;
         F.forcePerProbe = 1;
-        const int groups = (A.P + kProbesPerBlock - 1) / kProbesPerBlock;
-        hipLaunchKernelGGL((probe_update_kernel<SR, SA>), dim3(std::min(groups, 512)), dim3(256), 0, st, F, rad, dd, pI,
-                           pD, oI, oD, nullptr, nullptr, w.mismatch);
+        hipLaunchKernelGGL((fixup_kernel<SR, SA>), dim3((A.P + 255) / 256), dim3(256), 0, st, F, rad, dd, pI, pD, oI,
+                           oD, w.mismatch);
         LUXGI_LAUNCH_CHECK();
         return LUXGI_OK;
     }
@@ -892,13 +1040,14 @@ extern "C" int luxgi_ddgi_probe_update_noalloc(const LuxgiDDGIUniform* u, const 
     return luxgi_ddgi_probe_update(u, rays, prev, out, firstFrame, flags, ws, need, stream);
 }
 
-extern "C" int luxgi_ddgi_probe_update(const LuxgiDDGIUniform* u, const LuxgiRayBuffers* rays,
-                                       const LuxgiProbeAtlas* prev, const LuxgiProbeAtlas* out, uint32_t firstFrame,
-                                       uint32_t flags, void* workspace, size_t workspaceBytes, void* stream) {
+// The checked arguments of one update (luxgi_ddgi_probe_update's rules); LUXGI_OK with nothing to do
+// when neither map is requested (A.doIrr = A.doDepth = 0).
+static int make_update_args(const LuxgiDDGIUniform* u, const LuxgiRayBuffers* rays, const LuxgiProbeAtlas* prev,
+                            const LuxgiProbeAtlas* out, uint32_t firstFrame, uint32_t flags, UpdateArgs& A) {
     int rc = check_uniform(u);
     if (rc) return rc;
     if (!rays || !out || !rays->directionDistance) return LUXGI_ERR_INVALID_ARGUMENT;
-    UpdateArgs A{};
+    A = UpdateArgs{};
     A.px = u->probeCounts[0];
     A.py = u->probeCounts[1];
     A.pz = u->probeCounts[2];
@@ -930,6 +1079,16 @@ extern "C" int luxgi_ddgi_probe_update(const LuxgiDDGIUniform* u, const LuxgiRay
     if (!A.firstFrame && (!prev || (A.doIrr && !prev->irradiance) || (A.doDepth && !prev->depth)))
         return LUXGI_ERR_INVALID_ARGUMENT;
     if (prev && !A.firstFrame && prev->storage != out->storage) return LUXGI_ERR_INVALID_ARGUMENT;
+    return LUXGI_OK;
+}
+
+extern "C" int luxgi_ddgi_probe_update(const LuxgiDDGIUniform* u, const LuxgiRayBuffers* rays,
+                                       const LuxgiProbeAtlas* prev, const LuxgiProbeAtlas* out, uint32_t firstFrame,
+                                       uint32_t flags, void* workspace, size_t workspaceBytes, void* stream) {
+    UpdateArgs A;
+    const int rc = make_update_args(u, rays, prev, out, firstFrame, flags, A);
+    if (rc) return rc;
+    if (!A.doIrr && !A.doDepth) return LUXGI_OK;
     const size_t need = update_workspace_layout(u, nullptr, nullptr);
     if (!workspace || workspaceBytes < need) return LUXGI_ERR_WORKSPACE_TOO_SMALL;
     UpdateWorkspace w;
@@ -943,6 +1102,131 @@ extern "C" int luxgi_ddgi_probe_update(const LuxgiDDGIUniform* u, const LuxgiRay
         return launch_update<float, __half>(A, rays, prev, out, w, st);
     if (rs == LUXGI_STORAGE_FP16 && as == LUXGI_STORAGE_FP32)
         return launch_update<__half, float>(A, rays, prev, out, w, st);
+    return LUXGI_ERR_UNSUPPORTED;
+}
+
+// ------------------------------------------------------------------------- several volumes
+
+static bool batchable(const UpdateArgs& A) {  // the shared-direction MFMA path
+    return !A.forcePerProbe && A.ni * A.ni <= 64 && A.nd * A.nd <= 256 && A.maxDistance > 0.0f;
+}
+
+extern "C" size_t luxgi_ddgi_update_batch_descriptor_bytes(int32_t count) {
+    if (count <= 0 || count > LUXGI_MAX_UPDATE_BATCH) return 0;
+    return (size_t)count * sizeof(UpdateVolumeDesc);
+}
+
+extern "C" size_t luxgi_ddgi_update_batch_workspace_size(const LuxgiUpdateVolume* volumes, int32_t count) {
+    if (!volumes || count <= 0 || count > LUXGI_MAX_UPDATE_BATCH) return 0;
+    size_t total = 0;
+    for (int k = 0; k < count; ++k) total += align256(update_workspace_layout(&volumes[k].uniform, nullptr, nullptr));
+    return total;
+}
+
+extern "C" int luxgi_ddgi_update_batch_prepare(const LuxgiUpdateVolume* volumes, int32_t count, uint32_t flags,
+                                               void* workspace, size_t workspaceBytes, void* descriptors,
+                                               LuxgiUpdateBatch* batch) {
+    if (!volumes || !descriptors || !batch || count <= 0 || count > LUXGI_MAX_UPDATE_BATCH)
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    const size_t need = luxgi_ddgi_update_batch_workspace_size(volumes, count);
+    if (!workspace || workspaceBytes < need) return LUXGI_ERR_WORKSPACE_TOO_SMALL;
+    UpdateVolumeDesc* out = static_cast<UpdateVolumeDesc*>(descriptors);
+    char* base = static_cast<char*>(workspace);  // device address: only offsets are taken here
+    int groups = 0, maxR = 0, maxTiles = 0, maxChunks = 0;
+    for (int k = 0; k < count; ++k) {
+        const LuxgiUpdateVolume& v = volumes[k];
+        UpdateArgs A;
+        const int rc = make_update_args(&v.uniform, &v.rays, &v.prev, &v.out, v.firstFrame, flags, A);
+        if (rc) return rc;
+        if (!A.doIrr && !A.doDepth) return LUXGI_ERR_INVALID_ARGUMENT;
+        if (!batchable(A)) return LUXGI_ERR_UNSUPPORTED;
+        if (v.rays.storage != volumes[0].rays.storage || v.out.storage != volumes[0].out.storage)
+            return LUXGI_ERR_INVALID_ARGUMENT;
+        UpdateWorkspace w;
+        const size_t bytes = update_workspace_layout(&v.uniform, base, &w);
+        base += align256(bytes);
+        UpdateVolumeDesc& D = out[k];
+        D = UpdateVolumeDesc{};
+        D.A = A;
+        D.rad = v.rays.radiance;
+        D.dd = v.rays.directionDistance;
+        D.prevI = v.prev.irradiance;
+        D.prevD = v.prev.depth;
+        D.outI = v.out.irradiance;
+        D.outD = v.out.depth;
+        D.dirs0 = w.dirs0;
+        D.Wih = w.Wih;
+        D.Wil = w.Wil;
+        D.Wdh = w.Wdh;
+        D.Wdl = w.Wdl;
+        D.Sih = w.Sih;
+        D.Sdh = w.Sdh;
+        D.mismatch = w.mismatch;
+        D.tpI = pad16(A.ni * A.ni);
+        D.tpD = pad16(A.nd * A.nd);
+        D.Rp = pad32(A.R);
+        D.groupStart = groups;
+        groups += (A.P + kMG - 1) / kMG;
+        maxR = std::max(maxR, A.R);
+        maxTiles = std::max(maxTiles, std::max(D.tpI, D.tpD) / 16);
+        maxChunks = std::max(maxChunks, (D.Rp + kWChunk - 1) / kWChunk);
+    }
+    batch->count = count;
+    batch->groups = groups;
+    batch->flags = (int32_t)flags;
+    batch->rayStorage = volumes[0].rays.storage;
+    batch->atlasStorage = volumes[0].out.storage;
+    batch->maxRays = maxR;
+    batch->maxWeightBlocks = maxTiles * maxChunks;
+    batch->pad = 0;
+    return LUXGI_OK;
+}
+
+template <class SR, class SA, class T, bool SPLIT>
+static int launch_batched(const LuxgiUpdateBatch& b, const UpdateVolumeDesc* d, hipStream_t st) {
+    const bool doIrr = (b.flags & LUXGI_UPDATE_IRRADIANCE) != 0, doDepth = (b.flags & LUXGI_UPDATE_DEPTH) != 0;
+    hipLaunchKernelGGL(row0_batched_kernel<SR>, dim3((b.maxRays + 255) / 256, b.count), dim3(256), 0, st, d);
+    LUXGI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(weights_batched_kernel<T>, dim3(b.maxWeightBlocks, b.count, 2), dim3(256), 0, st, d,
+                       SPLIT ? 1 : 0);
+    LUXGI_LAUNCH_CHECK();
+    if (doDepth) {
+        hipLaunchKernelGGL((probe_blend_map_batched_kernel<SR, SA, T, SPLIT, 0, false>), dim3(b.groups), dim3(256), 0,
+                           st, d, b.count);
+        LUXGI_LAUNCH_CHECK();
+    }
+    if (doIrr) {
+        if (doDepth)
+            hipLaunchKernelGGL((probe_blend_map_batched_kernel<SR, SA, T, SPLIT, 1, false>), dim3(b.groups), dim3(256),
+                               0, st, d, b.count);
+        else
+            hipLaunchKernelGGL((probe_blend_map_batched_kernel<SR, SA, T, SPLIT, 1, true>), dim3(b.groups), dim3(256),
+                               0, st, d, b.count);
+        LUXGI_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL((fixup_batched_kernel<SR, SA>), dim3(b.groups), dim3(256), 0, st, d, b.count);
+    LUXGI_LAUNCH_CHECK();
+    return LUXGI_OK;
+}
+
+template <class SR, class SA>
+static int launch_batched_forms(const LuxgiUpdateBatch& b, const UpdateVolumeDesc* d, hipStream_t st) {
+    if (b.flags & LUXGI_UPDATE_BF16) return launch_batched<SR, SA, __bf16, false>(b, d, st);
+    if (b.flags & LUXGI_UPDATE_HALF) return launch_batched<SR, SA, _Float16, false>(b, d, st);
+    return launch_batched<SR, SA, _Float16, true>(b, d, st);
+}
+
+extern "C" int luxgi_ddgi_probe_update_batched(const LuxgiUpdateBatch* batch, const void* descriptors, void* stream) {
+    if (!batch || !descriptors || batch->count <= 0 || batch->count > LUXGI_MAX_UPDATE_BATCH || batch->groups <= 0 ||
+        batch->maxRays <= 0 || batch->maxWeightBlocks <= 0)
+        return LUXGI_ERR_INVALID_ARGUMENT;
+    const UpdateVolumeDesc* d = static_cast<const UpdateVolumeDesc*>(descriptors);
+    hipStream_t st = as_stream(stream);
+    const int rs = batch->rayStorage, as = batch->atlasStorage;
+    if (rs == LUXGI_STORAGE_FP32 && as == LUXGI_STORAGE_FP32) return launch_batched_forms<float, float>(*batch, d, st);
+    if (rs == LUXGI_STORAGE_FP16 && as == LUXGI_STORAGE_FP16) return launch_batched_forms<__half, __half>(*batch, d, st);
+    if (rs == LUXGI_STORAGE_FP32 && as == LUXGI_STORAGE_FP16) return launch_batched_forms<float, __half>(*batch, d, st);
+    if (rs == LUXGI_STORAGE_FP16 && as == LUXGI_STORAGE_FP32) return launch_batched_forms<__half, float>(*batch, d, st);
     return LUXGI_ERR_UNSUPPORTED;
 }
 

@@ -19,7 +19,7 @@
 //   - The history copy after iteration 1 (RaytracedReflection.cpp:488-491) is a
 //     second store of the same a-trous output.
 // texelFetch outside the image returns 0 (robust image access), as in the oracle.
-#include "common.hpp"
+#include "svgf_common.hpp"
 
 namespace luxgi {
 
@@ -33,21 +33,6 @@ struct GBuf {
     const float* depth;
 };
 
-__device__ __forceinline__ bool inside(int x, int y, int W, int H) { return x >= 0 && y >= 0 && x < W && y < H; }
-__device__ __forceinline__ float4 fetch4(const float4* p, int x, int y, int W, int H) {
-    return inside(x, y, W, H) ? p[(size_t)y * W + x] : make_float4(0.f, 0.f, 0.f, 0.f);
-}
-__device__ __forceinline__ float fetch1(const float* p, int x, int y, int W, int H) {
-    return inside(x, y, W, H) ? p[(size_t)y * W + x] : 0.0f;
-}
-template <class S>
-__device__ __forceinline__ float4 fetchS4(const S* p, int x, int y, int W, int H) {
-    return inside(x, y, W, H) ? Texel<S>::ld4(p, (size_t)y * W + x) : make_float4(0.f, 0.f, 0.f, 0.f);
-}
-template <class S>
-__device__ __forceinline__ float2 fetchS2(const S* p, int x, int y, int W, int H) {
-    return inside(x, y, W, H) ? Texel<S>::ld2(p, (size_t)y * W + x) : make_float2(0.f, 0.f);
-}
 
 // isValid (Reprojection.glsl:42-53)
 __device__ __forceinline__ bool is_valid(int hx, int hy, f3 pos, f3 prevPos, f3 n, f3 pn, float meshId,
@@ -60,9 +45,6 @@ __device__ __forceinline__ bool is_valid(int hx, int hy, f3 pos, f3 prevPos, f3 
     return true;
 }
 
-__device__ __forceinline__ float luminance(float r, float g, float b) {  // ReflectionCommon.glsl:19-22
-    return fmaxf(r * 0.299f + g * 0.587f + b * 0.114f, 0.0001f);
-}
 
 // ---------------------------------------------------------------------------
 // reflection reprojection
@@ -293,36 +275,6 @@ __global__ __launch_bounds__(256) void reflection_reproject_kernel(ReflArgs A, G
     reproject_block<S>(A, cur, prev, noisy, histOut, histMom, outColor, outMom, tileMap, blockIdx.x, blockIdx.y, sm);
 }
 
-// computeEdgeStoppingWeight (EdgeStop.glsl:24-40) with the per-pixel constants folded:
-// invPhiZ = 1/sigmaDepth, invPhiL = 1/phiLuma, and the normal power max(0, n.n')^phiN
-// evaluated by `sqN` squarings when phiN = 2^sqN (the reference's 32), else by powf.
-// exp(x) is evaluated as exp2(x * log2(e)) (v_exp_f32); the relative error stays below 1e-6
-// for the arguments here, well inside the driver-ULP freedom of GLSL exp/pow.
-__device__ __forceinline__ float pow_normal(float c, float phiN, int sqN) {
-    if (sqN < 0) return powf(c, phiN);
-    for (int i = 0; i < sqN; ++i) c *= c;
-    return c;
-}
-__device__ __forceinline__ float edge_stop(float cd, float sd, float invPhiZ, f3 cn, f3 sn, float phiN, int sqN,
-                                           float cl, float sl, float invPhiL) {
-    const float wZ = __expf(-fabsf(cd - sd) * invPhiZ);
-    const float wN = pow_normal(clampf(dot3(cn, sn), 0.0f, 1.0f), phiN, sqN);
-    const float wL = fabsf(cl - sl) * invPhiL;
-    return __expf(0.0f - fmaxf(wL, 0.0f) - fmaxf(wZ, 0.0f)) * wN;
-}
-
-struct AtrousArgs {
-    int W, H, tilesX, radius, stepSize;
-    float phi, phiNormal, sigmaDepth, power;
-    int sqNormal;  // log2(phiNormal) when it is a power of two in [1, 64], else -1
-};
-
-// squarings that evaluate x^phiNormal exactly as repeated products, or -1
-__host__ __device__ inline int normal_squarings(float phiN) {
-    for (int k = 0; k <= 6; ++k)
-        if (phiN == float(1 << k)) return k;
-    return -1;
-}
 
 // Reflection/DenoiseAtrous.comp:63-134 for one filtered pixel. tap(px, py, sc, sn, d) gives the
 // input, decoded normal and linear depth of an in-image pixel. The sums run in one fixed order,
@@ -334,6 +286,7 @@ __device__ __forceinline__ float4 reflection_filter(const AtrousArgs& A, int x, 
     const float cl = luminance(c.x, c.y, c.z);
     const float phiC = A.phi * sqrtf(fmaxf(0.0f, var)) + 1e-10f;
     const float invPhiC = 1.0f / phiC, invPhiZ = 1.0f / A.sigmaDepth;
+    const float invPhiL = A.lumScale ? invPhiC * A.lumScale[(size_t)y * A.W + x] : invPhiC;
     float sw = 1.0f;
     float4 s = c;
     for (int yy = -A.radius; yy <= A.radius; ++yy)
@@ -346,7 +299,7 @@ __device__ __forceinline__ float4 reflection_filter(const AtrousArgs& A, int x, 
                 tap(px, py, sc, sn, d);
                 const float k = kw[abs(xx)] * kw[abs(yy)];
                 const float sl = luminance(sc.x, sc.y, sc.z);
-                const float w = edge_stop(cd, d, invPhiZ, cn, sn, A.phiNormal, A.sqNormal, cl, sl, invPhiC);
+                const float w = edge_stop(cd, d, invPhiZ, cn, sn, A.phiNormal, A.sqNormal, cl, sl, invPhiL);
                 const float wv = w * k;
                 sw += wv;
                 s.x += wv * sc.x;
@@ -358,16 +311,6 @@ __device__ __forceinline__ float4 reflection_filter(const AtrousArgs& A, int x, 
     return make_float4(s.x / sw, s.y / sw, s.z / sw, s.w / (sw * sw));
 }
 
-__device__ __forceinline__ f3 load_normal(const float4* normals, size_t q) {
-    const float2 o = reinterpret_cast<const float2*>(normals)[2 * q];  // octahedral xy only
-    return oct_decode(o.x, o.y);
-}
-__device__ __forceinline__ float load_linear_depth(const float4* pbr, size_t q) {
-    return reinterpret_cast<const float*>(pbr)[4 * q + 3];
-}
-__device__ __forceinline__ float load_roughness(const float4* pbr, size_t q) {
-    return reinterpret_cast<const float*>(pbr)[4 * q + 1];
-}
 
 // Direct form, one 8x8 tile per 64-lane block: every tap read from global memory. Used when the
 // footprint radius * stepSize exceeds the LDS halo.
@@ -1000,15 +943,15 @@ extern "C" int luxgi_svgf_reflection_reproject(const LuxgiReflectionReprojectPar
     return LUXGI_ERR_UNSUPPORTED;
 }
 
-extern "C" int luxgi_svgf_reflection_atrous(const LuxgiReflectionAtrousParams* p, int32_t W, int32_t H,
-                                            int32_t storage, const LuxgiGBuffer* cur, const uint8_t* tileMap,
-                                            const void* input, void* output, void* historyOut, void* stream) {
+static int refl_atrous(const LuxgiReflectionAtrousParams* p, int32_t W, int32_t H, int32_t storage,
+                       const LuxgiGBuffer* cur, const uint8_t* tileMap, const void* input, void* output,
+                       void* historyOut, const float* lumScale, void* stream) {
     if (!p || W <= 0 || H <= 0 || !cur || !cur->normals || !cur->pbr || !tileMap || !input || !output)
         return LUXGI_ERR_INVALID_ARGUMENT;
     if (p->radius < 0 || p->radius > 2) return LUXGI_ERR_UNSUPPORTED;  // kernelWeights has 3 entries
     if (p->stepSize < 1 || p->stepSize > (1 << 16)) return LUXGI_ERR_INVALID_ARGUMENT;
     AtrousArgs A{W, H, (W + 7) / 8, p->radius, p->stepSize, p->phiColor, p->phiNormal, p->sigmaDepth, 0.0f,
-                 normal_squarings(p->phiNormal)};
+                 normal_squarings(p->phiNormal), lumScale};
     if (storage != LUXGI_STORAGE_FP32 && storage != LUXGI_STORAGE_FP16) return LUXGI_ERR_UNSUPPORTED;
     hipStream_t st = as_stream(stream);
     const float4* nrm = static_cast<const float4*>(static_cast<const void*>(cur->normals));
@@ -1021,6 +964,20 @@ extern "C" int luxgi_svgf_reflection_atrous(const LuxgiReflectionAtrousParams* p
                               tileMap, input, output, historyOut);
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
+}
+
+extern "C" int luxgi_svgf_reflection_atrous(const LuxgiReflectionAtrousParams* p, int32_t W, int32_t H,
+                                            int32_t storage, const LuxgiGBuffer* cur, const uint8_t* tileMap,
+                                            const void* input, void* output, void* historyOut, void* stream) {
+    return refl_atrous(p, W, H, storage, cur, tileMap, input, output, historyOut, nullptr, stream);
+}
+
+extern "C" int luxgi_svgf_reflection_atrous_scaled(const LuxgiReflectionAtrousParams* p, int32_t W, int32_t H,
+                                                   int32_t storage, const LuxgiGBuffer* cur, const uint8_t* tileMap,
+                                                   const void* input, void* output, void* historyOut,
+                                                   const float* lumScale, void* stream) {
+    if (!lumScale) return LUXGI_ERR_INVALID_ARGUMENT;
+    return refl_atrous(p, W, H, storage, cur, tileMap, input, output, historyOut, lumScale, stream);
 }
 
 extern "C" int luxgi_svgf_reflection_denoise(const LuxgiReflectionReprojectParams* rp,

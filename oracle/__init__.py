@@ -83,6 +83,12 @@ def _declare(L):
         getattr(L, "oracle_sdf_rasterize_" + s_).argtypes = [P, P, P, P, I, P, P, P, I, I, I, P, I]
     L.oracle_learned_atrous_forward.argtypes = [P, P, I, I, P, P, P, P, P, P]
     L.oracle_learned_atrous_backward.argtypes = [P, P, I, I, P, P, P, P, P, P]
+    L.oracle_reflection_atrous_scaled_f64.argtypes = [P, I, I, P, P, P, P, P, P, I]
+    L.oracle_learned_pixel_features.argtypes = [I, I, P, P, P, P, P, I]
+    L.oracle_learned_pixel_mlp.argtypes = [P, I, P, I, P, P, I]
+    L.oracle_learned_pixel_mlp_backward.argtypes = [P, I, P, I, P, P]
+    L.oracle_learned_pixel_atrous_backward.argtypes = [P, I, I, P, P, P, P, P, P, P, P]
+    L.oracle_learned_pixel_param_count.restype = I
     L.oracle_round_to_half.argtypes = [C.c_float]
     L.oracle_round_to_half.restype = C.c_float
     L.oracle_sdf_reflection_f64.argtypes = [P, P, P, I, I, P, P, P, P, P, P, I, I, P, P, P, P, P, P, P, P, I, P, P, P,
@@ -417,9 +423,9 @@ def shadow_mask(params, sobol, scrambling, W, H, depth, normals, sdf_data, sdf, 
     out = np.zeros((mh, mw), dtype=np.uint32) if mask is None else np.ascontiguousarray(mask, dtype=np.uint32).copy()
     so = np.ascontiguousarray(sobol, dtype=np.uint8)
     sc = np.ascontiguousarray(scrambling, dtype=np.uint8)
-    lib().oracle_shadow_mask_f64(C.byref(params), _p(so), _p(sc), int(W), int(H), _p(_f32(depth)), _p(_f32(normals)),
-                                 C.byref(sdf_data), _p(_f32(sdf)), _p(_f32(mip)), int(res), int(mip_res), _p(out),
-                                 nthreads or threads())
+    dp, nrm, sd, mp = _f32(depth), _f32(normals), _f32(sdf), _f32(mip)  # held for the call
+    lib().oracle_shadow_mask_f64(C.byref(params), _p(so), _p(sc), int(W), int(H), _p(dp), _p(nrm), C.byref(sdf_data),
+                                 _p(sd), _p(mp), int(res), int(mip_res), _p(out), nthreads or threads())
     return out
 
 
@@ -438,8 +444,9 @@ def sdf_reflection(params, sobol, scrambling, W, H, normals, pbr, depth, sdf_dat
     irr = None if irradiance is None else _f32(irradiance)
     dep = None if depth_atlas is None else _f32(depth_atlas)
     off, st = _reloc(offsets, states)
-    lib().oracle_sdf_reflection_f64(C.byref(params), _p(so), _p(sc), int(W), int(H), _p(_f32(normals)), _p(_f32(pbr)),
-                                    _p(_f32(depth)), C.byref(sdf_data), _p(_f32(sdf)), _p(_f32(mip)), int(res),
+    nrm, pb, dp, sd, mp = _f32(normals), _f32(pbr), _f32(depth), _f32(sdf), _f32(mip)  # held for the call
+    lib().oracle_sdf_reflection_f64(C.byref(params), _p(so), _p(sc), int(W), int(H), _p(nrm), _p(pb),
+                                    _p(dp), C.byref(sdf_data), _p(sd), _p(mp), int(res),
                                     int(mip_res), C.byref(atlas.data), _p(chunks), _p(culled),
                                     C.cast(atlas.objects, C.c_void_p), C.cast(atlas.tiles, C.c_void_p), _p(adep),
                                     _p(acol), _p(sky), int(sky_size), C.byref(u) if u is not None else None, _p(irr),
@@ -482,9 +489,9 @@ def learned_atrous_forward(params, weights, normals, pbr, image, color=None):
     out = np.zeros((H, W, 4))
     sw = np.zeros((H, W))
     wts = np.ascontiguousarray(weights, dtype=np.float64)
-    col = image if color is None else color
-    lib().oracle_learned_atrous_forward(C.byref(params), _p(wts), W, H, _p(_f32(normals)), _p(_f32(pbr)),
-                                        _p(_f32(image)), _p(_f32(col)), _p(out), _p(sw))
+    nrm, pb, img, col = _f32(normals), _f32(pbr), _f32(image), _f32(image if color is None else color)
+    lib().oracle_learned_atrous_forward(C.byref(params), _p(wts), W, H, _p(nrm), _p(pb), _p(img), _p(col), _p(out),
+                                        _p(sw))
     return out, sw
 
 
@@ -495,9 +502,77 @@ def learned_atrous_backward(params, weights, normals, pbr, image, grad_out):
     gp = np.zeros(81)
     wts = np.ascontiguousarray(weights, dtype=np.float64)
     g = np.ascontiguousarray(grad_out, dtype=np.float64)
-    lib().oracle_learned_atrous_backward(C.byref(params), _p(wts), W, H, _p(_f32(normals)), _p(_f32(pbr)),
-                                         _p(_f32(image)), _p(g), _p(gin), _p(gp))
+    nrm, pb, img = _f32(normals), _f32(pbr), _f32(image)
+    lib().oracle_learned_atrous_backward(C.byref(params), _p(wts), W, H, _p(nrm), _p(pb), _p(img), _p(g), _p(gin),
+                                         _p(gp))
     return gin, gp
+
+
+# ---------------------------------------------------------------------------
+# Per-pixel learned edge-stopping scales (12 -> 32 -> 32 -> 5 MLP)
+# ---------------------------------------------------------------------------
+LEARNED_PIXEL_FEATURES, LEARNED_PIXEL_HIDDEN, LEARNED_PIXEL_OUT = 12, 32, 5
+
+
+def learned_pixel_param_count():
+    return lib().oracle_learned_pixel_param_count()
+
+
+def reflection_atrous_scaled(params, W, H, normals, pbr, tile_map, inp, scale=None, nthreads=None):
+    """One reflection a-trous pass (FP64) whose taps' luminance term is scaled by scale[y][x]
+    (None: the reference pass)."""
+    out = np.zeros((H, W, 4))
+    # converted arrays are held in locals: a temporary's buffer must outlive the call
+    nrm, pb, img = _f32(normals), _f32(pbr), _f32(inp)
+    tm = np.ascontiguousarray(tile_map, dtype=np.uint8)
+    sc = None if scale is None else _f32(scale)
+    lib().oracle_reflection_atrous_scaled_f64(C.byref(params), W, H, _p(nrm), _p(pb), _p(tm), _p(img), _p(sc), _p(out),
+                                              nthreads or threads())
+    return out
+
+
+def learned_pixel_features(color, moments, normals, pbr, nthreads=None):
+    """[H][W][12] features (FP64) of the pass-0 input, the reprojection moments and the G-buffer."""
+    H, W = color.shape[:2]
+    out = np.zeros((H, W, LEARNED_PIXEL_FEATURES))
+    c, m, n, pb = _f32(color), _f32(moments), _f32(normals), _f32(pbr)
+    lib().oracle_learned_pixel_features(W, H, _p(c), _p(m), _p(n), _p(pb), _p(out), nthreads or threads())
+    return out
+
+
+def learned_pixel_mlp(params, features, half=False, nthreads=None):
+    """(o, s) per pixel, [..., 5] each; `half` rounds the products' operands to FP16 as the GPU's
+    MFMA forward does."""
+    f = np.ascontiguousarray(features, dtype=np.float64)
+    n = f.size // LEARNED_PIXEL_FEATURES
+    o = np.zeros(f.shape[:-1] + (LEARNED_PIXEL_OUT,))
+    s = np.zeros_like(o)
+    prm = np.ascontiguousarray(params, dtype=np.float64)
+    lib().oracle_learned_pixel_mlp(_p(prm), n, _p(f), int(half), _p(o), _p(s), nthreads or threads())
+    return o, s
+
+
+def learned_pixel_mlp_backward(params, features, grad_scale, half=False):
+    """dL/dparams from dL/ds ([..., 5] per pixel)."""
+    f = np.ascontiguousarray(features, dtype=np.float64)
+    n = f.size // LEARNED_PIXEL_FEATURES
+    gp = np.zeros(learned_pixel_param_count())
+    prm = np.ascontiguousarray(params, dtype=np.float64)
+    gs = np.ascontiguousarray(grad_scale, dtype=np.float64)
+    lib().oracle_learned_pixel_mlp_backward(_p(prm), n, _p(f), int(half), _p(gs), _p(gp))
+    return gp
+
+
+def learned_pixel_atrous_backward(params, W, H, normals, pbr, tile_map, inp, scale, grad_out):
+    """(dL/d input [H][W][4] (rgb; 0 in the variance channel), dL/d scale [H][W]) of one scaled pass."""
+    gin = np.zeros((H, W, 4))
+    gsc = np.zeros((H, W))
+    nrm, pb, img, sc = _f32(normals), _f32(pbr), _f32(inp), _f32(scale)
+    tm = np.ascontiguousarray(tile_map, dtype=np.uint8)
+    g = np.ascontiguousarray(grad_out, dtype=np.float64)
+    lib().oracle_learned_pixel_atrous_backward(C.byref(params), W, H, _p(nrm), _p(pb), _p(tm), _p(img), _p(sc), _p(g),
+                                               _p(gin), _p(gsc))
+    return gin, gsc
 
 
 # ---------------------------------------------------------------------------

@@ -875,9 +875,11 @@ inline T edge_stop(T cd, T sd, T phiZ, v3<T> cn, v3<T> sn, T phiN, T cl, T sl, T
 }
 
 // Reflection/DenoiseAtrous.comp:63-134 for one pixel of a denoise tile.
+// `lumScale` multiplies the luminance term of every tap's weight (the learned per-pixel scale; 1 is
+// the reference pass).
 template <class T>
 void reflection_atrous_pixel(const LuxgiReflectionAtrousParams& pc, int W, int H, int x, int y, Img normals, Img pbr,
-                             Img in, T* out) {
+                             Img in, T* out, T lumScale = T(1)) {
     const T kw[3] = {T(1.0), T(2.0 / 3.0), T(1.0 / 6.0)};
     v4<T> c = fetch<T>(in, W, H, x, y);
     T cl = luminance(c.x, c.y, c.z);
@@ -905,7 +907,7 @@ void reflection_atrous_pixel(const LuxgiReflectionAtrousParams& pc, int W, int H
                 v4<T> sn = fetch<T>(normals, W, H, px, py);
                 T d = fetch<T>(pbr, W, H, px, py).w;
                 T w = edge_stop<T>(cd, d, T(pc.sigmaDepth), cn, octahedral_to_direction<T>(sn.x, sn.y),
-                                   T(pc.phiNormal), cl, sl, phiC);
+                                   T(pc.phiNormal), cl, sl, phiC / lumScale);
                 T wv = w * k;
                 sw += wv;
                 s[0] += wv * sc.x; s[1] += wv * sc.y; s[2] += wv * sc.z; s[3] += wv * wv * sc.w;
@@ -1764,6 +1766,197 @@ inline void la_backward(const LearnedAtrousImage& im, const LuxgiLearnedAtrousPa
                         for (int i = 0; i < 6; ++i) gP[j * 6 + i] += dh * f[i];
                     }
                 }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Per-pixel learned edge-stopping scales (no counterpart in the reference). A 12 -> 32 -> 32 -> 5
+// ReLU MLP of per-pixel features gives, for each of the five a-trous passes
+// (Reflection/DenoiseAtrous.comp:63-134), a scale s_i(p) = exp(clamp(o_i, -4, 4)) of the luminance
+// term of the edge-stopping weight (EdgeStop.glsl:24-40): wL = s_i(p) |l_p - l_q| / phi_p. s = 1
+// is the reference pass. Parameters, packed: W1 [32][12], b1 [32], W2 [32][32], b2 [32],
+// W3 [5][32], b3 [5] (1637 floats).
+// ---------------------------------------------------------------------------
+constexpr int kLpIn = 12, kLpHid = 32, kLpOut = 5;
+constexpr int kLpW1 = 0, kLpB1 = kLpW1 + kLpHid * kLpIn, kLpW2 = kLpB1 + kLpHid, kLpB2 = kLpW2 + kLpHid * kLpHid,
+              kLpW3 = kLpB2 + kLpHid, kLpB3 = kLpW3 + kLpOut * kLpHid, kLpParams = kLpB3 + kLpOut;
+constexpr double kLpClamp = 4.0;
+
+// The 12 features of pixel (x, y) from the pass-0 input (rgb + variance), the reprojection's
+// moments (m1, m2, history length), the G-buffer normal (octahedral xy) and pbr (.w linear depth):
+//   f0, f1  |z(x+1) - z(x-1)| / 2z and the same along y (edge-clamped neighbours), at most 1
+//   f2..f4  the decoded normal
+//   f5      log(1 + l), l the luminance of the input
+//   f6, f7  log(1 + mean) and min(std / (mean + 1e-3), 4) of the luminance over the 3x3 neighbourhood
+//   f8      min(sqrt(variance) / (l + 1e-3), 4)
+//   f9, f10 log(1 + m1) and min(sqrt(max(0, m2 - m1^2)) / (m1 + 1e-3), 4)
+//   f11     min(history length, 32) / 32
+template <class T>
+void learned_pixel_features(int W, int H, int x, int y, Img color, Img moments, Img normals, Img pbr, T* f) {
+    auto cx = [&](int v) { return std::min(std::max(v, 0), W - 1); };
+    auto cy = [&](int v) { return std::min(std::max(v, 0), H - 1); };
+    const T z = fetch<T>(pbr, W, H, x, y).w;
+    const T iz = T(1) / std::max(z, T(1e-4f));
+    const T zl = fetch<T>(pbr, W, H, cx(x - 1), y).w, zr = fetch<T>(pbr, W, H, cx(x + 1), y).w;
+    const T zu = fetch<T>(pbr, W, H, x, cy(y - 1)).w, zd = fetch<T>(pbr, W, H, x, cy(y + 1)).w;
+    f[0] = std::min(std::fabs(zr - zl) * T(0.5f) * iz, T(1));
+    f[1] = std::min(std::fabs(zd - zu) * T(0.5f) * iz, T(1));
+    const v4<T> nr = fetch<T>(normals, W, H, x, y);
+    const v3<T> n = octahedral_to_direction<T>(nr.x, nr.y);
+    f[2] = n.x;
+    f[3] = n.y;
+    f[4] = n.z;
+    const v4<T> c = fetch<T>(color, W, H, x, y);
+    const T l = luminance(c.x, c.y, c.z);
+    f[5] = std::log1p(l);
+    T s1 = 0, s2 = 0;
+    for (int yy = -1; yy <= 1; ++yy)
+        for (int xx = -1; xx <= 1; ++xx) {
+            const v4<T> q = fetch<T>(color, W, H, cx(x + xx), cy(y + yy));
+            const T lq = luminance(q.x, q.y, q.z);
+            s1 += lq;
+            s2 += lq * lq;
+        }
+    const T mu = s1 / T(9);
+    const T sd = std::sqrt(std::max(T(0), s2 / T(9) - mu * mu));
+    f[6] = std::log1p(mu);
+    f[7] = std::min(sd / (mu + T(1e-3f)), T(4));
+    f[8] = std::min(std::sqrt(std::max(c.w, T(0))) / (l + T(1e-3f)), T(4));
+    const v4<T> m = fetch<T>(moments, W, H, x, y);
+    const T m1 = std::max(m.x, T(0));
+    f[9] = std::log1p(m1);
+    f[10] = std::min(std::sqrt(std::max(T(0), m.y - m1 * m1)) / (m1 + T(1e-3f)), T(4));
+    f[11] = std::min(m.z, T(32)) / T(32);
+}
+
+// The MLP on one feature vector: pre-activations a1, a2, hidden h1, h2, outputs o and scales s.
+// `half` rounds the parameters, the features and the hidden activations to FP16 before each
+// product (the MFMA operands of the GPU forward); sums are exact in T.
+template <class T>
+struct LpForward {
+    T a1[kLpHid], h1[kLpHid], a2[kLpHid], h2[kLpHid], o[kLpOut], s[kLpOut];
+};
+template <class T>
+void learned_pixel_mlp(const double* P, const T* f, bool half, LpForward<T>& r) {
+    auto q = [&](T v) { return half ? T(round_to_half(float(v))) : v; };
+    for (int j = 0; j < kLpHid; ++j) {
+        T a = T(P[kLpB1 + j]);
+        for (int i = 0; i < kLpIn; ++i) a += q(T(P[kLpW1 + j * kLpIn + i])) * q(f[i]);
+        r.a1[j] = a;
+        r.h1[j] = std::max(a, T(0));
+    }
+    for (int j = 0; j < kLpHid; ++j) {
+        T a = T(P[kLpB2 + j]);
+        for (int i = 0; i < kLpHid; ++i) a += q(T(P[kLpW2 + j * kLpHid + i])) * q(r.h1[i]);
+        r.a2[j] = a;
+        r.h2[j] = std::max(a, T(0));
+    }
+    for (int j = 0; j < kLpOut; ++j) {
+        T a = T(P[kLpB3 + j]);
+        for (int i = 0; i < kLpHid; ++i) a += q(T(P[kLpW3 + j * kLpHid + i])) * q(r.h2[i]);
+        r.o[j] = a;
+        r.s[j] = std::exp(clampT(a, T(-kLpClamp), T(kLpClamp)));
+    }
+}
+
+// dL/dP += the MLP's backward for one pixel, from gS = dL/ds (the clamp passes no gradient). With
+// `half` the products use the rounded operands as the forward does (the gradient of the rounded
+// network, applied to the unrounded parameters).
+template <class T>
+void learned_pixel_mlp_backward(const double* P, const T* f, bool half, const T* gS, T* gP) {
+    LpForward<T> r;
+    learned_pixel_mlp<T>(P, f, half, r);
+    auto q = [&](T v) { return half ? T(round_to_half(float(v))) : v; };
+    T go[kLpOut], gh2[kLpHid] = {}, gh1[kLpHid] = {};
+    for (int j = 0; j < kLpOut; ++j)
+        go[j] = (r.o[j] > T(-kLpClamp) && r.o[j] < T(kLpClamp)) ? gS[j] * r.s[j] : T(0);
+    for (int j = 0; j < kLpOut; ++j) {
+        gP[kLpB3 + j] += go[j];
+        for (int i = 0; i < kLpHid; ++i) {
+            gP[kLpW3 + j * kLpHid + i] += go[j] * q(r.h2[i]);
+            gh2[i] += go[j] * q(T(P[kLpW3 + j * kLpHid + i]));
+        }
+    }
+    for (int j = 0; j < kLpHid; ++j) {
+        const T g = r.a2[j] > T(0) ? gh2[j] : T(0);
+        gP[kLpB2 + j] += g;
+        for (int i = 0; i < kLpHid; ++i) {
+            gP[kLpW2 + j * kLpHid + i] += g * q(r.h1[i]);
+            gh1[i] += g * q(T(P[kLpW2 + j * kLpHid + i]));
+        }
+    }
+    for (int j = 0; j < kLpHid; ++j) {
+        const T g = r.a1[j] > T(0) ? gh1[j] : T(0);
+        gP[kLpB1 + j] += g;
+        for (int i = 0; i < kLpIn; ++i) gP[kLpW1 + j * kLpIn + i] += g * q(f[i]);
+    }
+}
+
+// Backward of one scaled reflection a-trous pass over the image (the pass of
+// reflection_atrous_pixel with lumScale = scale[p], copy tiles passed through): from dL/d(out rgb)
+// the gradient into the input rgb and into the per-pixel scale. The weights' dependence on the
+// input (luminance, the variance behind phi) is held fixed; the variance channel carries no
+// gradient (the loss reads rgb).
+template <class T>
+void learned_pixel_atrous_backward(const LuxgiReflectionAtrousParams& pc, int W, int H, Img normals, Img pbr,
+                                   const uint8_t* tileMap, Img in, const float* scale, const T* gOut, T* gIn,
+                                   T* gScale) {
+    const T kw[3] = {T(1.0), T(2.0 / 3.0), T(1.0 / 6.0)};
+    const T kv[2][2] = {{T(1.0 / 4.0), T(1.0 / 8.0)}, {T(1.0 / 8.0), T(1.0 / 16.0)}};
+    const int tw = (W + 7) / 8;
+    for (size_t e = 0; e < (size_t)W * H * 4; ++e) gIn[e] = 0;
+    for (size_t e = 0; e < (size_t)W * H; ++e) gScale[e] = 0;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t e = (size_t)y * W + x;
+            const T* g = gOut + e * 4;
+            const v4<T> c = fetch<T>(in, W, H, x, y);
+            const v4<T> pb = fetch<T>(pbr, W, H, x, y);
+            const T cd = pb.w, rough = pb.y;
+            if (!tileMap[(y / 8) * tw + x / 8] || (cd != T(1) && (rough <= T(0.05f) || rough >= T(0.45f)))) {
+                for (int k = 0; k < 3; ++k) gIn[e * 4 + k] += g[k];  // copied or passed through
+                continue;
+            }
+            if (cd == T(1)) continue;  // sky: 0
+            const T s = T(scale[e]);
+            const T cl = luminance(c.x, c.y, c.z);
+            T var = 0;
+            for (int yy = -1; yy <= 1; ++yy)
+                for (int xx = -1; xx <= 1; ++xx) var += fetch<T>(in, W, H, x + xx, y + yy).w * kv[std::abs(xx)][std::abs(yy)];
+            const T phiC = T(pc.phiColor) * std::sqrt(std::max(T(0), var)) + T(1e-10f);
+            const v4<T> nrm = fetch<T>(normals, W, H, x, y);
+            const v3<T> cn = octahedral_to_direction<T>(nrm.x, nrm.y);
+            // forward sums
+            T sw = 1, o[3] = {c.x, c.y, c.z};
+            struct Tap { int q; T wk, dl; };
+            std::vector<Tap> taps;
+            for (int yy = -pc.radius; yy <= pc.radius; ++yy)
+                for (int xx = -pc.radius; xx <= pc.radius; ++xx) {
+                    const int px = x + xx * pc.stepSize, py = y + yy * pc.stepSize;
+                    if (!(px >= 0 && py >= 0 && px < W && py < H) || (xx == 0 && yy == 0)) continue;
+                    const v4<T> sc = fetch<T>(in, W, H, px, py);
+                    const T sl = luminance(sc.x, sc.y, sc.z);
+                    const v4<T> sn = fetch<T>(normals, W, H, px, py);
+                    const T d = fetch<T>(pbr, W, H, px, py).w;
+                    const T w = edge_stop<T>(cd, d, T(pc.sigmaDepth), cn, octahedral_to_direction<T>(sn.x, sn.y),
+                                             T(pc.phiNormal), cl, sl, phiC / s);
+                    const T wk = w * kw[std::abs(xx)] * kw[std::abs(yy)];
+                    sw += wk;
+                    o[0] += wk * sc.x; o[1] += wk * sc.y; o[2] += wk * sc.z;
+                    taps.push_back({py * W + px, wk, std::fabs(cl - sl) / phiC});
+                }
+            for (int k = 0; k < 3; ++k) o[k] /= sw;
+            for (int k = 0; k < 3; ++k) gIn[e * 4 + k] += g[k] / sw;
+            T gs = 0;
+            for (const Tap& t : taps) {
+                const v4<T> sc = fetch<T>(in, W, H, t.q % W, t.q / W);
+                const T xq[3] = {sc.x, sc.y, sc.z};
+                for (int k = 0; k < 3; ++k) {
+                    gIn[(size_t)t.q * 4 + k] += g[k] * t.wk / sw;
+                    gs += g[k] * (-t.dl * t.wk) * (xq[k] - o[k]) / sw;
+                }
+            }
+            gScale[e] = gs;
         }
 }
 

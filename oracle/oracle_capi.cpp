@@ -495,3 +495,54 @@ extern "C" void oracle_learned_atrous_backward(const LuxgiLearnedAtrousParams* p
     LearnedAtrousImage im{W, H, normals, pbr, image, image};
     la_backward<double>(im, *p, P, gOut, gIn, gP);
 }
+
+// Per-pixel learned edge-stopping scales (FP64).
+extern "C" void oracle_reflection_atrous_scaled_f64(const LuxgiReflectionAtrousParams* pc, int W, int H,
+                                                    const float* normals, const float* pbr, const uint8_t* tileMap,
+                                                    const float* in, const float* scale, double* out, int threads) {
+    const int tw = (W + 7) / 8;
+    parallel_for(H, threads, [&](int y) {
+        for (int x = 0; x < W; ++x) {
+            const size_t e = (size_t)y * W + x;
+            if (tileMap[(y / 8) * tw + x / 8]) {
+                reflection_atrous_pixel<double>(*pc, W, H, x, y, {normals, 4}, {pbr, 4}, {in, 4}, out + e * 4,
+                                                scale ? double(scale[e]) : 1.0);
+            } else {
+                for (int c = 0; c < 4; ++c) out[e * 4 + c] = double(in[e * 4 + c]);
+            }
+        }
+    });
+}
+extern "C" void oracle_learned_pixel_features(int W, int H, const float* color, const float* moments,
+                                              const float* normals, const float* pbr, double* out, int threads) {
+    parallel_for(H, threads, [&](int y) {
+        for (int x = 0; x < W; ++x)
+            learned_pixel_features<double>(W, H, x, y, {color, 4}, {moments, 4}, {normals, 4}, {pbr, 4},
+                                           out + ((size_t)y * W + x) * kLpIn);
+    });
+}
+extern "C" void oracle_learned_pixel_mlp(const double* P, int n, const double* features, int half, double* o,
+                                         double* s, int threads) {
+    parallel_for(n, threads, [&](int i) {
+        LpForward<double> r;
+        learned_pixel_mlp<double>(P, features + (size_t)i * kLpIn, half != 0, r);
+        for (int j = 0; j < kLpOut; ++j) {
+            o[(size_t)i * kLpOut + j] = r.o[j];
+            s[(size_t)i * kLpOut + j] = r.s[j];
+        }
+    });
+}
+extern "C" void oracle_learned_pixel_mlp_backward(const double* P, int n, const double* features, int half,
+                                                  const double* gS, double* gP) {
+    for (int k = 0; k < kLpParams; ++k) gP[k] = 0;
+    for (int i = 0; i < n; ++i)
+        learned_pixel_mlp_backward<double>(P, features + (size_t)i * kLpIn, half != 0, gS + (size_t)i * kLpOut, gP);
+}
+extern "C" void oracle_learned_pixel_atrous_backward(const LuxgiReflectionAtrousParams* pc, int W, int H,
+                                                     const float* normals, const float* pbr, const uint8_t* tileMap,
+                                                     const float* in, const float* scale, const double* gOut,
+                                                     double* gIn, double* gScale) {
+    learned_pixel_atrous_backward<double>(*pc, W, H, {normals, 4}, {pbr, 4}, tileMap, {in, 4}, scale, gOut, gIn,
+                                          gScale);
+}
+extern "C" int oracle_learned_pixel_param_count() { return kLpParams; }

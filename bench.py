@@ -24,7 +24,8 @@ chain at 1920x1080 (learned_atrous: 4 passes, and one training step), the batche
 from the autotuner; ddgi_batched: eight uneven probe volumes one after another, on four streams,
 in one trace launch sequence and as one graph replay; batched_scenes: 64 volumes from 64 seeded
 scenes, SDF 64^3-256^3 in FP16, per-volume calls against one graph replay of the several-scene
-trace and the batched blend).
+trace and the batched blend), and the per-pixel edge-stopping network (learned_pixel: trained on
+rendered config-1-style views, evaluated at 1920x1080 against plain SVGF).
 value = rays / DDGI-update time (Grays/s). All timing uses HIP events on the
 stream the kernels are launched on (torch's current stream), W warm-up steps
 then exactly K timed steps between device synchronisations.
@@ -234,6 +235,21 @@ def bench_batched_scenes(args, q):
     del singles, graphed, scenes
     torch.cuda.empty_cache()
     return out
+
+
+def bench_learned_pixel(args, q):
+    """The per-pixel 12-32-32-5 edge-stopping network: trained on config-1-style views the method
+    renders (reduced here to 24 scenes at 480x270 and 400 steps; the full run is
+    `python -m luxgi_amd.learned_pixel`), evaluated at 1920x1080 on two held-out scenes against
+    plain SVGF, both feeding back their own filtered history; its 1080p cost beside the reference
+    chain's."""
+    from luxgi_amd import learned_pixel as lp
+    rep, _ = lp.run(train_scenes=24, eval_scenes=2, width=480, height=270, steps=400,
+                    log=None if q else (lambda m: log(f"[bench] learned pixel: {m}", q)))
+    log(f"[bench] learned pixel: held-out MSE plain {rep['eval_mse']['plain']:.6f}, learned "
+        f"{rep['eval_mse']['learned']:.6f} ({rep['learned_over_plain']:.3f}x); 1080p chain "
+        f"{rep['timing_ms']['learned_chain_ms']:.3f} ms vs reference {rep['timing_ms']['reference_chain_ms']:.3f} ms", q)
+    return rep
 
 
 def bench_ddgi_batched(args, ds, q):
@@ -569,6 +585,7 @@ def main():
     svgf_batched = None if args.no_batch else bench_svgf_batched(args, prims_dev, len(prims), g, sdf, res, q)
     ddgi_batched = None if args.no_batch else bench_ddgi_batched(args, ds, q)
     batched_scenes = None if args.no_batch else bench_batched_scenes(args, q)
+    learned_pixel = None if args.no_batch else bench_learned_pixel(args, q)
     # ---------------- reduced-precision form (reported beside the step, not in value) ----------------
     # FP16 SDF bricks with per-brick scale and FP32 re-evaluation at hits, FP16-MFMA blend (one
     # rounding per operand, FP32 accumulation), SVGF with RGBA16F intermediates. Ray buffers stay
@@ -714,6 +731,7 @@ def main():
         "svgf_batched": svgf_batched,
         "ddgi_batched": ddgi_batched,
         "batched_scenes": batched_scenes,
+        "learned_pixel": learned_pixel,
         "trace_grays_per_s": rays_per_frame / trace_s / 1e9,
         "trace_counts": {"lookups": n_lookup, "normals": n_normal, "objects": c[3], "tile_gathers": n_tile,
                          "rays": n_rays, "steps": c[6], "hits": c[7]},

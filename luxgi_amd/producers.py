@@ -44,11 +44,16 @@ def sdf_reflection(params, noise, gbuffer, scene, out, volume=None):
     """luxgi_sdf_reflection: gbuffer a dict of FP32 device tensors ("normals", "pbr", "depth", and
     optionally "color"), scene a ddgi.DeviceScene (global SDF, surface atlas, sky), out an [H][W][4]
     FP32 or FP16 tensor (sky pixels are left untouched), volume a ddgi.DDGIVolume for
-    approximateWithDDGI."""
+    approximateWithDDGI. Allocates only on the first call for a scene."""
     H, W = gbuffer["depth"].shape
     gb = abi.GBuffer(ptr(gbuffer.get("color")), ptr(gbuffer["normals"]), ptr(gbuffer["pbr"]), ptr(gbuffer["depth"]))
-    ws_bytes = lib().luxgi_sdf_reflection_workspace_size(scene.n_objects)
-    ws = torch.empty(max(ws_bytes, 256), dtype=torch.uint8, device="cuda")
+    # the workspace is kept on the scene and grown when needed (calls on one scene share it, so
+    # calls that overlap on different streams race)
+    ws_bytes = max(lib().luxgi_sdf_reflection_workspace_size(scene.n_objects), 256)
+    ws = getattr(scene, "_reflection_ws", None)
+    if ws is None or ws.numel() < ws_bytes:
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+        scene._reflection_ws = ws
     storage = abi.STORAGE_FP16 if out.dtype == torch.float16 else abi.STORAGE_FP32
     u = C.byref(volume.u) if volume is not None else None
     probes = C.byref(volume.atlas[volume.ping_pong]) if volume is not None else None

@@ -55,6 +55,18 @@ def scene_small(seed=1):
     return prims
 
 
+def scene_small_varied(seed):
+    """scene_small(seed) (seeded albedo and roughness) with the two blocks and three spheres moved by
+    up to 0.15 along each axis and scaled by 0.8-1.2, seeded: config-1-style scenes that differ in
+    layout as well as in material."""
+    prims = scene_small(seed)
+    rng = np.random.default_rng(seed + 7919)
+    for k in range(6, 11):
+        prims[k]["center"] = prims[k]["center"] + rng.uniform(-0.15, 0.15, size=3)
+        prims[k]["size"] = prims[k]["size"] * rng.uniform(0.8, 1.2)
+    return prims
+
+
 def scene_random(n=2000, seed=2, size_range=(0.01, 0.2)):
     """n primitives (box / sphere / capsule) at uniform positions in [-1, 1]^3 with
     log-uniform sizes in size_range (0.005-0.1 of the extent 2)."""

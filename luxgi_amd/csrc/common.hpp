@@ -413,4 +413,23 @@ __global__ __launch_bounds__(BLOCK) void block_scan_kernel(const uint32_t* in, u
     if (threadIdx.x == BLOCK - 1) totals[blockIdx.x] = before + x;
 }
 
+// XCD-aware block order. The dispatcher deals the workgroups of a launch to the 8 XCDs round-robin
+// (workgroup b, counted x-fastest, runs on XCD b % 8); xcd_block(b, nb) renumbers them so that each
+// XCD takes a contiguous 1/8 of the work, whose data then stays in that XCD's L2. The last
+// nb % 8 blocks keep their numbers.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ long long xcd_block(long long b, long long nb) {
+    const long long full = (nb / kXcds) * kXcds;
+    if (b >= full) return b;
+    return (b % kXcds) * (nb / kXcds) + b / kXcds;
+}
+
+// the (x, y) block of a 2D grid after xcd_block: each XCD takes a contiguous band of block rows
+__device__ __forceinline__ int2 xcd_block_2d() {
+    const long long nb = (long long)gridDim.x * gridDim.y;
+    const long long t = xcd_block((long long)blockIdx.y * gridDim.x + blockIdx.x, nb);
+    return make_int2(int(t % gridDim.x), int(t / gridDim.x));
+}
+
 }  // namespace luxgi

@@ -94,13 +94,6 @@ __device__ __forceinline__ uint32_t wave_aggregated_add(uint32_t* __restrict__ c
 // its own L2. The hit record (hitTime, texture UV of the hit step, hit SDF,
 // cascade) goes to the workspace and shade_kernel turns it into the outputs.
 // ---------------------------------------------------------------------------
-constexpr int kXcds = 8;
-
-__device__ __forceinline__ long long xcd_block(long long b, long long nb) {
-    const long long full = (nb / kXcds) * kXcds;
-    if (b >= full) return b;
-    return (b % kXcds) * (nb / kXcds) + b / kXcds;
-}
 
 __device__ __forceinline__ void ray_setup(const TraceArgs& A, const LuxgiTraceVolume& v, int probe, int ray,
                                           f3& origin, f3& dir) {

@@ -19,6 +19,8 @@
 //   - The history copy after iteration 1 (RaytracedReflection.cpp:488-491) is a
 //     second store of the same a-trous output.
 // texelFetch outside the image returns 0 (robust image access), as in the oracle.
+#include <algorithm>
+
 #include "svgf_common.hpp"
 
 namespace luxgi {
@@ -272,7 +274,8 @@ __global__ __launch_bounds__(256) void reflection_reproject_kernel(ReflArgs A, G
                                                                    const S* __restrict__ histMom, S* __restrict__ outColor,
                                                                    S* __restrict__ outMom, uint8_t* __restrict__ tileMap) {
     __shared__ ReprojectShared sm;
-    reproject_block<S>(A, cur, prev, noisy, histOut, histMom, outColor, outMom, tileMap, blockIdx.x, blockIdx.y, sm);
+    const int2 b = xcd_block_2d();
+    reproject_block<S>(A, cur, prev, noisy, histOut, histMom, outColor, outMom, tileMap, b.x, b.y, sm);
 }
 
 
@@ -351,7 +354,8 @@ __global__ __launch_bounds__(64) void reflection_atrous_kernel(AtrousArgs A, con
                                                                const uint8_t* __restrict__ tileMap,
                                                                const S* __restrict__ in, S* __restrict__ out,
                                                                S* __restrict__ hist) {
-    const int x = blockIdx.x * 8 + (threadIdx.x & 7), y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    const int2 b = xcd_block_2d();
+    const int x = b.x * 8 + (threadIdx.x & 7), y = b.y * 8 + (threadIdx.x >> 3);
     if (x >= A.W || y >= A.H) return;
     atrous_direct_pixel<S>(A, normals, pbr, tileMap, in, out, hist, x, y);
 }
@@ -461,9 +465,14 @@ __global__ __launch_bounds__(256) void reflection_atrous_lds_kernel(AtrousArgs A
                                                                     const uint8_t* __restrict__ tileMap,
                                                                     const S* __restrict__ in, S* __restrict__ out,
                                                                     S* __restrict__ hist) {
-    __shared__ float4 sIn[kAtrousMaxSide * kAtrousMaxSide];
-    __shared__ float4 sGuide[kAtrousMaxSide * kAtrousMaxSide];
-    atrous_lds_block<S>(A, normals, pbr, tileMap, in, out, hist, blockIdx.x, blockIdx.y, sIn, sGuide);
+    // LDS sized by the launch to the region this step stages ((16 + 2 h)^2 texels of input and
+    // guide): small steps keep more blocks resident
+    extern __shared__ float4 atrousLds[];
+    const int side = kAtrousBlock + 2 * max(A.radius * A.stepSize, 1);
+    float4* sIn = atrousLds;
+    float4* sGuide = atrousLds + side * side;
+    const int2 b = xcd_block_2d();
+    atrous_lds_block<S>(A, normals, pbr, tileMap, in, out, hist, b.x, b.y, sIn, sGuide);
 }
 
 // ---------------------------------------------------------------------------
@@ -582,7 +591,8 @@ __global__ __launch_bounds__(256) void shadow_reproject_kernel(ShadowArgs A, GBu
     __shared__ float hs[32][16];
     __shared__ int vote[4];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int bx = blockIdx.x * 16, by = blockIdx.y * 16;
+    const int2 blk = xcd_block_2d();
+    const int bx = blk.x * 16, by = blk.y * 16;
     const int W = A.W, H = A.H;
     if (threadIdx.x < 4) vote[threadIdx.x] = 0;
     for (int i = threadIdx.x; i < 32 * 32; i += 256) {
@@ -631,7 +641,7 @@ __global__ __launch_bounds__(256) void shadow_reproject_kernel(ShadowArgs A, GBu
     }
     __syncthreads();
     if (threadIdx.x < 4) {
-        const int tX = blockIdx.x * 2 + (threadIdx.x & 1), tY = blockIdx.y * 2 + (threadIdx.x >> 1);
+        const int tX = blk.x * 2 + (threadIdx.x & 1), tY = blk.y * 2 + (threadIdx.x >> 1);
         if (tX * 8 < W && tY * 8 < H) tileMap[tY * A.tilesX + tX] = (uint8_t)vote[threadIdx.x];
     }
 }
@@ -672,12 +682,13 @@ __global__ __launch_bounds__(64) void shadow_atrous_kernel(AtrousArgs A, const f
                                                            const float4* __restrict__ pbr,
                                                            const uint8_t* __restrict__ tileMap, const S* __restrict__ in,
                                                            S* __restrict__ out, S* __restrict__ hist) {
-    const int x = blockIdx.x * 8 + (threadIdx.x & 7), y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    const int2 blk = xcd_block_2d();
+    const int x = blk.x * 8 + (threadIdx.x & 7), y = blk.y * 8 + (threadIdx.x >> 3);
     const int W = A.W, H = A.H;
     if (x >= W || y >= H) return;
     const size_t e = (size_t)y * W + x;
     float2 o = make_float2(0.f, 0.f);
-    if (tileMap[blockIdx.y * A.tilesX + blockIdx.x]) {
+    if (tileMap[blk.y * A.tilesX + blk.x]) {
         const float2 c = Texel<S>::ld2(in, e);
         const float kv[2][2] = {{1.0f / 4.0f, 1.0f / 8.0f}, {1.0f / 8.0f, 1.0f / 16.0f}};
         float var = 0.0f;
@@ -707,14 +718,17 @@ __global__ __launch_bounds__(256) void shadow_atrous_lds_kernel(AtrousArgs A, co
                                                                 const uint8_t* __restrict__ tileMap,
                                                                 const S* __restrict__ in, S* __restrict__ out,
                                                                 S* __restrict__ hist) {
-    __shared__ float2 sIn[kAtrousMaxSide * kAtrousMaxSide];
-    __shared__ float4 sGuide[kAtrousMaxSide * kAtrousMaxSide];
+    extern __shared__ float4 atrousLds[];  // sized by the launch, as in reflection_atrous_lds_kernel
+    const int side = kAtrousBlock + 2 * max(A.radius * A.stepSize, 1);
+    float4* sGuide = atrousLds;
+    float2* sIn = reinterpret_cast<float2*>(atrousLds + side * side);
     const int W = A.W, H = A.H;
     const int tx = threadIdx.x & (kAtrousBlock - 1), ty = threadIdx.x / kAtrousBlock;
-    const int x = blockIdx.x * kAtrousBlock + tx, y = blockIdx.y * kAtrousBlock + ty;
+    const int2 blk = xcd_block_2d();
+    const int x = blk.x * kAtrousBlock + tx, y = blk.y * kAtrousBlock + ty;
     const bool inImage = x < W && y < H;
     const size_t e = (size_t)y * W + x;
-    if (!block_has_filter_tile(A, tileMap, blockIdx.x, blockIdx.y)) {
+    if (!block_has_filter_tile(A, tileMap, blk.x, blk.y)) {
         if (inImage) {
             Texel<S>::st2(out, e, make_float2(0.f, 0.f));
             if (hist) Texel<S>::st2(hist, e, make_float2(0.f, 0.f));
@@ -722,7 +736,7 @@ __global__ __launch_bounds__(256) void shadow_atrous_lds_kernel(AtrousArgs A, co
         return;
     }
     const AtrousRegion r = stage_region<S>(A, normals, pbr, sIn, sGuide,
-                                           [&](size_t q) { return Texel<S>::ld2(in, q); }, blockIdx.x, blockIdx.y);
+                                           [&](size_t q) { return Texel<S>::ld2(in, q); }, blk.x, blk.y);
     if (!inImage) return;
     float2 o = make_float2(0.f, 0.f);
     if (tileMap[(y >> 3) * A.tilesX + (x >> 3)]) {
@@ -834,8 +848,9 @@ __global__ __launch_bounds__(256) void batch_reproject_kernel(const LuxgiDenoise
     __shared__ ReprojectShared sm;
     const int count = hdr->count, total = hdr->totalBlocks;
     if (count <= 0 || count > LUXGI_MAX_DENOISE_BATCH) return;
-    if (ONE) {
-        if ((int)blockIdx.x < total) batch_reproject_unit<S>(items, count, blockIdx.x, sm);
+    if (ONE) {  // units in XCD order: each XCD takes a contiguous run of units
+        const long long u = xcd_block(blockIdx.x, gridDim.x);
+        if (u < total) batch_reproject_unit<S>(items, count, (int)u, sm);
         return;
     }
     const BatchWalk wk = batch_walk(total, order);
@@ -878,9 +893,9 @@ __global__ __launch_bounds__(256) void batch_atrous_kernel(const LuxgiDenoiseBat
     __shared__ float4 sGuide[kAtrousMaxSide * kAtrousMaxSide];
     const int count = hdr->count, total = hdr->totalBlocks;
     if (count <= 0 || count > LUXGI_MAX_DENOISE_BATCH) return;
-    if (ONE) {
-        if ((int)blockIdx.x < total)
-            batch_atrous_unit<S>(items, count, blockIdx.x, iteration, readIdx, writeIdx, sIn, sGuide);
+    if (ONE) {  // units in XCD order, as above
+        const long long u = xcd_block(blockIdx.x, gridDim.x);
+        if (u < total) batch_atrous_unit<S>(items, count, (int)u, iteration, readIdx, writeIdx, sIn, sGuide);
         return;
     }
     const BatchWalk wk = batch_walk(total, order);
@@ -897,16 +912,19 @@ using namespace luxgi;
 // steps through the direct kernel (8x8 pixels, 64 lanes).
 template <class S, class K>
 static void launch_atrous(const AtrousArgs& A, K ldsKernel, K directKernel, hipStream_t st, const float4* nrm,
-                          const float4* pbr, const uint8_t* tileMap, const void* input, void* output, void* hist) {
+                          const float4* pbr, const uint8_t* tileMap, const void* input, void* output, void* hist,
+                          size_t ldsBytesPerTexel) {
     const S* in = static_cast<const S*>(input);
     S* out = static_cast<S*>(output);
     S* h = static_cast<S*>(hist);
-    if (A.stepSize <= kAtrousMaxHalo && A.radius * A.stepSize <= kAtrousMaxHalo)
+    if (A.stepSize <= kAtrousMaxHalo && A.radius * A.stepSize <= kAtrousMaxHalo) {
+        const int side = kAtrousBlock + 2 * std::max(A.radius * A.stepSize, 1);
         hipLaunchKernelGGL(ldsKernel, dim3((A.W + kAtrousBlock - 1) / kAtrousBlock, (A.H + kAtrousBlock - 1) / kAtrousBlock),
-                           dim3(256), 0, st, A, nrm, pbr, tileMap, in, out, h);
-    else
+                           dim3(256), (size_t)side * side * ldsBytesPerTexel, st, A, nrm, pbr, tileMap, in, out, h);
+    } else {
         hipLaunchKernelGGL(directKernel, dim3((A.W + 7) / 8, (A.H + 7) / 8), dim3(64), 0, st, A, nrm, pbr, tileMap, in,
                            out, h);
+    }
 }
 
 extern "C" int32_t luxgi_svgf_tile_count(int32_t width, int32_t height) {
@@ -958,10 +976,10 @@ static int refl_atrous(const LuxgiReflectionAtrousParams* p, int32_t W, int32_t 
     const float4* pbr = static_cast<const float4*>(static_cast<const void*>(cur->pbr));
     if (storage == LUXGI_STORAGE_FP32)
         launch_atrous<float>(A, reflection_atrous_lds_kernel<float>, reflection_atrous_kernel<float>, st, nrm, pbr,
-                             tileMap, input, output, historyOut);
+                             tileMap, input, output, historyOut, 2 * sizeof(float4));
     else
         launch_atrous<__half>(A, reflection_atrous_lds_kernel<__half>, reflection_atrous_kernel<__half>, st, nrm, pbr,
-                              tileMap, input, output, historyOut);
+                              tileMap, input, output, historyOut, 2 * sizeof(float4));
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }
@@ -1055,10 +1073,10 @@ extern "C" int luxgi_svgf_shadow_atrous(const LuxgiShadowAtrousParams* p, int32_
     const float4* pbr = static_cast<const float4*>(static_cast<const void*>(cur->pbr));
     if (storage == LUXGI_STORAGE_FP32)
         launch_atrous<float>(A, shadow_atrous_lds_kernel<float>, shadow_atrous_kernel<float>, st, nrm, pbr, tileMap,
-                             input, output, historyOut);
+                             input, output, historyOut, sizeof(float4) + sizeof(float2));
     else
         launch_atrous<__half>(A, shadow_atrous_lds_kernel<__half>, shadow_atrous_kernel<__half>, st, nrm, pbr, tileMap,
-                              input, output, historyOut);
+                              input, output, historyOut, sizeof(float4) + sizeof(float2));
     LUXGI_LAUNCH_CHECK();
     return LUXGI_OK;
 }

@@ -652,6 +652,27 @@ def main():
                          "speedup_ddgi_vs_fp32": ddgi_ms / (b_trace + b_upd)}
         log(f"[bench] reduced precision bf16: trace {b_trace:.3f} ms, update {b_upd:.3f} ms", q)
         del volbf
+        # FP16 cells without the FP32 refine at hits (the hits keep the FP16 distances)
+        dsnr = DeviceScene(g, sdf, mip, atlas, sdf_storage=abi.STORAGE_FP16, layout=layout)
+        volnr = DDGIVolume(u)
+        tn = []
+        for k in range(args.warmup + args.steps):
+            e3 = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            e3[0].record()
+            volnr.trace_rays(dsnr, default_trace_params(rotations[k], frame=volnr.frames))
+            e3[1].record()
+            volnr.probe_update(fuse_border=True, half=True)
+            e3[2].record()
+            volnr.end_frame()
+            if k >= args.warmup:
+                tn.append(e3)
+        torch.cuda.synchronize()
+        n_trace = float(np.mean([e[0].elapsed_time(e[1]) for e in tn]))
+        n_upd = float(np.mean([e[1].elapsed_time(e[2]) for e in tn]))
+        mixed["fp16_no_refine"] = {"trace_ms": n_trace, "probe_update_ms": n_upd, "ddgi_update_ms": n_trace + n_upd,
+                                   "speedup_ddgi_vs_fp32": ddgi_ms / (n_trace + n_upd)}
+        log(f"[bench] reduced precision fp16 without refine: trace {n_trace:.3f} ms, update {n_upd:.3f} ms", q)
+        del volnr
 
     # ---------------- counting run (algorithmic bytes of the trace) ----------------
     counters = torch.zeros(8, dtype=torch.int64, device="cuda")
@@ -682,7 +703,8 @@ def main():
     if not args.no_cpu:
         first = {}
         forms = [("fp32", ds, False, False)] + ([] if mixed is None else [("reduced", ds16, True, False),
-                                                                         ("reduced_bf16", dsbf, True, True)])
+                                                                         ("reduced_bf16", dsbf, True, True),
+                                                                         ("reduced_fp16_no_refine", dsnr, True, False)])
         for name, scene_, half, bf16 in forms:
             v = DDGIVolume(u)
             v.trace_rays(scene_, default_trace_params(rotations[-1]))

@@ -15,7 +15,8 @@ volume at 1920x1080 (probe_sample_ms; bytes = G-buffer depth + normal + RGBA32F
 output per pixel + both atlases once), and the surface-atlas passes (surface_atlas): chunk
 culling of the objects and the relight of every atlas tile (emissive copy, one directional
 light with an SDF shadow ray per lit texel, DDGI bounce) into an RGBA16F light cache, and
-the SDF soft-shadow mask at 1920x1080 (sdf_shadow_mask_ms: one blue-noise disk ray per pixel) and
+the SDF soft-shadow mask at 1920x1080 (sdf_shadow_mask_ms: one blue-noise disk ray per pixel), its
+shadow SVGF frame (shadow_svgf_ms: reprojection + 4 a-trous with power on the last), and
 the SDF reflection rays at 1920x1080 (sdf_reflection_ms: one GGX-sampled ray per pixel with the
 surface-atlas lookup at its hit, RGBA16F out), and the global SDF build from mesh SDFs (sdf_build:
 2,000 instances of three baked 32^3 meshes rasterized into the 256^3 cascade), the learned a-trous
@@ -385,7 +386,7 @@ def main():
         f"svgf {t_svgf.mean():.3f} ms, wall/step {ms_step:.3f} ms", q)
 
     # the passes reported beside the step (skipped with --step-only, e.g. for profiler runs)
-    sample_ms = sample_bytes = learned = surf = shadow_ms = refl_ms = sdf_build = None
+    sample_ms = sample_bytes = learned = surf = shadow_ms = shadow_svgf_ms = refl_ms = sdf_build = None
     if not args.step_only:
         # ---------------- probe sampling at 1080p (sample_probe pass; reported beside the step) ----------------
         pev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
@@ -521,6 +522,22 @@ def main():
         torch.cuda.synchronize()
         shadow_ms = mev[0].elapsed_time(mev[1]) / args.steps
         log(f"[bench] sdf shadow mask 1080p {shadow_ms:.3f} ms", q)
+        # the shadow SVGF frame at 1080p (RaytracedShadow.cpp:543-609): reprojection of the mask, 4
+        # a-trous passes with the history copy after iteration 1 and `power` on the last one
+        from luxgi_amd.svgf import ShadowDenoiser
+        sdn = ShadowDenoiser(W, H)
+        sdn.set_camera(VPI)
+        for _ in range(2):
+            sdn.denoise(cur, prev, smask)
+        sev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        sev[0].record()
+        for _ in range(args.steps):
+            sdn.denoise(cur, prev, smask)
+        sev[1].record()
+        torch.cuda.synchronize()
+        shadow_svgf_ms = sev[0].elapsed_time(sev[1]) / args.steps
+        log(f"[bench] shadow svgf 1080p {shadow_svgf_ms:.3f} ms", q)
+        del sdn
         # SDF reflection rays at 1080p (the reflection SVGF's noisy input): one GGX-sampled ray per pixel
         # (the G-buffer's roughness lies in (0.05, 0.45)), coloured by the surface-atlas lookup or the sky
         from luxgi_amd.producers import sdf_reflection
@@ -748,6 +765,7 @@ def main():
         "probe_sample_gbs": None if sample_ms is None else sample_bytes / (sample_ms * 1e-3) / 1e9,
         "surface_atlas": surf,
         "sdf_shadow_mask_ms": shadow_ms,
+        "shadow_svgf_ms": shadow_svgf_ms,
         "sdf_reflection_ms": refl_ms,
         "sdf_build": sdf_build,
         "svgf_batched": svgf_batched,

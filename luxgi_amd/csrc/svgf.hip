@@ -857,6 +857,14 @@ __global__ __launch_bounds__(256) void batch_reproject_kernel(const LuxgiDenoise
     for (int w = wk.begin; w < wk.end; w += wk.stride) batch_reproject_unit<S>(items, count, w, sm);
 }
 
+// LDS side of the batched a-trous at iteration i: the LDS-tiled items stage (16 + 2h)^2 texels with
+// h = max(radius * 2^i, 1) <= 8 and radius <= 2
+__host__ __device__ inline int batch_atrous_side(int iteration) {
+    const int step = 1 << iteration;
+    const int h = 2 * step <= kAtrousMaxHalo ? 2 * step : kAtrousMaxHalo;
+    return kAtrousBlock + 2 * (h > 1 ? h : 1);
+}
+
 // iteration i of every item: input outputs[0] (i = 0) or atrous[readIdx], output atrous[writeIdx],
 // the history copy into outputs[1] after iteration 1 (RaytracedReflection.cpp:419-494)
 template <class S>
@@ -889,8 +897,11 @@ template <class S, bool ONE>
 __global__ __launch_bounds__(256) void batch_atrous_kernel(const LuxgiDenoiseBatchHeader* __restrict__ hdr,
                                                            const Item* __restrict__ items, int iteration, int readIdx,
                                                            int writeIdx, int order) {
-    __shared__ float4 sIn[kAtrousMaxSide * kAtrousMaxSide];
-    __shared__ float4 sGuide[kAtrousMaxSide * kAtrousMaxSide];
+    // LDS sized by the launch for the largest radius (2) at this iteration's step (batch_atrous_side)
+    extern __shared__ float4 batchAtrousLds[];
+    const int side = batch_atrous_side(iteration);
+    float4* sIn = batchAtrousLds;
+    float4* sGuide = batchAtrousLds + side * side;
     const int count = hdr->count, total = hdr->totalBlocks;
     if (count <= 0 || count > LUXGI_MAX_DENOISE_BATCH) return;
     if (ONE) {  // units in XCD order, as above
@@ -1147,11 +1158,13 @@ static void launch_batched(const LuxgiDenoiseBatchHeader* hdr, const LuxgiReflec
     bool pp = false;
     for (int i = 0; i < iterations; ++i) {
         const int readIdx = pp ? 1 : 0, writeIdx = pp ? 0 : 1;
+        const int side = batch_atrous_side(i);
+        const size_t lds = (size_t)side * side * 2 * sizeof(float4);
         if (order == 2)
-            hipLaunchKernelGGL((batch_atrous_kernel<S, true>), dim3(grid), dim3(256), 0, st, hdr, items, i, readIdx,
+            hipLaunchKernelGGL((batch_atrous_kernel<S, true>), dim3(grid), dim3(256), lds, st, hdr, items, i, readIdx,
                                writeIdx, order);
         else
-            hipLaunchKernelGGL((batch_atrous_kernel<S, false>), dim3(grid), dim3(256), 0, st, hdr, items, i, readIdx,
+            hipLaunchKernelGGL((batch_atrous_kernel<S, false>), dim3(grid), dim3(256), lds, st, hdr, items, i, readIdx,
                                writeIdx, order);
         pp = !pp;
     }

@@ -87,20 +87,22 @@ class PixelScaleChain:
         self._inputs = [None] * iterations
         self._tiles = None
 
-    def compute_scales(self, gbuffer, inp, moments):
+    def compute_scales(self, gbuffer, inp, moments, keep_features=True):
+        """The network's scales of the passes; keep_features stores the features for backward()."""
         check(lib().luxgi_learned_pixel_scales(ptr(self.params), self.W, self.H, C.byref(gbuffer_struct(gbuffer)),
-                                               ptr(inp), ptr(moments), ptr(self.scales), ptr(self.features),
-                                               stream_handle()), "luxgi_learned_pixel_scales")
+                                               ptr(inp), ptr(moments), ptr(self.scales),
+                                               ptr(self.features) if keep_features else None, stream_handle()),
+              "luxgi_learned_pixel_scales")
         return self.scales
 
-    def forward(self, gbuffer, inp, moments, tile_map, history_out=None):
+    def forward(self, gbuffer, inp, moments, tile_map, history_out=None, keep_features=True):
         """The chain on `inp` ([H][W][4] FP32, the reprojection's colour + variance) with the
         reprojection's `moments` and tile map; pass 1's output is also written to `history_out`
         when given (the reference's filtered-history feedback). Returns the last pass's output (an
-        internal buffer, valid until the next call)."""
+        internal buffer, valid until the next call). Inference passes keep_features=False."""
         if inp.shape != (self.H, self.W, 4) or inp.dtype != torch.float32:
             raise ValueError("input must be an [H][W][4] float32 tensor")
-        self.compute_scales(gbuffer, inp, moments)
+        self.compute_scales(gbuffer, inp, moments, keep_features)
         gb = gbuffer_struct(gbuffer)
         src = inp.contiguous()
         self._tiles = tile_map
@@ -199,7 +201,7 @@ class LearnedPixelDenoiser:
             ptr(noisy), ptr(d.outputs[1]), ptr(d.moments[1 - d.ping_pong]), ptr(d.outputs[0]),
             ptr(d.moments[d.ping_pong]), ptr(d.tile_map), stream_handle()), "luxgi_svgf_reflection_reproject")
         out = self.chain.forward(cur, d.outputs[0], d.moments[d.ping_pong], d.tile_map,
-                                 history_out=d.outputs[1] if self.chain.iterations > 1 else None)
+                                 history_out=d.outputs[1] if self.chain.iterations > 1 else None, keep_features=False)
         d.ping_pong = 1 - d.ping_pong
         d.result = out
         return out
@@ -350,8 +352,8 @@ def time_chains(params, view, reps=20, iterations=5):
 
     out = {}
     for name, fn in (("reference_chain_ms", reference),
-                     ("learned_chain_ms", lambda: chain.forward(view.gbuffer, inp, mom, tiles)),
-                     ("scales_ms", lambda: chain.compute_scales(view.gbuffer, inp, mom))):
+                     ("learned_chain_ms", lambda: chain.forward(view.gbuffer, inp, mom, tiles, keep_features=False)),
+                     ("scales_ms", lambda: chain.compute_scales(view.gbuffer, inp, mom, keep_features=False))):
         for _ in range(3):
             fn()
         torch.cuda.synchronize()

@@ -488,20 +488,26 @@ __device__ __forceinline__ floatx4 mfma_lp(V ah, V al, V bh, V bl, floatx4 acc) 
 // ---------------------------------------------------------------------------
 constexpr int kWB = 4;  // previous-atlas loads each lane keeps in flight in the tile writer
 
+// The rows are padded so that the LDS accesses of a wave spread over the banks: a probe's ray row
+// is 40 elements (80 B), so the 16-byte B-operand reads of 8 consecutive lanes (probes col..col+7,
+// one ray octet) start 20 dwords apart and cover all 32 banks once; an output row is one texel
+// longer than the tile, so the epilogue's stores of the 16 probes of one texel fall in different
+// banks instead of all in one.
+constexpr int kHKPad = kHK + 8;
 template <int CH, class T>
 struct MapStage {
-    T b[2][CH][kMG][kHK];  // [hi / lo][channel][probe][ray]
+    T b[2][CH][kMG][kHKPad];  // [hi / lo][channel][probe][ray]
 };
 template <int MAP>
 struct MapOut;
 template <>
 struct MapOut<0> {
-    float2 v[kMG][256];
+    float2 v[kMG][256 + 1];
     __device__ __forceinline__ float4 get(int g, int t) const { return make_float4(v[g][t].x, v[g][t].y, 0.f, 0.f); }
 };
 template <>
 struct MapOut<1> {
-    float4 v[kMG][64];
+    float4 v[kMG][64 + 1];
     __device__ __forceinline__ float4 get(int g, int t) const { return v[g][t]; }
 };
 template <int MAP, class T>

@@ -79,6 +79,35 @@ def parse():
     return ap.parse_args()
 
 
+def device_copy_gbs():
+    """Measured HBM bandwidth of a plain device copy (1 GiB read + 1 GiB written), beside the nominal peak."""
+    import torch
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(10):
+        dst.copy_(src)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / 10
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * (1 << 28) * 4 / (ms * 1e-3) / 1e9
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def log(msg, quiet=False):
     if not quiet:
         print(msg, file=sys.stderr, flush=True)
@@ -701,6 +730,7 @@ def main():
     trace_s = float(np.mean(t_trace)) * 1e-3
     achieved = trace_bytes / trace_s / 1e9
     traffic = None
+    copy_gbs = None if args.step_only else device_copy_gbs()
     tpath = os.path.join(ROOT, "profiles", "trace_hbm_traffic.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
@@ -776,7 +806,7 @@ def main():
         "trace_counts": {"lookups": n_lookup, "normals": n_normal, "objects": c[3], "tile_gathers": n_tile,
                          "rays": n_rays, "steps": c[6], "hits": c[7]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "measured_copy_gbs": copy_gbs,
                      "kernel": "luxgi_ddgi_trace_sdf launch sequence (march + hit sort + shade)",
                      "algorithmic_bytes_per_launch": trace_bytes},
         "cpu_baseline": cpu,
@@ -832,7 +862,7 @@ def cpu_baseline(u_full, g, sdf_t, mip_t, res, atlas, rotation, threads, quiet, 
               "sample": f"FP32 C++ transcription, {u.num_probes} probes (32x16x{nz} slab of the same volume) x "
                         f"{u.raysPerProbe} rays: trace {tt:.2f} s + blend {tu:.2f} s on {n} threads, median of "
                         f"3 runs (host reports {hw} hardware threads; one GPU's host share is 16)",
-              "trace_s": tt, "update_s": tu, "runs_s": totals}
+              "trace_s": tt, "update_s": tu, "runs_s": totals, "hardware_threads": hw, "cpu_model": cpu_model()}
     if svgf is not None:
         # SVGF on the CPU: the FP32 transcription of the reflection reprojection and a-trous passes
         # over image rows (std::thread), the bench's 1080p frame, median of three

@@ -169,7 +169,9 @@ __device__ bool reproject_rgb(const RA& A, int cx, int cy, float depth, float ra
 
 // LDS of one reprojection block
 struct ReprojectShared {
-    float3 tile[32][33];
+    // rows of 48 float3 (144 dwords, 16 banks apart): in the row sums the two rows a half-wave
+    // reads then fall in disjoint banks (a 33-wide row, 99 dwords, aliases them)
+    float3 tile[32][48];
     float3 h1[32][16], h2[32][16];
     int vote[4];
 };

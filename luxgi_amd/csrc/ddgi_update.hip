@@ -526,6 +526,17 @@ __device__ __forceinline__ void write_staged(const UpdateArgs& A, const Staged& 
     const int pw = n + 2, pxy = A.px * A.py;
     const int total = kMG * pw * pw;
     const int tstride = CH == 4 ? 64 : 256;
+    // the 16 probes' tile origins in the atlas, computed once per block
+    __shared__ int sOrg[2][kMG];
+    if (threadIdx.x < kMG) {
+        const int p = p0 + (int)threadIdx.x;
+        sOrg[0][threadIdx.x] = (p % pxy) * pw + 1;
+        sOrg[1][threadIdx.x] = (p / pxy) * pw + 1;
+    }
+    __syncthreads();
+    // i / pw as a multiply-high by m = floor(2^32 / pw) + 1: exact while i * pw < 2^32, and here
+    // i < 16 pw^2 + 1024 with pw <= 66
+    const uint64_t mPw = (1ull << 32) / (uint32_t)pw + 1;
     for (int i0 = threadIdx.x; i0 < total; i0 += 256 * kWB) {
         float4 pv[kWB];
         size_t dst[kWB];
@@ -534,14 +545,15 @@ __device__ __forceinline__ void write_staged(const UpdateArgs& A, const Staged& 
 #pragma unroll
         for (int u = 0; u < kWB; ++u) {
             const int i = i0 + u * 256;
-            const int bx = i % pw, g = (i / pw) % kMG, by = i / (pw * kMG);
+            const int q = (int)(((uint64_t)(uint32_t)i * mPw) >> 32);  // i / pw
+            const int bx = i - q * pw, g = q & (kMG - 1), by = q / kMG;
             const int p = p0 + g;
             const bool interior = bx >= 1 && bx <= n && by >= 1 && by <= n;
             ok[u] = i < total && p < A.P && (interior || A.fuseBorder);
             int sx = 0, sy = 0;
             border_source(bx, by, n, sx, sy);
             src[u] = g * tstride + (sy - 1) * n + (sx - 1);
-            const int ox = (p % pxy) * pw + 1, oy = (p / pxy) * pw + 1;
+            const int ox = sOrg[0][g], oy = sOrg[1][g];
             dst[u] = (size_t)(oy + by) * W + ox + bx;
             pv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (ok[u] && !A.firstFrame) {
@@ -579,23 +591,27 @@ __device__ __forceinline__ void write_staged(const UpdateArgs& A, const Staged& 
 
 // one k-step's ray data of this map: thread t stages rays 2(t%16), 2(t%16)+1 of probe t/16
 struct MapPrefetch {
-    float4 a[2];  // dd (depth) or radiance (irradiance)
-    float4 d[2];  // dd, irradiance with CHECK only
+    float4 a[2];   // dd (depth) or radiance (irradiance)
+    float4 d[2];   // dd, irradiance with CHECK only
+    float4 r0[2];  // ray row 0's directions of the same rays (depth, or CHECK), loaded with them
 };
 
 template <int MAP, bool CHECK, class SR>
 __device__ __forceinline__ void map_load(const UpdateArgs& A, int k0, int p0, const SR* __restrict__ radiance,
-                                         const SR* __restrict__ dirDist, MapPrefetch& f) {
+                                         const SR* __restrict__ dirDist, const float* __restrict__ dirs0,
+                                         MapPrefetch& f) {
     const int g = threadIdx.x >> 4, rp = (threadIdx.x & 15) * 2, p = p0 + g;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int r = k0 + rp + q;
         f.a[q] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (CHECK) f.d[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (MAP == 0 || CHECK) f.r0[q] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (p < A.P && r < A.R) {
             const size_t e = (size_t)p * A.rayWidth + r;
             f.a[q] = Texel<SR>::ld4(MAP == 0 ? dirDist : radiance, e);
             if (CHECK) f.d[q] = Texel<SR>::ld4(dirDist, e);
+            if (MAP == 0 || CHECK) f.r0[q] = reinterpret_cast<const float4*>(dirs0)[r];
         }
     }
 }
@@ -627,7 +643,7 @@ __device__ __forceinline__ void map_store(const UpdateArgs& A, int k0, int p0, c
             }
             if (MAP == 0 || CHECK) {
                 const float4 d = MAP == 0 ? f.a[q] : f.d[q];
-                if (d.x != dirs0[r * 4 + 0] || d.y != dirs0[r * 4 + 1] || d.z != dirs0[r * 4 + 2]) sMis[g] = 1;
+                if (d.x != f.r0[q].x || d.y != f.r0[q].y || d.z != f.r0[q].z) sMis[g] = 1;
             }
         }
 #pragma unroll
@@ -661,14 +677,14 @@ __device__ __forceinline__ void blend_map_block(const UpdateArgs& A, int p0, con
         for (int c = 0; c < CH; ++c) acc[j][c] = floatx4{0.f, 0.f, 0.f, 0.f};
     const int nSteps = Rpad / kHK;
     MapPrefetch f;
-    map_load<MAP, CHECK, SR>(A, 0, p0, radiance, dirDist, f);
+    map_load<MAP, CHECK, SR>(A, 0, p0, radiance, dirDist, dirs0, f);
     __syncthreads();  // sMis initialised
     map_store<MAP, CHECK, SPLIT, T>(A, 0, p0, dirs0, f, lds.st[0], sMis);
     __syncthreads();
     for (int s = 0; s < nSteps; ++s) {
         const int k0 = s * kHK;
         const bool more = s + 1 < nSteps;
-        if (more) map_load<MAP, CHECK, SR>(A, k0 + kHK, p0, radiance, dirDist, f);
+        if (more) map_load<MAP, CHECK, SR>(A, k0 + kHK, p0, radiance, dirDist, dirs0, f);
         const MapStage<CH, T>& st = lds.st[s & 1];
         const int ko = k0 + 8 * kq;
         V bh[CH], bl[CH];

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 namespace luxgi {
 
@@ -452,12 +453,20 @@ __device__ __forceinline__ void weights_block(int tile, int chunk, const float* 
     }
 }
 
+// both maps' weights in one launch: blockIdx.z 0 irradiance, 1 depth; x: 16-texel tile, y: ray chunk
 template <class T>
-__global__ __launch_bounds__(256) void weights_lowp_kernel(const float* __restrict__ dirs0, int R, int Rpad, int n,
-                                                           int tpad, int depth, float sharpness, int split,
-                                                           T* __restrict__ Whi, T* __restrict__ Wlo,
-                                                           float* __restrict__ Sp) {
-    weights_block<T>(blockIdx.x, blockIdx.y, dirs0, R, Rpad, n, tpad, depth, sharpness, split, Whi, Wlo, Sp);
+__global__ __launch_bounds__(256) void weights_lowp_kernel(const float* __restrict__ dirs0, int R, int Rpad, int ni,
+                                                           int tpI, int nd, int tpD, float sharpness, int split,
+                                                           int doIrr, int doDepth, T* __restrict__ Wih,
+                                                           T* __restrict__ Wil, float* __restrict__ Sih,
+                                                           T* __restrict__ Wdh, T* __restrict__ Wdl,
+                                                           float* __restrict__ Sdh) {
+    const bool depth = blockIdx.z == 1;
+    if (depth ? !doDepth : !doIrr) return;  // uniform per block
+    const int tp = depth ? tpD : tpI;
+    if ((int)blockIdx.x * 16 >= tp) return;
+    weights_block<T>(blockIdx.x, blockIdx.y, dirs0, R, Rpad, depth ? nd : ni, tp, depth ? 1 : 0, sharpness, split,
+                     depth ? Wdh : Wih, depth ? Wdl : Wil, depth ? Sdh : Sih);
 }
 
 __device__ __forceinline__ floatx4 mfma32(half8v a, half8v b, floatx4 acc) {
@@ -935,40 +944,36 @@ extern "C" size_t luxgi_ddgi_probe_update_workspace_size(const LuxgiDDGIUniform*
     return update_workspace_layout(u, nullptr, nullptr);
 }
 
-// weights of both maps and the per-map MFMA blends with operand type T (FP16 or BF16)
+// weights of both maps and the per-map MFMA blends with operand type T (FP16 or BF16); dirs0: ray
+// row 0's directions as float4 per ray
 template <class SR, class SA, class T, bool SPLIT>
 static int launch_maps(const UpdateArgs& A, const SR* rad, const SR* dd, const SA* pI, const SA* pD, SA* oI, SA* oD,
-                       const UpdateWorkspace& w, int tpI, int tpD, int Rp, hipStream_t st) {
+                       const UpdateWorkspace& w, const float* dirs0, int tpI, int tpD, int Rp, hipStream_t st) {
     T* Wih = reinterpret_cast<T*>(w.Wih);
     T* Wil = reinterpret_cast<T*>(w.Wil);
     T* Wdh = reinterpret_cast<T*>(w.Wdh);
     T* Wdl = reinterpret_cast<T*>(w.Wdl);
-    if (A.doIrr) {
-        hipLaunchKernelGGL(weights_lowp_kernel<T>, dim3(tpI / 16, (Rp + kWChunk - 1) / kWChunk), dim3(256), 0, st,
-                           w.dirs0, A.R, Rp, A.ni, tpI, 0,
-                           A.sharpness, SPLIT ? 1 : 0, Wih, Wil, w.Sih);
-        LUXGI_LAUNCH_CHECK();
-    }
-    if (A.doDepth) {
-        hipLaunchKernelGGL(weights_lowp_kernel<T>, dim3(tpD / 16, (Rp + kWChunk - 1) / kWChunk), dim3(256), 0, st,
-                           w.dirs0, A.R, Rp, A.nd, tpD, 1,
-                           A.sharpness, SPLIT ? 1 : 0, Wdh, Wdl, w.Sdh);
+    if (A.doIrr || A.doDepth) {
+        const int tiles = std::max(A.doIrr ? tpI / 16 : 0, A.doDepth ? tpD / 16 : 0);
+        hipLaunchKernelGGL(weights_lowp_kernel<T>, dim3(tiles, (Rp + kWChunk - 1) / kWChunk, 2), dim3(256), 0, st,
+                           dirs0, A.R, Rp, A.ni, tpI, A.nd, tpD, A.sharpness, SPLIT ? 1 : 0, A.doIrr, A.doDepth, Wih,
+                           Wil, w.Sih, Wdh, Wdl, w.Sdh);
         LUXGI_LAUNCH_CHECK();
     }
     const dim3 grid((A.P + kMG - 1) / kMG);
     if (A.doDepth) {
         hipLaunchKernelGGL((probe_blend_map_kernel<SR, SA, T, SPLIT, 0, false>), grid, dim3(256), 0, st, A, rad, dd, pD,
-                           oD, w.dirs0, Wdh, Wdl, w.Sdh, w.mismatch, tpD, Rp);
+                           oD, dirs0, Wdh, Wdl, w.Sdh, w.mismatch, tpD, Rp);
         LUXGI_LAUNCH_CHECK();
     }
     if (A.doIrr) {
         // the directions were checked by the depth launch; alone, this launch checks them
         if (A.doDepth)
             hipLaunchKernelGGL((probe_blend_map_kernel<SR, SA, T, SPLIT, 1, false>), grid, dim3(256), 0, st, A, rad, dd,
-                               pI, oI, w.dirs0, Wih, Wil, w.Sih, w.mismatch, tpI, Rp);
+                               pI, oI, dirs0, Wih, Wil, w.Sih, w.mismatch, tpI, Rp);
         else
             hipLaunchKernelGGL((probe_blend_map_kernel<SR, SA, T, SPLIT, 1, true>), grid, dim3(256), 0, st, A, rad, dd,
-                               pI, oI, w.dirs0, Wih, Wil, w.Sih, w.mismatch, tpI, Rp);
+                               pI, oI, dirs0, Wih, Wil, w.Sih, w.mismatch, tpI, Rp);
         LUXGI_LAUNCH_CHECK();
     }
     return LUXGI_OK;
@@ -986,13 +991,19 @@ static int launch_update(const UpdateArgs& A, const LuxgiRayBuffers* rays, const
     const bool mfma = !A.forcePerProbe && A.ni * A.ni <= 64 && A.nd * A.nd <= 256 && A.maxDistance > 0.0f;
     if (mfma) {
         const int tpI = pad16(A.ni * A.ni), tpD = pad16(A.nd * A.nd);
-        hipLaunchKernelGGL(row0_kernel<SR>, dim3((A.R + 255) / 256), dim3(256), 0, st, dd, A.R, w.dirs0);
-        LUXGI_LAUNCH_CHECK();
+        // ray row 0's directions as float4 per ray: FP32 rays already hold them so (probe 0's rays
+        // come first); FP16 rays are widened into the workspace
+        const float* dirs0 = reinterpret_cast<const float*>(dd);
+        if constexpr (!std::is_same<SR, float>::value) {
+            hipLaunchKernelGGL(row0_kernel<SR>, dim3((A.R + 255) / 256), dim3(256), 0, st, dd, A.R, w.dirs0);
+            LUXGI_LAUNCH_CHECK();
+            dirs0 = w.dirs0;
+        }
         const int Rp = pad32(A.R);
         int rc;
-        if (A.bf16) rc = launch_maps<SR, SA, __bf16, false>(A, rad, dd, pI, pD, oI, oD, w, tpI, tpD, Rp, st);
-        else if (A.half) rc = launch_maps<SR, SA, _Float16, false>(A, rad, dd, pI, pD, oI, oD, w, tpI, tpD, Rp, st);
-        else rc = launch_maps<SR, SA, _Float16, true>(A, rad, dd, pI, pD, oI, oD, w, tpI, tpD, Rp, st);
+        if (A.bf16) rc = launch_maps<SR, SA, __bf16, false>(A, rad, dd, pI, pD, oI, oD, w, dirs0, tpI, tpD, Rp, st);
+        else if (A.half) rc = launch_maps<SR, SA, _Float16, false>(A, rad, dd, pI, pD, oI, oD, w, dirs0, tpI, tpD, Rp, st);
+        else rc = launch_maps<SR, SA, _Float16, true>(A, rad, dd, pI, pD, oI, oD, w, dirs0, tpI, tpD, Rp, st);
         if (rc != LUXGI_OK) return rc;
         // probes whose directions differ from row 0: reference arithmetic with their own directions
         UpdateArgs F = A;

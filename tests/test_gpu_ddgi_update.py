@@ -212,3 +212,48 @@ def test_probe_update_bf16_mfma_error_bound(counts, rays, ni, nd):
             ref = oracle.border_update(atlas.astype(np.float64), side, *counts).astype(np.float32)
             assert np.array_equal(ref, atlas)
 
+
+
+@pytest.mark.parametrize("storage", [abi.STORAGE_FP32, abi.STORAGE_FP16])
+def test_probe_update_padded_ray_rows(storage):
+    """Ray buffers whose rows are wider than the ray count (LuxgiRayBuffers.width > raysPerProbe):
+    the blend reads row 0's directions (in place for FP32 rays, widened for FP16 rays) and every
+    probe's rays by the row pitch, and never the padding, which holds NaN here."""
+    from luxgi_amd import _lib
+    from luxgi_amd.ddgi import ptr, stream_handle
+    u = abi.make_uniform((4, 3, 2), (-1, -1, -1), (0.3, 0.4, 0.5), 96, irr_oct=8, depth_oct=16, max_distance=1.5)
+    rad, dd = _rays(u, seed=11)
+    if storage == abi.STORAGE_FP16:
+        rad, dd = rad.astype(np.float16).astype(np.float32), dd.astype(np.float16).astype(np.float32)
+    P, R = rad.shape[:2]
+    width = R + 13
+    dt = torch.float16 if storage == abi.STORAGE_FP16 else torch.float32
+    t_rad = torch.full((P, width, 4), float("nan"), dtype=dt, device="cuda")
+    t_dd = torch.full((P, width, 4), float("nan"), dtype=dt, device="cuda")
+    t_rad[:, :R] = torch.from_numpy(rad).to(dt).cuda()
+    t_dd[:, :R] = torch.from_numpy(dd).to(dt).cuda()
+    rays = abi.RayBuffers(ptr(t_rad), ptr(t_dd), width, P, storage, 0)
+    rng = np.random.default_rng(5)
+    prev_i = rng.uniform(0, 1, (u.irradianceTextureHeight, u.irradianceTextureWidth, 4)).astype(np.float32)
+    prev_d = rng.uniform(0, 1, (u.depthTextureHeight, u.depthTextureWidth, 2)).astype(np.float32)
+    if storage == abi.STORAGE_FP16:
+        prev_i, prev_d = prev_i.astype(np.float16).astype(np.float32), prev_d.astype(np.float16).astype(np.float32)
+    p_i = torch.from_numpy(prev_i).to(dt).cuda()
+    p_d = torch.from_numpy(prev_d).to(dt).cuda()
+    o_i, o_d = p_i.clone(), p_d.clone()
+    prev = abi.ProbeAtlas(ptr(p_i), ptr(p_d), storage, 0)
+    out = abi.ProbeAtlas(ptr(o_i), ptr(o_d), storage, 0)
+    flags = abi.UPDATE_IRRADIANCE | abi.UPDATE_DEPTH | abi.UPDATE_FUSE_BORDER
+    ws = torch.empty(_lib.lib().luxgi_ddgi_probe_update_workspace_size(C.byref(u)), dtype=torch.uint8, device="cuda")
+    for mask in (flags, abi.UPDATE_IRRADIANCE | abi.UPDATE_FUSE_BORDER):  # with and without the depth map
+        rc = _lib.lib().luxgi_ddgi_probe_update(C.byref(u), C.byref(rays), C.byref(prev), C.byref(out), 0, mask,
+                                                ptr(ws), ws.numel(), stream_handle())
+        assert rc == 0
+        torch.cuda.synchronize()
+        gi, gd = o_i.float().cpu().numpy(), o_d.float().cpu().numpy()
+        oi, od = oracle.probe_update(u, rad, dd, prev_i, prev_d, 0, mask)
+        tol = 2 ** -11 + 2e-5 if storage == abi.STORAGE_FP16 else 2e-5
+        assert np.isfinite(gi).all() and np.isfinite(gd).all()
+        np.testing.assert_allclose(gi, oi, rtol=tol, atol=1e-6)
+        if mask & abi.UPDATE_DEPTH:
+            np.testing.assert_allclose(gd, od, rtol=tol, atol=1e-6)
